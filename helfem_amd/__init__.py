@@ -1,0 +1,9 @@
+"""helfem_amd: MI355X-native Fock-build engine for HelFEM.
+
+The shipped path is libhfgpu.so (hand-written gfx950 HIP kernels behind the
+C ABI of include/hfgpu.h); this package is its Python host mirror.
+"""
+from ._lib import HfError, lib  # noqa: F401
+from .atomic import AtomicBasis, Context, angular_basis, element_grid  # noqa: F401
+
+__all__ = ["AtomicBasis", "Context", "HfError", "angular_basis", "element_grid", "lib"]

@@ -1,0 +1,197 @@
+"""Python mirror of HelFEM's atomic TwoDBasis Fock-build interface.
+
+`AtomicBasis` follows helfem::atomic::basis::TwoDBasisT<double>
+(src/atomic/TwoDBasis.h:52-283): compute_tei(), coulomb(P), exchange(P),
+overlap(), kinetic(), nuclear(), get_sym_idx().  coulomb/exchange accept a
+numpy array (host path: copied to the GPU and back) or a torch CUDA tensor
+(device path: asynchronous on torch's current stream, result stays in HBM).
+
+torch tensors are row-major; the kernels read column-major.  J and K commute
+with transposition of P (the two-electron integrals are symmetric under
+(ij|kl) = (ji|lk)), so reading a row-major P as its transpose and writing
+the row-major result gives exactly J(P) and K(P).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def angular_basis(lmax: int, mmax: int):
+    """src/atomic/basis.cpp:179-203: |m| outer, l inner, (+m, -m) pairs."""
+    lval, mval = [], []
+    for mabs in range(mmax + 1):
+        for l in range(mabs, lmax + 1):
+            lval.append(l)
+            mval.append(mabs)
+            if mabs > 0:
+                lval.append(l)
+                mval.append(-mabs)
+    return np.array(lval, dtype=np.int32), np.array(mval, dtype=np.int32)
+
+
+def element_grid(rmax: float, nelem: int, igrid: int = 4, zexp: float = 2.0) -> np.ndarray:
+    """libhelfem/include/grid.h:37-101 (igrid 1-4), Eigen LinSpaced semantics."""
+    b = np.zeros(nelem + 1)
+    for i in range(nelem + 1):
+        def lin(hi):
+            return hi if i == nelem else i * (hi / nelem)
+        if igrid == 1:
+            b[i] = lin(rmax)
+        elif igrid == 2:
+            b[i] = i * i * rmax * (1.0 / (nelem * nelem))
+        elif igrid == 3:
+            b[i] = rmax * (i * (1.0 / nelem)) ** zexp
+        elif igrid == 4:
+            b[i] = np.exp(lin(np.log(rmax + 1.0) ** (1.0 / zexp)) ** zexp) - 1.0
+        else:
+            raise ValueError("igrid must be 1..4")
+    b[0] = 0.0
+    b[nelem] = rmax
+    return b
+
+
+class Context:
+    """One GPU and one stream (hf_ctx)."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        h = ctypes.c_void_p()
+        check(lib().hf_ctx_create(device, 0, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+        self.use_torch_stream = use_torch_stream
+
+    @property
+    def handle(self):
+        return self._h
+
+    def bind_torch_stream(self):
+        """Issue this context's work on torch's current stream."""
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().hf_ctx_set_stream(self._h, ctypes.c_void_p(s)))
+
+    def synchronize(self):
+        check(lib().hf_ctx_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().hf_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f64_fortran(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+class AtomicBasis:
+    """Radial finite elements x complex spherical harmonics (hf_atomic)."""
+
+    def __init__(self, Z, lmax=None, mmax=None, nelem=None, rmax=40.0, igrid=4, zexp=2.0, nnodes=15, nquad=0,
+                 ctx: Context | None = None, lval=None, mval=None, bval=None):
+        if lval is None:
+            lval, mval = angular_basis(lmax, mmax)
+        self.lval = np.ascontiguousarray(lval, dtype=np.int32)
+        self.mval = np.ascontiguousarray(mval, dtype=np.int32)
+        self.bval = element_grid(rmax, nelem, igrid, zexp) if bval is None else np.ascontiguousarray(bval, np.float64)
+        self.Z = int(Z)
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().hf_atomic_create(ctx.handle if ctx else None, self.Z, nnodes, nquad,
+                                     self.bval.ctypes.data, self.bval.size, self.lval.ctypes.data,
+                                     self.mval.ctypes.data, self.lval.size, ctypes.byref(h)))
+        self._h = h
+        sz = np.zeros(5, dtype=np.int64)
+        check(lib().hf_atomic_sizes(self._h, sz.ctypes.data))
+        self.Nbf, self.Nrad, self.Nang, self.Nel, self.N_L = (int(v) for v in sz)
+        self.first = np.zeros(self.Nel, np.int32)
+        self.nprim = np.zeros(self.Nel, np.int32)
+        check(lib().hf_atomic_elements(self._h, self.first.ctypes.data, self.nprim.ctypes.data))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._LIB is not None:
+            _lib._LIB.hf_atomic_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    # --- setup ---------------------------------------------------------
+    def compute_tei(self, exchange: bool = True):
+        check(lib().hf_atomic_compute_tei(self._h, int(exchange)))
+
+    def set_cache(self, kind, L, iel, mat):
+        mat = _f64_fortran(mat)
+        check(lib().hf_atomic_set_cache(self._h, kind, L, iel, mat.shape[0], mat.shape[1], mat.ctypes.data))
+
+    def commit_tei(self):
+        check(lib().hf_atomic_commit_tei(self._h))
+
+    def get_cache(self, kind, L, iel):
+        dims = np.zeros(2, dtype=np.int32)
+        check(lib().hf_atomic_get_cache(self._h, kind, L, iel, dims.ctypes.data, None))
+        out = np.zeros((int(dims[0]), int(dims[1])), order="F")
+        if out.size:
+            check(lib().hf_atomic_get_cache(self._h, kind, L, iel, dims.ctypes.data, out.ctypes.data))
+        return out
+
+    def _one_e(self, which):
+        out = np.zeros((self.Nbf, self.Nbf), order="F")
+        check(lib().hf_atomic_one_electron(self._h, which, out.ctypes.data, self.Nbf))
+        return out
+
+    def overlap(self):
+        return self._one_e(0)
+
+    def kinetic(self):
+        return self._one_e(1)
+
+    def nuclear(self):
+        return self._one_e(2)
+
+    def get_sym_idx(self, symm=1):
+        """TwoDBasis.cpp:202-228: one index list per unique m (symm=1)."""
+        if symm == 0:
+            return [np.arange(self.Nbf)]
+        out = []
+        for m in sorted(set(self.mval.tolist())):
+            out.append(np.concatenate([np.arange(a * self.Nrad, (a + 1) * self.Nrad)
+                                       for a in range(self.Nang) if self.mval[a] == m]))
+        return out
+
+    # --- Fock build ------------------------------------------------------
+    def _apply(self, P, host_fn, dev_fn, out=None):
+        if isinstance(P, np.ndarray):
+            P = _f64_fortran(P)
+            if P.shape != (self.Nbf, self.Nbf):
+                raise ValueError(f"P must be {self.Nbf}x{self.Nbf}")
+            O = np.zeros((self.Nbf, self.Nbf), order="F")
+            check(host_fn(self._h, P.ctypes.data, self.Nbf, O.ctypes.data, self.Nbf))
+            return O
+        import torch
+        if not (isinstance(P, torch.Tensor) and P.is_cuda and P.dtype == torch.float64):
+            raise TypeError("P must be a numpy array or a float64 CUDA tensor")
+        if P.shape != (self.Nbf, self.Nbf) or not P.is_contiguous():
+            raise ValueError(f"P must be a contiguous {self.Nbf}x{self.Nbf} tensor")
+        if self.ctx is None:
+            raise RuntimeError("basis was created without a GPU context")
+        self.ctx.bind_torch_stream()
+        O = torch.empty_like(P) if out is None else out
+        check(dev_fn(self._h, P.data_ptr(), self.Nbf, O.data_ptr(), self.Nbf))
+        return O
+
+    def coulomb(self, P, out=None):
+        """J(P), TwoDBasis.cpp:774-877."""
+        return self._apply(P, lib().hf_atomic_coulomb, lib().hf_atomic_coulomb_dev, out)
+
+    def exchange(self, P, out=None):
+        """-K(P) for a spin density P, TwoDBasis.cpp:880-999."""
+        return self._apply(P, lib().hf_atomic_exchange, lib().hf_atomic_exchange_dev, out)

@@ -1,0 +1,329 @@
+// Atomic Coulomb and exchange kernels (FP64) for gfx950.
+//
+// Data layout (device):
+//   P, J, K     dense Nbf x Nbf column-major, angular-major blocks:
+//               index (ang*Nrad + rad) as in TwoDBasis::add_sub
+//               (src/atomic/TwoDBasis.cpp:310-318).
+//   rs, rb      per (L, element) 16x16 zero-padded tiles of the r^L and
+//               r^-L-1 element matrices (CoulombExchangeFE.h:180-217).
+//   chol        per (L, element) the Cholesky vectors reshaped to 16x16
+//               tiles M_p(a,b) = L(a + b*Ni, p) (CoulombExchangeFE.h:503-510).
+//
+// Coulomb (TwoDBasis.cpp:774-877 + CoulombExchangeFE.h:432-482):
+//   only the element-diagonal Ni x Ni bands of each P block enter J, and J
+//   is nonzero only on those bands.  Kernel 1 gathers the Gaunt-weighted
+//   band of every (L,M) into LDS, reduces the two multipole traces and the
+//   Cholesky matvec; kernel 2 prefix-sums the cross-element traces; kernel 3
+//   expands the (L,M) bands back into the angular blocks of J.
+// Exchange (TwoDBasis.cpp:880-999 + CoulombExchangeFE.h:484-530):
+//   one workgroup per (output block, element pair (e,f)): the Gaunt-weighted
+//   density sub-block R_L(e,f) is formed in registers for each coupled L and
+//   immediately contracted with the radial factors (A R B' across elements,
+//   sum_p M_p R M_p' inside an element), so the Nrad x Nrad Rmat of the
+//   reference never exists.
+#include "hfgpu_internal.hpp"
+
+namespace hfg {
+namespace {
+
+constexpr int NT = 256;  // threads per workgroup (4 wave64)
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Reduce two values over the 256-thread block; result valid in thread 0.
+__device__ inline void block_sum2(double &x, double &y, double *red) {
+  x = wave_sum(x);
+  y = wave_sum(y);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[w] = x;
+    red[4 + w] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    x = red[0] + red[1] + red[2] + red[3];
+    y = red[4] + red[5] + red[6] + red[7];
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
+  const int64_t n = rows * cols;
+  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n; q += (int64_t)gridDim.x * NT) {
+    const int64_t j = q / rows, i = q - j * rows;
+    A[i + j * ld] = 0.0;
+  }
+}
+
+// --- Coulomb -----------------------------------------------------------------
+
+struct JArgs {
+  const double *P;
+  int64_t ldp;
+  int Nel, Nrad, Nang;
+  const int *first, *ni;
+  const double *rs, *rb, *chol;
+  const int *chol_off, *chol_rank;
+  const int *lm_L, *lm_pstart, *pair_kl;
+  const double *pair_c;
+  double *Jin, *jsm, *jbg;
+};
+
+// One workgroup per ((L,M), element).
+__global__ void __launch_bounds__(NT) k_atomic_j_gather(JArgs A) {
+  const int lm = blockIdx.x / A.Nel, e = blockIdx.x - lm * A.Nel;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int ni = A.ni[e], f0 = A.first[e], L = A.lm_L[lm];
+  __shared__ double sP[TILE2];
+  __shared__ double sS[TILE2];
+  __shared__ double red[8];
+  double v = 0.0;
+  if (a < ni && b < ni) {
+    const int p0 = A.lm_pstart[lm], p1 = A.lm_pstart[lm + 1];
+    for (int p = p0; p < p1; ++p) {
+      const int kl = A.pair_kl[p];
+      const int k = kl / A.Nang, l = kl - k * A.Nang;
+      v += A.pair_c[p] * A.P[(int64_t)(k * A.Nrad + f0 + a) + (int64_t)(l * A.Nrad + f0 + b) * A.ldp];
+    }
+  }
+  sP[t] = v;
+  __syncthreads();
+  // tr(r_small P) and tr(r_big P): sum_ab r(a,b) P(b,a)
+  const size_t tile = ((size_t)L * A.Nel + e) * TILE2;
+  const double pt = sP[b + TILE * a];
+  double ts = A.rs[tile + t] * pt;
+  double tb = e > 0 ? A.rb[tile + t] * pt : 0.0;
+  block_sum2(ts, tb, red);
+  if (t == 0) {
+    A.jsm[(size_t)lm * A.Nel + e] = ts;
+    A.jbg[(size_t)lm * A.Nel + e] = tb;
+  }
+  // s_p = <M_p, P>, wave w handles p = w, w+4, ...
+  const int rank = A.chol_rank[L * A.Nel + e];
+  const double *M = A.chol + A.chol_off[L * A.Nel + e];
+  const int lane = t & 63, w = t >> 6;
+  for (int p = w; p < rank; p += 4) {
+    const double *Mp = M + (size_t)p * TILE2;
+    double s = Mp[lane] * sP[lane] + Mp[lane + 64] * sP[lane + 64] + Mp[lane + 128] * sP[lane + 128] +
+               Mp[lane + 192] * sP[lane + 192];
+    s = wave_sum(s);
+    if (lane == 0) sS[p] = s;
+  }
+  __syncthreads();
+  double jin = 0.0;
+  for (int p = 0; p < rank; ++p) jin += M[(size_t)p * TILE2 + t] * sS[p];
+  A.Jin[((size_t)lm * A.Nel + e) * TILE2 + t] = jin;
+}
+
+// Cross-element weights: pre_a(lm,e) = sum_{j>e} jbig(j), pre_b = sum_{j<e} jsmall(j)
+__global__ void k_atomic_j_prefix(int n_lm, int Nel, const double *jsm, const double *jbg, double *preA,
+                                  double *preB) {
+  const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lm >= n_lm) return;
+  const double *s = jsm + (size_t)lm * Nel;
+  const double *g = jbg + (size_t)lm * Nel;
+  double acc = 0.0;
+  for (int e = 0; e < Nel; ++e) {
+    preB[(size_t)lm * Nel + e] = acc;
+    acc += s[e];
+  }
+  acc = 0.0;
+  for (int e = Nel - 1; e >= 0; --e) {
+    preA[(size_t)lm * Nel + e] = acc;
+    acc += g[e];
+  }
+}
+
+struct JOutArgs {
+  double *J;
+  int64_t ldj;
+  int Nel, Nrad, Nang;
+  const int *first, *ni, *lm_L;
+  const double *rs, *rb, *Jin, *preA, *preB;
+  const int *out_start, *out_lm;
+  const double *out_c;
+};
+
+__device__ inline double jband(const JOutArgs &A, int lm, int e, int t) {
+  const int L = A.lm_L[lm];
+  const size_t tile = ((size_t)L * A.Nel + e) * TILE2;
+  const size_t q = (size_t)lm * A.Nel + e;
+  double v = A.Jin[q * TILE2 + t] + A.preA[q] * A.rs[tile + t];
+  if (e > 0) v += A.preB[q] * A.rb[tile + t];
+  return v;
+}
+
+// One workgroup per (angular output block (i,j), element).
+__global__ void __launch_bounds__(NT) k_atomic_j_expand(JOutArgs A) {
+  const int ij = blockIdx.x / A.Nel, e = blockIdx.x - ij * A.Nel;
+  const int q0 = A.out_start[ij], q1 = A.out_start[ij + 1];
+  if (q0 == q1) return;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int ni = A.ni[e], f0 = A.first[e];
+  if (a >= ni || b >= ni) return;
+  if (e > 0 && a == 0 && b == 0) return;  // shared corner, written by e-1
+  const bool corner = (a == ni - 1 && b == ni - 1 && e + 1 < A.Nel);
+  double v = 0.0;
+  for (int q = q0; q < q1; ++q) {
+    const int lm = A.out_lm[q];
+    double x = jband(A, lm, e, t);
+    if (corner) x += jband(A, lm, e + 1, 0);
+    v += A.out_c[q] * x;
+  }
+  const int i = ij / A.Nang, j = ij - i * A.Nang;
+  A.J[(int64_t)(i * A.Nrad + f0 + a) + (int64_t)(j * A.Nrad + f0 + b) * A.ldj] = v;
+}
+
+// --- Exchange ----------------------------------------------------------------
+
+__global__ void __launch_bounds__(NT) k_block_norms(const double *P, int64_t ldp, int Nrad, int Nang,
+                                                     double *bnorm) {
+  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
+  __shared__ double red[8];
+  double s = 0.0, dummy = 0.0;
+  const int64_t n = (int64_t)Nrad * Nrad;
+  for (int64_t q = threadIdx.x; q < n; q += NT) {
+    const int64_t c = q / Nrad, r = q - c * Nrad;
+    const double v = P[(int64_t)(i * Nrad) + r + ((int64_t)l * Nrad + c) * ldp];
+    s += v * v;
+  }
+  block_sum2(s, dummy, red);
+  if (threadIdx.x == 0) bnorm[il] = sqrt(s);
+}
+
+struct KArgs {
+  const double *P;
+  int64_t ldp;
+  double *K;
+  int64_t ldk;
+  int Nel, Nrad, Nang, N_L, n_act, n_ef_pad;
+  const int *first, *ni;
+  const double *rs, *rb, *chol;
+  const int *chol_off, *chol_rank;
+  const int *act_jk, *term_start, *term_il;
+  const double *term_c, *bnorm;
+  double thr;
+};
+
+// One workgroup per (active output block (j,k), element pair (e,f)).  The
+// workgroup index is mapped so that consecutive workgroups on one XCD share
+// the element pair (e,f): their P sub-blocks then stay in that XCD's L2.
+__global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
+  const int w = blockIdx.x;
+  const int xcd = w & 7, loc = w >> 3;
+  const int ef = (loc / A.n_act) * 8 + xcd;
+  const int act = loc - (loc / A.n_act) * A.n_act;
+  if (ef >= A.Nel * A.Nel) return;
+  const int e = ef / A.Nel, f = ef - e * A.Nel;
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
+  const bool inside = (a < ni && b < nj);
+  __shared__ double sR[TILE2], sA[TILE2], sB[TILE2], sY[TILE2];
+  double acc = 0.0;
+  for (int L = 0; L < A.N_L; ++L) {
+    const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
+    if (q0 == q1) continue;
+    double r = 0.0;
+    bool any = false;
+    for (int q = q0; q < q1; ++q) {
+      const int il = A.term_il[q];
+      if (A.bnorm[il] < A.thr) continue;
+      any = true;
+      if (inside) {
+        const int i = il / A.Nang, l = il - i * A.Nang;
+        r += A.term_c[q] * A.P[(int64_t)(i * A.Nrad + fi + a) + (int64_t)(l * A.Nrad + fj + b) * A.ldp];
+      }
+    }
+    if (!any) continue;  // uniform over the workgroup
+    sR[t] = r;
+    if (e != f) {
+      const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
+      sA[t] = (e > f) ? A.rb[te + t] : A.rs[te + t];
+      sB[t] = (e > f) ? A.rs[tf + t] : A.rb[tf + t];
+      __syncthreads();
+      double y = 0.0;
+#pragma unroll
+      for (int c = 0; c < TILE; ++c) y += sR[a + TILE * c] * sB[b + TILE * c];
+      sY[t] = y;
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < TILE; ++c) acc += sA[a + TILE * c] * sY[c + TILE * b];
+      __syncthreads();
+    } else {
+      const int rank = A.chol_rank[L * A.Nel + e];
+      const double *M = A.chol + A.chol_off[L * A.Nel + e];
+      for (int p = 0; p < rank; ++p) {
+        sA[t] = M[(size_t)p * TILE2 + t];
+        __syncthreads();
+        double y = 0.0;
+#pragma unroll
+        for (int c = 0; c < TILE; ++c) y += sR[a + TILE * c] * sA[b + TILE * c];
+        sY[t] = y;
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < TILE; ++c) acc += sA[a + TILE * c] * sY[c + TILE * b];
+        __syncthreads();
+      }
+    }
+  }
+  if (!inside) return;
+  const int64_t row = (int64_t)j * A.Nrad + fi + a, col = (int64_t)k * A.Nrad + fj + b;
+  const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
+  const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+  double *dst = A.K + row + col * A.ldk;
+  if (srow || scol)
+    atomicAdd(dst, -acc);
+  else
+    *dst = -acc;
+}
+
+}  // namespace
+
+void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
+  const int64_t n = rows * cols;
+  if (n == 0) return;
+  int64_t blocks = (n + NT - 1) / NT;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(NT), 0, s, A, rows, cols, ld);
+  hip_check(hipGetLastError(), "fill_zero");
+}
+
+void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
+                           hipStream_t s) {
+  const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+  fill_zero_launch(J, nbf, nbf, ldj, s);
+  if (d.n_lm == 0) return;
+  JArgs a{P, ldp, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.rs.p, d.rb.p, d.chol.p, d.chol_off.p,
+          d.chol_rank.p, d.lm_L.p, d.lm_pstart.p, d.pair_kl.p, d.pair_c.p, d.Jin.p, d.jsm.p, d.jbg.p};
+  hipLaunchKernelGGL(k_atomic_j_gather, dim3(d.n_lm * d.Nel), dim3(NT), 0, s, a);
+  hip_check(hipGetLastError(), "atomic_j_gather");
+  hipLaunchKernelGGL(k_atomic_j_prefix, dim3((d.n_lm + 63) / 64), dim3(64), 0, s, d.n_lm, d.Nel, d.jsm.p,
+                     d.jbg.p, d.preA.p, d.preB.p);
+  hip_check(hipGetLastError(), "atomic_j_prefix");
+  JOutArgs o{J, ldj, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.lm_L.p, d.rs.p, d.rb.p, d.Jin.p,
+             d.preA.p, d.preB.p, d.out_start.p, d.out_lm.p, d.out_c.p};
+  hipLaunchKernelGGL(k_atomic_j_expand, dim3(d.Nang * d.Nang * d.Nel), dim3(NT), 0, s, o);
+  hip_check(hipGetLastError(), "atomic_j_expand");
+}
+
+void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                            hipStream_t s) {
+  const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+  fill_zero_launch(K, nbf, nbf, ldk, s);
+  if (d.n_act == 0) return;
+  hipLaunchKernelGGL(k_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.bnorm.p);
+  hip_check(hipGetLastError(), "block_norms");
+  const int n_ef_pad = ((d.Nel * d.Nel + 7) / 8) * 8;
+  KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, n_ef_pad, d.first.p, d.ni.p, d.rs.p, d.rb.p,
+          d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p,
+          d.bnorm.p, 10.0 * 2.220446049250313e-16};
+  const int64_t grid = (int64_t)n_ef_pad * d.n_act;
+  if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+  hipLaunchKernelGGL(k_atomic_k, dim3((unsigned)grid), dim3(NT), 0, s, a);
+  hip_check(hipGetLastError(), "atomic_k");
+}
+
+}  // namespace hfg

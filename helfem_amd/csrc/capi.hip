@@ -1,0 +1,499 @@
+// C-ABI layer (include/hfgpu.h): handles, table construction, error
+// translation.  Host code only; the kernels live in atomic_jk.hip.
+#include "hfgpu_internal.hpp"
+#include "../../include/hfgpu.h"
+#include <cstring>
+#include <cmath>
+#include <memory>
+
+using namespace hfg;
+
+struct hf_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t cur = nullptr;
+};
+
+struct hf_atomic {
+  hf_ctx *ctx = nullptr;
+  AtomicSetup host;
+  AtomicDevice dev;
+  bool uploaded = false;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return HF_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return HF_ENOMEM;
+  } catch (const std::logic_error &e) {
+    g_err = e.what();
+    return HF_EINVAL;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return HF_EINTERNAL;
+  }
+}
+
+void need(bool cond, const char *msg) {
+  if (!cond) throw Error(HF_EINVAL, msg);
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hip_check(hipGetDevice(&prev), "hipGetDevice");
+    if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+  }
+  ~DeviceGuard() {
+    int now = -1;
+    if (hipGetDevice(&now) == hipSuccess && now != prev && prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+constexpr double kFourPi = 12.566370614359172953850573533118;
+
+// Build the Gaunt coupling tables and upload caches for the atomic kernels.
+void atomic_upload(hf_atomic *h) {
+  const AtomicSetup &S = h->host;
+  AtomicDevice &d = h->dev;
+  hipStream_t s = h->ctx->cur;
+  const int Nel = S.rad.Nel(), Nrad = S.Nrad(), Na = S.Nang(), nL = S.N_L;
+  d.Nel = Nel;
+  d.Nrad = Nrad;
+  d.Nang = Na;
+  d.N_L = nL;
+  d.first.upload(S.rad.first, s);
+  d.ni.upload(S.rad.ni, s);
+  for (int e = 0; e < Nel; ++e)
+    if (S.rad.ni[e] > TILE) throw Error(HF_EINVAL, "elements with more than 16 functions are not supported");
+
+  // radial factor tiles
+  std::vector<double> rs((size_t)nL * Nel * TILE2, 0.0), rb((size_t)nL * Nel * TILE2, 0.0);
+  std::vector<double> ch;
+  std::vector<int> choff((size_t)nL * Nel), chrank((size_t)nL * Nel);
+  int maxr = 0;
+  for (int L = 0; L < nL; ++L)
+    for (int e = 0; e < Nel; ++e) {
+      const size_t q = (size_t)L * Nel + e;
+      const int n = S.rad.ni[e];
+      const Blk &a = S.rsmall[q];
+      need(a.r == n && a.c == n, "r^L cache has the wrong shape");
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) rs[q * TILE2 + i + TILE * j] = a(i, j);
+      if (e > 0) {
+        const Blk &b = S.rbig[q];
+        need(b.r == n && b.c == n, "r^-L-1 cache has the wrong shape");
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i) rb[q * TILE2 + i + TILE * j] = b(i, j);
+      }
+      const Blk &c = S.chol[q];
+      need(c.r == n * n, "Cholesky cache has the wrong number of rows");
+      choff[q] = (int)ch.size();
+      chrank[q] = c.c;
+      maxr = std::max(maxr, c.c);
+      need(c.c <= TILE2, "Cholesky rank above 256 is not supported");
+      const size_t base = ch.size();
+      ch.resize(base + (size_t)c.c * TILE2, 0.0);
+      for (int p = 0; p < c.c; ++p)
+        for (int bb = 0; bb < n; ++bb)
+          for (int aa = 0; aa < n; ++aa) ch[base + (size_t)p * TILE2 + aa + TILE * bb] = c(aa + bb * n, p);
+    }
+  d.max_rank = maxr;
+  d.rs.upload(rs, s);
+  d.rb.upload(rb, s);
+  if (ch.empty()) ch.push_back(0.0);
+  d.chol.upload(ch, s);
+  d.chol_off.upload(choff, s);
+  d.chol_rank.upload(chrank, s);
+
+  // Coulomb tables (TwoDBasis.cpp:790-822, :852-874)
+  const auto &lv = S.lval;
+  const auto &mv = S.mval;
+  const int Mmax = *std::max_element(mv.begin(), mv.end()) - *std::min_element(mv.begin(), mv.end());
+  std::vector<int> lmL, lmIndex((size_t)nL * (2 * Mmax + 1), -1);
+  for (int L = 0; L < nL; ++L)
+    for (int M = -std::min(L, Mmax); M <= std::min(L, Mmax); ++M) {
+      lmIndex[(size_t)L * (2 * Mmax + 1) + (M + Mmax)] = (int)lmL.size();
+      lmL.push_back(L);
+    }
+  const int nlm = (int)lmL.size();
+  std::vector<std::vector<std::pair<int, double>>> pairs(nlm);
+  for (int k = 0; k < Na; ++k)
+    for (int l = 0; l < Na; ++l) {
+      const int M = mv[k] - mv[l];
+      for (int L = std::max(std::abs(lv[k] - lv[l]), std::abs(M)); L <= lv[k] + lv[l]; ++L) {
+        const double c = gaunt_coeff(lv[k], mv[k], L, M, lv[l], mv[k] - M);
+        if (c == 0.0) continue;
+        pairs[lmIndex[(size_t)L * (2 * Mmax + 1) + (M + Mmax)]].push_back({k * Na + l, c});
+      }
+    }
+  std::vector<int> pstart(nlm + 1, 0), pkl;
+  std::vector<double> pc;
+  for (int q = 0; q < nlm; ++q) {
+    for (auto &pr : pairs[q]) {
+      pkl.push_back(pr.first);
+      pc.push_back(pr.second);
+    }
+    pstart[q + 1] = (int)pkl.size();
+  }
+  std::vector<int> ostart((size_t)Na * Na + 1, 0), olm;
+  std::vector<double> oc;
+  for (int i = 0; i < Na; ++i)
+    for (int j = 0; j < Na; ++j) {
+      const int M = mv[j] - mv[i];
+      for (int L = std::max(std::abs(lv[j] - lv[i]), std::abs(M)); L <= lv[j] + lv[i]; ++L) {
+        const double c = gaunt_coeff(lv[j], mv[j], L, M, lv[i], mv[j] - M);
+        if (c == 0.0) continue;
+        olm.push_back(lmIndex[(size_t)L * (2 * Mmax + 1) + (M + Mmax)]);
+        oc.push_back(c * (kFourPi / double(2 * L + 1)));
+      }
+      ostart[(size_t)i * Na + j + 1] = (int)olm.size();
+    }
+  d.n_lm = nlm;
+  d.n_pairs = (int64_t)pkl.size();
+  d.lm_L.upload(lmL, s);
+  d.lm_pstart.upload(pstart, s);
+  if (pkl.empty()) {
+    pkl.push_back(0);
+    pc.push_back(0.0);
+  }
+  d.pair_kl.upload(pkl, s);
+  d.pair_c.upload(pc, s);
+  d.out_start.upload(ostart, s);
+  if (olm.empty()) {
+    olm.push_back(0);
+    oc.push_back(0.0);
+  }
+  d.out_lm.upload(olm, s);
+  d.out_c.upload(oc, s);
+
+  // Exchange tables (TwoDBasis.cpp:918-968)
+  std::vector<int> act, tstart(1, 0), til;
+  std::vector<double> tc;
+  for (int j = 0; j < Na; ++j)
+    for (int k = 0; k < Na; ++k) {
+      std::vector<std::vector<std::pair<int, double>>> perL(nL);
+      bool anyt = false;
+      for (int i = 0; i < Na; ++i)
+        for (int l = 0; l < Na; ++l) {
+          const int M = mv[j] - mv[i];
+          if (M != mv[k] - mv[l]) continue;
+          const int Lmin = std::max(std::max(std::abs(lv[i] - lv[j]), std::abs(lv[k] - lv[l])), std::abs(M));
+          const int Lmax = std::min(lv[i] + lv[j], lv[k] + lv[l]);
+          for (int L = Lmin; L <= Lmax; ++L) {
+            const double c = gaunt_coeff(lv[j], mv[j], L, M, lv[i], mv[i]) *
+                             gaunt_coeff(lv[k], mv[k], L, M, lv[l], mv[l]);
+            if (c == 0.0) continue;
+            perL[L].push_back({i * Na + l, (kFourPi / double(2 * L + 1)) * c});
+            anyt = true;
+          }
+        }
+      if (!anyt) continue;
+      act.push_back(j * Na + k);
+      for (int L = 0; L < nL; ++L) {
+        for (auto &pr : perL[L]) {
+          til.push_back(pr.first);
+          tc.push_back(pr.second);
+        }
+        tstart.push_back((int)til.size());
+      }
+    }
+  d.n_act = (int)act.size();
+  d.n_terms = (int64_t)til.size();
+  if (act.empty()) act.push_back(0);
+  if (til.empty()) {
+    til.push_back(0);
+    tc.push_back(0.0);
+  }
+  d.act_jk.upload(act, s);
+  d.term_start.upload(tstart, s);
+  d.term_il.upload(til, s);
+  d.term_c.upload(tc, s);
+
+  d.Jin.alloc((size_t)std::max(nlm, 1) * Nel * TILE2);
+  d.jsm.alloc((size_t)std::max(nlm, 1) * Nel);
+  d.jbg.alloc((size_t)std::max(nlm, 1) * Nel);
+  d.preA.alloc((size_t)std::max(nlm, 1) * Nel);
+  d.preB.alloc((size_t)std::max(nlm, 1) * Nel);
+  d.bnorm.alloc((size_t)Na * Na);
+  hip_check(hipStreamSynchronize(s), "upload sync");
+  h->uploaded = true;
+}
+
+void check_matrix(const void *p, int64_t ld, int64_t n, const char *name) {
+  if (!p) throw Error(HF_EINVAL, std::string(name) + " is NULL");
+  if (ld < n) throw Error(HF_EINVAL, std::string("leading dimension of ") + name + " is smaller than Nbf");
+}
+
+void ensure_ready(hf_atomic *h) {
+  if (!h->ctx) throw Error(HF_ESTATE, "handle was created without a device context");
+  if (!h->host.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+  if (!h->uploaded) atomic_upload(h);
+}
+
+// host-pointer entry: stage through device buffers
+template <typename Launch>
+void host_roundtrip(hf_atomic *h, const double *P, int64_t ldp, double *O, int64_t ldo, Launch launch) {
+  const int64_t n = h->host.Nbf();
+  check_matrix(P, ldp, n, "P");
+  check_matrix(O, ldo, n, "output");
+  hipStream_t s = h->ctx->cur;
+  h->dev.hostP.alloc((size_t)n * n);
+  h->dev.hostOut.alloc((size_t)n * n);
+  hip_check(hipMemcpy2DAsync(h->dev.hostP.p, n * sizeof(double), P, ldp * sizeof(double), n * sizeof(double), n,
+                             hipMemcpyHostToDevice, s),
+            "copy P to device");
+  launch(h->dev.hostP.p, n, h->dev.hostOut.p, n);
+  hip_check(hipMemcpy2DAsync(O, ldo * sizeof(double), h->dev.hostOut.p, n * sizeof(double), n * sizeof(double), n,
+                             hipMemcpyDeviceToHost, s),
+            "copy result to host");
+  hip_check(hipStreamSynchronize(s), "stream sync");
+}
+}  // namespace
+
+extern "C" {
+
+const char *hf_last_error(void) { return g_err.c_str(); }
+const char *hf_version(void) { return "hfgpu 0.1 (gfx950)"; }
+
+int hf_ctx_create(int device, unsigned flags, hf_ctx **out) {
+  return guarded([&] {
+    need(out != nullptr, "out is NULL");
+    need(flags == 0, "flags must be 0");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    need(device >= 0 && device < n, "device index out of range");
+    std::unique_ptr<hf_ctx> c(new hf_ctx);
+    c->device = device;
+    DeviceGuard g(device);
+    hip_check(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking), "hipStreamCreate");
+    c->cur = c->own;
+    *out = c.release();
+  });
+}
+
+int hf_ctx_destroy(hf_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->cur);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+  });
+}
+
+int hf_ctx_set_stream(hf_ctx *ctx, void *stream) {
+  return guarded([&] {
+    need(ctx != nullptr, "ctx is NULL");
+    ctx->cur = stream ? (hipStream_t)stream : ctx->own;
+  });
+}
+
+int hf_ctx_get_stream(hf_ctx *ctx, void **stream) {
+  return guarded([&] {
+    need(ctx && stream, "NULL argument");
+    *stream = (void *)ctx->cur;
+  });
+}
+
+int hf_ctx_synchronize(hf_ctx *ctx) {
+  return guarded([&] {
+    need(ctx != nullptr, "ctx is NULL");
+    DeviceGuard g(ctx->device);
+    hip_check(hipStreamSynchronize(ctx->cur), "hipStreamSynchronize");
+  });
+}
+
+int hf_atomic_create(hf_ctx *ctx, int Z, int nnodes, int nquad, const double *bval, int nbval, const int *lval,
+                     const int *mval, int nang, hf_atomic **out) {
+  return guarded([&] {
+    need(bval && lval && mval && out, "NULL argument");
+    need(nbval >= 2, "need at least one element (nbval >= 2)");
+    need(nang >= 1, "need at least one angular shell");
+    need(nnodes >= 2 && nnodes <= TILE, "nnodes must be in [2, 16]");
+    need(Z >= 0, "Z must be non-negative");
+    for (int i = 0; i + 1 < nbval; ++i) need(bval[i + 1] > bval[i], "element boundaries must increase");
+    need(bval[0] == 0.0, "the first element boundary must be r=0");
+    std::unique_ptr<hf_atomic> h(new hf_atomic);
+    h->ctx = ctx;
+    h->host.build(Z, nnodes, nquad > 0 ? nquad : 5 * nnodes, std::vector<double>(bval, bval + nbval),
+                  std::vector<int>(lval, lval + nang), std::vector<int>(mval, mval + nang));
+    *out = h.release();
+  });
+}
+
+int hf_atomic_destroy(hf_atomic *h) {
+  return guarded([&] {
+    if (!h) return;
+    if (h->ctx) {
+      DeviceGuard g(h->ctx->device);
+      (void)hipStreamSynchronize(h->ctx->cur);
+      delete h;
+    } else {
+      delete h;
+    }
+  });
+}
+
+int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    out[0] = h->host.Nbf();
+    out[1] = h->host.Nrad();
+    out[2] = h->host.Nang();
+    out[3] = h->host.rad.Nel();
+    out[4] = h->host.N_L;
+  });
+}
+
+int hf_atomic_elements(const hf_atomic *h, int *first, int *nprim) {
+  return guarded([&] {
+    need(h && first && nprim, "NULL argument");
+    for (int e = 0; e < h->host.rad.Nel(); ++e) {
+      first[e] = h->host.rad.first[e];
+      nprim[e] = h->host.rad.ni[e];
+    }
+  });
+}
+
+int hf_atomic_compute_tei(hf_atomic *h, int exchange) {
+  (void)exchange;
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    h->host.compute_tei();
+    h->uploaded = false;
+  });
+}
+
+int hf_atomic_set_cache(hf_atomic *h, int kind, int L, int iel, int rows, int cols, const double *data) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    AtomicSetup &S = h->host;
+    const int Nel = S.rad.Nel();
+    need(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    need(L >= 0 && L < S.N_L && iel >= 0 && iel < Nel, "(L, iel) out of range");
+    need(rows >= 0 && cols >= 0 && (data || rows * cols == 0), "bad dimensions");
+    const size_t n = (size_t)S.N_L * Nel;
+    if (S.rsmall.size() != n) {
+      S.rsmall.assign(n, Blk());
+      S.rbig.assign(n, Blk());
+      S.chol.assign(n, Blk());
+    }
+    Blk b(rows, cols);
+    if (rows * cols) std::memcpy(b.v.data(), data, sizeof(double) * (size_t)rows * cols);
+    std::vector<Blk> &v = kind == 0 ? S.rsmall : kind == 1 ? S.rbig : S.chol;
+    v[(size_t)L * Nel + iel] = std::move(b);
+    S.have_tei = false;
+    h->uploaded = false;
+  });
+}
+
+int hf_atomic_commit_tei(hf_atomic *h) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    AtomicSetup &S = h->host;
+    const int Nel = S.rad.Nel();
+    need(S.chol.size() == (size_t)S.N_L * Nel, "no caches have been set");
+    for (int L = 0; L < S.N_L; ++L)
+      for (int e = 0; e < Nel; ++e) {
+        const size_t q = (size_t)L * Nel + e;
+        const int n = S.rad.ni[e];
+        need(S.rsmall[q].r == n && S.rsmall[q].c == n, "r^L cache missing or of the wrong shape");
+        need(e == 0 || (S.rbig[q].r == n && S.rbig[q].c == n), "r^-L-1 cache missing or of the wrong shape");
+        need(S.chol[q].r == n * n, "Cholesky cache missing or of the wrong shape");
+      }
+    S.have_tei = true;
+    h->uploaded = false;
+  });
+}
+
+int hf_atomic_get_cache(const hf_atomic *h, int kind, int L, int iel, int dims[2], double *data) {
+  return guarded([&] {
+    need(h && dims, "NULL argument");
+    const AtomicSetup &S = h->host;
+    const int Nel = S.rad.Nel();
+    need(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    need(L >= 0 && L < S.N_L && iel >= 0 && iel < Nel, "(L, iel) out of range");
+    if (!S.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+    const std::vector<Blk> &v = kind == 0 ? S.rsmall : kind == 1 ? S.rbig : S.chol;
+    const Blk &b = v[(size_t)L * Nel + iel];
+    dims[0] = b.r;
+    dims[1] = b.c;
+    if (data && !b.v.empty()) std::memcpy(data, b.v.data(), sizeof(double) * b.v.size());
+  });
+}
+
+int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t ld) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    check_matrix(out, ld, h->host.Nbf(), "out");
+    if (which == 0)
+      h->host.overlap(out, ld);
+    else if (which == 1)
+      h->host.kinetic(out, ld);
+    else if (which == 2)
+      h->host.nuclear(out, ld);
+    else
+      throw Error(HF_EINVAL, "which must be 0 (overlap), 1 (kinetic) or 2 (nuclear)");
+  });
+}
+
+int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dJ, ldj, h->host.Nbf(), "J");
+    atomic_coulomb_launch(h->dev, dP, ldp, dJ, ldj, h->ctx->cur);
+  });
+}
+
+int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dK, ldk, h->host.Nbf(), "K");
+    atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur);
+  });
+}
+
+int hf_atomic_coulomb(hf_atomic *h, const double *P, int64_t ldp, double *J, int64_t ldj) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    host_roundtrip(h, P, ldp, J, ldj, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+      atomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur);
+    });
+  });
+}
+
+int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, int64_t ldk) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    host_roundtrip(h, P, ldp, K, ldk, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+      atomic_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur);
+    });
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Internal types shared by the C-ABI layer and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include <cstdint>
+#include "host/fem.hpp"
+
+namespace hfg {
+
+// Element tiles are padded to 16 x 16 (LIP-15 elements hold 14 or 15
+// functions), the MFMA f64 tile edge on CDNA.
+constexpr int TILE = 16;
+constexpr int TILE2 = TILE * TILE;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw Error(-3, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer owned by a handle.
+template <typename T>
+struct DevArray {
+  T *p = nullptr;
+  size_t n = 0;
+  DevArray() {}
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) return;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      throw Error(-4, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  }
+};
+
+// Everything the atomic J/K kernels read, flattened for the device.
+struct AtomicDevice {
+  int Nel = 0, Nrad = 0, Nang = 0, N_L = 0, n_lm = 0, n_act = 0;
+  DevArray<int> first, ni;                // per element
+  DevArray<double> rs, rb;                // [N_L][Nel][TILE2]
+  DevArray<double> chol;                  // packed M_p tiles
+  DevArray<int> chol_off, chol_rank;      // [N_L*Nel]
+  // Coulomb tables
+  DevArray<int> lm_L, lm_pstart, pair_kl; // pair_kl = k*Nang + l
+  DevArray<double> pair_c;
+  DevArray<int> out_start, out_lm;        // per output (i,j): lm list
+  DevArray<double> out_c;                 // gaunt * 4pi/(2L+1)
+  // Exchange tables
+  DevArray<int> act_jk;                   // active output blocks j*Nang+k
+  DevArray<int> term_start;               // [n_act*N_L + 1]
+  DevArray<int> term_il;                  // i*Nang + l
+  DevArray<double> term_c;                // 4pi/(2L+1) * G_j * G_k
+  // work buffers
+  DevArray<double> Jin, jsm, jbg, preA, preB, bnorm;
+  DevArray<double> hostP, hostOut;        // staging for host-pointer entry points
+  int max_rank = 0;
+  int64_t n_terms = 0, n_pairs = 0;
+};
+
+// launchers (atomic_jk.hip)
+void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
+                           hipStream_t s);
+void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                            hipStream_t s);
+void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
+
+}  // namespace hfg

@@ -1,0 +1,112 @@
+/*
+ * hfgpu: MI355X (gfx950) Fock-build engine for HelFEM -- C ABI.
+ *
+ * The entry points replace the Fock-build members of HelFEM's basis and
+ * grid classes.  Each declaration names the reference interface it stands
+ * in for (paths relative to the HelFEM source tree).
+ *
+ * Conventions
+ *   - Matrices are column-major doubles with an explicit leading dimension,
+ *     as helfem::Mat<double> (an Eigen column-major matrix,
+ *     libhelfem/include/Matrix.h:61).
+ *   - Outputs are caller-allocated.  Every function returns 0 on success and
+ *     a negative HF_E* code on failure; hf_last_error() then returns a
+ *     thread-local message.  No C++ exception crosses this boundary.
+ *   - One handle is stream-ordered and must not be used from two host
+ *     threads at once; different handles may be used concurrently.
+ *   - Functions ending in _dev take device pointers and are asynchronous on
+ *     the context stream; all others take host pointers and return when the
+ *     result is in host memory.
+ */
+#ifndef HFGPU_H
+#define HFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HF_OK 0
+#define HF_EINVAL (-1)   /* bad argument (shape, index, null pointer) */
+#define HF_ESTATE (-2)   /* call out of order, e.g. J/K before compute_tei */
+#define HF_EDEVICE (-3)  /* HIP runtime error */
+#define HF_ENOMEM (-4)   /* device or host allocation failed */
+#define HF_EINTERNAL (-5)
+
+typedef struct hf_ctx hf_ctx;
+typedef struct hf_atomic hf_atomic;
+
+/* Thread-local text of the last error. */
+const char *hf_last_error(void);
+/* Library version string. */
+const char *hf_version(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* Bind device `device` and create a stream-ordered context.  flags: 0. */
+int hf_ctx_create(int device, unsigned flags, hf_ctx **out);
+int hf_ctx_destroy(hf_ctx *ctx);
+/* Run subsequent work of this context on `stream` (a hipStream_t; NULL = the
+ * context's own stream). */
+int hf_ctx_set_stream(hf_ctx *ctx, void *stream);
+/* The hipStream_t work is issued on. */
+int hf_ctx_get_stream(hf_ctx *ctx, void **stream);
+int hf_ctx_synchronize(hf_ctx *ctx);
+
+/* ----------------------------------------------------------- atomic basis */
+
+/* Radial finite elements x complex Y_lm.  Replaces the constructor
+ * helfem::atomic::basis::TwoDBasisT<double>::TwoDBasisT
+ * (src/atomic/TwoDBasis.h:142-149) for a point nucleus, LIP shape functions
+ * on `nnodes` Gauss-Lobatto nodes (primbas 4), zero value at r=0 and
+ * r=bval[nbval-1].  bval: element boundaries (nbval = Nel+1); lval/mval:
+ * angular shells; nquad: radial rule size (0 -> 5*nnodes, main.cpp:265).
+ * ctx may be NULL: the handle then supports the host-side setup
+ * (compute_tei, caches, one-electron matrices) but not J/K. */
+int hf_atomic_create(hf_ctx *ctx, int Z, int nnodes, int nquad, const double *bval, int nbval,
+                     const int *lval, const int *mval, int nang, hf_atomic **out);
+int hf_atomic_destroy(hf_atomic *h);
+
+/* Sizes: out[0]=Nbf, out[1]=Nrad, out[2]=Nang, out[3]=Nel, out[4]=N_L. */
+int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]);
+/* First radial index and size of every element (FEMRadialBasisT::get_idx,
+ * libhelfem/include/RadialBasis.h:128). */
+int hf_atomic_elements(const hf_atomic *h, int *first, int *nprim);
+
+/* Build the per-(L, element) factor caches on the host and upload them:
+ * r^L and r^-L-1 element matrices and the pivoted Cholesky factor of the
+ * in-element Slater tensor.  Replaces TwoDBasisT::compute_tei
+ * (src/atomic/TwoDBasis.h:181, TwoDBasis.cpp:709-735). */
+int hf_atomic_compute_tei(hf_atomic *h, int exchange);
+
+/* Supply one cached factor computed elsewhere (e.g. by HelFEM's own
+ * compute_tei), instead of hf_atomic_compute_tei.  kind 0: disjoint_L
+ * (r^L, Ni x Ni), 1: disjoint_m1L (r^-L-1, Ni x Ni, ignored for iel 0),
+ * 2: prim_chol (Ni^2 x rank).  Caches are indexed like TwoDBasis.h:83-104.
+ * After every (kind, L, iel) has been set, call hf_atomic_commit_tei. */
+int hf_atomic_set_cache(hf_atomic *h, int kind, int L, int iel, int rows, int cols, const double *data);
+int hf_atomic_commit_tei(hf_atomic *h);
+/* Read a cached factor back.  dims gets (rows, cols); data may be NULL. */
+int hf_atomic_get_cache(const hf_atomic *h, int kind, int L, int iel, int dims[2], double *data);
+
+/* One-electron matrices (Nbf x Nbf): which 0 overlap (TwoDBasis.cpp:321),
+ * 1 kinetic (:354), 2 nuclear attraction (:377). */
+int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t ld);
+
+/* Coulomb matrix J(P) for the total density P.  Replaces
+ * TwoDBasisT::coulomb (TwoDBasis.h:223, TwoDBasis.cpp:774-877). */
+int hf_atomic_coulomb(hf_atomic *h, const double *P, int64_t ldp, double *J, int64_t ldj);
+/* Exchange contribution for the spin density P, returned with the sign the
+ * reference gives it (-K, to be added to the Fock matrix).  Replaces
+ * TwoDBasisT::exchange (TwoDBasis.h:225, TwoDBasis.cpp:880-999). */
+int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, int64_t ldk);
+/* Device-pointer forms of the two above, asynchronous on the context stream. */
+int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
+int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFGPU_H */

@@ -1,0 +1,174 @@
+// Oracle (test infrastructure only): C entry points for the Python tests.
+// Loaded by tests/ and by bench.py's cpu_baseline leg through ctypes.
+#include "orc.hpp"
+#include <cstring>
+#include <string>
+
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+template <typename F>
+int guard(F f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+void put(const Mat &m, double *out) { std::memcpy(out, m.data(), sizeof(double) * m.size()); }
+Mat get(const double *p, int r, int c) {
+  Mat m(r, c);
+  std::memcpy(m.data(), p, sizeof(double) * m.size());
+  return m;
+}
+}  // namespace
+
+extern "C" {
+
+const char *orc_last_error() { return g_err.c_str(); }
+
+int orc_lobatto(int n, double *x, double *w) {
+  return guard([&] {
+    std::vector<double> xx, ww;
+    lobatto(n, xx, ww);
+    std::memcpy(x, xx.data(), sizeof(double) * n);
+    std::memcpy(w, ww.data(), sizeof(double) * n);
+  });
+}
+
+int orc_chebyshev(int n, double *x, double *w) {
+  return guard([&] {
+    std::vector<double> xx, ww;
+    chebyshev(n, xx, ww);
+    std::memcpy(x, xx.data(), sizeof(double) * n);
+    std::memcpy(w, ww.data(), sizeof(double) * n);
+  });
+}
+
+int orc_grid(double rmax, int nel, int igrid, double zexp, double *out) {
+  return guard([&] {
+    const auto b = get_grid(rmax, nel, igrid, zexp);
+    std::memcpy(out, b.data(), sizeof(double) * b.size());
+  });
+}
+
+double orc_gaunt(int L, int M, int l, int m, int lp, int mp) { return gaunt_coefficient(L, M, l, m, lp, mp); }
+double orc_mod_gaunt(int lj, int mj, int L, int M, int li, int mi) {
+  return modified_gaunt_coefficient(lj, mj, L, M, li, mi);
+}
+
+// LIP primitives on Gauss-Lobatto nodes: out is (npts x nnodes)
+int orc_lip_eval(int nnodes, int npts, const double *x, int n, double *out) {
+  return guard([&] {
+    LIPBasis p(nnodes);
+    const Mat m = LIPBasis::eval_prim(std::vector<double>(x, x + npts), p.x0, n);
+    put(m, out);
+  });
+}
+
+// In-element radial 2e tensor at a fixed Lobatto order (no convergence loop)
+int orc_twoe_rule(double rmin, double rmax, int nquad, int nnodes, int drop_first, int drop_last, int L,
+                  double *out) {
+  return guard([&] {
+    LIPBasis p(nnodes);
+    if (drop_first) p.drop_first();
+    if (drop_last) p.drop_last();
+    std::vector<double> x, w;
+    lobatto(nquad, x, w);
+    put(twoe_integral_rule(rmin, rmax, x, w, p, L), out);
+  });
+}
+
+void *orc_atomic_create(int Z, int nnodes, int nquad, int nbval, const double *bval, int nang, const int *lval,
+                        const int *mval) {
+  try {
+    return new AtomicBasis(Z, nnodes, nquad, std::vector<double>(bval, bval + nbval),
+                           std::vector<int>(lval, lval + nang), std::vector<int>(mval, mval + nang));
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void orc_atomic_destroy(void *h) { delete (AtomicBasis *)h; }
+
+// out: Nbf, Nrad, Nel, N_L, Nang
+int orc_atomic_sizes(void *h, long *out) {
+  AtomicBasis *b = (AtomicBasis *)h;
+  out[0] = b->Nbf();
+  out[1] = b->Nrad();
+  out[2] = b->radial.Nel();
+  out[3] = b->N_L();
+  out[4] = b->Nang();
+  return 0;
+}
+
+// first index and size of every element
+int orc_atomic_elements(void *h, int *first, int *nprim) {
+  AtomicBasis *b = (AtomicBasis *)h;
+  for (int e = 0; e < b->radial.Nel(); ++e) {
+    first[e] = b->radial.first[e];
+    nprim[e] = b->radial.Nprim(e);
+  }
+  return 0;
+}
+
+int orc_atomic_compute_tei(void *h) { return guard([&] { ((AtomicBasis *)h)->compute_tei(); }); }
+
+// which: 0 overlap, 1 kinetic, 2 nuclear
+int orc_atomic_matrix(void *h, int which, double *out) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    Mat m = which == 0 ? b->overlap() : which == 1 ? b->kinetic() : b->nuclear();
+    put(m, out);
+  });
+}
+
+int orc_atomic_coulomb(void *h, const double *P, double *J) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    put(b->coulomb(get(P, b->Nbf(), b->Nbf())), J);
+  });
+}
+
+int orc_atomic_exchange(void *h, const double *P, double *K) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    put(b->exchange(get(P, b->Nbf(), b->Nbf())), K);
+  });
+}
+
+// Cached factor export.  kind: 0 disjoint_L (r^L), 1 disjoint_m1L (r^-L-1),
+// 2 prim_chol.  dims receives (rows, cols); out may be null to query dims.
+int orc_atomic_cache(void *h, int kind, int L, int iel, int *dims, double *out) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    const size_t idx = (size_t)L * b->radial.Nel() + iel;
+    const std::vector<Mat> &v = kind == 0 ? b->disjoint_L : kind == 1 ? b->disjoint_m1L : b->prim_chol;
+    if (idx >= v.size()) throw std::logic_error("cache index out of range");
+    dims[0] = v[idx].r;
+    dims[1] = v[idx].c;
+    if (out) put(v[idx], out);
+  });
+}
+
+int orc_atomic_twoe(void *h, int L, int iel, double *out) {
+  return guard([&] { put(((AtomicBasis *)h)->radial.twoe_integral(L, iel), out); });
+}
+
+int orc_atomic_radial_integral(void *h, int n, int iel, double *out) {
+  return guard([&] { put(((AtomicBasis *)h)->radial.radial_integral(n, iel), out); });
+}
+
+// Radial basis values R = B/r at arbitrary element coordinates x in [-1,1]
+int orc_atomic_radial_bf(void *h, int iel, int npts, const double *x, double *out) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    put(b->radial.get_bf(std::vector<double>(x, x + npts), iel), out);
+  });
+}
+
+}  // extern "C"

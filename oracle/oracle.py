@@ -1,0 +1,308 @@
+"""Python face of the CPU oracle (test infrastructure only).
+
+Loads oracle/_build/liboracle.so (built by oracle/Makefile) and exposes the
+restated HelFEM atomic basis plus a plain DIIS SCF driver.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this.
+
+The SCF here stands in for OpenOrbitalOptimizer (fetched at a3abae9 by the
+reference, CMakeLists.txt:310-341, absent here).  It follows the Fock-builder
+of src/atomic/main.cpp:369-470: per-symmetry-block Sinvh (scf_driver_common.h
+:176-187, utils.cpp:120-159), P = sum_k Sinvh_k C diag(occ) C' Sinvh_k'
+(scf_driver_common.h:407-425), Etot = tr(PT)+tr(PV)+1/2 tr(PJ)+Exc+Exx with
+Exx = 1/2 tr(Pa Ka) doubled when restricted (:460-474).  Converged energies do
+not depend on the iteration algorithm, only on reaching the same state.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "_build" / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"oracle library not built: {path} (run make -C oracle)")
+        L = ctypes.CDLL(str(path))
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_gaunt.restype = ctypes.c_double
+        L.orc_mod_gaunt.restype = ctypes.c_double
+        L.orc_atomic_create.restype = ctypes.c_void_p
+        L.orc_atomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        for name in ("orc_atomic_destroy", "orc_atomic_sizes", "orc_atomic_elements", "orc_atomic_compute_tei",
+                     "orc_atomic_matrix", "orc_atomic_coulomb", "orc_atomic_exchange", "orc_atomic_cache",
+                     "orc_atomic_twoe", "orc_atomic_radial_integral", "orc_atomic_radial_bf"):
+            getattr(L, name).argtypes = None
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+def gaunt(L, M, l, m, lp, mp):
+    return lib().orc_gaunt(L, M, l, m, lp, mp)
+
+
+def mod_gaunt(lj, mj, L, M, li, mi):
+    return lib().orc_mod_gaunt(lj, mj, L, M, li, mi)
+
+
+def lobatto(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(lib().orc_lobatto(n, _ptr(x), _ptr(w)))
+    return x, w
+
+
+def chebyshev(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(lib().orc_chebyshev(n, _ptr(x), _ptr(w)))
+    return x, w
+
+
+def grid(rmax, nel, igrid=4, zexp=2.0):
+    out = np.zeros(nel + 1)
+    _check(lib().orc_grid(ctypes.c_double(rmax), nel, igrid, ctypes.c_double(zexp), _ptr(out)))
+    return out
+
+
+def lip_eval(nnodes, x, n=0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((nnodes, x.size)).T.copy(order="F")
+    out = np.zeros(x.size * nnodes)
+    _check(lib().orc_lip_eval(nnodes, x.size, _ptr(x), n, _ptr(out)))
+    return out.reshape((nnodes, x.size)).T
+
+
+def twoe_rule(rmin, rmax, nquad, nnodes, drop_first, drop_last, L):
+    nb = nnodes - int(drop_first) - int(drop_last)
+    out = np.zeros(nb ** 4)
+    _check(lib().orc_twoe_rule(ctypes.c_double(rmin), ctypes.c_double(rmax), nquad, nnodes, int(drop_first),
+                               int(drop_last), L, _ptr(out)))
+    return out.reshape((nb * nb, nb * nb), order="F")
+
+
+def angular_basis(lmax, mmax):
+    """src/atomic/basis.cpp:179-203"""
+    lval, mval = [], []
+    for mabs in range(mmax + 1):
+        for l in range(mabs, lmax + 1):
+            lval.append(l)
+            mval.append(mabs)
+            if mabs > 0:
+                lval.append(l)
+                mval.append(-mabs)
+    return np.array(lval, dtype=np.int32), np.array(mval, dtype=np.int32)
+
+
+class AtomicBasis:
+    """Oracle restatement of helfem::atomic::basis::TwoDBasis (TwoDBasis.h:52-283)."""
+
+    def __init__(self, Z, lmax, mmax, nelem, rmax=40.0, igrid=4, zexp=2.0, nnodes=15, nquad=None,
+                 lval=None, mval=None, bval=None):
+        self.Z = Z
+        if lval is None:
+            lval, mval = angular_basis(lmax, mmax)
+        self.lval = np.ascontiguousarray(lval, dtype=np.int32)
+        self.mval = np.ascontiguousarray(mval, dtype=np.int32)
+        self.bval = grid(rmax, nelem, igrid, zexp) if bval is None else np.ascontiguousarray(bval, np.float64)
+        self.nnodes = nnodes
+        self.nquad = 5 * nnodes if nquad is None else nquad
+        h = lib().orc_atomic_create(Z, nnodes, self.nquad, self.bval.size, _ptr(self.bval), self.lval.size,
+                                    _ptr(self.lval), _ptr(self.mval))
+        if not h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+        s = (ctypes.c_long * 5)()
+        lib().orc_atomic_sizes(self._h, s)
+        self.Nbf, self.Nrad, self.Nel, self.N_L, self.Nang = (int(v) for v in s)
+        first = np.zeros(self.Nel, np.int32)
+        nprim = np.zeros(self.Nel, np.int32)
+        lib().orc_atomic_elements(self._h, _ptr(first), _ptr(nprim))
+        self.first, self.nprim = first, nprim
+        self._tei = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.orc_atomic_destroy(h)
+
+    def compute_tei(self):
+        _check(lib().orc_atomic_compute_tei(self._h))
+        self._tei = True
+
+    def _matrix(self, which):
+        out = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_matrix(self._h, which, _ptr(out)))
+        return out
+
+    def overlap(self):
+        return self._matrix(0)
+
+    def kinetic(self):
+        return self._matrix(1)
+
+    def nuclear(self):
+        return self._matrix(2)
+
+    def coulomb(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        J = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_coulomb(self._h, _ptr(P), _ptr(J)))
+        return J
+
+    def exchange(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_exchange(self._h, _ptr(P), _ptr(K)))
+        return K
+
+    def cache(self, kind, L, iel):
+        dims = (ctypes.c_int * 2)()
+        _check(lib().orc_atomic_cache(self._h, kind, L, iel, dims, None))
+        out = np.zeros(dims[0] * dims[1])
+        if out.size:
+            _check(lib().orc_atomic_cache(self._h, kind, L, iel, dims, _ptr(out)))
+        return out.reshape((dims[0], dims[1]), order="F")
+
+    def twoe(self, L, iel):
+        n = int(self.nprim[iel]) ** 2
+        out = np.zeros(n * n)
+        _check(lib().orc_atomic_twoe(self._h, L, iel, _ptr(out)))
+        return out.reshape((n, n), order="F")
+
+    def radial_integral(self, n, iel):
+        ni = int(self.nprim[iel])
+        out = np.zeros(ni * ni)
+        _check(lib().orc_atomic_radial_integral(self._h, n, iel, _ptr(out)))
+        return out.reshape((ni, ni), order="F")
+
+    def radial_bf(self, iel, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ni = int(self.nprim[iel])
+        out = np.zeros(x.size * ni)
+        _check(lib().orc_atomic_radial_bf(self._h, iel, x.size, _ptr(x), _ptr(out)))
+        return out.reshape((x.size, ni), order="F")
+
+    def sym_idx(self, symm=1):
+        """TwoDBasis.cpp:202-228: index list per unique m (symm=1) or one block."""
+        Nrad = self.Nrad
+        if symm == 0:
+            return [np.arange(self.Nbf)]
+        blocks = []
+        for m in sorted(set(self.mval.tolist())):
+            idx = [a * Nrad + i for a in range(self.Nang) if self.mval[a] == m for i in range(Nrad)]
+            blocks.append(np.array(idx))
+        return blocks
+
+
+def invh(S):
+    """utils.cpp:120-159 with chol=false: normalised symmetric orthonormalisation."""
+    d = 1.0 / np.sqrt(np.diag(S))
+    Sn = d[:, None] * S * d[None, :]
+    e, v = np.linalg.eigh(Sn)
+    X = (v * (1.0 / np.sqrt(e))) @ v.T
+    return d[:, None] * X
+
+
+class SCFResult(dict):
+    pass
+
+
+def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None):
+    """Restricted closed-shell HF with Pulay DIIS over per-m symmetry blocks.
+
+    jk(P, Pa) -> (J, Ka) may be supplied to swap in another Fock engine; the
+    default is the oracle's own coulomb/exchange."""
+    S = basis.overlap()
+    T = basis.kinetic()
+    V = basis.nuclear()
+    H = T + V if H1 is None else H1
+    blocks = basis.sym_idx(symm)
+    X = [invh(S[np.ix_(b, b)]) for b in blocks]
+    nocc = nelec // 2
+    if jk is None:
+        def jk(P, Pa):
+            return basis.coulomb(P), basis.exchange(Pa)
+
+    def density(F):
+        eps, vecs = [], []
+        for k, b in enumerate(blocks):
+            Fo = X[k].T @ F[np.ix_(b, b)] @ X[k]
+            e, c = np.linalg.eigh(Fo)
+            eps.append(e)
+            vecs.append(c)
+        allev = sorted((e, k, i) for k, ev in enumerate(eps) for i, e in enumerate(ev))
+        occ = [np.zeros(len(ev)) for ev in eps]
+        for e, k, i in allev[:nocc]:
+            occ[k][i] = 2.0
+        P = np.zeros_like(F)
+        for k, b in enumerate(blocks):
+            C = X[k] @ vecs[k]
+            P[np.ix_(b, b)] += (C * occ[k]) @ C.T
+        return P, occ
+
+    P, occ = density(H)
+    diis_F, diis_e = [], []
+    Eold = 0.0
+    it = 0
+    E = 0.0
+    for it in range(1, maxit + 1):
+        Pa = 0.5 * P
+        J, Ka = jk(P, Pa)
+        Ekin = np.sum(P * T)
+        Enuc = np.sum(P * V)
+        Ecoul = 0.5 * np.sum(P * J)
+        Exx = 2.0 * 0.5 * np.sum(Pa * Ka)
+        E = Ekin + Enuc + Ecoul + Exx
+        F = H + J + Ka
+        # orthonormal-basis commutator per block
+        errs = []
+        for k, b in enumerate(blocks):
+            Fo = X[k].T @ F[np.ix_(b, b)] @ X[k]
+            Sinv = np.linalg.pinv(X[k])
+            Po = Sinv @ P[np.ix_(b, b)] @ Sinv.T
+            errs.append((Fo @ Po - Po @ Fo).ravel())
+        err = np.concatenate(errs)
+        emax = np.max(np.abs(err))
+        if verbose:
+            print(f"it {it:3d} E {E:.12f} dE {E - Eold:.3e} err {emax:.3e}")
+        if it > 1 and abs(E - Eold) < conv and emax < 1e-6:
+            break
+        Eold = E
+        diis_F.append(F)
+        diis_e.append(err)
+        if len(diis_F) > 10:
+            diis_F.pop(0)
+            diis_e.pop(0)
+        n = len(diis_F)
+        B = -np.ones((n + 1, n + 1))
+        B[n, n] = 0.0
+        for i in range(n):
+            for j in range(n):
+                B[i, j] = diis_e[i] @ diis_e[j]
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:n]
+            Fd = sum(ci * Fi for ci, Fi in zip(c, diis_F))
+        except np.linalg.LinAlgError:
+            Fd = F
+        P, occ = density(Fd)
+    return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, iterations=it, P=P)
