@@ -295,7 +295,7 @@ int hf_ctx_destroy(hf_ctx *ctx) {
 int hf_ctx_set_stream(hf_ctx *ctx, void *stream) {
   return guarded([&] {
     need(ctx != nullptr, "ctx is NULL");
-    ctx->cur = stream ? (hipStream_t)stream : ctx->own;
+    ctx->cur = (hipStream_t)stream;
   });
 }
 

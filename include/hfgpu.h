@@ -47,8 +47,8 @@ const char *hf_version(void);
 /* Bind device `device` and create a stream-ordered context.  flags: 0. */
 int hf_ctx_create(int device, unsigned flags, hf_ctx **out);
 int hf_ctx_destroy(hf_ctx *ctx);
-/* Run subsequent work of this context on `stream` (a hipStream_t; NULL = the
- * context's own stream). */
+/* Run subsequent work of this context on `stream`, a hipStream_t (NULL = the
+ * default stream).  A new context starts on a stream of its own. */
 int hf_ctx_set_stream(hf_ctx *ctx, void *stream);
 /* The hipStream_t work is issued on. */
 int hf_ctx_get_stream(hf_ctx *ctx, void **stream);
