@@ -26,6 +26,9 @@ SIGNATURES = {
     "hf_ctx_set_stream": (c_int, [c_vp, c_vp]),
     "hf_ctx_get_stream": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "hf_ctx_synchronize": (c_int, [c_vp]),
+    "hf_ctx_set_timing": (c_int, [c_vp, c_int]),
+    "hf_ctx_kernel_stats": (c_int, [c_vp, ctypes.c_char_p, c_vp]),
+    "hf_atomic_work": (c_int, [c_vp, c_vp]),
     "hf_atomic_create": (c_int, [c_vp, c_int, c_int, c_int, c_dp, c_int, c_vp, c_vp, c_int, ctypes.POINTER(c_vp)]),
     "hf_atomic_destroy": (c_int, [c_vp]),
     "hf_atomic_sizes": (c_int, [c_vp, c_vp]),

@@ -292,25 +292,31 @@ void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStre
 }
 
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
-                           hipStream_t s) {
+                           hipStream_t s, KernelTimer *timer) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   fill_zero_launch(J, nbf, nbf, ldj, s);
   if (d.n_lm == 0) return;
   JArgs a{P, ldp, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.rs.p, d.rb.p, d.chol.p, d.chol_off.p,
           d.chol_rank.p, d.lm_L.p, d.lm_pstart.p, d.pair_kl.p, d.pair_c.p, d.Jin.p, d.jsm.p, d.jbg.p};
-  hipLaunchKernelGGL(k_atomic_j_gather, dim3(d.n_lm * d.Nel), dim3(NT), 0, s, a);
-  hip_check(hipGetLastError(), "atomic_j_gather");
+  {
+    TimedLaunch tl(timer, "atomic_j_gather", s);
+    hipLaunchKernelGGL(k_atomic_j_gather, dim3(d.n_lm * d.Nel), dim3(NT), 0, s, a);
+    hip_check(hipGetLastError(), "atomic_j_gather");
+  }
   hipLaunchKernelGGL(k_atomic_j_prefix, dim3((d.n_lm + 63) / 64), dim3(64), 0, s, d.n_lm, d.Nel, d.jsm.p,
                      d.jbg.p, d.preA.p, d.preB.p);
   hip_check(hipGetLastError(), "atomic_j_prefix");
   JOutArgs o{J, ldj, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.lm_L.p, d.rs.p, d.rb.p, d.Jin.p,
              d.preA.p, d.preB.p, d.out_start.p, d.out_lm.p, d.out_c.p};
-  hipLaunchKernelGGL(k_atomic_j_expand, dim3(d.Nang * d.Nang * d.Nel), dim3(NT), 0, s, o);
-  hip_check(hipGetLastError(), "atomic_j_expand");
+  {
+    TimedLaunch tl(timer, "atomic_j_expand", s);
+    hipLaunchKernelGGL(k_atomic_j_expand, dim3(d.Nang * d.Nang * d.Nel), dim3(NT), 0, s, o);
+    hip_check(hipGetLastError(), "atomic_j_expand");
+  }
 }
 
 void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                            hipStream_t s) {
+                            hipStream_t s, KernelTimer *timer) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   fill_zero_launch(K, nbf, nbf, ldk, s);
   if (d.n_act == 0) return;
@@ -322,6 +328,7 @@ void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp,
           d.bnorm.p, 10.0 * 2.220446049250313e-16};
   const int64_t grid = (int64_t)n_ef_pad * d.n_act;
   if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+  TimedLaunch tl(timer, "atomic_k", s, d.flops_k_radial + d.flops_k_angular);
   hipLaunchKernelGGL(k_atomic_k, dim3((unsigned)grid), dim3(NT), 0, s, a);
   hip_check(hipGetLastError(), "atomic_k");
 }

@@ -12,6 +12,7 @@ struct hf_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t cur = nullptr;
+  KernelTimer timer;
 };
 
 struct hf_atomic {
@@ -210,6 +211,33 @@ void atomic_upload(hf_atomic *h) {
     }
   d.n_act = (int)act.size();
   d.n_terms = (int64_t)til.size();
+  // Algorithmic flops of one exchange build: the Gaunt-weighted density sums
+  // over the full Nrad x Nrad blocks (2 flops per element per term) plus the
+  // radial RI-K contraction per coupled (output block, L): 4 r Ni^3 inside
+  // each element and 2 Ni Nj (Ni + Nj) per element pair.
+  {
+    double rad_in_L = 0.0;
+    std::vector<double> radL(nL, 0.0);
+    for (int L = 0; L < nL; ++L) {
+      double f = 0.0;
+      for (int e = 0; e < Nel; ++e) {
+        const double n = S.rad.ni[e];
+        f += 4.0 * S.chol[(size_t)L * Nel + e].c * n * n * n;
+        for (int g = 0; g < Nel; ++g)
+          if (g != e) {
+            const double m = S.rad.ni[g];
+            f += 2.0 * n * m * (n + m);
+          }
+      }
+      radL[L] = f;
+    }
+    for (size_t q = 0; q < act.size(); ++q)
+      for (int L = 0; L < nL; ++L)
+        if (tstart[q * nL + L + 1] > tstart[q * nL + L]) rad_in_L += radL[L];
+    d.flops_k_radial = rad_in_L;
+    d.flops_k_angular = 2.0 * (double)til.size() * (double)Nrad * (double)Nrad;
+    (void)rad_in_L;
+  }
   if (act.empty()) act.push_back(0);
   if (til.empty()) {
     til.push_back(0);
@@ -306,6 +334,31 @@ int hf_ctx_get_stream(hf_ctx *ctx, void **stream) {
   });
 }
 
+int hf_ctx_set_timing(hf_ctx *ctx, int on) {
+  return guarded([&] {
+    need(ctx != nullptr, "ctx is NULL");
+    ctx->timer.collect();
+    ctx->timer.on = on != 0;
+    if (!on) ctx->timer.acc.clear();
+  });
+}
+
+int hf_ctx_kernel_stats(hf_ctx *ctx, const char *name, double out[3]) {
+  return guarded([&] {
+    need(ctx && name && out, "NULL argument");
+    DeviceGuard g(ctx->device);
+    ctx->timer.collect();
+    auto it = ctx->timer.acc.find(name);
+    if (it == ctx->timer.acc.end()) {
+      out[0] = out[1] = out[2] = 0.0;
+      return;
+    }
+    out[0] = (double)it->second.count;
+    out[1] = it->second.ms;
+    out[2] = it->second.work;
+  });
+}
+
 int hf_ctx_synchronize(hf_ctx *ctx) {
   return guarded([&] {
     need(ctx != nullptr, "ctx is NULL");
@@ -353,6 +406,20 @@ int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]) {
     out[2] = h->host.Nang();
     out[3] = h->host.rad.Nel();
     out[4] = h->host.N_L;
+  });
+}
+
+int hf_atomic_work(hf_atomic *h, double out[4]) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    if (!h->host.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+    if (!h->ctx) throw Error(HF_ESTATE, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    if (!h->uploaded) atomic_upload(h);
+    out[0] = h->dev.flops_k_angular;
+    out[1] = h->dev.flops_k_radial;
+    out[2] = (double)h->dev.n_terms;
+    out[3] = (double)h->dev.n_act;
   });
 }
 
@@ -456,7 +523,7 @@ int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *d
     ensure_ready(h);
     check_matrix(dP, ldp, h->host.Nbf(), "P");
     check_matrix(dJ, ldj, h->host.Nbf(), "J");
-    atomic_coulomb_launch(h->dev, dP, ldp, dJ, ldj, h->ctx->cur);
+    atomic_coulomb_launch(h->dev, dP, ldp, dJ, ldj, h->ctx->cur, &h->ctx->timer);
   });
 }
 
@@ -468,7 +535,7 @@ int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *
     ensure_ready(h);
     check_matrix(dP, ldp, h->host.Nbf(), "P");
     check_matrix(dK, ldk, h->host.Nbf(), "K");
-    atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur);
+    atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur, &h->ctx->timer);
   });
 }
 
@@ -479,7 +546,7 @@ int hf_atomic_coulomb(hf_atomic *h, const double *P, int64_t ldp, double *J, int
     DeviceGuard g(h->ctx->device);
     ensure_ready(h);
     host_roundtrip(h, P, ldp, J, ldj, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
-      atomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur);
+      atomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
     });
   });
 }
@@ -491,7 +558,7 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
     DeviceGuard g(h->ctx->device);
     ensure_ready(h);
     host_roundtrip(h, P, ldp, K, ldk, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
-      atomic_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur);
+      atomic_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
     });
   });
 }

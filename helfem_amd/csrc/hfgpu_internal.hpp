@@ -5,6 +5,7 @@
 #include <vector>
 #include <stdexcept>
 #include <cstdint>
+#include <map>
 #include "host/fem.hpp"
 
 namespace hfg {
@@ -22,6 +23,60 @@ struct Error : std::runtime_error {
 inline void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw Error(-3, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+struct KernelTimer {
+  bool on = false;
+  struct Pending {
+    const char *name;
+    hipEvent_t a, b;
+    double work;
+  };
+  std::vector<Pending> pending;
+  struct Acc {
+    double ms = 0.0, work = 0.0;
+    long count = 0;
+  };
+  std::map<std::string, Acc> acc;
+  void begin(const char *name, hipStream_t s, double work, hipEvent_t *ev) {
+    if (!on) return;
+    hip_check(hipEventCreate(&ev[0]), "hipEventCreate");
+    hip_check(hipEventCreate(&ev[1]), "hipEventCreate");
+    hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+    pending.push_back({name, ev[0], ev[1], work});
+  }
+  void end(hipStream_t s, hipEvent_t *ev) {
+    if (!on) return;
+    hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+  }
+  void collect() {
+    for (auto &p : pending) {
+      hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      Acc &a = acc[p.name];
+      a.ms += ms;
+      a.work += p.work;
+      a.count += 1;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
+  }
+};
+
+// Scoped timing of one kernel launch.
+struct TimedLaunch {
+  KernelTimer *t;
+  hipStream_t s;
+  hipEvent_t ev[2];
+  TimedLaunch(KernelTimer *t_, const char *name, hipStream_t s_, double work = 0.0) : t(t_), s(s_) {
+    if (t) t->begin(name, s, work, ev);
+  }
+  ~TimedLaunch() {
+    if (t && t->on) (void)hipEventRecord(ev[1], s);
+  }
+};
 
 // Device buffer owned by a handle.
 template <typename T>
@@ -75,13 +130,15 @@ struct AtomicDevice {
   DevArray<double> hostP, hostOut;        // staging for host-pointer entry points
   int max_rank = 0;
   int64_t n_terms = 0, n_pairs = 0;
+  // algorithmic work per launch (flops), for the roofline report
+  double flops_k_radial = 0.0, flops_k_angular = 0.0, flops_j = 0.0;
 };
 
 // launchers (atomic_jk.hip)
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
-                           hipStream_t s);
+                           hipStream_t s, KernelTimer *timer);
 void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                            hipStream_t s);
+                            hipStream_t s, KernelTimer *timer);
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
 
 }  // namespace hfg

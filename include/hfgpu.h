@@ -53,6 +53,14 @@ int hf_ctx_set_stream(hf_ctx *ctx, void *stream);
 /* The hipStream_t work is issued on. */
 int hf_ctx_get_stream(hf_ctx *ctx, void **stream);
 int hf_ctx_synchronize(hf_ctx *ctx);
+/* Per-kernel timing: with timing on, every kernel launch of this context is
+ * bracketed by HIP events on its stream.  hf_ctx_kernel_stats returns, for
+ * kernel `name`, out[0] = launches, out[1] = summed milliseconds, out[2] =
+ * summed algorithmic flops.  Turning timing off clears the totals.  (No
+ * counterpart in the reference, which has only wall-clock FockTimer,
+ * src/general/scf_driver_common.h:69-170.) */
+int hf_ctx_set_timing(hf_ctx *ctx, int on);
+int hf_ctx_kernel_stats(hf_ctx *ctx, const char *name, double out[3]);
 
 /* ----------------------------------------------------------- atomic basis */
 
@@ -73,6 +81,12 @@ int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]);
 /* First radial index and size of every element (FEMRadialBasisT::get_idx,
  * libhelfem/include/RadialBasis.h:128). */
 int hf_atomic_elements(const hf_atomic *h, int *first, int *nprim);
+/* Algorithmic work of one exchange build: out[0] = flops of the angular
+ * (Gaunt-weighted density) sums over full Nrad x Nrad blocks, out[1] = flops
+ * of the radial RI-K contraction, out[2] = number of (output block, L,
+ * density block) terms, out[3] = number of coupled output blocks.  Counted
+ * as the reference's TwoDBasis.cpp:918-995 would perform them. */
+int hf_atomic_work(hf_atomic *h, double out[4]);
 
 /* Build the per-(L, element) factor caches on the host and upload them:
  * r^L and r^-L-1 element matrices and the pivoted Cholesky factor of the

@@ -215,6 +215,12 @@ Mat AtomicBasis::coulomb(const Mat &P) const {
 
 // TwoDBasis.cpp:880-999
 Mat AtomicBasis::exchange(const Mat &P) const {
+  std::vector<int> all;
+  for (int q = 0; q < Nang() * Nang(); ++q) all.push_back(q);
+  return exchange_blocks(P, all);
+}
+
+Mat AtomicBasis::exchange_blocks(const Mat &P, const std::vector<int> &jks) const {
   if (prim_chol.empty()) throw std::logic_error("Primitive teis have not been computed!");
   const int Nel = radial.Nel(), Nrad = radial.Nbf(), Na = Nang();
   const int lmx = *std::max_element(lval.begin(), lval.end());
@@ -225,6 +231,7 @@ Mat AtomicBasis::exchange(const Mat &P) const {
   // Frobenius norms of every radial block (the reference recomputes them
   // inside the (jang,kang) loop, TwoDBasis.cpp:950; the values are equal).
   std::vector<double> bnorm((size_t)Na * Na);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int i = 0; i < Na; ++i)
     for (int l = 0; l < Na; ++l) {
       double s = 0.0;
@@ -236,9 +243,11 @@ Mat AtomicBasis::exchange(const Mat &P) const {
       bnorm[(size_t)i * Na + l] = std::sqrt(s);
     }
   Mat K(Nbf(), Nbf());
-#pragma omp parallel for collapse(2) schedule(dynamic)
-  for (int j = 0; j < Na; ++j)
-    for (int k = 0; k < Na; ++k) {
+  const int njk = (int)jks.size();
+#pragma omp parallel for schedule(dynamic)
+  for (int q = 0; q < njk; ++q) {
+    {
+      const int j = jks[q] / Na, k = jks[q] % Na;
       const int lj = lval[j], mj = mval[j], lk = lval[k], mk = mval[k];
       std::vector<Mat> R(nL);
       std::vector<bool> couple(nL, false);
@@ -268,11 +277,12 @@ Mat AtomicBasis::exchange(const Mat &P) const {
       for (int L = 0; L < nL; ++L) {
         if (!couple[L]) continue;
         const Mat t = assemble_K_chol(radial, disjoint_L, disjoint_m1L, prim_chol, L * Nel, R[L]);
-        for (size_t q = 0; q < t.a.size(); ++q) Kb.a[q] += t.a[q];
+        for (size_t u = 0; u < t.a.size(); ++u) Kb.a[u] += t.a[u];
       }
       for (int b = 0; b < Nrad; ++b)
         for (int a = 0; a < Nrad; ++a) K(j * Nrad + a, k * Nrad + b) -= Kb(a, b);
     }
+  }
   return K;
 }
 

@@ -141,6 +141,13 @@ int orc_atomic_exchange(void *h, const double *P, double *K) {
   });
 }
 
+int orc_atomic_exchange_blocks(void *h, const double *P, double *K, int nblocks, const int *jk) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    put(b->exchange_blocks(get(P, b->Nbf(), b->Nbf()), std::vector<int>(jk, jk + nblocks)), K);
+  });
+}
+
 // Cached factor export.  kind: 0 disjoint_L (r^L), 1 disjoint_m1L (r^-L-1),
 // 2 prim_chol.  dims receives (rows, cols); out may be null to query dims.
 int orc_atomic_cache(void *h, int kind, int L, int iel, int *dims, double *out) {

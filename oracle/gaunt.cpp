@@ -255,10 +255,25 @@ double gaunt3(int l1, int m1, int l2, int m2, int l3, int m3) {
 
 }  // namespace
 
-// gaunt.cpp:87-93: int Y_L^M* Y_l^m Y_lp^mp = (-1)^M gaunt(L,-M,l,m,lp,mp)
+// gaunt.cpp:87-93: int Y_L^M* Y_l^m Y_lp^mp = (-1)^M gaunt(L,-M,l,m,lp,mp).
+// Exact values are memoised: the reference rebuilds its table on every
+// coulomb/exchange call (TwoDBasis.cpp:787-788) from a fast library; here the
+// exact evaluation is paid once per distinct argument tuple.
 double gaunt_coefficient(int L, int M, int l, int m, int lp, int mp) {
+  if (M - m - mp != 0 || ((L + l + lp) & 1) || lp < std::abs(L - l) || lp > L + l) return 0.0;
+  static std::mutex mtx;
+  static std::map<std::tuple<int, int, int, int, int, int>, double> memo;
+  const auto key = std::make_tuple(L, M, l, m, lp, mp);
+  {
+    std::lock_guard<std::mutex> g(mtx);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+  }
   const double sign = (M & 1) ? -1.0 : 1.0;
-  return sign * gaunt3(L, -M, l, m, lp, mp);
+  const double v = sign * gaunt3(L, -M, l, m, lp, mp);
+  std::lock_guard<std::mutex> g(mtx);
+  memo[key] = v;
+  return v;
 }
 
 // gaunt.cpp:99-112

@@ -39,7 +39,8 @@ def lib():
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         for name in ("orc_atomic_destroy", "orc_atomic_sizes", "orc_atomic_elements", "orc_atomic_compute_tei",
                      "orc_atomic_matrix", "orc_atomic_coulomb", "orc_atomic_exchange", "orc_atomic_cache",
-                     "orc_atomic_twoe", "orc_atomic_radial_integral", "orc_atomic_radial_bf"):
+                     "orc_atomic_twoe", "orc_atomic_radial_integral", "orc_atomic_radial_bf",
+                     "orc_atomic_exchange_blocks"):
             getattr(L, name).argtypes = None
         _LIB = L
     return _LIB
@@ -171,6 +172,14 @@ class AtomicBasis:
         P = np.asfortranarray(P, dtype=np.float64)
         K = np.zeros((self.Nbf, self.Nbf), order="F")
         _check(lib().orc_atomic_exchange(self._h, _ptr(P), _ptr(K)))
+        return K
+
+    def exchange_blocks(self, P, jk):
+        """-K restricted to the output blocks j*Nang+k listed in jk."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        jk = np.ascontiguousarray(jk, dtype=np.int32)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_exchange_blocks(self._h, _ptr(P), _ptr(K), jk.size, _ptr(jk)))
         return K
 
     def cache(self, kind, L, iel):

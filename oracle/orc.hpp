@@ -146,6 +146,8 @@ struct AtomicBasis {
   Mat nuclear() const;                // TwoDBasis.cpp:377-392 (point nucleus, Zl=Zr=0)
   Mat coulomb(const Mat &P) const;    // TwoDBasis.cpp:774-877
   Mat exchange(const Mat &P) const;   // TwoDBasis.cpp:880-999 (returns -K)
+  // Same, restricted to the listed output blocks j*Nang+k (CPU-baseline sample).
+  Mat exchange_blocks(const Mat &P, const std::vector<int> &jk) const;
 };
 // angular_basis, src/atomic/basis.cpp:179-203
 void angular_basis(int lmax, int mmax, std::vector<int> &lval, std::vector<int> &mval);
