@@ -198,18 +198,34 @@ struct KArgs {
   int64_t ldp;
   double *K;
   int64_t ldk;
-  int Nel, Nrad, Nang, N_L, n_act, n_ef_pad;
+  int Nel, Nrad, Nang, N_L, n_act;
   const int *first, *ni;
   const double *rs, *rb, *chol;
   const int *chol_off, *chol_rank;
   const int *act_jk, *term_start, *term_il;
-  const double *term_c, *bnorm;
-  double thr;
+  const double *term_c;
 };
 
-// One workgroup per (active output block (j,k), element pair (e,f)).  The
-// workgroup index is mapped so that consecutive workgroups on one XCD share
-// the element pair (e,f): their P sub-blocks then stay in that XCD's L2.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// D(16x16) += X(16x16) * Y(16x16), one wave, v_mfma_f64_16x16x4f64.
+// Operand layout (lane l, g = l/16, c = l%16): the k index of step s is
+// 4g+s, so lane l supplies X(c, 4g+s) as xs[s] and Y'(c, 4g+s) i.e.
+// Y(4g+s, c) as ys[s]; the result lands as D(4g+r, c) in component r.
+__device__ inline v4d mfma_16x16x16(const double xs[4], const double ys[4], v4d d) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[s], ys[s], d, 0, 0, 0);
+  return d;
+}
+
+// One workgroup per (coupled output block (j,k), element pair (e,f)); the
+// four waves split the multipoles L.  For each L a wave forms the
+// Gaunt-weighted density tile R_L(e,f) = sum_terms c * P_il(e,f) directly in
+// MFMA operand layout (lane (g,c) holds R(c, 4g+s)), then contracts it with
+// the radial factors on the matrix cores:
+//   e != f:  K += A R B'       (A, B the r^L / r^-L-1 element matrices)
+//   e == f:  K += sum_p M_p R M_p'   (M_p the Cholesky vectors as 16x16)
+// Workgroups are mapped so that those sharing (e,f) run on one XCD.
 __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
   const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
@@ -218,66 +234,73 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
   if (ef >= A.Nel * A.Nel) return;
   const int e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
-  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
-  const bool inside = (a < ni && b < nj);
-  __shared__ double sR[TILE2], sA[TILE2], sB[TILE2], sY[TILE2];
-  double acc = 0.0;
-  for (int L = 0; L < A.N_L; ++L) {
+  __shared__ double red[4 * TILE2];
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  const bool rowok = c < ni;
+  bool colok[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) colok[s] = (4 * g + s) < nj;
+  for (int L = wv; L < A.N_L; L += 4) {
     const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
     if (q0 == q1) continue;
-    double r = 0.0;
-    bool any = false;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = q0; q < q1; ++q) {
       const int il = A.term_il[q];
-      if (A.bnorm[il] < A.thr) continue;
-      any = true;
-      if (inside) {
-        const int i = il / A.Nang, l = il - i * A.Nang;
-        r += A.term_c[q] * A.P[(int64_t)(i * A.Nrad + fi + a) + (int64_t)(l * A.Nrad + fj + b) * A.ldp];
+      const double cq = A.term_c[q];
+      const int i = il / A.Nang, l = il - i * A.Nang;
+      const double *Pb = A.P + (int64_t)(i * A.Nrad + fi + c) + (int64_t)(l * A.Nrad + fj + 4 * g) * A.ldp;
+      if (rowok) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          if (colok[s]) r[s] += cq * Pb[(int64_t)s * A.ldp];
       }
     }
-    if (!any) continue;  // uniform over the workgroup
-    sR[t] = r;
+    const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
     if (e != f) {
-      const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
-      sA[t] = (e > f) ? A.rb[te + t] : A.rs[te + t];
-      sB[t] = (e > f) ? A.rs[tf + t] : A.rb[tf + t];
-      __syncthreads();
-      double y = 0.0;
+      const double *At = (e > f) ? A.rb + te : A.rs + te;
+      const double *Bt = (e > f) ? A.rs + tf : A.rb + tf;
+      double as[4], bs[4];
 #pragma unroll
-      for (int c = 0; c < TILE; ++c) y += sR[a + TILE * c] * sB[b + TILE * c];
-      sY[t] = y;
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < TILE; ++c) acc += sA[a + TILE * c] * sY[c + TILE * b];
-      __syncthreads();
+      for (int s = 0; s < 4; ++s) {
+        as[s] = At[c + TILE * (4 * g + s)];
+        bs[s] = Bt[c + TILE * (4 * g + s)];
+      }
+      v4d y = {0.0, 0.0, 0.0, 0.0};
+      y = mfma_16x16x16(r, bs, y);           // Y = R B'
+      const double ys[4] = {y[0], y[1], y[2], y[3]};
+      acc = mfma_16x16x16(as, ys, acc);      // K += A Y
     } else {
       const int rank = A.chol_rank[L * A.Nel + e];
       const double *M = A.chol + A.chol_off[L * A.Nel + e];
       for (int p = 0; p < rank; ++p) {
-        sA[t] = M[(size_t)p * TILE2 + t];
-        __syncthreads();
-        double y = 0.0;
+        const double *Mp = M + (size_t)p * TILE2;
+        double ms[4];
 #pragma unroll
-        for (int c = 0; c < TILE; ++c) y += sR[a + TILE * c] * sA[b + TILE * c];
-        sY[t] = y;
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < TILE; ++c) acc += sA[a + TILE * c] * sY[c + TILE * b];
-        __syncthreads();
+        for (int s = 0; s < 4; ++s) ms[s] = Mp[c + TILE * (4 * g + s)];
+        v4d y = {0.0, 0.0, 0.0, 0.0};
+        y = mfma_16x16x16(r, ms, y);         // Y = R M_p'
+        const double ys[4] = {y[0], y[1], y[2], y[3]};
+        acc = mfma_16x16x16(ms, ys, acc);    // K += M_p Y
       }
     }
   }
-  if (!inside) return;
+  // combine the four waves' partial tiles (lane holds K(4g+r, c))
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (4 * g + rr) + TILE * c] = acc[rr];
+  __syncthreads();
+  const int a = t & 15, b = t >> 4;
+  if (a >= ni || b >= nj) return;
+  const double v = red[t] + red[TILE2 + t] + red[2 * TILE2 + t] + red[3 * TILE2 + t];
   const int64_t row = (int64_t)j * A.Nrad + fi + a, col = (int64_t)k * A.Nrad + fj + b;
   const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
   const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
   double *dst = A.K + row + col * A.ldk;
   if (srow || scol)
-    atomicAdd(dst, -acc);
+    atomicAdd(dst, -v);
   else
-    *dst = -acc;
+    *dst = -v;
 }
 
 }  // namespace
@@ -315,17 +338,72 @@ void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, 
   }
 }
 
-void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+// Compact the static coupling lists to the density blocks that pass the
+// norm screen (|P_il|_F >= 10 eps, TwoDBasis.cpp:949-952).  The result only
+// changes when the screening pattern does, so it is cached by pattern.
+static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
+  const int Na = d.Nang, nL = d.N_L;
+  std::vector<unsigned char> mask((size_t)Na * Na);
+  const double thr = 10.0 * 2.220446049250313e-16;
+  for (size_t q = 0; q < mask.size(); ++q) mask[q] = d.h_bnorm.p[q] >= thr ? 1 : 0;
+  if (mask == d.screen_mask && !d.screen_mask.empty()) return;
+  std::vector<int> act, tstart(1, 0), til;
+  std::vector<double> tc;
+  double frad = 0.0;
+  for (size_t a = 0; a < d.h_act.size(); ++a) {
+    const size_t t0 = tstart.size(), n0 = til.size();
+    std::vector<int> starts;
+    for (int L = 0; L < nL; ++L) {
+      const int q0 = d.h_tstart[a * nL + L], q1 = d.h_tstart[a * nL + L + 1];
+      const size_t before = til.size();
+      for (int q = q0; q < q1; ++q)
+        if (mask[d.h_til[q]]) {
+          til.push_back(d.h_til[q]);
+          tc.push_back(d.h_tc[q]);
+        }
+      if (til.size() > before) frad += d.h_radL[L];
+      starts.push_back((int)til.size());
+    }
+    if (til.size() == n0) continue;  // nothing reaches this output block
+    act.push_back(d.h_act[a]);
+    tstart.insert(tstart.end(), starts.begin(), starts.end());
+    (void)t0;
+  }
+  d.n_act = (int)act.size();
+  d.n_terms = (int64_t)til.size();
+  d.flops_k_radial = frad;
+  d.flops_k_angular = 2.0 * (double)til.size() * (double)d.Nrad * (double)d.Nrad;
+  if (act.empty()) act.push_back(0);
+  if (til.empty()) {
+    til.push_back(0);
+    tc.push_back(0.0);
+  }
+  d.act_jk.upload(act, s);
+  d.term_start.upload(tstart, s);
+  d.term_il.upload(til, s);
+  d.term_c.upload(tc, s);
+  hip_check(hipStreamSynchronize(s), "exchange list upload");
+  d.screen_mask = mask;
+}
+
+void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
-  fill_zero_launch(K, nbf, nbf, ldk, s);
-  if (d.n_act == 0) return;
+  if (d.h_act.empty()) {
+    fill_zero_launch(K, nbf, nbf, ldk, s);
+    return;
+  }
   hipLaunchKernelGGL(k_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.bnorm.p);
   hip_check(hipGetLastError(), "block_norms");
+  hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
+            "copy block norms");
+  hip_check(hipStreamSynchronize(s), "block norms sync");
+  compact_exchange_lists(d, s);
+  fill_zero_launch(K, nbf, nbf, ldk, s);
+  if (d.n_act == 0) return;
   const int n_ef_pad = ((d.Nel * d.Nel + 7) / 8) * 8;
-  KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, n_ef_pad, d.first.p, d.ni.p, d.rs.p, d.rb.p,
-          d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p,
-          d.bnorm.p, 10.0 * 2.220446049250313e-16};
+  KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
+          d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p};
   const int64_t grid = (int64_t)n_ef_pad * d.n_act;
   if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
   TimedLaunch tl(timer, "atomic_k", s, d.flops_k_radial + d.flops_k_angular);

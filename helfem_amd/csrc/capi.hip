@@ -209,45 +209,30 @@ void atomic_upload(hf_atomic *h) {
         tstart.push_back((int)til.size());
       }
     }
-  d.n_act = (int)act.size();
-  d.n_terms = (int64_t)til.size();
-  // Algorithmic flops of one exchange build: the Gaunt-weighted density sums
-  // over the full Nrad x Nrad blocks (2 flops per element per term) plus the
-  // radial RI-K contraction per coupled (output block, L): 4 r Ni^3 inside
-  // each element and 2 Ni Nj (Ni + Nj) per element pair.
-  {
-    double rad_in_L = 0.0;
-    std::vector<double> radL(nL, 0.0);
-    for (int L = 0; L < nL; ++L) {
-      double f = 0.0;
-      for (int e = 0; e < Nel; ++e) {
-        const double n = S.rad.ni[e];
-        f += 4.0 * S.chol[(size_t)L * Nel + e].c * n * n * n;
-        for (int g = 0; g < Nel; ++g)
-          if (g != e) {
-            const double m = S.rad.ni[g];
-            f += 2.0 * n * m * (n + m);
-          }
-      }
-      radL[L] = f;
+  // Radial RI-K flops per coupled (output block, L): 4 r Ni^3 inside each
+  // element and 2 Ni Nj (Ni + Nj) per element pair.
+  d.h_radL.assign(nL, 0.0);
+  for (int L = 0; L < nL; ++L) {
+    double f = 0.0;
+    for (int e = 0; e < Nel; ++e) {
+      const double n = S.rad.ni[e];
+      f += 4.0 * S.chol[(size_t)L * Nel + e].c * n * n * n;
+      for (int g = 0; g < Nel; ++g)
+        if (g != e) {
+          const double m = S.rad.ni[g];
+          f += 2.0 * n * m * (n + m);
+        }
     }
-    for (size_t q = 0; q < act.size(); ++q)
-      for (int L = 0; L < nL; ++L)
-        if (tstart[q * nL + L + 1] > tstart[q * nL + L]) rad_in_L += radL[L];
-    d.flops_k_radial = rad_in_L;
-    d.flops_k_angular = 2.0 * (double)til.size() * (double)Nrad * (double)Nrad;
-    (void)rad_in_L;
+    d.h_radL[L] = f;
   }
-  if (act.empty()) act.push_back(0);
-  if (til.empty()) {
-    til.push_back(0);
-    tc.push_back(0.0);
-  }
-  d.act_jk.upload(act, s);
-  d.term_start.upload(tstart, s);
-  d.term_il.upload(til, s);
-  d.term_c.upload(tc, s);
-
+  d.h_act = act;
+  d.h_tstart = tstart;
+  d.h_til = til;
+  d.h_tc = tc;
+  d.screen_mask.clear();
+  d.n_act = 0;
+  d.n_terms = 0;
+  d.h_bnorm.alloc((size_t)Na * Na);
   d.Jin.alloc((size_t)std::max(nlm, 1) * Nel * TILE2);
   d.jsm.alloc((size_t)std::max(nlm, 1) * Nel);
   d.jbg.alloc((size_t)std::max(nlm, 1) * Nel);

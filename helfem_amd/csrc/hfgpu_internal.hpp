@@ -108,6 +108,28 @@ struct DevArray {
   }
 };
 
+// Pinned host buffer (hipHostMalloc).
+template <typename T>
+struct PinnedArray {
+  T *p = nullptr;
+  size_t n = 0;
+  PinnedArray() {}
+  PinnedArray(const PinnedArray &) = delete;
+  PinnedArray &operator=(const PinnedArray &) = delete;
+  ~PinnedArray() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess)
+      throw Error(-4, "pinned host allocation failed");
+    n = count;
+  }
+};
+
 // Everything the atomic J/K kernels read, flattened for the device.
 struct AtomicDevice {
   int Nel = 0, Nrad = 0, Nang = 0, N_L = 0, n_lm = 0, n_act = 0;
@@ -120,11 +142,17 @@ struct AtomicDevice {
   DevArray<double> pair_c;
   DevArray<int> out_start, out_lm;        // per output (i,j): lm list
   DevArray<double> out_c;                 // gaunt * 4pi/(2L+1)
-  // Exchange tables
+  // Exchange tables: the static (density-independent) coupling lists live on
+  // the host; the device holds the lists compacted to the density blocks
+  // that survive the reference's norm screen (TwoDBasis.cpp:949-952).
+  std::vector<int> h_act, h_tstart, h_til;
+  std::vector<double> h_tc, h_radL;
+  std::vector<unsigned char> screen_mask;  // mask the compacted lists were built for
   DevArray<int> act_jk;                   // active output blocks j*Nang+k
   DevArray<int> term_start;               // [n_act*N_L + 1]
   DevArray<int> term_il;                  // i*Nang + l
   DevArray<double> term_c;                // 4pi/(2L+1) * G_j * G_k
+  PinnedArray<double> h_bnorm;
   // work buffers
   DevArray<double> Jin, jsm, jbg, preA, preB, bnorm;
   DevArray<double> hostP, hostOut;        // staging for host-pointer entry points
@@ -137,7 +165,7 @@ struct AtomicDevice {
 // launchers (atomic_jk.hip)
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                            hipStream_t s, KernelTimer *timer);
-void atomic_exchange_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer);
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
 

@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): C entry points for the Python tests.
 // Loaded by tests/ and by bench.py's cpu_baseline leg through ctypes.
 #include "orc.hpp"
+#include "diatomic.hpp"
 #include <cstring>
 #include <string>
 
@@ -148,6 +149,13 @@ int orc_atomic_exchange_blocks(void *h, const double *P, double *K, int nblocks,
   });
 }
 
+int orc_legendre(int lmax, int mmax, double x, double *P, double *Q) {
+  return guard([&] {
+    legendre_plm(P, lmax, mmax, x);
+    legendre_qlm(Q, lmax, mmax, x);
+  });
+}
+
 // Cached factor export.  kind: 0 disjoint_L (r^L), 1 disjoint_m1L (r^-L-1),
 // 2 prim_chol.  dims receives (rows, cols); out may be null to query dims.
 int orc_atomic_cache(void *h, int kind, int L, int iel, int *dims, double *out) {
@@ -175,6 +183,86 @@ int orc_atomic_radial_bf(void *h, int iel, int npts, const double *x, double *ou
   return guard([&] {
     AtomicBasis *b = (AtomicBasis *)h;
     put(b->radial.get_bf(std::vector<double>(x, x + npts), iel), out);
+  });
+}
+
+// ---- diatomic ----
+void *orc_diatomic_create(int Z1, int Z2, double Rhalf, int nnodes, int nquad, int nbval, const double *bval,
+                          int nang, const int *lval, const int *mval) {
+  try {
+    return new DiatomicBasis(Z1, Z2, Rhalf, nnodes, nquad, std::vector<double>(bval, bval + nbval),
+                             std::vector<int>(lval, lval + nang), std::vector<int>(mval, mval + nang));
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_diatomic_destroy(void *h) { delete (DiatomicBasis *)h; }
+// out: Nbf, Ndummy, Nrad, Nang, Nel, n_lm, n_LM
+int orc_diatomic_sizes(void *h, long *out) {
+  DiatomicBasis *b = (DiatomicBasis *)h;
+  out[0] = b->Nbf();
+  out[1] = b->Ndummy();
+  out[2] = b->Nrad();
+  out[3] = b->Nang();
+  out[4] = b->radial.Nel();
+  out[5] = (long)b->lm_map.size();
+  out[6] = (long)b->LM_map.size();
+  return 0;
+}
+int orc_diatomic_pure_idx(void *h, int *out) {
+  DiatomicBasis *b = (DiatomicBasis *)h;
+  for (size_t i = 0; i < b->pure_idx.size(); ++i) out[i] = b->pure_idx[i];
+  return 0;
+}
+int orc_diatomic_compute_tei(void *h) { return guard([&] { ((DiatomicBasis *)h)->compute_tei(); }); }
+int orc_diatomic_matrix(void *h, int which, double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    put(which == 0 ? b->overlap() : which == 1 ? b->kinetic() : b->nuclear(), out);
+  });
+}
+int orc_diatomic_coulomb(void *h, const double *P, double *J) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    put(b->coulomb(get(P, b->Nbf(), b->Nbf())), J);
+  });
+}
+int orc_diatomic_exchange(void *h, const double *P, double *K) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    put(b->exchange(get(P, b->Nbf(), b->Nbf())), K);
+  });
+}
+// lm map entry i: out = (L, |M|); LM map similarly (L, M)
+int orc_diatomic_maps(void *h, int *lm, int *LM) {
+  DiatomicBasis *b = (DiatomicBasis *)h;
+  for (size_t i = 0; i < b->lm_map.size(); ++i) {
+    lm[2 * i] = b->lm_map[i].first;
+    lm[2 * i + 1] = b->lm_map[i].second;
+  }
+  for (size_t i = 0; i < b->LM_map.size(); ++i) {
+    LM[2 * i] = b->LM_map[i].first;
+    LM[2 * i + 1] = b->LM_map[i].second;
+  }
+  return 0;
+}
+// kind 0 P0, 1 P2, 2 Q0, 3 Q2, 4 cd_B, 5 cd_sigma (as a column)
+int orc_diatomic_cache(void *h, int kind, int ilm, int iel, int *dims, double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    const size_t i = (size_t)ilm * b->radial.Nel() + iel;
+    if (i >= b->cd_B.size()) throw std::logic_error("cache index out of range");
+    if (kind == 5) {
+      dims[0] = (int)b->cd_sigma[i].size();
+      dims[1] = 1;
+      if (out) std::memcpy(out, b->cd_sigma[i].data(), sizeof(double) * b->cd_sigma[i].size());
+      return;
+    }
+    const Mat &m = kind == 0 ? b->P0[i] : kind == 1 ? b->P2[i] : kind == 2 ? b->Q0[i] : kind == 3 ? b->Q2[i] : b->cd_B[i];
+    dims[0] = m.r;
+    dims[1] = m.c;
+    if (out) put(m, out);
   });
 }
 

@@ -235,7 +235,8 @@ Mat LIPBasis::eval_over_r(const std::vector<double> &x, int n, double element_le
   return out;
 }
 
-RadialBasis::RadialBasis(int nnodes, const std::vector<double> &bv, int nquad) : poly(nnodes), bval(bv) {
+RadialBasis::RadialBasis(int nnodes, const std::vector<double> &bv, int nquad, bool dl)
+    : poly(nnodes), bval(bv), dirichlet_left(dl) {
   const int nel = Nel();
   first.assign(nel, 0);
   last.assign(nel, 0);
@@ -248,7 +249,7 @@ RadialBasis::RadialBasis(int nnodes, const std::vector<double> &bv, int nquad) :
 
 LIPBasis RadialBasis::get_basis(int iel) const {
   LIPBasis p = poly;
-  if (iel == 0) p.drop_first();
+  if (iel == 0 && dirichlet_left) p.drop_first();
   if (iel == Nel() - 1) p.drop_last();
   return p;
 }
@@ -301,6 +302,11 @@ Mat matel_fixed(const RadialBasis &rb, int iel, int lk, int rk, const std::vecto
   return matmul_tn(lh, rh);
 }
 }  // namespace
+
+Mat RadialBasis::matel_rule(int iel, int lk, int rk, const std::vector<double> &x, const std::vector<double> &w,
+                            const std::function<double(double)> *f) const {
+  return matel_fixed(*this, iel, lk, rk, x, w, f);
+}
 
 // converge_panel, FiniteElementBasis.cpp:500-553, with the start order of
 // matrix_element_auto (:556-604): basis degree nnodes-1, one panel [-1,1].

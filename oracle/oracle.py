@@ -40,8 +40,15 @@ def lib():
         for name in ("orc_atomic_destroy", "orc_atomic_sizes", "orc_atomic_elements", "orc_atomic_compute_tei",
                      "orc_atomic_matrix", "orc_atomic_coulomb", "orc_atomic_exchange", "orc_atomic_cache",
                      "orc_atomic_twoe", "orc_atomic_radial_integral", "orc_atomic_radial_bf",
-                     "orc_atomic_exchange_blocks"):
+                     "orc_atomic_exchange_blocks", "orc_legendre", "orc_diatomic_destroy",
+                     "orc_diatomic_sizes", "orc_diatomic_pure_idx", "orc_diatomic_compute_tei",
+                     "orc_diatomic_matrix", "orc_diatomic_coulomb", "orc_diatomic_exchange",
+                     "orc_diatomic_maps", "orc_diatomic_cache"):
             getattr(L, name).argtypes = None
+        L.orc_diatomic_create.restype = ctypes.c_void_p
+        L.orc_diatomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -221,6 +228,112 @@ class AtomicBasis:
         return blocks
 
 
+def legendre(lmax, mmax, x):
+    """P_l^m(x), Q_l^m(x) as (lmax+1, mmax+1) arrays (src/legendre/Legendre.h)."""
+    P = np.zeros((mmax + 1) * (lmax + 1))
+    Q = np.zeros_like(P)
+    _check(lib().orc_legendre(lmax, mmax, ctypes.c_double(x), _ptr(P), _ptr(Q)))
+    return P.reshape((mmax + 1, lmax + 1)).T, Q.reshape((mmax + 1, lmax + 1)).T
+
+
+def diatomic_angular(lmmax):
+    """lm_to_l_m, src/diatomic/basis.cpp:500-516."""
+    lval, mval = [], []
+    for mabs, lm in enumerate(lmmax):
+        for l in range(mabs, lm + 1):
+            lval.append(l)
+            mval.append(mabs)
+            if mabs > 0:
+                lval.append(l)
+                mval.append(-mabs)
+    return np.array(lval, dtype=np.int32), np.array(mval, dtype=np.int32)
+
+
+class DiatomicBasis:
+    """Oracle restatement of helfem::diatomic::basis::TwoDBasis (basis.h:127-432).
+
+    Rbond in bohr; mu_max = arcosh(Rmax / Rhalf); elements on the igrid=4
+    grid with zexp = 1 (the diatomic driver's defaults, main.cpp:70-72)."""
+
+    def __init__(self, Z1, Z2, Rbond, lmmax, nelem, rmax=40.0, igrid=4, zexp=1.0, nnodes=15, nquad=None):
+        self.Z1, self.Z2, self.Rbond = Z1, Z2, Rbond
+        self.Rhalf = 0.5 * Rbond
+        self.lval, self.mval = diatomic_angular(lmmax)
+        self.mumax = float(np.arccosh(max(rmax / self.Rhalf, 1.0)))
+        self.bval = grid(self.mumax, nelem, igrid, zexp)
+        self.nquad = 5 * nnodes if nquad is None else nquad
+        h = lib().orc_diatomic_create(Z1, Z2, self.Rhalf, nnodes, self.nquad, self.bval.size, _ptr(self.bval),
+                                      self.lval.size, _ptr(self.lval), _ptr(self.mval))
+        if not h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+        sz = (ctypes.c_long * 7)()
+        lib().orc_diatomic_sizes(self._h, sz)
+        self.Nbf, self.Ndummy, self.Nrad, self.Nang, self.Nel, self.n_lm, self.n_LM = (int(v) for v in sz)
+        self.pure_idx = np.zeros(self.Nbf, np.int32)
+        lib().orc_diatomic_pure_idx(self._h, _ptr(self.pure_idx))
+        self.Enucr = Z1 * Z2 / Rbond
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _LIB is not None:
+            _LIB.orc_diatomic_destroy(h)
+
+    def compute_tei(self):
+        _check(lib().orc_diatomic_compute_tei(self._h))
+
+    def _matrix(self, which):
+        out = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_diatomic_matrix(self._h, which, _ptr(out)))
+        return out
+
+    def overlap(self):
+        return self._matrix(0)
+
+    def kinetic(self):
+        return self._matrix(1)
+
+    def nuclear(self):
+        return self._matrix(2)
+
+    def coulomb(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        J = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_diatomic_coulomb(self._h, _ptr(P), _ptr(J)))
+        return J
+
+    def exchange(self, P):
+        P = np.asfortranarray(P, dtype=np.float64)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_diatomic_exchange(self._h, _ptr(P), _ptr(K)))
+        return K
+
+    def maps(self):
+        lm = np.zeros(2 * self.n_lm, np.int32)
+        LM = np.zeros(2 * self.n_LM, np.int32)
+        lib().orc_diatomic_maps(self._h, _ptr(lm), _ptr(LM))
+        return lm.reshape(-1, 2), LM.reshape(-1, 2)
+
+    def cache(self, kind, ilm, iel):
+        dims = (ctypes.c_int * 2)()
+        _check(lib().orc_diatomic_cache(self._h, kind, ilm, iel, dims, None))
+        out = np.zeros(dims[0] * dims[1])
+        if out.size:
+            _check(lib().orc_diatomic_cache(self._h, kind, ilm, iel, dims, _ptr(out)))
+        return out.reshape((dims[0], dims[1]), order="F")
+
+    def sym_idx(self, symm=1):
+        """get_sym_idx(1), basis.cpp:913-935: pure indices grouped by m."""
+        if symm == 0:
+            return [np.arange(self.Nbf)]
+        blocks, off = {}, 0
+        for a in range(self.Nang):
+            n = self.Nrad if self.mval[a] == 0 else self.Nrad - 1
+            blocks.setdefault(int(self.mval[a]), []).extend(range(off, off + n))
+            off += n
+        return [np.array(blocks[m]) for m in sorted(blocks)]
+
+
 def invh(S):
     """utils.cpp:120-159 with chol=false: normalised symmetric orthonormalisation."""
     d = 1.0 / np.sqrt(np.diag(S))
@@ -314,4 +427,5 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
         except np.linalg.LinAlgError:
             Fd = F
         P, occ = density(Fd)
+    E += getattr(basis, "Enucr", 0.0)
     return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, iterations=it, P=P)

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <functional>
 #include <limits>
+#include <unordered_map>
 
 namespace orc {
 
@@ -84,8 +85,12 @@ struct RadialBasis {
   std::vector<double> bval;
   std::vector<int> first, last;  // FiniteElementBasis.cpp:51-64
   std::vector<double> xq, wq;    // Chebyshev rule, RadialBasis.cpp:179-188
+  bool dirichlet_left = true;     // drop the r=0 (mu=0) function
   RadialBasis() {}
-  RadialBasis(int nnodes, const std::vector<double> &bval, int nquad);
+  RadialBasis(int nnodes, const std::vector<double> &bval, int nquad, bool dirichlet_left = true);
+  // fixed-rule element matrix (FiniteElementBasis.cpp:460-491 on [-1,1])
+  Mat matel_rule(int iel, int lkind, int rkind, const std::vector<double> &x, const std::vector<double> &w,
+                 const std::function<double(double)> *f) const;
   int Nel() const { return (int)bval.size() - 1; }
   int Nbf() const { return last.back() + 1; }
   int Nprim(int iel) const { return last[iel] - first[iel] + 1; }
@@ -151,6 +156,20 @@ struct AtomicBasis {
 };
 // angular_basis, src/atomic/basis.cpp:179-203
 void angular_basis(int lmax, int mmax, std::vector<int> &lval, std::vector<int> &mval);
+
+// ---- Legendre functions (src/legendre/Legendre.h) --------------------------
+// out[m*(lmax+1) + l] = P_l^m(x) / Q_l^m(x), 0 <= m <= mmax
+void legendre_plm(double *out, int lmax, int mmax, double x);
+void legendre_qlm(double *out, int lmax, int mmax, double x);
+// One |M| column cached per argument (src/diatomic/quadrature.h:47-84)
+struct LegendreColumn {
+  int Mabs, Lmax;
+  std::unordered_map<double, std::vector<double>> Ptab, Qtab;
+  LegendreColumn(int Mabs, int Lmax);
+  const std::vector<double> &fetch(bool wantP, double x);
+  double P(int L, double x);
+  double Q(int L, double x);
+};
 
 // FE assemblers, libhelfemqc/include/CoulombExchangeFE.h:432-530
 Mat assemble_J_chol(const RadialBasis &radial, const std::vector<Mat> &rs, const std::vector<Mat> &rb,
