@@ -21,6 +21,7 @@ c_dp = ctypes.c_void_p
 SIGNATURES = {
     "hf_last_error": (ctypes.c_char_p, []),
     "hf_version": (ctypes.c_char_p, []),
+    "hf_element_grid": (c_int, [c_dbl, c_int, c_int, c_dbl, c_dp]),
     "hf_ctx_create": (c_int, [c_int, ctypes.c_uint, ctypes.POINTER(c_vp)]),
     "hf_ctx_destroy": (c_int, [c_vp]),
     "hf_ctx_set_stream": (c_int, [c_vp, c_vp]),
@@ -42,6 +43,21 @@ SIGNATURES = {
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_diatomic_create": (c_int, [c_vp, c_int, c_int, c_dbl, c_int, c_int, c_dp, c_int, c_vp, c_vp, c_int,
+                                   ctypes.POINTER(c_vp)]),
+    "hf_diatomic_destroy": (c_int, [c_vp]),
+    "hf_diatomic_sizes": (c_int, [c_vp, c_vp]),
+    "hf_diatomic_pure_index": (c_int, [c_vp, c_vp]),
+    "hf_diatomic_maps": (c_int, [c_vp, c_vp, c_vp]),
+    "hf_diatomic_set_absm_symmetric": (c_int, [c_vp, c_int]),
+    "hf_diatomic_compute_tei": (c_int, [c_vp, c_int]),
+    "hf_diatomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),
+    "hf_diatomic_one_electron": (c_int, [c_vp, c_int, c_dp, c_i64]),
+    "hf_diatomic_work": (c_int, [c_vp, c_vp]),
+    "hf_diatomic_coulomb": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_diatomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_diatomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_diatomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
 }
 
 _LIB = None

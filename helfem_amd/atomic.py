@@ -35,24 +35,10 @@ def angular_basis(lmax: int, mmax: int):
 
 
 def element_grid(rmax: float, nelem: int, igrid: int = 4, zexp: float = 2.0) -> np.ndarray:
-    """libhelfem/include/grid.h:37-101 (igrid 1-4), Eigen LinSpaced semantics."""
-    b = np.zeros(nelem + 1)
-    for i in range(nelem + 1):
-        def lin(hi):
-            return hi if i == nelem else i * (hi / nelem)
-        if igrid == 1:
-            b[i] = lin(rmax)
-        elif igrid == 2:
-            b[i] = i * i * rmax * (1.0 / (nelem * nelem))
-        elif igrid == 3:
-            b[i] = rmax * (i * (1.0 / nelem)) ** zexp
-        elif igrid == 4:
-            b[i] = np.exp(lin(np.log(rmax + 1.0) ** (1.0 / zexp)) ** zexp) - 1.0
-        else:
-            raise ValueError("igrid must be 1..4")
-    b[0] = 0.0
-    b[nelem] = rmax
-    return b
+    """Element boundaries on [0, rmax] (hf_element_grid; libhelfem/include/grid.h:37-101)."""
+    out = np.zeros(nelem + 1)
+    check(lib().hf_element_grid(float(rmax), int(nelem), int(igrid), float(zexp), out.ctypes.data))
+    return out
 
 
 class Context:

@@ -22,32 +22,12 @@
 //   sum_p M_p R M_p' inside an element), so the Nrad x Nrad Rmat of the
 //   reference never exists.
 #include "hfgpu_internal.hpp"
+#include "device_common.hpp"
 
 namespace hfg {
 namespace {
 
-constexpr int NT = 256;  // threads per workgroup (4 wave64)
-
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Reduce two values over the 256-thread block; result valid in thread 0.
-__device__ inline void block_sum2(double &x, double &y, double *red) {
-  x = wave_sum(x);
-  y = wave_sum(y);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[w] = x;
-    red[4 + w] = y;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    x = red[0] + red[1] + red[2] + red[3];
-    y = red[4] + red[5] + red[6] + red[7];
-  }
-}
+using namespace dev;
 
 __global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
   const int64_t n = rows * cols;
@@ -206,22 +186,10 @@ struct KArgs {
   const double *term_c;
 };
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// D(16x16) += X(16x16) * Y(16x16), one wave, v_mfma_f64_16x16x4f64.
-// Operand layout (lane l, g = l/16, c = l%16): the k index of step s is
-// 4g+s, so lane l supplies X(c, 4g+s) as xs[s] and Y'(c, 4g+s) i.e.
-// Y(4g+s, c) as ys[s]; the result lands as D(4g+r, c) in component r.
-__device__ inline v4d mfma_16x16x16(const double xs[4], const double ys[4], v4d d) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s) d = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[s], ys[s], d, 0, 0, 0);
-  return d;
-}
-
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
 // four waves split the multipoles L.  For each L a wave forms the
 // Gaunt-weighted density tile R_L(e,f) = sum_terms c * P_il(e,f) directly in
-// MFMA operand layout (lane (g,c) holds R(c, 4g+s)), then contracts it with
+// MFMA operand layout (lane (g,c) holds R(c, g+4s)), then contracts it with
 // the radial factors on the matrix cores:
 //   e != f:  K += A R B'       (A, B the r^L / r^-L-1 element matrices)
 //   e == f:  K += sum_p M_p R M_p'   (M_p the Cholesky vectors as 16x16)
@@ -241,7 +209,7 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
   const bool rowok = c < ni;
   bool colok[4];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) colok[s] = (4 * g + s) < nj;
+  for (int s = 0; s < 4; ++s) colok[s] = (g + 4 * s) < nj;
   for (int L = wv; L < A.N_L; L += 4) {
     const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
     if (q0 == q1) continue;
@@ -250,11 +218,11 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
       const int il = A.term_il[q];
       const double cq = A.term_c[q];
       const int i = il / A.Nang, l = il - i * A.Nang;
-      const double *Pb = A.P + (int64_t)(i * A.Nrad + fi + c) + (int64_t)(l * A.Nrad + fj + 4 * g) * A.ldp;
+      const double *Pb = A.P + (int64_t)(i * A.Nrad + fi + c) + (int64_t)(l * A.Nrad + fj + g) * A.ldp;
       if (rowok) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-          if (colok[s]) r[s] += cq * Pb[(int64_t)s * A.ldp];
+          if (colok[s]) r[s] += cq * Pb[(int64_t)(4 * s) * A.ldp];
       }
     }
     const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
@@ -264,8 +232,8 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
       double as[4], bs[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        as[s] = At[c + TILE * (4 * g + s)];
-        bs[s] = Bt[c + TILE * (4 * g + s)];
+        as[s] = At[c + TILE * (g + 4 * s)];
+        bs[s] = Bt[c + TILE * (g + 4 * s)];
       }
       v4d y = {0.0, 0.0, 0.0, 0.0};
       y = mfma_16x16x16(r, bs, y);           // Y = R B'
@@ -278,7 +246,7 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
         const double *Mp = M + (size_t)p * TILE2;
         double ms[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) ms[s] = Mp[c + TILE * (4 * g + s)];
+        for (int s = 0; s < 4; ++s) ms[s] = Mp[c + TILE * (g + 4 * s)];
         v4d y = {0.0, 0.0, 0.0, 0.0};
         y = mfma_16x16x16(r, ms, y);         // Y = R M_p'
         const double ys[4] = {y[0], y[1], y[2], y[3]};
@@ -286,9 +254,9 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
       }
     }
   }
-  // combine the four waves' partial tiles (lane holds K(4g+r, c))
+  // combine the four waves' partial tiles (lane holds K(g+4r, c))
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (4 * g + rr) + TILE * c] = acc[rr];
+  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
   __syncthreads();
   const int a = t & 15, b = t >> 4;
   if (a >= ni || b >= nj) return;

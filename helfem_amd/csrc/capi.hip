@@ -1,19 +1,11 @@
 // C-ABI layer (include/hfgpu.h): handles, table construction, error
 // translation.  Host code only; the kernels live in atomic_jk.hip.
-#include "hfgpu_internal.hpp"
-#include "../../include/hfgpu.h"
+#include "capi_common.hpp"
 #include <cstring>
 #include <cmath>
 #include <memory>
 
 using namespace hfg;
-
-struct hf_ctx {
-  int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t cur = nullptr;
-  KernelTimer timer;
-};
 
 struct hf_atomic {
   hf_ctx *ctx = nullptr;
@@ -22,46 +14,11 @@ struct hf_atomic {
   bool uploaded = false;
 };
 
-namespace {
+namespace hfg {
 thread_local std::string g_err;
-
-template <typename F>
-int guarded(F &&f) {
-  try {
-    f();
-    return HF_OK;
-  } catch (const Error &e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_err = "host allocation failed";
-    return HF_ENOMEM;
-  } catch (const std::logic_error &e) {
-    g_err = e.what();
-    return HF_EINVAL;
-  } catch (const std::exception &e) {
-    g_err = e.what();
-    return HF_EINTERNAL;
-  }
 }
 
-void need(bool cond, const char *msg) {
-  if (!cond) throw Error(HF_EINVAL, msg);
-}
-
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    hip_check(hipGetDevice(&prev), "hipGetDevice");
-    if (prev != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
-  }
-  ~DeviceGuard() {
-    int now = -1;
-    if (hipGetDevice(&now) == hipSuccess && now != prev && prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
-constexpr double kFourPi = 12.566370614359172953850573533118;
+namespace {
 
 // Build the Gaunt coupling tables and upload caches for the atomic kernels.
 void atomic_upload(hf_atomic *h) {
@@ -243,41 +200,29 @@ void atomic_upload(hf_atomic *h) {
   h->uploaded = true;
 }
 
-void check_matrix(const void *p, int64_t ld, int64_t n, const char *name) {
-  if (!p) throw Error(HF_EINVAL, std::string(name) + " is NULL");
-  if (ld < n) throw Error(HF_EINVAL, std::string("leading dimension of ") + name + " is smaller than Nbf");
-}
-
 void ensure_ready(hf_atomic *h) {
   if (!h->ctx) throw Error(HF_ESTATE, "handle was created without a device context");
   if (!h->host.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
   if (!h->uploaded) atomic_upload(h);
 }
 
-// host-pointer entry: stage through device buffers
-template <typename Launch>
-void host_roundtrip(hf_atomic *h, const double *P, int64_t ldp, double *O, int64_t ldo, Launch launch) {
-  const int64_t n = h->host.Nbf();
-  check_matrix(P, ldp, n, "P");
-  check_matrix(O, ldo, n, "output");
-  hipStream_t s = h->ctx->cur;
-  h->dev.hostP.alloc((size_t)n * n);
-  h->dev.hostOut.alloc((size_t)n * n);
-  hip_check(hipMemcpy2DAsync(h->dev.hostP.p, n * sizeof(double), P, ldp * sizeof(double), n * sizeof(double), n,
-                             hipMemcpyHostToDevice, s),
-            "copy P to device");
-  launch(h->dev.hostP.p, n, h->dev.hostOut.p, n);
-  hip_check(hipMemcpy2DAsync(O, ldo * sizeof(double), h->dev.hostOut.p, n * sizeof(double), n * sizeof(double), n,
-                             hipMemcpyDeviceToHost, s),
-            "copy result to host");
-  hip_check(hipStreamSynchronize(s), "stream sync");
-}
 }  // namespace
 
 extern "C" {
 
 const char *hf_last_error(void) { return g_err.c_str(); }
 const char *hf_version(void) { return "hfgpu 0.1 (gfx950)"; }
+
+int hf_element_grid(double rmax, int nel, int igrid, double zexp, double *out) {
+  return guarded([&] {
+    need(out != nullptr, "out is NULL");
+    need(nel >= 1, "need at least one element");
+    need(rmax > 0.0, "rmax must be positive");
+    need(igrid >= 1 && igrid <= 4, "igrid must be 1..4");
+    const std::vector<double> b = element_grid(rmax, nel, igrid, zexp);
+    std::copy(b.begin(), b.end(), out);
+  });
+}
 
 int hf_ctx_create(int device, unsigned flags, hf_ctx **out) {
   return guarded([&] {
@@ -530,7 +475,7 @@ int hf_atomic_coulomb(hf_atomic *h, const double *P, int64_t ldp, double *J, int
     need(h->ctx != nullptr, "handle was created without a device context");
     DeviceGuard g(h->ctx->device);
     ensure_ready(h);
-    host_roundtrip(h, P, ldp, J, ldj, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+    host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, J, ldj, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
       atomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
     });
   });
@@ -542,7 +487,7 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
     need(h->ctx != nullptr, "handle was created without a device context");
     DeviceGuard g(h->ctx->device);
     ensure_ready(h);
-    host_roundtrip(h, P, ldp, K, ldk, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+    host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, K, ldk, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
       atomic_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
     });
   });

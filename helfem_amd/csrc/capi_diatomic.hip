@@ -1,0 +1,539 @@
+// C-ABI for the diatomic basis (include/hfgpu.h): handle, device tables and
+// the coupling lists of the Coulomb and exchange kernels (diatomic_jk.hip).
+#include "capi_common.hpp"
+#include <cstring>
+#include <cmath>
+#include <memory>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace hfg;
+
+struct hf_diatomic {
+  hf_ctx *ctx = nullptr;
+  DiatomicSetup host;
+  DiatomicDevice dev;
+  bool absm = false;
+  bool uploaded = false;
+};
+
+namespace {
+
+// Per (j, i, L) Gaunt couplings with M = mj - mi: g0 = mod_coeff (cosh^0
+// channel), g2 = coeff (cosh^2 channel), basis.cpp:1650-1665, 1908-1916.
+void gaunt_tables(const DiatomicSetup &S, DiatomicDevice &d) {
+  const int Na = S.Nang();
+  int lmax = 0;
+  for (int l : S.lval) lmax = std::max(lmax, l);
+  d.gL = 2 * lmax + 3;
+  d.h_g0.assign((size_t)Na * Na * d.gL, 0.0);
+  d.h_g2.assign((size_t)Na * Na * d.gL, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int j = 0; j < Na; ++j)
+    for (int i = 0; i < Na; ++i) {
+      const int lj = S.lval[j], mj = S.mval[j], li = S.lval[i], mi = S.mval[i], M = mj - mi;
+      for (int L = std::max(std::abs(lj - li) - 2, std::abs(M)); L <= lj + li + 2; ++L) {
+        const size_t x = ((size_t)j * Na + i) * d.gL + L;
+        d.h_g0[x] = gaunt_mod(lj, mj, L, M, li, mi);
+        d.h_g2[x] = gaunt3(lj, mj, L, M, li);
+      }
+    }
+}
+
+void diatomic_upload(hf_diatomic *h) {
+  const DiatomicSetup &S = h->host;
+  DiatomicDevice &d = h->dev;
+  hipStream_t s = h->ctx->cur;
+  const int Nel = S.Nel(), Na = S.Nang(), nlm = (int)S.lm_map.size(), nLM = (int)S.LM_map.size();
+  d.Nel = Nel;
+  d.Nrad = S.Nrad();
+  d.Nang = Na;
+  d.n_lm = nlm;
+  d.n_LM = nLM;
+  d.Nbf = S.Nbf();
+  for (int e = 0; e < Nel; ++e)
+    if (S.rad.ni[e] > TILE) throw Error(HF_EINVAL, "elements with more than 16 functions are not supported");
+  d.first.upload(S.rad.first, s);
+  d.ni.upload(S.rad.ni, s);
+  std::vector<int> skip(Na);
+  for (int a = 0; a < Na; ++a) skip[a] = S.skip(a);
+  d.skip.upload(skip, s);
+  std::vector<long long> poff(S.poff.begin(), S.poff.end());
+  d.poff.upload(poff, s);
+
+  // P/Q tiles and Cholesky channel tiles
+  std::vector<double> pq((size_t)std::max(nlm, 1) * Nel * 4 * TILE2, 0.0), ch, sg;
+  std::vector<int> choff((size_t)std::max(nlm, 1) * Nel, 0), chrank((size_t)std::max(nlm, 1) * Nel, 0);
+  int maxr = 0;
+  for (int q = 0; q < nlm; ++q)
+    for (int e = 0; e < Nel; ++e) {
+      const size_t x = (size_t)q * Nel + e;
+      const int n = S.rad.ni[e];
+      const Blk *src[4] = {&S.P0[x], &S.P2[x], &S.Q0[x], &S.Q2[x]};
+      for (int kind = 0; kind < 4; ++kind) {
+        need(src[kind]->r == n && src[kind]->c == n, "P/Q cache has the wrong shape");
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a) pq[(x * 4 + kind) * TILE2 + a + TILE * b] = (*src[kind])(a, b);
+      }
+      const Blk &B = S.cdB[x];
+      const int nn = n * n;
+      need(B.r == 2 * nn, "Cholesky cache has the wrong number of rows");
+      need((int)S.cdS[x].size() == B.c, "Cholesky signs have the wrong length");
+      need(B.c <= 2 * TILE2, "Cholesky rank above 512 is not supported");
+      choff[x] = (int)sg.size();
+      chrank[x] = B.c;
+      maxr = std::max(maxr, B.c);
+      const size_t base = ch.size();
+      ch.resize(base + (size_t)B.c * 2 * TILE2, 0.0);
+      for (int p = 0; p < B.c; ++p) {
+        double *m0 = &ch[base + (size_t)p * 2 * TILE2], *m2 = m0 + TILE2;
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a) {
+            m0[a + TILE * b] = B(a + b * n, p);
+            m2[a + TILE * b] = B(nn + a + b * n, p);
+          }
+        sg.push_back(S.cdS[x][p]);
+      }
+    }
+  d.max_rank = maxr;
+  if (ch.empty()) ch.push_back(0.0);
+  if (sg.empty()) sg.push_back(0.0);
+  d.pq.upload(pq, s);
+  d.chol.upload(ch, s);
+  d.sigma.upload(sg, s);
+  d.chol_off.upload(choff, s);
+  d.chol_rank.upload(chrank, s);
+
+  // Coulomb tables (basis.cpp:1633-1660, 1790-1810)
+  gaunt_tables(S, d);
+  const auto &lv = S.lval;
+  const auto &mv = S.mval;
+  std::vector<int> LMilm(std::max(nLM, 1), 0);
+  std::vector<double> LMfac(std::max(nLM, 1), 0.0);
+  for (int q = 0; q < nLM; ++q) {
+    LMilm[q] = S.lmind(S.LM_map[q].first, S.LM_map[q].second);
+    LMfac[q] = S.LMfac(S.LM_map[q].first, S.LM_map[q].second);
+  }
+  std::vector<std::vector<std::pair<int, std::pair<double, double>>>> pairs(nLM);
+  for (int k = 0; k < Na; ++k)
+    for (int l = 0; l < Na; ++l) {
+      const int M = mv[k] - mv[l];
+      for (int L = std::max(std::abs(lv[k] - lv[l]) - 2, std::abs(M)); L <= lv[k] + lv[l] + 2; ++L) {
+        const size_t x = ((size_t)k * Na + l) * d.gL + L;
+        const double c0 = d.h_g0[x], c2 = d.h_g2[x];
+        if (c0 == 0.0 && c2 == 0.0) continue;
+        pairs[S.LMind(L, M)].push_back({k * Na + l, {c0, c2}});
+      }
+    }
+  std::vector<int> pstart(nLM + 1, 0), pkl;
+  std::vector<double> pc;
+  for (int q = 0; q < nLM; ++q) {
+    for (auto &pr : pairs[q]) {
+      pkl.push_back(pr.first);
+      pc.push_back(pr.second.first);
+      pc.push_back(pr.second.second);
+    }
+    pstart[q + 1] = (int)pkl.size();
+  }
+  std::vector<int> ostart((size_t)Na * Na + 1, 0), oq;
+  std::vector<double> oc;
+  for (int i = 0; i < Na; ++i)
+    for (int j = 0; j < Na; ++j) {
+      const int M = mv[j] - mv[i];
+      for (int L = std::max(std::abs(lv[j] - lv[i]) - 2, std::abs(M)); L <= lv[j] + lv[i] + 2; ++L) {
+        const size_t x = ((size_t)j * Na + i) * d.gL + L;
+        const double c0 = d.h_g0[x], c2 = d.h_g2[x];
+        if (c0 == 0.0 && c2 == 0.0) continue;
+        oq.push_back(S.LMind(L, M));
+        oc.push_back(c0);
+        oc.push_back(c2);
+      }
+      ostart[(size_t)i * Na + j + 1] = (int)oq.size();
+    }
+  d.n_pairs = (int64_t)pkl.size();
+  if (pkl.empty()) {
+    pkl.push_back(0);
+    pc.assign(2, 0.0);
+  }
+  if (oq.empty()) {
+    oq.push_back(0);
+    oc.assign(2, 0.0);
+  }
+  d.LM_ilm.upload(LMilm, s);
+  d.LM_fac.upload(LMfac, s);
+  d.pstart.upload(pstart, s);
+  d.pair_kl.upload(pkl, s);
+  d.pair_c.upload(pc, s);
+  d.out_start.upload(ostart, s);
+  d.out_q.upload(oq, s);
+  d.out_c.upload(oc, s);
+
+  // radial RI-K flops per coupled (output block, (L,|M|)): 12 r n^3 inside
+  // each element, 2 (4 ni nj^2 + 2 ni^2 nj) per element pair
+  d.h_rad_ilm.assign(std::max(nlm, 1), 0.0);
+  for (int q = 0; q < nlm; ++q) {
+    double f = 0.0;
+    for (int e = 0; e < Nel; ++e) {
+      const double n = S.rad.ni[e];
+      f += 12.0 * S.cdB[(size_t)q * Nel + e].c * n * n * n;
+      for (int g = 0; g < Nel; ++g)
+        if (g != e) {
+          const double m = S.rad.ni[g];
+          f += 2.0 * (4.0 * n * m * m + 2.0 * n * n * m);
+        }
+    }
+    d.h_rad_ilm[q] = f;
+  }
+  // absm mirror pairs: K(j,k) = K(jm,km) for mj, mk < 0 (basis.cpp:2067-2086)
+  std::vector<int> mir;
+  if (h->absm) {
+    std::vector<int> partner(Na, -1);
+    for (int a = 0; a < Na; ++a)
+      for (int b = 0; b < Na; ++b)
+        if (lv[b] == lv[a] && mv[b] == -mv[a]) {
+          partner[a] = b;
+          break;
+        }
+    for (int j = 0; j < Na; ++j) {
+      if (mv[j] >= 0 || partner[j] < 0) continue;
+      for (int k = 0; k < Na; ++k) {
+        if (mv[k] >= 0 || partner[k] < 0) continue;
+        mir.push_back(j * Na + k);
+        mir.push_back(partner[j] * Na + partner[k]);
+      }
+    }
+  }
+  d.absm = h->absm;
+  d.n_mirror = (int)mir.size() / 2;
+  if (mir.empty()) mir.assign(2, 0);
+  d.mirror.upload(mir, s);
+  d.screen_mask.clear();
+  d.n_act = 0;
+  d.h_bnorm.alloc((size_t)Na * Na);
+  d.bnorm.alloc((size_t)Na * Na);
+  d.Jin.alloc((size_t)std::max(nLM, 1) * Nel * 2 * TILE2);
+  d.ds.alloc((size_t)std::max(nLM, 1) * Nel);
+  d.db.alloc((size_t)std::max(nLM, 1) * Nel);
+  d.preA.alloc((size_t)std::max(nLM, 1) * Nel);
+  d.preB.alloc((size_t)std::max(nLM, 1) * Nel);
+  hip_check(hipStreamSynchronize(s), "upload sync");
+  h->uploaded = true;
+}
+
+// Exchange coupling lists for the density blocks that pass the norm screen
+// (|P_il|_F >= 10 eps, basis.cpp:1858-1875).  For each output block (j,k):
+// runs of terms per (L,|M|), each term a density block (i,l) with
+// mi - ml = mj - mk and the four channel weights LMfac * {mji mkl, -mji gkl,
+// -gji mkl, gji gkl} (basis.cpp:1905-1960).  Rebuilt only when the screening
+// pattern changes.
+void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
+  const DiatomicSetup &S = h->host;
+  DiatomicDevice &d = h->dev;
+  const int Na = S.Nang(), nlm = d.n_lm;
+  const auto &lv = S.lval;
+  const auto &mv = S.mval;
+  std::vector<unsigned char> mask((size_t)Na * Na);
+  const double thr = 10.0 * 2.220446049250313e-16;
+  for (size_t q = 0; q < mask.size(); ++q) mask[q] = d.h_bnorm.p[q] >= thr ? 1 : 0;
+  if (!d.screen_mask.empty() && mask == d.screen_mask) return;
+  int mabs = 0;
+  for (int m : mv) mabs = std::max(mabs, std::abs(m));
+  const int dmoff = 2 * mabs;
+  std::vector<std::vector<int>> bydm(4 * mabs + 1);
+  for (int i = 0; i < Na; ++i)
+    for (int l = 0; l < Na; ++l)
+      if (mask[(size_t)i * Na + l]) bydm[mv[i] - mv[l] + dmoff].push_back(i * Na + l);
+  // (L, M) -> ilm and LMfac lookups
+  const int Lext = d.gL, Mext = 2 * dmoff + 1;
+  std::vector<int> ilm_of((size_t)Lext * Mext, -1);
+  std::vector<double> fac_of((size_t)Lext * Mext, 0.0);
+  for (auto &e : S.LM_map)
+    if (e.first < Lext && std::abs(e.second) <= dmoff) {
+      ilm_of[(size_t)e.first * Mext + e.second + dmoff] = S.lmind(e.first, e.second);
+      fac_of[(size_t)e.first * Mext + e.second + dmoff] = S.LMfac(e.first, e.second);
+    }
+  struct Term {
+    int ilm, il;
+    double f[4];
+  };
+  const int njk = Na * Na;
+  std::vector<std::vector<Term>> per(njk);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int jk = 0; jk < njk; ++jk) {
+    const int j = jk / Na, k = jk - j * Na;
+    const int lj = lv[j], mj = mv[j], lk = lv[k], mk = mv[k];
+    if (h->absm && (mj < 0 || mk < 0)) continue;
+    const int dm = mj - mk;
+    if (std::abs(dm) > 2 * mabs) continue;
+    std::vector<Term> &out = per[jk];
+    for (int il : bydm[dm + dmoff]) {
+      const int i = il / Na, l = il - i * Na;
+      const int li = lv[i], mi = mv[i], ll = lv[l], ml = mv[l], M = mj - mi;
+      const int Lmin = std::max(std::max(std::abs(li - lj), std::abs(lk - ll)) - 2, std::abs(M));
+      const int Lmax = std::min(li + lj, lk + ll) + 2;
+      const size_t gji = ((size_t)j * Na + i) * Lext, gkl = ((size_t)k * Na + l) * Lext;
+      for (int L = Lmin; L <= Lmax; ++L) {
+        const double mji = d.h_g0[gji + L], g_ji = d.h_g2[gji + L];
+        const double mkl = d.h_g0[gkl + L], g_kl = d.h_g2[gkl + L];
+        const double c00 = mji * mkl, c02 = -mji * g_kl, c20 = -g_ji * mkl, c22 = g_ji * g_kl;
+        if (c00 == 0.0 && c02 == 0.0 && c20 == 0.0 && c22 == 0.0) continue;
+        const size_t lm = (size_t)L * Mext + M + dmoff;
+        const double fac = fac_of[lm];
+        out.push_back({ilm_of[lm], il, {fac * c00, fac * c02, fac * c20, fac * c22}});
+      }
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Term &a, const Term &b) { return a.ilm < b.ilm; });
+  }
+  std::vector<int> act, lstart(1, 0), rilm, rstart(1, 0), til;
+  std::vector<double> tf;
+  double frad = 0.0;
+  int64_t nterm = 0;
+  for (int jk = 0; jk < njk; ++jk) {
+    const std::vector<Term> &v = per[jk];
+    if (v.empty()) continue;
+    act.push_back(jk);
+    for (size_t a = 0; a < v.size();) {
+      size_t b = a;
+      while (b < v.size() && v[b].ilm == v[a].ilm) {
+        til.push_back(v[b].il);
+        tf.insert(tf.end(), v[b].f, v[b].f + 4);
+        ++b;
+      }
+      rilm.push_back(v[a].ilm);
+      rstart.push_back((int)til.size());
+      frad += d.h_rad_ilm[v[a].ilm];
+      a = b;
+    }
+    lstart.push_back((int)rilm.size());
+    nterm += (int64_t)v.size();
+  }
+  d.n_act = (int)act.size();
+  d.n_runs = (int64_t)rilm.size();
+  d.n_terms = nterm;
+  d.flops_k_radial = frad;
+  d.flops_k_angular = 8.0 * (double)nterm * (double)d.Nrad * (double)d.Nrad;
+  if (act.empty()) act.push_back(0);
+  if (rilm.empty()) rilm.push_back(0);
+  if (til.empty()) {
+    til.push_back(0);
+    tf.assign(4, 0.0);
+  }
+  (void)nlm;
+  d.act_jk.upload(act, s);
+  d.act_lstart.upload(lstart, s);
+  d.lrun_ilm.upload(rilm, s);
+  d.lrun_tstart.upload(rstart, s);
+  d.term_il.upload(til, s);
+  d.term_f.upload(tf, s);
+  hip_check(hipStreamSynchronize(s), "exchange list upload");
+  d.screen_mask = mask;
+}
+
+void ensure_ready(hf_diatomic *h) {
+  if (!h->ctx) throw Error(HF_ESTATE, "handle was created without a device context");
+  if (!h->host.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+  if (!h->uploaded) diatomic_upload(h);
+}
+
+void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk) {
+  DiatomicDevice &d = h->dev;
+  hipStream_t s = h->ctx->cur;
+  diatomic_block_norms_launch(d, P, ldp, s);
+  hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
+            "copy block norms");
+  hip_check(hipStreamSynchronize(s), "block norms sync");
+  build_exchange_lists(h, s);
+  diatomic_exchange_kernels(d, P, ldp, K, ldk, s, &h->ctx->timer);
+}
+
+hf_diatomic *checked(hf_diatomic *h) {
+  need(h != nullptr, "NULL handle");
+  need(h->ctx != nullptr, "handle was created without a device context");
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hf_diatomic_create(hf_ctx *ctx, int Z1, int Z2, double Rhalf, int nnodes, int nquad, const double *bval,
+                       int nbval, const int *lval, const int *mval, int nang, hf_diatomic **out) {
+  return guarded([&] {
+    need(bval && lval && mval && out, "NULL argument");
+    need(nbval >= 2, "need at least one element (nbval >= 2)");
+    need(nang >= 1, "need at least one angular shell");
+    need(nnodes >= 2 && nnodes <= TILE, "nnodes must be in [2, 16]");
+    need(Z1 >= 0 && Z2 >= 0, "nuclear charges must be non-negative");
+    need(Rhalf > 0.0, "Rhalf must be positive");
+    for (int i = 0; i + 1 < nbval; ++i) need(bval[i + 1] > bval[i], "element boundaries must increase");
+    need(bval[0] == 0.0, "the first element boundary must be mu=0");
+    std::unique_ptr<hf_diatomic> h(new hf_diatomic);
+    h->ctx = ctx;
+    h->host.build(Z1, Z2, Rhalf, nnodes, nquad > 0 ? nquad : 5 * nnodes, std::vector<double>(bval, bval + nbval),
+                  std::vector<int>(lval, lval + nang), std::vector<int>(mval, mval + nang));
+    *out = h.release();
+  });
+}
+
+int hf_diatomic_destroy(hf_diatomic *h) {
+  return guarded([&] {
+    if (!h) return;
+    if (h->ctx) {
+      DeviceGuard g(h->ctx->device);
+      (void)hipStreamSynchronize(h->ctx->cur);
+    }
+    delete h;
+  });
+}
+
+int hf_diatomic_sizes(const hf_diatomic *h, int64_t out[7]) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    out[0] = h->host.Nbf();
+    out[1] = h->host.Ndummy();
+    out[2] = h->host.Nrad();
+    out[3] = h->host.Nang();
+    out[4] = h->host.Nel();
+    out[5] = (int64_t)h->host.lm_map.size();
+    out[6] = (int64_t)h->host.LM_map.size();
+  });
+}
+
+int hf_diatomic_pure_index(const hf_diatomic *h, int64_t *idx) {
+  return guarded([&] {
+    need(h && idx, "NULL argument");
+    const DiatomicSetup &S = h->host;
+    int64_t q = 0;
+    for (int a = 0; a < S.Nang(); ++a)
+      for (int r = S.skip(a); r < S.Nrad(); ++r) idx[q++] = (int64_t)a * S.Nrad() + r;
+  });
+}
+
+int hf_diatomic_maps(const hf_diatomic *h, int *lm, int *LM) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    const DiatomicSetup &S = h->host;
+    if (lm)
+      for (size_t i = 0; i < S.lm_map.size(); ++i) {
+        lm[2 * i] = S.lm_map[i].first;
+        lm[2 * i + 1] = S.lm_map[i].second;
+      }
+    if (LM)
+      for (size_t i = 0; i < S.LM_map.size(); ++i) {
+        LM[2 * i] = S.LM_map[i].first;
+        LM[2 * i + 1] = S.LM_map[i].second;
+      }
+  });
+}
+
+int hf_diatomic_set_absm_symmetric(hf_diatomic *h, int sym) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    if (h->absm != (sym != 0)) {
+      h->absm = sym != 0;
+      h->uploaded = false;
+    }
+  });
+}
+
+int hf_diatomic_compute_tei(hf_diatomic *h, int exchange) {
+  (void)exchange;
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    h->host.compute_tei();
+    h->uploaded = false;
+  });
+}
+
+int hf_diatomic_get_cache(const hf_diatomic *h, int kind, int ilm, int iel, int dims[2], double *data) {
+  return guarded([&] {
+    need(h && dims, "NULL argument");
+    const DiatomicSetup &S = h->host;
+    need(kind >= 0 && kind <= 5, "kind must be in 0..5");
+    need(ilm >= 0 && ilm < (int)S.lm_map.size() && iel >= 0 && iel < S.Nel(), "(ilm, iel) out of range");
+    if (!S.have_tei) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+    const size_t x = (size_t)ilm * S.Nel() + iel;
+    if (kind == 5) {
+      dims[0] = (int)S.cdS[x].size();
+      dims[1] = 1;
+      if (data && dims[0]) std::memcpy(data, S.cdS[x].data(), sizeof(double) * S.cdS[x].size());
+      return;
+    }
+    const Blk &b = kind == 0 ? S.P0[x] : kind == 1 ? S.P2[x] : kind == 2 ? S.Q0[x] : kind == 3 ? S.Q2[x] : S.cdB[x];
+    dims[0] = b.r;
+    dims[1] = b.c;
+    if (data && !b.v.empty()) std::memcpy(data, b.v.data(), sizeof(double) * b.v.size());
+  });
+}
+
+int hf_diatomic_one_electron(const hf_diatomic *h, int which, double *out, int64_t ld) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    check_matrix(out, ld, h->host.Nbf(), "out");
+    if (which == 0)
+      h->host.overlap(out, ld);
+    else if (which == 1)
+      h->host.kinetic(out, ld);
+    else if (which == 2)
+      h->host.nuclear(out, ld);
+    else
+      throw Error(HF_EINVAL, "which must be 0 (overlap), 1 (kinetic) or 2 (nuclear)");
+  });
+}
+
+int hf_diatomic_work(hf_diatomic *h, double out[4]) {
+  return guarded([&] {
+    checked(h);
+    out[0] = h->dev.flops_k_angular;
+    out[1] = h->dev.flops_k_radial;
+    out[2] = (double)h->dev.n_terms;
+    out[3] = (double)h->dev.n_act;
+  });
+}
+
+int hf_diatomic_coulomb_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj) {
+  return guarded([&] {
+    checked(h);
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dJ, ldj, h->host.Nbf(), "J");
+    diatomic_coulomb_launch(h->dev, dP, ldp, dJ, ldj, h->ctx->cur, &h->ctx->timer);
+  });
+}
+
+int hf_diatomic_exchange_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk) {
+  return guarded([&] {
+    checked(h);
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dK, ldk, h->host.Nbf(), "K");
+    exchange_launch(h, dP, ldp, dK, ldk);
+  });
+}
+
+int hf_diatomic_coulomb(hf_diatomic *h, const double *P, int64_t ldp, double *J, int64_t ldj) {
+  return guarded([&] {
+    checked(h);
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, J, ldj,
+                   [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+                     diatomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
+                   });
+  });
+}
+
+int hf_diatomic_exchange(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk) {
+  return guarded([&] {
+    checked(h);
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, K, ldk,
+                   [&](const double *dp, int64_t lp, double *dout, int64_t lo) { exchange_launch(h, dp, lp, dout, lo); });
+  });
+}
+
+}  // extern "C"

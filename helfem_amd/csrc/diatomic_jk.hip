@@ -1,0 +1,395 @@
+// Diatomic (prolate spheroidal) Coulomb and exchange kernels (FP64), gfx950.
+//
+// Data layout (device):
+//   P, J, K   dense Nbf x Nbf column-major in the pure basis; shell a's
+//             radial function r is row poff[a] + r - skip[a] (r >= skip[a]).
+//   pq        per ((L,|M|), element) four 16x16 zero-padded tiles: the
+//             disjoint integrals P0, P2, Q0, Q2 (basis.cpp:352-394).
+//   chol      per ((L,|M|), element) and Cholesky vector p the two channel
+//             blocks M0_p(a,b) = B(a + b Ni, p), M2_p(a,b) = B(Ni^2 + a + b Ni, p)
+//             of W = B diag(sigma) B' (basis.cpp:1492-1537), and sigma.
+//
+// Coulomb (basis.cpp:1627-1816): two density channels per (L,M),
+// Paux0 = sum mod_coeff * P_kl and Paux2 = sum coeff * P_kl, of which only
+// the element-diagonal bands matter.  Kernel 1 gathers both bands, reduces
+// the P/Q traces and applies the in-element kernel through the Cholesky
+// factor; kernel 2 prefix-sums the cross-element traces; kernel 3 expands
+// the bands into the shell blocks of J.
+// Exchange (basis.cpp:1818-2089): one workgroup per (output block (j,k),
+// element pair (e,f)); for each coupled (L,|M|) the four Gaunt-weighted
+// density tiles R00, R02, R20, R22 are formed in registers and contracted on
+// the matrix cores with the disjoint P/Q factors (e != f) or the four-channel
+// Cholesky factor (e == f).
+#include "hfgpu_internal.hpp"
+#include "device_common.hpp"
+
+namespace hfg {
+namespace {
+
+using namespace dev;
+
+__device__ inline long long pure_row(const int *skip, const long long *poff, int shell, int r) {
+  const int s = skip[shell];
+  return r >= s ? poff[shell] + (r - s) : -1;
+}
+
+// --- Coulomb -----------------------------------------------------------------
+
+struct DJArgs {
+  const double *P;
+  int64_t ldp;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const double *pq, *chol, *sigma;
+  const int *chol_off, *chol_rank;
+  const int *LM_ilm;
+  const double *LM_fac;
+  const int *pstart, *pair_kl;
+  const double *pair_c;
+  double *Jin, *ds, *db;
+};
+
+// One workgroup per ((L,M), element).
+__global__ void __launch_bounds__(NT) k_dia_j_gather(DJArgs A) {
+  const int q = blockIdx.x / A.Nel, e = blockIdx.x - q * A.Nel;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int ni = A.ni[e], f0 = A.first[e], ilm = A.LM_ilm[q];
+  const double fac = A.LM_fac[q];
+  __shared__ double s0[TILE2], s2[TILE2];
+  __shared__ double sc[2 * TILE2];
+  __shared__ double red[8];
+  double v0 = 0.0, v2 = 0.0;
+  if (a < ni && b < ni) {
+    for (int p = A.pstart[q]; p < A.pstart[q + 1]; ++p) {
+      const int kl = A.pair_kl[p], k = kl / A.Nang, l = kl - k * A.Nang;
+      const long long r = pure_row(A.skip, A.poff, k, f0 + a), c = pure_row(A.skip, A.poff, l, f0 + b);
+      if (r < 0 || c < 0) continue;
+      const double x = A.P[r + c * A.ldp];
+      v0 += A.pair_c[2 * p] * x;
+      v2 += A.pair_c[2 * p + 1] * x;
+    }
+  }
+  s0[t] = v0;
+  s2[t] = v2;
+  __syncthreads();
+  // traces tr(X s) = sum_ab X(a,b) s(b,a)
+  const size_t tb = ((size_t)ilm * A.Nel + e) * 4 * TILE2;
+  const double x0 = s0[b + TILE * a], x2 = s2[b + TILE * a];
+  double dsv = A.pq[tb + t] * x0 - A.pq[tb + TILE2 + t] * x2;
+  double dbv = A.pq[tb + 2 * TILE2 + t] * x0 - A.pq[tb + 3 * TILE2 + t] * x2;
+  block_sum2(dsv, dbv, red);
+  if (t == 0) {
+    A.ds[(size_t)q * A.Nel + e] = fac * dsv;
+    A.db[(size_t)q * A.Nel + e] = fac * dbv;
+  }
+  // c_p = sigma_p <[M0_p; M2_p], [s0; s2]>, wave w handles p = w, w+4, ...
+  const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
+  const double *M = A.chol + (size_t)off * 2 * TILE2;
+  const int lane = t & 63, w = t >> 6;
+  for (int p = w; p < rank; p += 4) {
+    const double *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += M0[lane + 64 * u] * s0[lane + 64 * u] + M2[lane + 64 * u] * s2[lane + 64 * u];
+    s = wave_sum(s);
+    if (lane == 0) sc[p] = s * A.sigma[off + p];
+  }
+  __syncthreads();
+  double j0 = 0.0, j2 = 0.0;
+  for (int p = 0; p < rank; ++p) {
+    const double cp = sc[p];
+    j0 += M[(size_t)p * 2 * TILE2 + t] * cp;
+    j2 += M[(size_t)p * 2 * TILE2 + TILE2 + t] * cp;
+  }
+  double *out = A.Jin + ((size_t)q * A.Nel + e) * 2 * TILE2;
+  out[t] = j0 * fac;
+  out[TILE2 + t] = j2 * fac;
+}
+
+// preA(q,e) = sum_{j>e} db(q,j), preB(q,e) = sum_{j<e} ds(q,j)
+__global__ void k_dia_j_prefix(int n_q, int Nel, const double *ds, const double *db, double *preA, double *preB) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_q) return;
+  double acc = 0.0;
+  for (int e = 0; e < Nel; ++e) {
+    preB[(size_t)q * Nel + e] = acc;
+    acc += ds[(size_t)q * Nel + e];
+  }
+  acc = 0.0;
+  for (int e = Nel - 1; e >= 0; --e) {
+    preA[(size_t)q * Nel + e] = acc;
+    acc += db[(size_t)q * Nel + e];
+  }
+}
+
+struct DJOutArgs {
+  double *J;
+  int64_t ldj;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const double *pq, *Jin, *preA, *preB;
+  const int *LM_ilm, *out_start, *out_q;
+  const double *out_c;
+};
+
+// Ja0 = Jin0 + preA P0 + preB Q0, Ja2 = Jin2 - preA P2 - preB Q2 at entry t
+__device__ inline void dia_band(const DJOutArgs &A, int q, int e, int t, double &x0, double &x2) {
+  const size_t tb = ((size_t)A.LM_ilm[q] * A.Nel + e) * 4 * TILE2;
+  const size_t qe = (size_t)q * A.Nel + e;
+  const double pa = A.preA[qe], pb = A.preB[qe];
+  const double *jin = A.Jin + qe * 2 * TILE2;
+  x0 = jin[t] + pa * A.pq[tb + t] + pb * A.pq[tb + 2 * TILE2 + t];
+  x2 = jin[TILE2 + t] - pa * A.pq[tb + TILE2 + t] - pb * A.pq[tb + 3 * TILE2 + t];
+}
+
+// One workgroup per (output shell block (i,j), element).
+__global__ void __launch_bounds__(NT) k_dia_j_expand(DJOutArgs A) {
+  const int ij = blockIdx.x / A.Nel, e = blockIdx.x - ij * A.Nel;
+  const int o0 = A.out_start[ij], o1 = A.out_start[ij + 1];
+  if (o0 == o1) return;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int ni = A.ni[e], f0 = A.first[e];
+  if (a >= ni || b >= ni) return;
+  if (e > 0 && a == 0 && b == 0) return;  // shared corner, written by element e-1
+  const int i = ij / A.Nang, j = ij - i * A.Nang;
+  const long long row = pure_row(A.skip, A.poff, i, f0 + a), col = pure_row(A.skip, A.poff, j, f0 + b);
+  if (row < 0 || col < 0) return;
+  const bool corner = (a == ni - 1 && b == ni - 1 && e + 1 < A.Nel);
+  double v = 0.0;
+  for (int o = o0; o < o1; ++o) {
+    const int q = A.out_q[o];
+    double x0, x2;
+    dia_band(A, q, e, t, x0, x2);
+    if (corner) {
+      double y0, y2;
+      dia_band(A, q, e + 1, 0, y0, y2);
+      x0 += y0;
+      x2 += y2;
+    }
+    v += A.out_c[2 * o] * x0 + A.out_c[2 * o + 1] * x2;
+  }
+  A.J[row + col * A.ldj] = v;
+}
+
+// --- Exchange ----------------------------------------------------------------
+
+__global__ void __launch_bounds__(NT) k_dia_block_norms(const double *P, int64_t ldp, int Nrad, int Nang,
+                                                         const int *skip, const long long *poff, double *bnorm) {
+  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
+  __shared__ double red[8];
+  const int ri = Nrad - skip[i], rl = Nrad - skip[l];
+  const long long r0 = poff[i], c0 = poff[l];
+  double s = 0.0, dummy = 0.0;
+  const int64_t n = (int64_t)ri * rl;
+  for (int64_t q = threadIdx.x; q < n; q += NT) {
+    const int64_t c = q / ri, r = q - c * ri;
+    const double v = P[(r0 + r) + (c0 + c) * ldp];
+    s += v * v;
+  }
+  block_sum2(s, dummy, red);
+  if (threadIdx.x == 0) bnorm[il] = sqrt(s);
+}
+
+struct DKArgs {
+  const double *P;
+  int64_t ldp;
+  double *K;
+  int64_t ldk;
+  int Nel, Nang, n_act;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const double *pq, *chol, *sigma;
+  const int *chol_off, *chol_rank;
+  const int *act_jk, *act_lstart, *lrun_ilm, *lrun_tstart, *term_il;
+  const double *term_f;
+};
+
+// One workgroup per (active output block (j,k), element pair (e,f)), the
+// four waves splitting the block's coupled (L,|M|) runs.  Workgroups sharing
+// (e,f) are mapped to one XCD so the (e,f) slice of P stays in its L2.
+__global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
+  const int w = blockIdx.x;
+  const int xcd = w & 7, loc = w >> 3;
+  const int ef = (loc / A.n_act) * 8 + xcd;
+  const int act = loc - (loc / A.n_act) * A.n_act;
+  if (ef >= A.Nel * A.Nel) return;
+  const int e = ef / A.Nel, f = ef - e * A.Nel;
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
+  const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
+  __shared__ double red[4 * TILE2];
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  const bool rowok = c < ni;
+  bool colok[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) colok[s] = (g + 4 * s) < nj;
+  const int r0 = A.act_lstart[act], r1 = A.act_lstart[act + 1];
+  for (int run = r0 + wv; run < r1; run += 4) {
+    const int ilm = A.lrun_ilm[run];
+    double r00[4] = {0, 0, 0, 0}, r02[4] = {0, 0, 0, 0}, r20[4] = {0, 0, 0, 0}, r22[4] = {0, 0, 0, 0};
+    for (int q = A.lrun_tstart[run]; q < A.lrun_tstart[run + 1]; ++q) {
+      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+      const double f00 = A.term_f[4 * q], f02 = A.term_f[4 * q + 1], f20 = A.term_f[4 * q + 2],
+                   f22 = A.term_f[4 * q + 3];
+      const long long row = rowok ? pure_row(A.skip, A.poff, i, fi + c) : -1;
+      if (row < 0) continue;
+      const int sl = A.skip[l];
+      const long long cb = A.poff[l] - sl;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int cr = fj + g + 4 * s;
+        if (colok[s] && cr >= sl) {
+          const double x = A.P[row + (cb + cr) * A.ldp];
+          r00[s] += f00 * x;
+          r02[s] += f02 * x;
+          r20[s] += f20 * x;
+          r22[s] += f22 * x;
+        }
+      }
+    }
+    const double *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
+    if (e != f) {
+      const double *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
+      const bool up = e > f;
+      // row side: Q of e if e > f else P of e; column side the other kind
+      const double *i0 = te + (up ? 2 : 0) * TILE2, *i2 = te + (up ? 3 : 1) * TILE2;
+      const double *j0 = tf + (up ? 0 : 2) * TILE2, *j2 = tf + (up ? 1 : 3) * TILE2;
+      double a0[4], a2[4], b0[4], b2[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int kk = c + TILE * (g + 4 * s);
+        a0[s] = i0[kk];
+        a2[s] = i2[kk];
+        b0[s] = j0[kk];
+        b2[s] = j2[kk];
+      }
+      v4d y = {0.0, 0.0, 0.0, 0.0};
+      y = mfma_16x16x16(r00, b0, y);  // R00 j0'
+      y = mfma_16x16x16(r02, b2, y);  // + R02 j2'
+      acc = mfma_16x16x16(a0, y, acc);
+      y = v4d{0.0, 0.0, 0.0, 0.0};
+      y = mfma_16x16x16(r20, b0, y);  // R20 j0'
+      y = mfma_16x16x16(r22, b2, y);  // + R22 j2'
+      acc = mfma_16x16x16(a2, y, acc);
+    } else {
+      const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
+      const double *M = A.chol + (size_t)off * 2 * TILE2;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        r02[s] = -r02[s];
+        r20[s] = -r20[s];
+      }
+      for (int p = 0; p < rank; ++p) {
+        const double *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
+        const double sg = A.sigma[off + p];
+        double y0[4], y2[4], x0[4], x2[4];  // M' as Y operand and as X operand
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int kk = g + 4 * s;
+          y0[s] = M0[c + TILE * kk];
+          y2[s] = M2[c + TILE * kk];
+          x0[s] = M0[kk + TILE * c];
+          x2[s] = M2[kk + TILE * c];
+        }
+        v4d y = {0.0, 0.0, 0.0, 0.0};
+        y = mfma_16x16x16(r00, y0, y);  // R00 M0'
+        y = mfma_16x16x16(r02, y2, y);  // - R02 M2'
+        y *= sg;
+        acc = mfma_16x16x16(x0, y, acc);
+        y = v4d{0.0, 0.0, 0.0, 0.0};
+        y = mfma_16x16x16(r20, y0, y);  // - R20 M0'
+        y = mfma_16x16x16(r22, y2, y);  // + R22 M2'
+        y *= sg;
+        acc = mfma_16x16x16(x2, y, acc);
+      }
+    }
+  }
+  // combine the four waves' partial tiles (lane holds acc(g+4r, c))
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
+  __syncthreads();
+  const int a = t & 15, b = t >> 4;
+  if (a >= ni || b >= nj) return;
+  const long long row = pure_row(A.skip, A.poff, j, fi + a), col = pure_row(A.skip, A.poff, k, fj + b);
+  if (row < 0 || col < 0) return;
+  const double v = red[t] + red[TILE2 + t] + red[2 * TILE2 + t] + red[3 * TILE2 + t];
+  const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
+  const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+  double *dst = A.K + row + col * A.ldk;
+  if (srow || scol)
+    atomicAdd(dst, -v);
+  else
+    *dst = -v;
+}
+
+// K(-mj,-mk) = K(mj,mk) for a +-m symmetric density (basis.cpp:2067-2086):
+// one workgroup per mirrored shell block; m != 0 on both sides, so both
+// blocks hold (Nrad-1) x (Nrad-1) pure entries.
+__global__ void __launch_bounds__(NT) k_dia_mirror(double *K, int64_t ldk, int Nrad, const long long *poff,
+                                                    const int *mirror, int Nang) {
+  const int dst = mirror[2 * blockIdx.x], src = mirror[2 * blockIdx.x + 1];
+  const int jd = dst / Nang, kd = dst - jd * Nang, js = src / Nang, ks = src - js * Nang;
+  const int n = Nrad - 1;
+  for (int q = threadIdx.x; q < n * n; q += NT) {
+    const int cc = q / n, rr = q - cc * n;
+    K[(poff[jd] + rr) + (poff[kd] + cc) * ldk] = K[(poff[js] + rr) + (poff[ks] + cc) * ldk];
+  }
+}
+
+}  // namespace
+
+void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
+                             hipStream_t s, KernelTimer *timer) {
+  fill_zero_launch(J, d.Nbf, d.Nbf, ldj, s);
+  if (d.n_LM == 0) return;
+  DJArgs a{P, ldp, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.chol.p, d.sigma.p,
+           d.chol_off.p, d.chol_rank.p, d.LM_ilm.p, d.LM_fac.p, d.pstart.p, d.pair_kl.p, d.pair_c.p,
+           d.Jin.p, d.ds.p, d.db.p};
+  {
+    TimedLaunch tl(timer, "diatomic_j_gather", s);
+    hipLaunchKernelGGL(k_dia_j_gather, dim3(d.n_LM * d.Nel), dim3(NT), 0, s, a);
+    hip_check(hipGetLastError(), "diatomic_j_gather");
+  }
+  hipLaunchKernelGGL(k_dia_j_prefix, dim3((d.n_LM + 63) / 64), dim3(64), 0, s, d.n_LM, d.Nel, d.ds.p, d.db.p,
+                     d.preA.p, d.preB.p);
+  hip_check(hipGetLastError(), "diatomic_j_prefix");
+  DJOutArgs o{J, ldj, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.Jin.p, d.preA.p,
+              d.preB.p, d.LM_ilm.p, d.out_start.p, d.out_q.p, d.out_c.p};
+  {
+    TimedLaunch tl(timer, "diatomic_j_expand", s);
+    hipLaunchKernelGGL(k_dia_j_expand, dim3(d.Nang * d.Nang * d.Nel), dim3(NT), 0, s, o);
+    hip_check(hipGetLastError(), "diatomic_j_expand");
+  }
+}
+
+void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s) {
+  hipLaunchKernelGGL(k_dia_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.skip.p,
+                     d.poff.p, d.bnorm.p);
+  hip_check(hipGetLastError(), "diatomic_block_norms");
+}
+
+void diatomic_exchange_kernels(const DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer) {
+  fill_zero_launch(K, d.Nbf, d.Nbf, ldk, s);
+  if (d.n_act == 0) return;
+  const int n_ef_pad = ((d.Nel * d.Nel + 7) / 8) * 8;
+  DKArgs a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.chol.p,
+           d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p, d.lrun_tstart.p,
+           d.term_il.p, d.term_f.p};
+  const int64_t grid = (int64_t)n_ef_pad * d.n_act;
+  if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+  {
+    TimedLaunch tl(timer, "diatomic_k", s, d.flops_k_radial + d.flops_k_angular);
+    hipLaunchKernelGGL(k_dia_k, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    hip_check(hipGetLastError(), "diatomic_k");
+  }
+  if (d.n_mirror > 0) {
+    hipLaunchKernelGGL(k_dia_mirror, dim3(d.n_mirror), dim3(NT), 0, s, K, ldk, d.Nrad, d.poff.p, d.mirror.p,
+                       d.Nang);
+    hip_check(hipGetLastError(), "diatomic_mirror");
+  }
+}
+
+}  // namespace hfg

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include "host/fem.hpp"
+#include "host/diatomic.hpp"
 
 namespace hfg {
 
@@ -162,11 +163,57 @@ struct AtomicDevice {
   double flops_k_radial = 0.0, flops_k_angular = 0.0, flops_j = 0.0;
 };
 
+// Everything the diatomic J/K kernels read.  Matrices are in the pure basis
+// (Nbf x Nbf); shell a's radial function r sits at poff[a] + r - skip[a] when
+// r >= skip[a] (m != 0 shells drop r = 0, basis.cpp:717-737).
+struct DiatomicDevice {
+  int Nel = 0, Nrad = 0, Nang = 0, n_lm = 0, n_LM = 0, n_act = 0, max_rank = 0;
+  int64_t Nbf = 0;
+  bool absm = false;
+  DevArray<int> first, ni, skip;
+  DevArray<long long> poff;
+  DevArray<double> pq;                    // [n_lm][Nel][4][TILE2]: P0, P2, Q0, Q2
+  DevArray<double> chol;                  // per (ilm, e): rank x {M0_p, M2_p} tiles
+  DevArray<double> sigma;                 // per (ilm, e): rank signs
+  DevArray<int> chol_off, chol_rank;      // [n_lm*Nel]; chol_off in units of 2*TILE2
+  // Coulomb tables
+  DevArray<int> LM_ilm;                   // per (L,M): index into the (L,|M|) caches
+  DevArray<double> LM_fac;                // LMfac(L,M)
+  DevArray<int> pstart, pair_kl;          // per (L,M) density pairs k*Nang + l
+  DevArray<double> pair_c;                // (c0, c2) per pair
+  DevArray<int> out_start, out_q;         // per output (i,j): (L,M) list
+  DevArray<double> out_c;                 // (c0, c2) per entry
+  // Exchange: per-shell Gaunt tables on the host; coupling lists built for
+  // the density blocks that pass the norm screen (basis.cpp:1858-1875).
+  std::vector<double> h_g0, h_g2;         // [j][i][L]: gaunt_mod / gaunt3 (M = mj - mi)
+  int gL = 0;                             // L extent of h_g0/h_g2
+  std::vector<unsigned char> screen_mask;
+  std::vector<double> h_rad_ilm;          // radial flops per coupled (jk, ilm)
+  DevArray<int> act_jk, act_lstart;       // active output blocks; ilm runs per block
+  DevArray<int> lrun_ilm, lrun_tstart;    // per run: ilm and first term
+  DevArray<int> term_il;
+  DevArray<double> term_f;                // 4 per term: f00, f02, f20, f22 (LMfac included)
+  DevArray<int> mirror;                   // absm: pairs (dst jk, src jk)
+  int n_mirror = 0;
+  PinnedArray<double> h_bnorm;
+  DevArray<double> Jin, ds, db, preA, preB, bnorm;
+  DevArray<double> hostP, hostOut;
+  int64_t n_terms = 0, n_runs = 0, n_pairs = 0;
+  double flops_k_radial = 0.0, flops_k_angular = 0.0;
+};
+
 // launchers (atomic_jk.hip)
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                            hipStream_t s, KernelTimer *timer);
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer);
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
+
+// launchers (diatomic_jk.hip)
+void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
+                             hipStream_t s, KernelTimer *timer);
+void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s);
+void diatomic_exchange_kernels(const DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer);
 
 }  // namespace hfg

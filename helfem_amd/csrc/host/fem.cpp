@@ -207,7 +207,7 @@ Blk LipShape::eval_over_r(const std::vector<double> &x, double h) const {
   return out;
 }
 
-RadialFE::RadialFE(int nn, const std::vector<double> &bv, int nq) : nnodes(nn), nquad(nq), bval(bv) {
+RadialFE::RadialFE(int nn, const std::vector<double> &bv, int nq, bool dl) : nnodes(nn), nquad(nq), drop_left(dl), bval(bv) {
   std::vector<double> w;
   gauss_lobatto(nnodes, nodes, w);
   std::sort(nodes.begin(), nodes.end());
@@ -224,7 +224,7 @@ RadialFE::RadialFE(int nn, const std::vector<double> &bv, int nq) : nnodes(nn), 
 LipShape RadialFE::shape(int iel) const {
   LipShape s;
   s.nodes = nodes;
-  s.lo = iel == 0 ? 1 : 0;
+  s.lo = (iel == 0 && drop_left) ? 1 : 0;
   s.hi = iel == Nel() - 1 ? nnodes - 1 : nnodes;
   return s;
 }
@@ -244,26 +244,34 @@ Blk RadialFE::values(int kind, const std::vector<double> &x, int iel) const {
   return shape(iel).eval(x, kind, half_len(iel));
 }
 
-// FiniteElementBasis.cpp:460-491 + :556-604 (one panel, start order for a
-// degree nnodes-1 basis, Lobatto doubling to 512).
+// Fixed-rule element matrix, FiniteElementBasis.cpp:460-491 on [-1,1]; a
+// non-finite weight at a node counts as zero (the basis vanishes there).
+Blk RadialFE::matel_rule(int iel, int lk, int rk, const std::vector<double> &x, const std::vector<double> &w,
+                         const std::function<double(double)> *f) const {
+  const double h = half_len(iel), m = mid(iel);
+  const int n = (int)x.size();
+  Blk lh = values(lk, x, iel);
+  const Blk rh = values(rk, x, iel);
+  for (int i = 0; i < n; ++i) {
+    double wp = w[i] * h * 1.0;
+    if (f) {
+      const double fv = (*f)(m + h * x[i]);
+      wp = std::isfinite(fv) ? wp * fv : 0.0;
+    }
+    for (int j = 0; j < lh.c; ++j) lh(i, j) *= wp;
+  }
+  return blk_tn(lh, rh);
+}
+
+// FiniteElementBasis.cpp:556-604 (one panel, start order for a degree
+// nnodes-1 basis, Lobatto doubling to 512).
 Blk RadialFE::matel(int iel, int lk, int rk, const std::function<double(double)> *f) const {
   const int nstart = std::min(std::max((2 * (nnodes - 1) + 4) / 2 + 2, 5), 512);
-  const double h = half_len(iel), m = mid(iel);
   return refine(
       [&](int n) {
         std::vector<double> x, w;
         gauss_lobatto(n, x, w);
-        Blk lh = values(lk, x, iel);
-        const Blk rh = values(rk, x, iel);
-        for (int i = 0; i < n; ++i) {
-          double wp = w[i] * h * 1.0;
-          if (f) {
-            const double fv = (*f)(m + h * x[i]);
-            wp = std::isfinite(fv) ? wp * fv : 0.0;
-          }
-          for (int j = 0; j < lh.c; ++j) lh(i, j) *= wp;
-        }
-        return blk_tn(lh, rh);
+        return matel_rule(iel, lk, rk, x, w, f);
       },
       nstart, 512);
 }

@@ -55,10 +55,14 @@ Blk lagrange_prim(const std::vector<double> &x, const std::vector<double> &nodes
 // (FiniteElementBasisT + FEMRadialBasisT).
 struct RadialFE {
   int nnodes = 0, nquad = 0;
+  bool drop_left = true;       // zero value at the left end (r = 0)
   std::vector<double> nodes, bval;
   std::vector<int> first, ni;  // first global index and size of each element
   RadialFE() {}
-  RadialFE(int nnodes, const std::vector<double> &bval, int nquad);
+  RadialFE(int nnodes, const std::vector<double> &bval, int nquad, bool drop_left = true);
+  // fixed-rule element matrix <lkind| f |rkind> on [-1,1] (kinds 0, 1, 10)
+  Blk matel_rule(int iel, int lk, int rk, const std::vector<double> &x, const std::vector<double> &w,
+                 const std::function<double(double)> *f) const;
   int Nel() const { return (int)bval.size() - 1; }
   int Nrad() const { return first.back() + ni.back(); }
   LipShape shape(int iel) const;

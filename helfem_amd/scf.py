@@ -51,6 +51,7 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
     X = [_invh(S[b][:, b]) for b in blocks]
     nocc = nelec // 2
     have_exx = xc is None
+    Enucr = float(getattr(basis, "Enucr", 0.0))  # nuclear repulsion of a diatomic basis
 
     def density(F):
         evs, vecs = [], []
@@ -93,7 +94,7 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
             F = F + Vxc
         Ekin = torch.sum(P * T)
         Enuc = torch.sum(P * V)
-        E = float(Ekin + Enuc + Ecoul + Exx + Exc)
+        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + Enucr
         errs = []
         for k, b in enumerate(blocks):
             Fk, Pk, Sk = F[b][:, b], P[b][:, b], S[b][:, b]
@@ -128,5 +129,5 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
             Fd = F
         P, nk, _ = density(Fd)
     torch.cuda.synchronize(dev)
-    return SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc),
+    return SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc), Enucr=Enucr,
                      iterations=it, P=P, occupations=nk, seconds=time.perf_counter() - t0, history=hist)

@@ -36,6 +36,7 @@ extern "C" {
 
 typedef struct hf_ctx hf_ctx;
 typedef struct hf_atomic hf_atomic;
+typedef struct hf_diatomic hf_diatomic;
 
 /* Thread-local text of the last error. */
 const char *hf_last_error(void);
@@ -62,6 +63,13 @@ int hf_ctx_synchronize(hf_ctx *ctx);
 int hf_ctx_set_timing(hf_ctx *ctx, int on);
 int hf_ctx_kernel_stats(hf_ctx *ctx, const char *name, double out[3]);
 
+/* Element boundaries of the radial (atomic: r, diatomic: mu) grid on
+ * [0, rmax]: igrid 1 linear, 2 quadratic, 3 polynomial (exponent zexp),
+ * 4 generalised exponential; out gets nel+1 values.  Replaces
+ * helfem::utils::get_grid (libhelfem/include/grid.h:37-101), called by
+ * atomic::basis::normal_grid (src/atomic/basis.cpp:45-53). */
+int hf_element_grid(double rmax, int nel, int igrid, double zexp, double *out);
+
 /* ----------------------------------------------------------- atomic basis */
 
 /* Radial finite elements x complex Y_lm.  Replaces the constructor
@@ -81,11 +89,13 @@ int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]);
 /* First radial index and size of every element (FEMRadialBasisT::get_idx,
  * libhelfem/include/RadialBasis.h:128). */
 int hf_atomic_elements(const hf_atomic *h, int *first, int *nprim);
-/* Algorithmic work of one exchange build: out[0] = flops of the angular
- * (Gaunt-weighted density) sums over full Nrad x Nrad blocks, out[1] = flops
- * of the radial RI-K contraction, out[2] = number of (output block, L,
- * density block) terms, out[3] = number of coupled output blocks.  Counted
- * as the reference's TwoDBasis.cpp:918-995 would perform them. */
+/* Algorithmic work of the most recent exchange build, which depends on the
+ * density through the block-norm screen (TwoDBasis.cpp:949-952): out[0] =
+ * flops of the angular (Gaunt-weighted density) sums over full Nrad x Nrad
+ * blocks, out[1] = flops of the radial RI-K contraction, out[2] = number of
+ * (output block, L, density block) terms, out[3] = number of coupled output
+ * blocks.  Counted as the reference's TwoDBasis.cpp:918-995 performs them;
+ * all zero before the first exchange call. */
 int hf_atomic_work(hf_atomic *h, double out[4]);
 
 /* Build the per-(L, element) factor caches on the host and upload them:
@@ -118,6 +128,60 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
 /* Device-pointer forms of the two above, asynchronous on the context stream. */
 int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+
+/* --------------------------------------------------------- diatomic basis */
+
+/* Prolate spheroidal basis: radial finite elements in mu x Y_lm(nu, phi)
+ * for nuclei Z1 at z = -Rhalf and Z2 at z = +Rhalf.  Replaces the
+ * constructor helfem::diatomic::basis::TwoDBasis::TwoDBasis
+ * (src/diatomic/basis.h:222) with LIP shape functions on `nnodes`
+ * Gauss-Lobatto nodes (primbas 4); the mu = 0 function is kept and the
+ * mu = mu_max function dropped.  bval: element boundaries in mu (nbval =
+ * Nel+1, bval[0] = 0); lval/mval: shells in lm_to_l_m order
+ * (basis.cpp:500-516).  Matrices are in the pure basis, which drops radial
+ * function 0 of every m != 0 shell (pure_indices, basis.h:250).  ctx may be
+ * NULL: the handle then supports the host-side setup but not J/K. */
+int hf_diatomic_create(hf_ctx *ctx, int Z1, int Z2, double Rhalf, int nnodes, int nquad, const double *bval,
+                       int nbval, const int *lval, const int *mval, int nang, hf_diatomic **out);
+int hf_diatomic_destroy(hf_diatomic *h);
+/* out[0]=Nbf (pure), out[1]=Ndummy, out[2]=Nrad, out[3]=Nang, out[4]=Nel,
+ * out[5]=size of the (L,|M|) map, out[6]=size of the (L,M) map. */
+int hf_diatomic_sizes(const hf_diatomic *h, int64_t out[7]);
+/* Dummy index (shell*Nrad + radial) of every pure function, Nbf entries
+ * (TwoDBasis::pure_indices, basis.h:250). */
+int hf_diatomic_pure_index(const hf_diatomic *h, int64_t *idx);
+/* (L,|M|) map (2 ints per entry, |M|-major) and (L,M) map (L-major); either
+ * pointer may be NULL (lm_map / LM_map, basis.cpp:612-647). */
+int hf_diatomic_maps(const hf_diatomic *h, int *lm, int *LM);
+/* With sym != 0 the density is taken to be symmetric under m -> -m: the
+ * exchange builds only output blocks with mj, mk >= 0 and mirrors
+ * K(-mj,-mk) = K(mj,mk).  Replaces TwoDBasis::set_absm_symmetric
+ * (basis.h:344). */
+int hf_diatomic_set_absm_symmetric(hf_diatomic *h, int sym);
+/* Build the per-((L,|M|), element) caches on the host: disjoint integrals
+ * P0, P2, Q0, Q2 and the sign-aware Cholesky factor of the in-element
+ * two-channel kernel.  Replaces TwoDBasis::compute_tei (basis.h:265,
+ * basis.cpp:1382-1547). */
+int hf_diatomic_compute_tei(hf_diatomic *h, int exchange);
+/* Read a cache back: kind 0 P0, 1 P2, 2 Q0, 3 Q2 (Ni x Ni), 4 the Cholesky
+ * vectors B (2 Ni^2 x rank), 5 their signs (rank x 1).  dims gets
+ * (rows, cols); data may be NULL. */
+int hf_diatomic_get_cache(const hf_diatomic *h, int kind, int ilm, int iel, int dims[2], double *data);
+/* Pure-basis one-electron matrices: which 0 overlap, 1 kinetic, 2 nuclear
+ * attraction (basis.h:280-284, basis.cpp:1032-1156). */
+int hf_diatomic_one_electron(const hf_diatomic *h, int which, double *out, int64_t ld);
+/* Work of the most recent exchange build, as hf_atomic_work
+ * (basis.cpp:1818-2089). */
+int hf_diatomic_work(hf_diatomic *h, double out[4]);
+/* Coulomb matrix of the total density, pure basis.  Replaces
+ * TwoDBasis::coulomb (basis.h:300, basis.cpp:1627-1816). */
+int hf_diatomic_coulomb(hf_diatomic *h, const double *P, int64_t ldp, double *J, int64_t ldj);
+/* Exchange contribution (-K) of the spin density, pure basis.  Replaces
+ * TwoDBasis::exchange (basis.h:302, basis.cpp:1818-2089). */
+int hf_diatomic_exchange(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk);
+/* Device-pointer forms, asynchronous on the context stream. */
+int hf_diatomic_coulomb_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
+int hf_diatomic_exchange_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,19 @@ Mat converge_block(const std::function<Mat(int)> &probe, int nstart, int *nconv,
 }
 
 double ipow(double x, int n) { return std::pow(x, double(n)); }
+
+// Integer power by repeated squaring: the weights of the two-electron
+// quadrature use Eigen's array pow with an int exponent (quadrature.cpp:216,
+// :241), which multiplies rather than calling pow().
+double ipow_sq(double x, int n) {
+  double r = 1.0, b = x;
+  for (unsigned e = (unsigned)n; e; ) {
+    if (e & 1u) r *= b;
+    e >>= 1;
+    if (e) b *= b;
+  }
+  return r;
+}
 }  // namespace
 
 // gaunt.h:53-132 semantics without storage bounds (the reference sizes its
@@ -254,7 +267,7 @@ Mat DiatomicBasis::twoe_integral(const TwoEl &el, int k, int l, int L, LegendreC
       for (int i = 0; i < nq; ++i) {
         double v = s.mulen * el.w[i];
         v *= s.shmu[i];
-        if (lv) v *= ipow(s.chmu[i], lv);
+        if (lv) v *= ipow_sq(s.chmu[i], lv);
         v *= leg.P(L, s.chmu[i]);
         wp[i] = v;
       }
@@ -270,7 +283,7 @@ Mat DiatomicBasis::twoe_integral(const TwoEl &el, int k, int l, int L, LegendreC
     for (int i = 0; i < nq; ++i) {
       double v = el.mulen * el.w[i];
       v *= el.shmu[i];
-      if (kv) v *= ipow(el.chmu[i], kv);
+      if (kv) v *= ipow_sq(el.chmu[i], kv);
       v *= leg.Q(L, el.chmu[i]);
       wp[i] = v;
     }
@@ -279,7 +292,7 @@ Mat DiatomicBasis::twoe_integral(const TwoEl &el, int k, int l, int L, LegendreC
       for (int i = 0; i < nq; ++i) wbp(i, c) *= wp[i];
     return matmul_tn(wbp, inner);
   };
-  const Mat t1 = outer(k, l), t2 = outer(l, k);
+  const Mat t1 = outer(k, l), t2 = k == l ? t1 : outer(l, k);
   Mat out(nb2, nb2);
   for (int j = 0; j < nb2; ++j)
     for (int i = 0; i < nb2; ++i) out(i, j) = t1(i, j) + t2(j, i);

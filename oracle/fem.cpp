@@ -21,15 +21,35 @@ Mat matmul(const Mat &A, const Mat &B) {
   return C;
 }
 
+// Every entry is a dot product summed in index order; four columns of B are
+// processed together only to let the loop vectorise (same sums, same order).
 Mat matmul_tn(const Mat &A, const Mat &B) {
   if (A.r != B.r) throw std::logic_error("matmul_tn: shape mismatch");
   Mat C(A.c, B.c);
-  for (int j = 0; j < B.c; ++j)
+  const int K = A.r;
+  int j = 0;
+  for (; j + 4 <= B.c; j += 4)
     for (int i = 0; i < A.c; ++i) {
-      const double *a = &A.a[(size_t)i * A.r];
-      const double *b = &B.a[(size_t)j * B.r];
+      const double *a = &A.a[(size_t)i * K];
+      const double *b = &B.a[(size_t)j * K];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int k = 0; k < K; ++k) {
+        s0 += a[k] * b[k];
+        s1 += a[k] * b[k + K];
+        s2 += a[k] * b[k + 2 * (size_t)K];
+        s3 += a[k] * b[k + 3 * (size_t)K];
+      }
+      C(i, j) = s0;
+      C(i, j + 1) = s1;
+      C(i, j + 2) = s2;
+      C(i, j + 3) = s3;
+    }
+  for (; j < B.c; ++j)
+    for (int i = 0; i < A.c; ++i) {
+      const double *a = &A.a[(size_t)i * K];
+      const double *b = &B.a[(size_t)j * K];
       double s = 0.0;
-      for (int k = 0; k < A.r; ++k) s += a[k] * b[k];
+      for (int k = 0; k < K; ++k) s += a[k] * b[k];
       C(i, j) = s;
     }
   return C;
