@@ -3,10 +3,10 @@
 
 A step is one Fock build of the two-electron part: J(P) and K(P/2) on a
 seeded density resident in HBM (FP64 plain mode, the drop-in path).  The
-workload defaults to the atomic Xe configuration (Z=54, lmax=mmax=6, 25
-LIP-15 elements to 40 bohr, Nbf = 17101), the largest single-GPU atomic
-configuration; the diatomic N2 configuration will replace it once the
-prolate-spheroidal path is built.
+default workload is diatomic N2 at R = 2.074 bohr in prolate spheroidal
+coordinates: 8 mu-elements to mu_max = arcosh(40 / Rhalf), LIP-15, lmax 16
+for m = 0..3 (Nbf 11894), 14 electrons in sigma/pi blocks.  Atomic
+workloads (Xe lmax=mmax=6, Ne) are selectable with --workload.
 
 Prints one JSON line (rank 0):
   value        Fock builds/s over the timed steps (inputs already in HBM)
@@ -39,13 +39,19 @@ HBM_PEAK_GBS = 8000.0
 _MADELUNG = sorted(((n, l) for n in range(1, 8) for l in range(0, n)), key=lambda s: (s[0] + s[1], s[0]))
 
 WORKLOADS = {
-    "xe-hf-atomic": dict(Z=54, lmax=6, mmax=6, nelem=25, seed=1003,
+    "n2-hf-diatomic": dict(kind="diatomic", Z1=7, Z2=7, R=2.074, lmmax=[16, 16, 16, 16], nelem=8, seed=1002,
+                           occ={0: 5, 1: 1, -1: 1},
+                           desc="N2 HF diatomic, R=2.074 bohr, 8 mu-elements to r~40 bohr, LIP-15, lmax=16 for "
+                                "m=0..3, Nbf 11894"),
+    "xe-hf-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003,
                          desc="Xe HF atomic, lmax=mmax=6, 25 LIP-15 elements to 40 bohr (exp grid), Nbf 17101"),
-    "ne-hf-atomic": dict(Z=10, lmax=0, mmax=0, nelem=10, seed=1001,
+    "ne-hf-atomic": dict(kind="atomic", Z=10, lmax=0, mmax=0, nelem=10, seed=1001,
                          desc="Ne HF atomic, lmax=0, 10 LIP-15 elements to 40 bohr (exp grid), Nbf 139"),
-    "ne-hf-atomic-l1": dict(Z=10, lmax=1, mmax=1, nelem=10, seed=1001,
+    "ne-hf-atomic-l1": dict(kind="atomic", Z=10, lmax=1, mmax=1, nelem=10, seed=1001,
                             desc="Ne HF atomic, lmax=mmax=1, 10 LIP-15 elements to 40 bohr, Nbf 556"),
 }
+KERNELS = {"atomic": ("atomic_k", "atomic_j_gather", "atomic_j_expand"),
+           "diatomic": ("diatomic_k", "diatomic_j_gather", "diatomic_j_expand")}
 
 
 def occupied_shells(Z: int):
@@ -73,10 +79,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="xe-hf-atomic", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="n2-hf-diatomic", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-stride", type=int, default=24,
-                    help="time every n-th exchange output block on the CPU and extrapolate")
+    ap.add_argument("--cpu-sample-stride", type=int, default=0,
+                    help="time every n-th exchange output block on the CPU and extrapolate "
+                         "(0: 24 for atomic, 64 for diatomic workloads)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of known size (for PMC byte calibration)")
     return ap.parse_args()
@@ -86,12 +93,29 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def make_basis(H, cfg, ctx):
+    if cfg["kind"] == "atomic":
+        return H.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"], ctx=ctx)
+    return H.DiatomicBasis(cfg["Z1"], cfg["Z2"], cfg["R"], cfg["lmmax"], cfg["nelem"], ctx=ctx)
+
+
+def seeded_occupations(cfg, hb):
+    """Doubly occupied orbitals per m-block (blocks in ascending m)."""
+    ms = sorted(set(hb.mval.tolist()))
+    if cfg["kind"] == "atomic":
+        return block_occupations(cfg["Z"], hb.mval.tolist(), cfg["lmax"])
+    return [cfg["occ"].get(m, 0) for m in ms]
+
+
 def cpu_baseline(cfg, P, nblocks_total, stride):
     """Oracle J (full) and K (strided sample of output blocks, extrapolated)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ob = O.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"])
+    if cfg["kind"] == "atomic":
+        ob = O.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"])
+    else:
+        ob = O.DiatomicBasis(cfg["Z1"], cfg["Z2"], cfg["R"], cfg["lmmax"], cfg["nelem"])
     t = time.perf_counter()
     ob.compute_tei()
     log(f"oracle setup {time.perf_counter() - t:.1f}s")
@@ -123,17 +147,19 @@ def main():
     from helfem_amd.synthetic import seeded_density
 
     cfg = WORKLOADS[args.workload]
+    kind = cfg["kind"]
+    kname, jg, je = KERNELS[kind]
     ctx = H.Context(0)
     t = time.perf_counter()
-    hb = H.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"], ctx=ctx)
+    hb = make_basis(H, cfg, ctx)
     hb.compute_tei()
     log(f"basis Nbf={hb.Nbf} Nrad={hb.Nrad} Nang={hb.Nang}; host setup {time.perf_counter() - t:.1f}s")
     S = hb.overlap()
     blocks = hb.get_sym_idx(1)
-    nocc = block_occupations(cfg["Z"], hb.mval.tolist(), cfg["lmax"])
+    nocc = seeded_occupations(cfg, hb)
     P = seeded_density(S, blocks, nocc, seed=cfg["seed"])
     nelec = float(np.sum(P * S))
-    log(f"seeded density: tr(PS) = {nelec:.6f}")
+    log(f"seeded density: tr(PS) = {nelec:.6f}, occupied orbitals per m-block {nocc}")
 
     dev = torch.device("cuda:0")
     Pd = torch.as_tensor(P, device=dev)
@@ -141,12 +167,12 @@ def main():
     J = torch.empty_like(Pd)
     K = torch.empty_like(Pd)
     ctx.bind_torch_stream()
-    work = np.zeros(4)
-    _lib.check(_lib.lib().hf_atomic_work(hb._h, work.ctypes.data))
+    t = time.perf_counter()
     for _ in range(args.warmup):
         hb.coulomb(Pd, out=J)
         hb.exchange(Pa, out=K)
     torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps {time.perf_counter() - t:.2f}s (includes device upload of the caches)")
     _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 1))
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -161,13 +187,13 @@ def main():
     t1 = time.perf_counter()
     lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
     stats = {}
-    for name in ("atomic_k", "atomic_j_gather", "atomic_j_expand"):
+    for name in (kname, jg, je):
         out = np.zeros(3)
         _lib.check(_lib.lib().hf_ctx_kernel_stats(ctx.handle, name.encode(), out.ctypes.data))
         stats[name] = out
     _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 0))
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    kst = stats["atomic_k"]
+    kst = stats[kname]
     k_ms = kst[1] / max(kst[0], 1)
     k_flops = kst[2] / max(kst[0], 1)
     achieved = k_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
@@ -183,7 +209,6 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: 3 x {nbytes} bytes copied")
 
-    # sanity: J symmetric, K symmetric, energies finite
     Ecoul = 0.5 * float(torch.sum(Pd * J))
     Exx = float(torch.sum(Pa * K))
     log(f"Ecoul {Ecoul:.10f} Exx {Exx:.10f}")
@@ -202,17 +227,20 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg["desc"], "mode": "plain FP64 J+K (HF Fock build)", "Nbf": hb.Nbf,
-                   "density": f"seeded SplitMix64 (seed {cfg['seed']}), P = 2CC', tr(PS) = {nelec:.3f}",
+                   "density": f"seeded SplitMix64 (seed {cfg['seed']}), P = 2CC' per m-block, "
+                              f"tr(PS) = {nelec:.3f}",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "roofline": {"bound": "mfma", "kernel": "k_atomic_k (exchange)", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": k_ms, "flops_per_launch": k_flops},
+        "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k", "diatomic": "k_dia_k"}[kind] + " (exchange)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "avg_launch_ms": k_ms,
+                     "flops_per_launch": k_flops},
         "kernels_ms": {k: (v[1] / max(v[0], 1)) for k, v in stats.items()},
         "energies": {"coulomb": Ecoul, "exchange": Exx},
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(cfg, P, hb.Nang * hb.Nang, args.cpu_sample_stride)
+        stride = args.cpu_sample_stride or (24 if kind == "atomic" else 64)
+        result["cpu_baseline"] = cpu_baseline(cfg, P, hb.Nang * hb.Nang, stride)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

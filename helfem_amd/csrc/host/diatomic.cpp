@@ -528,6 +528,7 @@ template <class F>
 void fill_pure(const DiatomicSetup &s, double *out, int64_t ld, F block_value) {
   const int N = s.Nrad(), na = s.Nang();
   const int64_t nbf = s.Nbf();
+#pragma omp parallel for schedule(static)
   for (int64_t j = 0; j < nbf; ++j)
     for (int64_t i = 0; i < nbf; ++i) out[i + j * ld] = 0.0;
 #pragma omp parallel for schedule(dynamic)
@@ -540,15 +541,27 @@ void fill_pure(const DiatomicSetup &s, double *out, int64_t ld, F block_value) {
           out[(s.poff[i] + a - si) + (s.poff[j] + b - sj) * ld] = block_value(i, j, a, b);
     }
 }
+
+// per shell pair coupling coefficient, evaluated once per block
+template <class G>
+std::vector<double> pair_table(const DiatomicSetup &s, G coef) {
+  const int na = s.Nang();
+  std::vector<double> t((size_t)na * na);
+  for (int j = 0; j < na; ++j)
+    for (int i = 0; i < na; ++i) t[(size_t)i * na + j] = coef(i, j);
+  return t;
+}
 }  // namespace
 
 void DiatomicSetup::overlap(double *out, int64_t ld) const {
   const Blk I10 = radial_integral(1, 0), I12 = radial_integral(1, 2);
   const double f = std::pow(Rh, 3);
+  const std::vector<double> cc =
+      pair_table(*this, [&](int i, int j) { return gaunt_cos2(lval[j], mval[j], lval[i], mval[i]); });
   fill_pure(*this, out, ld, [&](int i, int j, int a, int b) {
     double v = 0.0;
     if (lval[i] == lval[j]) v += 1.0 * I12(a, b);
-    const double c = gaunt_cos2(lval[j], mval[j], lval[i], mval[i]);
+    const double c = cc[(size_t)i * Nang() + j];
     if (c != 0.0) v += -c * I10(a, b);
     return v * f;
   });
@@ -570,11 +583,13 @@ void DiatomicSetup::kinetic(double *out, int64_t ld) const {
 void DiatomicSetup::nuclear(double *out, int64_t ld) const {
   const Blk I10 = radial_integral(1, 0), I11 = radial_integral(1, 1);
   const double f = -std::pow(Rh, 2);
+  const std::vector<double> cc =
+      pair_table(*this, [&](int i, int j) { return gaunt_cos(lval[j], mval[j], lval[i], mval[i]); });
   fill_pure(*this, out, ld, [&](int i, int j, int a, int b) {
     double v = 0.0;
     if (lval[i] == lval[j]) v += double(Z1 + Z2) * I11(a, b);
     if (Z1 != Z2) {
-      const double c = gaunt_cos(lval[j], mval[j], lval[i], mval[i]);
+      const double c = cc[(size_t)i * Nang() + j];
       if (c != 0.0) v += (double(Z2 - Z1) * c) * I10(a, b);
     }
     return v * f;

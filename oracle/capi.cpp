@@ -2,6 +2,7 @@
 // Loaded by tests/ and by bench.py's cpu_baseline leg through ctypes.
 #include "orc.hpp"
 #include "diatomic.hpp"
+#include "dftgrid.hpp"
 #include <cstring>
 #include <string>
 
@@ -232,6 +233,28 @@ int orc_diatomic_exchange(void *h, const double *P, double *K) {
   return guard([&] {
     DiatomicBasis *b = (DiatomicBasis *)h;
     put(b->exchange(get(P, b->Nbf(), b->Nbf())), K);
+  });
+}
+int orc_diatomic_exchange_blocks(void *h, const double *P, double *K, int nblocks, const int *jk) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    put(b->exchange_blocks(get(P, b->Nbf(), b->Nbf()), std::vector<int>(jk, jk + nblocks)), K);
+  });
+}
+// LDA functional values at n densities
+int orc_lda(int xf, int cf, int n, const double *rho, double thr, double *exc, double *vrho) {
+  return guard([&] {
+    for (int i = 0; i < n; ++i) lda_unpolarized(xf, cf, rho[i], thr, exc[i], vrho[i]);
+  });
+}
+// Atomic DFT grid: H (Nbf x Nbf) and out = (Exc, Nel)
+int orc_atomic_xc(void *h, int lang, int mang, int xf, int cf, const double *P, double thr, double *H, double *out) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    AtomicXC g(*b, lang, mang);
+    Mat Hm;
+    g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    put(Hm, H);
   });
 }
 // lm map entry i: out = (L, |M|); LM map similarly (L, M)

@@ -1,0 +1,22 @@
+// Oracle (test infrastructure only): LDA exchange-correlation declarations.
+#pragma once
+#include "orc.hpp"
+#include <complex>
+
+namespace orc {
+
+// lda_x (1) + lda_c_vwn (7, VWN5), spin-unpolarized; thr as libxc's density threshold
+void lda_unpolarized(int xf, int cf, double rho, double thr, double &exc, double &vrho);
+void angular_chebyshev(int l, int m, std::vector<double> &cth, std::vector<double> &phi, std::vector<double> &w);
+std::complex<double> spherical_harmonic(int l, int m, double cth, double phi);
+
+// atomic::dftgrid::DFTGrid (src/atomic/dftgrid.h:140-173), LDA restricted
+struct AtomicXC {
+  const AtomicBasis &b;
+  int lang, mang;
+  std::vector<double> cth, phi, wang;
+  AtomicXC(const AtomicBasis &b, int lang, int mang);
+  void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
+};
+
+}  // namespace orc

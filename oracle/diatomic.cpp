@@ -606,6 +606,13 @@ Mat DiatomicBasis::coulomb(const Mat &Pin) const {
 
 // exchange, basis.cpp:1818-2089 (absm_symmetric = false)
 Mat DiatomicBasis::exchange(const Mat &Pin) const {
+  std::vector<int> all((size_t)Nang() * Nang());
+  for (size_t q = 0; q < all.size(); ++q) all[q] = (int)q;
+  return exchange_blocks(Pin, all);
+}
+
+// The same restricted to the output blocks j*Nang+k listed in jks.
+Mat DiatomicBasis::exchange_blocks(const Mat &Pin, const std::vector<int> &jks) const {
   if (cd_B.empty()) throw std::logic_error("Primitive teis have not been computed!");
   const Mat P = expand_boundaries(Pin);
   const int Nel = radial.Nel(), N = Nrad(), Na = Nang(), nlm = (int)lm_map.size();
@@ -627,9 +634,9 @@ Mat DiatomicBasis::exchange(const Mat &Pin) const {
       bydm[dm + dmoff].push_back({i, l});
     }
   Mat K(Ndummy(), Ndummy());
-#pragma omp parallel for collapse(2) schedule(dynamic)
-  for (int j = 0; j < Na; ++j)
-    for (int k = 0; k < Na; ++k) {
+#pragma omp parallel for schedule(dynamic)
+  for (size_t t = 0; t < jks.size(); ++t) {
+      const int j = jks[t] / Na, k = jks[t] % Na;
       const int lj = lval[j], mj = mval[j], lk = lval[k], mk = mval[k];
       const int dmjk = mj - mk;
       if (std::abs(dmjk) > 2 * mabs) continue;

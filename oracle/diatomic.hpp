@@ -62,6 +62,7 @@ struct DiatomicBasis {
   Mat nuclear() const;
   Mat coulomb(const Mat &P) const;
   Mat exchange(const Mat &P) const;
+  Mat exchange_blocks(const Mat &P, const std::vector<int> &jk) const;
 };
 
 }  // namespace orc

@@ -43,8 +43,12 @@ def lib():
                      "orc_atomic_exchange_blocks", "orc_legendre", "orc_diatomic_destroy",
                      "orc_diatomic_sizes", "orc_diatomic_pure_idx", "orc_diatomic_compute_tei",
                      "orc_diatomic_matrix", "orc_diatomic_coulomb", "orc_diatomic_exchange",
-                     "orc_diatomic_maps", "orc_diatomic_cache"):
+                     "orc_diatomic_maps", "orc_diatomic_cache", "orc_diatomic_exchange_blocks", "orc_atomic_xc"):
             getattr(L, name).argtypes = None
+        L.orc_lda.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
+                              ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_create.restype = ctypes.c_void_p
         L.orc_diatomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -82,6 +86,15 @@ def chebyshev(n):
     w = np.zeros(n)
     _check(lib().orc_chebyshev(n, _ptr(x), _ptr(w)))
     return x, w
+
+
+def lda(rho, x_func=1, c_func=7, thr=1e-12):
+    """lda_x + lda_c_vwn (VWN5), spin-unpolarized: (exc per particle, vrho)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    exc = np.zeros_like(rho)
+    v = np.zeros_like(rho)
+    _check(lib().orc_lda(x_func, c_func, rho.size, _ptr(rho), thr, _ptr(exc), _ptr(v)))
+    return exc, v
 
 
 def grid(rmax, nel, igrid=4, zexp=2.0):
@@ -188,6 +201,20 @@ class AtomicBasis:
         K = np.zeros((self.Nbf, self.Nbf), order="F")
         _check(lib().orc_atomic_exchange_blocks(self._h, _ptr(P), _ptr(K), jk.size, _ptr(jk)))
         return K
+
+    def xc(self, P, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
+        """DFTGrid::eval_Fxc (src/atomic/dftgrid.cpp:591-653), LDA, restricted.
+
+        Grid defaults ldft = 4 lmax + 12, mdft = 4 mmax + 12 (atomic/main.cpp:329-330).
+        Returns (Vxc, Exc, Nel)."""
+        lmax, mmax = int(np.max(self.lval)), int(np.max(np.abs(self.mval)))
+        lang = 4 * lmax + 12 if lang is None else lang
+        mang = 4 * mmax + 12 if mang is None else mang
+        P = np.asfortranarray(P, dtype=np.float64)
+        H = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_atomic_xc(self._h, lang, mang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        return H, float(out[0]), float(out[1])
 
     def cache(self, kind, L, iel):
         dims = (ctypes.c_int * 2)()
@@ -308,6 +335,14 @@ class DiatomicBasis:
         _check(lib().orc_diatomic_exchange(self._h, _ptr(P), _ptr(K)))
         return K
 
+    def exchange_blocks(self, P, jk):
+        """-K restricted to the output blocks j*Nang+k listed in jk."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        jk = np.ascontiguousarray(jk, dtype=np.int32)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_diatomic_exchange_blocks(self._h, _ptr(P), _ptr(K), jk.size, _ptr(jk)))
+        return K
+
     def maps(self):
         lm = np.zeros(2 * self.n_lm, np.int32)
         LM = np.zeros(2 * self.n_LM, np.int32)
@@ -347,11 +382,13 @@ class SCFResult(dict):
     pass
 
 
-def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None):
-    """Restricted closed-shell HF with Pulay DIIS over per-m symmetry blocks.
+def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None, xc=None):
+    """Restricted closed-shell HF (or DFT) with Pulay DIIS over per-m symmetry blocks.
 
     jk(P, Pa) -> (J, Ka) may be supplied to swap in another Fock engine; the
-    default is the oracle's own coulomb/exchange."""
+    default is the oracle's own coulomb/exchange.  xc(P) -> (Vxc, Exc) turns
+    the run into Kohn-Sham DFT: no exact exchange, Etot gains Exc
+    (atomic/main.cpp:369-470)."""
     S = basis.overlap()
     T = basis.kinetic()
     V = basis.nuclear()
@@ -387,13 +424,20 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
     E = 0.0
     for it in range(1, maxit + 1):
         Pa = 0.5 * P
-        J, Ka = jk(P, Pa)
         Ekin = np.sum(P * T)
         Enuc = np.sum(P * V)
+        if xc is None:
+            J, Ka = jk(P, Pa)
+            Exx = 2.0 * 0.5 * np.sum(Pa * Ka)
+            Exc = 0.0
+            F = H + J + Ka
+        else:
+            J = basis.coulomb(P)
+            Vxc, Exc = xc(P)
+            Exx = 0.0
+            F = H + J + Vxc
         Ecoul = 0.5 * np.sum(P * J)
-        Exx = 2.0 * 0.5 * np.sum(Pa * Ka)
-        E = Ekin + Enuc + Ecoul + Exx
-        F = H + J + Ka
+        E = Ekin + Enuc + Ecoul + Exx + Exc
         # orthonormal-basis commutator per block
         errs = []
         for k, b in enumerate(blocks):
@@ -428,4 +472,4 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
             Fd = F
         P, occ = density(Fd)
     E += getattr(basis, "Enucr", 0.0)
-    return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, iterations=it, P=P)
+    return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, P=P)
