@@ -184,6 +184,8 @@ struct KArgs {
   const int *chol_off, *chol_rank;
   const int *act_jk, *term_start, *term_il;
   const double *term_c;
+  const int *efl;  // element pairs e*Nel+f of this launch
+  int n_ef;
 };
 
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
@@ -194,18 +196,19 @@ struct KArgs {
 //   e != f:  K += A R B'       (A, B the r^L / r^-L-1 element matrices)
 //   e == f:  K += sum_p M_p R M_p'   (M_p the Cholesky vectors as 16x16)
 // Workgroups are mapped so that those sharing (e,f) run on one XCD.
+template <bool INSIDE>
 __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
   const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
-  const int ef = (loc / A.n_act) * 8 + xcd;
+  const int efi = (loc / A.n_act) * 8 + xcd;
   const int act = loc - (loc / A.n_act) * A.n_act;
-  if (ef >= A.Nel * A.Nel) return;
-  const int e = ef / A.Nel, f = ef - e * A.Nel;
+  if (efi >= A.n_ef) return;
+  const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
   __shared__ double red[4 * TILE2];
-  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
   const bool rowok = c < ni;
   bool colok[4];
 #pragma unroll
@@ -226,7 +229,7 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
       }
     }
     const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
-    if (e != f) {
+    if (!INSIDE) {
       const double *At = (e > f) ? A.rb + te : A.rs + te;
       const double *Bt = (e > f) ? A.rs + tf : A.rb + tf;
       double as[4], bs[4];
@@ -242,19 +245,34 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
     } else {
       const int rank = A.chol_rank[L * A.Nel + e];
       const double *M = A.chol + A.chol_off[L * A.Nel + e];
-      for (int p = 0; p < rank; ++p) {
-        const double *Mp = M + (size_t)p * TILE2;
-        double ms[4];
+      // two Cholesky vectors per step, so two independent chains overlap;
+      // M_p(c, k) serves as both the Y operand of R M_p' and the X operand of M_p Y
+      for (int p = 0; p < rank; p += 2) {
+        const double *Ma = M + (size_t)p * TILE2;
+        const double *Mb = M + (size_t)(p + 1 < rank ? p + 1 : p) * TILE2;
+        const double wb = p + 1 < rank ? 1.0 : 0.0;
+        double ma[4], mb[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) ms[s] = Mp[c + TILE * (g + 4 * s)];
-        v4d y = {0.0, 0.0, 0.0, 0.0};
-        y = mfma_16x16x16(r, ms, y);         // Y = R M_p'
-        const double ys[4] = {y[0], y[1], y[2], y[3]};
-        acc = mfma_16x16x16(ms, ys, acc);    // K += M_p Y
+        for (int s = 0; s < 4; ++s) {
+          ma[s] = Ma[c + TILE * (g + 4 * s)];
+          mb[s] = wb * Mb[c + TILE * (g + 4 * s)];
+        }
+        v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          ya = __builtin_amdgcn_mfma_f64_16x16x4f64(r[s], ma[s], ya, 0, 0, 0);  // R M_a'
+          yb = __builtin_amdgcn_mfma_f64_16x16x4f64(r[s], mb[s], yb, 0, 0, 0);  // R M_b'
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[s], ya[s], acc, 0, 0, 0);   // M_a Y_a
+          acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(mb[s], yb[s], acc2, 0, 0, 0); // M_b Y_b
+        }
       }
     }
   }
   // combine the four waves' partial tiles (lane holds K(g+4r, c))
+  acc += acc2;
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
   __syncthreads();
@@ -317,7 +335,7 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   if (mask == d.screen_mask && !d.screen_mask.empty()) return;
   std::vector<int> act, tstart(1, 0), til;
   std::vector<double> tc;
-  double frad = 0.0;
+  double frad = 0.0, frad_in = 0.0;
   for (size_t a = 0; a < d.h_act.size(); ++a) {
     const size_t t0 = tstart.size(), n0 = til.size();
     std::vector<int> starts;
@@ -329,7 +347,10 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
           til.push_back(d.h_til[q]);
           tc.push_back(d.h_tc[q]);
         }
-      if (til.size() > before) frad += d.h_radL[L];
+      if (til.size() > before) {
+        frad += d.h_radL[L];
+        frad_in += d.h_radL_in[L];
+      }
       starts.push_back((int)til.size());
     }
     if (til.size() == n0) continue;  // nothing reaches this output block
@@ -341,6 +362,15 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   d.n_terms = (int64_t)til.size();
   d.flops_k_radial = frad;
   d.flops_k_angular = 2.0 * (double)til.size() * (double)d.Nrad * (double)d.Nrad;
+  // split between the in-element and cross-element launches, the angular
+  // sums in proportion to the tile areas they cover
+  double area_in = 0.0, area_all = 0.0;
+  for (int e = 0; e < d.Nel; ++e) {
+    area_in += (double)d.h_ni[e] * d.h_ni[e];
+    for (int f = 0; f < d.Nel; ++f) area_all += (double)d.h_ni[e] * d.h_ni[f];
+  }
+  d.flops_in = frad_in + d.flops_k_angular * area_in / area_all;
+  d.flops_off = d.flops_k_radial + d.flops_k_angular - d.flops_in;
   if (act.empty()) act.push_back(0);
   if (til.empty()) {
     til.push_back(0);
@@ -369,14 +399,24 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
   compact_exchange_lists(d, s);
   fill_zero_launch(K, nbf, nbf, ldk, s);
   if (d.n_act == 0) return;
-  const int n_ef_pad = ((d.Nel * d.Nel + 7) / 8) * 8;
-  KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
-          d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p};
-  const int64_t grid = (int64_t)n_ef_pad * d.n_act;
-  if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-  TimedLaunch tl(timer, "atomic_k", s, d.flops_k_radial + d.flops_k_angular);
-  hipLaunchKernelGGL(k_atomic_k, dim3((unsigned)grid), dim3(NT), 0, s, a);
-  hip_check(hipGetLastError(), "atomic_k");
+  // two launches: element pairs e != f and the in-element tiles; boundary
+  // rows/cols shared by neighbouring tiles are accumulated atomically in both
+  for (int part = 0; part < 2; ++part) {
+    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+    if (n_ef == 0) continue;
+    const int n_ef_pad = ((n_ef + 7) / 8) * 8;
+    KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
+            d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p,
+            part == 0 ? d.ef_off.p : d.ef_in.p, n_ef};
+    const int64_t grid = (int64_t)n_ef_pad * d.n_act;
+    if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+    TimedLaunch tl(timer, part == 0 ? "atomic_k_off" : "atomic_k_in", s, part == 0 ? d.flops_off : d.flops_in);
+    if (part == 0)
+      hipLaunchKernelGGL(k_atomic_k<false>, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_atomic_k<true>, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    hip_check(hipGetLastError(), "atomic_k");
+  }
 }
 
 }  // namespace hfg

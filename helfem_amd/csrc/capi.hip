@@ -7,13 +7,6 @@
 
 using namespace hfg;
 
-struct hf_atomic {
-  hf_ctx *ctx = nullptr;
-  AtomicSetup host;
-  AtomicDevice dev;
-  bool uploaded = false;
-};
-
 namespace hfg {
 thread_local std::string g_err;
 }
@@ -169,11 +162,22 @@ void atomic_upload(hf_atomic *h) {
   // Radial RI-K flops per coupled (output block, L): 4 r Ni^3 inside each
   // element and 2 Ni Nj (Ni + Nj) per element pair.
   d.h_radL.assign(nL, 0.0);
+  d.h_radL_in.assign(nL, 0.0);
+  d.h_ni = S.rad.ni;
+  {
+    std::vector<int> efi, efo;
+    for (int e = 0; e < Nel; ++e)
+      for (int f = 0; f < Nel; ++f) (e == f ? efi : efo).push_back(e * Nel + f);
+    if (efo.empty()) efo.push_back(0);
+    d.ef_in.upload(efi, s);
+    d.ef_off.upload(efo, s);
+  }
   for (int L = 0; L < nL; ++L) {
     double f = 0.0;
     for (int e = 0; e < Nel; ++e) {
       const double n = S.rad.ni[e];
       f += 4.0 * S.chol[(size_t)L * Nel + e].c * n * n * n;
+      d.h_radL_in[L] += 4.0 * S.chol[(size_t)L * Nel + e].c * n * n * n;
       for (int g = 0; g < Nel; ++g)
         if (g != e) {
           const double m = S.rad.ni[g];

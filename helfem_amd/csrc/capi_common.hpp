@@ -13,6 +13,21 @@ struct hf_ctx {
   hfg::KernelTimer timer;
 };
 
+struct hf_atomic {
+  hf_ctx *ctx = nullptr;
+  hfg::AtomicSetup host;
+  hfg::AtomicDevice dev;
+  bool uploaded = false;
+};
+
+struct hf_diatomic {
+  hf_ctx *ctx = nullptr;
+  hfg::DiatomicSetup host;
+  hfg::DiatomicDevice dev;
+  bool absm = false;
+  bool uploaded = false;
+};
+
 namespace hfg {
 
 extern thread_local std::string g_err;

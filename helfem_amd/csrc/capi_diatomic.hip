@@ -10,14 +10,6 @@
 
 using namespace hfg;
 
-struct hf_diatomic {
-  hf_ctx *ctx = nullptr;
-  DiatomicSetup host;
-  DiatomicDevice dev;
-  bool absm = false;
-  bool uploaded = false;
-};
-
 namespace {
 
 // Per (j, i, L) Gaunt couplings with M = mj - mi: g0 = mod_coeff (cosh^0
@@ -171,20 +163,28 @@ void diatomic_upload(hf_diatomic *h) {
 
   // radial RI-K flops per coupled (output block, (L,|M|)): 12 r n^3 inside
   // each element, 2 (4 ni nj^2 + 2 ni^2 nj) per element pair
-  d.h_rad_ilm.assign(std::max(nlm, 1), 0.0);
+  d.h_rad_in.assign(std::max(nlm, 1), 0.0);
+  d.h_rad_off.assign(std::max(nlm, 1), 0.0);
   for (int q = 0; q < nlm; ++q) {
-    double f = 0.0;
+    double fi = 0.0, fo = 0.0;
     for (int e = 0; e < Nel; ++e) {
       const double n = S.rad.ni[e];
-      f += 12.0 * S.cdB[(size_t)q * Nel + e].c * n * n * n;
+      fi += 12.0 * S.cdB[(size_t)q * Nel + e].c * n * n * n;
       for (int g = 0; g < Nel; ++g)
         if (g != e) {
           const double m = S.rad.ni[g];
-          f += 2.0 * (4.0 * n * m * m + 2.0 * n * n * m);
+          fo += 2.0 * (4.0 * n * m * m + 2.0 * n * n * m);
         }
     }
-    d.h_rad_ilm[q] = f;
+    d.h_rad_in[q] = fi;
+    d.h_rad_off[q] = fo;
   }
+  std::vector<int> efi, efo;
+  for (int e = 0; e < Nel; ++e)
+    for (int f = 0; f < Nel; ++f) (e == f ? efi : efo).push_back(e * Nel + f);
+  if (efo.empty()) efo.push_back(0);
+  d.ef_in.upload(efi, s);
+  d.ef_off.upload(efo, s);
   // absm mirror pairs: K(j,k) = K(jm,km) for mj, mk < 0 (basis.cpp:2067-2086)
   std::vector<int> mir;
   if (h->absm) {
@@ -287,7 +287,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   }
   std::vector<int> act, lstart(1, 0), rilm, rstart(1, 0), til;
   std::vector<double> tf;
-  double frad = 0.0;
+  double frad_in = 0.0, frad_off = 0.0;
   int64_t nterm = 0;
   for (int jk = 0; jk < njk; ++jk) {
     const std::vector<Term> &v = per[jk];
@@ -302,7 +302,8 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
       }
       rilm.push_back(v[a].ilm);
       rstart.push_back((int)til.size());
-      frad += d.h_rad_ilm[v[a].ilm];
+      frad_in += d.h_rad_in[v[a].ilm];
+      frad_off += d.h_rad_off[v[a].ilm];
       a = b;
     }
     lstart.push_back((int)rilm.size());
@@ -311,8 +312,16 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.n_act = (int)act.size();
   d.n_runs = (int64_t)rilm.size();
   d.n_terms = nterm;
-  d.flops_k_radial = frad;
-  d.flops_k_angular = 8.0 * (double)nterm * (double)d.Nrad * (double)d.Nrad;
+  // Gaunt-weighted density sums: 4 channels x 2 flops per term and tile entry
+  double sum_in = 0.0, sum_all = 0.0;
+  for (int e = 0; e < S.Nel(); ++e) {
+    sum_in += (double)S.rad.ni[e] * S.rad.ni[e];
+    for (int f = 0; f < S.Nel(); ++f) sum_all += (double)S.rad.ni[e] * S.rad.ni[f];
+  }
+  d.flops_k_radial = frad_in + frad_off;
+  d.flops_k_angular = 8.0 * (double)nterm * sum_all;
+  d.flops_in = frad_in + 8.0 * (double)nterm * sum_in;
+  d.flops_off = frad_off + 8.0 * (double)nterm * (sum_all - sum_in);
   if (act.empty()) act.push_back(0);
   if (rilm.empty()) rilm.push_back(0);
   if (til.empty()) {

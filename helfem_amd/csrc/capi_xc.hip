@@ -1,0 +1,243 @@
+// C-ABI for the DFT grids (include/hfgpu.h): grid tables and the LDA
+// exchange-correlation matrix (kernels in xc_grid.hip).
+#include "capi_common.hpp"
+#include <cmath>
+#include <memory>
+
+using namespace hfg;
+
+struct hf_dftgrid {
+  hf_ctx *ctx = nullptr;
+  const hfg::AtomicSetup *atomic = nullptr;
+  const hfg::DiatomicSetup *diatomic = nullptr;
+  int lang = 0, mang = 0;
+  hfg::XCGridDevice dev;
+  bool uploaded = false;
+  int64_t Nbf() const { return atomic ? atomic->Nbf() : diatomic->Nbf(); }
+};
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Theta_a(x) = Y_a(arccos x, phi = 0) from std::sph_legendre (Condon-Shortley
+// phase; Y_l^-m = (-1)^m conj Y_l^m, spherical_harmonics.cpp:19-35).
+double theta_value(int l, int m, double x) {
+  const int ma = std::abs(m);
+  double y = std::sph_legendre((unsigned)l, (unsigned)ma, std::acos(x));
+  if (m < 0 && (ma & 1)) y = -y;
+  return y;
+}
+
+// Grid tables shared by both bases.  Atomic (dftgrid.cpp:470-576): R = B/r
+// at the radial Chebyshev points, weight w_cth 2pi/mang * h w_i * r_i^2 with
+// the compound angular rule of angular.cpp:20-72.  Pure-m diatomic
+// (dftgrid_purem.cpp:87-120): R = B at the mu points, weight
+// 2pi w_nu h w_i Rh^3 sinh(mu) (sinh^2 mu + sin^2 nu), one phi point.
+void xc_upload(hf_dftgrid *g) {
+  XCGridDevice &d = g->dev;
+  hipStream_t s = g->ctx->cur;
+  const RadialFE &rad = g->atomic ? g->atomic->rad : g->diatomic->rad;
+  const std::vector<int> &lv = g->atomic ? g->atomic->lval : g->diatomic->lval;
+  const std::vector<int> &mv = g->atomic ? g->atomic->mval : g->diatomic->mval;
+  const int Nel = rad.Nel(), Na = (int)lv.size(), nrad = rad.nquad;
+  need(nrad >= 1 && nrad <= 1024, "radial quadrature size out of range");
+  d.Nel = Nel;
+  d.Nrad = rad.Nrad();
+  d.Nang = Na;
+  d.Nbf = g->Nbf();
+  d.nrad = nrad;
+  d.lang = g->lang;
+  d.mang = g->mang;
+  int mmax = 0;
+  for (int m : mv) mmax = std::max(mmax, std::abs(m));
+  d.dmax = g->atomic ? 2 * mmax : 0;
+  const int nd = 2 * d.dmax + 1;
+  // shell pairs within reach of the phi grid, numbered row by row
+  std::vector<int> rstart(Na + 1, 0), rb, rp;
+  std::vector<std::vector<int>> byd_a(nd), byd_b(nd), byd_p(nd);
+  int np = 0;
+  for (int a = 0; a < Na; ++a) {
+    for (int b = 0; b < Na; ++b) {
+      const int dd = mv[a] - mv[b];
+      if (std::abs(dd) > d.dmax) continue;
+      rb.push_back(b);
+      rp.push_back(np);
+      byd_a[dd + d.dmax].push_back(a);
+      byd_b[dd + d.dmax].push_back(b);
+      byd_p[dd + d.dmax].push_back(np);
+      ++np;
+    }
+    rstart[a + 1] = (int)rb.size();
+  }
+  d.npair = np;
+  std::vector<int> dstart(nd + 1, 0), da, db, dp;
+  for (int dd = 0; dd < nd; ++dd) {
+    da.insert(da.end(), byd_a[dd].begin(), byd_a[dd].end());
+    db.insert(db.end(), byd_b[dd].begin(), byd_b[dd].end());
+    dp.insert(dp.end(), byd_p[dd].begin(), byd_p[dd].end());
+    dstart[dd + 1] = (int)da.size();
+  }
+  const size_t lds = sizeof(double) * ((size_t)np + (size_t)g->lang * nd + (size_t)g->lang * g->mang);
+  if (lds > 64 * 1024)
+    throw Error(HF_EINVAL, "angular basis and DFT grid too large for one workgroup's LDS (need shell pairs + "
+                           "lang*(4 mmax + 1) + lang*mang <= 8192)");
+  if (sizeof(double) * ((size_t)nrad * TILE + 4 * TILE2) > 64 * 1024) throw Error(HF_EINVAL, "radial quadrature too large");
+  std::vector<double> xq, wq, xt, wt;
+  gauss_chebyshev2(nrad, xq, wq);
+  gauss_chebyshev2(g->lang, xt, wt);
+  const double dphi = 2.0 * kPi / g->mang;
+  std::vector<double> R((size_t)Nel * nrad * TILE, 0.0), wpt((size_t)Nel * nrad * g->lang);
+  std::vector<double> th((size_t)g->lang * Na), cs((size_t)nd * g->mang);
+  for (int t = 0; t < g->lang; ++t)
+    for (int a = 0; a < Na; ++a) th[(size_t)t * Na + a] = theta_value(lv[a], mv[a], xt[t]);
+  for (int dd = 0; dd < nd; ++dd)
+    for (int k = 0; k < g->mang; ++k) cs[(size_t)dd * g->mang + k] = std::cos(double(dd - d.dmax) * (k * dphi));
+  for (int e = 0; e < Nel; ++e) {
+    const Blk v = g->atomic ? rad.rvalues(xq, e) : rad.values(0, xq, e);
+    const double h = rad.half_len(e), mid = rad.mid(e);
+    for (int i = 0; i < nrad; ++i) {
+      for (int c = 0; c < v.c; ++c) R[((size_t)e * nrad + i) * TILE + c] = v(i, c);
+      const double x = mid + h * xq[i], wr = h * wq[i];
+      double *w = &wpt[((size_t)e * nrad + i) * g->lang];
+      if (g->atomic) {
+        for (int t = 0; t < g->lang; ++t) w[t] = (wt[t] * dphi) * wr * std::pow(x, 2);
+      } else {
+        const double shmu = std::sinh(x), Rh3 = std::pow(g->diatomic->Rh, 3);
+        for (int t = 0; t < g->lang; ++t) {
+          const double sth = std::sqrt(std::max((1.0 - xt[t]) * (1.0 + xt[t]), 0.0));
+          w[t] = 2.0 * kPi * wt[t] * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
+        }
+      }
+    }
+  }
+  std::vector<int> skip(Na, 0);
+  std::vector<long long> poff(Na + 1, 0);
+  for (int a = 0; a < Na; ++a) {
+    skip[a] = g->atomic ? 0 : g->diatomic->skip(a);
+    poff[a] = g->atomic ? (long long)a * d.Nrad : (long long)g->diatomic->poff[a];
+  }
+  d.first.upload(rad.first, s);
+  d.ni.upload(rad.ni, s);
+  d.skip.upload(skip, s);
+  d.poff.upload(poff, s);
+  d.R.upload(R, s);
+  d.wpt.upload(wpt, s);
+  d.th.upload(th, s);
+  d.cs.upload(cs, s);
+  d.rstart.upload(rstart, s);
+  d.rb.upload(rb, s);
+  d.rp.upload(rp, s);
+  d.dstart.upload(dstart, s);
+  d.da.upload(da, s);
+  d.db.upload(db, s);
+  d.dp.upload(dp, s);
+  d.D.alloc((size_t)Nel * np * nrad);
+  d.W.alloc((size_t)Nel * np * nrad);
+  d.part.alloc((size_t)2 * Nel * nrad);
+  d.h_part.alloc((size_t)2 * Nel * nrad);
+  hip_check(hipStreamSynchronize(s), "dft grid upload");
+  g->uploaded = true;
+}
+
+void check_functionals(int x_func, int c_func) {
+  if (x_func != 0 && x_func != 1) throw Error(HF_EINVAL, "unsupported exchange functional (0 or 1 = lda_x)");
+  if (c_func != 0 && c_func != 7) throw Error(HF_EINVAL, "unsupported correlation functional (0 or 7 = lda_c_vwn)");
+}
+
+void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H, int64_t ldh,
+              double thr, double *out) {
+  hipStream_t s = g->ctx->cur;
+  XCGridDevice &d = g->dev;
+  xc_grid_launch(d, x_func, c_func, thr, P, ldp, H, ldh, s, &g->ctx->timer);
+  hip_check(hipMemcpyAsync(d.h_part.p, d.part.p, sizeof(double) * 2 * d.Nel * d.nrad, hipMemcpyDeviceToHost, s),
+            "copy XC partial sums");
+  hip_check(hipStreamSynchronize(s), "XC sync");
+  double exc = 0.0, nel = 0.0;
+  for (int q = 0; q < d.Nel * d.nrad; ++q) {
+    exc += d.h_part.p[2 * q];
+    nel += d.h_part.p[2 * q + 1];
+  }
+  out[0] = exc;
+  out[1] = nel;
+  out[2] = 0.0;  // kinetic energy density integral: zero for LDA
+}
+
+hf_dftgrid *ready(hf_dftgrid *g) {
+  need(g != nullptr, "NULL grid");
+  need(g->ctx != nullptr, "basis was created without a device context");
+  return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hf_atomic_dftgrid_create(hf_atomic *h, int lang, int mang, hf_dftgrid **out) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    need(lang >= 1 && mang >= 1, "grid orders must be positive");
+    std::unique_ptr<hf_dftgrid> g(new hf_dftgrid);
+    g->ctx = h->ctx;
+    g->atomic = &h->host;
+    g->lang = lang;
+    g->mang = mang;
+    *out = g.release();
+  });
+}
+
+int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    need(lang >= 1, "grid order must be positive");
+    std::unique_ptr<hf_dftgrid> g(new hf_dftgrid);
+    g->ctx = h->ctx;
+    g->diatomic = &h->host;
+    g->lang = lang;
+    g->mang = 1;
+    *out = g.release();
+  });
+}
+
+int hf_dftgrid_destroy(hf_dftgrid *g) {
+  return guarded([&] {
+    if (!g) return;
+    if (g->ctx) {
+      DeviceGuard dg(g->ctx->device);
+      (void)hipStreamSynchronize(g->ctx->cur);
+    }
+    delete g;
+  });
+}
+
+int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double *dP, int64_t ldp, double *dH,
+                            int64_t ldh, double thr, double out[3]) {
+  return guarded([&] {
+    ready(g);
+    need(out != nullptr, "out is NULL");
+    check_functionals(x_func, c_func);
+    DeviceGuard dg(g->ctx->device);
+    const int64_t n = g->Nbf();
+    check_matrix(dP, ldp, n, "P");
+    check_matrix(dH, ldh, n, "H");
+    if (!g->uploaded) xc_upload(g);
+    eval_fxc(g, x_func, c_func, dP, ldp, dH, ldh, thr, out);
+  });
+}
+
+int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H,
+                        int64_t ldh, double thr, double out[3]) {
+  return guarded([&] {
+    ready(g);
+    need(out != nullptr, "out is NULL");
+    check_functionals(x_func, c_func);
+    DeviceGuard dg(g->ctx->device);
+    if (!g->uploaded) xc_upload(g);
+    host_roundtrip(g->ctx->cur, g->Nbf(), g->dev.hostP, g->dev.hostOut, P, ldp, H, ldh,
+                   [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+                     eval_fxc(g, x_func, c_func, dp, lp, dout, lo, thr, out);
+                   });
+  });
+}
+
+}  // extern "C"

@@ -192,6 +192,40 @@ __global__ void __launch_bounds__(NT) k_dia_block_norms(const double *P, int64_t
   if (threadIdx.x == 0) bnorm[il] = sqrt(s);
 }
 
+// Two products sharing the Y operand, issued interleaved so the two
+// accumulation chains overlap: da += Xa Y, db += Xb Y (16x16x16 each).
+__device__ inline void mfma_pair(const double xa[4], const double xb[4], const double ys[4], v4d &da, v4d &db) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    da = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], ys[s], da, 0, 0, 0);
+    db = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[s], ys[s], db, 0, 0, 0);
+  }
+}
+
+// da += Xa Ya, db += Xb Yb with Ya, Yb result tiles, interleaved.
+__device__ inline void mfma_pair2(const double xa[4], v4d ya, v4d &da, const double xb[4], v4d yb, v4d &db) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    da = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], ya[s], da, 0, 0, 0);
+    db = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[s], yb[s], db, 0, 0, 0);
+  }
+}
+
+// Cholesky channel tiles of vector p in both operand layouts:
+// y = M(c, k) (M' as Y operand), x = M(k, c) (M' as X operand), k = g + 4s.
+__device__ inline void load_chol(const double *M, int p, int g, int c, double y0[4], double y2[4], double x0[4],
+                                 double x2[4]) {
+  const double *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = g + 4 * s;
+    y0[s] = M0[c + TILE * kk];
+    y2[s] = M2[c + TILE * kk];
+    x0[s] = M0[kk + TILE * c];
+    x2[s] = M2[kk + TILE * c];
+  }
+}
+
 struct DKArgs {
   const double *P;
   int64_t ldp;
@@ -204,23 +238,26 @@ struct DKArgs {
   const int *chol_off, *chol_rank;
   const int *act_jk, *act_lstart, *lrun_ilm, *lrun_tstart, *term_il;
   const double *term_f;
+  const int *efl;  // element pairs e*Nel+f of this launch
+  int n_ef;
 };
 
 // One workgroup per (active output block (j,k), element pair (e,f)), the
 // four waves splitting the block's coupled (L,|M|) runs.  Workgroups sharing
 // (e,f) are mapped to one XCD so the (e,f) slice of P stays in its L2.
+template <bool INSIDE>
 __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
   const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
-  const int ef = (loc / A.n_act) * 8 + xcd;
+  const int efi = (loc / A.n_act) * 8 + xcd;
   const int act = loc - (loc / A.n_act) * A.n_act;
-  if (ef >= A.Nel * A.Nel) return;
-  const int e = ef / A.Nel, f = ef - e * A.Nel;
+  if (efi >= A.n_ef) return;
+  const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
   __shared__ double red[4 * TILE2];
-  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
   const bool rowok = c < ni;
   bool colok[4];
 #pragma unroll
@@ -250,7 +287,7 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
       }
     }
     const double *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
-    if (e != f) {
+    if (!INSIDE) {
       const double *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
       const bool up = e > f;
       // row side: Q of e if e > f else P of e; column side the other kind
@@ -265,14 +302,11 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
         b0[s] = j0[kk];
         b2[s] = j2[kk];
       }
-      v4d y = {0.0, 0.0, 0.0, 0.0};
-      y = mfma_16x16x16(r00, b0, y);  // R00 j0'
-      y = mfma_16x16x16(r02, b2, y);  // + R02 j2'
-      acc = mfma_16x16x16(a0, y, acc);
-      y = v4d{0.0, 0.0, 0.0, 0.0};
-      y = mfma_16x16x16(r20, b0, y);  // R20 j0'
-      y = mfma_16x16x16(r22, b2, y);  // + R22 j2'
-      acc = mfma_16x16x16(a2, y, acc);
+      // two independent chains: ya = R00 j0' + R02 j2', yb = R20 j0' + R22 j2'
+      v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
+      mfma_pair(r00, r20, b0, ya, yb);
+      mfma_pair(r02, r22, b2, ya, yb);
+      mfma_pair2(a0, ya, acc, a2, yb, acc2);  // acc += i0 ya, acc2 += i2 yb
     } else {
       const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
       const double *M = A.chol + (size_t)off * 2 * TILE2;
@@ -281,32 +315,40 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
         r02[s] = -r02[s];
         r20[s] = -r20[s];
       }
+      // M0', M2' as Y operands (y0, y2) and as X operands (x0, x2); the next
+      // vector's tiles are loaded while the current one is on the matrix cores
+      double y0[4], y2[4], x0[4], x2[4];
+      load_chol(M, 0, g, c, y0, y2, x0, x2);
+      double sg = rank > 0 ? A.sigma[off] : 0.0;
       for (int p = 0; p < rank; ++p) {
-        const double *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
-        const double sg = A.sigma[off + p];
-        double y0[4], y2[4], x0[4], x2[4];  // M' as Y operand and as X operand
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int kk = g + 4 * s;
-          y0[s] = M0[c + TILE * kk];
-          y2[s] = M2[c + TILE * kk];
-          x0[s] = M0[kk + TILE * c];
-          x2[s] = M2[kk + TILE * c];
+        double ny0[4], ny2[4], nx0[4], nx2[4];
+        double nsg = 0.0;
+        if (p + 1 < rank) {
+          load_chol(M, p + 1, g, c, ny0, ny2, nx0, nx2);
+          nsg = A.sigma[off + p + 1];
         }
-        v4d y = {0.0, 0.0, 0.0, 0.0};
-        y = mfma_16x16x16(r00, y0, y);  // R00 M0'
-        y = mfma_16x16x16(r02, y2, y);  // - R02 M2'
-        y *= sg;
-        acc = mfma_16x16x16(x0, y, acc);
-        y = v4d{0.0, 0.0, 0.0, 0.0};
-        y = mfma_16x16x16(r20, y0, y);  // - R20 M0'
-        y = mfma_16x16x16(r22, y2, y);  // + R22 M2'
-        y *= sg;
-        acc = mfma_16x16x16(x2, y, acc);
+        // ya = R00 M0' - R02 M2', yb = -R20 M0' + R22 M2' (signs folded into r02, r20)
+        v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
+        mfma_pair(r00, r20, y0, ya, yb);
+        mfma_pair(r02, r22, y2, ya, yb);
+        ya *= sg;
+        yb *= sg;
+        mfma_pair2(x0, ya, acc, x2, yb, acc2);  // acc += M0' ya, acc2 += M2' yb
+        if (p + 1 < rank) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            y0[s] = ny0[s];
+            y2[s] = ny2[s];
+            x0[s] = nx0[s];
+            x2[s] = nx2[s];
+          }
+          sg = nsg;
+        }
       }
     }
   }
   // combine the four waves' partial tiles (lane holds acc(g+4r, c))
+  acc += acc2;
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
   __syncthreads();
@@ -374,15 +416,24 @@ void diatomic_exchange_kernels(const DiatomicDevice &d, const double *P, int64_t
                                hipStream_t s, KernelTimer *timer) {
   fill_zero_launch(K, d.Nbf, d.Nbf, ldk, s);
   if (d.n_act == 0) return;
-  const int n_ef_pad = ((d.Nel * d.Nel + 7) / 8) * 8;
-  DKArgs a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.chol.p,
-           d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p, d.lrun_tstart.p,
-           d.term_il.p, d.term_f.p};
-  const int64_t grid = (int64_t)n_ef_pad * d.n_act;
-  if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-  {
-    TimedLaunch tl(timer, "diatomic_k", s, d.flops_k_radial + d.flops_k_angular);
-    hipLaunchKernelGGL(k_dia_k, dim3((unsigned)grid), dim3(NT), 0, s, a);
+  // two launches: element pairs e != f (disjoint P/Q factors) and the
+  // in-element tiles (Cholesky factor); boundary rows/cols shared between
+  // neighbouring tiles are accumulated atomically in both
+  for (int part = 0; part < 2; ++part) {
+    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+    if (n_ef == 0) continue;
+    const int n_ef_pad = ((n_ef + 7) / 8) * 8;
+    DKArgs a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.chol.p,
+             d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p, d.lrun_tstart.p,
+             d.term_il.p, d.term_f.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef};
+    const int64_t grid = (int64_t)n_ef_pad * d.n_act;
+    if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+    TimedLaunch tl(timer, part == 0 ? "diatomic_k_off" : "diatomic_k_in", s,
+                   part == 0 ? d.flops_off : d.flops_in);
+    if (part == 0)
+      hipLaunchKernelGGL(k_dia_k<false>, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_dia_k<true>, dim3((unsigned)grid), dim3(NT), 0, s, a);
     hip_check(hipGetLastError(), "diatomic_k");
   }
   if (d.n_mirror > 0) {

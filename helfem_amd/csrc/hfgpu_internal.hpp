@@ -147,7 +147,9 @@ struct AtomicDevice {
   // the host; the device holds the lists compacted to the density blocks
   // that survive the reference's norm screen (TwoDBasis.cpp:949-952).
   std::vector<int> h_act, h_tstart, h_til;
-  std::vector<double> h_tc, h_radL;
+  std::vector<double> h_tc, h_radL, h_radL_in;  // radial flops per coupled (block, L): all / in-element
+  std::vector<int> h_ni;
+  DevArray<int> ef_in, ef_off;            // element pairs e*Nel+f of the two exchange launches
   std::vector<unsigned char> screen_mask;  // mask the compacted lists were built for
   DevArray<int> act_jk;                   // active output blocks j*Nang+k
   DevArray<int> term_start;               // [n_act*N_L + 1]
@@ -161,6 +163,7 @@ struct AtomicDevice {
   int64_t n_terms = 0, n_pairs = 0;
   // algorithmic work per launch (flops), for the roofline report
   double flops_k_radial = 0.0, flops_k_angular = 0.0, flops_j = 0.0;
+  double flops_in = 0.0, flops_off = 0.0;
 };
 
 // Everything the diatomic J/K kernels read.  Matrices are in the pure basis
@@ -188,7 +191,8 @@ struct DiatomicDevice {
   std::vector<double> h_g0, h_g2;         // [j][i][L]: gaunt_mod / gaunt3 (M = mj - mi)
   int gL = 0;                             // L extent of h_g0/h_g2
   std::vector<unsigned char> screen_mask;
-  std::vector<double> h_rad_ilm;          // radial flops per coupled (jk, ilm)
+  std::vector<double> h_rad_in, h_rad_off; // radial flops per coupled (jk, ilm): in / across elements
+  DevArray<int> ef_in, ef_off;            // element pairs e*Nel+f of the two exchange launches
   DevArray<int> act_jk, act_lstart;       // active output blocks; ilm runs per block
   DevArray<int> lrun_ilm, lrun_tstart;    // per run: ilm and first term
   DevArray<int> term_il;
@@ -200,6 +204,31 @@ struct DiatomicDevice {
   DevArray<double> hostP, hostOut;
   int64_t n_terms = 0, n_runs = 0, n_pairs = 0;
   double flops_k_radial = 0.0, flops_k_angular = 0.0;
+  double flops_in = 0.0, flops_off = 0.0;  // split of the two above between the launches
+};
+
+// DFT grid of a two-dimensional basis, evaluated by sum factorisation:
+// basis function (a, r) at point (i, t, k) is R_r(i) Theta_a(t) e^(i m_a phi_k).
+// Atomic grid: R = B/r at the radial Chebyshev points, Theta_a = Y_a(theta, 0)
+// on the cos(theta) rule, mang even phi points.  Pure-m diatomic grid: R = B
+// at the mu points, Theta_a = Y_a(nu, 0), one phi "point" (the phi integral is
+// analytic), only shell pairs with m_a = m_b.  Shell pairs (a, b) with
+// |m_a - m_b| <= dmax are numbered p = 0..npair-1.
+struct XCGridDevice {
+  int Nel = 0, Nrad = 0, Nang = 0, nrad = 0, lang = 0, mang = 0, dmax = 0, npair = 0;
+  int64_t Nbf = 0;
+  DevArray<int> first, ni, skip;
+  DevArray<long long> poff;
+  DevArray<double> R;        // [Nel][nrad][TILE]
+  DevArray<double> wpt;      // [Nel][nrad][lang] quadrature weight (phi weight included)
+  DevArray<double> th;       // [lang][Nang]
+  DevArray<double> cs;       // [2 dmax + 1][mang] cos(d phi_k)
+  DevArray<int> rstart, rb, rp;       // pairs of row a: b and pair index
+  DevArray<int> dstart, da, db, dp;   // pairs bucketed by d = m_a - m_b + dmax
+  DevArray<double> D, W;     // [Nel][npair][nrad]
+  DevArray<double> part;     // [Nel*nrad][2]: Exc and Nel partial sums
+  PinnedArray<double> h_part;
+  DevArray<double> hostP, hostOut;
 };
 
 // launchers (atomic_jk.hip)
@@ -208,6 +237,10 @@ void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, 
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer);
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
+
+// launchers (xc_grid.hip); exc/nel summed on the host from h_part
+void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P, int64_t ldp, double *H,
+                      int64_t ldh, hipStream_t s, KernelTimer *timer);
 
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,

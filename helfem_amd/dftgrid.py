@@ -1,0 +1,80 @@
+"""Python mirror of HelFEM's DFT grids: the atomic grid
+(helfem::atomic::dftgrid::DFTGrid, src/atomic/dftgrid.h:140-173) and the
+pure-m diatomic grid (helfem::diatomic::dftgrid_purem::PureMDFTGrid,
+src/diatomic/dftgrid_purem.h), chosen by the basis type.
+
+`DFTGrid(basis, lang, mang).eval_Fxc(x_func, c_func, P)` returns the
+restricted exchange-correlation matrix with Exc, the integrated electron
+count and the kinetic-energy-density integral, like DFTGrid::eval_Fxc
+(dftgrid.cpp:591-653).  Functionals are libxc ids; parse_method turns the
+driver's --method string (e.g. "lda_x-lda_c_vwn") into them.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .atomic import _f64_fortran
+
+FUNCTIONALS = {"none": 0, "lda_x": 1, "lda_c_vwn": 7}
+
+
+def parse_method(method: str):
+    """parse_xc_func (src/general/dftfuncs.cpp:94-117): "x-c" or a single name."""
+    x, _, c = method.partition("-")
+    try:
+        return FUNCTIONALS[x], FUNCTIONALS[c] if c else 0
+    except KeyError as err:
+        raise ValueError(f"unsupported functional in {method!r}; supported: {sorted(FUNCTIONALS)}") from err
+
+
+class DFTGrid:
+    """DFT quadrature on the GPU (hf_dftgrid): atomic grid for an AtomicBasis,
+    pure-m grid (mang unused) for a DiatomicBasis."""
+
+    def __init__(self, basis, lang: int | None = None, mang: int | None = None):
+        from .diatomic import DiatomicBasis
+        lmax = int(np.max(basis.lval))
+        mmax = int(np.max(np.abs(basis.mval)))
+        self.lang = 4 * lmax + 12 if lang is None else int(lang)
+        self.basis = basis  # the grid points into the basis handle
+        h = ctypes.c_void_p()
+        if isinstance(basis, DiatomicBasis):
+            self.mang = 1
+            check(lib().hf_diatomic_dftgrid_create(basis._h, self.lang, ctypes.byref(h)))
+        else:
+            self.mang = 4 * mmax + 12 if mang is None else int(mang)
+            check(lib().hf_atomic_dftgrid_create(basis._h, self.lang, self.mang, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._LIB is not None:
+            _lib._LIB.hf_dftgrid_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def eval_Fxc(self, x_func: int, c_func: int, P, thr: float = 1e-12, out=None):
+        """(H, Exc, Nel, Ekin) for the total density P (numpy or float64 CUDA tensor)."""
+        n = self.basis.Nbf
+        res = np.zeros(3)
+        if isinstance(P, np.ndarray):
+            P = _f64_fortran(P)
+            if P.shape != (n, n):
+                raise ValueError(f"P must be {n}x{n}")
+            H = np.zeros((n, n), order="F")
+            check(lib().hf_dftgrid_eval_fxc(self._h, x_func, c_func, P.ctypes.data, n, H.ctypes.data, n, thr,
+                                            res.ctypes.data))
+            return H, float(res[0]), float(res[1]), float(res[2])
+        import torch
+        if not (isinstance(P, torch.Tensor) and P.is_cuda and P.dtype == torch.float64):
+            raise TypeError("P must be a numpy array or a float64 CUDA tensor")
+        if P.shape != (n, n) or not P.is_contiguous():
+            raise ValueError(f"P must be a contiguous {n}x{n} tensor")
+        self.basis.ctx.bind_torch_stream()
+        H = torch.empty_like(P) if out is None else out
+        check(lib().hf_dftgrid_eval_fxc_dev(self._h, x_func, c_func, P.data_ptr(), n, H.data_ptr(), n, thr,
+                                            res.ctypes.data))
+        return H, float(res[0]), float(res[1]), float(res[2])

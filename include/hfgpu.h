@@ -37,6 +37,7 @@ extern "C" {
 typedef struct hf_ctx hf_ctx;
 typedef struct hf_atomic hf_atomic;
 typedef struct hf_diatomic hf_diatomic;
+typedef struct hf_dftgrid hf_dftgrid;
 
 /* Thread-local text of the last error. */
 const char *hf_last_error(void);
@@ -128,6 +129,39 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
 /* Device-pointer forms of the two above, asynchronous on the context stream. */
 int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+
+/* ------------------------------------------------------- atomic DFT grid */
+
+/* DFT quadrature of an atomic basis: the basis' radial Chebyshev points in
+ * every element x `lang` cos(theta) Chebyshev nodes x `mang` even phi points.
+ * Replaces the constructor helfem::atomic::dftgrid::DFTGrid(basis, lang,
+ * mang) (src/atomic/dftgrid.h:140-173; the driver's defaults are lang =
+ * 4 lmax + 12, mang = 4 mmax + 12, atomic/main.cpp:329-330).  The grid keeps
+ * a pointer to h: destroy it before the basis. */
+int hf_atomic_dftgrid_create(hf_atomic *h, int lang, int mang, hf_dftgrid **out);
+int hf_dftgrid_destroy(hf_dftgrid *g);
+/* Pure-m DFT quadrature of a diatomic basis: the mu Chebyshev points of every
+ * element x `lang` nu Chebyshev nodes, the phi integral done analytically
+ * (2 pi), for densities without m mixing.  Replaces the constructor
+ * helfem::diatomic::dftgrid_purem::PureMDFTGrid(basis, lang)
+ * (src/diatomic/dftgrid_purem.h; the driver's default is lang = 4 lmax + 12,
+ * diatomic/main.cpp:316).  Matrices are in the pure basis.  Destroy the grid
+ * before the basis. */
+int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out);
+/* Restricted exchange-correlation matrix H (Nbf x Nbf, overwritten) of the
+ * total density P, with out[0] = Exc, out[1] = Nel = integral of the
+ * density, out[2] = kinetic energy density integral (0 for LDA).  x_func /
+ * c_func are libxc functional ids: 0 (none), 1 (lda_x) and 7 (lda_c_vwn,
+ * VWN5); densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
+ * (src/atomic/dftgrid.h:156, dftgrid.cpp:591-653) and
+ * PureMDFTGrid::eval_Fxc (src/diatomic/dftgrid_purem.h:154,
+ * dftgrid_purem.cpp:675-703) for LDA functionals without external
+ * parameters. */
+int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H,
+                        int64_t ldh, double thr, double out[3]);
+/* Device-pointer form; synchronises the context stream to return out. */
+int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double *dP, int64_t ldp, double *dH,
+                            int64_t ldh, double thr, double out[3]);
 
 /* --------------------------------------------------------- diatomic basis */
 

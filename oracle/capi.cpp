@@ -257,6 +257,16 @@ int orc_atomic_xc(void *h, int lang, int mang, int xf, int cf, const double *P, 
     put(Hm, H);
   });
 }
+// Diatomic pure-m DFT grid: H (Nbf x Nbf, pure) and out = (Exc, Nel)
+int orc_diatomic_xc(void *h, int lang, int xf, int cf, const double *P, double thr, double *H, double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    DiatomicXC g(*b, lang);
+    Mat Hm;
+    g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    put(Hm, H);
+  });
+}
 // lm map entry i: out = (L, |M|); LM map similarly (L, M)
 int orc_diatomic_maps(void *h, int *lm, int *LM) {
   DiatomicBasis *b = (DiatomicBasis *)h;

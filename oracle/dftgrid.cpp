@@ -10,6 +10,8 @@
 // potential matrix by (a.*wv) a' + (b.*wv) b'.
 #include "orc.hpp"
 #include "dftgrid.hpp"
+#include "diatomic.hpp"
+#include <algorithm>
 #include <complex>
 
 namespace orc {
@@ -151,6 +153,84 @@ void AtomicXC::eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, doubl
     for (int v = 0; v < nbf; ++v)
       for (int u = 0; u < nbf; ++u) H(idx[u], idx[v]) += Ha(u, v) + Hb(u, v);
   }
+}
+
+DiatomicXC::DiatomicXC(const DiatomicBasis &b_, int lang_) : b(b_), lang(lang_) { chebyshev(lang, cth, wang); }
+
+// PureMDFTGrid::eval_Fxc (restricted), src/diatomic/dftgrid_purem.cpp:675-703,
+// with the worker's compute_bf (:87-199), update_density (:212-260) and
+// eval_Fxc (:468-500): per element and radial point, real per-m basis values
+// B_j(mu) Y_l^m(nu, 0) on the nu rule, the phi integral done analytically.
+void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pin, Mat &Hout, double &Exc, double &Nel, double thr) const {
+  const RadialBasis &rad = b.radial;
+  const Mat P = b.expand_boundaries(Pin);
+  const int Na = b.Nang(), Nr = rad.Nbf(), nang = (int)cth.size();
+  Mat H(b.Ndummy(), b.Ndummy());
+  Exc = 0.0;
+  Nel = 0.0;
+  std::vector<int> mlist(b.mval.begin(), b.mval.end());
+  std::sort(mlist.begin(), mlist.end());
+  mlist.erase(std::unique(mlist.begin(), mlist.end()), mlist.end());
+  const double Rh3 = std::pow(b.Rhalf, 3);
+  for (int iel = 0; iel < rad.Nel(); ++iel) {
+    const Mat Bv = rad.eval_dnf(rad.xq, 0, iel);  // (nq x Ni)
+    const std::vector<double> mu = rad.eval_coord(rad.xq, iel);
+    const double h = rad.scaling_factor(iel);
+    const int Ni = Bv.c;
+    for (int ir = 0; ir < (int)mu.size(); ++ir) {
+      const double wr = h * rad.wq[ir], shmu = std::sinh(mu[ir]);
+      std::vector<double> wtot(nang);
+      for (int ia = 0; ia < nang; ++ia) {
+        const double sth = std::sqrt(std::max((1.0 - cth[ia]) * (1.0 + cth[ia]), 0.0));
+        wtot[ia] = 2.0 * kPi * wang[ia] * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
+      }
+      std::vector<std::vector<int>> idx(mlist.size());
+      std::vector<Mat> f(mlist.size());
+      std::vector<double> rho(nang, 0.0);
+      for (size_t im = 0; im < mlist.size(); ++im) {
+        for (int a = 0; a < Na; ++a)
+          if (b.mval[a] == mlist[im])
+            for (int j = 0; j < Ni; ++j) idx[im].push_back(a * Nr + rad.first[iel] + j);
+        const int nbf = (int)idx[im].size();
+        f[im] = Mat(nbf, nang);
+        int row = 0;
+        for (int a = 0; a < Na; ++a) {
+          if (b.mval[a] != mlist[im]) continue;
+          for (int ia = 0; ia < nang; ++ia) {
+            const double Y = std::real(spherical_harmonic(b.lval[a], b.mval[a], cth[ia], 0.0));
+            for (int j = 0; j < Ni; ++j) f[im](row + j, ia) = Y * Bv(ir, j);
+          }
+          row += Ni;
+        }
+        Mat Pb(nbf, nbf);
+        for (int v = 0; v < nbf; ++v)
+          for (int u = 0; u < nbf; ++u) Pb(u, v) = P(idx[im][u], idx[im][v]);
+        const Mat Pv = matmul(Pb, f[im]);
+        for (int ia = 0; ia < nang; ++ia) {
+          double s = 0.0;
+          for (int u = 0; u < nbf; ++u) s += Pv(u, ia) * f[im](u, ia);
+          rho[ia] += s;
+        }
+      }
+      std::vector<double> wv(nang);
+      for (int ia = 0; ia < nang; ++ia) {
+        double exc, v;
+        lda_unpolarized(xf, cf, rho[ia], thr, exc, v);
+        Nel += wtot[ia] * rho[ia];
+        Exc += wtot[ia] * exc * rho[ia];
+        wv[ia] = v * wtot[ia];
+      }
+      for (size_t im = 0; im < mlist.size(); ++im) {
+        Mat fw = f[im];
+        for (int ia = 0; ia < nang; ++ia)
+          for (int u = 0; u < fw.r; ++u) fw(u, ia) *= wv[ia];
+        const Mat Hm = matmul_nt(fw, f[im]);
+        for (int v = 0; v < Hm.c; ++v)
+          for (int u = 0; u < Hm.r; ++u) H(idx[im][u], idx[im][v]) += Hm(u, v);
+      }
+    }
+  }
+  Hout = b.remove_boundaries(H);
 }
 
 }  // namespace orc

@@ -19,4 +19,14 @@ struct AtomicXC {
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
 };
 
+struct DiatomicBasis;
+// diatomic::dftgrid_purem::PureMDFTGrid (src/diatomic/dftgrid_purem.h), LDA restricted
+struct DiatomicXC {
+  const DiatomicBasis &b;
+  int lang;
+  std::vector<double> cth, wang;
+  DiatomicXC(const DiatomicBasis &b, int lang);
+  void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
+};
+
 }  // namespace orc

@@ -49,6 +49,8 @@ def lib():
                               ctypes.c_void_p, ctypes.c_void_p]
         L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_diatomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_create.restype = ctypes.c_void_p
         L.orc_diatomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -342,6 +344,17 @@ class DiatomicBasis:
         K = np.zeros((self.Nbf, self.Nbf), order="F")
         _check(lib().orc_diatomic_exchange_blocks(self._h, _ptr(P), _ptr(K), jk.size, _ptr(jk)))
         return K
+
+    def xc(self, P, lang=None, x_func=1, c_func=7, thr=1e-12):
+        """PureMDFTGrid::eval_Fxc (src/diatomic/dftgrid_purem.cpp:675-703), LDA,
+        restricted; lang defaults to 4 lmax + 12 (diatomic/main.cpp:316).
+        Returns (Vxc, Exc, Nel) in the pure basis."""
+        lang = 4 * int(np.max(self.lval)) + 12 if lang is None else lang
+        P = np.asfortranarray(P, dtype=np.float64)
+        H = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_diatomic_xc(self._h, lang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        return H, float(out[0]), float(out[1])
 
     def maps(self):
         lm = np.zeros(2 * self.n_lm, np.int32)

@@ -92,3 +92,14 @@ def test_spherical_density_gives_m_degenerate_potential():
         for c in range(b.Nang):
             if a != c:
                 assert np.max(np.abs(blocks[a * b.Nang + c])) < 1e-12 * scale
+
+
+def test_purem_lda_ne_dummy_centre():
+    """diatomic-Ne-dummy-lda (tests/refs/ci.json): Ne through the diatomic code
+    with a zero-charge centre at 1 bohr, lmax 13 (m=0) and 9 (|m|=1), 3 elements."""
+    b = O.DiatomicBasis(10, 0, 1.0, [13, 9], 3)
+    b.compute_tei()
+    r = O.scf_rhf(b, 10, xc=lambda P: b.xc(P)[:2])
+    assert abs(r["E"] - REF["diatomic-Ne-dummy-lda"]["total"]) < 1e-9
+    _, _, nel = b.xc(r["P"])
+    assert abs(nel - 10) < 1e-9
