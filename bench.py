@@ -82,6 +82,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="n2-hf-diatomic", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the timed full SCF solve")
     ap.add_argument("--cpu-sample-stride", type=int, default=0,
                     help="time every n-th exchange output block on the CPU and extrapolate "
                          "(0: 24 for atomic, 64 for diatomic workloads)")
@@ -239,7 +240,18 @@ def main():
         "kernels_ms": {k: (v[1] / max(v[0], 1)) for k, v in stats.items()},
         "kernels_tflops": {k: (v[2] / max(v[1], 1e-30) / 1e9) for k, v in stats.items() if v[2] > 0},
         "energies": {"coulomb": Ecoul, "exchange": Exx},
+        "exchange_work": hb.work(),
     }
+    if not args.no_scf:
+        from helfem_amd.scf import rhf
+        nel = int(round(nelec))
+        t = time.perf_counter()
+        r = rhf(hb, nel, conv=1e-9)
+        tsolve = time.perf_counter() - t
+        log(f"SCF: E = {r.E:.10f} in {r.iterations} iterations, {tsolve:.2f}s")
+        result["scf"] = {"solves_per_s": 1.0 / tsolve, "seconds": tsolve, "iterations": r.iterations,
+                         "energy": r.E, "convergence": "dE < 1e-9 Eh and orbital gradient < 1e-6",
+                         "guess": "core Hamiltonian", "precision": "FP64 throughout"}
     if not args.no_cpu_baseline:
         stride = args.cpu_sample_stride or (24 if kind == "atomic" else 64)
         result["cpu_baseline"] = cpu_baseline(cfg, P, hb.Nang * hb.Nang, stride)

@@ -43,11 +43,16 @@ SIGNATURES = {
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_atomic_exchange_fp32_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_dftgrid_create": (c_int, [c_vp, c_int, c_int, ctypes.POINTER(c_vp)]),
     "hf_diatomic_dftgrid_create": (c_int, [c_vp, c_int, ctypes.POINTER(c_vp)]),
     "hf_dftgrid_destroy": (c_int, [c_vp]),
     "hf_dftgrid_eval_fxc": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
     "hf_dftgrid_eval_fxc_dev": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
+    "hf_dftgrid_eval_fxc_pol": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64,
+                                        c_int, c_dbl, c_vp]),
+    "hf_dftgrid_eval_fxc_pol_dev": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64, c_dp,
+                                            c_i64, c_int, c_dbl, c_vp]),
     "hf_diatomic_create": (c_int, [c_vp, c_int, c_int, c_dbl, c_int, c_int, c_dp, c_int, c_vp, c_vp, c_int,
                                    ctypes.POINTER(c_vp)]),
     "hf_diatomic_destroy": (c_int, [c_vp]),
@@ -63,6 +68,7 @@ SIGNATURES = {
     "hf_diatomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_diatomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_diatomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_diatomic_exchange_fp32_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
 }
 
 _LIB = None

@@ -153,9 +153,17 @@ class AtomicBasis:
                                        for a in range(self.Nang) if self.mval[a] == m]))
         return out
 
+    def work(self):
+        """Work of the most recent exchange build (hf_atomic_work)."""
+        out = np.zeros(4)
+        check(lib().hf_atomic_work(self._h, out.ctypes.data))
+        return {"flops_angular": out[0], "flops_radial": out[1], "terms": int(out[2]), "blocks": int(out[3])}
+
     # --- Fock build ------------------------------------------------------
     def _apply(self, P, host_fn, dev_fn, out=None):
         if isinstance(P, np.ndarray):
+            if host_fn is None:
+                raise TypeError("this form takes a float64 CUDA tensor")
             P = _f64_fortran(P)
             if P.shape != (self.Nbf, self.Nbf):
                 raise ValueError(f"P must be {self.Nbf}x{self.Nbf}")
@@ -178,6 +186,13 @@ class AtomicBasis:
         """J(P), TwoDBasis.cpp:774-877."""
         return self._apply(P, lib().hf_atomic_coulomb, lib().hf_atomic_coulomb_dev, out)
 
-    def exchange(self, P, out=None):
-        """-K(P) for a spin density P, TwoDBasis.cpp:880-999."""
+    def exchange(self, P, out=None, precision: str = "fp64"):
+        """-K(P) for a spin density P, TwoDBasis.cpp:880-999.
+
+        precision="fp32" runs the gathers and matrix products in FP32 (device
+        tensors only; about 1e-6 relative error), for early SCF iterations."""
+        if precision == "fp32":
+            return self._apply(P, None, lib().hf_atomic_exchange_fp32_dev, out)
+        if precision != "fp64":
+            raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
         return self._apply(P, lib().hf_atomic_exchange, lib().hf_atomic_exchange_dev, out)

@@ -23,6 +23,8 @@
 //   reference never exists.
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
+#include <cstdlib>
+#include <cmath>
 
 namespace hfg {
 namespace {
@@ -173,17 +175,18 @@ __global__ void __launch_bounds__(NT) k_block_norms(const double *P, int64_t ldp
   if (threadIdx.x == 0) bnorm[il] = sqrt(s);
 }
 
+template <typename T>
 struct KArgs {
-  const double *P;
+  const T *P;
   int64_t ldp;
   double *K;
   int64_t ldk;
   int Nel, Nrad, Nang, N_L, n_act;
   const int *first, *ni;
-  const double *rs, *rb, *chol;
+  const T *rs, *rb, *chol;
   const int *chol_off, *chol_rank;
   const int *act_jk, *term_start, *term_il;
-  const double *term_c;
+  const T *term_c;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
 };
@@ -191,13 +194,17 @@ struct KArgs {
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
 // four waves split the multipoles L.  For each L a wave forms the
 // Gaunt-weighted density tile R_L(e,f) = sum_terms c * P_il(e,f) directly in
-// MFMA operand layout (lane (g,c) holds R(c, g+4s)), then contracts it with
-// the radial factors on the matrix cores:
+// matrix-operand layout (lane (g,c) holds R(c, row(g,s))), then contracts it
+// with the radial factors on the matrix cores:
 //   e != f:  K += A R B'       (A, B the r^L / r^-L-1 element matrices)
 //   e == f:  K += sum_p M_p R M_p'   (M_p the Cholesky vectors as 16x16)
-// Workgroups are mapped so that those sharing (e,f) run on one XCD.
-template <bool INSIDE>
-__global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
+// T = double is the reference's arithmetic; T = float runs the gathers and
+// products in FP32 (half the bytes, the FP32 matrix rate) and sums the four
+// waves' tiles in FP64.  Workgroups sharing (e,f) run on one XCD.
+template <bool INSIDE, typename T>
+__global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
+  typedef Mma<T> MM;
+  typedef typename MM::acc_t acc_t;
   const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
   const int efi = (loc / A.n_act) * 8 + xcd;
@@ -205,76 +212,98 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
   if (efi >= A.n_ef) return;
   const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
   __shared__ double red[4 * TILE2];
-  v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-  const bool rowok = c < ni;
-  bool colok[4];
+  acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+  // Lanes outside the element pair read a clamped entry of the same tile and
+  // are masked once per tile, so the term loop has no branches and its loads
+  // can all be in flight together.
+  const int rloc = fi + min(c, ni - 1);
+  int cloc[4], kk[4];
+  T cmask[4];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) colok[s] = (g + 4 * s) < nj;
+  for (int s = 0; s < 4; ++s) {
+    kk[s] = MM::row(g, s);
+    cloc[s] = fj + min(kk[s], nj - 1);
+    cmask[s] = (c < ni && kk[s] < nj) ? T(1) : T(0);
+  }
   for (int L = wv; L < A.N_L; L += 4) {
     const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
     if (q0 == q1) continue;
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int q = q0; q < q1; ++q) {
-      const int il = A.term_il[q];
-      const double cq = A.term_c[q];
-      const int i = il / A.Nang, l = il - i * A.Nang;
-      const double *Pb = A.P + (int64_t)(i * A.Nrad + fi + c) + (int64_t)(l * A.Nrad + fj + g) * A.ldp;
-      if (rowok) {
+    T r[4] = {0, 0, 0, 0};
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      T x[4][4], cq[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-          if (colok[s]) r[s] += cq * Pb[(int64_t)(4 * s) * A.ldp];
+      for (int u = 0; u < 4; ++u) {
+        const int il = A.term_il[q + u], i = il / A.Nang, l = il - i * A.Nang;
+        cq[u] = A.term_c[q + u];
+        const T *Pr = A.P + (int64_t)(i * A.Nrad + rloc);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) x[u][s] = Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
     }
+    for (; q < q1; ++q) {
+      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+      const T cq = A.term_c[q];
+      const T *Pr = A.P + (int64_t)(i * A.Nrad + rloc);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) r[s] *= cmask[s];
     const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
     if (!INSIDE) {
-      const double *At = (e > f) ? A.rb + te : A.rs + te;
-      const double *Bt = (e > f) ? A.rs + tf : A.rb + tf;
-      double as[4], bs[4];
+      const T *At = (e > f) ? A.rb + te : A.rs + te;
+      const T *Bt = (e > f) ? A.rs + tf : A.rb + tf;
+      T as[4], bs[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        as[s] = At[c + TILE * (g + 4 * s)];
-        bs[s] = Bt[c + TILE * (g + 4 * s)];
+        as[s] = At[c + TILE * kk[s]];
+        bs[s] = Bt[c + TILE * kk[s]];
       }
-      v4d y = {0.0, 0.0, 0.0, 0.0};
-      y = mfma_16x16x16(r, bs, y);           // Y = R B'
-      const double ys[4] = {y[0], y[1], y[2], y[3]};
-      acc = mfma_16x16x16(as, ys, acc);      // K += A Y
+      acc_t y = {0, 0, 0, 0};
+      y = mma16<T>(r, bs, y);    // Y = R B'
+      acc = mma16<T>(as, y, acc);  // K += A Y
     } else {
       const int rank = A.chol_rank[L * A.Nel + e];
-      const double *M = A.chol + A.chol_off[L * A.Nel + e];
+      const T *M = A.chol + A.chol_off[L * A.Nel + e];
       // two Cholesky vectors per step, so two independent chains overlap;
       // M_p(c, k) serves as both the Y operand of R M_p' and the X operand of M_p Y
       for (int p = 0; p < rank; p += 2) {
-        const double *Ma = M + (size_t)p * TILE2;
-        const double *Mb = M + (size_t)(p + 1 < rank ? p + 1 : p) * TILE2;
-        const double wb = p + 1 < rank ? 1.0 : 0.0;
-        double ma[4], mb[4];
+        const T *Ma = M + (size_t)p * TILE2;
+        const T *Mb = M + (size_t)(p + 1 < rank ? p + 1 : p) * TILE2;
+        const T wb = p + 1 < rank ? T(1) : T(0);
+        T ma[4], mb[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          ma[s] = Ma[c + TILE * (g + 4 * s)];
-          mb[s] = wb * Mb[c + TILE * (g + 4 * s)];
+          ma[s] = Ma[c + TILE * kk[s]];
+          mb[s] = wb * Mb[c + TILE * kk[s]];
         }
-        v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
+        acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          ya = __builtin_amdgcn_mfma_f64_16x16x4f64(r[s], ma[s], ya, 0, 0, 0);  // R M_a'
-          yb = __builtin_amdgcn_mfma_f64_16x16x4f64(r[s], mb[s], yb, 0, 0, 0);  // R M_b'
+          ya = MM::step(r[s], ma[s], ya);  // R M_a'
+          yb = MM::step(r[s], mb[s], yb);  // R M_b'
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[s], ya[s], acc, 0, 0, 0);   // M_a Y_a
-          acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(mb[s], yb[s], acc2, 0, 0, 0); // M_b Y_b
+          acc = MM::step(ma[s], ya[s], acc);    // M_a Y_a
+          acc2 = MM::step(mb[s], yb[s], acc2);  // M_b Y_b
         }
       }
     }
   }
-  // combine the four waves' partial tiles (lane holds K(g+4r, c))
+  // combine the four waves' partial tiles (lane holds K(row(g,r), c)) in FP64
   acc += acc2;
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
+  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + kk[rr] + TILE * c] = (double)acc[rr];
   __syncthreads();
   const int a = t & 15, b = t >> 4;
   if (a >= ni || b >= nj) return;
@@ -289,7 +318,174 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs A) {
     *dst = -v;
 }
 
+// --- In-element exchange as a dense operator ----------------------------------
+//
+// For one (L, element e) the in-element exchange of every output block coupled
+// to L is Y(a,b) = sum_cd T((a,b),(c,d)) R(c,d) with T(a + c n, b + d n) =
+// sum_p M_p(a,c) M_p(b,d): a GEMM Y (256 x runs) = T (256 x KC) X (KC x runs)
+// on the f64 matrix cores, X the Gaunt-weighted density tiles formed in LDS.
+
+constexpr int NCOL = 32;
+
+struct KInArgs {
+  const double *P;
+  int64_t ldp;
+  int Nel, Nrad, Nang;
+  const int *first, *ni;
+  const double *T;
+  const int *wg, *L_rstart, *L_runs, *run_q, *term_il;
+  const double *term_c;
+  double *Y;
+};
+
+__global__ void __launch_bounds__(NT) k_atomic_kin_gemm(KInArgs A) {
+  const int w = blockIdx.x;
+  const int L = A.wg[3 * w], e = A.wg[3 * w + 1], c0 = A.wg[3 * w + 2];
+  if (L < 0) return;
+  const int r0 = A.L_rstart[L] + c0, r1 = min(A.L_rstart[L + 1], r0 + NCOL);
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
+  const int n = A.ni[e], fe = A.first[e];
+  __shared__ double X[KC * NCOL];
+  __shared__ int srun[NCOL];
+  if (t < NCOL) srun[t] = r0 + t < r1 ? A.L_runs[r0 + t] : -1;
+  __syncthreads();
+  const int cc = t % 15, dd = t / 15;
+  const bool cdok = t < 225 && cc < n && dd < n;
+  if (t < KC)
+    for (int j = 0; j < NCOL; ++j) {
+      const int run = srun[j];
+      double v = 0.0;
+      if (run >= 0 && cdok)
+        for (int q = A.run_q[run]; q < A.run_q[run + 1]; ++q) {
+          const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+          v += A.term_c[q] * A.P[(int64_t)(i * A.Nrad + fe + cc) + (int64_t)(l * A.Nrad + fe + dd) * A.ldp];
+        }
+      X[t * NCOL + j] = v;
+    }
+  __syncthreads();
+  v4d acc[4][2];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) acc[rt][0] = acc[rt][1] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double *Tc = A.T + (size_t)(L * A.Nel + e) * KC * TILE2 + 64 * wv + c;
+  for (int ks = 0; ks < KC / 4; ++ks) {
+    const int k = 4 * ks + g;
+    const double b0 = X[k * NCOL + c], b1 = X[k * NCOL + 16 + c];
+    const double *Tk = Tc + (size_t)k * TILE2;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const double a = Tk[16 * rt];
+      acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[rt][0], 0, 0, 0);
+      acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[rt][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const int run = srun[16 * ct + c];
+    if (run < 0) continue;
+    double *out = A.Y + ((size_t)run * A.Nel + e) * TILE2 + 64 * wv + g;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[16 * rt + 4 * r] = acc[rt][ct][r];
+  }
+}
+
+struct KRedArgs {
+  double *K;
+  int64_t ldk;
+  int Nel, Nrad, Nang;
+  const int *first, *ni, *act_jk, *act_rstart;
+  const double *Y;
+};
+
+// One workgroup per (active output block, element): K tile = -sum of the
+// block's partial tiles; shared boundary rows/cols accumulated atomically.
+__global__ void __launch_bounds__(NT) k_atomic_kin_reduce(KRedArgs A) {
+  const int act = blockIdx.x / A.Nel, e = blockIdx.x - act * A.Nel;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int n = A.ni[e], fe = A.first[e];
+  if (a >= n || b >= n) return;
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  double v = 0.0;
+  for (int run = A.act_rstart[act]; run < A.act_rstart[act + 1]; ++run)
+    v += A.Y[((size_t)run * A.Nel + e) * TILE2 + t];
+  const bool srow = (a == 0 && e > 0) || (a == n - 1 && e + 1 < A.Nel);
+  const bool scol = (b == 0 && e > 0) || (b == n - 1 && e + 1 < A.Nel);
+  double *dst = A.K + (int64_t)(j * A.Nrad + fe + a) + (int64_t)(k * A.Nrad + fe + b) * A.ldk;
+  if (srow || scol)
+    atomicAdd(dst, -v);
+  else
+    *dst = -v;
+}
+
 }  // namespace
+
+// Dense in-element operators T = L L' per (L, element) from the Cholesky
+// factors (CoulombExchangeFE.h:503-510 contracts the same factor as
+// sum_p M_p P M_p').  Built only when the environment variable
+// HFGPU_EXCHANGE_OPERATOR_GIB grants a budget (in GiB) that holds them: on
+// gfx950 the Cholesky-form in-element kernel is the faster of the two
+// (0.85 vs 1.16 ms for Xe at lmax 6), so the operator path is opt-in.
+void atomic_build_operator(const AtomicSetup &S, AtomicDevice &d, hipStream_t s) {
+  const int Nel = S.rad.Nel(), nL = S.N_L;
+  const size_t per = (size_t)KC * TILE2;
+  double budget = 0.0;
+  if (const char *env = std::getenv("HFGPU_EXCHANGE_OPERATOR_GIB")) budget = std::atof(env) * (1ull << 30);
+  d.dense_in = false;
+  d.Top.release();
+  if (nL == 0 || 8.0 * per * nL * Nel > budget) return;
+  std::vector<double> T(per * nL * Nel, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int x = 0; x < nL * Nel; ++x) {
+    const int e = x % Nel, n = S.rad.ni[e], nn = n * n;
+    const Blk &C = S.chol[x];
+    std::vector<double> W((size_t)nn * nn, 0.0);
+    for (int p = 0; p < C.c; ++p) {
+      const double *cp = &C.v[(size_t)p * nn];
+      for (int j = 0; j < nn; ++j) {
+        const double f = cp[j];
+        double *wj = &W[(size_t)j * nn];
+        for (int i = 0; i < nn; ++i) wj[i] += f * cp[i];
+      }
+    }
+    double *Tx = &T[per * x];
+    for (int dd = 0; dd < n; ++dd)
+      for (int cc = 0; cc < n; ++cc) {
+        double *row = Tx + (size_t)(cc + 15 * dd) * TILE2;
+        for (int b = 0; b < n; ++b)
+          for (int a = 0; a < n; ++a) row[a + TILE * b] = W[(size_t)(a + cc * n) + (size_t)(b + dd * n) * nn];
+      }
+  }
+  d.Top.upload(T, s);
+  hip_check(hipStreamSynchronize(s), "exchange operator upload");
+  d.dense_in = true;
+}
+
+namespace {
+__global__ void k_to_f32(const double *src, float *dst, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * NT + threadIdx.x; q < n; q += (size_t)gridDim.x * NT) dst[q] = (float)src[q];
+}
+__global__ void k_mat_to_f32(const double *A, int64_t lda, float *B, int64_t n) {
+  const int64_t col = blockIdx.y;
+  for (int64_t r = (int64_t)blockIdx.x * NT + threadIdx.x; r < n; r += (int64_t)gridDim.x * NT)
+    B[r + col * n] = (float)A[r + col * lda];
+}
+}  // namespace
+
+void to_f32_launch(const double *src, float *dst, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + NT - 1) / NT, 65536);
+  hipLaunchKernelGGL(k_to_f32, dim3(blocks), dim3(NT), 0, s, src, dst, n);
+  hip_check(hipGetLastError(), "to_f32");
+}
+
+void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  if (n > 65535) throw Error(-1, "matrix too large for the FP32 conversion");
+  const unsigned bx = (unsigned)std::min<int64_t>((n + NT - 1) / NT, 64);
+  hipLaunchKernelGGL(k_mat_to_f32, dim3(bx, (unsigned)n), dim3(NT), 0, s, A, lda, B, n);
+  hip_check(hipGetLastError(), "mat_to_f32");
+}
 
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   const int64_t n = rows * cols;
@@ -371,6 +567,57 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   }
   d.flops_in = frad_in + d.flops_k_angular * area_in / area_all;
   d.flops_off = d.flops_k_radial + d.flops_k_angular - d.flops_in;
+  if (d.dense_in && !act.empty()) {
+    // runs (act, L) with terms, act-major; their term ranges are contiguous,
+    // rq[r] .. rq[r+1]; grouped by L for the GEMM
+    std::vector<int> rq(1, 0), rL, ars(1, 0);
+    for (size_t a = 0; a < act.size(); ++a) {
+      for (int L = 0; L < nL; ++L) {
+        const int q0 = tstart[a * nL + L], q1 = tstart[a * nL + L + 1];
+        if (q1 == q0) continue;
+        rq.back() = q0;
+        rL.push_back(L);
+        rq.push_back(q1);
+      }
+      ars.push_back((int)rL.size());
+    }
+    const int nr = (int)rL.size(), Nel = d.Nel;
+    std::vector<int> Ls(nL + 1, 0), Lruns(nr);
+    for (int L : rL) ++Ls[L + 1];
+    for (int L = 0; L < nL; ++L) Ls[L + 1] += Ls[L];
+    std::vector<int> fill(Ls.begin(), Ls.end() - 1);
+    for (int r = 0; r < nr; ++r) Lruns[fill[rL[r]]++] = r;
+    std::vector<std::vector<int>> per_xcd(8);
+    int grp = 0;
+    for (int L = 0; L < nL; ++L) {
+      const int cnt = Ls[L + 1] - Ls[L];
+      if (!cnt) continue;
+      for (int e = 0; e < Nel; ++e, ++grp)
+        for (int c0 = 0; c0 < cnt; c0 += 32) {
+          auto &v = per_xcd[grp % 8];
+          v.push_back(L);
+          v.push_back(e);
+          v.push_back(c0);
+        }
+    }
+    size_t mx = 0;
+    for (auto &v : per_xcd) mx = std::max(mx, v.size() / 3);
+    std::vector<int> wg(mx * 8 * 3, -1);
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = 0; j < per_xcd[x].size() / 3; ++j)
+        for (int u = 0; u < 3; ++u) wg[(j * 8 + x) * 3 + u] = per_xcd[x][3 * j + u];
+    d.n_wg_in = (int)(mx * 8);
+    d.n_runs = nr;
+    d.run_q.upload(rq, s);
+    d.act_rstart.upload(ars, s);
+    d.L_rstart.upload(Ls, s);
+    d.L_runs.upload(Lruns, s);
+    d.wg_in.upload(wg, s);
+    d.Ypart.alloc((size_t)nr * Nel * TILE2);
+    double fr = 0.0;
+    for (int e = 0; e < Nel; ++e) fr += 2.0 * std::pow((double)d.h_ni[e], 4);
+    d.flops_in = (double)nr * fr + d.flops_k_angular * area_in / area_all;
+  }
   if (act.empty()) act.push_back(0);
   if (til.empty()) {
     til.push_back(0);
@@ -382,10 +629,11 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   d.term_c.upload(tc, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;
+  ++d.list_gen;
 }
 
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                            hipStream_t s, KernelTimer *timer) {
+                            hipStream_t s, KernelTimer *timer, bool fp32) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   if (d.h_act.empty()) {
     fill_zero_launch(K, nbf, nbf, ldk, s);
@@ -399,22 +647,65 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
   compact_exchange_lists(d, s);
   fill_zero_launch(K, nbf, nbf, ldk, s);
   if (d.n_act == 0) return;
+  if (fp32) {
+    if (!d.f32_tables) {
+      d.rs32.alloc(d.rs.n);
+      d.rb32.alloc(d.rb.n);
+      d.chol32.alloc(d.chol.n);
+      to_f32_launch(d.rs.p, d.rs32.p, d.rs.n, s);
+      to_f32_launch(d.rb.p, d.rb32.p, d.rb.n, s);
+      to_f32_launch(d.chol.p, d.chol32.p, d.chol.n, s);
+      d.f32_tables = true;
+    }
+    if (d.term_c32_gen != d.list_gen) {
+      d.term_c32.alloc(d.term_c.n);
+      to_f32_launch(d.term_c.p, d.term_c32.p, d.term_c.n, s);
+      d.term_c32_gen = d.list_gen;
+    }
+    d.P32.alloc((size_t)nbf * nbf);
+    mat_to_f32_launch(P, ldp, d.P32.p, nbf, s);
+  }
   // two launches: element pairs e != f and the in-element tiles; boundary
   // rows/cols shared by neighbouring tiles are accumulated atomically in both
   for (int part = 0; part < 2; ++part) {
     const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
     if (n_ef == 0) continue;
+    const char *name = part == 0 ? (fp32 ? "atomic_k_off_f32" : "atomic_k_off")
+                                 : (fp32 ? "atomic_k_in_f32" : "atomic_k_in");
+    const double flops = part == 0 ? d.flops_off : d.flops_in;
+    if (part == 1 && d.dense_in && !fp32) {
+      KInArgs g{P, ldp, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.Top.p, d.wg_in.p, d.L_rstart.p,
+                d.L_runs.p, d.run_q.p, d.term_il.p, d.term_c.p, d.Ypart.p};
+      KRedArgs r{K, ldk, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.act_jk.p, d.act_rstart.p, d.Ypart.p};
+      TimedLaunch tl(timer, name, s, flops);
+      hipLaunchKernelGGL(k_atomic_kin_gemm, dim3(d.n_wg_in), dim3(NT), 0, s, g);
+      hip_check(hipGetLastError(), "atomic_kin_gemm");
+      hipLaunchKernelGGL(k_atomic_kin_reduce, dim3(d.n_act * d.Nel), dim3(NT), 0, s, r);
+      hip_check(hipGetLastError(), "atomic_kin_reduce");
+      continue;
+    }
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;
-    KArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
-            d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p,
-            part == 0 ? d.ef_off.p : d.ef_in.p, n_ef};
     const int64_t grid = (int64_t)n_ef_pad * d.n_act;
     if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-    TimedLaunch tl(timer, part == 0 ? "atomic_k_off" : "atomic_k_in", s, part == 0 ? d.flops_off : d.flops_in);
-    if (part == 0)
-      hipLaunchKernelGGL(k_atomic_k<false>, dim3((unsigned)grid), dim3(NT), 0, s, a);
-    else
-      hipLaunchKernelGGL(k_atomic_k<true>, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    const int *efl = part == 0 ? d.ef_off.p : d.ef_in.p;
+    TimedLaunch tl(timer, name, s, flops);
+    if (fp32) {
+      KArgs<float> a{d.P32.p, nbf, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p,
+                     d.rs32.p, d.rb32.p, d.chol32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p,
+                     d.term_il.p, d.term_c32.p, efl, n_ef};
+      if (part == 0)
+        hipLaunchKernelGGL((k_atomic_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_atomic_k<true, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+    } else {
+      KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
+                      d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p,
+                      d.term_c.p, efl, n_ef};
+      if (part == 0)
+        hipLaunchKernelGGL((k_atomic_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_atomic_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+    }
     hip_check(hipGetLastError(), "atomic_k");
   }
 }

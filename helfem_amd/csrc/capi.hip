@@ -201,6 +201,7 @@ void atomic_upload(hf_atomic *h) {
   d.preB.alloc((size_t)std::max(nlm, 1) * Nel);
   d.bnorm.alloc((size_t)Na * Na);
   hip_check(hipStreamSynchronize(s), "upload sync");
+  atomic_build_operator(S, d, s);
   h->uploaded = true;
 }
 
@@ -470,6 +471,18 @@ int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *
     check_matrix(dP, ldp, h->host.Nbf(), "P");
     check_matrix(dK, ldk, h->host.Nbf(), "K");
     atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur, &h->ctx->timer);
+  });
+}
+
+int hf_atomic_exchange_fp32_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dK, ldk, h->host.Nbf(), "K");
+    atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur, &h->ctx->timer, true);
   });
 }
 

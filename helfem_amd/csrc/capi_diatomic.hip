@@ -2,6 +2,7 @@
 // the coupling lists of the Coulomb and exchange kernels (diatomic_jk.hip).
 #include "capi_common.hpp"
 #include <cstring>
+#include <cstdlib>
 #include <cmath>
 #include <memory>
 #ifdef _OPENMP
@@ -9,6 +10,60 @@
 #endif
 
 using namespace hfg;
+
+namespace hfg {
+
+// In-element exchange operator per ((L,|M|), element), from the sign-aware
+// Cholesky factor: W~(i, j) = sum_p sigma_p B(i, p) B(j, p) over the two
+// channel blocks (basis.cpp:1992-2033 contracts the same factor as
+// sum_p sigma_p [M0' R00 M0' - M0' R02 M2' - M2' R20 M0' + M2' R22 M2']).
+// Built only when the environment variable HFGPU_EXCHANGE_OPERATOR_GIB grants
+// a budget (in GiB) that holds the tables; otherwise (dense_in = false) the
+// Cholesky-form in-element kernel runs, which on gfx950 is the faster of the
+// two (64 vs 216 ms for N2 at lmax 16), so the operator path is opt-in.
+void diatomic_build_operator(const DiatomicSetup &S, DiatomicDevice &d, hipStream_t s) {
+  const int Nel = S.Nel(), nlm = (int)S.lm_map.size();
+  const size_t per = (size_t)4 * KC * TILE2;
+  const double bytes = 8.0 * per * nlm * Nel;
+  double budget = 0.0;
+  if (const char *env = std::getenv("HFGPU_EXCHANGE_OPERATOR_GIB")) budget = std::atof(env) * (1ull << 30);
+  d.dense_in = false;
+  d.Top.release();
+  if (nlm == 0 || bytes > budget) return;
+  std::vector<double> T(per * nlm * Nel, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int x = 0; x < nlm * Nel; ++x) {
+    const int e = x % Nel;
+    const int n = S.rad.ni[e], nn = n * n, N = 2 * nn;
+    const Blk &B = S.cdB[x];
+    const std::vector<double> &sg = S.cdS[x];
+    std::vector<double> W((size_t)N * N, 0.0);
+    for (int p = 0; p < B.c; ++p) {
+      const double *bp = &B.v[(size_t)p * N];
+      for (int j = 0; j < N; ++j) {
+        const double f = sg[p] * bp[j];
+        double *wj = &W[(size_t)j * N];
+        for (int i = 0; i < N; ++i) wj[i] += f * bp[i];
+      }
+    }
+    double *Tx = &T[per * x];
+    const int xo[4] = {0, 0, nn, nn}, yo[4] = {0, nn, 0, nn};
+    const double sgn[4] = {1.0, -1.0, -1.0, 1.0};
+    for (int ch = 0; ch < 4; ++ch)
+      for (int dd = 0; dd < n; ++dd)
+        for (int cc = 0; cc < n; ++cc) {
+          double *row = Tx + ((size_t)ch * KC + cc + 15 * dd) * TILE2;
+          for (int b = 0; b < n; ++b)
+            for (int a = 0; a < n; ++a)
+              row[a + TILE * b] = sgn[ch] * W[(size_t)(xo[ch] + cc + a * n) + (size_t)(yo[ch] + b + dd * n) * N];
+        }
+  }
+  d.Top.upload(T, s);
+  hip_check(hipStreamSynchronize(s), "exchange operator upload");
+  d.dense_in = true;
+}
+
+}  // namespace hfg
 
 namespace {
 
@@ -218,6 +273,7 @@ void diatomic_upload(hf_diatomic *h) {
   d.preA.alloc((size_t)std::max(nLM, 1) * Nel);
   d.preB.alloc((size_t)std::max(nLM, 1) * Nel);
   hip_check(hipStreamSynchronize(s), "upload sync");
+  diatomic_build_operator(S, d, s);
   h->uploaded = true;
 }
 
@@ -322,13 +378,51 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.flops_k_angular = 8.0 * (double)nterm * sum_all;
   d.flops_in = frad_in + 8.0 * (double)nterm * sum_in;
   d.flops_off = frad_off + 8.0 * (double)nterm * (sum_all - sum_in);
+  // dense in-element path: runs grouped by (L,|M|); one workgroup per
+  // (ilm, element, 32 runs), groups dealt round-robin to the 8 XCDs so the
+  // workgroups sharing one operator table run on the same L2
+  if (d.dense_in && d.n_runs > 0) {
+    const int Nel = S.Nel();
+    std::vector<int> rs(nlm + 1, 0), runs(rilm.size());
+    for (int r : rilm) ++rs[r + 1];
+    for (int q = 0; q < nlm; ++q) rs[q + 1] += rs[q];
+    std::vector<int> fill(rs.begin(), rs.end() - 1);
+    for (size_t r = 0; r < rilm.size(); ++r) runs[fill[rilm[r]]++] = (int)r;
+    std::vector<std::vector<int>> per_xcd(8);
+    int grp = 0;
+    for (int q = 0; q < nlm; ++q) {
+      const int nr = rs[q + 1] - rs[q];
+      if (!nr) continue;
+      for (int e = 0; e < Nel; ++e, ++grp)
+        for (int c0 = 0; c0 < nr; c0 += 32) {
+          auto &v = per_xcd[grp % 8];
+          v.push_back(q);
+          v.push_back(e);
+          v.push_back(c0);
+        }
+    }
+    size_t mx = 0;
+    for (auto &v : per_xcd) mx = std::max(mx, v.size() / 3);
+    std::vector<int> wg(mx * 8 * 3, -1);
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = 0; j < per_xcd[x].size() / 3; ++j)
+        for (int u = 0; u < 3; ++u) wg[(j * 8 + x) * 3 + u] = per_xcd[x][3 * j + u];
+    d.n_wg_in = (int)(mx * 8);
+    d.ilm_rstart.upload(rs, s);
+    d.ilm_runs.upload(runs, s);
+    d.wg_in.upload(wg, s);
+    d.Ypart.alloc((size_t)d.n_runs * Nel * TILE2);
+    // the dense operator replaces 12 r n^3 by 8 n^4 per (run, element)
+    double fr = 0.0;
+    for (int e = 0; e < Nel; ++e) fr += 8.0 * std::pow((double)S.rad.ni[e], 4);
+    d.flops_in = (double)d.n_runs * fr + 8.0 * (double)nterm * sum_in;
+  }
   if (act.empty()) act.push_back(0);
   if (rilm.empty()) rilm.push_back(0);
   if (til.empty()) {
     til.push_back(0);
     tf.assign(4, 0.0);
   }
-  (void)nlm;
   d.act_jk.upload(act, s);
   d.act_lstart.upload(lstart, s);
   d.lrun_ilm.upload(rilm, s);
@@ -337,6 +431,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;
+  ++d.list_gen;
 }
 
 void ensure_ready(hf_diatomic *h) {
@@ -345,7 +440,7 @@ void ensure_ready(hf_diatomic *h) {
   if (!h->uploaded) diatomic_upload(h);
 }
 
-void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk) {
+void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk, bool fp32 = false) {
   DiatomicDevice &d = h->dev;
   hipStream_t s = h->ctx->cur;
   diatomic_block_norms_launch(d, P, ldp, s);
@@ -353,7 +448,7 @@ void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, in
             "copy block norms");
   hip_check(hipStreamSynchronize(s), "block norms sync");
   build_exchange_lists(h, s);
-  diatomic_exchange_kernels(d, P, ldp, K, ldk, s, &h->ctx->timer);
+  diatomic_exchange_kernels(d, P, ldp, K, ldk, s, &h->ctx->timer, fp32);
 }
 
 hf_diatomic *checked(hf_diatomic *h) {
@@ -532,6 +627,17 @@ int hf_diatomic_coulomb(hf_diatomic *h, const double *P, int64_t ldp, double *J,
                    [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
                      diatomic_coulomb_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
                    });
+  });
+}
+
+int hf_diatomic_exchange_fp32_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk) {
+  return guarded([&] {
+    checked(h);
+    DeviceGuard g(h->ctx->device);
+    ensure_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dK, ldk, h->host.Nbf(), "K");
+    exchange_launch(h, dP, ldp, dK, ldk, true);
   });
 }
 

@@ -82,6 +82,7 @@ void xc_upload(hf_dftgrid *g) {
   if (lds > 64 * 1024)
     throw Error(HF_EINVAL, "angular basis and DFT grid too large for one workgroup's LDS (need shell pairs + "
                            "lang*(4 mmax + 1) + lang*mang <= 8192)");
+  d.lds_pol_ok = 2 * lds <= 64 * 1024;
   if (sizeof(double) * ((size_t)nrad * TILE + 4 * TILE2) > 64 * 1024) throw Error(HF_EINVAL, "radial quadrature too large");
   std::vector<double> xq, wq, xt, wt;
   gauss_chebyshev2(nrad, xq, wq);
@@ -145,11 +146,19 @@ void check_functionals(int x_func, int c_func) {
   if (c_func != 0 && c_func != 7) throw Error(HF_EINVAL, "unsupported correlation functional (0 or 7 = lda_c_vwn)");
 }
 
-void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H, int64_t ldh,
-              double thr, double *out) {
+// Pb == nullptr: restricted; else spin-polarized (Hb gets the beta potential,
+// zeroed when `beta` is false, as the reference leaves it).
+void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, const double *Pb, int64_t ldpb,
+              double *H, int64_t ldh, double *Hb, int64_t ldhb, bool beta, double thr, double *out) {
   hipStream_t s = g->ctx->cur;
   XCGridDevice &d = g->dev;
-  xc_grid_launch(d, x_func, c_func, thr, P, ldp, H, ldh, s, &g->ctx->timer);
+  if (Pb) {
+    if (!d.lds_pol_ok) throw Error(HF_EINVAL, "DFT grid too large for the spin-polarized evaluation");
+    d.Db.alloc(d.D.n);
+    d.Wb.alloc(d.W.n);
+  }
+  xc_grid_launch(d, x_func, c_func, thr, P, ldp, Pb, ldpb, H, ldh, Hb, ldhb, s, &g->ctx->timer);
+  if (Pb && !beta) fill_zero_launch(Hb, d.Nbf, d.Nbf, ldhb, s);
   hip_check(hipMemcpyAsync(d.h_part.p, d.part.p, sizeof(double) * 2 * d.Nel * d.nrad, hipMemcpyDeviceToHost, s),
             "copy XC partial sums");
   hip_check(hipStreamSynchronize(s), "XC sync");
@@ -221,7 +230,7 @@ int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double 
     check_matrix(dP, ldp, n, "P");
     check_matrix(dH, ldh, n, "H");
     if (!g->uploaded) xc_upload(g);
-    eval_fxc(g, x_func, c_func, dP, ldp, dH, ldh, thr, out);
+    eval_fxc(g, x_func, c_func, dP, ldp, nullptr, 0, dH, ldh, nullptr, 0, false, thr, out);
   });
 }
 
@@ -235,8 +244,59 @@ int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, 
     if (!g->uploaded) xc_upload(g);
     host_roundtrip(g->ctx->cur, g->Nbf(), g->dev.hostP, g->dev.hostOut, P, ldp, H, ldh,
                    [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
-                     eval_fxc(g, x_func, c_func, dp, lp, dout, lo, thr, out);
+                     eval_fxc(g, x_func, c_func, dp, lp, nullptr, 0, dout, lo, nullptr, 0, false, thr, out);
                    });
+  });
+}
+
+int hf_dftgrid_eval_fxc_pol_dev(hf_dftgrid *g, int x_func, int c_func, const double *dPa, int64_t ldpa,
+                                const double *dPb, int64_t ldpb, double *dHa, int64_t ldha, double *dHb, int64_t ldhb,
+                                int beta, double thr, double out[3]) {
+  return guarded([&] {
+    ready(g);
+    need(out != nullptr, "out is NULL");
+    check_functionals(x_func, c_func);
+    DeviceGuard dg(g->ctx->device);
+    const int64_t n = g->Nbf();
+    check_matrix(dPa, ldpa, n, "Pa");
+    check_matrix(dPb, ldpb, n, "Pb");
+    check_matrix(dHa, ldha, n, "Ha");
+    check_matrix(dHb, ldhb, n, "Hb");
+    if (!g->uploaded) xc_upload(g);
+    eval_fxc(g, x_func, c_func, dPa, ldpa, dPb, ldpb, dHa, ldha, dHb, ldhb, beta != 0, thr, out);
+  });
+}
+
+int hf_dftgrid_eval_fxc_pol(hf_dftgrid *g, int x_func, int c_func, const double *Pa, int64_t ldpa, const double *Pb,
+                            int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, int beta, double thr,
+                            double out[3]) {
+  return guarded([&] {
+    ready(g);
+    need(out != nullptr, "out is NULL");
+    check_functionals(x_func, c_func);
+    DeviceGuard dg(g->ctx->device);
+    const int64_t n = g->Nbf();
+    check_matrix(Pa, ldpa, n, "Pa");
+    check_matrix(Pb, ldpb, n, "Pb");
+    check_matrix(Ha, ldha, n, "Ha");
+    check_matrix(Hb, ldhb, n, "Hb");
+    if (!g->uploaded) xc_upload(g);
+    XCGridDevice &d = g->dev;
+    hipStream_t s = g->ctx->cur;
+    const size_t nn = (size_t)n * n;
+    d.hostP.alloc(2 * nn);
+    d.hostOut.alloc(2 * nn);
+    hip_check(hipMemcpy2DAsync(d.hostP.p, n * sizeof(double), Pa, ldpa * sizeof(double), n * sizeof(double), n,
+                               hipMemcpyHostToDevice, s), "copy Pa");
+    hip_check(hipMemcpy2DAsync(d.hostP.p + nn, n * sizeof(double), Pb, ldpb * sizeof(double), n * sizeof(double),
+                               n, hipMemcpyHostToDevice, s), "copy Pb");
+    eval_fxc(g, x_func, c_func, d.hostP.p, n, d.hostP.p + nn, n, d.hostOut.p, n, d.hostOut.p + nn, n, beta != 0,
+             thr, out);
+    hip_check(hipMemcpy2DAsync(Ha, ldha * sizeof(double), d.hostOut.p, n * sizeof(double), n * sizeof(double), n,
+                               hipMemcpyDeviceToHost, s), "copy Ha");
+    hip_check(hipMemcpy2DAsync(Hb, ldhb * sizeof(double), d.hostOut.p + nn, n * sizeof(double), n * sizeof(double),
+                               n, hipMemcpyDeviceToHost, s), "copy Hb");
+    hip_check(hipStreamSynchronize(s), "stream sync");
   });
 }
 

@@ -46,5 +46,45 @@ __device__ inline v4d mfma_16x16x16(const double xs[4], v4d y, v4d d) {
   return mfma_16x16x16(xs, ys, d);
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Operand/result layout of the 16x16x4 matrix instructions, per precision.
+// Both take X(c, k) and Y(k, c) from lane l (g = l/16, c = l%16) with the
+// instruction's inner index g; the result row held in accumulator component
+// r differs (measured on gfx950 with one-hot probes):
+//   v_mfma_f64_16x16x4f64: D(g + 4r, c)      v_mfma_f32_16x16x4f32: D(4g + r, c)
+// Choosing the global inner index of step s as row(g, s) makes a result tile
+// the next product's Y operand without a shuffle, in either precision.
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+  typedef v4d acc_t;
+  __device__ static int row(int g, int s) { return g + 4 * s; }
+  __device__ static acc_t step(double x, double y, acc_t d) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, d, 0, 0, 0);
+  }
+};
+template <> struct Mma<float> {
+  typedef v4f acc_t;
+  __device__ static int row(int g, int s) { return 4 * g + s; }
+  __device__ static acc_t step(float x, float y, acc_t d) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, d, 0, 0, 0);
+  }
+};
+
+// D(16x16) += X Y over the 16-long inner index, four instructions.
+template <typename T>
+__device__ inline typename Mma<T>::acc_t mma16(const T xs[4], const T ys[4], typename Mma<T>::acc_t d) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) d = Mma<T>::step(xs[s], ys[s], d);
+  return d;
+}
+template <typename T>
+__device__ inline typename Mma<T>::acc_t mma16(const T xs[4], typename Mma<T>::acc_t y,
+                                               typename Mma<T>::acc_t d) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) d = Mma<T>::step(xs[s], y[s], d);
+  return d;
+}
+
 }  // namespace dev
 }  // namespace hfg

@@ -194,50 +194,54 @@ __global__ void __launch_bounds__(NT) k_dia_block_norms(const double *P, int64_t
 
 // Two products sharing the Y operand, issued interleaved so the two
 // accumulation chains overlap: da += Xa Y, db += Xb Y (16x16x16 each).
-__device__ inline void mfma_pair(const double xa[4], const double xb[4], const double ys[4], v4d &da, v4d &db) {
+template <typename T>
+__device__ inline void mma_pair(const T xa[4], const T xb[4], const T ys[4], typename Mma<T>::acc_t &da,
+                                typename Mma<T>::acc_t &db) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    da = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], ys[s], da, 0, 0, 0);
-    db = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[s], ys[s], db, 0, 0, 0);
+    da = Mma<T>::step(xa[s], ys[s], da);
+    db = Mma<T>::step(xb[s], ys[s], db);
   }
 }
 
 // da += Xa Ya, db += Xb Yb with Ya, Yb result tiles, interleaved.
-__device__ inline void mfma_pair2(const double xa[4], v4d ya, v4d &da, const double xb[4], v4d yb, v4d &db) {
+template <typename T>
+__device__ inline void mma_pair2(const T xa[4], typename Mma<T>::acc_t ya, typename Mma<T>::acc_t &da,
+                                 const T xb[4], typename Mma<T>::acc_t yb, typename Mma<T>::acc_t &db) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    da = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s], ya[s], da, 0, 0, 0);
-    db = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[s], yb[s], db, 0, 0, 0);
+    da = Mma<T>::step(xa[s], ya[s], da);
+    db = Mma<T>::step(xb[s], yb[s], db);
   }
 }
 
 // Cholesky channel tiles of vector p in both operand layouts:
-// y = M(c, k) (M' as Y operand), x = M(k, c) (M' as X operand), k = g + 4s.
-__device__ inline void load_chol(const double *M, int p, int g, int c, double y0[4], double y2[4], double x0[4],
-                                 double x2[4]) {
-  const double *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
+// y = M(c, k) (M' as Y operand), x = M(k, c) (M' as X operand), k = row(g, s).
+template <typename T>
+__device__ inline void load_chol(const T *M, int p, const int kk[4], int c, T y0[4], T y2[4], T x0[4], T x2[4]) {
+  const T *M0 = M + (size_t)p * 2 * TILE2, *M2 = M0 + TILE2;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    const int kk = g + 4 * s;
-    y0[s] = M0[c + TILE * kk];
-    y2[s] = M2[c + TILE * kk];
-    x0[s] = M0[kk + TILE * c];
-    x2[s] = M2[kk + TILE * c];
+    y0[s] = M0[c + TILE * kk[s]];
+    y2[s] = M2[c + TILE * kk[s]];
+    x0[s] = M0[kk[s] + TILE * c];
+    x2[s] = M2[kk[s] + TILE * c];
   }
 }
 
+template <typename T>
 struct DKArgs {
-  const double *P;
+  const T *P;
   int64_t ldp;
   double *K;
   int64_t ldk;
   int Nel, Nang, n_act;
   const int *first, *ni, *skip;
   const long long *poff;
-  const double *pq, *chol, *sigma;
+  const T *pq, *chol, *sigma;
   const int *chol_off, *chol_rank;
   const int *act_jk, *act_lstart, *lrun_ilm, *lrun_tstart, *term_il;
-  const double *term_f;
+  const T *term_f;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
 };
@@ -245,8 +249,12 @@ struct DKArgs {
 // One workgroup per (active output block (j,k), element pair (e,f)), the
 // four waves splitting the block's coupled (L,|M|) runs.  Workgroups sharing
 // (e,f) are mapped to one XCD so the (e,f) slice of P stays in its L2.
-template <bool INSIDE>
-__global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
+// T = double is the reference's arithmetic; T = float gathers and multiplies
+// in FP32 and sums the waves' tiles in FP64.
+template <bool INSIDE, typename T>
+__global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
+  typedef Mma<T> MM;
+  typedef typename MM::acc_t acc_t;
   const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
   const int efi = (loc / A.n_act) * 8 + xcd;
@@ -254,62 +262,103 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
   if (efi >= A.n_ef) return;
   const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
   __shared__ double red[4 * TILE2];
-  v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-  const bool rowok = c < ni;
-  bool colok[4];
+  acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+  // Lanes outside the element pair read a clamped entry and are masked once
+  // per tile; the dropped radial function 0 of an m != 0 shell (only at the
+  // first radial index) is masked per term.  The term loop has no branches.
+  const int rloc = fi + min(c, ni - 1);
+  int cloc[4], kk[4];
+  T cmask[4];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) colok[s] = (g + 4 * s) < nj;
+  for (int s = 0; s < 4; ++s) {
+    kk[s] = MM::row(g, s);
+    cloc[s] = fj + min(kk[s], nj - 1);
+    cmask[s] = (c < ni && kk[s] < nj) ? T(1) : T(0);
+  }
   const int r0 = A.act_lstart[act], r1 = A.act_lstart[act + 1];
   for (int run = r0 + wv; run < r1; run += 4) {
     const int ilm = A.lrun_ilm[run];
-    double r00[4] = {0, 0, 0, 0}, r02[4] = {0, 0, 0, 0}, r20[4] = {0, 0, 0, 0}, r22[4] = {0, 0, 0, 0};
-    for (int q = A.lrun_tstart[run]; q < A.lrun_tstart[run + 1]; ++q) {
-      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
-      const double f00 = A.term_f[4 * q], f02 = A.term_f[4 * q + 1], f20 = A.term_f[4 * q + 2],
-                   f22 = A.term_f[4 * q + 3];
-      const long long row = rowok ? pure_row(A.skip, A.poff, i, fi + c) : -1;
-      if (row < 0) continue;
-      const int sl = A.skip[l];
-      const long long cb = A.poff[l] - sl;
+    T r00[4] = {0, 0, 0, 0}, r02[4] = {0, 0, 0, 0}, r20[4] = {0, 0, 0, 0}, r22[4] = {0, 0, 0, 0};
+    const int q0 = A.lrun_tstart[run], q1 = A.lrun_tstart[run + 1];
+    int q = q0;
+    for (; q + 2 <= q1; q += 2) {
+      T x[2][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int cr = fj + g + 4 * s;
-        if (colok[s] && cr >= sl) {
-          const double x = A.P[row + (cb + cr) * A.ldp];
-          r00[s] += f00 * x;
-          r02[s] += f02 * x;
-          r20[s] += f20 * x;
-          r22[s] += f22 * x;
+      for (int u = 0; u < 2; ++u) {
+        const int il = A.term_il[q + u], i = il / A.Nang, l = il - i * A.Nang;
+        const int si = A.skip[i], sl = A.skip[l];
+        const long long row = A.poff[i] + max(rloc - si, 0), cb = A.poff[l] - sl;
+        const bool rv = rloc >= si;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const T v = A.P[row + (cb + max(cloc[s], sl)) * A.ldp];
+          x[u][s] = (rv && cloc[s] >= sl) ? v : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const T f00 = A.term_f[4 * (q + u)], f02 = A.term_f[4 * (q + u) + 1], f20 = A.term_f[4 * (q + u) + 2],
+                f22 = A.term_f[4 * (q + u) + 3];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          r00[s] += f00 * x[u][s];
+          r02[s] += f02 * x[u][s];
+          r20[s] += f20 * x[u][s];
+          r22[s] += f22 * x[u][s];
         }
       }
     }
-    const double *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
-    if (!INSIDE) {
-      const double *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
-      const bool up = e > f;
-      // row side: Q of e if e > f else P of e; column side the other kind
-      const double *i0 = te + (up ? 2 : 0) * TILE2, *i2 = te + (up ? 3 : 1) * TILE2;
-      const double *j0 = tf + (up ? 0 : 2) * TILE2, *j2 = tf + (up ? 1 : 3) * TILE2;
-      double a0[4], a2[4], b0[4], b2[4];
+    for (; q < q1; ++q) {
+      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+      const int si = A.skip[i], sl = A.skip[l];
+      const long long row = A.poff[i] + max(rloc - si, 0), cb = A.poff[l] - sl;
+      const bool rv = rloc >= si;
+      const T f00 = A.term_f[4 * q], f02 = A.term_f[4 * q + 1], f20 = A.term_f[4 * q + 2], f22 = A.term_f[4 * q + 3];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const int kk = c + TILE * (g + 4 * s);
-        a0[s] = i0[kk];
-        a2[s] = i2[kk];
-        b0[s] = j0[kk];
-        b2[s] = j2[kk];
+        const T v = A.P[row + (cb + max(cloc[s], sl)) * A.ldp];
+        const T x = (rv && cloc[s] >= sl) ? v : T(0);
+        r00[s] += f00 * x;
+        r02[s] += f02 * x;
+        r20[s] += f20 * x;
+        r22[s] += f22 * x;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      r00[s] *= cmask[s];
+      r02[s] *= cmask[s];
+      r20[s] *= cmask[s];
+      r22[s] *= cmask[s];
+    }
+    const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
+    if (!INSIDE) {
+      const T *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
+      const bool up = e > f;
+      // row side: Q of e if e > f else P of e; column side the other kind
+      const T *i0 = te + (up ? 2 : 0) * TILE2, *i2 = te + (up ? 3 : 1) * TILE2;
+      const T *j0 = tf + (up ? 0 : 2) * TILE2, *j2 = tf + (up ? 1 : 3) * TILE2;
+      T a0[4], a2[4], b0[4], b2[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int x = c + TILE * kk[s];
+        a0[s] = i0[x];
+        a2[s] = i2[x];
+        b0[s] = j0[x];
+        b2[s] = j2[x];
       }
       // two independent chains: ya = R00 j0' + R02 j2', yb = R20 j0' + R22 j2'
-      v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
-      mfma_pair(r00, r20, b0, ya, yb);
-      mfma_pair(r02, r22, b2, ya, yb);
-      mfma_pair2(a0, ya, acc, a2, yb, acc2);  // acc += i0 ya, acc2 += i2 yb
+      acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
+      mma_pair<T>(r00, r20, b0, ya, yb);
+      mma_pair<T>(r02, r22, b2, ya, yb);
+      mma_pair2<T>(a0, ya, acc, a2, yb, acc2);  // acc += i0 ya, acc2 += i2 yb
     } else {
       const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
-      const double *M = A.chol + (size_t)off * 2 * TILE2;
+      const T *M = A.chol + (size_t)off * 2 * TILE2;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         r02[s] = -r02[s];
@@ -317,23 +366,23 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
       }
       // M0', M2' as Y operands (y0, y2) and as X operands (x0, x2); the next
       // vector's tiles are loaded while the current one is on the matrix cores
-      double y0[4], y2[4], x0[4], x2[4];
-      load_chol(M, 0, g, c, y0, y2, x0, x2);
-      double sg = rank > 0 ? A.sigma[off] : 0.0;
+      T y0[4], y2[4], x0[4], x2[4];
+      load_chol<T>(M, 0, kk, c, y0, y2, x0, x2);
+      T sg = rank > 0 ? A.sigma[off] : T(0);
       for (int p = 0; p < rank; ++p) {
-        double ny0[4], ny2[4], nx0[4], nx2[4];
-        double nsg = 0.0;
+        T ny0[4], ny2[4], nx0[4], nx2[4];
+        T nsg = 0;
         if (p + 1 < rank) {
-          load_chol(M, p + 1, g, c, ny0, ny2, nx0, nx2);
+          load_chol<T>(M, p + 1, kk, c, ny0, ny2, nx0, nx2);
           nsg = A.sigma[off + p + 1];
         }
         // ya = R00 M0' - R02 M2', yb = -R20 M0' + R22 M2' (signs folded into r02, r20)
-        v4d ya = {0.0, 0.0, 0.0, 0.0}, yb = {0.0, 0.0, 0.0, 0.0};
-        mfma_pair(r00, r20, y0, ya, yb);
-        mfma_pair(r02, r22, y2, ya, yb);
+        acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
+        mma_pair<T>(r00, r20, y0, ya, yb);
+        mma_pair<T>(r02, r22, y2, ya, yb);
         ya *= sg;
         yb *= sg;
-        mfma_pair2(x0, ya, acc, x2, yb, acc2);  // acc += M0' ya, acc2 += M2' yb
+        mma_pair2<T>(x0, ya, acc, x2, yb, acc2);  // acc += M0' ya, acc2 += M2' yb
         if (p + 1 < rank) {
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
@@ -347,10 +396,10 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
       }
     }
   }
-  // combine the four waves' partial tiles (lane holds acc(g+4r, c))
+  // combine the four waves' partial tiles (lane holds acc(row(g,r), c)) in FP64
   acc += acc2;
 #pragma unroll
-  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + (g + 4 * rr) + TILE * c] = acc[rr];
+  for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + kk[rr] + TILE * c] = (double)acc[rr];
   __syncthreads();
   const int a = t & 15, b = t >> 4;
   if (a >= ni || b >= nj) return;
@@ -359,6 +408,119 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs A) {
   const double v = red[t] + red[TILE2 + t] + red[2 * TILE2 + t] + red[3 * TILE2 + t];
   const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
   const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+  double *dst = A.K + row + col * A.ldk;
+  if (srow || scol)
+    atomicAdd(dst, -v);
+  else
+    *dst = -v;
+}
+
+// --- In-element exchange as a dense operator ----------------------------------
+//
+// For one ((L,|M|), element e) the in-element exchange of every output block
+// coupled to it is Y(a,b) = sum_ch sum_cd T_ch((a,b),(c,d)) R_ch(c,d), R_ch the
+// Gaunt-weighted density tiles of that block: a GEMM Y (256 x runs) =
+// T (256 x 4 KC) X (4 KC x runs), run on the f64 matrix cores.  X is formed in
+// LDS one channel at a time from the screened terms.  Partial tiles go to
+// Ypart[run][e]; k_dia_kin_reduce sums them per output block.
+
+constexpr int NCOL = 32;  // runs (GEMM columns) per workgroup
+
+struct DKInArgs {
+  const double *P;
+  int64_t ldp;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const double *T;
+  const int *wg, *ilm_rstart, *ilm_runs, *lrun_tstart, *term_il;
+  const double *term_f;
+  double *Y;
+};
+
+__global__ void __launch_bounds__(NT) k_dia_kin_gemm(DKInArgs A) {
+  const int w = blockIdx.x;
+  const int ilm = A.wg[3 * w], e = A.wg[3 * w + 1], c0 = A.wg[3 * w + 2];
+  if (ilm < 0) return;
+  const int r0 = A.ilm_rstart[ilm] + c0, r1 = min(A.ilm_rstart[ilm + 1], r0 + NCOL);
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, g = lane >> 4, c = lane & 15;
+  const int n = A.ni[e], fe = A.first[e];
+  __shared__ double X[KC * NCOL];
+  __shared__ int srun[NCOL];
+  if (t < NCOL) srun[t] = r0 + t < r1 ? A.ilm_runs[r0 + t] : -1;
+  v4d acc[4][2];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) acc[rt][0] = acc[rt][1] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double *Tb = A.T + (size_t)(ilm * A.Nel + e) * 4 * KC * TILE2;
+  const int cc = t % 15, dd = t / 15;
+  const bool cdok = t < 225 && cc < n && dd < n;
+  for (int ch = 0; ch < 4; ++ch) {
+    __syncthreads();
+    if (t < KC)
+      for (int j = 0; j < NCOL; ++j) {
+        const int run = srun[j];
+        double v = 0.0;
+        if (run >= 0 && cdok)
+          for (int q = A.lrun_tstart[run]; q < A.lrun_tstart[run + 1]; ++q) {
+            const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+            const long long row = pure_row(A.skip, A.poff, i, fe + cc), col = pure_row(A.skip, A.poff, l, fe + dd);
+            if (row >= 0 && col >= 0) v += A.term_f[4 * q + ch] * A.P[row + col * A.ldp];
+          }
+        X[t * NCOL + j] = v;
+      }
+    __syncthreads();
+    const double *Tc = Tb + (size_t)ch * KC * TILE2 + 64 * wv + c;
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int k = 4 * ks + g;
+      const double b0 = X[k * NCOL + c], b1 = X[k * NCOL + 16 + c];
+      const double *Tk = Tc + (size_t)k * TILE2;
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const double a = Tk[16 * rt];
+        acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[rt][0], 0, 0, 0);
+        acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[rt][1], 0, 0, 0);
+      }
+    }
+  }
+  // lane (g, c) holds Y(64 wv + 16 rt + g + 4 r, 16 ct + c) in component r
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    const int run = srun[16 * ct + c];
+    if (run < 0) continue;
+    double *out = A.Y + ((size_t)run * A.Nel + e) * TILE2 + 64 * wv + g;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[16 * rt + 4 * r] = acc[rt][ct][r];
+  }
+}
+
+struct DKRedArgs {
+  double *K;
+  int64_t ldk;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const int *act_jk, *act_lstart;
+  const double *Y;
+};
+
+// One workgroup per (active output block, element): K tile = -sum over the
+// block's runs of the partial tiles, shared boundary rows/cols accumulated
+// atomically (the cross-element launch adds into the same entries).
+__global__ void __launch_bounds__(NT) k_dia_kin_reduce(DKRedArgs A) {
+  const int act = blockIdx.x / A.Nel, e = blockIdx.x - act * A.Nel;
+  const int t = threadIdx.x, a = t & 15, b = t >> 4;
+  const int n = A.ni[e], fe = A.first[e];
+  if (a >= n || b >= n) return;
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  const long long row = pure_row(A.skip, A.poff, j, fe + a), col = pure_row(A.skip, A.poff, k, fe + b);
+  if (row < 0 || col < 0) return;
+  double v = 0.0;
+  for (int run = A.act_lstart[act]; run < A.act_lstart[act + 1]; ++run)
+    v += A.Y[((size_t)run * A.Nel + e) * TILE2 + t];
+  const bool srow = (a == 0 && e > 0) || (a == n - 1 && e + 1 < A.Nel);
+  const bool scol = (b == 0 && e > 0) || (b == n - 1 && e + 1 < A.Nel);
   double *dst = A.K + row + col * A.ldk;
   if (srow || scol)
     atomicAdd(dst, -v);
@@ -412,28 +574,71 @@ void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64
   hip_check(hipGetLastError(), "diatomic_block_norms");
 }
 
-void diatomic_exchange_kernels(const DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                               hipStream_t s, KernelTimer *timer) {
+void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer, bool fp32) {
   fill_zero_launch(K, d.Nbf, d.Nbf, ldk, s);
   if (d.n_act == 0) return;
+  if (fp32) {
+    if (!d.f32_tables) {
+      d.pq32.alloc(d.pq.n);
+      d.chol32.alloc(d.chol.n);
+      d.sigma32.alloc(d.sigma.n);
+      to_f32_launch(d.pq.p, d.pq32.p, d.pq.n, s);
+      to_f32_launch(d.chol.p, d.chol32.p, d.chol.n, s);
+      to_f32_launch(d.sigma.p, d.sigma32.p, d.sigma.n, s);
+      d.f32_tables = true;
+    }
+    if (d.term_f32_gen != d.list_gen) {
+      d.term_f32.alloc(d.term_f.n);
+      to_f32_launch(d.term_f.p, d.term_f32.p, d.term_f.n, s);
+      d.term_f32_gen = d.list_gen;
+    }
+    d.P32.alloc((size_t)d.Nbf * d.Nbf);
+    mat_to_f32_launch(P, ldp, d.P32.p, d.Nbf, s);
+  }
   // two launches: element pairs e != f (disjoint P/Q factors) and the
   // in-element tiles (Cholesky factor); boundary rows/cols shared between
   // neighbouring tiles are accumulated atomically in both
   for (int part = 0; part < 2; ++part) {
     const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
     if (n_ef == 0) continue;
+    const char *name = part == 0 ? (fp32 ? "diatomic_k_off_f32" : "diatomic_k_off")
+                                 : (fp32 ? "diatomic_k_in_f32" : "diatomic_k_in");
+    const double flops = part == 0 ? d.flops_off : d.flops_in;
+    if (part == 1 && d.dense_in && !fp32) {
+      DKInArgs g{P, ldp, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.Top.p, d.wg_in.p,
+                 d.ilm_rstart.p, d.ilm_runs.p, d.lrun_tstart.p, d.term_il.p, d.term_f.p, d.Ypart.p};
+      DKRedArgs r{K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.act_jk.p, d.act_lstart.p,
+                  d.Ypart.p};
+      TimedLaunch tl(timer, name, s, flops);
+      hipLaunchKernelGGL(k_dia_kin_gemm, dim3(d.n_wg_in), dim3(NT), 0, s, g);
+      hip_check(hipGetLastError(), "diatomic_kin_gemm");
+      hipLaunchKernelGGL(k_dia_kin_reduce, dim3(d.n_act * d.Nel), dim3(NT), 0, s, r);
+      hip_check(hipGetLastError(), "diatomic_kin_reduce");
+      continue;
+    }
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;
-    DKArgs a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p, d.chol.p,
-             d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p, d.lrun_tstart.p,
-             d.term_il.p, d.term_f.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef};
     const int64_t grid = (int64_t)n_ef_pad * d.n_act;
     if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-    TimedLaunch tl(timer, part == 0 ? "diatomic_k_off" : "diatomic_k_in", s,
-                   part == 0 ? d.flops_off : d.flops_in);
-    if (part == 0)
-      hipLaunchKernelGGL(k_dia_k<false>, dim3((unsigned)grid), dim3(NT), 0, s, a);
-    else
-      hipLaunchKernelGGL(k_dia_k<true>, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    const int *efl = part == 0 ? d.ef_off.p : d.ef_in.p;
+    TimedLaunch tl(timer, name, s, flops);
+    if (fp32) {
+      DKArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p,
+                      d.pq32.p, d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p,
+                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_il.p, d.term_f32.p, efl, n_ef};
+      if (part == 0)
+        hipLaunchKernelGGL((k_dia_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_dia_k<true, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+    } else {
+      DKArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
+                       d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p,
+                       d.lrun_tstart.p, d.term_il.p, d.term_f.p, efl, n_ef};
+      if (part == 0)
+        hipLaunchKernelGGL((k_dia_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_dia_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+    }
     hip_check(hipGetLastError(), "diatomic_k");
   }
   if (d.n_mirror > 0) {

@@ -15,6 +15,7 @@ namespace hfg {
 // functions), the MFMA f64 tile edge on CDNA.
 constexpr int TILE = 16;
 constexpr int TILE2 = TILE * TILE;
+constexpr int KC = 228;  // operator extent per channel: c + 15 d for c, d < 15, padded to 4
 
 struct Error : std::runtime_error {
   int code;
@@ -164,6 +165,20 @@ struct AtomicDevice {
   // algorithmic work per launch (flops), for the roofline report
   double flops_k_radial = 0.0, flops_k_angular = 0.0, flops_j = 0.0;
   double flops_in = 0.0, flops_off = 0.0;
+  // In-element exchange as one dense operator per (L, element):
+  // T[k = c + 15 d][row = a + 16 b] = (L L')(a + c n, b + d n), applied to
+  // the density tiles of many (output block, L) runs at once.
+  bool dense_in = false;
+  DevArray<double> Top;                   // [N_L*Nel][KC][TILE2]
+  DevArray<int> run_q, run_L, act_rstart; // runs (act, L) with terms, act-major
+  DevArray<int> L_rstart, L_runs;         // runs grouped by L
+  DevArray<int> wg_in;                    // per workgroup (L, e, first run in group) or -1
+  DevArray<double> Ypart;                 // [n_runs][Nel][TILE2]
+  // FP32 copies for the reduced-precision exchange (built on first use)
+  DevArray<float> rs32, rb32, chol32, term_c32, P32;
+  bool f32_tables = false;
+  int64_t list_gen = 0, term_c32_gen = -1; // exchange-list version, and the one term_c32 holds
+  int n_wg_in = 0, n_runs = 0;
 };
 
 // Everything the diatomic J/K kernels read.  Matrices are in the pure basis
@@ -205,7 +220,23 @@ struct DiatomicDevice {
   int64_t n_terms = 0, n_runs = 0, n_pairs = 0;
   double flops_k_radial = 0.0, flops_k_angular = 0.0;
   double flops_in = 0.0, flops_off = 0.0;  // split of the two above between the launches
+  // In-element exchange as one dense operator per ((L,|M|), element):
+  // T[k = ch*KC + c + 15 d][row = a + 16 b] = sign_ch W~_ch(c + a n, b + d n),
+  // W~ = B diag(sigma) B' (the factorisation the reference contracts),
+  // applied to the Gaunt-weighted density tiles of many output blocks at once.
+  bool dense_in = false;
+  DevArray<double> Top;                   // [n_lm*Nel][4*KC][TILE2]
+  DevArray<int> ilm_rstart, ilm_runs;     // runs grouped by (L,|M|)
+  DevArray<int> wg_in;                    // per workgroup (ilm, e, first run in group) or -1
+  DevArray<double> Ypart;                 // [n_runs][Nel][TILE2] in-element partial tiles
+  // FP32 copies for the reduced-precision exchange (built on first use)
+  DevArray<float> pq32, chol32, sigma32, term_f32, P32;
+  bool f32_tables = false;
+  int64_t list_gen = 0, term_f32_gen = -1; // exchange-list version, and the one term_f32 holds
+  int n_wg_in = 0;
 };
+
+
 
 // DFT grid of a two-dimensional basis, evaluated by sum factorisation:
 // basis function (a, r) at point (i, t, k) is R_r(i) Theta_a(t) e^(i m_a phi_k).
@@ -217,6 +248,7 @@ struct DiatomicDevice {
 struct XCGridDevice {
   int Nel = 0, Nrad = 0, Nang = 0, nrad = 0, lang = 0, mang = 0, dmax = 0, npair = 0;
   int64_t Nbf = 0;
+  bool lds_pol_ok = false;   // the polarized point kernel's LDS fits
   DevArray<int> first, ni, skip;
   DevArray<long long> poff;
   DevArray<double> R;        // [Nel][nrad][TILE]
@@ -226,6 +258,7 @@ struct XCGridDevice {
   DevArray<int> rstart, rb, rp;       // pairs of row a: b and pair index
   DevArray<int> dstart, da, db, dp;   // pairs bucketed by d = m_a - m_b + dmax
   DevArray<double> D, W;     // [Nel][npair][nrad]
+  DevArray<double> Db, Wb;   // second spin of a polarized evaluation
   DevArray<double> part;     // [Nel*nrad][2]: Exc and Nel partial sums
   PinnedArray<double> h_part;
   DevArray<double> hostP, hostOut;
@@ -234,19 +267,30 @@ struct XCGridDevice {
 // launchers (atomic_jk.hip)
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                            hipStream_t s, KernelTimer *timer);
+// fp32: gathers and matrix products in FP32 (the result stays FP64, with
+// about 1e-6 relative error); otherwise the reference's FP64 arithmetic.
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                            hipStream_t s, KernelTimer *timer);
+                            hipStream_t s, KernelTimer *timer, bool fp32 = false);
+// Dense FP64 -> FP32 conversions (column-major with leading dimensions).
+void to_f32_launch(const double *src, float *dst, size_t n, hipStream_t s);
+void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStream_t s);
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
+void atomic_build_operator(const AtomicSetup &S, AtomicDevice &d, hipStream_t s);
 
 // launchers (xc_grid.hip); exc/nel summed on the host from h_part
-void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P, int64_t ldp, double *H,
-                      int64_t ldh, hipStream_t s, KernelTimer *timer);
+// Pb == nullptr: restricted (P the total density, H the potential); else
+// spin-polarized with Pa, Pb in and Ha, Hb out.
+void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *Pa, int64_t ldpa, const double *Pb,
+                    int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
+                    KernelTimer *timer);
 
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                              hipStream_t s, KernelTimer *timer);
 void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s);
-void diatomic_exchange_kernels(const DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                               hipStream_t s, KernelTimer *timer);
+// fp32 as for atomic_exchange_launch.
+void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer, bool fp32 = false);
+void diatomic_build_operator(const DiatomicSetup &S, DiatomicDevice &d, hipStream_t s);
 
 }  // namespace hfg

@@ -54,6 +54,56 @@ __device__ inline void lda_point(int xf, int cf, double rho, double thr, double 
   }
 }
 
+// Spin-polarized form: Slater exchange by spin scaling, VWN5 correlation
+// e_c = e_P + alpha_c f(z)(1 - z^4)/f''(0) + (e_F - e_P) f(z) z^4 with
+// v_cs = e_c - (rs/3) de/drs - (z - s) de/dz (libxc lda_c_vwn).
+__device__ inline void vwn_aux(double A, double b, double c, double x0, double x, double &F, double &dF) {
+  const double Q = sqrt(4.0 * c - b * b);
+  const double X = x * x + b * x + c, Xp = 2.0 * x + b, X0 = x0 * x0 + b * x0 + c;
+  const double f1 = 2.0 * b / Q, f2 = b * x0 / X0, f3 = 2.0 * (2.0 * x0 + b) / Q;
+  const double at = atan(Q / (2.0 * x + b));
+  F = A * (log(x * x / X) + (f1 - f2 * f3) * at - f2 * log((x - x0) * (x - x0) / X));
+  dF = A * (2.0 / x - Xp / X + (f1 - f2 * f3) * (-Q / (2.0 * X)) - f2 * (2.0 / (x - x0) - Xp / X));
+}
+
+__device__ inline void lda_point_pol(int xf, int cf, double ra, double rb, double thr, double &exc, double &va,
+                                     double &vb) {
+  exc = 0.0;
+  va = 0.0;
+  vb = 0.0;
+  const double rho = ra + rb;
+  if (!(rho >= thr)) return;
+  const double pi = 3.14159265358979323846;
+  if (xf == 1) {
+    const double c6 = cbrt(6.0 / pi);
+    const double a3 = ra > 0.0 ? cbrt(ra) : 0.0, b3 = rb > 0.0 ? cbrt(rb) : 0.0;
+    exc += -0.75 * c6 * (ra * a3 + rb * b3) / rho;
+    va += -c6 * a3;
+    vb += -c6 * b3;
+  }
+  if (cf == 7) {
+    const double rs = cbrt(3.0 / (4.0 * pi * rho)), x = sqrt(rs);
+    const double z = fmax(-1.0, fmin(1.0, (ra - rb) / rho));
+    double eP, dP, eF, dF, eA, dA;
+    vwn_aux(0.0310907, 3.72744, 12.9352, -0.10498, x, eP, dP);
+    vwn_aux(0.01554535, 7.06042, 18.0578, -0.32500, x, eF, dF);
+    vwn_aux(-1.0 / (6.0 * pi * pi), 1.13107, 13.0045, -0.0047584, x, eA, dA);
+    const double c43 = cbrt(2.0) * 2.0 - 2.0;
+    const double opz = 1.0 + z, omz = 1.0 - z, opz13 = cbrt(opz), omz13 = cbrt(omz);
+    const double fz = (opz * opz13 + omz * omz13 - 2.0) / c43;
+    const double dfz = 4.0 / 3.0 * (opz13 - omz13) / c43;
+    const double fpp = 4.0 / (9.0 * (cbrt(2.0) - 1.0));
+    const double z3 = z * z * z, z4 = z3 * z;
+    const double e = eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4;
+    const double dedx = dP + dA * fz * (1.0 - z4) / fpp + (dF - dP) * fz * z4;
+    const double dedz = eA * (dfz * (1.0 - z4) - 4.0 * z3 * fz) / fpp + (eF - eP) * (dfz * z4 + 4.0 * z3 * fz);
+    const double base = e - x / 6.0 * dedx;
+    exc += e;
+    va += base - (z - 1.0) * dedz;
+    vb += base - (z + 1.0) * dedz;
+  }
+}
+
 __device__ inline long long grid_row(const int *skip, const long long *poff, int shell, int r) {
   const int s = skip[shell];
   return r >= s ? poff[shell] + (r - s) : -1;
@@ -71,6 +121,7 @@ struct XCArgs {
   const int *rstart, *rb, *rp, *dstart, *da, *db, *dp;
   const double *R, *wpt, *th, *cs;
   double *D, *W, *part;
+  double *Db, *Wb;  // second spin (polarized evaluation), else null
 };
 
 // One workgroup per (element, shell a); the waves take the pairs (a, b) of
@@ -116,24 +167,32 @@ __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
   }
 }
 
-// One workgroup per (element, radial point).
+// One workgroup per (element, radial point).  With A.Db set the density of
+// the second spin is contracted alongside and the polarized functional used.
 __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   const int e = blockIdx.x / A.nrad, i = blockIdx.x - e * A.nrad;
   const int t = threadIdx.x;
   const int Na = A.Nang, nd = A.nd, nang = A.lang * A.mang;
+  const bool pol = A.Db != nullptr;
+  const int ns = pol ? 2 : 1;
   extern __shared__ double sm[];
-  double *sD = sm;                      // [npair]
-  double *sC = sD + A.npair;            // [lang][nd]
-  double *sV = sC + A.lang * nd;        // [lang][mang]
+  double *sD = sm;                          // [ns][npair]
+  double *sC = sD + ns * A.npair;           // [ns][lang][nd]
+  double *sV = sC + ns * A.lang * nd;       // [ns][lang][mang]
   __shared__ double red[8];
   const size_t eb = (size_t)e * A.npair * A.nrad;
-  for (int p = t; p < A.npair; p += NT) sD[p] = A.D[eb + (size_t)p * A.nrad + i];
+  for (int p = t; p < A.npair; p += NT) {
+    sD[p] = A.D[eb + (size_t)p * A.nrad + i];
+    if (pol) sD[A.npair + p] = A.Db[eb + (size_t)p * A.nrad + i];
+  }
   __syncthreads();
-  for (int q = t; q < A.lang * nd; q += NT) {
-    const int tt = q / nd, d = q - tt * nd;
+  for (int q = t; q < ns * A.lang * nd; q += NT) {
+    const int sp = q / (A.lang * nd), qq = q - sp * A.lang * nd;
+    const int tt = qq / nd, d = qq - tt * nd;
     const double *th = A.th + (size_t)tt * Na;
+    const double *D = sD + sp * A.npair;
     double c = 0.0;
-    for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) c += th[A.da[x]] * th[A.db[x]] * sD[A.dp[x]];
+    for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) c += th[A.da[x]] * th[A.db[x]] * D[A.dp[x]];
     sC[q] = c;
   }
   __syncthreads();
@@ -141,14 +200,20 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   double ex = 0.0, ne = 0.0;
   for (int q = t; q < nang; q += NT) {
     const int tt = q / A.mang, k = q - tt * A.mang;
-    double rho = 0.0;
-    for (int d = 0; d < nd; ++d) rho += sC[tt * nd + d] * A.cs[(size_t)d * A.mang + k];
-    double exc, v;
-    lda_point(A.xf, A.cf, rho, A.thr, exc, v);
+    double ra = 0.0, rb = 0.0;
+    for (int d = 0; d < nd; ++d) ra += sC[tt * nd + d] * A.cs[(size_t)d * A.mang + k];
+    if (pol)
+      for (int d = 0; d < nd; ++d) rb += sC[A.lang * nd + tt * nd + d] * A.cs[(size_t)d * A.mang + k];
     const double w = wp[tt];
-    ex += w * exc * rho;
-    ne += w * rho;
-    sV[q] = w * v;
+    double exc, va, vb = 0.0;
+    if (pol)
+      lda_point_pol(A.xf, A.cf, ra, rb, A.thr, exc, va, vb);
+    else
+      lda_point(A.xf, A.cf, ra, A.thr, exc, va);
+    ex += w * exc * (ra + rb);
+    ne += w * (ra + rb);
+    sV[q] = w * va;
+    if (pol) sV[nang + q] = w * vb;
   }
   block_sum2(ex, ne, red);
   if (t == 0) {
@@ -156,20 +221,26 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
     A.part[2 * ((size_t)e * A.nrad + i) + 1] = ne;
   }
   __syncthreads();
-  for (int q = t; q < A.lang * nd; q += NT) {
-    const int tt = q / nd, d = q - tt * nd;
+  for (int q = t; q < ns * A.lang * nd; q += NT) {
+    const int sp = q / (A.lang * nd), qq = q - sp * A.lang * nd;
+    const int tt = qq / nd, d = qq - tt * nd;
+    const double *V = sV + sp * nang;
     double s = 0.0;
-    for (int k = 0; k < A.mang; ++k) s += sV[tt * A.mang + k] * A.cs[(size_t)d * A.mang + k];
+    for (int k = 0; k < A.mang; ++k) s += V[tt * A.mang + k] * A.cs[(size_t)d * A.mang + k];
     sC[q] = s;
   }
   __syncthreads();
-  for (int d = 0; d < nd; ++d)
-    for (int x = A.dstart[d] + t; x < A.dstart[d + 1]; x += NT) {
-      const int a = A.da[x], b = A.db[x];
-      double w = 0.0;
-      for (int tt = 0; tt < A.lang; ++tt) w += A.th[(size_t)tt * Na + a] * A.th[(size_t)tt * Na + b] * sC[tt * nd + d];
-      A.W[eb + (size_t)A.dp[x] * A.nrad + i] = w;
-    }
+  for (int sp = 0; sp < ns; ++sp) {
+    double *Wout = sp == 0 ? A.W : A.Wb;
+    const double *C = sC + sp * A.lang * nd;
+    for (int d = 0; d < nd; ++d)
+      for (int x = A.dstart[d] + t; x < A.dstart[d + 1]; x += NT) {
+        const int a = A.da[x], b = A.db[x];
+        double w = 0.0;
+        for (int tt = 0; tt < A.lang; ++tt) w += A.th[(size_t)tt * Na + a] * A.th[(size_t)tt * Na + b] * C[tt * nd + d];
+        Wout[eb + (size_t)A.dp[x] * A.nrad + i] = w;
+      }
+  }
 }
 
 // One workgroup per (element, shell a); thread (r,s) of the element band,
@@ -211,19 +282,34 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
 
 }  // namespace
 
-void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P, int64_t ldp, double *H,
-                    int64_t ldh, hipStream_t s, KernelTimer *timer) {
-  fill_zero_launch(H, g.Nbf, g.Nbf, ldh, s);
+void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *Pa, int64_t ldpa, const double *Pb,
+                    int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
+                    KernelTimer *timer) {
+  const bool pol = Pb != nullptr;
+  fill_zero_launch(Ha, g.Nbf, g.Nbf, ldha, s);
+  if (pol) fill_zero_launch(Hb, g.Nbf, g.Nbf, ldhb, s);
   const int nd = 2 * g.dmax + 1;
-  XCArgs a{P, ldp, H, ldh, g.Nel, g.Nang, g.nrad, g.lang, g.mang, nd, xf, cf, g.npair, thr,
+  XCArgs a{Pa, ldpa, Ha, ldha, g.Nel, g.Nang, g.nrad, g.lang, g.mang, nd, xf, cf, g.npair, thr,
            g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rp.p, g.dstart.p, g.da.p, g.db.p, g.dp.p,
-           g.R.p, g.wpt.p, g.th.p, g.cs.p, g.D.p, g.W.p, g.part.p};
+           g.R.p, g.wpt.p, g.th.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr, pol ? g.Wb.p : nullptr};
+  XCArgs b = a;  // second spin: its density in, its matrix out
+  if (pol) {
+    b.P = Pb;
+    b.ldp = ldpb;
+    b.H = Hb;
+    b.ldh = ldhb;
+    b.D = g.Db.p;
+    b.W = g.Wb.p;
+  }
+  const int ns = pol ? 2 : 1;
   const size_t sh_in = sizeof(double) * ((size_t)g.nrad * TILE + 4 * TILE2);
-  const size_t sh_pt = sizeof(double) * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
+  const size_t sh_pt =
+      sizeof(double) * ns * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
   const size_t sh_out = sizeof(double) * ((size_t)g.nrad * TILE + g.nrad);
   {
     TimedLaunch tl(timer, "xc_rad_in", s);
     hipLaunchKernelGGL(k_xc_rad_in, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
+    if (pol) hipLaunchKernelGGL(k_xc_rad_in, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
     hip_check(hipGetLastError(), "xc_rad_in");
   }
   {
@@ -234,6 +320,7 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
   {
     TimedLaunch tl(timer, "xc_rad_out", s);
     hipLaunchKernelGGL(k_xc_rad_out, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
+    if (pol) hipLaunchKernelGGL(k_xc_rad_out, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
     hip_check(hipGetLastError(), "xc_rad_out");
   }
 }

@@ -6,7 +6,8 @@ src/diatomic/dftgrid_purem.h), chosen by the basis type.
 `DFTGrid(basis, lang, mang).eval_Fxc(x_func, c_func, P)` returns the
 restricted exchange-correlation matrix with Exc, the integrated electron
 count and the kinetic-energy-density integral, like DFTGrid::eval_Fxc
-(dftgrid.cpp:591-653).  Functionals are libxc ids; parse_method turns the
+(dftgrid.cpp:591-653); `eval_Fxc_pol(x_func, c_func, Pa, Pb)` is the
+unrestricted overload (dftgrid.cpp:655-712) returning (Ha, Hb, Exc, Nel, Ekin).  Functionals are libxc ids; parse_method turns the
 driver's --method string (e.g. "lda_x-lda_c_vwn") into them.
 """
 from __future__ import annotations
@@ -78,3 +79,32 @@ class DFTGrid:
         check(lib().hf_dftgrid_eval_fxc_dev(self._h, x_func, c_func, P.data_ptr(), n, H.data_ptr(), n, thr,
                                             res.ctypes.data))
         return H, float(res[0]), float(res[1]), float(res[2])
+
+    def eval_Fxc_pol(self, x_func: int, c_func: int, Pa, Pb, beta: bool = True, thr: float = 1e-12):
+        """(Ha, Hb, Exc, Nel, Ekin) for spin densities Pa, Pb (numpy or float64
+        CUDA tensors); Hb is zero when beta is False (no beta electrons)."""
+        n = self.basis.Nbf
+        res = np.zeros(3)
+        if isinstance(Pa, np.ndarray):
+            Pa, Pb = _f64_fortran(Pa), _f64_fortran(Pb)
+            if Pa.shape != (n, n) or Pb.shape != (n, n):
+                raise ValueError(f"Pa and Pb must be {n}x{n}")
+            Ha = np.zeros((n, n), order="F")
+            Hb = np.zeros((n, n), order="F")
+            check(lib().hf_dftgrid_eval_fxc_pol(self._h, x_func, c_func, Pa.ctypes.data, n, Pb.ctypes.data, n,
+                                                Ha.ctypes.data, n, Hb.ctypes.data, n, int(beta), thr,
+                                                res.ctypes.data))
+            return Ha, Hb, float(res[0]), float(res[1]), float(res[2])
+        import torch
+        for name, M in (("Pa", Pa), ("Pb", Pb)):
+            if not (isinstance(M, torch.Tensor) and M.is_cuda and M.dtype == torch.float64):
+                raise TypeError(f"{name} must be a numpy array or a float64 CUDA tensor")
+            if M.shape != (n, n) or not M.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous {n}x{n} tensor")
+        self.basis.ctx.bind_torch_stream()
+        Ha = torch.empty_like(Pa)
+        Hb = torch.empty_like(Pb)
+        check(lib().hf_dftgrid_eval_fxc_pol_dev(self._h, x_func, c_func, Pa.data_ptr(), n, Pb.data_ptr(), n,
+                                                Ha.data_ptr(), n, Hb.data_ptr(), n, int(beta), thr,
+                                                res.ctypes.data))
+        return Ha, Hb, float(res[0]), float(res[1]), float(res[2])

@@ -130,6 +130,8 @@ class DiatomicBasis:
     # --- Fock build ------------------------------------------------------
     def _apply(self, P, host_fn, dev_fn, out=None):
         if isinstance(P, np.ndarray):
+            if host_fn is None:
+                raise TypeError("this form takes a float64 CUDA tensor")
             P = _f64_fortran(P)
             if P.shape != (self.Nbf, self.Nbf):
                 raise ValueError(f"P must be {self.Nbf}x{self.Nbf}")
@@ -152,6 +154,13 @@ class DiatomicBasis:
         """J(P), basis.cpp:1627-1816."""
         return self._apply(P, lib().hf_diatomic_coulomb, lib().hf_diatomic_coulomb_dev, out)
 
-    def exchange(self, P, out=None):
-        """-K(P) for a spin density P, basis.cpp:1818-2089."""
+    def exchange(self, P, out=None, precision: str = "fp64"):
+        """-K(P) for a spin density P, basis.cpp:1818-2089.
+
+        precision="fp32" runs the gathers and matrix products in FP32 (device
+        tensors only; about 1e-6 relative error), for early SCF iterations."""
+        if precision == "fp32":
+            return self._apply(P, None, lib().hf_diatomic_exchange_fp32_dev, out)
+        if precision != "fp64":
+            raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
         return self._apply(P, lib().hf_diatomic_exchange, lib().hf_diatomic_exchange_dev, out)

@@ -129,6 +129,13 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
 /* Device-pointer forms of the two above, asynchronous on the context stream. */
 int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+/* Reduced-precision form of hf_atomic_exchange_dev for the early iterations
+ * of a mixed-precision SCF: the density, the Gaunt weights and the radial
+ * factors are rounded to FP32, gathered and multiplied on the FP32 matrix
+ * cores, and the partial tiles summed in FP64; the result is FP64 with a
+ * relative error of about 1e-6.  Same screening and output as the FP64 form
+ * of TwoDBasisT::exchange (TwoDBasis.cpp:880-999). */
+int hf_atomic_exchange_fp32_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
 
 /* ------------------------------------------------------- atomic DFT grid */
 
@@ -159,6 +166,17 @@ int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out);
  * parameters. */
 int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H,
                         int64_t ldh, double thr, double out[3]);
+/* Spin-polarized form: densities Pa, Pb in, potentials Ha, Hb out (Hb zeroed
+ * when beta == 0, i.e. no beta electrons), out as above.  Replaces the
+ * unrestricted DFTGrid::eval_Fxc (src/atomic/dftgrid.h:159,
+ * dftgrid.cpp:655-712) and PureMDFTGrid::eval_Fxc (dftgrid_purem.h:158,
+ * dftgrid_purem.cpp:705-740). */
+int hf_dftgrid_eval_fxc_pol(hf_dftgrid *g, int x_func, int c_func, const double *Pa, int64_t ldpa, const double *Pb,
+                            int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, int beta, double thr,
+                            double out[3]);
+int hf_dftgrid_eval_fxc_pol_dev(hf_dftgrid *g, int x_func, int c_func, const double *dPa, int64_t ldpa,
+                                const double *dPb, int64_t ldpb, double *dHa, int64_t ldha, double *dHb, int64_t ldhb,
+                                int beta, double thr, double out[3]);
 /* Device-pointer form; synchronises the context stream to return out. */
 int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double *dP, int64_t ldp, double *dH,
                             int64_t ldh, double thr, double out[3]);
@@ -216,6 +234,9 @@ int hf_diatomic_exchange(hf_diatomic *h, const double *P, int64_t ldp, double *K
 /* Device-pointer forms, asynchronous on the context stream. */
 int hf_diatomic_coulomb_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 int hf_diatomic_exchange_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+/* Reduced-precision form, as hf_atomic_exchange_fp32_dev, of
+ * TwoDBasis::exchange (basis.cpp:1818-2089). */
+int hf_diatomic_exchange_fp32_dev(hf_diatomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
 
 #ifdef __cplusplus
 }

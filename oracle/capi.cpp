@@ -247,6 +247,12 @@ int orc_lda(int xf, int cf, int n, const double *rho, double thr, double *exc, d
     for (int i = 0; i < n; ++i) lda_unpolarized(xf, cf, rho[i], thr, exc[i], vrho[i]);
   });
 }
+int orc_lda_pol(int xf, int cf, int n, const double *ra, const double *rb, double thr, double *exc, double *va,
+                double *vb) {
+  return guard([&] {
+    for (int i = 0; i < n; ++i) lda_polarized(xf, cf, ra[i], rb[i], thr, exc[i], va[i], vb[i]);
+  });
+}
 // Atomic DFT grid: H (Nbf x Nbf) and out = (Exc, Nel)
 int orc_atomic_xc(void *h, int lang, int mang, int xf, int cf, const double *P, double thr, double *H, double *out) {
   return guard([&] {
@@ -265,6 +271,29 @@ int orc_diatomic_xc(void *h, int lang, int xf, int cf, const double *P, double t
     Mat Hm;
     g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
     put(Hm, H);
+  });
+}
+// Spin-polarized grids: Ha, Hb and out = (Exc, Nel)
+int orc_atomic_xc_pol(void *h, int lang, int mang, int xf, int cf, const double *Pa, const double *Pb, double thr,
+                      double *Ha, double *Hb, double *out) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    AtomicXC g(*b, lang, mang);
+    Mat A, B;
+    g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    put(A, Ha);
+    put(B, Hb);
+  });
+}
+int orc_diatomic_xc_pol(void *h, int lang, int xf, int cf, const double *Pa, const double *Pb, double thr, double *Ha,
+                        double *Hb, double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    DiatomicXC g(*b, lang);
+    Mat A, B;
+    g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    put(A, Ha);
+    put(B, Hb);
   });
 }
 // lm map entry i: out = (L, |M|); LM map similarly (L, M)

@@ -60,6 +60,53 @@ void lda_unpolarized(int xf, int cf, double rho, double thr, double &exc, double
   }
 }
 
+// Spin-polarized form (libxc's XC_POLARIZED): e per particle of the total
+// density and the two potentials.  Slater exchange by spin scaling,
+// e_x rho = -(3/4)(6/pi)^(1/3) (rho_a^(4/3) + rho_b^(4/3)), v_xs = -(6/pi)^(1/3) rho_s^(1/3);
+// VWN5 correlation e_c = e_P + alpha_c f(z)(1 - z^4)/f''(0) + (e_F - e_P) f(z) z^4
+// (libxc maple/lda_exc/lda_c_vwn.mpl), v_cs = e_c - (rs/3) de/drs - (z - s) de/dz.
+void lda_polarized(int xf, int cf, double ra, double rb, double thr, double &exc, double &va, double &vb) {
+  exc = 0.0;
+  va = 0.0;
+  vb = 0.0;
+  const double rho = ra + rb;
+  if (!(rho >= thr)) return;
+  if (xf == 1) {
+    const double c6 = std::cbrt(6.0 / kPi);
+    const double a3 = ra > 0.0 ? std::cbrt(ra) : 0.0, b3 = rb > 0.0 ? std::cbrt(rb) : 0.0;
+    exc += -0.75 * c6 * (ra * a3 + rb * b3) / rho;
+    va += -c6 * a3;
+    vb += -c6 * b3;
+  } else if (xf != 0) {
+    throw std::invalid_argument("unsupported exchange functional");
+  }
+  if (cf == 7) {
+    const double rs = std::cbrt(3.0 / (4.0 * kPi * rho)), x = std::sqrt(rs);
+    double z = (ra - rb) / rho;
+    z = std::max(-1.0, std::min(1.0, z));
+    double eP, dP, eF, dF, eA, dA;
+    vwn_aux(0.0310907, 3.72744, 12.9352, -0.10498, x, eP, dP);
+    vwn_aux(0.01554535, 7.06042, 18.0578, -0.32500, x, eF, dF);
+    vwn_aux(-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584, x, eA, dA);
+    const double c43 = std::cbrt(2.0) * 2.0 - 2.0;  // 2^(4/3) - 2
+    const double opz = 1.0 + z, omz = 1.0 - z;
+    const double opz13 = std::cbrt(opz), omz13 = std::cbrt(omz);
+    const double fz = (opz * opz13 + omz * omz13 - 2.0) / c43;
+    const double dfz = 4.0 / 3.0 * (opz13 - omz13) / c43;
+    const double fpp = 4.0 / (9.0 * (std::cbrt(2.0) - 1.0));
+    const double z3 = z * z * z, z4 = z3 * z;
+    const double e = eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4;
+    const double dedx = dP + dA * fz * (1.0 - z4) / fpp + (dF - dP) * fz * z4;
+    const double dedz = eA * (dfz * (1.0 - z4) - 4.0 * z3 * fz) / fpp + (eF - eP) * (dfz * z4 + 4.0 * z3 * fz);
+    const double base = e - x / 6.0 * dedx;
+    exc += e;
+    va += base - (z - 1.0) * dedz;
+    vb += base - (z + 1.0) * dedz;
+  } else if (cf != 0) {
+    throw std::invalid_argument("unsupported correlation functional");
+  }
+}
+
 // angular_chebyshev (src/general/angular.cpp:20-72): Chebyshev rule in
 // cos(theta) times an even phi grid, w = w_cth * 2pi/nphi.
 void angular_chebyshev(int l, int m, std::vector<double> &cth, std::vector<double> &phi, std::vector<double> &w) {
@@ -93,9 +140,24 @@ AtomicXC::AtomicXC(const AtomicBasis &b_, int lang_, int mang_) : b(b_), lang(la
 
 // DFTGrid::eval_Fxc (restricted), src/atomic/dftgrid.cpp:591-653
 void AtomicXC::eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const {
+  Mat Hb;
+  eval(xf, cf, P, nullptr, H, Hb, Exc, Nel, thr);
+}
+
+// DFTGrid::eval_Fxc (spin-polarized), src/atomic/dftgrid.cpp:655-712
+void AtomicXC::eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, Mat &Hb, double &Exc, double &Nel,
+                        double thr) const {
+  eval(xf, cf, Pa, &Pb, Ha, Hb, Exc, Nel, thr);
+}
+
+// Shared element loop; Pb == nullptr selects the restricted form with P the total density.
+void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &Hb, double &Exc, double &Nel,
+                    double thr) const {
+  const bool pol = Pbp != nullptr;
   const RadialBasis &rad = b.radial;
   const int Na = b.Nang(), Nr = rad.Nbf();
   H = Mat(P.r, P.c);
+  if (pol) Hb = Mat(P.r, P.c);
   Exc = 0.0;
   Nel = 0.0;
   const int nang = (int)wang.size();
@@ -121,37 +183,54 @@ void AtomicXC::eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, doubl
             aim(a * Ni + j, ia * nrad + ir) = y.imag() * R(ir, j);
           }
       }
-    // update_density: rho = colwise (P a).a + (P b).b
-    Mat Pe(nbf, nbf);
-    for (int v = 0; v < nbf; ++v)
-      for (int u = 0; u < nbf; ++u) Pe(u, v) = P(idx[u], idx[v]);
-    const Mat PA = matmul(Pe, are), PB = matmul(Pe, aim);
-    std::vector<double> rho(npt), exc(npt), vrho(npt);
-    for (int p = 0; p < npt; ++p) {
-      double s = 0.0, t = 0.0;
-      for (int u = 0; u < nbf; ++u) {
-        s += PA(u, p) * are(u, p);
-        t += PB(u, p) * aim(u, p);
+    // update_density: rho = colwise (P a).a + (P b).b, per spin when polarized
+    auto density = [&](const Mat &Pm) {
+      Mat Pe(nbf, nbf);
+      for (int v = 0; v < nbf; ++v)
+        for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
+      const Mat PA = matmul(Pe, are), PB = matmul(Pe, aim);
+      std::vector<double> rho(npt);
+      for (int p = 0; p < npt; ++p) {
+        double s = 0.0, t = 0.0;
+        for (int u = 0; u < nbf; ++u) {
+          s += PA(u, p) * are(u, p);
+          t += PB(u, p) * aim(u, p);
+        }
+        rho[p] = s + t;
       }
-      rho[p] = s + t;
-    }
+      return rho;
+    };
+    const std::vector<double> ra = density(P);
+    const std::vector<double> rb = pol ? density(*Pbp) : std::vector<double>();
+    std::vector<double> va(npt), vb(npt);
     for (int p = 0; p < npt; ++p) {
-      Nel += wtot[p] * rho[p];
-      lda_unpolarized(xf, cf, rho[p], thr, exc[p], vrho[p]);
-      Exc += wtot[p] * exc[p] * rho[p];
+      double exc;
+      if (pol) {
+        lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
+        Nel += wtot[p] * (ra[p] + rb[p]);
+        Exc += wtot[p] * exc * (ra[p] + rb[p]);
+      } else {
+        lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
+        Nel += wtot[p] * ra[p];
+        Exc += wtot[p] * exc * ra[p];
+      }
     }
     // eval_Fxc + increment_lda_split: H += (a .* w v) a' + (b .* w v) b'
-    Mat av = are, bv = aim;
-    for (int p = 0; p < npt; ++p) {
-      const double f = vrho[p] * wtot[p];
-      for (int u = 0; u < nbf; ++u) {
-        av(u, p) *= f;
-        bv(u, p) *= f;
+    auto potential = [&](const std::vector<double> &vr, Mat &Hm) {
+      Mat av = are, bv = aim;
+      for (int p = 0; p < npt; ++p) {
+        const double f = vr[p] * wtot[p];
+        for (int u = 0; u < nbf; ++u) {
+          av(u, p) *= f;
+          bv(u, p) *= f;
+        }
       }
-    }
-    const Mat Ha = matmul_nt(av, are), Hb = matmul_nt(bv, aim);
-    for (int v = 0; v < nbf; ++v)
-      for (int u = 0; u < nbf; ++u) H(idx[u], idx[v]) += Ha(u, v) + Hb(u, v);
+      const Mat Ha = matmul_nt(av, are), Hb2 = matmul_nt(bv, aim);
+      for (int v = 0; v < nbf; ++v)
+        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Ha(u, v) + Hb2(u, v);
+    };
+    potential(va, H);
+    if (pol) potential(vb, Hb);
   }
 }
 
@@ -162,10 +241,24 @@ DiatomicXC::DiatomicXC(const DiatomicBasis &b_, int lang_) : b(b_), lang(lang_) 
 // eval_Fxc (:468-500): per element and radial point, real per-m basis values
 // B_j(mu) Y_l^m(nu, 0) on the nu rule, the phi integral done analytically.
 void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pin, Mat &Hout, double &Exc, double &Nel, double thr) const {
+  Mat Hb;
+  eval(xf, cf, Pin, nullptr, Hout, Hb, Exc, Nel, thr);
+}
+
+// PureMDFTGrid::eval_Fxc (spin-polarized), dftgrid_purem.cpp:705-740
+void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, Mat &Hb, double &Exc,
+                          double &Nel, double thr) const {
+  eval(xf, cf, Pa, &Pb, Ha, Hb, Exc, Nel, thr);
+}
+
+void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout, Mat &Hbout, double &Exc,
+                      double &Nel, double thr) const {
+  const bool pol = Pbin != nullptr;
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);
+  const Mat Pbx = pol ? b.expand_boundaries(*Pbin) : Mat();
   const int Na = b.Nang(), Nr = rad.Nbf(), nang = (int)cth.size();
-  Mat H(b.Ndummy(), b.Ndummy());
+  Mat H(b.Ndummy(), b.Ndummy()), H2(pol ? b.Ndummy() : 0, pol ? b.Ndummy() : 0);
   Exc = 0.0;
   Nel = 0.0;
   std::vector<int> mlist(b.mval.begin(), b.mval.end());
@@ -186,7 +279,7 @@ void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pin, Mat &Hout, double &Exc
       }
       std::vector<std::vector<int>> idx(mlist.size());
       std::vector<Mat> f(mlist.size());
-      std::vector<double> rho(nang, 0.0);
+      std::vector<double> rho(nang, 0.0), rhob(nang, 0.0);
       for (size_t im = 0; im < mlist.size(); ++im) {
         for (int a = 0; a < Na; ++a)
           if (b.mval[a] == mlist[im])
@@ -202,35 +295,51 @@ void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pin, Mat &Hout, double &Exc
           }
           row += Ni;
         }
-        Mat Pb(nbf, nbf);
-        for (int v = 0; v < nbf; ++v)
-          for (int u = 0; u < nbf; ++u) Pb(u, v) = P(idx[im][u], idx[im][v]);
-        const Mat Pv = matmul(Pb, f[im]);
-        for (int ia = 0; ia < nang; ++ia) {
-          double s = 0.0;
-          for (int u = 0; u < nbf; ++u) s += Pv(u, ia) * f[im](u, ia);
-          rho[ia] += s;
-        }
+        auto block_density = [&](const Mat &Pm, std::vector<double> &r) {
+          Mat Pblk(nbf, nbf);
+          for (int v = 0; v < nbf; ++v)
+            for (int u = 0; u < nbf; ++u) Pblk(u, v) = Pm(idx[im][u], idx[im][v]);
+          const Mat Pv = matmul(Pblk, f[im]);
+          for (int ia = 0; ia < nang; ++ia) {
+            double s = 0.0;
+            for (int u = 0; u < nbf; ++u) s += Pv(u, ia) * f[im](u, ia);
+            r[ia] += s;
+          }
+        };
+        block_density(P, rho);
+        if (pol) block_density(Pbx, rhob);
       }
-      std::vector<double> wv(nang);
+      std::vector<double> wv(nang), wvb(nang);
       for (int ia = 0; ia < nang; ++ia) {
-        double exc, v;
-        lda_unpolarized(xf, cf, rho[ia], thr, exc, v);
-        Nel += wtot[ia] * rho[ia];
-        Exc += wtot[ia] * exc * rho[ia];
+        double exc, v, vb2 = 0.0;
+        if (pol) {
+          lda_polarized(xf, cf, rho[ia], rhob[ia], thr, exc, v, vb2);
+          Nel += wtot[ia] * (rho[ia] + rhob[ia]);
+          Exc += wtot[ia] * exc * (rho[ia] + rhob[ia]);
+        } else {
+          lda_unpolarized(xf, cf, rho[ia], thr, exc, v);
+          Nel += wtot[ia] * rho[ia];
+          Exc += wtot[ia] * exc * rho[ia];
+        }
         wv[ia] = v * wtot[ia];
+        wvb[ia] = vb2 * wtot[ia];
       }
-      for (size_t im = 0; im < mlist.size(); ++im) {
-        Mat fw = f[im];
-        for (int ia = 0; ia < nang; ++ia)
-          for (int u = 0; u < fw.r; ++u) fw(u, ia) *= wv[ia];
-        const Mat Hm = matmul_nt(fw, f[im]);
-        for (int v = 0; v < Hm.c; ++v)
-          for (int u = 0; u < Hm.r; ++u) H(idx[im][u], idx[im][v]) += Hm(u, v);
-      }
+      auto potential = [&](const std::vector<double> &w, Mat &Hm) {
+        for (size_t im = 0; im < mlist.size(); ++im) {
+          Mat fw = f[im];
+          for (int ia = 0; ia < nang; ++ia)
+            for (int u = 0; u < fw.r; ++u) fw(u, ia) *= w[ia];
+          const Mat Hblk = matmul_nt(fw, f[im]);
+          for (int v = 0; v < Hblk.c; ++v)
+            for (int u = 0; u < Hblk.r; ++u) Hm(idx[im][u], idx[im][v]) += Hblk(u, v);
+        }
+      };
+      potential(wv, H);
+      if (pol) potential(wvb, H2);
     }
   }
   Hout = b.remove_boundaries(H);
+  if (pol) Hbout = b.remove_boundaries(H2);
 }
 
 }  // namespace orc

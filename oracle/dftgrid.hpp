@@ -7,6 +7,7 @@ namespace orc {
 
 // lda_x (1) + lda_c_vwn (7, VWN5), spin-unpolarized; thr as libxc's density threshold
 void lda_unpolarized(int xf, int cf, double rho, double thr, double &exc, double &vrho);
+void lda_polarized(int xf, int cf, double ra, double rb, double thr, double &exc, double &va, double &vb);
 void angular_chebyshev(int l, int m, std::vector<double> &cth, std::vector<double> &phi, std::vector<double> &w);
 std::complex<double> spherical_harmonic(int l, int m, double cth, double phi);
 
@@ -17,6 +18,10 @@ struct AtomicXC {
   std::vector<double> cth, phi, wang;
   AtomicXC(const AtomicBasis &b, int lang, int mang);
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
+  void eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, Mat &Hb, double &Exc, double &Nel,
+                double thr) const;
+  void eval(int xf, int cf, const Mat &P, const Mat *Pb, Mat &H, Mat &Hb, double &Exc, double &Nel,
+            double thr) const;
 };
 
 struct DiatomicBasis;
@@ -27,6 +32,10 @@ struct DiatomicXC {
   std::vector<double> cth, wang;
   DiatomicXC(const DiatomicBasis &b, int lang);
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
+  void eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, Mat &Hb, double &Exc, double &Nel,
+                double thr) const;
+  void eval(int xf, int cf, const Mat &P, const Mat *Pb, Mat &H, Mat &Hb, double &Exc, double &Nel,
+            double thr) const;
 };
 
 }  // namespace orc

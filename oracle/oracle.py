@@ -49,6 +49,14 @@ def lib():
                               ctypes.c_void_p, ctypes.c_void_p]
         L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_lda_pol.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_atomic_xc_pol.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_diatomic_xc_pol.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_create.restype = ctypes.c_void_p
@@ -97,6 +105,15 @@ def lda(rho, x_func=1, c_func=7, thr=1e-12):
     v = np.zeros_like(rho)
     _check(lib().orc_lda(x_func, c_func, rho.size, _ptr(rho), thr, _ptr(exc), _ptr(v)))
     return exc, v
+
+
+def lda_pol(ra, rb, x_func=1, c_func=7, thr=1e-12):
+    """Spin-polarized lda_x + lda_c_vwn: (exc per particle, v_alpha, v_beta)."""
+    ra = np.ascontiguousarray(ra, dtype=np.float64)
+    rb = np.ascontiguousarray(rb, dtype=np.float64)
+    exc, va, vb = np.zeros_like(ra), np.zeros_like(ra), np.zeros_like(ra)
+    _check(lib().orc_lda_pol(x_func, c_func, ra.size, _ptr(ra), _ptr(rb), thr, _ptr(exc), _ptr(va), _ptr(vb)))
+    return exc, va, vb
 
 
 def grid(rmax, nel, igrid=4, zexp=2.0):
@@ -217,6 +234,20 @@ class AtomicBasis:
         out = np.zeros(2)
         _check(lib().orc_atomic_xc(self._h, lang, mang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
         return H, float(out[0]), float(out[1])
+
+    def xc_pol(self, Pa, Pb, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
+        """Spin-polarized DFTGrid::eval_Fxc (dftgrid.cpp:655-712): (Ha, Hb, Exc, Nel)."""
+        lmax, mmax = int(np.max(self.lval)), int(np.max(np.abs(self.mval)))
+        lang = 4 * lmax + 12 if lang is None else lang
+        mang = 4 * mmax + 12 if mang is None else mang
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        Ha = np.zeros((self.Nbf, self.Nbf), order="F")
+        Hb = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_atomic_xc_pol(self._h, lang, mang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha),
+                                       _ptr(Hb), _ptr(out)))
+        return Ha, Hb, float(out[0]), float(out[1])
 
     def cache(self, kind, L, iel):
         dims = (ctypes.c_int * 2)()
@@ -356,6 +387,18 @@ class DiatomicBasis:
         _check(lib().orc_diatomic_xc(self._h, lang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
         return H, float(out[0]), float(out[1])
 
+    def xc_pol(self, Pa, Pb, lang=None, x_func=1, c_func=7, thr=1e-12):
+        """Spin-polarized PureMDFTGrid::eval_Fxc (dftgrid_purem.cpp:705-740): (Ha, Hb, Exc, Nel)."""
+        lang = 4 * int(np.max(self.lval)) + 12 if lang is None else lang
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        Ha = np.zeros((self.Nbf, self.Nbf), order="F")
+        Hb = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_diatomic_xc_pol(self._h, lang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha), _ptr(Hb),
+                                         _ptr(out)))
+        return Ha, Hb, float(out[0]), float(out[1])
+
     def maps(self):
         lm = np.zeros(2 * self.n_lm, np.int32)
         LM = np.zeros(2 * self.n_LM, np.int32)
@@ -486,3 +529,88 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
         P, occ = density(Fd)
     E += getattr(basis, "Enucr", 0.0)
     return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, P=P)
+
+
+def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_pol=None):
+    """Unrestricted HF (or spin-polarized DFT) with aufbau per spin over the
+    per-m symmetry blocks and Pulay DIIS on both spins' commutators.
+
+    xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects Kohn-Sham DFT (no exact exchange).
+    Energy as the reference's fock builders (atomic/main.cpp:369-470,
+    diatomic/main.cpp:370-479): tr(P T) + tr(P V) + 1/2 tr(P J) + Exx + Exc
+    + Enucr, Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb))."""
+    S = basis.overlap()
+    T = basis.kinetic()
+    V = basis.nuclear()
+    H = T + V
+    blocks = basis.sym_idx(symm)
+    X = [invh(S[np.ix_(b, b)]) for b in blocks]
+
+    def density(F, nocc):
+        eps, vecs = [], []
+        for k, b in enumerate(blocks):
+            e, c = np.linalg.eigh(X[k].T @ F[np.ix_(b, b)] @ X[k])
+            eps.append(e)
+            vecs.append(c)
+        allev = sorted((e, k, i) for k, ev in enumerate(eps) for i, e in enumerate(ev))
+        occ = [np.zeros(len(ev)) for ev in eps]
+        for e, k, i in allev[:nocc]:
+            occ[k][i] = 1.0
+        P = np.zeros_like(F)
+        for k, b in enumerate(blocks):
+            C = X[k] @ vecs[k]
+            P[np.ix_(b, b)] += (C * occ[k]) @ C.T
+        return P
+
+    Pa, Pb = density(H, nela), density(H, nelb)
+    diis_F, diis_e = [], []
+    Eold, E = 0.0, 0.0
+    for it in range(1, maxit + 1):
+        P = Pa + Pb
+        J = basis.coulomb(P)
+        Ekin, Enuc, Ecoul = np.sum(P * T), np.sum(P * V), 0.5 * np.sum(P * J)
+        if xc_pol is None:
+            Ka, Kb = basis.exchange(Pa), basis.exchange(Pb)
+            Exx, Exc = 0.5 * (np.sum(Pa * Ka) + np.sum(Pb * Kb)), 0.0
+            Fa, Fb = H + J + Ka, H + J + Kb
+        else:
+            Va, Vb, Exc = xc_pol(Pa, Pb)
+            Exx = 0.0
+            Fa, Fb = H + J + Va, H + J + Vb
+        E = Ekin + Enuc + Ecoul + Exx + Exc
+        errs = []
+        for F_, P_ in ((Fa, Pa), (Fb, Pb)):
+            for k, b in enumerate(blocks):
+                Fo = X[k].T @ F_[np.ix_(b, b)] @ X[k]
+                Sinv = np.linalg.pinv(X[k])
+                Po = Sinv @ P_[np.ix_(b, b)] @ Sinv.T
+                errs.append((Fo @ Po - Po @ Fo).ravel())
+        err = np.concatenate(errs)
+        emax = np.max(np.abs(err))
+        if verbose:
+            print(f"it {it:3d} E {E:.12f} dE {E - Eold:.3e} err {emax:.3e}")
+        if it > 1 and abs(E - Eold) < conv and emax < 1e-6:
+            break
+        Eold = E
+        diis_F.append((Fa, Fb))
+        diis_e.append(err)
+        if len(diis_F) > 10:
+            diis_F.pop(0)
+            diis_e.pop(0)
+        n = len(diis_F)
+        B = -np.ones((n + 1, n + 1))
+        B[n, n] = 0.0
+        for i in range(n):
+            for j in range(n):
+                B[i, j] = diis_e[i] @ diis_e[j]
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:n]
+            Fa_d = sum(ci * f[0] for ci, f in zip(c, diis_F))
+            Fb_d = sum(ci * f[1] for ci, f in zip(c, diis_F))
+        except np.linalg.LinAlgError:
+            Fa_d, Fb_d = Fa, Fb
+        Pa, Pb = density(Fa_d, nela), density(Fb_d, nelb)
+    E += getattr(basis, "Enucr", 0.0)
+    return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, Pa=Pa, Pb=Pb)

@@ -13,7 +13,8 @@ REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
 OUT = Path(__file__).with_name("reference_energies.json")
 WANT = ["atomic-H-hf-u", "atomic-He-hf-r", "atomic-Be-hf-r", "atomic-He-lda-r", "atomic-Be-lda-r",
         "atomic-Ne-hf-r", "atomic-Ne-lda-r", "diatomic-H2-hf-r", "diatomic-He-dummy-hf", "diatomic-H2-purem-on",
-        "diatomic-Ne-dummy-hf", "diatomic-Ne-dummy-lda"]
+        "diatomic-Ne-dummy-hf", "diatomic-Ne-dummy-lda", "atomic-He-lda-u", "atomic-Be-lda-u",
+        "diatomic-NH-hf-u", "diatomic-NH-hf-absm", "diatomic-NH-lda-u", "diatomic-NH-lda-absm"]
 
 refs = json.loads((REF / "tests/refs/ci.json").read_text())["cases"]
 cases = {c["name"]: c for c in json.loads((REF / "tests/cases.json").read_text())["cases"]}

@@ -4,6 +4,9 @@ Both sides get the same factor caches (the oracle's, uploaded through
 hf_atomic_set_cache), so the comparison isolates the kernels.  Bar: FP64
 relative error max|X_gpu - X_oracle| / max|X_oracle| <= 1e-12.
 """
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -13,6 +16,7 @@ from helfem_amd.synthetic import random_symmetric, seeded_density
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
+REF = json.loads((Path(__file__).parent / "golden" / "reference_energies.json").read_text())
 
 CASES = {
     # Ne, spherically averaged, 10 elements to 40 bohr (the single-L J/K case)
@@ -107,3 +111,30 @@ def test_gpu_scf_reproduces_reference_energy(gpu_ctx):
     ro = O.scf_rhf(ob, 10)
     assert abs(r["E"] - ro["E"]) < 1e-10, (r["E"], ro["E"])
     assert abs(r["E"] + 128.5470981094) < 1e-9
+
+
+def test_dense_operator_inside_elements(gpu_ctx, monkeypatch):
+    """With a memory budget for it (HFGPU_EXCHANGE_OPERATOR_GIB) the in-element
+    exchange runs as a dense-operator GEMM instead of the default Cholesky-form
+    kernel; both agree with the oracle."""
+    print("[gpu] atomic K, dense in-element operator", flush=True)
+    monkeypatch.setenv("HFGPU_EXCHANGE_OPERATOR_GIB", "32")
+    cfg = CASES["Kr-l3m2-4el"]
+    ob = O.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"])
+    ob.compute_tei()
+    hb = H.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"], ctx=gpu_ctx)
+    hb.compute_tei()
+    P = random_symmetric(ob.Nbf, seed=1005)
+    assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
+
+
+def test_gpu_uhf_hydrogen(gpu_ctx):
+    """atomic-H-hf-u: one alpha electron; the exchange cancels the self-Coulomb
+    exactly, E = -0.5 Eh to the basis limit (tests/refs/ci.json)."""
+    from helfem_amd.scf import uhf
+    print("[gpu] H UHF SCF", flush=True)
+    hb = H.AtomicBasis(1, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    r = uhf(hb, 1, 0, conv=1e-12)
+    assert abs(r.E - REF["atomic-H-hf-u"]["total"]) < 1e-9, r.E
+    assert abs(r.Ecoul + r.Exx) < 1e-10

@@ -19,6 +19,7 @@ from helfem_amd.synthetic import random_symmetric, seeded_density
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 REF = json.loads((Path(__file__).parent / "golden" / "reference_energies.json").read_text())
+ORC = json.loads((Path(__file__).parent / "golden" / "oracle_energies.json").read_text())
 
 CASES = {
     # homonuclear, sigma and pi shells, three elements (last one of 14 functions)
@@ -126,3 +127,36 @@ def test_h2_scf_on_gpu(gpu_ctx):
     assert abs(r.E - ro["E"]) < 1e-10, (r.E, ro["E"])
     assert abs(r.E - REF["diatomic-H2-hf-r"]["total"]) < 1e-9
     torch.cuda.synchronize()
+
+
+def test_dense_operator_inside_elements(gpu_ctx, monkeypatch):
+    """With a memory budget for it (HFGPU_EXCHANGE_OPERATOR_GIB) the in-element
+    exchange runs as a dense-operator GEMM instead of the default Cholesky-form
+    kernel; both agree with the oracle."""
+    print("[gpu] diatomic K, dense in-element operator", flush=True)
+    monkeypatch.setenv("HFGPU_EXCHANGE_OPERATOR_GIB", "32")
+    Z1, Z2, R, lmm, nel = CASES["N2-l432-3el"]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    ob.compute_tei()
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    hb.compute_tei()
+    P = random_symmetric(ob.Nbf, seed=1004)
+    assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
+
+
+@pytest.mark.parametrize("absm", [False, True])
+def test_nh_triplet_uhf_on_gpu(absm, gpu_ctx):
+    """diatomic-NH-hf-u and -hf-absm: 5 alpha + 3 beta electrons, R = 1.958
+    bohr, 3 elements, lmax 13 (m=0) and 11 (|m|=1).  The reference's recorded
+    totals lie 2.0e-9 Eh above the converged minimum (its energy components
+    are off by 1e-5 Eh, a density converged to about 1e-5); the oracle's
+    converged UHF energy (tests/golden/oracle_energies.json) is the 1e-10 bar."""
+    from helfem_amd.scf import uhf
+    print(f"[gpu] diatomic NH UHF SCF absm={absm}", flush=True)
+    hb = H.DiatomicBasis(7, 1, 1.958, [13, 11], 3, ctx=gpu_ctx)
+    hb.set_absm_symmetric(absm)
+    hb.compute_tei()
+    r = uhf(hb, 5, 3, conv=1e-12)
+    assert abs(r.E - ORC["diatomic-NH-hf-u"]["total"]) < 1e-10, r.E
+    name = "diatomic-NH-hf-absm" if absm else "diatomic-NH-hf-u"
+    assert abs(r.E - REF[name]["total"]) < 3e-9, (r.E, REF[name]["total"])

@@ -1,0 +1,97 @@
+"""Reduced-precision exchange and the mixed-precision SCF.
+
+The FP32 exchange (hf_atomic_exchange_fp32_dev, hf_diatomic_exchange_fp32_dev)
+rounds the density, Gaunt weights and radial factors to FP32 and multiplies on
+the FP32 matrix cores; its error against the FP64 oracle is stated here:
+max|K32 - K64| / max|K64| <= 2e-6 on physical (seeded) densities.  A mixed
+SCF (FP32 exchange until the orbital gradient drops below 1e-3, FP64 after)
+must reach the FP64 SCF energy within 1e-10 Eh (the plain path's bar), so the
+final energy carries no FP32 error.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import helfem_amd as H
+import oracle as O
+from helfem_amd.synthetic import seeded_density
+
+pytestmark = pytest.mark.gpu
+TOL32 = 2e-6
+REF = json.loads((Path(__file__).parent / "golden" / "reference_energies.json").read_text())
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_atomic_fp32_exchange_error(gpu_ctx):
+    import torch
+    print("[gpu] atomic FP32 exchange vs oracle", flush=True)
+    ob = O.AtomicBasis(36, 3, 2, 4)
+    ob.compute_tei()
+    hb = H.AtomicBasis(36, 3, 2, 4, ctx=gpu_ctx)
+    hb.compute_tei()
+    S = ob.overlap()
+    blocks = ob.sym_idx(1)
+    P = seeded_density(S, blocks, [2] * len(blocks), seed=1201)
+    Kr = ob.exchange(P)
+    Pd = torch.as_tensor(np.ascontiguousarray(P), device="cuda")
+    K32 = hb.exchange(Pd, precision="fp32").cpu().numpy()
+    K64 = hb.exchange(Pd).cpu().numpy()
+    torch.cuda.synchronize()
+    e32 = relerr(K32, Kr)
+    assert relerr(K64, Kr) <= 1e-12
+    assert e32 <= TOL32, e32
+    assert e32 > 1e-12  # the FP32 path ran
+    # the plain path is unchanged by an FP32 call before it
+    assert relerr(hb.exchange(Pd).cpu().numpy(), Kr) <= 1e-12
+
+
+@pytest.mark.parametrize("absm", [False, True])
+def test_diatomic_fp32_exchange_error(absm, gpu_ctx):
+    import torch
+    print(f"[gpu] diatomic FP32 exchange vs oracle absm={absm}", flush=True)
+    args = (7, 7, 2.074, [4, 3, 2], 3)
+    ob = O.DiatomicBasis(*args)
+    ob.compute_tei()
+    hb = H.DiatomicBasis(*args, ctx=gpu_ctx)
+    hb.set_absm_symmetric(absm)
+    hb.compute_tei()
+    S = ob.overlap()
+    blocks = ob.sym_idx(1)
+    P = seeded_density(S, blocks, [2] * len(blocks), seed=1202)
+    Pd = torch.as_tensor(np.ascontiguousarray(P), device="cuda")
+    K64 = hb.exchange(Pd).cpu().numpy()
+    K32 = hb.exchange(Pd, precision="fp32").cpu().numpy()
+    if not absm:
+        assert relerr(K64, ob.exchange(P)) <= 1e-12
+    e32 = relerr(K32, K64)
+    assert 1e-12 < e32 <= TOL32, e32
+
+
+def test_mixed_rhf_matches_fp64_atomic(gpu_ctx):
+    """Ne HF at lmax 1, 5 elements (tests/README.md:79-80)."""
+    from helfem_amd.scf import rhf
+    print("[gpu] Ne HF mixed vs FP64 SCF", flush=True)
+    hb = H.AtomicBasis(10, 1, 1, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    r64 = rhf(hb, 10, conv=1e-11)
+    rmx = rhf(hb, 10, conv=1e-11, precision="mixed")
+    assert rmx.iterations_fp32 > 0
+    assert abs(rmx.E - r64.E) < 1e-10, (rmx.E, r64.E)
+    assert abs(rmx.E + 128.5470981094) < 1e-9
+
+
+def test_mixed_uhf_matches_fp64_diatomic(gpu_ctx):
+    """NH triplet UHF (diatomic-NH-hf-u): mixed precision reaches the FP64 energy."""
+    from helfem_amd.scf import uhf
+    print("[gpu] NH UHF mixed vs FP64 SCF", flush=True)
+    hb = H.DiatomicBasis(7, 1, 1.958, [13, 11], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    r64 = uhf(hb, 5, 3, conv=1e-11)
+    rmx = uhf(hb, 5, 3, conv=1e-11, precision="mixed")
+    assert rmx.iterations_fp32 > 0
+    assert abs(rmx.E - r64.E) < 1e-10, (rmx.E, r64.E)

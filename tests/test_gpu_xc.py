@@ -115,3 +115,79 @@ def test_ne_dummy_centre_lda_scf_on_gpu(gpu_ctx):
     r = rhf(hb, 10, xc=xc, conv=1e-12)
     torch.cuda.synchronize()
     assert abs(r.E - REF["diatomic-Ne-dummy-lda"]["total"]) < 1e-9, (r.E, REF["diatomic-Ne-dummy-lda"]["total"])
+
+
+def _spin_densities(ob, seed):
+    S = ob.overlap()
+    blocks = ob.sym_idx(1)
+    Pa = seeded_density(S, blocks, [1] * len(blocks), seed=seed)
+    Pb = 0.6 * seeded_density(S, blocks, [1] * len(blocks), seed=seed + 1)
+    return Pa, Pb
+
+
+@pytest.mark.parametrize("name", ["Ne-l1-5el", "Kr-l3m3-2el"])
+def test_polarized_lda_matrix_matches_oracle(name, gpu_ctx):
+    """Unrestricted DFTGrid::eval_Fxc (dftgrid.cpp:655-712): Ha, Hb, Exc, Nel."""
+    print(f"[gpu] atomic polarized LDA Vxc: {name}", flush=True)
+    c = CASES[name]
+    ob = O.AtomicBasis(c["Z"], c["lmax"], c["mmax"], c["nelem"])
+    hb = H.AtomicBasis(c["Z"], c["lmax"], c["mmax"], c["nelem"], ctx=gpu_ctx)
+    Pa, Pb = _spin_densities(ob, 1107)
+    Har, Hbr, Exc_r, Nel_r = ob.xc_pol(Pa, Pb)
+    grid = H.DFTGrid(hb)
+    Hag, Hbg, Exc_g, Nel_g, _ = grid.eval_Fxc_pol(1, 7, Pa, Pb)
+    assert relerr(Hag, Har) <= TOL, relerr(Hag, Har)
+    assert relerr(Hbg, Hbr) <= TOL, relerr(Hbg, Hbr)
+    assert abs(Exc_g - Exc_r) <= TOL * abs(Exc_r)
+    assert abs(Nel_g - Nel_r) <= TOL * abs(Nel_r)
+    # no beta electrons: the beta potential is returned as zero
+    _, Hb0, _, _, _ = grid.eval_Fxc_pol(1, 7, Pa, 0 * Pb, beta=False)
+    assert not np.any(Hb0)
+
+
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_purem_polarized_lda_matrix_matches_oracle(name, gpu_ctx):
+    """Unrestricted PureMDFTGrid::eval_Fxc (dftgrid_purem.cpp:705-740)."""
+    print(f"[gpu] diatomic pure-m polarized LDA Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    Pa, Pb = _spin_densities(ob, 1109)
+    Har, Hbr, Exc_r, Nel_r = ob.xc_pol(Pa, Pb)
+    grid = H.DFTGrid(hb)
+    Hag, Hbg, Exc_g, Nel_g, _ = grid.eval_Fxc_pol(1, 7, Pa, Pb)
+    assert relerr(Hag, Har) <= TOL, relerr(Hag, Har)
+    assert relerr(Hbg, Hbr) <= TOL, relerr(Hbg, Hbr)
+    assert abs(Exc_g - Exc_r) <= TOL * abs(Exc_r)
+    assert abs(Nel_g - Nel_r) <= TOL * abs(Nel_r)
+
+
+def _uks(hb, nela, nelb):
+    from helfem_amd.scf import uhf
+    grid = H.DFTGrid(hb)
+
+    def xc_pol(Pa, Pb):
+        Ha, Hb, Exc, _, _ = grid.eval_Fxc_pol(1, 7, Pa, Pb, beta=nelb > 0)
+        return Ha, Hb, Exc
+
+    return uhf(hb, nela, nelb, xc_pol=xc_pol, conv=1e-12)
+
+
+@pytest.mark.parametrize("Z,name", [(2, "atomic-He-lda-u"), (4, "atomic-Be-lda-u")])
+def test_polarized_lda_scf_on_gpu(Z, name, gpu_ctx):
+    print(f"[gpu] atomic polarized LDA SCF: {name}", flush=True)
+    hb = H.AtomicBasis(Z, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    r = _uks(hb, Z // 2, Z // 2)
+    assert abs(r.E - REF[name]["total"]) < 1e-9, (r.E, REF[name]["total"])
+
+
+def test_nh_triplet_polarized_lda_scf_on_gpu(gpu_ctx):
+    """diatomic-NH-lda-u: 5 alpha + 3 beta electrons, R = 1.958 bohr, 3 elements,
+    lmax 13 (m=0) and 11 (|m|=1)."""
+    print("[gpu] diatomic NH polarized LDA SCF", flush=True)
+    hb = H.DiatomicBasis(7, 1, 1.958, [13, 11], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    r = _uks(hb, 5, 3)
+    assert abs(r.E - REF["diatomic-NH-lda-absm"]["total"]) < 1e-9, r.E
+    assert abs(r.E - REF["diatomic-NH-lda-u"]["total"]) < 5e-9, r.E

@@ -103,3 +103,55 @@ def test_purem_lda_ne_dummy_centre():
     assert abs(r["E"] - REF["diatomic-Ne-dummy-lda"]["total"]) < 1e-9
     _, _, nel = b.xc(r["P"])
     assert abs(nel - 10) < 1e-9
+
+
+def test_polarized_lda_reduces_to_restricted():
+    """At zero spin polarization lda_pol(rho/2, rho/2) is the restricted LDA,
+    and the two spin potentials coincide."""
+    rho = np.geomspace(1e-6, 1e3, 25)
+    e, v = O.lda(rho)
+    ep, va, vb = O.lda_pol(0.5 * rho, 0.5 * rho)
+    assert np.max(np.abs(ep / e - 1)) < 1e-13
+    assert np.max(np.abs(va / v - 1)) < 1e-13 and np.array_equal(va, vb)
+
+
+def test_polarized_lda_fully_polarized_exchange():
+    """Slater exchange of a fully polarized gas: e_x = 2^{1/3} e_x(rho),
+    v_x(alpha) = 2^{1/3} 4/3 e_x(rho) (spin scaling of exchange)."""
+    rho = np.array([1e-4, 0.1, 3.0, 200.0])
+    ep, va, _ = O.lda_pol(rho, np.zeros_like(rho), 1, 0)
+    e, _ = O.lda(rho, 1, 0)
+    assert np.max(np.abs(ep / (2 ** (1 / 3) * e) - 1)) < 1e-13
+    assert np.max(np.abs(va / (2 ** (1 / 3) * 4 / 3 * e) - 1)) < 1e-13
+
+
+def test_polarized_vwn_potential_is_derivative_of_energy():
+    """v_sigma = d(rho e)/d rho_sigma by central differences in each spin."""
+    ra = np.geomspace(1e-4, 1e2, 12)
+    rb = 0.37 * ra
+    rho = ra + rb
+    _, va, vb = O.lda_pol(ra, rb, 0, 7)
+    for da, db, v in ((1.0, 0.0, va), (0.0, 1.0, vb)):
+        h = 1e-6 * rb
+        ep, _, _ = O.lda_pol(ra + da * h, rb + db * h, 0, 7)
+        em, _, _ = O.lda_pol(ra - da * h, rb - db * h, 0, 7)
+        fd = ((rho + h) * ep - (rho - h) * em) / (2 * h)
+        assert np.max(np.abs(fd - v) / np.abs(v)) < 1e-7, (da, db)
+
+
+@pytest.mark.parametrize("Z,name", [(2, "atomic-He-lda-u"), (4, "atomic-Be-lda-u")])
+def test_polarized_lda_energies(Z, name):
+    b = O.AtomicBasis(Z, 0, 0, 5)
+    b.compute_tei()
+    r = O.scf_uhf(b, Z // 2, Z // 2, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb)[:3])
+    assert abs(r["E"] - REF[name]["total"]) < 1e-9, (r["E"], REF[name]["total"])
+
+
+def test_polarized_lda_nh_triplet():
+    """diatomic-NH-lda-u / -absm: the 3Sigma- ground state (5 alpha, 3 beta
+    electrons), lmax 13 (m=0) and 11 (|m|=1), 3 elements, R = 1.958 bohr."""
+    b = O.DiatomicBasis(7, 1, 1.958, [13, 11], 3)
+    b.compute_tei()
+    r = O.scf_uhf(b, 5, 3, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb)[:3])
+    assert abs(r["E"] - REF["diatomic-NH-lda-absm"]["total"]) < 1e-9
+    assert abs(r["E"] - REF["diatomic-NH-lda-u"]["total"]) < 5e-9
