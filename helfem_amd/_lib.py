@@ -43,9 +43,14 @@ SIGNATURES = {
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_atomic_compute_yukawa": (c_int, [c_vp, c_dbl]),
+    "hf_atomic_compute_erfc": (c_int, [c_vp, c_dbl]),
+    "hf_atomic_rs_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_atomic_rs_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange_fp32_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_dftgrid_create": (c_int, [c_vp, c_int, c_int, ctypes.POINTER(c_vp)]),
     "hf_diatomic_dftgrid_create": (c_int, [c_vp, c_int, ctypes.POINTER(c_vp)]),
+    "hf_diatomic_dftgrid3d_create": (c_int, [c_vp, c_int, c_int, c_vp]),
     "hf_dftgrid_destroy": (c_int, [c_vp]),
     "hf_dftgrid_eval_fxc": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
     "hf_dftgrid_eval_fxc_dev": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
@@ -86,6 +91,14 @@ def lib():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C helfem_amd/csrc` "
                                "(or __graft_entry__.build())")
+        # PyTorch ships its own HIP runtime under the same soname as ROCm's.
+        # Whichever loads first serves the whole process; if libhfgpu pulled
+        # in ROCm's before torch initialises its own device layer, torch then
+        # finds no GPU.  Loading torch first keeps one runtime for both.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

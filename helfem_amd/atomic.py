@@ -182,6 +182,18 @@ class AtomicBasis:
         check(dev_fn(self._h, P.data_ptr(), self.Nbf, O.data_ptr(), self.Nbf))
         return O
 
+    def compute_yukawa(self, lam: float):
+        """Yukawa range-separation caches (TwoDBasis.cpp:738-760)."""
+        check(lib().hf_atomic_compute_yukawa(self._h, float(lam)))
+
+    def compute_erfc(self, mu: float):
+        """erfc range-separation caches (TwoDBasis.cpp:762-771)."""
+        check(lib().hf_atomic_compute_erfc(self._h, float(mu)))
+
+    def rs_exchange(self, P, out=None):
+        """-K of the short-range kernel, TwoDBasis.cpp:1002-1131."""
+        return self._apply(P, lib().hf_atomic_rs_exchange, lib().hf_atomic_rs_exchange_dev, out)
+
     def coulomb(self, P, out=None):
         """J(P), TwoDBasis.cpp:774-877."""
         return self._apply(P, lib().hf_atomic_coulomb, lib().hf_atomic_coulomb_dev, out)

@@ -189,6 +189,7 @@ struct KArgs {
   const T *term_c;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
+  const T *Lscale;  // optional per-L factor on the Gaunt weights (range separation)
 };
 
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
@@ -256,8 +257,9 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
     }
+    const T msc = A.Lscale ? A.Lscale[L] : T(1);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) r[s] *= cmask[s];
+    for (int s = 0; s < 4; ++s) r[s] *= cmask[s] * msc;
     const size_t te = ((size_t)L * A.Nel + e) * TILE2, tf = ((size_t)L * A.Nel + f) * TILE2;
     if (!INSIDE) {
       const T *At = (e > f) ? A.rb + te : A.rs + te;
@@ -304,6 +306,95 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
   acc += acc2;
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + kk[rr] + TILE * c] = (double)acc[rr];
+  __syncthreads();
+  const int a = t & 15, b = t >> 4;
+  if (a >= ni || b >= nj) return;
+  const double v = red[t] + red[TILE2 + t] + red[2 * TILE2 + t] + red[3 * TILE2 + t];
+  const int64_t row = (int64_t)j * A.Nrad + fi + a, col = (int64_t)k * A.Nrad + fj + b;
+  const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
+  const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+  double *dst = A.K + row + col * A.ldk;
+  if (srow || scol)
+    atomicAdd(dst, -v);
+  else
+    *dst = -v;
+}
+
+// --- Range-separated exchange with non-separable (erfc) element pairs ------------
+//
+// One workgroup per (coupled output block, element pair (e,f)), all pairs
+// including e == f, waves splitting L.  The Gaunt-weighted density tile
+// R_L(e,f) is gathered as in k_atomic_k, staged in LDS, and multiplied by
+// the pair's dense operator: K(a,b) += sum_cd T_L,ef[c + 16d][a + 16b]
+// R(c,d) (CoulombExchangeFE.h:412-440, which applies the exchange-ordered
+// element-pair tensor to vec(P_ij)).  The operator rows are read coalesced.
+
+struct KPairArgs {
+  const double *P;
+  int64_t ldp;
+  double *K;
+  int64_t ldk;
+  int Nel, Nrad, Nang, N_L, n_act;
+  const int *first, *ni;
+  const double *T, *Lscale;
+  const int *act_jk, *term_start, *term_il;
+  const double *term_c;
+};
+
+__global__ void __launch_bounds__(NT) k_atomic_k_pair(KPairArgs A) {
+  const int w = blockIdx.x;
+  const int xcd = w & 7, loc = w >> 3;
+  const int n_ef = A.Nel * A.Nel;
+  const int efi = (loc / A.n_act) * 8 + xcd;
+  const int act = loc - (loc / A.n_act) * A.n_act;
+  if (efi >= n_ef) return;
+  const int e = efi / A.Nel, f = efi - e * A.Nel;
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
+  __shared__ double Rl[4 * TILE2];
+  __shared__ double red[4 * TILE2];
+  const int rloc = fi + min(c, ni - 1);
+  int cloc[4], kk[4];
+  double cmask[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    kk[s] = g + 4 * s;
+    cloc[s] = fj + min(kk[s], nj - 1);
+    cmask[s] = (c < ni && kk[s] < nj) ? 1.0 : 0.0;
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double *R = Rl + wv * TILE2;
+  for (int L = wv; L < A.N_L; L += 4) {
+    const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
+    if (q0 == q1) continue;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = q0; q < q1; ++q) {
+      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+      const double cq = A.term_c[q];
+      const double *Pr = A.P + (int64_t)(i * A.Nrad + rloc);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
+    }
+    const double msc = A.Lscale[L];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) R[c + TILE * kk[s]] = r[s] * cmask[s] * msc;
+    // the wave's own LDS slice: a wave-wide barrier orders the writes
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double *Tp = A.T + (((size_t)L * A.Nel + e) * A.Nel + f) * TILE2 * TILE2 + lane;
+    for (int kx = 0; kx < TILE2; ++kx) {
+      const double rv = R[kx];
+      const double *Tk = Tp + (size_t)kx * TILE2;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += Tk[64 * u] * rv;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) red[wv * TILE2 + lane + 64 * u] = acc[u];
   __syncthreads();
   const int a = t & 15, b = t >> 4;
   if (a >= ni || b >= nj) return;
@@ -632,12 +723,14 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   ++d.list_gen;
 }
 
-void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                            hipStream_t s, KernelTimer *timer, bool fp32) {
+// Block-norm screen of P, compacted coupling lists, zeroed K; false when no
+// output block is coupled (K stays zero).
+static bool exchange_prologue(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                              hipStream_t s) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   if (d.h_act.empty()) {
     fill_zero_launch(K, nbf, nbf, ldk, s);
-    return;
+    return false;
   }
   hipLaunchKernelGGL(k_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.bnorm.p);
   hip_check(hipGetLastError(), "block_norms");
@@ -646,7 +739,46 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
   hip_check(hipStreamSynchronize(s), "block norms sync");
   compact_exchange_lists(d, s);
   fill_zero_launch(K, nbf, nbf, ldk, s);
-  if (d.n_act == 0) return;
+  return d.n_act > 0;
+}
+
+void atomic_rs_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer) {
+  if (!exchange_prologue(d, P, ldp, K, ldk, s)) return;
+  if (d.rs_kind == 1) {
+    for (int part = 0; part < 2; ++part) {
+      const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+      if (n_ef == 0) continue;
+      const int64_t grid = (int64_t)(((n_ef + 7) / 8) * 8) * d.n_act;
+      if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+      KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.y_rs.p,
+                      d.y_rb.p, d.y_chol.p, d.y_chol_off.p, d.y_chol_rank.p, d.act_jk.p, d.term_start.p,
+                      d.term_il.p, d.term_c.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef, d.rs_Lscale.p};
+      TimedLaunch tl(timer, "atomic_rs_k_yukawa", s);
+      if (part == 0)
+        hipLaunchKernelGGL((k_atomic_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_atomic_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+      hip_check(hipGetLastError(), "atomic_rs_k");
+    }
+  } else if (d.rs_kind == 2) {
+    const int n_ef = d.Nel * d.Nel;
+    const int64_t grid = (int64_t)(((n_ef + 7) / 8) * 8) * d.n_act;
+    if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+    KPairArgs a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.e_T.p,
+                d.rs_Lscale.p, d.act_jk.p, d.term_start.p, d.term_il.p, d.term_c.p};
+    TimedLaunch tl(timer, "atomic_rs_k_erfc", s);
+    hipLaunchKernelGGL(k_atomic_k_pair, dim3((unsigned)grid), dim3(NT), 0, s, a);
+    hip_check(hipGetLastError(), "atomic_rs_k_pair");
+  } else {
+    throw Error(-1, "range-separated tables are not on the device");
+  }
+}
+
+void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                            hipStream_t s, KernelTimer *timer, bool fp32) {
+  const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+  if (!exchange_prologue(d, P, ldp, K, ldk, s)) return;
   if (fp32) {
     if (!d.f32_tables) {
       d.rs32.alloc(d.rs.n);
@@ -692,7 +824,7 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
     if (fp32) {
       KArgs<float> a{d.P32.p, nbf, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p,
                      d.rs32.p, d.rb32.p, d.chol32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p,
-                     d.term_il.p, d.term_c32.p, efl, n_ef};
+                     d.term_il.p, d.term_c32.p, efl, n_ef, nullptr};
       if (part == 0)
         hipLaunchKernelGGL((k_atomic_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
@@ -700,7 +832,7 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
     } else {
       KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
                       d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p,
-                      d.term_c.p, efl, n_ef};
+                      d.term_c.p, efl, n_ef, nullptr};
       if (part == 0)
         hipLaunchKernelGGL((k_atomic_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else

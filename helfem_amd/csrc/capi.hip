@@ -13,6 +13,97 @@ thread_local std::string g_err;
 
 namespace {
 
+// Pack per-(L, element) factors as zero-padded 16x16 tiles: the "small"
+// (r^L or i_L) and "big" (r^-L-1 or k_L, empty on element 0) element
+// matrices, and the in-element Cholesky vectors M_p(a,b) = L(a + b n, p).
+void upload_factor_tiles(const AtomicSetup &S, const std::vector<Blk> &small, const std::vector<Blk> &big,
+                         const std::vector<Blk> &chol, DevArray<double> &d_rs, DevArray<double> &d_rb,
+                         DevArray<double> &d_chol, DevArray<int> &d_off, DevArray<int> &d_rank, int &maxr,
+                         hipStream_t s) {
+  const int Nel = S.rad.Nel(), nL = S.N_L;
+  std::vector<double> rs((size_t)nL * Nel * TILE2, 0.0), rb((size_t)nL * Nel * TILE2, 0.0);
+  std::vector<double> ch;
+  std::vector<int> choff((size_t)nL * Nel), chrank((size_t)nL * Nel);
+  maxr = 0;
+  for (int L = 0; L < nL; ++L)
+    for (int e = 0; e < Nel; ++e) {
+      const size_t q = (size_t)L * Nel + e;
+      const int n = S.rad.ni[e];
+      const Blk &a = small[q];
+      need(a.r == n && a.c == n, "small-r factor cache has the wrong shape");
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) rs[q * TILE2 + i + TILE * j] = a(i, j);
+      if (e > 0) {
+        const Blk &b = big[q];
+        need(b.r == n && b.c == n, "large-r factor cache has the wrong shape");
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i) rb[q * TILE2 + i + TILE * j] = b(i, j);
+      }
+      const Blk &c = chol[q];
+      need(c.r == n * n, "Cholesky cache has the wrong number of rows");
+      choff[q] = (int)ch.size();
+      chrank[q] = c.c;
+      maxr = std::max(maxr, c.c);
+      need(c.c <= TILE2, "Cholesky rank above 256 is not supported");
+      const size_t base = ch.size();
+      ch.resize(base + (size_t)c.c * TILE2, 0.0);
+      for (int p = 0; p < c.c; ++p)
+        for (int bb = 0; bb < n; ++bb)
+          for (int aa = 0; aa < n; ++aa) ch[base + (size_t)p * TILE2 + aa + TILE * bb] = c(aa + bb * n, p);
+    }
+  d_rs.upload(rs, s);
+  d_rb.upload(rb, s);
+  if (ch.empty()) ch.push_back(0.0);
+  d_chol.upload(ch, s);
+  d_off.upload(choff, s);
+  d_rank.upload(chrank, s);
+}
+
+// Range-separated caches on the device: Yukawa factor tiles with the L
+// scale lambda (2L+1) that turns the stored 4 pi/(2L+1) Gaunt weights into
+// 4 pi lambda; erfc element-pair operators, T[k = c + 16 d][t = a + 16 b] =
+// (a c | b d) for output (a, b) and density (c, d) entries, with the scale mu
+// (4 pi mu/(2L+1), TwoDBasis.cpp:1082-1083).
+void atomic_upload_rs(hf_atomic *h) {
+  const AtomicSetup &S = h->host;
+  AtomicDevice &d = h->dev;
+  hipStream_t s = h->ctx->cur;
+  const int Nel = S.rad.Nel(), nL = S.N_L;
+  std::vector<double> lscale(nL);
+  if (S.rs_kind == 1) {
+    int maxr = 0;
+    upload_factor_tiles(S, S.rs_small, S.rs_big, S.rs_chol, d.y_rs, d.y_rb, d.y_chol, d.y_chol_off,
+                        d.y_chol_rank, maxr, s);
+    for (int L = 0; L < nL; ++L) lscale[L] = S.rs_param * (2 * L + 1);
+    d.e_T.release();
+  } else if (S.rs_kind == 2) {
+    const size_t per = (size_t)TILE2 * TILE2;
+    std::vector<double> T(per * nL * Nel * Nel, 0.0);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (int L = 0; L < nL; ++L)
+      for (int e = 0; e < Nel; ++e)
+        for (int f = 0; f < Nel; ++f) {
+          const size_t x = ((size_t)L * Nel + e) * Nel + f;
+          const Blk &E = S.rs_pair[x];
+          const int ni = S.rad.ni[e], nj = S.rad.ni[f];
+          need(E.r == ni * ni && E.c == nj * nj, "erfc pair cache has the wrong shape");
+          double *t = &T[x * per];
+          for (int a = 0; a < ni; ++a)
+            for (int c = 0; c < ni; ++c)
+              for (int b = 0; b < nj; ++b)
+                for (int dd = 0; dd < nj; ++dd)
+                  t[(size_t)(c + TILE * dd) * TILE2 + a + TILE * b] = E(a * ni + c, b * nj + dd);
+        }
+    d.e_T.upload(T, s);
+    for (int L = 0; L < nL; ++L) lscale[L] = S.rs_param;
+  } else {
+    throw Error(HF_ESTATE, "range-separated integrals have not been computed");
+  }
+  d.rs_Lscale.upload(lscale, s);
+  d.rs_kind = S.rs_kind;
+  hip_check(hipStreamSynchronize(s), "range-separated upload");
+}
+
 // Build the Gaunt coupling tables and upload caches for the atomic kernels.
 void atomic_upload(hf_atomic *h) {
   const AtomicSetup &S = h->host;
@@ -29,43 +120,7 @@ void atomic_upload(hf_atomic *h) {
     if (S.rad.ni[e] > TILE) throw Error(HF_EINVAL, "elements with more than 16 functions are not supported");
 
   // radial factor tiles
-  std::vector<double> rs((size_t)nL * Nel * TILE2, 0.0), rb((size_t)nL * Nel * TILE2, 0.0);
-  std::vector<double> ch;
-  std::vector<int> choff((size_t)nL * Nel), chrank((size_t)nL * Nel);
-  int maxr = 0;
-  for (int L = 0; L < nL; ++L)
-    for (int e = 0; e < Nel; ++e) {
-      const size_t q = (size_t)L * Nel + e;
-      const int n = S.rad.ni[e];
-      const Blk &a = S.rsmall[q];
-      need(a.r == n && a.c == n, "r^L cache has the wrong shape");
-      for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) rs[q * TILE2 + i + TILE * j] = a(i, j);
-      if (e > 0) {
-        const Blk &b = S.rbig[q];
-        need(b.r == n && b.c == n, "r^-L-1 cache has the wrong shape");
-        for (int j = 0; j < n; ++j)
-          for (int i = 0; i < n; ++i) rb[q * TILE2 + i + TILE * j] = b(i, j);
-      }
-      const Blk &c = S.chol[q];
-      need(c.r == n * n, "Cholesky cache has the wrong number of rows");
-      choff[q] = (int)ch.size();
-      chrank[q] = c.c;
-      maxr = std::max(maxr, c.c);
-      need(c.c <= TILE2, "Cholesky rank above 256 is not supported");
-      const size_t base = ch.size();
-      ch.resize(base + (size_t)c.c * TILE2, 0.0);
-      for (int p = 0; p < c.c; ++p)
-        for (int bb = 0; bb < n; ++bb)
-          for (int aa = 0; aa < n; ++aa) ch[base + (size_t)p * TILE2 + aa + TILE * bb] = c(aa + bb * n, p);
-    }
-  d.max_rank = maxr;
-  d.rs.upload(rs, s);
-  d.rb.upload(rb, s);
-  if (ch.empty()) ch.push_back(0.0);
-  d.chol.upload(ch, s);
-  d.chol_off.upload(choff, s);
-  d.chol_rank.upload(chrank, s);
+  upload_factor_tiles(S, S.rsmall, S.rbig, S.chol, d.rs, d.rb, d.chol, d.chol_off, d.chol_rank, d.max_rank, s);
 
   // Coulomb tables (TwoDBasis.cpp:790-822, :852-874)
   const auto &lv = S.lval;
@@ -377,6 +432,24 @@ int hf_atomic_compute_tei(hf_atomic *h, int exchange) {
   });
 }
 
+int hf_atomic_compute_yukawa(hf_atomic *h, double lambda) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(lambda > 0.0 && std::isfinite(lambda), "Yukawa range parameter must be positive");
+    h->host.compute_yukawa(lambda);
+    h->rs_uploaded = false;
+  });
+}
+
+int hf_atomic_compute_erfc(hf_atomic *h, double mu) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(mu > 0.0 && std::isfinite(mu), "erfc range parameter must be positive");
+    h->host.compute_erfc(mu);
+    h->rs_uploaded = false;
+  });
+}
+
 int hf_atomic_set_cache(hf_atomic *h, int kind, int L, int iel, int rows, int cols, const double *data) {
   return guarded([&] {
     need(h != nullptr, "NULL handle");
@@ -483,6 +556,42 @@ int hf_atomic_exchange_fp32_dev(hf_atomic *h, const double *dP, int64_t ldp, dou
     check_matrix(dP, ldp, h->host.Nbf(), "P");
     check_matrix(dK, ldk, h->host.Nbf(), "K");
     atomic_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur, &h->ctx->timer, true);
+  });
+}
+
+namespace {
+void rs_ready(hf_atomic *h) {
+  ensure_ready(h);
+  if (h->host.rs_kind == 0) throw Error(HF_ESTATE, "Primitive teis have not been computed!");
+  if (!h->rs_uploaded) {
+    atomic_upload_rs(h);
+    h->rs_uploaded = true;
+  }
+}
+}  // namespace
+
+int hf_atomic_rs_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    rs_ready(h);
+    check_matrix(dP, ldp, h->host.Nbf(), "P");
+    check_matrix(dK, ldk, h->host.Nbf(), "K");
+    atomic_rs_exchange_launch(h->dev, dP, ldp, dK, ldk, h->ctx->cur, &h->ctx->timer);
+  });
+}
+
+int hf_atomic_rs_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, int64_t ldk) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    rs_ready(h);
+    host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, K, ldk,
+                   [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
+                     atomic_rs_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
+                   });
   });
 }
 

@@ -18,6 +18,7 @@ struct hf_atomic {
   hfg::AtomicSetup host;
   hfg::AtomicDevice dev;
   bool uploaded = false;
+  bool rs_uploaded = false;  // range-separated tables on the device
 };
 
 struct hf_diatomic {

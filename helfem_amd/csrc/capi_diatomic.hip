@@ -147,6 +147,25 @@ void diatomic_upload(hf_diatomic *h) {
   if (ch.empty()) ch.push_back(0.0);
   if (sg.empty()) sg.push_back(0.0);
   d.pq.upload(pq, s);
+  // log scale per (L,|M|, element) that brings the P and Q tiles to the same
+  // magnitude, sqrt(max|P| max|Q|): P_L(cosh mu) reaches 1e75 and Q_L 1e-56
+  // at L = 34, outside FP32, while their products across elements are O(1)
+  {
+    std::vector<double> lsig((size_t)std::max(nlm, 1) * Nel, 0.0);
+    for (size_t x = 0; x < (size_t)nlm * Nel; ++x) {
+      double mp = 0.0, mq = 0.0;
+      for (int kind = 0; kind < 4; ++kind)
+        for (int u = 0; u < TILE2; ++u) {
+          const double v = std::fabs(pq[(x * 4 + kind) * TILE2 + u]);
+          if (kind < 2)
+            mp = std::max(mp, v);
+          else
+            mq = std::max(mq, v);
+        }
+      if (mp > 0.0 && mq > 0.0) lsig[x] = 0.5 * (std::log(mp) - std::log(mq));
+    }
+    d.lsig.upload(lsig, s);
+  }
   d.chol.upload(ch, s);
   d.sigma.upload(sg, s);
   d.chol_off.upload(choff, s);

@@ -11,6 +11,7 @@ struct hf_dftgrid {
   const hfg::AtomicSetup *atomic = nullptr;
   const hfg::DiatomicSetup *diatomic = nullptr;
   int lang = 0, mang = 0;
+  bool full3d = false;  // diatomic: the general (mu, nu, phi) grid instead of the pure-m one
   hfg::XCGridDevice dev;
   bool uploaded = false;
   int64_t Nbf() const { return atomic ? atomic->Nbf() : diatomic->Nbf(); }
@@ -33,7 +34,10 @@ double theta_value(int l, int m, double x) {
 // at the radial Chebyshev points, weight w_cth 2pi/mang * h w_i * r_i^2 with
 // the compound angular rule of angular.cpp:20-72.  Pure-m diatomic
 // (dftgrid_purem.cpp:87-120): R = B at the mu points, weight
-// 2pi w_nu h w_i Rh^3 sinh(mu) (sinh^2 mu + sin^2 nu), one phi point.
+// 2pi w_nu h w_i Rh^3 sinh(mu) (sinh^2 mu + sin^2 nu), one phi point, only
+// shell pairs of equal m.  General 3D diatomic (src/diatomic/dftgrid.cpp:
+// 414-460): R = B, weight w_nu 2pi/mang h w_i Rh^3 sinh(mu) (sinh^2 mu +
+// sin^2 nu) on the same compound rule as the atomic grid, all shell pairs.
 void xc_upload(hf_dftgrid *g) {
   XCGridDevice &d = g->dev;
   hipStream_t s = g->ctx->cur;
@@ -51,7 +55,7 @@ void xc_upload(hf_dftgrid *g) {
   d.mang = g->mang;
   int mmax = 0;
   for (int m : mv) mmax = std::max(mmax, std::abs(m));
-  d.dmax = g->atomic ? 2 * mmax : 0;
+  d.dmax = (g->atomic || g->full3d) ? 2 * mmax : 0;
   const int nd = 2 * d.dmax + 1;
   // shell pairs within reach of the phi grid, numbered row by row
   std::vector<int> rstart(Na + 1, 0), rb, rp;
@@ -107,7 +111,8 @@ void xc_upload(hf_dftgrid *g) {
         const double shmu = std::sinh(x), Rh3 = std::pow(g->diatomic->Rh, 3);
         for (int t = 0; t < g->lang; ++t) {
           const double sth = std::sqrt(std::max((1.0 - xt[t]) * (1.0 + xt[t]), 0.0));
-          w[t] = 2.0 * kPi * wt[t] * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
+          const double wa = g->full3d ? wt[t] * dphi : 2.0 * kPi * wt[t];
+          w[t] = wa * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
         }
       }
     }
@@ -204,6 +209,20 @@ int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out) {
     g->diatomic = &h->host;
     g->lang = lang;
     g->mang = 1;
+    *out = g.release();
+  });
+}
+
+int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid **out) {
+  return guarded([&] {
+    need(h && out, "NULL argument");
+    need(lang >= 1 && mang >= 1, "grid orders must be positive");
+    std::unique_ptr<hf_dftgrid> g(new hf_dftgrid);
+    g->ctx = h->ctx;
+    g->diatomic = &h->host;
+    g->lang = lang;
+    g->mang = mang;
+    g->full3d = true;
     *out = g.release();
   });
 }

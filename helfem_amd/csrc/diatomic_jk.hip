@@ -244,6 +244,7 @@ struct DKArgs {
   const T *term_f;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
+  const double *lsig;  // FP32 only: log scale of the (ilm, element) P/Q tiles
 };
 
 // One workgroup per (active output block (j,k), element pair (e,f)), the
@@ -355,6 +356,14 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
       acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
       mma_pair<T>(r00, r20, b0, ya, yb);
       mma_pair<T>(r02, r22, b2, ya, yb);
+      if constexpr (sizeof(T) == 4) {
+        // FP32 tiles hold P / s and Q * s per (ilm, element): restore the
+        // pair's Q(e) P(f) or P(e) Q(f) scale, s_f / s_e or s_e / s_f
+        const double le = A.lsig[ilm * A.Nel + e], lf = A.lsig[ilm * A.Nel + f];
+        const T rho = (T)exp(up ? lf - le : le - lf);
+        ya *= rho;
+        yb *= rho;
+      }
       mma_pair2<T>(a0, ya, acc, a2, yb, acc2);  // acc += i0 ya, acc2 += i2 yb
     } else {
       const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
@@ -574,6 +583,20 @@ void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64
   hip_check(hipGetLastError(), "diatomic_block_norms");
 }
 
+namespace {
+// FP32 copy of the P/Q tiles with P kinds divided and Q kinds multiplied by
+// exp(lsig) of their (ilm, element).
+__global__ void k_scale_pq_f32(const double *pq, const double *lsig, float *pq32, int n_x) {
+  const int x = blockIdx.x;
+  if (x >= n_x) return;
+  const double up = exp(-lsig[x]), dn = exp(lsig[x]);
+  for (int u = threadIdx.x; u < 4 * TILE2; u += NT) {
+    const int kind = u / TILE2;
+    pq32[(size_t)x * 4 * TILE2 + u] = (float)(pq[(size_t)x * 4 * TILE2 + u] * (kind < 2 ? up : dn));
+  }
+}
+}  // namespace
+
 void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                                hipStream_t s, KernelTimer *timer, bool fp32) {
   fill_zero_launch(K, d.Nbf, d.Nbf, ldk, s);
@@ -583,7 +606,9 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       d.pq32.alloc(d.pq.n);
       d.chol32.alloc(d.chol.n);
       d.sigma32.alloc(d.sigma.n);
-      to_f32_launch(d.pq.p, d.pq32.p, d.pq.n, s);
+      const int n_x = (int)(d.pq.n / (4 * TILE2));
+      hipLaunchKernelGGL(k_scale_pq_f32, dim3(n_x), dim3(NT), 0, s, d.pq.p, d.lsig.p, d.pq32.p, n_x);
+      hip_check(hipGetLastError(), "scale_pq_f32");
       to_f32_launch(d.chol.p, d.chol32.p, d.chol.n, s);
       to_f32_launch(d.sigma.p, d.sigma32.p, d.sigma.n, s);
       d.f32_tables = true;
@@ -625,7 +650,7 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
     if (fp32) {
       DKArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                       d.pq32.p, d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p,
-                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_il.p, d.term_f32.p, efl, n_ef};
+                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_il.p, d.term_f32.p, efl, n_ef, d.lsig.p};
       if (part == 0)
         hipLaunchKernelGGL((k_dia_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
@@ -633,7 +658,7 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
     } else {
       DKArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
                        d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p,
-                       d.lrun_tstart.p, d.term_il.p, d.term_f.p, efl, n_ef};
+                       d.lrun_tstart.p, d.term_il.p, d.term_f.p, efl, n_ef, nullptr};
       if (part == 0)
         hipLaunchKernelGGL((k_dia_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else

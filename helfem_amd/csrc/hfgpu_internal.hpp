@@ -174,6 +174,12 @@ struct AtomicDevice {
   DevArray<int> L_rstart, L_runs;         // runs grouped by L
   DevArray<int> wg_in;                    // per workgroup (L, e, first run in group) or -1
   DevArray<double> Ypart;                 // [n_runs][Nel][TILE2]
+  // range-separated exchange (rs_kind 1 Yukawa: y_* factor tiles; 2 erfc:
+  // element-pair operators e_T [N_L][Nel][Nel][TILE2][TILE2]); per-L scale
+  // of the stored Gaunt weights
+  int rs_kind = 0;
+  DevArray<double> y_rs, y_rb, y_chol, e_T, rs_Lscale;
+  DevArray<int> y_chol_off, y_chol_rank;
   // FP32 copies for the reduced-precision exchange (built on first use)
   DevArray<float> rs32, rb32, chol32, term_c32, P32;
   bool f32_tables = false;
@@ -231,6 +237,7 @@ struct DiatomicDevice {
   DevArray<double> Ypart;                 // [n_runs][Nel][TILE2] in-element partial tiles
   // FP32 copies for the reduced-precision exchange (built on first use)
   DevArray<float> pq32, chol32, sigma32, term_f32, P32;
+  DevArray<double> lsig;                  // [n_lm*Nel] log of the P/Q scale of the FP32 tiles
   bool f32_tables = false;
   int64_t list_gen = 0, term_f32_gen = -1; // exchange-list version, and the one term_f32 holds
   int n_wg_in = 0;
@@ -271,6 +278,9 @@ void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, 
 // about 1e-6 relative error); otherwise the reference's FP64 arithmetic.
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer, bool fp32 = false);
+// Range-separated exchange -K_sr with the tables of atomic_upload_rs.
+void atomic_rs_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
+                               hipStream_t s, KernelTimer *timer);
 // Dense FP64 -> FP32 conversions (column-major with leading dimensions).
 void to_f32_launch(const double *src, float *dst, size_t n, hipStream_t s);
 void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStream_t s);

@@ -33,6 +33,8 @@ double blk_maxdiff(const Blk &A, const Blk &B) {
   return m;
 }
 
+}  // namespace
+
 // Order refinement shared by the one- and two-electron element integrals:
 // double the rule until the block changes by <= 8 eps, or until the change
 // stalls at the roundoff floor (FiniteElementBasis.cpp:500-553,
@@ -60,7 +62,7 @@ Blk refine(const std::function<Blk(int)> &eval, int nstart, int nmax) {
     n = std::min(2 * n, nmax);
   }
 }
-}  // namespace
+
 
 void gauss_lobatto(int n, std::vector<double> &x, std::vector<double> &w) {
   if (n < 2) throw std::runtime_error("Gauss-Lobatto rule needs n >= 2");
@@ -299,6 +301,20 @@ Blk RadialFE::kin_l(int iel) const {
 // contraction with B_i B_j, symmetrised.
 Blk twoe_fixed_rule(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &w,
                     const LipShape &shape, int L) {
+  const std::function<double(double, double)> fsb = [L](double r, double R) {
+    return std::pow(r / R, double(L)) / R;
+  };
+  const std::function<double(double)> fbig = [L](double r) { return std::pow(r, double(-L - 1)); };
+  return twoe_fixed_rule_k(rmin, rmax, x, w, shape, fsb, fbig);
+}
+
+// The same rule for a kernel f(r<, r>) = fsb(r<, r>) with the r>-dependence
+// fbig(r>) used to carry the cumulative inner integral between nodes
+// (r^L / r^(L+1) for Coulomb, i_L(lambda r) k_L(lambda R) for Yukawa,
+// quadrature.cpp:164-198).
+Blk twoe_fixed_rule_k(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &w,
+                      const LipShape &shape, const std::function<double(double, double)> &fsb,
+                      const std::function<double(double)> &fbig) {
   const int n = (int)x.size(), nb = shape.nbf(), nb2 = nb * nb;
   const double rmid = 0.5 * (rmax + rmin), rlen = 0.5 * (rmax - rmin);
   std::vector<double> r(n);
@@ -309,7 +325,7 @@ Blk twoe_fixed_rule(double rmin, double rmax, const std::vector<double> &x, cons
     std::vector<double> rr(n), wp(n), xp(n);
     for (int i = 0; i < n; ++i) {
       rr[i] = sm + sl * x[i];
-      wp[i] = w[i] * (std::pow(rr[i] / b, double(L)) / b) * sl;
+      wp[i] = w[i] * fsb(rr[i], b) * sl;
       xp[i] = (rr[i] - rmid) / rlen;
     }
     const Blk bf = shape.eval(xp, 0, rlen);
@@ -324,7 +340,7 @@ Blk twoe_fixed_rule(double rmin, double rmax, const std::vector<double> &x, cons
   for (int p = 1; p < n; ++p) segment(r[p - 1], r[p], p);
   for (int p = 1; p < n; ++p) {
     if (p == 1 && empty_first) continue;
-    const double fac = std::pow(r[p], double(-L - 1)) / std::pow(r[p - 1], double(-L - 1));
+    const double fac = fbig(r[p]) / fbig(r[p - 1]);
     for (int k = 0; k < nb2; ++k) inner(p, k) += inner(p - 1, k) * fac;
   }
   const Blk bf = shape.eval(x, 0, rlen);

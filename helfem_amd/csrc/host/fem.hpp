@@ -78,11 +78,30 @@ struct RadialFE {
   Blk kin(int iel) const;          // 1/2 int B' B'
   Blk kin_l(int iel) const;        // 1/2 int R R
   Blk twoe(int L, int iel) const;  // in-element Slater tensor, (ni^2 x ni^2)
+  // range-separated element integrals (RadialBasis.cpp:321-330, 719-812)
+  Blk bessel_i(int L, double lambda, int iel) const;  // int B B i_L(lambda r)
+  Blk bessel_k(int L, double lambda, int iel) const;  // int B B k_L(lambda r)
+  Blk yukawa(int L, double lambda, int iel) const;    // in-element Yukawa tensor
+  // erfc tensor of an element pair on the fixed Chebyshev rule, Coulomb
+  // ordered: rows a*ni + c (element iel), cols b*nk + d (element kel)
+  Blk erfc_pair(int L, double mu, int iel, int kel) const;
 };
 
 Blk twoe_fixed_rule(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &w,
                     const LipShape &shape, int L);
+Blk twoe_fixed_rule_k(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &w,
+                      const LipShape &shape, const std::function<double(double, double)> &fsb,
+                      const std::function<double(double)> &fbig);
+// Modified spherical Bessel functions i_L = sqrt(pi/2x) I_{L+1/2},
+// k_L = sqrt(2/(pi x)) K_{L+1/2} (libhelfem/include/math.h:52-115), and the
+// multipole kernel of erfc(mu d)/d = mu sum_L Phi_L(mu r, mu r') P_L(cos g).
+double sph_bessel_i(double x, int L);
+double sph_bessel_k(double x, int L);
+void erfc_multipoles(int Lmax, double x, double y, double *out);
 Blk pivoted_cholesky(const Blk &A, double tol);
+// Lobatto order doubling until the block changes by <= 8 eps (or stalls at
+// the roundoff floor), from nstart up to nmax points.
+Blk refine(const std::function<Blk(int)> &eval, int nstart, int nmax);
 
 // Gaunt coefficient int Y_L^M* Y_l^m Y_lp^mp, exact rational rounded once.
 double gaunt_coeff(int L, int M, int l, int m, int lp, int mp);
@@ -96,6 +115,14 @@ struct AtomicSetup {
   std::vector<Blk> rsmall, rbig, chol;  // index L*Nel + iel
   double chol_tol = 1e-12;
   bool have_tei = false;
+  // range separation (TwoDBasis.cpp:738-771): kind 1 Yukawa (rs_small/big =
+  // i_L/k_L element integrals, rs_chol), kind 2 erfc (rs_pair, index
+  // L*Nel*Nel + iel*Nel + kel, Coulomb-ordered element-pair tensors)
+  int rs_kind = 0;
+  double rs_param = 0.0;
+  std::vector<Blk> rs_small, rs_big, rs_chol, rs_pair;
+  void compute_yukawa(double lambda);
+  void compute_erfc(double mu);
   void build(int Z, int nnodes, int nquad, const std::vector<double> &bval, const std::vector<int> &lval,
              const std::vector<int> &mval);
   int Nrad() const { return rad.Nrad(); }

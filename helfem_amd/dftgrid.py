@@ -33,19 +33,23 @@ def parse_method(method: str):
 
 
 class DFTGrid:
-    """DFT quadrature on the GPU (hf_dftgrid): atomic grid for an AtomicBasis,
-    pure-m grid (mang unused) for a DiatomicBasis."""
+    """DFT quadrature on the GPU (hf_dftgrid): atomic grid for an AtomicBasis;
+    for a DiatomicBasis the pure-m grid (purem=True, mang unused) or the
+    general 3D grid (purem=False), as the diatomic driver's --purem switch."""
 
-    def __init__(self, basis, lang: int | None = None, mang: int | None = None):
+    def __init__(self, basis, lang: int | None = None, mang: int | None = None, purem: bool = True):
         from .diatomic import DiatomicBasis
         lmax = int(np.max(basis.lval))
         mmax = int(np.max(np.abs(basis.mval)))
         self.lang = 4 * lmax + 12 if lang is None else int(lang)
         self.basis = basis  # the grid points into the basis handle
         h = ctypes.c_void_p()
-        if isinstance(basis, DiatomicBasis):
+        if isinstance(basis, DiatomicBasis) and purem:
             self.mang = 1
             check(lib().hf_diatomic_dftgrid_create(basis._h, self.lang, ctypes.byref(h)))
+        elif isinstance(basis, DiatomicBasis):
+            self.mang = 4 * mmax + 12 if mang is None else int(mang)
+            check(lib().hf_diatomic_dftgrid3d_create(basis._h, self.lang, self.mang, ctypes.byref(h)))
         else:
             self.mang = 4 * mmax + 12 if mang is None else int(mang)
             check(lib().hf_atomic_dftgrid_create(basis._h, self.lang, self.mang, ctypes.byref(h)))

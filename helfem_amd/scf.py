@@ -1,12 +1,13 @@
-"""Device-resident restricted SCF around the GPU Fock build.
+"""Device-resident SCF (restricted and unrestricted) around the GPU Fock build.
 
-Counterpart of the Fock-builder of src/atomic/main.cpp:369-470 together with
-the SCF glue of src/general/scf_driver_common.h:
+Counterpart of the Fock builders of src/atomic/main.cpp:369-470 and
+src/diatomic/main.cpp:370-479 together with the SCF glue of
+src/general/scf_driver_common.h:
   * per-symmetry-block half-inverse overlap (build_per_block_Sinvh, :176-187;
     utils::invh, libhelfem/src/utils.cpp:120-159, chol=false),
   * P = sum_k Sinvh_k C diag(occ) C' Sinvh_k' (accumulate_density_block, :407-425),
-  * Etot = tr(PT) + tr(PV) + 1/2 tr(PJ) + Exc + Exx, Exx = 1/2 tr(Pa Ka)
-    doubled when restricted (assemble_hf_exchange, :460-474),
+  * Etot = tr(PT) + tr(PV) + 1/2 tr(PJ) + Exc + Exx + Enucr, Exx = 1/2 sum_s
+    tr(P_s K_s) (assemble_hf_exchange, :460-474),
   * F_k = Sinvh_k' (H1 + J + XC + K) Sinvh_k (assemble_fock_blocks, :490-520).
 The reference iterates with OpenOrbitalOptimizer (DIIS/ODA/CG); this driver
 uses aufbau over all blocks and Pulay DIIS on the orthonormal commutators.
@@ -14,12 +15,41 @@ Converged energies depend only on the state reached, not on the iteration.
 
 All matrices stay in HBM (torch float64); the per-block eigensolves run on
 the GPU (torch.linalg.eigh), J and K through the HIP kernels of libhfgpu.
+Only the symmetry blocks of the Fock matrix enter the eigensolves, so the
+DIIS history holds blocks, not Nbf x Nbf matrices.
 """
 from __future__ import annotations
 
 import time
 
 import numpy as np
+
+
+def _one_electron(basis, dev):
+    """S, T, V on the device.  The host matrices are Fortran-ordered, so their
+    transposes are C-contiguous views: those are sent and transposed back on
+    the device, avoiding a host-side copy of each Nbf x Nbf matrix."""
+    import torch
+    out = []
+    for M in (basis.overlap(), basis.kinetic(), basis.nuclear()):
+        out.append(torch.from_numpy(np.asarray(M).T).to(dev, dtype=torch.float64).T.contiguous())
+    return out
+
+
+class _Clock:
+    """Per-phase wall time, synchronised with the device at phase ends."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.t = {}
+        self.last = time.perf_counter()
+
+    def tick(self, name):
+        self.torch.cuda.synchronize()
+        now = time.perf_counter()
+        self.t[name] = self.t.get(name, 0.0) + now - self.last
+        self.last = now
 
 
 def _invh(S):
@@ -34,6 +64,324 @@ class SCFResult(dict):
     __getattr__ = dict.get
 
 
+class _Blocks:
+    """Per-symmetry-block orthogonalisation and eigensolves
+    (build_per_block_Sinvh, scf_driver_common.h:176-187).
+
+    Two exact reductions of the eigenproblems, both re-checked every call:
+      * a block (one m) splits into independent sub-problems wherever the
+        Fock and overlap matrices have no coupling between groups of shells
+        (the l shells of an atom with a spherical density);
+      * sub-problems holding the same matrix (the same l in every m of such
+        an atom, the m and -m blocks of a +-m symmetric density) are solved
+        once.
+    "No coupling" and "the same" are judged at 1e-12 of the block's largest
+    element, so an open shell that breaks a symmetry falls back to the full
+    eigensolve of its block."""
+
+    TOL = 1e-12
+
+    def __init__(self, basis, S, symm, dev):
+        import torch
+        self.torch = torch
+        idx = basis.get_sym_idx(symm)
+        self.b = [torch.as_tensor(b, dtype=torch.long, device=dev) for b in idx]
+        self.S = [S[b][:, b] for b in self.b]
+        self.X = [_invh(Sk) for Sk in self.S]
+        self.dev = dev
+        # shells of each block, in block order: (l, local start, size)
+        lval = np.asarray(basis.lval).tolist()
+        mval = np.asarray(basis.mval).tolist()
+        self.shells = []
+        if symm == 1:
+            for m in sorted(set(mval)):
+                sh, off = [], 0
+                for a in range(len(lval)):
+                    if mval[a] != m:
+                        continue
+                    n = basis.Nrad - (1 if (getattr(basis, "Ndummy", None) is not None and m != 0) else 0)
+                    sh.append((lval[a], off, n))
+                    off += n
+                self.shells.append(sh)
+        else:
+            self.shells = [[(-1, 0, len(b))] for b in idx]
+        for k, sh in enumerate(self.shells):
+            if sum(x[2] for x in sh) != len(idx[k]):
+                self.shells[k] = [(-1, 0, len(idx[k]))]
+        # overlap couplings between shells (fixed)
+        self.Scpl = [self._coupling(Sk, k) for k, Sk in enumerate(self.S)]
+        self.solves = 0
+        self.paired_solves = 0
+
+    def _coupling(self, M, k):
+        """(ns x ns) host matrix of max |M_st| over shell pairs (None when the
+        block is one shell group)."""
+        sh = self.shells[k]
+        if len(sh) == 1:
+            return None
+        n = sh[0][2]
+        if any(x[2] != n for x in sh):
+            return None
+        ns = len(sh)
+        return M.reshape(ns, n, ns, n).abs().amax(dim=(1, 3))
+
+    def blocks_of(self, F):
+        return [F[b][:, b] for b in self.b]
+
+    def _subproblems(self, Fb):
+        """Independent (block, shells) groups and, for each, the index of the
+        group whose solution it reuses (itself if solved)."""
+        torch = self.torch
+        cpl = [self._coupling(F, k) for k, F in enumerate(Fb)]
+        scale = torch.stack([torch.max(torch.abs(F)) for F in Fb])
+        host = [c.cpu().numpy() if c is not None else None for c in cpl]
+        scale = scale.cpu().numpy()
+        shost = [c.cpu().numpy() if c is not None else None for c in self.Scpl]
+        subs = []
+        for k, sh in enumerate(self.shells):
+            ns = len(sh)
+            if host[k] is None:
+                subs.append((k, list(range(ns))))
+                continue
+            thr = self.TOL * scale[k]
+            sthr = self.TOL * float(np.max(shost[k]))
+            parent = list(range(ns))
+
+            def find(x):
+                while parent[x] != x:
+                    parent[x] = parent[parent[x]]
+                    x = parent[x]
+                return x
+
+            for s_ in range(ns):
+                for t in range(s_ + 1, ns):
+                    if host[k][s_, t] > thr or host[k][t, s_] > thr or shost[k][s_, t] > sthr:
+                        parent[find(s_)] = find(t)
+            comps = {}
+            for s_ in range(ns):
+                comps.setdefault(find(s_), []).append(s_)
+            for c in sorted(comps.values()):
+                subs.append((k, c))
+        # candidate duplicates: same shell l's and sizes
+        keys = {}
+        for i, (k, c) in enumerate(subs):
+            key = tuple((self.shells[k][s_][0], self.shells[k][s_][2]) for s_ in c)
+            keys.setdefault(key, []).append(i)
+        loc = [self._local(k, c) for k, c in subs]
+        reuse = list(range(len(subs)))
+        cand = [(i, j) for members in keys.values() for a, i in enumerate(members) for j in members[:a]]
+        if cand:
+            diffs = []
+            for i, j in cand:
+                (ki, _), (kj, _) = subs[i], subs[j]
+                li, lj = loc[i], loc[j]
+                dF = torch.max(torch.abs(Fb[ki][li][:, li] - Fb[kj][lj][:, lj]))
+                dX = torch.max(torch.abs(self.X[ki][li][:, li] - self.X[kj][lj][:, lj]))
+                diffs.append(torch.stack([dF, dX]))
+            d = torch.stack(diffs).cpu().numpy()
+            xscale = [float(torch.max(torch.abs(X))) for X in self.X]
+            same = {(i, j): (dF <= self.TOL * scale[subs[i][0]] and dX <= self.TOL * xscale[subs[i][0]])
+                    for (i, j), (dF, dX) in zip(cand, d)}
+            for members in keys.values():
+                for a, i in enumerate(members):
+                    for j in members[:a]:
+                        if reuse[j] == j and same[(i, j)]:
+                            reuse[i] = j
+                            break
+        return subs, loc, reuse
+
+    def _local(self, k, comp):
+        sh = self.shells[k]
+        return self.torch.cat([self.torch.arange(sh[s_][1], sh[s_][1] + sh[s_][2], device=self.dev)
+                               for s_ in comp])
+
+    def density(self, Fb, nocc, weight, like):
+        """Aufbau over all sub-problems; returns the full P (for J/K), its
+        symmetry blocks and the occupied orbitals per block."""
+        torch = self.torch
+        subs, loc, reuse = self._subproblems(Fb)
+        sol = {}
+        for i, (k, c) in enumerate(subs):
+            if reuse[i] != i:
+                continue
+            li = loc[i]
+            X = self.X[k][li][:, li]
+            e, v = torch.linalg.eigh(X.T @ Fb[k][li][:, li] @ X)
+            sol[i] = (e, X @ v)
+            self.solves += 1
+        self.paired_solves += sum(1 for i in range(len(subs)) if reuse[i] != i)
+        ev = {i: sol[i][0].cpu().numpy() for i in sol}
+        order = sorted((float(e), i, j) for i in range(len(subs)) for j, e in enumerate(ev[reuse[i]]))
+        nsub = [0] * len(subs)
+        for _, i, _j in order[:nocc]:
+            nsub[i] += 1
+        P = like.new_zeros(like.shape)
+        Pb = [like.new_zeros((len(b), len(b))) for b in self.b]
+        nk = [0] * len(self.b)
+        for i, (k, c) in enumerate(subs):
+            if not nsub[i]:
+                continue
+            Cs = sol[reuse[i]][1][:, :nsub[i]]
+            Pk = weight * (Cs @ Cs.T)
+            li = loc[i]
+            Pb[k][li[:, None], li[None, :]] = Pk
+            nk[k] += nsub[i]
+        for k, b in enumerate(self.b):
+            if nk[k]:
+                P[b[:, None], b[None, :]] = Pb[k]
+        return P, Pb, nk
+
+    def commutator(self, Fb, Pb):
+        """Orthonormal-basis commutators X_k' (F P S - S P F) X_k per block."""
+        out = []
+        for k in range(len(self.b)):
+            FPS = Fb[k] @ Pb[k] @ self.S[k]
+            out.append(self.X[k].T @ (FPS - FPS.T) @ self.X[k])
+        return out
+
+
+class _DIIS:
+    """Pulay DIIS over lists of block tensors; the B matrix is kept on the
+    host and extended by one row of dot products per iteration."""
+
+    def __init__(self, size):
+        self.size = size
+        self.F, self.e = [], []
+        self.B = np.zeros((0, 0))
+
+    def reset(self):
+        self.F, self.e = [], []
+        self.B = np.zeros((0, 0))
+
+    def push(self, F, e):
+        import torch
+        if len(self.F) == self.size:
+            self.F.pop(0)
+            self.e.pop(0)
+            self.B = self.B[1:, 1:]
+        dots = [sum(torch.sum(a * b) for a, b in zip(e, ei)) for ei in self.e] + [sum(torch.sum(a * a) for a in e)]
+        row = torch.stack([torch.as_tensor(d) for d in dots]).cpu().numpy()
+        n = len(self.F) + 1
+        B = np.zeros((n, n))
+        B[:n - 1, :n - 1] = self.B
+        B[n - 1, :] = row
+        B[:, n - 1] = row
+        self.B = B
+        self.F.append(F)
+        self.e.append(e)
+
+    def extrapolate(self):
+        n = len(self.F)
+        A = -np.ones((n + 1, n + 1))
+        A[:n, :n] = self.B
+        A[n, n] = 0.0
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        try:
+            c = np.linalg.solve(A, rhs)[:n]
+        except np.linalg.LinAlgError:
+            return self.F[-1]
+        return [sum(float(c[i]) * self.F[i][k] for i in range(n)) for k in range(len(self.F[-1]))]
+
+
+def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch):
+    """Shared SCF loop.  Restricted: nel_spin = (nocc,), xc(P) -> (Vxc, Exc).
+    Unrestricted: nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
+    import torch
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+    dev = torch.device(device)
+    f64 = torch.float64
+    t0 = time.perf_counter()
+    clk = _Clock()
+    S, T, V = _one_electron(basis, dev)
+    H = T + V
+    bs = _Blocks(basis, S, symm, dev)
+    Hb = bs.blocks_of(H)
+    Enucr = float(getattr(basis, "Enucr", 0.0))  # nuclear repulsion of a diatomic basis
+    clk.tick("setup")
+    nspin = 1 if restricted else 2
+    weight = 2.0 if restricted else 1.0
+    fp32 = precision == "mixed"
+    it_fp32 = 0
+    dens = [bs.density(Hb, nel_spin[s], weight, H) for s in range(nspin)]
+    clk.tick("density")
+    diis = _DIIS(diis_len)
+    Eold, E, it = None, 0.0, 0
+    hist = []
+    Ekin = Enuc = Ecoul = Exx = Exc = torch.zeros((), dtype=f64, device=dev)
+    for it in range(1, maxit + 1):
+        Ps = [d[0] for d in dens]
+        P = Ps[0] if restricted else Ps[0] + Ps[1]
+        J = basis.coulomb(P)
+        clk.tick("coulomb")
+        Ecoul = 0.5 * torch.sum(P * J)
+        Fs = [H + J for _ in range(nspin)]
+        Exx = torch.zeros((), dtype=f64, device=dev)
+        Exc = torch.zeros((), dtype=f64, device=dev)
+        if xc is None:
+            kp = "fp32" if fp32 else "fp64"
+            for s in range(nspin):
+                Psp = 0.5 * Ps[s] if restricted else Ps[s]
+                if nel_spin[s] == 0:
+                    continue
+                Ks = basis.exchange(Psp, precision=kp)
+                Exx = Exx + (2.0 if restricted else 1.0) * 0.5 * torch.sum(Psp * Ks)
+                Fs[s] = Fs[s] + Ks
+            clk.tick("exchange")
+        else:
+            if restricted:
+                Vxc, Exc = xc(P)
+                Fs[0] = Fs[0] + Vxc
+            else:
+                Va, Vb, Exc = xc(Ps[0], Ps[1])
+                Fs[0] = Fs[0] + Va
+                Fs[1] = Fs[1] + Vb
+            clk.tick("xc")
+        Ekin = torch.sum(P * T)
+        Enuc = torch.sum(P * V)
+        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + Enucr
+        Fb = [bs.blocks_of(F) for F in Fs]
+        err = []
+        for s in range(nspin):
+            err += bs.commutator(Fb[s], dens[s][1])
+        emax = float(torch.max(torch.stack([torch.max(torch.abs(x)) for x in err])))
+        hist.append(E)
+        clk.tick("energy+diis")
+        if verbose:
+            print(f"it {it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if fp32 else ''}", flush=True)
+        if fp32:
+            it_fp32 = it
+            if emax < switch:
+                # FP64 from here on: the FP32 Fock matrices leave the DIIS
+                # history and this step is a plain diagonalisation
+                fp32 = False
+                diis.reset()
+                Eold = None
+                dens = [bs.density(Fb[s], nel_spin[s], weight, H) for s in range(nspin)]
+                clk.tick("density")
+                continue
+        elif Eold is not None and abs(E - Eold) < conv and emax < 1e-6:
+            break
+        else:
+            Eold = E
+        nb = len(bs.b)
+        diis.push([x for F in Fb for x in F], err)
+        Fd = diis.extrapolate()
+        clk.tick("energy+diis")
+        dens = [bs.density(Fd[s * nb:(s + 1) * nb], nel_spin[s], weight, H) for s in range(nspin)]
+        clk.tick("density")
+    torch.cuda.synchronize(dev)
+    out = SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc),
+                    Enucr=Enucr, iterations=it, iterations_fp32=it_fp32, seconds=time.perf_counter() - t0,
+                    history=hist, phases=clk.t, eigensolves=bs.solves, reused_solves=bs.paired_solves)
+    if restricted:
+        out.update(P=dens[0][0], occupations=dens[0][2])
+    else:
+        out.update(Pa=dens[0][0], Pb=dens[1][0], occupations=(dens[0][2], dens[1][2]))
+    return out
+
+
 def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100, diis_len: int = 10,
         verbose: bool = False, xc=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3):
     """Closed-shell SCF; returns energies, iterations and the final density.
@@ -43,119 +391,7 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
     (the largest DIIS commutator element) is above `switch`, then restarts
     DIIS and finishes in FP64; overlap, orthogonalisation, eigensolves and
     energy sums are FP64 throughout."""
-    import torch
-    dev = torch.device(device)
-    f64 = torch.float64
-    t0 = time.perf_counter()
-    S = torch.as_tensor(np.ascontiguousarray(basis.overlap()), dtype=f64, device=dev)
-    T = torch.as_tensor(np.ascontiguousarray(basis.kinetic()), dtype=f64, device=dev)
-    V = torch.as_tensor(np.ascontiguousarray(basis.nuclear()), dtype=f64, device=dev)
-    H = T + V
-    blocks = [torch.as_tensor(b, dtype=torch.long, device=dev) for b in basis.get_sym_idx(symm)]
-    X = [_invh(S[b][:, b]) for b in blocks]
-    nocc = nelec // 2
-    have_exx = xc is None
-    Enucr = float(getattr(basis, "Enucr", 0.0))  # nuclear repulsion of a diatomic basis
-
-    def density(F):
-        evs, vecs = [], []
-        for k, b in enumerate(blocks):
-            Fo = X[k].T @ F[b][:, b] @ X[k]
-            e, c = torch.linalg.eigh(Fo)
-            evs.append(e)
-            vecs.append(c)
-        ev_host = [e.cpu().numpy() for e in evs]
-        order = sorted((float(e), k, i) for k, ev in enumerate(ev_host) for i, e in enumerate(ev))
-        nk = [0] * len(blocks)
-        for _, k, _i in order[:nocc]:
-            nk[k] += 1
-        P = torch.zeros_like(F)
-        for k, b in enumerate(blocks):
-            if nk[k]:
-                C = X[k] @ vecs[k][:, :nk[k]]
-                P[b[:, None], b[None, :]] += 2.0 * (C @ C.T)
-        return P, nk, ev_host
-
-    if precision not in ("fp64", "mixed"):
-        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
-    fp32 = precision == "mixed"
-    it_fp32 = 0
-    P, nk, _ = density(H)
-    diis_F, diis_e = [], []
-    Eold = None
-    hist = []
-    E = 0.0
-    it = 0
-    for it in range(1, maxit + 1):
-        J = basis.coulomb(P)
-        Ecoul = 0.5 * torch.sum(P * J)
-        F = H + J
-        Exx = torch.zeros((), dtype=f64, device=dev)
-        Exc = torch.zeros((), dtype=f64, device=dev)
-        if have_exx:
-            Pa = 0.5 * P
-            Ka = basis.exchange(Pa, precision="fp32" if fp32 else "fp64")
-            Exx = torch.sum(Pa * Ka)
-            F = F + Ka
-        else:
-            Vxc, Exc = xc(P)
-            F = F + Vxc
-        Ekin = torch.sum(P * T)
-        Enuc = torch.sum(P * V)
-        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + Enucr
-        errs = []
-        for k, b in enumerate(blocks):
-            Fk, Pk, Sk = F[b][:, b], P[b][:, b], S[b][:, b]
-            FPS = Fk @ Pk @ Sk
-            errs.append((X[k].T @ (FPS - FPS.T) @ X[k]).reshape(-1))
-        err = torch.cat(errs)
-        emax = float(torch.max(torch.abs(err)))
-        hist.append(E)
-        if verbose:
-            print(f"it {it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if fp32 else ''}", flush=True)
-        if fp32:
-            it_fp32 = it
-            if emax < switch:
-                # FP64 from here on: the FP32 Fock matrices leave the DIIS
-                # history and this step is a plain diagonalisation
-                fp32 = False
-                diis_F, diis_e, Eold = [], [], None
-                P, nk, _ = density(F)
-                continue
-        elif Eold is not None and abs(E - Eold) < conv and emax < 1e-6:
-            break
-        else:
-            Eold = E
-        diis_F.append(F)
-        diis_e.append(err)
-        if len(diis_F) > diis_len:
-            diis_F.pop(0)
-            diis_e.pop(0)
-        c = _diis(diis_F, diis_e, dev)
-        Fd = F if c is None else sum(c[i] * Fi for i, Fi in enumerate(diis_F))
-        P, nk, _ = density(Fd)
-    torch.cuda.synchronize(dev)
-    return SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc), Enucr=Enucr,
-                     iterations=it, iterations_fp32=it_fp32, P=P, occupations=nk,
-                     seconds=time.perf_counter() - t0, history=hist)
-
-
-def _diis(diis_F, diis_e, dev):
-    """Pulay extrapolation coefficients over the stored commutators."""
-    import torch
-    n = len(diis_e)
-    E_mat = torch.stack(diis_e)
-    B = torch.empty((n + 1, n + 1), dtype=torch.float64, device=dev)
-    B[:n, :n] = E_mat @ E_mat.T
-    B[n, :] = -1.0
-    B[:, n] = -1.0
-    B[n, n] = 0.0
-    rhs = torch.zeros(n + 1, dtype=torch.float64, device=dev)
-    rhs[n] = -1.0
-    try:
-        return torch.linalg.solve(B, rhs)[:n]
-    except RuntimeError:
-        return None
+    return _scf(basis, (nelec // 2,), True, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch)
 
 
 def uhf(basis, nela: int, nelb: int, symm: int = 1, conv: float = 1e-10, maxit: int = 200, diis_len: int = 10,
@@ -166,96 +402,4 @@ def uhf(basis, nela: int, nelb: int, symm: int = 1, conv: float = 1e-10, maxit: 
     xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects spin-polarized Kohn-Sham DFT;
     None is UHF with Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb)).  precision="mixed"
     as for rhf."""
-    import torch
-    dev = torch.device(device)
-    f64 = torch.float64
-    t0 = time.perf_counter()
-    S = torch.as_tensor(np.ascontiguousarray(basis.overlap()), dtype=f64, device=dev)
-    T = torch.as_tensor(np.ascontiguousarray(basis.kinetic()), dtype=f64, device=dev)
-    V = torch.as_tensor(np.ascontiguousarray(basis.nuclear()), dtype=f64, device=dev)
-    H = T + V
-    blocks = [torch.as_tensor(b, dtype=torch.long, device=dev) for b in basis.get_sym_idx(symm)]
-    X = [_invh(S[b][:, b]) for b in blocks]
-    Enucr = float(getattr(basis, "Enucr", 0.0))
-
-    def density(F, nocc):
-        evs, vecs = [], []
-        for k, b in enumerate(blocks):
-            e, c = torch.linalg.eigh(X[k].T @ F[b][:, b] @ X[k])
-            evs.append(e)
-            vecs.append(c)
-        ev_host = [e.cpu().numpy() for e in evs]
-        order = sorted((float(e), k, i) for k, ev in enumerate(ev_host) for i, e in enumerate(ev))
-        nk = [0] * len(blocks)
-        for _, k, _i in order[:nocc]:
-            nk[k] += 1
-        P = torch.zeros_like(F)
-        for k, b in enumerate(blocks):
-            if nk[k]:
-                C = X[k] @ vecs[k][:, :nk[k]]
-                P[b[:, None], b[None, :]] += C @ C.T
-        return P, nk
-
-    if precision not in ("fp64", "mixed"):
-        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
-    fp32 = precision == "mixed"
-    it_fp32 = 0
-    (Pa, nka), (Pb, nkb) = density(H, nela), density(H, nelb)
-    diis_F, diis_e = [], []
-    Eold, E, it = None, 0.0, 0
-    hist = []
-    for it in range(1, maxit + 1):
-        P = Pa + Pb
-        J = basis.coulomb(P)
-        Ecoul = 0.5 * torch.sum(P * J)
-        Exx = torch.zeros((), dtype=f64, device=dev)
-        Exc = torch.zeros((), dtype=f64, device=dev)
-        if xc_pol is None:
-            kp = "fp32" if fp32 else "fp64"
-            Ka = basis.exchange(Pa, precision=kp)
-            Kb = basis.exchange(Pb, precision=kp) if nelb else torch.zeros_like(Pb)
-            Exx = 0.5 * (torch.sum(Pa * Ka) + torch.sum(Pb * Kb))
-            Fa, Fb = H + J + Ka, H + J + Kb
-        else:
-            Va, Vb, Exc = xc_pol(Pa, Pb)
-            Fa, Fb = H + J + Va, H + J + Vb
-        Ekin = torch.sum(P * T)
-        Enuc = torch.sum(P * V)
-        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + Enucr
-        errs = []
-        for F_, P_ in ((Fa, Pa), (Fb, Pb)):
-            for k, b in enumerate(blocks):
-                FPS = F_[b][:, b] @ P_[b][:, b] @ S[b][:, b]
-                errs.append((X[k].T @ (FPS - FPS.T) @ X[k]).reshape(-1))
-        err = torch.cat(errs)
-        emax = float(torch.max(torch.abs(err)))
-        hist.append(E)
-        if verbose:
-            print(f"it {it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if fp32 else ''}", flush=True)
-        if fp32:
-            it_fp32 = it
-            if emax < switch:
-                fp32 = False
-                diis_F, diis_e, Eold = [], [], None
-                (Pa, nka), (Pb, nkb) = density(Fa, nela), density(Fb, nelb)
-                continue
-        elif Eold is not None and abs(E - Eold) < conv and emax < 1e-6:
-            break
-        else:
-            Eold = E
-        diis_F.append((Fa, Fb))
-        diis_e.append(err)
-        if len(diis_F) > diis_len:
-            diis_F.pop(0)
-            diis_e.pop(0)
-        c = _diis(diis_F, diis_e, dev)
-        if c is None:
-            Fad, Fbd = Fa, Fb
-        else:
-            Fad = sum(c[i] * f[0] for i, f in enumerate(diis_F))
-            Fbd = sum(c[i] * f[1] for i, f in enumerate(diis_F))
-        (Pa, nka), (Pb, nkb) = density(Fad, nela), density(Fbd, nelb)
-    torch.cuda.synchronize(dev)
-    return SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc),
-                     Enucr=Enucr, iterations=it, iterations_fp32=it_fp32, Pa=Pa, Pb=Pb, occupations=(nka, nkb),
-                     seconds=time.perf_counter() - t0, history=hist)
+    return _scf(basis, (nela, nelb), False, symm, conv, maxit, diis_len, verbose, xc_pol, device, precision, switch)

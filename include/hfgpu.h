@@ -119,6 +119,19 @@ int hf_atomic_get_cache(const hf_atomic *h, int kind, int L, int iel, int dims[2
  * 1 kinetic (:354), 2 nuclear attraction (:377). */
 int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t ld);
 
+/* Range-separated two-electron caches: the Yukawa kernel exp(-lambda r12)/r12
+ * (TwoDBasisT::compute_yukawa, TwoDBasis.h:183, TwoDBasis.cpp:738-760) or
+ * the erfc kernel erfc(mu r12)/r12 (compute_erfc, TwoDBasis.h:185,
+ * TwoDBasis.cpp:762-771).  The latest call decides the kernel. */
+int hf_atomic_compute_yukawa(hf_atomic *h, double lambda);
+int hf_atomic_compute_erfc(hf_atomic *h, double mu);
+/* Short-range exchange -K_sr(P) of the spin density P with that kernel.
+ * Replaces TwoDBasisT::rs_exchange (TwoDBasis.h:227, TwoDBasis.cpp:1002-1131;
+ * erfc element pairs as CoulombExchangeFE.h:412-440).  HF_ESTATE before
+ * either compute call. */
+int hf_atomic_rs_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, int64_t ldk);
+int hf_atomic_rs_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+
 /* Coulomb matrix J(P) for the total density P.  Replaces
  * TwoDBasisT::coulomb (TwoDBasis.h:223, TwoDBasis.cpp:774-877). */
 int hf_atomic_coulomb(hf_atomic *h, const double *P, int64_t ldp, double *J, int64_t ldj);
@@ -155,6 +168,15 @@ int hf_dftgrid_destroy(hf_dftgrid *g);
  * diatomic/main.cpp:316).  Matrices are in the pure basis.  Destroy the grid
  * before the basis. */
 int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out);
+/* General 3D DFT quadrature of a diatomic basis: the mu points of every
+ * element x the compound (cos nu, phi) rule of `lang` Chebyshev x `mang`
+ * uniform points, all shell pairs (densities may mix m).  Replaces the
+ * constructor helfem::diatomic::dftgrid::DFTGrid(basis, lang, mang)
+ * (src/diatomic/dftgrid.h, dftgrid.cpp:523-528; the driver uses it when the
+ * pure-m path is off, diatomic/main.cpp:326-330, defaults lang = 4 lmax + 12,
+ * mang = 4 mmax + 12).  Evaluated with hf_dftgrid_eval_fxc[_pol][_dev]
+ * (dftgrid.cpp:533-620). */
+int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid **out);
 /* Restricted exchange-correlation matrix H (Nbf x Nbf, overwritten) of the
  * total density P, with out[0] = Exc, out[1] = Nel = integral of the
  * density, out[2] = kinetic energy density integral (0 for LDA).  x_func /

@@ -4,6 +4,7 @@
 #include "diatomic.hpp"
 #include "dftgrid.hpp"
 #include <cstring>
+#include <stdexcept>
 #include <string>
 
 using namespace orc;
@@ -141,6 +142,41 @@ int orc_atomic_exchange(void *h, const double *P, double *K) {
     AtomicBasis *b = (AtomicBasis *)h;
     put(b->exchange(get(P, b->Nbf(), b->Nbf())), K);
   });
+}
+
+// Range-separated exchange: caches, then -K_sr (TwoDBasis.cpp:738-771, 1002-1131)
+int orc_atomic_compute_yukawa(void *h, double lambda) {
+  return guard([&] { ((AtomicBasis *)h)->compute_yukawa(lambda); });
+}
+int orc_atomic_compute_erfc(void *h, double mu) {
+  return guard([&] { ((AtomicBasis *)h)->compute_erfc(mu); });
+}
+int orc_atomic_rs_exchange(void *h, const double *P, double *K) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    put(b->rs_exchange(get(P, b->Nbf(), b->Nbf())), K);
+  });
+}
+// Range-separated cache read-back: kind 0 i_L element integral, 1 k_L, 2
+// Yukawa Cholesky factor (L*Nel + iel), 3 erfc exchange-ordered tensor
+// (L*Nel*Nel + iel*Nel + jel).  dims = (rows, cols); data may be NULL.
+int orc_atomic_rs_cache(void *h, int kind, int index, int *dims, double *data) {
+  return guard([&] {
+    AtomicBasis *b = (AtomicBasis *)h;
+    const std::vector<Mat> *v = kind == 0 ? &b->disjoint_iL : kind == 1 ? &b->disjoint_kL
+                                : kind == 2 ? &b->rs_chol : &b->rs_ktei;
+    if (index < 0 || index >= (int)v->size()) throw std::invalid_argument("cache index out of range");
+    const Mat &M = (*v)[index];
+    dims[0] = M.r;
+    dims[1] = M.c;
+    if (data) put(M, data);
+  });
+}
+// Special functions: i_L, k_L and the erfc multipole kernel Phi_0..Lmax
+double orc_bessel_il(double x, int L) { return bessel_il(x, L); }
+double orc_bessel_kl(double x, int L) { return bessel_kl(x, L); }
+int orc_erfc_phi(int Lmax, double x, double y, double *out) {
+  return guard([&] { erfc_phi(Lmax, x, y, out); });
 }
 
 int orc_atomic_exchange_blocks(void *h, const double *P, double *K, int nblocks, const int *jk) {
@@ -290,6 +326,30 @@ int orc_diatomic_xc_pol(void *h, int lang, int xf, int cf, const double *Pa, con
   return guard([&] {
     DiatomicBasis *b = (DiatomicBasis *)h;
     DiatomicXC g(*b, lang);
+    Mat A, B;
+    g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    put(A, Ha);
+    put(B, Hb);
+  });
+}
+// General 3D diatomic grid (lang x mang): restricted and spin-polarized
+int orc_diatomic_xc3d(void *h, int lang, int mang, int xf, int cf, const double *P, double thr, double *H,
+                      double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    if (mang < 1) throw std::invalid_argument("mang must be positive");
+    DiatomicXC g(*b, lang, mang);
+    Mat Hm;
+    g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    put(Hm, H);
+  });
+}
+int orc_diatomic_xc3d_pol(void *h, int lang, int mang, int xf, int cf, const double *Pa, const double *Pb,
+                          double thr, double *Ha, double *Hb, double *out) {
+  return guard([&] {
+    DiatomicBasis *b = (DiatomicBasis *)h;
+    if (mang < 1) throw std::invalid_argument("mang must be positive");
+    DiatomicXC g(*b, lang, mang);
     Mat A, B;
     g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
     put(A, Ha);

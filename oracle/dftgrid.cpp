@@ -234,7 +234,10 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
   }
 }
 
-DiatomicXC::DiatomicXC(const DiatomicBasis &b_, int lang_) : b(b_), lang(lang_) { chebyshev(lang, cth, wang); }
+DiatomicXC::DiatomicXC(const DiatomicBasis &b_, int lang_, int mang_) : b(b_), lang(lang_), mang(mang_) {
+  chebyshev(lang, cth, wang);
+  if (mang > 0) angular_chebyshev(lang, mang, cth3, phi3, wang3);
+}
 
 // PureMDFTGrid::eval_Fxc (restricted), src/diatomic/dftgrid_purem.cpp:675-703,
 // with the worker's compute_bf (:87-199), update_density (:212-260) and
@@ -251,8 +254,102 @@ void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha,
   eval(xf, cf, Pa, &Pb, Ha, Hb, Exc, Nel, thr);
 }
 
+// The general 3D grid (DFTGrid::eval_Fxc, src/diatomic/dftgrid.cpp:533-620):
+// per element and radial point, the complex basis values B_j(mu) Y_l^m(nu, phi)
+// (eval_bf) on the compound (cos nu, phi) rule, total weight
+// wang * w_mu * Rh^3 sinh(mu) (sinh^2 mu + sin^2 nu) (compute_bf, :414-460),
+// density and potential as in the atomic grid (update_density, increment_lda).
+void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout, Mat &Hbout, double &Exc,
+                        double &Nel, double thr) const {
+  const bool pol = Pbin != nullptr;
+  const RadialBasis &rad = b.radial;
+  const Mat P = b.expand_boundaries(Pin);
+  const Mat Pbx = pol ? b.expand_boundaries(*Pbin) : Mat();
+  const int Na = b.Nang(), Nr = rad.Nbf(), nang = (int)wang3.size();
+  Mat H(b.Ndummy(), b.Ndummy()), H2(pol ? b.Ndummy() : 0, pol ? b.Ndummy() : 0);
+  Exc = 0.0;
+  Nel = 0.0;
+  const double Rh3 = std::pow(b.Rhalf, 3);
+  for (int iel = 0; iel < rad.Nel(); ++iel) {
+    const Mat Bv = rad.eval_dnf(rad.xq, 0, iel);  // (nq x Ni)
+    const std::vector<double> mu = rad.eval_coord(rad.xq, iel);
+    const double h = rad.scaling_factor(iel);
+    const int nrad = (int)mu.size(), Ni = Bv.c, nbf = Na * Ni, npt = nrad * nang;
+    std::vector<int> idx(nbf);
+    for (int a = 0; a < Na; ++a)
+      for (int j = 0; j < Ni; ++j) idx[a * Ni + j] = a * Nr + rad.first[iel] + j;
+    std::vector<double> wtot(npt);
+    for (int ia = 0; ia < nang; ++ia) {
+      const double sth = std::sqrt(std::max((1.0 - cth3[ia]) * (1.0 + cth3[ia]), 0.0));
+      for (int ir = 0; ir < nrad; ++ir) {
+        const double shmu = std::sinh(mu[ir]);
+        wtot[ia * nrad + ir] = wang3[ia] * (h * rad.wq[ir]) * Rh3 * shmu * (shmu * shmu + sth * sth);
+      }
+    }
+    Mat are(nbf, npt), aim(nbf, npt);  // real and imaginary parts of conj(chi)
+    for (int ia = 0; ia < nang; ++ia)
+      for (int a = 0; a < Na; ++a) {
+        const std::complex<double> y = std::conj(spherical_harmonic(b.lval[a], b.mval[a], cth3[ia], phi3[ia]));
+        for (int j = 0; j < Ni; ++j)
+          for (int ir = 0; ir < nrad; ++ir) {
+            are(a * Ni + j, ia * nrad + ir) = y.real() * Bv(ir, j);
+            aim(a * Ni + j, ia * nrad + ir) = y.imag() * Bv(ir, j);
+          }
+      }
+    auto density = [&](const Mat &Pm) {
+      Mat Pe(nbf, nbf);
+      for (int v = 0; v < nbf; ++v)
+        for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
+      const Mat PA = matmul(Pe, are), PB = matmul(Pe, aim);
+      std::vector<double> rho(npt);
+      for (int p = 0; p < npt; ++p) {
+        double s = 0.0, t = 0.0;
+        for (int u = 0; u < nbf; ++u) {
+          s += PA(u, p) * are(u, p);
+          t += PB(u, p) * aim(u, p);
+        }
+        rho[p] = s + t;
+      }
+      return rho;
+    };
+    const std::vector<double> ra = density(P);
+    const std::vector<double> rb = pol ? density(Pbx) : std::vector<double>();
+    std::vector<double> va(npt), vb(npt);
+    for (int p = 0; p < npt; ++p) {
+      double exc;
+      if (pol) {
+        lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
+        Nel += wtot[p] * (ra[p] + rb[p]);
+        Exc += wtot[p] * exc * (ra[p] + rb[p]);
+      } else {
+        lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
+        Nel += wtot[p] * ra[p];
+        Exc += wtot[p] * exc * ra[p];
+      }
+    }
+    auto potential = [&](const std::vector<double> &vr, Mat &Hm) {
+      Mat av = are, bv = aim;
+      for (int p = 0; p < npt; ++p) {
+        const double f = vr[p] * wtot[p];
+        for (int u = 0; u < nbf; ++u) {
+          av(u, p) *= f;
+          bv(u, p) *= f;
+        }
+      }
+      const Mat Ha = matmul_nt(av, are), Hb2 = matmul_nt(bv, aim);
+      for (int v = 0; v < nbf; ++v)
+        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Ha(u, v) + Hb2(u, v);
+    };
+    potential(va, H);
+    if (pol) potential(vb, H2);
+  }
+  Hout = b.remove_boundaries(H);
+  if (pol) Hbout = b.remove_boundaries(H2);
+}
+
 void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout, Mat &Hbout, double &Exc,
                       double &Nel, double thr) const {
+  if (mang > 0) return eval3d(xf, cf, Pin, Pbin, Hout, Hbout, Exc, Nel, thr);
   const bool pol = Pbin != nullptr;
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);

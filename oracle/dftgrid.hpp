@@ -25,17 +25,21 @@ struct AtomicXC {
 };
 
 struct DiatomicBasis;
-// diatomic::dftgrid_purem::PureMDFTGrid (src/diatomic/dftgrid_purem.h), LDA restricted
+// diatomic::dftgrid_purem::PureMDFTGrid (src/diatomic/dftgrid_purem.h) when
+// mang == 0; the general 3D diatomic::dftgrid::DFTGrid (src/diatomic/dftgrid.h,
+// dftgrid.cpp:533-620) on the lang x mang (cos nu, phi) rule when mang > 0.
 struct DiatomicXC {
   const DiatomicBasis &b;
-  int lang;
-  std::vector<double> cth, wang;
-  DiatomicXC(const DiatomicBasis &b, int lang);
+  int lang, mang;
+  std::vector<double> cth, wang, phi3, cth3, wang3;
+  DiatomicXC(const DiatomicBasis &b, int lang, int mang = 0);
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
   void eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, Mat &Hb, double &Exc, double &Nel,
                 double thr) const;
   void eval(int xf, int cf, const Mat &P, const Mat *Pb, Mat &H, Mat &Hb, double &Exc, double &Nel,
             double thr) const;
+  void eval3d(int xf, int cf, const Mat &P, const Mat *Pb, Mat &H, Mat &Hb, double &Exc, double &Nel,
+              double thr) const;
 };
 
 }  // namespace orc

@@ -379,17 +379,18 @@ Mat RadialBasis::kinetic_l(int iel) const {
 }
 
 namespace {
-// quadrature.cpp:36-74
+// quadrature.cpp:36-74; fsb(r, R) is the kernel r<-part times R>-part
+// (r^L / R^(L+1) for Coulomb, i_L(lambda r) k_L(lambda R) for Yukawa)
 std::vector<double> inner_wrk(double rmin, double rmax, double rmin0, double rmax0,
                               const std::vector<double> &x, const std::vector<double> &wx,
-                              const LIPBasis &poly, int L) {
+                              const LIPBasis &poly, const std::function<double(double, double)> &fsbf) {
   const double rmid = 0.5 * (rmax + rmin), rlen = 0.5 * (rmax - rmin);
   const double rmid0 = 0.5 * (rmax0 + rmin0), rlen0 = 0.5 * (rmax0 - rmin0);
   const size_t n = x.size();
   std::vector<double> r(n), wp(n), xp(n);
   for (size_t i = 0; i < n; ++i) {
     r[i] = rmid + rlen * x[i];
-    const double fsb = std::pow(r[i] / rmax, double(L)) / rmax;
+    const double fsb = fsbf(r[i], rmax);
     wp[i] = wx[i] * fsb * rlen;
     xp[i] = (r[i] - rmid0) / rlen0;
   }
@@ -402,9 +403,10 @@ std::vector<double> inner_wrk(double rmin, double rmax, double rmin0, double rma
 }
 }  // namespace
 
-// quadrature.cpp:76-161
-Mat twoe_integral_rule(double rmin, double rmax, const std::vector<double> &x,
-                       const std::vector<double> &wx, const LIPBasis &poly, int L) {
+// quadrature.cpp:76-161 with the kernel pieces fsb(r, R) and fbig(R)
+Mat twoe_integral_rule_k(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &wx,
+                         const LIPBasis &poly, const std::function<double(double, double)> &fsb,
+                         const std::function<double(double)> &fbig) {
   const double rmid = 0.5 * (rmax + rmin), rlen = 0.5 * (rmax - rmin);
   const int n = (int)x.size(), nbf = poly.nbf(), nb2 = nbf * nbf;
   std::vector<double> r(n);
@@ -412,16 +414,16 @@ Mat twoe_integral_rule(double rmin, double rmax, const std::vector<double> &x,
   Mat inner(n, nb2);
   const bool empty_first = (r[0] == rmin);
   if (!empty_first) {
-    const std::vector<double> v = inner_wrk(rmin, r[0], rmin, rmax, x, wx, poly, L);
+    const std::vector<double> v = inner_wrk(rmin, r[0], rmin, rmax, x, wx, poly, fsb);
     for (int k = 0; k < nb2; ++k) inner(0, k) = v[k];
   }
   for (int ip = 1; ip < n; ++ip) {
-    const std::vector<double> v = inner_wrk(r[ip - 1], r[ip], rmin, rmax, x, wx, poly, L);
+    const std::vector<double> v = inner_wrk(r[ip - 1], r[ip], rmin, rmax, x, wx, poly, fsb);
     for (int k = 0; k < nb2; ++k) inner(ip, k) = v[k];
   }
   for (int ip = 1; ip < n; ++ip) {
     if (ip == 1 && empty_first) continue;
-    const double fac = std::pow(r[ip], double(-L - 1)) / std::pow(r[ip - 1], double(-L - 1));
+    const double fac = fbig(r[ip]) / fbig(r[ip - 1]);
     for (int k = 0; k < nb2; ++k) inner(ip, k) += inner(ip - 1, k) * fac;
   }
   const Mat bf = poly.eval_dnf(x, 0, rlen);
@@ -436,8 +438,26 @@ Mat twoe_integral_rule(double rmin, double rmax, const std::vector<double> &x,
   return out;
 }
 
-// converge_rule, RadialBasis.cpp:114-162, seeded at nquad (twoe_nstart :165-173)
+Mat twoe_integral_rule(double rmin, double rmax, const std::vector<double> &x, const std::vector<double> &wx,
+                       const LIPBasis &poly, int L) {
+  const std::function<double(double, double)> fsb = [L](double r, double R) {
+    return std::pow(r / R, double(L)) / R;
+  };
+  const std::function<double(double)> fbig = [L](double r) { return std::pow(r, double(-L - 1)); };
+  return twoe_integral_rule_k(rmin, rmax, x, wx, poly, fsb, fbig);
+}
+
 Mat RadialBasis::twoe_integral(int L, int iel) const {
+  const std::function<double(double, double)> fsb = [L](double r, double R) {
+    return std::pow(r / R, double(L)) / R;
+  };
+  const std::function<double(double)> fbig = [L](double r) { return std::pow(r, double(-L - 1)); };
+  return twoe_integral_conv(iel, fsb, fbig);
+}
+
+// converge_rule, RadialBasis.cpp:114-162, seeded at nquad (twoe_nstart :165-173)
+Mat RadialBasis::twoe_integral_conv(int iel, const std::function<double(double, double)> &fsb,
+                                    const std::function<double(double)> &fbig) const {
   const double Rmin = bval[iel], Rmax = bval[iel + 1];
   const LIPBasis p = get_basis(iel);
   const int nmax = 512;
@@ -451,7 +471,7 @@ Mat RadialBasis::twoe_integral(int L, int iel) const {
   std::vector<double> x, w;
   for (;;) {
     lobatto(n, x, w);
-    cur = twoe_integral_rule(Rmin, Rmax, x, w, p, L);
+    cur = twoe_integral_rule_k(Rmin, Rmax, x, w, p, fsb, fbig);
     if (have) {
       const double diff = max_abs_diff(cur, prev), scale = max_abs(cur);
       if (diff <= tol * (scale + tol)) return cur;

@@ -34,6 +34,16 @@ def lib():
         L.orc_last_error.restype = ctypes.c_char_p
         L.orc_gaunt.restype = ctypes.c_double
         L.orc_mod_gaunt.restype = ctypes.c_double
+        L.orc_bessel_il.restype = ctypes.c_double
+        L.orc_bessel_il.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_bessel_kl.restype = ctypes.c_double
+        L.orc_bessel_kl.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_erfc_phi.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L.orc_atomic_compute_yukawa.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_atomic_compute_erfc.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_atomic_rs_exchange.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_atomic_rs_cache.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p]
         L.orc_atomic_create.restype = ctypes.c_void_p
         L.orc_atomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
@@ -59,6 +69,11 @@ def lib():
                                           ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_diatomic_xc3d.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_diatomic_xc3d_pol.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]
         L.orc_diatomic_create.restype = ctypes.c_void_p
         L.orc_diatomic_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -212,6 +227,29 @@ class AtomicBasis:
         K = np.zeros((self.Nbf, self.Nbf), order="F")
         _check(lib().orc_atomic_exchange(self._h, _ptr(P), _ptr(K)))
         return K
+
+    def compute_yukawa(self, lam):
+        """Yukawa range-separation caches (TwoDBasis.cpp:738-760)."""
+        _check(lib().orc_atomic_compute_yukawa(self._h, float(lam)))
+
+    def compute_erfc(self, mu):
+        """erfc range-separation caches (TwoDBasis.cpp:762-771)."""
+        _check(lib().orc_atomic_compute_erfc(self._h, float(mu)))
+
+    def rs_exchange(self, P):
+        """-K of the short-range kernel (TwoDBasis.cpp:1002-1131)."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_rs_exchange(self._h, _ptr(P), _ptr(K)))
+        return K
+
+    def rs_cache(self, kind, index):
+        dims = np.zeros(2, dtype=np.int32)
+        _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, None))
+        out = np.zeros((int(dims[0]), int(dims[1])), order="F")
+        if out.size:
+            _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, _ptr(out)))
+        return out
 
     def exchange_blocks(self, P, jk):
         """-K restricted to the output blocks j*Nang+k listed in jk."""
@@ -368,6 +406,29 @@ class DiatomicBasis:
         _check(lib().orc_diatomic_exchange(self._h, _ptr(P), _ptr(K)))
         return K
 
+    def compute_yukawa(self, lam):
+        """Yukawa range-separation caches (TwoDBasis.cpp:738-760)."""
+        _check(lib().orc_atomic_compute_yukawa(self._h, float(lam)))
+
+    def compute_erfc(self, mu):
+        """erfc range-separation caches (TwoDBasis.cpp:762-771)."""
+        _check(lib().orc_atomic_compute_erfc(self._h, float(mu)))
+
+    def rs_exchange(self, P):
+        """-K of the short-range kernel (TwoDBasis.cpp:1002-1131)."""
+        P = np.asfortranarray(P, dtype=np.float64)
+        K = np.zeros((self.Nbf, self.Nbf), order="F")
+        _check(lib().orc_atomic_rs_exchange(self._h, _ptr(P), _ptr(K)))
+        return K
+
+    def rs_cache(self, kind, index):
+        dims = np.zeros(2, dtype=np.int32)
+        _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, None))
+        out = np.zeros((int(dims[0]), int(dims[1])), order="F")
+        if out.size:
+            _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, _ptr(out)))
+        return out
+
     def exchange_blocks(self, P, jk):
         """-K restricted to the output blocks j*Nang+k listed in jk."""
         P = np.asfortranarray(P, dtype=np.float64)
@@ -397,6 +458,29 @@ class DiatomicBasis:
         out = np.zeros(2)
         _check(lib().orc_diatomic_xc_pol(self._h, lang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha), _ptr(Hb),
                                          _ptr(out)))
+        return Ha, Hb, float(out[0]), float(out[1])
+
+    def xc3d(self, P, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
+        """General 3D diatomic DFTGrid::eval_Fxc (src/diatomic/dftgrid.cpp:533-572): (H, Exc, Nel)."""
+        lang = 4 * int(np.max(self.lval)) + 12 if lang is None else lang
+        mang = 4 * int(np.max(np.abs(self.mval))) + 12 if mang is None else mang
+        P = np.asfortranarray(P, dtype=np.float64)
+        H = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_diatomic_xc3d(self._h, lang, mang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        return H, float(out[0]), float(out[1])
+
+    def xc3d_pol(self, Pa, Pb, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
+        """Spin-polarized 3D DFTGrid::eval_Fxc (dftgrid.cpp:574-620): (Ha, Hb, Exc, Nel)."""
+        lang = 4 * int(np.max(self.lval)) + 12 if lang is None else lang
+        mang = 4 * int(np.max(np.abs(self.mval))) + 12 if mang is None else mang
+        Pa = np.asfortranarray(Pa, dtype=np.float64)
+        Pb = np.asfortranarray(Pb, dtype=np.float64)
+        Ha = np.zeros((self.Nbf, self.Nbf), order="F")
+        Hb = np.zeros((self.Nbf, self.Nbf), order="F")
+        out = np.zeros(2)
+        _check(lib().orc_diatomic_xc3d_pol(self._h, lang, mang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha),
+                                           _ptr(Hb), _ptr(out)))
         return Ha, Hb, float(out[0]), float(out[1])
 
     def maps(self):
@@ -614,3 +698,20 @@ def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_
         Pa, Pb = density(Fa_d, nela), density(Fb_d, nelb)
     E += getattr(basis, "Enucr", 0.0)
     return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, Pa=Pa, Pb=Pb)
+
+
+def bessel_il(x, L):
+    """i_L(x) = sqrt(pi/(2x)) I_{L+1/2}(x) (libhelfem/include/math.h:52-89)."""
+    return lib().orc_bessel_il(float(x), int(L))
+
+
+def bessel_kl(x, L):
+    """k_L(x) = sqrt(2/(pi x)) K_{L+1/2}(x) (libhelfem/include/math.h:91-115)."""
+    return lib().orc_bessel_kl(float(x), int(L))
+
+
+def erfc_phi(Lmax, x, y):
+    """Phi_L(x, y), L = 0..Lmax: erfc(mu d)/d = mu sum_L Phi_L(mu r, mu r') P_L(cos g)."""
+    out = np.zeros(Lmax + 1)
+    _check(lib().orc_erfc_phi(int(Lmax), float(x), float(y), _ptr(out)))
+    return out

@@ -109,7 +109,20 @@ struct RadialBasis {
   Mat kinetic_l(int iel) const;                // RadialBasis.cpp:489-491
   Mat twoe_integral(int L, int iel) const;     // RadialBasis.cpp:644-663
   Mat twoe_integral_cholesky(int L, int iel, double tol) const;  // RadialBasis.cpp:712-716
+  // auto-converged in-element tensor for a kernel fsb(r<, R>), fbig(R)
+  Mat twoe_integral_conv(int iel, const std::function<double(double, double)> &fsb,
+                         const std::function<double(double)> &fbig) const;
+  Mat bessel_il_integral(int L, double lambda, int iel) const;  // RadialBasis.cpp:321-324
+  Mat bessel_kl_integral(int L, double lambda, int iel) const;  // RadialBasis.cpp:327-330
+  Mat yukawa_integral(int L, double lambda, int iel) const;     // RadialBasis.cpp:727-741
+  Mat erfc_integral(int L, double mu, int iel, int kel) const;  // RadialBasis.cpp:743-812
 };
+
+// Range-separated kernels (libhelfem/include/math.h, erfc_expn.h)
+double bessel_il(double x, int L);
+double bessel_kl(double x, int L);
+void erfc_phi(int Lmax, double x, double y, double *out);  // Phi_0..Lmax(x, y)
+Mat exchange_order(const Mat &T, int Ni, int Nk);
 
 // quadrature::twoe_integral, libhelfem/src/quadrature.cpp:136-161
 Mat twoe_integral_rule(double rmin, double rmax, const std::vector<double> &x,
@@ -138,6 +151,10 @@ struct AtomicBasis {
   std::vector<int> lval, mval;
   std::vector<Mat> disjoint_L, disjoint_m1L, prim_chol;  // TwoDBasis.h:83-104
   double chol_tol = 1e-12;                               // TwoDBasis.h:118-119
+  // range-separated caches: Yukawa (disjoint_iL/kL, rs_chol) or erfc (rs_ktei)
+  bool yukawa = false;
+  double lambda = 0.0;
+  std::vector<Mat> disjoint_iL, disjoint_kL, rs_chol, rs_ktei;
   AtomicBasis() {}
   AtomicBasis(int Z, int nnodes, int nquad, const std::vector<double> &bval,
               const std::vector<int> &lval, const std::vector<int> &mval);
@@ -153,6 +170,9 @@ struct AtomicBasis {
   Mat exchange(const Mat &P) const;   // TwoDBasis.cpp:880-999 (returns -K)
   // Same, restricted to the listed output blocks j*Nang+k (CPU-baseline sample).
   Mat exchange_blocks(const Mat &P, const std::vector<int> &jk) const;
+  void compute_yukawa(double lambda);  // TwoDBasis.cpp:738-760
+  void compute_erfc(double mu);        // TwoDBasis.cpp:762-771
+  Mat rs_exchange(const Mat &P) const; // TwoDBasis.cpp:1002-1131 (returns -K)
 };
 // angular_basis, src/atomic/basis.cpp:179-203
 void angular_basis(int lmax, int mmax, std::vector<int> &lval, std::vector<int> &mval);
@@ -171,6 +191,7 @@ struct LegendreColumn {
   double Q(int L, double x);
 };
 
+Mat assemble_K_pairwise(const RadialBasis &radial, const std::vector<Mat> &ktei, size_t off, const Mat &P);
 // FE assemblers, libhelfemqc/include/CoulombExchangeFE.h:432-530
 Mat assemble_J_chol(const RadialBasis &radial, const std::vector<Mat> &rs, const std::vector<Mat> &rb,
                     const std::vector<Mat> &chol, int off, const Mat &P);

@@ -72,6 +72,27 @@ def test_diatomic_fp32_exchange_error(absm, gpu_ctx):
     assert 1e-12 < e32 <= TOL32, e32
 
 
+def test_diatomic_fp32_exchange_high_multipoles(gpu_ctx):
+    """N2 at lmax 16, 8 elements (the benchmark basis): the P/Q Legendre tiles
+    span 1e-56 .. 1e75 at L = 34, so the FP32 path must rescale them; its error
+    against the FP64 GPU path (pinned to the oracle above) stays within TOL32."""
+    import torch
+    from helfem_amd.synthetic import seeded_density
+    print("[gpu] diatomic FP32 exchange, lmax 16", flush=True)
+    hb = H.DiatomicBasis(7, 7, 2.074, [16, 16, 16, 16], 8, ctx=gpu_ctx)
+    hb.compute_tei()
+    S = hb.overlap()
+    blocks = hb.get_sym_idx(1)
+    P = seeded_density(S, blocks, [5 if i == 3 else (1 if i in (2, 4) else 0) for i in range(len(blocks))],
+                       seed=1203)
+    Pd = torch.as_tensor(np.ascontiguousarray(P), device="cuda")
+    K64 = hb.exchange(Pd).cpu().numpy()
+    K32 = hb.exchange(Pd, precision="fp32").cpu().numpy()
+    assert np.all(np.isfinite(K32))
+    e32 = relerr(K32, K64)
+    assert 1e-12 < e32 <= TOL32, e32
+
+
 def test_mixed_rhf_matches_fp64_atomic(gpu_ctx):
     """Ne HF at lmax 1, 5 elements (tests/README.md:79-80)."""
     from helfem_amd.scf import rhf

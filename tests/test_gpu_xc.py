@@ -191,3 +191,46 @@ def test_nh_triplet_polarized_lda_scf_on_gpu(gpu_ctx):
     r = _uks(hb, 5, 3)
     assert abs(r.E - REF["diatomic-NH-lda-absm"]["total"]) < 1e-9, r.E
     assert abs(r.E - REF["diatomic-NH-lda-u"]["total"]) < 5e-9, r.E
+
+
+@pytest.mark.parametrize("name", ["LiH-l32-4el", "HeH-l21-1el"])
+def test_3d_diatomic_grid_matches_oracle(name, gpu_ctx):
+    """General 3D diatomic DFTGrid::eval_Fxc (dftgrid.cpp:533-620) on a density
+    that mixes m (all shell pairs enter), restricted and spin-polarized."""
+    from helfem_amd.synthetic import random_symmetric
+    print(f"[gpu] diatomic 3D-grid LDA Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    S = ob.overlap()
+    X = random_symmetric(ob.Nbf, seed=1311)
+    P = X @ S @ X.T
+    P *= 4.0 / np.sum(P * S)
+    grid = H.DFTGrid(hb, purem=False)
+    Hr, Exc_r, Nel_r = ob.xc3d(P)
+    Hg, Exc_g, Nel_g, _ = grid.eval_Fxc(1, 7, P)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Exc_g - Exc_r) <= TOL * abs(Exc_r)
+    assert abs(Nel_g - Nel_r) <= TOL * abs(Nel_r)
+    Pb = 0.3 * P
+    Har, Hbr, Er, Nr = ob.xc3d_pol(P, Pb)
+    Hag, Hbg, Eg, Ng, _ = grid.eval_Fxc_pol(1, 7, P, Pb)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+
+
+def test_h2_3d_grid_lda_scf_on_gpu(gpu_ctx):
+    """diatomic-H2-purem-off: H2 LDA on the general 3D grid, R = 1.4 bohr,
+    3 elements, lmax 4 for |m| <= 4."""
+    from helfem_amd.scf import rhf
+    print("[gpu] diatomic H2 LDA SCF on the 3D grid", flush=True)
+    hb = H.DiatomicBasis(1, 1, 1.4, [4, 4, 4, 4, 4], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb, purem=False)
+
+    def xc(P):
+        Hx, Exc, _, _ = grid.eval_Fxc(1, 7, P)
+        return Hx, Exc
+
+    r = rhf(hb, 2, xc=xc, conv=1e-12)
+    assert abs(r.E - REF["diatomic-H2-purem-off"]["total"]) < 1e-9, r.E

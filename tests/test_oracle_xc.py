@@ -155,3 +155,43 @@ def test_polarized_lda_nh_triplet():
     r = O.scf_uhf(b, 5, 3, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb)[:3])
     assert abs(r["E"] - REF["diatomic-NH-lda-absm"]["total"]) < 1e-9
     assert abs(r["E"] - REF["diatomic-NH-lda-u"]["total"]) < 5e-9
+
+
+def test_3d_diatomic_grid_matches_purem_and_reference():
+    """diatomic-H2-purem-on / -purem-off (tests/refs/ci.json, both -1.1374807779):
+    the pure-m SCF reproduces the reference; on that m-diagonal density the
+    general 3D grid (dftgrid.cpp:533-572) gives the same Vxc, Exc and Nel, so
+    the 3D SCF of the purem-off case has the same stationary point."""
+    b = O.DiatomicBasis(1, 1, 1.4, [4, 4, 4, 4, 4], 3)
+    b.compute_tei()
+    r = O.scf_rhf(b, 2, xc=lambda P: b.xc(P)[:2])
+    assert abs(r["E"] - REF["diatomic-H2-purem-on"]["total"]) < 1e-9
+    assert abs(r["E"] - REF["diatomic-H2-purem-off"]["total"]) < 1e-9
+    H3, E3, N3 = b.xc3d(r["P"])
+    Hp, Ep, Np = b.xc(r["P"])
+    assert np.max(np.abs(H3 - Hp)) < 1e-12 * np.max(np.abs(Hp))
+    assert abs(E3 - Ep) < 1e-12 * abs(Ep)
+    assert abs(N3 - 2) < 1e-10
+
+
+def test_3d_diatomic_grid_m_mixing_density():
+    """With a density that mixes m the 3D grid still integrates it: Nel = tr(PS)
+    to quadrature accuracy, and Vxc is symmetric; the pure-m grid cannot
+    represent this density (it only sees the m-diagonal blocks)."""
+    from helfem_amd.synthetic import random_symmetric
+    b = O.DiatomicBasis(3, 1, 3.0, [3, 2], 4)
+    S = b.overlap()
+    X = random_symmetric(b.Nbf, seed=1301)
+    P = X @ S @ X.T  # positive semidefinite
+    P *= 4.0 / np.sum(P * S)
+    H3, E3, N3 = b.xc3d(P)
+    assert abs(N3 - 4.0) < 1e-8
+    assert np.max(np.abs(H3 - H3.T)) < 1e-12 * np.max(np.abs(H3))
+    # the m-diagonal part alone: 3D grid and pure-m grid agree
+    Pd = np.zeros_like(P)
+    for blk in b.sym_idx(1):
+        Pd[np.ix_(blk, blk)] = P[np.ix_(blk, blk)]
+    H3d, E3d, _ = b.xc3d(Pd)
+    Hpd, Epd, _ = b.xc(Pd)
+    assert np.max(np.abs(H3d - Hpd)) < 1e-11 * np.max(np.abs(Hpd))
+    assert abs(E3 - E3d) > 1e-6 * abs(E3d)  # the m mixing does change the functional
