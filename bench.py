@@ -75,6 +75,17 @@ def block_occupations(Z, mvals, lmax):
     return out
 
 
+def measured_traffic(workload):
+    """HBM bytes per exchange launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE x 2 and WRITE_SIZE x 1, the factors
+    a plain device copy of known size gives), or None when none is committed."""
+    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    d = json.loads(f.read_text())
+    return d.get("exchange_bytes_per_build")
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,8 +246,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k", "diatomic": "k_dia_k"}[kind] +
                                                 " (exchange: off- and in-element launches)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "avg_launch_ms": k_ms,
-                     "flops_per_launch": k_flops},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),
+                     "avg_launch_ms": k_ms, "flops_per_launch": k_flops},
         "kernels_ms": {k: (v[1] / max(v[0], 1)) for k, v in stats.items()},
         "kernels_tflops": {k: (v[2] / max(v[1], 1e-30) / 1e9) for k, v in stats.items() if v[2] > 0},
         "energies": {"coulomb": Ecoul, "exchange": Exx},
@@ -245,13 +256,23 @@ def main():
     if not args.no_scf:
         from helfem_amd.scf import rhf
         nel = int(round(nelec))
-        t = time.perf_counter()
-        r = rhf(hb, nel, conv=1e-9)
-        tsolve = time.perf_counter() - t
-        log(f"SCF: E = {r.E:.10f} in {r.iterations} iterations, {tsolve:.2f}s")
-        result["scf"] = {"solves_per_s": 1.0 / tsolve, "seconds": tsolve, "iterations": r.iterations,
-                         "energy": r.E, "convergence": "dE < 1e-9 Eh and orbital gradient < 1e-6",
-                         "guess": "core Hamiltonian", "precision": "FP64 throughout"}
+        scf = {}
+        for prec in ("fp64", "mixed"):
+            t = time.perf_counter()
+            r = rhf(hb, nel, conv=1e-10, precision=prec)
+            tsolve = time.perf_counter() - t
+            log(f"SCF {prec}: E = {r.E:.10f} in {r.iterations} iterations ({r.iterations_fp32} with FP32 "
+                f"exchange), {tsolve:.2f}s")
+            scf[prec] = {"solves_per_s": 1.0 / tsolve, "seconds": tsolve, "iterations": r.iterations,
+                         "iterations_fp32_exchange": r.iterations_fp32, "energy": r.E,
+                         "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
+                         "eigensolves": r.eigensolves, "reused_eigensolves": r.reused_solves}
+        scf["mixed_vs_fp64"] = {"speedup": scf["mixed"]["solves_per_s"] / scf["fp64"]["solves_per_s"],
+                                "energy_difference_Eh": scf["mixed"]["energy"] - scf["fp64"]["energy"]}
+        scf["convergence"] = "dE < 1e-10 Eh and orbital gradient < 1e-6, core-Hamiltonian guess"
+        scf["precision"] = ("fp64: FP64 throughout; mixed: FP32 exchange until the orbital gradient is below "
+                            "1e-3, FP64 after (overlap, eigensolves and energies always FP64)")
+        result["scf"] = scf
     if not args.no_cpu_baseline:
         stride = args.cpu_sample_stride or (24 if kind == "atomic" else 64)
         result["cpu_baseline"] = cpu_baseline(cfg, P, hb.Nang * hb.Nang, stride)

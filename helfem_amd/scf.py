@@ -197,20 +197,31 @@ class _Blocks:
 
     def density(self, Fb, nocc, weight, like):
         """Aufbau over all sub-problems; returns the full P (for J/K), its
-        symmetry blocks and the occupied orbitals per block."""
+        symmetry blocks and the occupied orbitals per block.  Sub-problems
+        of equal size are diagonalised in one batched call."""
         torch = self.torch
         subs, loc, reuse = self._subproblems(Fb)
-        sol = {}
-        for i, (k, c) in enumerate(subs):
-            if reuse[i] != i:
-                continue
-            li = loc[i]
-            X = self.X[k][li][:, li]
-            e, v = torch.linalg.eigh(X.T @ Fb[k][li][:, li] @ X)
-            sol[i] = (e, X @ v)
-            self.solves += 1
-        self.paired_solves += sum(1 for i in range(len(subs)) if reuse[i] != i)
-        ev = {i: sol[i][0].cpu().numpy() for i in sol}
+        uniq = [i for i in range(len(subs)) if reuse[i] == i]
+        by_n = {}
+        for i in uniq:
+            by_n.setdefault(len(loc[i]), []).append(i)
+        vecs, evs = {}, []
+        for n, members in by_n.items():
+            Xs = [self.X[subs[i][0]][loc[i]][:, loc[i]] for i in members]
+            A = torch.stack([X.T @ Fb[subs[i][0]][loc[i]][:, loc[i]] @ X for X, i in zip(Xs, members)])
+            e, v = torch.linalg.eigh(A)
+            for j, i in enumerate(members):
+                vecs[i] = Xs[j] @ v[j]
+            evs.append((members, e))
+        self.solves += len(uniq)
+        self.paired_solves += len(subs) - len(uniq)
+        host = torch.cat([e.reshape(-1) for _, e in evs]).cpu().numpy()
+        ev, off = {}, 0
+        for members, e in evs:
+            n = e.shape[1]
+            for i in members:
+                ev[i] = host[off:off + n]
+                off += n
         order = sorted((float(e), i, j) for i in range(len(subs)) for j, e in enumerate(ev[reuse[i]]))
         nsub = [0] * len(subs)
         for _, i, _j in order[:nocc]:
@@ -221,10 +232,9 @@ class _Blocks:
         for i, (k, c) in enumerate(subs):
             if not nsub[i]:
                 continue
-            Cs = sol[reuse[i]][1][:, :nsub[i]]
-            Pk = weight * (Cs @ Cs.T)
+            Cs = vecs[reuse[i]][:, :nsub[i]]
             li = loc[i]
-            Pb[k][li[:, None], li[None, :]] = Pk
+            Pb[k][li[:, None], li[None, :]] = weight * (Cs @ Cs.T)
             nk[k] += nsub[i]
         for k, b in enumerate(self.b):
             if nk[k]:
