@@ -436,6 +436,36 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
     for (int e = 0; e < Nel; ++e) fr += 8.0 * std::pow((double)S.rad.ni[e], 4);
     d.flops_in = (double)d.n_runs * fr + 8.0 * (double)nterm * sum_in;
   }
+  // in-element workgroups of the LDS-resident kernel: runs grouped by ilm,
+  // batches of KL_BATCH per (ilm, element), element e's batches on XCD e % 8
+  d.n_wg_kl = 0;
+  if (!d.dense_in && d.max_rank <= KL_RANK_MAX && !rilm.empty()) {
+    const int Nel = S.Nel(), nr = (int)rilm.size();
+    std::vector<int> run_act(nr), cnt(nlm + 1, 0), runs(nr);
+    for (size_t a = 0; a < act.size(); ++a)
+      for (int r = lstart[a]; r < lstart[a + 1]; ++r) run_act[r] = (int)a;
+    for (int r : rilm) ++cnt[r + 1];
+    for (int q = 0; q < nlm; ++q) cnt[q + 1] += cnt[q];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < nr; ++r) runs[fill[rilm[r]]++] = r;
+    std::vector<std::vector<int>> per_xcd(8);
+    for (int q = 0; q < nlm; ++q)
+      for (int e = 0; e < Nel; ++e)
+        for (int b = cnt[q]; b < cnt[q + 1]; b += KL_BATCH) {
+          auto &v = per_xcd[e % 8];
+          v.insert(v.end(), {q, e, b, std::min(KL_BATCH, cnt[q + 1] - b)});
+        }
+    size_t mx = 0;
+    for (auto &v : per_xcd) mx = std::max(mx, v.size() / 4);
+    std::vector<int> wg(mx * 8 * 4, -1);
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = 0; j < per_xcd[x].size() / 4; ++j)
+        for (int u = 0; u < 4; ++u) wg[(j * 8 + x) * 4 + u] = per_xcd[x][4 * j + u];
+    d.wg_kl.upload(wg, s);
+    d.kl_runs.upload(runs, s);
+    d.run_act.upload(run_act, s);
+    d.n_wg_kl = (int)(mx * 8);
+  }
   if (act.empty()) act.push_back(0);
   if (rilm.empty()) rilm.push_back(0);
   if (til.empty()) {

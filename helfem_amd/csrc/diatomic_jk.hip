@@ -229,6 +229,71 @@ __device__ inline void load_chol(const T *M, int p, const int kk[4], int c, T y0
   }
 }
 
+// Gaunt-weighted density tiles of one run, four channels, in matrix-operand
+// layout (lane (g,c) holds R(c, row(g,s)) in entry s).  Lanes outside the
+// element pair read a clamped entry (rloc, cloc) and are masked by cmask; the
+// dropped radial function 0 of an m != 0 shell (only at the first radial
+// index) is masked per term.  No branches, two terms' loads in flight.
+template <typename T, int U = 2>
+__device__ inline void gather_terms(const T *P, int64_t ldp, int Nang, const int *skip, const long long *poff,
+                                    const int *term_il, const T *term_f, int q0, int q1, int rloc,
+                                    const int cloc[4], const T cmask[4], T r00[4], T r02[4], T r20[4],
+                                    T r22[4]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) r00[s] = r02[s] = r20[s] = r22[s] = T(0);
+  int q = q0;
+  for (; q + U <= q1; q += U) {
+    T x[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int il = term_il[q + u], i = il / Nang, l = il - i * Nang;
+      const int si = skip[i], sl = skip[l];
+      const long long row = poff[i] + max(rloc - si, 0), cb = poff[l] - sl;
+      const bool rv = rloc >= si;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const T v = P[row + (cb + max(cloc[s], sl)) * ldp];
+        x[u][s] = (rv && cloc[s] >= sl) ? v : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T f00 = term_f[4 * (q + u)], f02 = term_f[4 * (q + u) + 1], f20 = term_f[4 * (q + u) + 2],
+              f22 = term_f[4 * (q + u) + 3];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        r00[s] += f00 * x[u][s];
+        r02[s] += f02 * x[u][s];
+        r20[s] += f20 * x[u][s];
+        r22[s] += f22 * x[u][s];
+      }
+    }
+  }
+  for (; q < q1; ++q) {
+    const int il = term_il[q], i = il / Nang, l = il - i * Nang;
+    const int si = skip[i], sl = skip[l];
+    const long long row = poff[i] + max(rloc - si, 0), cb = poff[l] - sl;
+    const bool rv = rloc >= si;
+    const T f00 = term_f[4 * q], f02 = term_f[4 * q + 1], f20 = term_f[4 * q + 2], f22 = term_f[4 * q + 3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const T v = P[row + (cb + max(cloc[s], sl)) * ldp];
+      const T x = (rv && cloc[s] >= sl) ? v : T(0);
+      r00[s] += f00 * x;
+      r02[s] += f02 * x;
+      r20[s] += f20 * x;
+      r22[s] += f22 * x;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    r00[s] *= cmask[s];
+    r02[s] *= cmask[s];
+    r20[s] *= cmask[s];
+    r22[s] *= cmask[s];
+  }
+}
+
 template <typename T>
 struct DKArgs {
   const T *P;
@@ -283,59 +348,9 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
   const int r0 = A.act_lstart[act], r1 = A.act_lstart[act + 1];
   for (int run = r0 + wv; run < r1; run += 4) {
     const int ilm = A.lrun_ilm[run];
-    T r00[4] = {0, 0, 0, 0}, r02[4] = {0, 0, 0, 0}, r20[4] = {0, 0, 0, 0}, r22[4] = {0, 0, 0, 0};
-    const int q0 = A.lrun_tstart[run], q1 = A.lrun_tstart[run + 1];
-    int q = q0;
-    for (; q + 2 <= q1; q += 2) {
-      T x[2][4];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int il = A.term_il[q + u], i = il / A.Nang, l = il - i * A.Nang;
-        const int si = A.skip[i], sl = A.skip[l];
-        const long long row = A.poff[i] + max(rloc - si, 0), cb = A.poff[l] - sl;
-        const bool rv = rloc >= si;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const T v = A.P[row + (cb + max(cloc[s], sl)) * A.ldp];
-          x[u][s] = (rv && cloc[s] >= sl) ? v : T(0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const T f00 = A.term_f[4 * (q + u)], f02 = A.term_f[4 * (q + u) + 1], f20 = A.term_f[4 * (q + u) + 2],
-                f22 = A.term_f[4 * (q + u) + 3];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          r00[s] += f00 * x[u][s];
-          r02[s] += f02 * x[u][s];
-          r20[s] += f20 * x[u][s];
-          r22[s] += f22 * x[u][s];
-        }
-      }
-    }
-    for (; q < q1; ++q) {
-      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
-      const int si = A.skip[i], sl = A.skip[l];
-      const long long row = A.poff[i] + max(rloc - si, 0), cb = A.poff[l] - sl;
-      const bool rv = rloc >= si;
-      const T f00 = A.term_f[4 * q], f02 = A.term_f[4 * q + 1], f20 = A.term_f[4 * q + 2], f22 = A.term_f[4 * q + 3];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const T v = A.P[row + (cb + max(cloc[s], sl)) * A.ldp];
-        const T x = (rv && cloc[s] >= sl) ? v : T(0);
-        r00[s] += f00 * x;
-        r02[s] += f02 * x;
-        r20[s] += f20 * x;
-        r22[s] += f22 * x;
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      r00[s] *= cmask[s];
-      r02[s] *= cmask[s];
-      r20[s] *= cmask[s];
-      r22[s] *= cmask[s];
-    }
+    T r00[4], r02[4], r20[4], r22[4];
+    gather_terms<T, INSIDE ? 2 : 4>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
+                                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
     const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
     if (!INSIDE) {
       const T *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
@@ -422,6 +437,111 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
     atomicAdd(dst, -v);
   else
     *dst = -v;
+}
+
+// --- In-element exchange with the Cholesky factors resident in LDS -------------
+//
+// One 1024-thread workgroup per ((L,|M|), element e, batch of runs of that
+// (L,|M|)): the rank x {M0_p, M2_p} channel tiles of (ilm, e) are read once
+// into LDS (rows padded to 17 so both the M and M' operand reads are free of
+// bank conflicts), then each of the 16 waves takes runs of the batch, gathers
+// their density tiles and applies sum_p sigma_p [M0'(R00 M0' - R02 M2') +
+// M2'(-R20 M0' + R22 M2')] from LDS.  Runs of one output block sit in
+// different workgroups, so the tiles are added to K atomically.  Compared
+// with k_dia_k<true> (one workgroup per (block, element), factors re-read
+// from L2 for every run) this moves the factor traffic from ~100 GB to a few
+// GB per N2 build.
+
+constexpr int KL_THREADS = 1024, KL_WAVES = 16, KL_RMAX = KL_RANK_MAX, KL_LD = 17;
+
+template <typename T>
+struct DKLArgs {
+  const T *P;
+  int64_t ldp;
+  double *K;
+  int64_t ldk;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const T *chol, *sigma;
+  const int *chol_off, *chol_rank;
+  const int *act_jk, *lrun_tstart, *term_il;
+  const T *term_f;
+  const int *wg;       // per workgroup: ilm, e, first entry in runs, count (or ilm = -1)
+  const int *runs;     // runs grouped by ilm
+  const int *run_act;  // output block of each run
+};
+
+template <typename T>
+__global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
+  typedef Mma<T> MM;
+  typedef typename MM::acc_t acc_t;
+  constexpr int TL = TILE * KL_LD;  // one padded 16x16 tile
+  __shared__ T Mt[KL_RMAX * 2 * TL];
+  __shared__ T sg[KL_RMAX];
+  const int w = blockIdx.x;
+  const int ilm = A.wg[4 * w], e = A.wg[4 * w + 1], b0 = A.wg[4 * w + 2], nb = A.wg[4 * w + 3];
+  if (ilm < 0) return;
+  const int rank = A.chol_rank[ilm * A.Nel + e], off = A.chol_off[ilm * A.Nel + e];
+  const T *M = A.chol + (size_t)off * 2 * TILE2;
+  for (int q = threadIdx.x; q < rank * 2 * TILE2; q += KL_THREADS) {
+    const int pc = q / TILE2, u = q - pc * TILE2;  // pc = 2 p + channel
+    Mt[pc * TL + (u & 15) + KL_LD * (u >> 4)] = M[q];
+  }
+  for (int q = threadIdx.x; q < rank; q += KL_THREADS) sg[q] = A.sigma[off + q];
+  __syncthreads();
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int n = A.ni[e], fe = A.first[e];
+  const int rloc = fe + min(c, n - 1);
+  int cloc[4], kk[4];
+  T cmask[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    kk[s] = MM::row(g, s);
+    cloc[s] = fe + min(kk[s], n - 1);
+    cmask[s] = (c < n && kk[s] < n) ? T(1) : T(0);
+  }
+  for (int i = wv; i < nb; i += KL_WAVES) {
+    const int run = A.runs[b0 + i], act = A.run_act[run];
+    T r00[4], r02[4], r20[4], r22[4];
+    gather_terms<T>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
+                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      r02[s] = -r02[s];
+      r20[s] = -r20[s];
+    }
+    acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+    for (int p = 0; p < rank; ++p) {
+      const T *M0 = Mt + 2 * p * TL, *M2 = M0 + TL;
+      T y0[4], y2[4], x0[4], x2[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        y0[s] = M0[c + KL_LD * kk[s]];
+        y2[s] = M2[c + KL_LD * kk[s]];
+        x0[s] = M0[kk[s] + KL_LD * c];
+        x2[s] = M2[kk[s] + KL_LD * c];
+      }
+      acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
+      mma_pair<T>(r00, r20, y0, ya, yb);
+      mma_pair<T>(r02, r22, y2, ya, yb);
+      const T sp = sg[p];
+      ya *= sp;
+      yb *= sp;
+      mma_pair2<T>(x0, ya, acc, x2, yb, acc2);
+    }
+    acc += acc2;
+    const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+    const long long col = pure_row(A.skip, A.poff, k, fe + c);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int a = kk[rr];
+      if (a >= n || c >= n) continue;
+      const long long row = pure_row(A.skip, A.poff, j, fe + a);
+      if (row >= 0 && col >= 0) atomicAdd(A.K + row + col * A.ldk, -(double)acc[rr]);
+    }
+  }
 }
 
 // --- In-element exchange as a dense operator ----------------------------------
@@ -640,6 +760,22 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       hip_check(hipGetLastError(), "diatomic_kin_gemm");
       hipLaunchKernelGGL(k_dia_kin_reduce, dim3(d.n_act * d.Nel), dim3(NT), 0, s, r);
       hip_check(hipGetLastError(), "diatomic_kin_reduce");
+      continue;
+    }
+    if (part == 1 && d.n_wg_kl > 0) {
+      TimedLaunch tl(timer, name, s, flops);
+      if (fp32) {
+        DKLArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p,
+                         d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p,
+                         d.term_il.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
+        hipLaunchKernelGGL((k_dia_kin_lds<float>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+      } else {
+        DKLArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.chol.p,
+                          d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p, d.term_il.p,
+                          d.term_f.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
+        hipLaunchKernelGGL((k_dia_kin_lds<double>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+      }
+      hip_check(hipGetLastError(), "diatomic_kin_lds");
       continue;
     }
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;

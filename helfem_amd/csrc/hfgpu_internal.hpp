@@ -15,7 +15,11 @@ namespace hfg {
 // functions), the MFMA f64 tile edge on CDNA.
 constexpr int TILE = 16;
 constexpr int TILE2 = TILE * TILE;
-constexpr int KC = 228;  // operator extent per channel: c + 15 d for c, d < 15, padded to 4
+constexpr int KC = 228;
+// largest Cholesky rank the LDS-resident in-element diatomic exchange holds
+constexpr int KL_RANK_MAX = 32;
+// runs per workgroup of that kernel
+constexpr int KL_BATCH = 64;  // operator extent per channel: c + 15 d for c, d < 15, padded to 4
 
 struct Error : std::runtime_error {
   int code;
@@ -235,6 +239,10 @@ struct DiatomicDevice {
   DevArray<int> ilm_rstart, ilm_runs;     // runs grouped by (L,|M|)
   DevArray<int> wg_in;                    // per workgroup (ilm, e, first run in group) or -1
   DevArray<double> Ypart;                 // [n_runs][Nel][TILE2] in-element partial tiles
+  // in-element exchange with LDS-resident factors: per workgroup (ilm, e,
+  // first entry in kl_runs, count); runs grouped by ilm; output block per run
+  DevArray<int> wg_kl, kl_runs, run_act;
+  int n_wg_kl = 0;
   // FP32 copies for the reduced-precision exchange (built on first use)
   DevArray<float> pq32, chol32, sigma32, term_f32, P32;
   DevArray<double> lsig;                  // [n_lm*Nel] log of the P/Q scale of the FP32 tiles
