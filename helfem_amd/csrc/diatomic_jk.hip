@@ -349,8 +349,8 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
   for (int run = r0 + wv; run < r1; run += 4) {
     const int ilm = A.lrun_ilm[run];
     T r00[4], r02[4], r20[4], r22[4];
-    gather_terms<T, INSIDE ? 2 : 4>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
-                                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
+    gather_terms<T>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
+                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
     const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
     if (!INSIDE) {
       const T *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
