@@ -185,7 +185,8 @@ struct KArgs {
   const int *first, *ni;
   const T *rs, *rb, *chol;
   const int *chol_off, *chol_rank;
-  const int *act_jk, *term_start, *term_il;
+  const int *act_jk, *term_start;
+  const int2 *term_off;  // per term (i*Nrad, l*Nrad) of its density block
   const T *term_c;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
@@ -203,7 +204,7 @@ struct KArgs {
 // products in FP32 (half the bytes, the FP32 matrix rate) and sums the four
 // waves' tiles in FP64.  Workgroups sharing (e,f) run on one XCD.
 template <bool INSIDE, typename T>
-__global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
+__device__ inline void atomic_k_body(const KArgs<T> &A) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
   const int w = blockIdx.x;
@@ -239,11 +240,11 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
       T x[4][4], cq[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int il = A.term_il[q + u], i = il / A.Nang, l = il - i * A.Nang;
+        const int2 to = A.term_off[q + u];
         cq[u] = A.term_c[q + u];
-        const T *Pr = A.P + (int64_t)(i * A.Nrad + rloc);
+        const T *Pr = A.P + (int64_t)(to.x + rloc);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) x[u][s] = Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
+        for (int s = 0; s < 4; ++s) x[u][s] = Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -251,11 +252,11 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
         for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
     }
     for (; q < q1; ++q) {
-      const int il = A.term_il[q], i = il / A.Nang, l = il - i * A.Nang;
+      const int2 to = A.term_off[q];
       const T cq = A.term_c[q];
-      const T *Pr = A.P + (int64_t)(i * A.Nrad + rloc);
+      const T *Pr = A.P + (int64_t)(to.x + rloc);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(l * A.Nrad + cloc[s]) * A.ldp];
+      for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
     }
     const T msc = A.Lscale ? A.Lscale[L] : T(1);
 #pragma unroll
@@ -319,6 +320,18 @@ __global__ void __launch_bounds__(NT) k_atomic_k(KArgs<T> A) {
   else
     *dst = -v;
 }
+
+// Launch wrappers.  The cross-element body fits 6 waves per SIMD in FP64
+// (75 registers) and 8 in FP32 without spills; the in-element body keeps the
+// compiler's register choice.
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_atomic_k_off_f64(KArgs<double> A) {
+  atomic_k_body<false, double>(A);
+}
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_atomic_k_off_f32(KArgs<float> A) {
+  atomic_k_body<false, float>(A);
+}
+__global__ void __launch_bounds__(NT) k_atomic_k_in_f64(KArgs<double> A) { atomic_k_body<true, double>(A); }
+__global__ void __launch_bounds__(NT) k_atomic_k_in_f32(KArgs<float> A) { atomic_k_body<true, float>(A); }
 
 // --- Range-separated exchange with non-separable (erfc) element pairs ------------
 //
@@ -717,6 +730,12 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   d.act_jk.upload(act, s);
   d.term_start.upload(tstart, s);
   d.term_il.upload(til, s);
+  {
+    std::vector<int2> toff(til.size());
+    for (size_t q = 0; q < til.size(); ++q)
+      toff[q] = make_int2((til[q] / d.Nang) * d.Nrad, (til[q] % d.Nang) * d.Nrad);
+    d.term_off.upload(toff, s);
+  }
   d.term_c.upload(tc, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;
@@ -753,12 +772,12 @@ void atomic_rs_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, do
       if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
       KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.y_rs.p,
                       d.y_rb.p, d.y_chol.p, d.y_chol_off.p, d.y_chol_rank.p, d.act_jk.p, d.term_start.p,
-                      d.term_il.p, d.term_c.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef, d.rs_Lscale.p};
+                      d.term_off.p, d.term_c.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef, d.rs_Lscale.p};
       TimedLaunch tl(timer, "atomic_rs_k_yukawa", s);
       if (part == 0)
-        hipLaunchKernelGGL((k_atomic_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_off_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
-        hipLaunchKernelGGL((k_atomic_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_in_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
       hip_check(hipGetLastError(), "atomic_rs_k");
     }
   } else if (d.rs_kind == 2) {
@@ -824,19 +843,19 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
     if (fp32) {
       KArgs<float> a{d.P32.p, nbf, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p,
                      d.rs32.p, d.rb32.p, d.chol32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p,
-                     d.term_il.p, d.term_c32.p, efl, n_ef, nullptr};
+                     d.term_off.p, d.term_c32.p, efl, n_ef, nullptr};
       if (part == 0)
-        hipLaunchKernelGGL((k_atomic_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_off_f32, dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
-        hipLaunchKernelGGL((k_atomic_k<true, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_in_f32, dim3((unsigned)grid), dim3(NT), 0, s, a);
     } else {
       KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
-                      d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_il.p,
+                      d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_off.p,
                       d.term_c.p, efl, n_ef, nullptr};
       if (part == 0)
-        hipLaunchKernelGGL((k_atomic_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_off_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
-        hipLaunchKernelGGL((k_atomic_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL(k_atomic_k_in_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
     }
     hip_check(hipGetLastError(), "atomic_k");
   }

@@ -477,6 +477,14 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.lrun_ilm.upload(rilm, s);
   d.lrun_tstart.upload(rstart, s);
   d.term_il.upload(til, s);
+  {
+    std::vector<int4> tpos(til.size());
+    for (size_t q = 0; q < til.size(); ++q) {
+      const int i = til[q] / Na, l = til[q] % Na;
+      tpos[q] = make_int4((int)S.poff[i], (int)(S.poff[l] - S.skip(l)), S.skip(i), S.skip(l));
+    }
+    d.term_pos.upload(tpos, s);
+  }
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;

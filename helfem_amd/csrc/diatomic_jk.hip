@@ -235,9 +235,8 @@ __device__ inline void load_chol(const T *M, int p, const int kk[4], int c, T y0
 // dropped radial function 0 of an m != 0 shell (only at the first radial
 // index) is masked per term.  No branches, two terms' loads in flight.
 template <typename T, int U = 2>
-__device__ inline void gather_terms(const T *P, int64_t ldp, int Nang, const int *skip, const long long *poff,
-                                    const int *term_il, const T *term_f, int q0, int q1, int rloc,
-                                    const int cloc[4], const T cmask[4], T r00[4], T r02[4], T r20[4],
+__device__ inline void gather_terms(const T *P, int64_t ldp, const int4 *tpos, const T *term_f, int q0, int q1,
+                                    int rloc, const int cloc[4], const T cmask[4], T r00[4], T r02[4], T r20[4],
                                     T r22[4]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) r00[s] = r02[s] = r20[s] = r22[s] = T(0);
@@ -246,14 +245,14 @@ __device__ inline void gather_terms(const T *P, int64_t ldp, int Nang, const int
     T x[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int il = term_il[q + u], i = il / Nang, l = il - i * Nang;
-      const int si = skip[i], sl = skip[l];
-      const long long row = poff[i] + max(rloc - si, 0), cb = poff[l] - sl;
-      const bool rv = rloc >= si;
+      // (poff[i], poff[l] - skip[l], skip[i], skip[l]) of the term's density block
+      const int4 tp = tpos[q + u];
+      const long long row = tp.x + max(rloc - tp.z, 0), cb = tp.y;
+      const bool rv = rloc >= tp.z;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const T v = P[row + (cb + max(cloc[s], sl)) * ldp];
-        x[u][s] = (rv && cloc[s] >= sl) ? v : T(0);
+        const T v = P[row + (cb + max(cloc[s], tp.w)) * ldp];
+        x[u][s] = (rv && cloc[s] >= tp.w) ? v : T(0);
       }
     }
 #pragma unroll
@@ -270,15 +269,14 @@ __device__ inline void gather_terms(const T *P, int64_t ldp, int Nang, const int
     }
   }
   for (; q < q1; ++q) {
-    const int il = term_il[q], i = il / Nang, l = il - i * Nang;
-    const int si = skip[i], sl = skip[l];
-    const long long row = poff[i] + max(rloc - si, 0), cb = poff[l] - sl;
-    const bool rv = rloc >= si;
+    const int4 tp = tpos[q];
+    const long long row = tp.x + max(rloc - tp.z, 0), cb = tp.y;
+    const bool rv = rloc >= tp.z;
     const T f00 = term_f[4 * q], f02 = term_f[4 * q + 1], f20 = term_f[4 * q + 2], f22 = term_f[4 * q + 3];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const T v = P[row + (cb + max(cloc[s], sl)) * ldp];
-      const T x = (rv && cloc[s] >= sl) ? v : T(0);
+      const T v = P[row + (cb + max(cloc[s], tp.w)) * ldp];
+      const T x = (rv && cloc[s] >= tp.w) ? v : T(0);
       r00[s] += f00 * x;
       r02[s] += f02 * x;
       r20[s] += f20 * x;
@@ -305,7 +303,8 @@ struct DKArgs {
   const long long *poff;
   const T *pq, *chol, *sigma;
   const int *chol_off, *chol_rank;
-  const int *act_jk, *act_lstart, *lrun_ilm, *lrun_tstart, *term_il;
+  const int *act_jk, *act_lstart, *lrun_ilm, *lrun_tstart;
+  const int4 *tpos;
   const T *term_f;
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
@@ -318,7 +317,7 @@ struct DKArgs {
 // T = double is the reference's arithmetic; T = float gathers and multiplies
 // in FP32 and sums the waves' tiles in FP64.
 template <bool INSIDE, typename T>
-__global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
+__device__ inline void dia_k_body(const DKArgs<T> &A) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
   const int w = blockIdx.x;
@@ -349,8 +348,8 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
   for (int run = r0 + wv; run < r1; run += 4) {
     const int ilm = A.lrun_ilm[run];
     T r00[4], r02[4], r20[4], r22[4];
-    gather_terms<T>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
-                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
+    gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
+                    r00, r02, r20, r22);
     const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
     if (!INSIDE) {
       const T *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
@@ -439,6 +438,17 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
     *dst = -v;
 }
 
+// Cross-element launch held to 5 waves per SIMD (96 registers), which the
+// two-term gather fits; the in-element fallback keeps the compiler's choice.
+template <typename T>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) k_dia_k_off(DKArgs<T> A) {
+  dia_k_body<false, T>(A);
+}
+template <typename T>
+__global__ void __launch_bounds__(NT) k_dia_k_in(DKArgs<T> A) {
+  dia_k_body<true, T>(A);
+}
+
 // --- In-element exchange with the Cholesky factors resident in LDS -------------
 //
 // One 1024-thread workgroup per ((L,|M|), element e, batch of runs of that
@@ -448,7 +458,7 @@ __global__ void __launch_bounds__(NT) k_dia_k(DKArgs<T> A) {
 // their density tiles and applies sum_p sigma_p [M0'(R00 M0' - R02 M2') +
 // M2'(-R20 M0' + R22 M2')] from LDS.  Runs of one output block sit in
 // different workgroups, so the tiles are added to K atomically.  Compared
-// with k_dia_k<true> (one workgroup per (block, element), factors re-read
+// with k_dia_k_in (one workgroup per (block, element), factors re-read
 // from L2 for every run) this moves the factor traffic from ~100 GB to a few
 // GB per N2 build.
 
@@ -465,7 +475,8 @@ struct DKLArgs {
   const long long *poff;
   const T *chol, *sigma;
   const int *chol_off, *chol_rank;
-  const int *act_jk, *lrun_tstart, *term_il;
+  const int *act_jk, *lrun_tstart;
+  const int4 *tpos;
   const T *term_f;
   const int *wg;       // per workgroup: ilm, e, first entry in runs, count (or ilm = -1)
   const int *runs;     // runs grouped by ilm
@@ -505,8 +516,8 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
   for (int i = wv; i < nb; i += KL_WAVES) {
     const int run = A.runs[b0 + i], act = A.run_act[run];
     T r00[4], r02[4], r20[4], r22[4];
-    gather_terms<T>(A.P, A.ldp, A.Nang, A.skip, A.poff, A.term_il, A.term_f, A.lrun_tstart[run],
-                    A.lrun_tstart[run + 1], rloc, cloc, cmask, r00, r02, r20, r22);
+    gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
+                    r00, r02, r20, r22);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       r02[s] = -r02[s];
@@ -767,11 +778,11 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       if (fp32) {
         DKLArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                          d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p,
-                         d.term_il.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
+                         d.term_pos.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
         hipLaunchKernelGGL((k_dia_kin_lds<float>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
       } else {
         DKLArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.chol.p,
-                          d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p, d.term_il.p,
+                          d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p, d.term_pos.p,
                           d.term_f.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
         hipLaunchKernelGGL((k_dia_kin_lds<double>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
       }
@@ -786,19 +797,19 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
     if (fp32) {
       DKArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                       d.pq32.p, d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p,
-                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_il.p, d.term_f32.p, efl, n_ef, d.lsig.p};
+                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_pos.p, d.term_f32.p, efl, n_ef, d.lsig.p};
       if (part == 0)
-        hipLaunchKernelGGL((k_dia_k<false, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_dia_k_off<float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
-        hipLaunchKernelGGL((k_dia_k<true, float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_dia_k_in<float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
     } else {
       DKArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
                        d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p,
-                       d.lrun_tstart.p, d.term_il.p, d.term_f.p, efl, n_ef, nullptr};
+                       d.lrun_tstart.p, d.term_pos.p, d.term_f.p, efl, n_ef, nullptr};
       if (part == 0)
-        hipLaunchKernelGGL((k_dia_k<false, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_dia_k_off<double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
       else
-        hipLaunchKernelGGL((k_dia_k<true, double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_dia_k_in<double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
     }
     hip_check(hipGetLastError(), "diatomic_k");
   }

@@ -159,6 +159,7 @@ struct AtomicDevice {
   DevArray<int> act_jk;                   // active output blocks j*Nang+k
   DevArray<int> term_start;               // [n_act*N_L + 1]
   DevArray<int> term_il;                  // i*Nang + l
+  DevArray<int2> term_off;                // (i*Nrad, l*Nrad)
   DevArray<double> term_c;                // 4pi/(2L+1) * G_j * G_k
   PinnedArray<double> h_bnorm;
   // work buffers
@@ -221,6 +222,7 @@ struct DiatomicDevice {
   DevArray<int> act_jk, act_lstart;       // active output blocks; ilm runs per block
   DevArray<int> lrun_ilm, lrun_tstart;    // per run: ilm and first term
   DevArray<int> term_il;
+  DevArray<int4> term_pos;                // per term (poff[i], poff[l] - skip[l], skip[i], skip[l])
   DevArray<double> term_f;                // 4 per term: f00, f02, f20, f22 (LMfac included)
   DevArray<int> mirror;                   // absm: pairs (dst jk, src jk)
   int n_mirror = 0;
