@@ -36,17 +36,22 @@ def _one_electron(basis, dev):
     return out
 
 
+def _sync(dev):
+    import torch
+    if dev.type == "cuda":
+        torch.cuda.synchronize(dev)
+
+
 class _Clock:
     """Per-phase wall time, synchronised with the device at phase ends."""
 
-    def __init__(self):
-        import torch
-        self.torch = torch
+    def __init__(self, dev):
+        self.dev = dev
         self.t = {}
         self.last = time.perf_counter()
 
     def tick(self, name):
-        self.torch.cuda.synchronize()
+        _sync(self.dev)
         now = time.perf_counter()
         self.t[name] = self.t.get(name, 0.0) + now - self.last
         self.last = now
@@ -195,33 +200,29 @@ class _Blocks:
         return self.torch.cat([self.torch.arange(sh[s_][1], sh[s_][1] + sh[s_][2], device=self.dev)
                                for s_ in comp])
 
-    def density(self, Fb, nocc, weight, like):
-        """Aufbau over all sub-problems; returns the full P (for J/K), its
-        symmetry blocks and the occupied orbitals per block.  Sub-problems
-        of equal size are diagonalised in one batched call."""
-        torch = self.torch
+    def prepare(self, Fb):
+        """Independent eigenproblems X' F X of this Fock matrix (one per
+        distinct shell group), with the bookkeeping `finish` needs."""
         subs, loc, reuse = self._subproblems(Fb)
         uniq = [i for i in range(len(subs)) if reuse[i] == i]
-        by_n = {}
+        X, mats = {}, []
         for i in uniq:
-            by_n.setdefault(len(loc[i]), []).append(i)
-        vecs, evs = {}, []
-        for n, members in by_n.items():
-            Xs = [self.X[subs[i][0]][loc[i]][:, loc[i]] for i in members]
-            A = torch.stack([X.T @ Fb[subs[i][0]][loc[i]][:, loc[i]] @ X for X, i in zip(Xs, members)])
-            e, v = torch.linalg.eigh(A)
-            for j, i in enumerate(members):
-                vecs[i] = Xs[j] @ v[j]
-            evs.append((members, e))
+            k, li = subs[i][0], loc[i]
+            X[i] = self.X[k][li][:, li]
+            mats.append(X[i].T @ Fb[k][li][:, li] @ X[i])
         self.solves += len(uniq)
         self.paired_solves += len(subs) - len(uniq)
-        host = torch.cat([e.reshape(-1) for _, e in evs]).cpu().numpy()
-        ev, off = {}, 0
-        for members, e in evs:
-            n = e.shape[1]
-            for i in members:
-                ev[i] = host[off:off + n]
-                off += n
+        return {"subs": subs, "loc": loc, "reuse": reuse, "uniq": uniq, "X": X, "mats": mats}
+
+    def finish(self, plan, sols, nocc, weight, like):
+        """Aufbau over all sub-problems from their eigenpairs (`sols`, in the
+        order of plan["mats"]); returns the full P (for J/K), its symmetry
+        blocks and the occupied orbitals per block."""
+        subs, loc, reuse = plan["subs"], plan["loc"], plan["reuse"]
+        ev, vecs = {}, {}
+        for i, (e, v) in zip(plan["uniq"], sols):
+            ev[i] = e
+            vecs[i] = plan["X"][i] @ v
         order = sorted((float(e), i, j) for i in range(len(subs)) for j, e in enumerate(ev[reuse[i]]))
         nsub = [0] * len(subs)
         for _, i, _j in order[:nocc]:
@@ -241,6 +242,10 @@ class _Blocks:
                 P[b[:, None], b[None, :]] = Pb[k]
         return P, Pb, nk
 
+    def density(self, Fb, nocc, weight, like):
+        plan = self.prepare(Fb)
+        return self.finish(plan, batched_eigh(plan["mats"]), nocc, weight, like)
+
     def commutator(self, Fb, Pb):
         """Orthonormal-basis commutators X_k' (F P S - S P F) X_k per block."""
         out = []
@@ -248,6 +253,32 @@ class _Blocks:
             FPS = Fb[k] @ Pb[k] @ self.S[k]
             out.append(self.X[k].T @ (FPS - FPS.T) @ self.X[k])
         return out
+
+
+def batched_eigh(mats):
+    """Eigenpairs of symmetric matrices; equal sizes share one batched call
+    and all eigenvalues come to the host in one transfer."""
+    import torch
+    if not mats:
+        return []
+    by_n = {}
+    for j, A in enumerate(mats):
+        by_n.setdefault(A.shape[0], []).append(j)
+    vals, vecs = [None] * len(mats), [None] * len(mats)
+    evs = []
+    for n, members in by_n.items():
+        e, v = torch.linalg.eigh(torch.stack([mats[j] for j in members]))
+        for q, j in enumerate(members):
+            vecs[j] = v[q]
+        evs.append((members, e))
+    host = torch.cat([e.reshape(-1) for _, e in evs]).cpu().numpy()
+    off = 0
+    for members, e in evs:
+        n = e.shape[1]
+        for j in members:
+            vals[j] = host[off:off + n]
+            off += n
+    return list(zip(vals, vecs))
 
 
 class _DIIS:
@@ -294,102 +325,189 @@ class _DIIS:
         return [sum(float(c[i]) * self.F[i][k] for i in range(n)) for k in range(len(self.F[-1]))]
 
 
-def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch):
-    """Shared SCF loop.  Restricted: nel_spin = (nocc,), xc(P) -> (Vxc, Exc).
-    Unrestricted: nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
-    import torch
-    if precision not in ("fp64", "mixed"):
-        raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
-    dev = torch.device(device)
-    f64 = torch.float64
-    t0 = time.perf_counter()
-    clk = _Clock()
-    S, T, V = _one_electron(basis, dev)
-    H = T + V
-    bs = _Blocks(basis, S, symm, dev)
-    Hb = bs.blocks_of(H)
-    Enucr = float(getattr(basis, "Enucr", 0.0))  # nuclear repulsion of a diatomic basis
-    clk.tick("setup")
-    nspin = 1 if restricted else 2
-    weight = 2.0 if restricted else 1.0
-    fp32 = precision == "mixed"
-    it_fp32 = 0
-    dens = [bs.density(Hb, nel_spin[s], weight, H) for s in range(nspin)]
-    clk.tick("density")
-    diis = _DIIS(diis_len)
-    Eold, E, it = None, 0.0, 0
-    hist = []
-    Ekin = Enuc = Ecoul = Exx = Exc = torch.zeros((), dtype=f64, device=dev)
-    for it in range(1, maxit + 1):
-        Ps = [d[0] for d in dens]
-        P = Ps[0] if restricted else Ps[0] + Ps[1]
-        J = basis.coulomb(P)
+class SCFState:
+    """One SCF in flight: the setup, the Fock step of every iteration and the
+    convergence test; the eigensolves it asks for are done by the caller
+    (`SCFEngine`), so many states can share batched eigensolves.
+
+    Restricted: nel_spin = (nocc,), xc(P) -> (Vxc, Exc).  Unrestricted:
+    nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
+
+    def __init__(self, basis, nel_spin, restricted, symm=1, conv=1e-10, maxit=100, diis_len=10, verbose=False,
+                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None):
+        import torch
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        self.torch = torch
+        self.basis, self.nel_spin, self.restricted = basis, tuple(nel_spin), restricted
+        self.conv, self.maxit, self.verbose, self.xc, self.switch = conv, maxit, verbose, xc, switch
+        self.tag = tag
+        self.dev = torch.device(device)
+        self.t0 = time.perf_counter()
+        self.clk = _Clock(self.dev)
+        self.S, self.T, self.V = _one_electron(basis, self.dev)
+        self.H = self.T + self.V
+        self.bs = _Blocks(basis, self.S, symm, self.dev)
+        self.Enucr = float(getattr(basis, "Enucr", 0.0))  # nuclear repulsion of a diatomic basis
+        self.clk.tick("setup")
+        self.nspin = 1 if restricted else 2
+        self.weight = 2.0 if restricted else 1.0
+        self.fp32 = precision == "mixed"
+        self.it_fp32 = 0
+        self.diis = _DIIS(diis_len)
+        self.Eold, self.E, self.it, self.hist = None, 0.0, 0, []
+        z = torch.zeros((), dtype=torch.float64, device=self.dev)
+        self.terms = dict(Ekin=z, Enuc=z, Ecoul=z, Exx=z, Exc=z)
+        self.dens = [None] * self.nspin
+        Hb = self.bs.blocks_of(self.H)
+        self.pending = [(s, Hb) for s in range(self.nspin)]
+        self.done = False
+
+    def requests(self):
+        """(spin, Fock blocks, occupied orbitals) still to be diagonalised."""
+        return [(s, Fb, self.nel_spin[s]) for s, Fb in self.pending]
+
+    def deliver(self, s, dens):
+        self.dens[s] = dens
+
+    def advance(self):
+        """One Fock build, energy and convergence test on the delivered
+        densities; leaves the next eigenproblems in `pending` or sets done."""
+        torch = self.torch
+        self.pending = []
+        if self.it >= self.maxit:
+            self.done = True
+            return
+        self.it += 1
+        clk, bs, nspin = self.clk, self.bs, self.nspin
+        f64 = torch.float64
+        Ps = [d[0] for d in self.dens]
+        P = Ps[0] if self.restricted else Ps[0] + Ps[1]
+        J = self.basis.coulomb(P)
         clk.tick("coulomb")
         Ecoul = 0.5 * torch.sum(P * J)
-        Fs = [H + J for _ in range(nspin)]
-        Exx = torch.zeros((), dtype=f64, device=dev)
-        Exc = torch.zeros((), dtype=f64, device=dev)
-        if xc is None:
-            kp = "fp32" if fp32 else "fp64"
+        Fs = [self.H + J for _ in range(nspin)]
+        Exx = torch.zeros((), dtype=f64, device=self.dev)
+        Exc = torch.zeros((), dtype=f64, device=self.dev)
+        if self.xc is None:
+            kp = "fp32" if self.fp32 else "fp64"
             for s in range(nspin):
-                Psp = 0.5 * Ps[s] if restricted else Ps[s]
-                if nel_spin[s] == 0:
+                Psp = 0.5 * Ps[s] if self.restricted else Ps[s]
+                if self.nel_spin[s] == 0:
                     continue
-                Ks = basis.exchange(Psp, precision=kp)
-                Exx = Exx + (2.0 if restricted else 1.0) * 0.5 * torch.sum(Psp * Ks)
+                Ks = self.basis.exchange(Psp, precision=kp)
+                Exx = Exx + (2.0 if self.restricted else 1.0) * 0.5 * torch.sum(Psp * Ks)
                 Fs[s] = Fs[s] + Ks
             clk.tick("exchange")
         else:
-            if restricted:
-                Vxc, Exc = xc(P)
+            if self.restricted:
+                Vxc, Exc = self.xc(P)
                 Fs[0] = Fs[0] + Vxc
             else:
-                Va, Vb, Exc = xc(Ps[0], Ps[1])
+                Va, Vb, Exc = self.xc(Ps[0], Ps[1])
                 Fs[0] = Fs[0] + Va
                 Fs[1] = Fs[1] + Vb
             clk.tick("xc")
-        Ekin = torch.sum(P * T)
-        Enuc = torch.sum(P * V)
-        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + Enucr
+        Ekin = torch.sum(P * self.T)
+        Enuc = torch.sum(P * self.V)
+        self.terms = dict(Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc)
+        E = float(Ekin + Enuc + Ecoul + Exx + Exc) + self.Enucr
+        self.E = E
         Fb = [bs.blocks_of(F) for F in Fs]
         err = []
         for s in range(nspin):
-            err += bs.commutator(Fb[s], dens[s][1])
+            err += bs.commutator(Fb[s], self.dens[s][1])
         emax = float(torch.max(torch.stack([torch.max(torch.abs(x)) for x in err])))
-        hist.append(E)
+        self.hist.append(E)
         clk.tick("energy+diis")
-        if verbose:
-            print(f"it {it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if fp32 else ''}", flush=True)
-        if fp32:
-            it_fp32 = it
-            if emax < switch:
+        if self.verbose:
+            print(f"it {self.it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if self.fp32 else ''}", flush=True)
+        if self.fp32:
+            self.it_fp32 = self.it
+            if emax < self.switch:
                 # FP64 from here on: the FP32 Fock matrices leave the DIIS
                 # history and this step is a plain diagonalisation
-                fp32 = False
-                diis.reset()
-                Eold = None
-                dens = [bs.density(Fb[s], nel_spin[s], weight, H) for s in range(nspin)]
-                clk.tick("density")
-                continue
-        elif Eold is not None and abs(E - Eold) < conv and emax < 1e-6:
-            break
+                self.fp32 = False
+                self.diis.reset()
+                self.Eold = None
+                self.pending = [(s, Fb[s]) for s in range(nspin)]
+                return
+        elif self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6:
+            self.done = True
+            return
         else:
-            Eold = E
+            self.Eold = E
         nb = len(bs.b)
-        diis.push([x for F in Fb for x in F], err)
-        Fd = diis.extrapolate()
+        self.diis.push([x for F in Fb for x in F], err)
+        Fd = self.diis.extrapolate()
         clk.tick("energy+diis")
-        dens = [bs.density(Fd[s * nb:(s + 1) * nb], nel_spin[s], weight, H) for s in range(nspin)]
-        clk.tick("density")
-    torch.cuda.synchronize(dev)
-    out = SCFResult(E=E, Ekin=float(Ekin), Enuc=float(Enuc), Ecoul=float(Ecoul), Exx=float(Exx), Exc=float(Exc),
-                    Enucr=Enucr, iterations=it, iterations_fp32=it_fp32, seconds=time.perf_counter() - t0,
-                    history=hist, phases=clk.t, eigensolves=bs.solves, reused_solves=bs.paired_solves)
-    if restricted:
-        out.update(P=dens[0][0], occupations=dens[0][2])
-    else:
-        out.update(Pa=dens[0][0], Pb=dens[1][0], occupations=(dens[0][2], dens[1][2]))
-    return out
+        self.pending = [(s, Fd[s * nb:(s + 1) * nb]) for s in range(nspin)]
+
+    def result(self):
+        _sync(self.dev)
+        t = self.terms
+        out = SCFResult(E=self.E, Ekin=float(t["Ekin"]), Enuc=float(t["Enuc"]), Ecoul=float(t["Ecoul"]),
+                        Exx=float(t["Exx"]), Exc=float(t["Exc"]), Enucr=self.Enucr, iterations=self.it,
+                        iterations_fp32=self.it_fp32, seconds=time.perf_counter() - self.t0, history=self.hist,
+                        phases=self.clk.t, eigensolves=self.bs.solves, reused_solves=self.bs.paired_solves,
+                        tag=self.tag)
+        if self.restricted:
+            out.update(P=self.dens[0][0], occupations=self.dens[0][2])
+        else:
+            out.update(Pa=self.dens[0][0], Pb=self.dens[1][0], occupations=(self.dens[0][2], self.dens[1][2]))
+        return out
+
+
+class SCFEngine:
+    """Advances many SCFStates in lockstep: every step diagonalises all the
+    pending Fock blocks of all states together (equal-size eigenproblems in
+    one batched call), then runs each state's Fock build.  States can join
+    between steps and leave when they converge (continuous batching)."""
+
+    def __init__(self):
+        self.active = []
+        self.steps = 0
+        self.eig_seconds = 0.0
+
+    def add(self, state):
+        self.active.append(state)
+
+    def step(self):
+        """One iteration of every active state; returns the states that
+        finished in it."""
+        import torch
+        t = time.perf_counter()
+        reqs = []
+        for st in self.active:
+            for s, Fb, nocc in st.requests():
+                reqs.append((st, s, st.bs.prepare(Fb), nocc))
+        sols = batched_eigh([m for r in reqs for m in r[2]["mats"]])
+        off = 0
+        for st, s, plan, nocc in reqs:
+            n = len(plan["mats"])
+            st.deliver(s, st.bs.finish(plan, sols[off:off + n], nocc, st.weight, st.H))
+            off += n
+        for dev in {st.dev for st in self.active}:
+            _sync(dev)
+        dt = time.perf_counter() - t
+        self.eig_seconds += dt
+        for st in self.active:
+            st.clk.t["density"] = st.clk.t.get("density", 0.0) + dt
+            st.clk.last = time.perf_counter()
+            st.advance()
+        self.steps += 1
+        done = [st for st in self.active if st.done]
+        self.active = [st for st in self.active if not st.done]
+        return done
+
+
+def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch):
+    st = SCFState(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch)
+    eng = SCFEngine()
+    eng.add(st)
+    while eng.active:
+        eng.step()
+    return st.result()
 
 
 def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100, diis_len: int = 10,
