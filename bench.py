@@ -52,7 +52,8 @@ WORKLOADS = {
 }
 # exchange launches (timed together for the roofline) and the Coulomb kernels
 KERNELS = {"atomic": (("atomic_k_off", "atomic_k_in"), ("atomic_j_gather", "atomic_j_expand")),
-           "diatomic": (("diatomic_k_off", "diatomic_k_in"), ("diatomic_j_gather", "diatomic_j_expand"))}
+           "diatomic": (("diatomic_k_off", "diatomic_k_in"),
+                        ("diatomic_ptile", "diatomic_j_gather", "diatomic_j_expand"))}
 
 
 def occupied_shells(Z: int):
@@ -243,8 +244,9 @@ def main():
                               f"tr(PS) = {nelec:.3f}",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k", "diatomic": "k_dia_k"}[kind] +
-                                                " (exchange: off- and in-element launches)",
+        "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k_off_f64 + k_atomic_k_in_f64",
+                                                 "diatomic": "k_dia_k_off<double> + k_dia_kin_lds<double>"}[kind] +
+                                                " (exchange: cross- and in-element launches of one build)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),
                      "avg_launch_ms": k_ms, "flops_per_launch": k_flops},

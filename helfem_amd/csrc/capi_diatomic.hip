@@ -485,6 +485,31 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
     }
     d.term_pos.upload(tpos, s);
   }
+  {
+    // source blocks referenced by the terms, and each term's tile among them
+    std::vector<int> id((size_t)Na * Na, -1), src, tt(til.size());
+    for (size_t q = 0; q < til.size(); ++q) {
+      int &x = id[til[q]];
+      if (x < 0) {
+        x = (int)src.size();
+        src.push_back(til[q]);
+      }
+      tt[q] = x;
+    }
+    const double bytes = (double)src.size() * S.Nel() * S.Nel() * TILE2 * sizeof(double);
+    double budget;
+    if (const char *env = std::getenv("HFGPU_EXCHANGE_TILE_GIB")) {
+      budget = std::atof(env) * (double)(1ull << 30);
+    } else {
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "device memory info");
+      budget = 0.5 * (double)(free_b + d.ptile.n * sizeof(double));
+    }
+    d.tiled = bytes <= budget;
+    d.n_src = (int)src.size();
+    d.src_blk.upload(src, s);
+    d.term_tile.upload(tt, s);
+  }
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;

@@ -19,7 +19,11 @@
 // element pair (e,f)); for each coupled (L,|M|) the four Gaunt-weighted
 // density tiles R00, R02, R20, R22 are formed in registers and contracted on
 // the matrix cores with the disjoint P/Q factors (e != f) or the four-channel
-// Cholesky factor (e == f).
+// Cholesky factor (e == f).  The density tiles are first copied out of P into
+// a tiled array (k_dia_ptile: one contiguous, aligned 2 KB tile per (element
+// pair, density block) in operand order), so a term's gather is one aligned
+// 32-byte read per lane and an element pair's slice (3.3 MB for N2 at lmax
+// 16) stays in the L2 of the XCD that works on it.
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
 
@@ -292,6 +296,74 @@ __device__ inline void gather_terms(const T *P, int64_t ldp, const int4 *tpos, c
   }
 }
 
+// Density tiles in gather order: tile (ef = e*Nel+f, src) holds the 16x16
+// block of P between element e's radial functions of shell i and element f's
+// of shell l (src = i*Nang+l from the source list), stored lane-major in the
+// matrix-core operand order: entry lane*4 + s is R(c, row(g,s)) for lane
+// (g,c), zero where a radial index is outside the element or is the dropped
+// radial function 0 of an m != 0 shell.  One wave per tile.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_dia_ptile(const double *P, int64_t ldp, int Nel, const int *first,
+                                                  const int *ni, const int *skip, const long long *poff,
+                                                  int Nang, const int *src_blk, int n_src, T *out) {
+  const int64_t tile = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int64_t n_tiles = (int64_t)Nel * Nel * n_src;
+  if (tile >= n_tiles) return;
+  const int ef = (int)(tile / n_src), src = (int)(tile - (int64_t)ef * n_src);
+  const int e = ef / Nel, f = ef - e * Nel;
+  const int blk = src_blk[src], i = blk / Nang, l = blk - i * Nang;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const int rr = first[e] + c, si = skip[i], sl = skip[l];
+  const bool rv = c < ni[e] && rr >= si;
+  typename Mma<T>::acc_t v;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = Mma<T>::row(g, s), cc = first[f] + kk;
+    const bool ok = rv && kk < ni[f] && cc >= sl;
+    v[s] = ok ? (T)P[(poff[i] + rr - si) + (poff[l] + cc - sl) * ldp] : T(0);
+  }
+  *reinterpret_cast<typename Mma<T>::acc_t *>(out + tile * TILE2 + lane * 4) = v;
+}
+
+// gather_terms over the tiled density: term q reads tile tt[q] of the
+// element pair's slice pt (one 4-vector per lane), masks already applied.
+template <typename T, int U = 2>
+__device__ inline void gather_tiles(const T *pt, const int *tt, const T *term_f, int q0, int q1, T r00[4],
+                                    T r02[4], T r20[4], T r22[4]) {
+  typedef typename Mma<T>::acc_t vec;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) r00[s] = r02[s] = r20[s] = r22[s] = T(0);
+  int q = q0;
+  for (; q + U <= q1; q += U) {
+    vec x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = *reinterpret_cast<const vec *>(pt + (size_t)tt[q + u] * TILE2);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T f00 = term_f[4 * (q + u)], f02 = term_f[4 * (q + u) + 1], f20 = term_f[4 * (q + u) + 2],
+              f22 = term_f[4 * (q + u) + 3];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        r00[s] += f00 * x[u][s];
+        r02[s] += f02 * x[u][s];
+        r20[s] += f20 * x[u][s];
+        r22[s] += f22 * x[u][s];
+      }
+    }
+  }
+  for (; q < q1; ++q) {
+    const vec x = *reinterpret_cast<const vec *>(pt + (size_t)tt[q] * TILE2);
+    const T f00 = term_f[4 * q], f02 = term_f[4 * q + 1], f20 = term_f[4 * q + 2], f22 = term_f[4 * q + 3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      r00[s] += f00 * x[s];
+      r02[s] += f02 * x[s];
+      r20[s] += f20 * x[s];
+      r22[s] += f22 * x[s];
+    }
+  }
+}
+
 template <typename T>
 struct DKArgs {
   const T *P;
@@ -309,14 +381,18 @@ struct DKArgs {
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
   const double *lsig;  // FP32 only: log scale of the (ilm, element) P/Q tiles
+  const T *pt;         // tiled density (k_dia_ptile) or null: gather from P
+  const int *ttile;    // per term: its tile in an element pair's slice
+  int n_src;
 };
 
 // One workgroup per (active output block (j,k), element pair (e,f)), the
 // four waves splitting the block's coupled (L,|M|) runs.  Workgroups sharing
-// (e,f) are mapped to one XCD so the (e,f) slice of P stays in its L2.
+// (e,f) are mapped to one XCD so the (e,f) slice of the density tiles (or of
+// P, when the tiles do not fit) stays in its L2.
 // T = double is the reference's arithmetic; T = float gathers and multiplies
 // in FP32 and sums the waves' tiles in FP64.
-template <bool INSIDE, typename T>
+template <bool INSIDE, bool TILED, typename T>
 __device__ inline void dia_k_body(const DKArgs<T> &A) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
@@ -345,11 +421,15 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
     cmask[s] = (c < ni && kk[s] < nj) ? T(1) : T(0);
   }
   const int r0 = A.act_lstart[act], r1 = A.act_lstart[act + 1];
+  const T *pt = TILED ? A.pt + (size_t)ef * A.n_src * TILE2 + lane * 4 : nullptr;
   for (int run = r0 + wv; run < r1; run += 4) {
     const int ilm = A.lrun_ilm[run];
     T r00[4], r02[4], r20[4], r22[4];
-    gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
-                    r00, r02, r20, r22);
+    if constexpr (TILED)
+      gather_tiles<T>(pt, A.ttile, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], r00, r02, r20, r22);
+    else
+      gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
+                      r00, r02, r20, r22);
     const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2;
     if (!INSIDE) {
       const T *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
@@ -440,13 +520,13 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
 
 // Cross-element launch held to 5 waves per SIMD (96 registers), which the
 // two-term gather fits; the in-element fallback keeps the compiler's choice.
-template <typename T>
+template <typename T, bool TILED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) k_dia_k_off(DKArgs<T> A) {
-  dia_k_body<false, T>(A);
+  dia_k_body<false, TILED, T>(A);
 }
-template <typename T>
+template <typename T, bool TILED>
 __global__ void __launch_bounds__(NT) k_dia_k_in(DKArgs<T> A) {
-  dia_k_body<true, T>(A);
+  dia_k_body<true, TILED, T>(A);
 }
 
 // --- In-element exchange with the Cholesky factors resident in LDS -------------
@@ -481,9 +561,12 @@ struct DKLArgs {
   const int *wg;       // per workgroup: ilm, e, first entry in runs, count (or ilm = -1)
   const int *runs;     // runs grouped by ilm
   const int *run_act;  // output block of each run
+  const T *pt;         // tiled density (k_dia_ptile) or null: gather from P
+  const int *ttile;
+  int n_src;
 };
 
-template <typename T>
+template <typename T, bool TILED>
 __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
@@ -513,11 +596,15 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
     cloc[s] = fe + min(kk[s], n - 1);
     cmask[s] = (c < n && kk[s] < n) ? T(1) : T(0);
   }
+  const T *pt = TILED ? A.pt + (size_t)(e * A.Nel + e) * A.n_src * TILE2 + lane * 4 : nullptr;
   for (int i = wv; i < nb; i += KL_WAVES) {
     const int run = A.runs[b0 + i], act = A.run_act[run];
     T r00[4], r02[4], r20[4], r22[4];
-    gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
-                    r00, r02, r20, r22);
+    if constexpr (TILED)
+      gather_tiles<T>(pt, A.ttile, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], r00, r02, r20, r22);
+    else
+      gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
+                      r00, r02, r20, r22);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       r02[s] = -r02[s];
@@ -717,6 +804,19 @@ void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64
 namespace {
 // FP32 copy of the P/Q tiles with P kinds divided and Q kinds multiplied by
 // exp(lsig) of their (ilm, element).
+template <typename T>
+void launch_dia_k(int part, bool tiled, int64_t grid, hipStream_t s, const DKArgs<T> &a) {
+  const dim3 gd((unsigned)grid), bd(NT);
+  if (part == 0 && tiled)
+    hipLaunchKernelGGL((k_dia_k_off<T, true>), gd, bd, 0, s, a);
+  else if (part == 0)
+    hipLaunchKernelGGL((k_dia_k_off<T, false>), gd, bd, 0, s, a);
+  else if (tiled)
+    hipLaunchKernelGGL((k_dia_k_in<T, true>), gd, bd, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_dia_k_in<T, false>), gd, bd, 0, s, a);
+}
+
 __global__ void k_scale_pq_f32(const double *pq, const double *lsig, float *pq32, int n_x) {
   const int x = blockIdx.x;
   if (x >= n_x) return;
@@ -749,8 +849,25 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       to_f32_launch(d.term_f.p, d.term_f32.p, d.term_f.n, s);
       d.term_f32_gen = d.list_gen;
     }
-    d.P32.alloc((size_t)d.Nbf * d.Nbf);
-    mat_to_f32_launch(P, ldp, d.P32.p, d.Nbf, s);
+    if (!d.tiled) {
+      d.P32.alloc((size_t)d.Nbf * d.Nbf);
+      mat_to_f32_launch(P, ldp, d.P32.p, d.Nbf, s);
+    }
+  }
+  if (d.tiled) {
+    const int64_t n_tiles = (int64_t)d.Nel * d.Nel * d.n_src, n_wg = (n_tiles + NT / 64 - 1) / (NT / 64);
+    if (n_wg > 0x7fffffff) throw Error(-1, "density tile grid too large");
+    TimedLaunch tl(timer, fp32 ? "diatomic_ptile_f32" : "diatomic_ptile", s, 0.0);
+    if (fp32) {
+      d.ptile32.alloc((size_t)n_tiles * TILE2);
+      hipLaunchKernelGGL((k_dia_ptile<float>), dim3((unsigned)n_wg), dim3(NT), 0, s, P, ldp, d.Nel, d.first.p,
+                         d.ni.p, d.skip.p, d.poff.p, d.Nang, d.src_blk.p, d.n_src, d.ptile32.p);
+    } else {
+      d.ptile.alloc((size_t)n_tiles * TILE2);
+      hipLaunchKernelGGL((k_dia_ptile<double>), dim3((unsigned)n_wg), dim3(NT), 0, s, P, ldp, d.Nel, d.first.p,
+                         d.ni.p, d.skip.p, d.poff.p, d.Nang, d.src_blk.p, d.n_src, d.ptile.p);
+    }
+    hip_check(hipGetLastError(), "diatomic_ptile");
   }
   // two launches: element pairs e != f (disjoint P/Q factors) and the
   // in-element tiles (Cholesky factor); boundary rows/cols shared between
@@ -778,13 +895,21 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       if (fp32) {
         DKLArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                          d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p,
-                         d.term_pos.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
-        hipLaunchKernelGGL((k_dia_kin_lds<float>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+                         d.term_pos.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p,
+                         d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src};
+        if (d.tiled)
+          hipLaunchKernelGGL((k_dia_kin_lds<float, true>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_dia_kin_lds<float, false>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
       } else {
         DKLArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.chol.p,
                           d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p, d.term_pos.p,
-                          d.term_f.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p};
-        hipLaunchKernelGGL((k_dia_kin_lds<double>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+                          d.term_f.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p,
+                          d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src};
+        if (d.tiled)
+          hipLaunchKernelGGL((k_dia_kin_lds<double, true>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_dia_kin_lds<double, false>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
       }
       hip_check(hipGetLastError(), "diatomic_kin_lds");
       continue;
@@ -797,19 +922,15 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
     if (fp32) {
       DKArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                       d.pq32.p, d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p,
-                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_pos.p, d.term_f32.p, efl, n_ef, d.lsig.p};
-      if (part == 0)
-        hipLaunchKernelGGL((k_dia_k_off<float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_dia_k_in<float>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+                      d.lrun_ilm.p, d.lrun_tstart.p, d.term_pos.p, d.term_f32.p, efl, n_ef, d.lsig.p,
+                      d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src};
+      launch_dia_k<float>(part, d.tiled, grid, s, a);
     } else {
       DKArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
                        d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p,
-                       d.lrun_tstart.p, d.term_pos.p, d.term_f.p, efl, n_ef, nullptr};
-      if (part == 0)
-        hipLaunchKernelGGL((k_dia_k_off<double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_dia_k_in<double>), dim3((unsigned)grid), dim3(NT), 0, s, a);
+                       d.lrun_tstart.p, d.term_pos.p, d.term_f.p, efl, n_ef, nullptr,
+                       d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src};
+      launch_dia_k<double>(part, d.tiled, grid, s, a);
     }
     hip_check(hipGetLastError(), "diatomic_k");
   }

@@ -245,6 +245,15 @@ struct DiatomicDevice {
   // first entry in kl_runs, count); runs grouped by ilm; output block per run
   DevArray<int> wg_kl, kl_runs, run_act;
   int n_wg_kl = 0;
+  // Density tiles of the screened source blocks in gather order
+  // ([e*Nel+f][src][lane][4], see k_dia_ptile), rebuilt per exchange call;
+  // src_blk lists the blocks i*Nang+l, term_tile maps each term to its tile.
+  // Used when the tiles fit tile_budget bytes, else gathers read P.
+  DevArray<double> ptile;
+  DevArray<float> ptile32;
+  DevArray<int> src_blk, term_tile;
+  int n_src = 0;
+  bool tiled = false;
   // FP32 copies for the reduced-precision exchange (built on first use)
   DevArray<float> pq32, chol32, sigma32, term_f32, P32;
   DevArray<double> lsig;                  // [n_lm*Nel] log of the P/Q scale of the FP32 tiles

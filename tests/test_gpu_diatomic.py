@@ -144,6 +144,34 @@ def test_dense_operator_inside_elements(gpu_ctx, monkeypatch):
     assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
 
 
+@pytest.mark.parametrize("name", ["N2-l432-3el", "HeH-l2-1el"])
+def test_direct_gather_without_tile_budget(name, gpu_ctx, monkeypatch):
+    """With no memory granted for the tiled density (HFGPU_EXCHANGE_TILE_GIB=0)
+    the exchange gathers straight from P; FP64 still meets the oracle bar and
+    FP32 agrees with the tiled FP32 path to rounding."""
+    import torch
+    print(f"[gpu] diatomic K, direct density gathers: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = CASES[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    ob.compute_tei()
+    P = random_symmetric(ob.Nbf, seed=1005)
+    Pd = torch.as_tensor(P, device="cuda")
+    out = {}
+    for budget in ("0", None):
+        if budget is None:
+            monkeypatch.delenv("HFGPU_EXCHANGE_TILE_GIB", raising=False)
+        else:
+            monkeypatch.setenv("HFGPU_EXCHANGE_TILE_GIB", budget)
+        hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+        hb.compute_tei()
+        out[budget] = (hb.exchange(Pd).cpu().numpy(), hb.exchange(Pd, precision="fp32").cpu().numpy())
+    torch.cuda.synchronize()
+    Kr = ob.exchange(P)
+    assert relerr(out["0"][0], Kr) <= TOL
+    assert relerr(out[None][0], Kr) <= TOL
+    assert relerr(out["0"][1], out[None][1]) <= 1e-6
+
+
 @pytest.mark.parametrize("absm", [False, True])
 def test_nh_triplet_uhf_on_gpu(absm, gpu_ctx):
     """diatomic-NH-hf-u and -hf-absm: 5 alpha + 3 beta electrons, R = 1.958
