@@ -51,7 +51,7 @@ WORKLOADS = {
                             desc="Ne HF atomic, lmax=mmax=1, 10 LIP-15 elements to 40 bohr, Nbf 556"),
 }
 # exchange launches (timed together for the roofline) and the Coulomb kernels
-KERNELS = {"atomic": (("atomic_k_off", "atomic_k_in"), ("atomic_j_gather", "atomic_j_expand")),
+KERNELS = {"atomic": (("atomic_k_off", "atomic_k_in"), ("atomic_ptile", "atomic_j_gather", "atomic_j_expand")),
            "diatomic": (("diatomic_k_off", "diatomic_k_in"),
                         ("diatomic_ptile", "diatomic_j_gather", "diatomic_j_expand"))}
 

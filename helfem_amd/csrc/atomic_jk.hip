@@ -191,7 +191,37 @@ struct KArgs {
   const int *efl;  // element pairs e*Nel+f of this launch
   int n_ef;
   const T *Lscale;  // optional per-L factor on the Gaunt weights (range separation)
+  const T *pt;      // tiled density (k_atomic_ptile) or null: gather from P
+  const int *ttile; // per term: its tile in an element pair's slice
+  int n_src;
+  int sym;          // symmetric density: e > f pairs (e == f: j <= k) also written to the mirror
 };
+
+// Density tiles in gather order: tile (ef = e*Nel+f, src) holds the 16x16
+// block of P between element e's radial functions of shell i and element f's
+// of shell l (src = i*Nang+l from the source list), lane-major in the
+// matrix-core operand order (entry lane*4 + s is P(c, row(g,s)) for lane
+// (g,c)), zero outside the elements.  One wave per tile.
+template <typename T>
+__global__ void __launch_bounds__(NT) k_atomic_ptile(const double *P, int64_t ldp, int Nel, int Nrad,
+                                                     const int *first, const int *ni, int Nang, const int *src_blk,
+                                                     int n_src, T *out) {
+  const int64_t tile = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  if (tile >= (int64_t)Nel * Nel * n_src) return;
+  const int ef = (int)(tile / n_src), src = (int)(tile - (int64_t)ef * n_src);
+  const int e = ef / Nel, f = ef - e * Nel;
+  const int blk = src_blk[src], i = blk / Nang, l = blk - i * Nang;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const bool rv = c < ni[e];
+  const int64_t row = (int64_t)i * Nrad + first[e] + c, col0 = (int64_t)l * Nrad + first[f];
+  typename Mma<T>::acc_t v;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = Mma<T>::row(g, s);
+    v[s] = (rv && kk < ni[f]) ? (T)P[row + (col0 + kk) * ldp] : T(0);
+  }
+  *reinterpret_cast<typename Mma<T>::acc_t *>(out + tile * TILE2 + lane * 4) = v;
+}
 
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
 // four waves split the multipoles L.  For each L a wave forms the
@@ -202,8 +232,11 @@ struct KArgs {
 //   e == f:  K += sum_p M_p R M_p'   (M_p the Cholesky vectors as 16x16)
 // T = double is the reference's arithmetic; T = float runs the gathers and
 // products in FP32 (half the bytes, the FP32 matrix rate) and sums the four
-// waves' tiles in FP64.  Workgroups sharing (e,f) run on one XCD.
-template <bool INSIDE, typename T>
+// waves' tiles in FP64.  Workgroups sharing (e,f) run on one XCD.  The
+// density tiles come from the tiled copy (k_atomic_ptile: one aligned 2 KB
+// tile per (element pair, density block)) when it fits in memory, else
+// straight from P.
+template <bool INSIDE, bool TILED, typename T>
 __device__ inline void atomic_k_body(const KArgs<T> &A) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
@@ -214,6 +247,7 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
   if (efi >= A.n_ef) return;
   const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  if (INSIDE && A.sym && j > k) return;  // the mirror of block (k, j) covers it
   const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
@@ -231,32 +265,57 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
     cloc[s] = fj + min(kk[s], nj - 1);
     cmask[s] = (c < ni && kk[s] < nj) ? T(1) : T(0);
   }
+  typedef typename MM::acc_t vec;
+  const T *pt = TILED ? A.pt + (size_t)ef * A.n_src * TILE2 + lane * 4 : nullptr;
   for (int L = wv; L < A.N_L; L += 4) {
     const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
     if (q0 == q1) continue;
     T r[4] = {0, 0, 0, 0};
     int q = q0;
-    for (; q + 4 <= q1; q += 4) {
-      T x[4][4], cq[4];
+    if constexpr (TILED) {
+      // one aligned 4-vector per lane and term, masks already in the tiles
+      for (; q + 4 <= q1; q += 4) {
+        vec x[4];
+        T cq[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int2 to = A.term_off[q + u];
-        cq[u] = A.term_c[q + u];
+        for (int u = 0; u < 4; ++u) {
+          x[u] = *reinterpret_cast<const vec *>(pt + (size_t)A.ttile[q + u] * TILE2);
+          cq[u] = A.term_c[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
+      }
+      for (; q < q1; ++q) {
+        const vec x = *reinterpret_cast<const vec *>(pt + (size_t)A.ttile[q] * TILE2);
+        const T cq = A.term_c[q];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) r[s] += cq * x[s];
+      }
+    } else {
+      for (; q + 4 <= q1; q += 4) {
+        T x[4][4], cq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int2 to = A.term_off[q + u];
+          cq[u] = A.term_c[q + u];
+          const T *Pr = A.P + (int64_t)(to.x + rloc);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) x[u][s] = Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
+      }
+      for (; q < q1; ++q) {
+        const int2 to = A.term_off[q];
+        const T cq = A.term_c[q];
         const T *Pr = A.P + (int64_t)(to.x + rloc);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) x[u][s] = Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
+        for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
-    }
-    for (; q < q1; ++q) {
-      const int2 to = A.term_off[q];
-      const T cq = A.term_c[q];
-      const T *Pr = A.P + (int64_t)(to.x + rloc);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) r[s] += cq * Pr[(int64_t)(to.y + cloc[s]) * A.ldp];
     }
     const T msc = A.Lscale ? A.Lscale[L] : T(1);
 #pragma unroll
@@ -319,19 +378,58 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
     atomicAdd(dst, -v);
   else
     *dst = -v;
+  if (A.sym && (!INSIDE || j != k)) {
+    // mirror tile (k, j, f, e) = this tile transposed
+    double *mdst = A.K + col + row * A.ldk;
+    if (srow || scol)
+      atomicAdd(mdst, -v);
+    else
+      *mdst = -v;
+  }
 }
 
 // Launch wrappers.  The cross-element body fits 6 waves per SIMD in FP64
 // (75 registers) and 8 in FP32 without spills; the in-element body keeps the
 // compiler's register choice.
+template <bool TILED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_atomic_k_off_f64(KArgs<double> A) {
-  atomic_k_body<false, double>(A);
+  atomic_k_body<false, TILED, double>(A);
 }
+template <bool TILED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_atomic_k_off_f32(KArgs<float> A) {
-  atomic_k_body<false, float>(A);
+  atomic_k_body<false, TILED, float>(A);
 }
-__global__ void __launch_bounds__(NT) k_atomic_k_in_f64(KArgs<double> A) { atomic_k_body<true, double>(A); }
-__global__ void __launch_bounds__(NT) k_atomic_k_in_f32(KArgs<float> A) { atomic_k_body<true, float>(A); }
+template <bool TILED>
+__global__ void __launch_bounds__(NT) k_atomic_k_in_f64(KArgs<double> A) {
+  atomic_k_body<true, TILED, double>(A);
+}
+template <bool TILED>
+__global__ void __launch_bounds__(NT) k_atomic_k_in_f32(KArgs<float> A) {
+  atomic_k_body<true, TILED, float>(A);
+}
+
+void launch_atomic_k(int part, bool tiled, int64_t grid, hipStream_t s, const KArgs<double> &a) {
+  const dim3 gd((unsigned)grid), bd(NT);
+  if (part == 0 && tiled)
+    hipLaunchKernelGGL(k_atomic_k_off_f64<true>, gd, bd, 0, s, a);
+  else if (part == 0)
+    hipLaunchKernelGGL(k_atomic_k_off_f64<false>, gd, bd, 0, s, a);
+  else if (tiled)
+    hipLaunchKernelGGL(k_atomic_k_in_f64<true>, gd, bd, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_atomic_k_in_f64<false>, gd, bd, 0, s, a);
+}
+void launch_atomic_k(int part, bool tiled, int64_t grid, hipStream_t s, const KArgs<float> &a) {
+  const dim3 gd((unsigned)grid), bd(NT);
+  if (part == 0 && tiled)
+    hipLaunchKernelGGL(k_atomic_k_off_f32<true>, gd, bd, 0, s, a);
+  else if (part == 0)
+    hipLaunchKernelGGL(k_atomic_k_off_f32<false>, gd, bd, 0, s, a);
+  else if (tiled)
+    hipLaunchKernelGGL(k_atomic_k_in_f32<true>, gd, bd, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_atomic_k_in_f32<false>, gd, bd, 0, s, a);
+}
 
 // --- Range-separated exchange with non-separable (erfc) element pairs ------------
 //
@@ -591,6 +689,13 @@ void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStr
   hip_check(hipGetLastError(), "mat_to_f32");
 }
 
+double exchange_tile_budget(size_t held_bytes) {
+  if (const char *env = std::getenv("HFGPU_EXCHANGE_TILE_GIB")) return std::atof(env) * (double)(1ull << 30);
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "device memory info");
+  return 0.5 * (double)(free_b + held_bytes);
+}
+
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   const int64_t n = rows * cols;
   if (n == 0) return;
@@ -632,12 +737,23 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
   std::vector<unsigned char> mask((size_t)Na * Na);
   const double thr = 10.0 * 2.220446049250313e-16;
   for (size_t q = 0; q < mask.size(); ++q) mask[q] = d.h_bnorm.p[q] >= thr ? 1 : 0;
-  if (mask == d.screen_mask && !d.screen_mask.empty()) return;
+  // the symmetric exchange needs the mirror of every coupled block coupled
+  if (d.sym)
+    for (int i = 0; i < Na; ++i)
+      for (int l = 0; l < i; ++l) {
+        const unsigned char m = mask[(size_t)i * Na + l] | mask[(size_t)l * Na + i];
+        mask[(size_t)i * Na + l] = mask[(size_t)l * Na + i] = m;
+      }
+  if (mask == d.screen_mask && !d.screen_mask.empty() && d.sym == d.list_sym) return;
   std::vector<int> act, tstart(1, 0), til;
   std::vector<double> tc;
-  double frad = 0.0, frad_in = 0.0;
+  // flops as executed: in the symmetric mode the cross-element launch covers
+  // the pairs e > f and the in-element one the blocks j <= k
+  double frad_off = 0.0, frad_in = 0.0;
+  int64_t nterm_in = 0;
   for (size_t a = 0; a < d.h_act.size(); ++a) {
-    const size_t t0 = tstart.size(), n0 = til.size();
+    const size_t n0 = til.size();
+    const bool in_done = !d.sym || d.h_act[a] / Na <= d.h_act[a] % Na;
     std::vector<int> starts;
     for (int L = 0; L < nL; ++L) {
       const int q0 = d.h_tstart[a * nL + L], q1 = d.h_tstart[a * nL + L + 1];
@@ -648,29 +764,29 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
           tc.push_back(d.h_tc[q]);
         }
       if (til.size() > before) {
-        frad += d.h_radL[L];
-        frad_in += d.h_radL_in[L];
+        frad_off += d.sym ? d.h_radL_low[L] : d.h_radL[L] - d.h_radL_in[L];
+        if (in_done) frad_in += d.h_radL_in[L];
       }
       starts.push_back((int)til.size());
     }
     if (til.size() == n0) continue;  // nothing reaches this output block
+    if (in_done) nterm_in += (int64_t)(til.size() - n0);
     act.push_back(d.h_act[a]);
     tstart.insert(tstart.end(), starts.begin(), starts.end());
-    (void)t0;
   }
   d.n_act = (int)act.size();
   d.n_terms = (int64_t)til.size();
-  d.flops_k_radial = frad;
-  d.flops_k_angular = 2.0 * (double)til.size() * (double)d.Nrad * (double)d.Nrad;
-  // split between the in-element and cross-element launches, the angular
-  // sums in proportion to the tile areas they cover
-  double area_in = 0.0, area_all = 0.0;
+  // angular sums: 2 flops per term and tile entry of the element pairs covered
+  double area_in = 0.0, area_off = 0.0;
   for (int e = 0; e < d.Nel; ++e) {
     area_in += (double)d.h_ni[e] * d.h_ni[e];
-    for (int f = 0; f < d.Nel; ++f) area_all += (double)d.h_ni[e] * d.h_ni[f];
+    for (int f = 0; f < d.Nel; ++f)
+      if (d.sym ? e > f : e != f) area_off += (double)d.h_ni[e] * d.h_ni[f];
   }
-  d.flops_in = frad_in + d.flops_k_angular * area_in / area_all;
-  d.flops_off = d.flops_k_radial + d.flops_k_angular - d.flops_in;
+  d.flops_in = frad_in + 2.0 * (double)nterm_in * area_in;
+  d.flops_off = frad_off + 2.0 * (double)d.n_terms * area_off;
+  d.flops_k_radial = frad_in + frad_off;
+  d.flops_k_angular = d.flops_in + d.flops_off - d.flops_k_radial;
   if (d.dense_in && !act.empty()) {
     // runs (act, L) with terms, act-major; their term ranges are contiguous,
     // rq[r] .. rq[r+1]; grouped by L for the GEMM
@@ -720,7 +836,7 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
     d.Ypart.alloc((size_t)nr * Nel * TILE2);
     double fr = 0.0;
     for (int e = 0; e < Nel; ++e) fr += 2.0 * std::pow((double)d.h_ni[e], 4);
-    d.flops_in = (double)nr * fr + d.flops_k_angular * area_in / area_all;
+    d.flops_in = (double)nr * fr + 2.0 * (double)nterm_in * area_in;
   }
   if (act.empty()) act.push_back(0);
   if (til.empty()) {
@@ -736,16 +852,34 @@ static void compact_exchange_lists(AtomicDevice &d, hipStream_t s) {
       toff[q] = make_int2((til[q] / d.Nang) * d.Nrad, (til[q] % d.Nang) * d.Nrad);
     d.term_off.upload(toff, s);
   }
+  {
+    // source blocks referenced by the terms, and each term's tile among them
+    std::vector<int> id((size_t)Na * Na, -1), src, tt(til.size());
+    for (size_t q = 0; q < til.size(); ++q) {
+      int &x = id[til[q]];
+      if (x < 0) {
+        x = (int)src.size();
+        src.push_back(til[q]);
+      }
+      tt[q] = x;
+    }
+    const double bytes = (double)src.size() * d.Nel * d.Nel * TILE2 * sizeof(double);
+    d.tiled = bytes <= exchange_tile_budget(d.ptile.n * sizeof(double) + d.ptile32.n * sizeof(float));
+    d.n_src = (int)src.size();
+    d.src_blk.upload(src, s);
+    d.term_tile.upload(tt, s);
+  }
   d.term_c.upload(tc, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;
+  d.list_sym = d.sym;
   ++d.list_gen;
 }
 
 // Block-norm screen of P, compacted coupling lists, zeroed K; false when no
 // output block is coupled (K stays zero).
 static bool exchange_prologue(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
-                              hipStream_t s) {
+                              hipStream_t s, bool allow_sym) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   if (d.h_act.empty()) {
     fill_zero_launch(K, nbf, nbf, ldk, s);
@@ -755,29 +889,56 @@ static bool exchange_prologue(AtomicDevice &d, const double *P, int64_t ldp, dou
   hip_check(hipGetLastError(), "block_norms");
   hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
             "copy block norms");
+  const bool try_sym = allow_sym && !d.dense_in && symmetric_exchange_allowed();
+  if (try_sym) {
+    block_asym_launch(P, ldp, d.Nrad, d.Nang, nullptr, nullptr, d.asym.p, s);
+    hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
+                             s), "copy block asymmetry");
+  }
   hip_check(hipStreamSynchronize(s), "block norms sync");
+  d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
   compact_exchange_lists(d, s);
   fill_zero_launch(K, nbf, nbf, ldk, s);
   return d.n_act > 0;
 }
 
+// The tiled density of this call (FP64 or FP32), when the tiles are in use.
+static void build_density_tiles(AtomicDevice &d, const double *P, int64_t ldp, bool fp32, hipStream_t s,
+                                KernelTimer *timer) {
+  if (!d.tiled) return;
+  const int64_t n_tiles = (int64_t)d.Nel * d.Nel * d.n_src, n_wg = (n_tiles + NT / 64 - 1) / (NT / 64);
+  if (n_wg > 0x7fffffff) throw Error(-1, "density tile grid too large");
+  TimedLaunch tl(timer, fp32 ? "atomic_ptile_f32" : "atomic_ptile", s, 0.0);
+  if (fp32) {
+    d.ptile32.alloc((size_t)n_tiles * TILE2);
+    hipLaunchKernelGGL(k_atomic_ptile<float>, dim3((unsigned)n_wg), dim3(NT), 0, s, P, ldp, d.Nel, d.Nrad,
+                       d.first.p, d.ni.p, d.Nang, d.src_blk.p, d.n_src, d.ptile32.p);
+  } else {
+    d.ptile.alloc((size_t)n_tiles * TILE2);
+    hipLaunchKernelGGL(k_atomic_ptile<double>, dim3((unsigned)n_wg), dim3(NT), 0, s, P, ldp, d.Nel, d.Nrad,
+                       d.first.p, d.ni.p, d.Nang, d.src_blk.p, d.n_src, d.ptile.p);
+  }
+  hip_check(hipGetLastError(), "atomic_ptile");
+}
+
 void atomic_rs_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                                hipStream_t s, KernelTimer *timer) {
-  if (!exchange_prologue(d, P, ldp, K, ldk, s)) return;
+  // the erfc pair kernel covers every element pair itself
+  if (!exchange_prologue(d, P, ldp, K, ldk, s, d.rs_kind == 1)) return;
   if (d.rs_kind == 1) {
+    build_density_tiles(d, P, ldp, false, s, timer);
     for (int part = 0; part < 2; ++part) {
-      const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+      const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) / (d.sym ? 2 : 1) : d.Nel;
       if (n_ef == 0) continue;
       const int64_t grid = (int64_t)(((n_ef + 7) / 8) * 8) * d.n_act;
       if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
+      const int *efl = part == 0 ? (d.sym ? d.ef_low.p : d.ef_off.p) : d.ef_in.p;
       KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.y_rs.p,
                       d.y_rb.p, d.y_chol.p, d.y_chol_off.p, d.y_chol_rank.p, d.act_jk.p, d.term_start.p,
-                      d.term_off.p, d.term_c.p, part == 0 ? d.ef_off.p : d.ef_in.p, n_ef, d.rs_Lscale.p};
+                      d.term_off.p, d.term_c.p, efl, n_ef, d.rs_Lscale.p,
+                      d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src, (int)d.sym};
       TimedLaunch tl(timer, "atomic_rs_k_yukawa", s);
-      if (part == 0)
-        hipLaunchKernelGGL(k_atomic_k_off_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
-      else
-        hipLaunchKernelGGL(k_atomic_k_in_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
+      launch_atomic_k(part, d.tiled, grid, s, a);
       hip_check(hipGetLastError(), "atomic_rs_k");
     }
   } else if (d.rs_kind == 2) {
@@ -797,7 +958,7 @@ void atomic_rs_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, do
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                             hipStream_t s, KernelTimer *timer, bool fp32) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
-  if (!exchange_prologue(d, P, ldp, K, ldk, s)) return;
+  if (!exchange_prologue(d, P, ldp, K, ldk, s, true)) return;
   if (fp32) {
     if (!d.f32_tables) {
       d.rs32.alloc(d.rs.n);
@@ -813,13 +974,16 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
       to_f32_launch(d.term_c.p, d.term_c32.p, d.term_c.n, s);
       d.term_c32_gen = d.list_gen;
     }
-    d.P32.alloc((size_t)nbf * nbf);
-    mat_to_f32_launch(P, ldp, d.P32.p, nbf, s);
+    if (!d.tiled) {
+      d.P32.alloc((size_t)nbf * nbf);
+      mat_to_f32_launch(P, ldp, d.P32.p, nbf, s);
+    }
   }
+  build_density_tiles(d, P, ldp, fp32, s, timer);
   // two launches: element pairs e != f and the in-element tiles; boundary
   // rows/cols shared by neighbouring tiles are accumulated atomically in both
   for (int part = 0; part < 2; ++part) {
-    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) / (d.sym ? 2 : 1) : d.Nel;
     if (n_ef == 0) continue;
     const char *name = part == 0 ? (fp32 ? "atomic_k_off_f32" : "atomic_k_off")
                                  : (fp32 ? "atomic_k_in_f32" : "atomic_k_in");
@@ -838,24 +1002,20 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;
     const int64_t grid = (int64_t)n_ef_pad * d.n_act;
     if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-    const int *efl = part == 0 ? d.ef_off.p : d.ef_in.p;
+    const int *efl = part == 0 ? (d.sym ? d.ef_low.p : d.ef_off.p) : d.ef_in.p;
     TimedLaunch tl(timer, name, s, flops);
     if (fp32) {
       KArgs<float> a{d.P32.p, nbf, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p,
                      d.rs32.p, d.rb32.p, d.chol32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p,
-                     d.term_off.p, d.term_c32.p, efl, n_ef, nullptr};
-      if (part == 0)
-        hipLaunchKernelGGL(k_atomic_k_off_f32, dim3((unsigned)grid), dim3(NT), 0, s, a);
-      else
-        hipLaunchKernelGGL(k_atomic_k_in_f32, dim3((unsigned)grid), dim3(NT), 0, s, a);
+                     d.term_off.p, d.term_c32.p, efl, n_ef, nullptr, d.tiled ? d.ptile32.p : nullptr,
+                     d.term_tile.p, d.n_src, (int)d.sym};
+      launch_atomic_k(part, d.tiled, grid, s, a);
     } else {
       KArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act, d.first.p, d.ni.p, d.rs.p, d.rb.p,
                       d.chol.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_off.p,
-                      d.term_c.p, efl, n_ef, nullptr};
-      if (part == 0)
-        hipLaunchKernelGGL(k_atomic_k_off_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
-      else
-        hipLaunchKernelGGL(k_atomic_k_in_f64, dim3((unsigned)grid), dim3(NT), 0, s, a);
+                      d.term_c.p, efl, n_ef, nullptr, d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src,
+                      (int)d.sym};
+      launch_atomic_k(part, d.tiled, grid, s, a);
     }
     hip_check(hipGetLastError(), "atomic_k");
   }

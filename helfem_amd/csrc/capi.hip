@@ -218,14 +218,20 @@ void atomic_upload(hf_atomic *h) {
   // element and 2 Ni Nj (Ni + Nj) per element pair.
   d.h_radL.assign(nL, 0.0);
   d.h_radL_in.assign(nL, 0.0);
+  d.h_radL_low.assign(nL, 0.0);
   d.h_ni = S.rad.ni;
   {
-    std::vector<int> efi, efo;
+    std::vector<int> efi, efo, efl;
     for (int e = 0; e < Nel; ++e)
-      for (int f = 0; f < Nel; ++f) (e == f ? efi : efo).push_back(e * Nel + f);
+      for (int f = 0; f < Nel; ++f) {
+        (e == f ? efi : efo).push_back(e * Nel + f);
+        if (e > f) efl.push_back(e * Nel + f);
+      }
     if (efo.empty()) efo.push_back(0);
+    if (efl.empty()) efl.push_back(0);
     d.ef_in.upload(efi, s);
     d.ef_off.upload(efo, s);
+    d.ef_low.upload(efl, s);
   }
   for (int L = 0; L < nL; ++L) {
     double f = 0.0;
@@ -237,6 +243,7 @@ void atomic_upload(hf_atomic *h) {
         if (g != e) {
           const double m = S.rad.ni[g];
           f += 2.0 * n * m * (n + m);
+          if (e > g) d.h_radL_low[L] += 2.0 * n * m * (n + m);
         }
     }
     d.h_radL[L] = f;
@@ -249,6 +256,8 @@ void atomic_upload(hf_atomic *h) {
   d.n_act = 0;
   d.n_terms = 0;
   d.h_bnorm.alloc((size_t)Na * Na);
+  d.h_asym.alloc((size_t)Na * Na * 2);
+  d.asym.alloc((size_t)Na * Na * 2);
   d.Jin.alloc((size_t)std::max(nlm, 1) * Nel * TILE2);
   d.jsm.alloc((size_t)std::max(nlm, 1) * Nel);
   d.jbg.alloc((size_t)std::max(nlm, 1) * Nel);

@@ -239,26 +239,34 @@ void diatomic_upload(hf_diatomic *h) {
   // each element, 2 (4 ni nj^2 + 2 ni^2 nj) per element pair
   d.h_rad_in.assign(std::max(nlm, 1), 0.0);
   d.h_rad_off.assign(std::max(nlm, 1), 0.0);
+  d.h_rad_low.assign(std::max(nlm, 1), 0.0);
   for (int q = 0; q < nlm; ++q) {
-    double fi = 0.0, fo = 0.0;
+    double fi = 0.0, fo = 0.0, fl = 0.0;
     for (int e = 0; e < Nel; ++e) {
       const double n = S.rad.ni[e];
       fi += 12.0 * S.cdB[(size_t)q * Nel + e].c * n * n * n;
       for (int g = 0; g < Nel; ++g)
         if (g != e) {
-          const double m = S.rad.ni[g];
-          fo += 2.0 * (4.0 * n * m * m + 2.0 * n * n * m);
+          const double m = S.rad.ni[g], x = 2.0 * (4.0 * n * m * m + 2.0 * n * n * m);
+          fo += x;
+          if (e > g) fl += x;
         }
     }
     d.h_rad_in[q] = fi;
     d.h_rad_off[q] = fo;
+    d.h_rad_low[q] = fl;
   }
-  std::vector<int> efi, efo;
+  std::vector<int> efi, efo, efl;
   for (int e = 0; e < Nel; ++e)
-    for (int f = 0; f < Nel; ++f) (e == f ? efi : efo).push_back(e * Nel + f);
+    for (int f = 0; f < Nel; ++f) {
+      (e == f ? efi : efo).push_back(e * Nel + f);
+      if (e > f) efl.push_back(e * Nel + f);
+    }
   if (efo.empty()) efo.push_back(0);
+  if (efl.empty()) efl.push_back(0);
   d.ef_in.upload(efi, s);
   d.ef_off.upload(efo, s);
+  d.ef_low.upload(efl, s);
   // absm mirror pairs: K(j,k) = K(jm,km) for mj, mk < 0 (basis.cpp:2067-2086)
   std::vector<int> mir;
   if (h->absm) {
@@ -286,6 +294,8 @@ void diatomic_upload(hf_diatomic *h) {
   d.n_act = 0;
   d.h_bnorm.alloc((size_t)Na * Na);
   d.bnorm.alloc((size_t)Na * Na);
+  d.h_asym.alloc((size_t)Na * Na * 2);
+  d.asym.alloc((size_t)Na * Na * 2);
   d.Jin.alloc((size_t)std::max(nLM, 1) * Nel * 2 * TILE2);
   d.ds.alloc((size_t)std::max(nLM, 1) * Nel);
   d.db.alloc((size_t)std::max(nLM, 1) * Nel);
@@ -311,7 +321,14 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   std::vector<unsigned char> mask((size_t)Na * Na);
   const double thr = 10.0 * 2.220446049250313e-16;
   for (size_t q = 0; q < mask.size(); ++q) mask[q] = d.h_bnorm.p[q] >= thr ? 1 : 0;
-  if (!d.screen_mask.empty() && mask == d.screen_mask) return;
+  // the symmetric exchange needs the mirror of every coupled block coupled
+  if (d.sym)
+    for (int i = 0; i < Na; ++i)
+      for (int l = 0; l < i; ++l) {
+        const unsigned char m = mask[(size_t)i * Na + l] | mask[(size_t)l * Na + i];
+        mask[(size_t)i * Na + l] = mask[(size_t)l * Na + i] = m;
+      }
+  if (!d.screen_mask.empty() && mask == d.screen_mask && d.sym == d.list_sym) return;
   int mabs = 0;
   for (int m : mv) mabs = std::max(mabs, std::abs(m));
   const int dmoff = 2 * mabs;
@@ -362,11 +379,14 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   }
   std::vector<int> act, lstart(1, 0), rilm, rstart(1, 0), til;
   std::vector<double> tf;
+  // flops as executed: in the symmetric mode the cross-element launch covers
+  // the pairs e > f and the in-element one the blocks j <= k
   double frad_in = 0.0, frad_off = 0.0;
-  int64_t nterm = 0;
+  int64_t nterm = 0, nterm_in = 0;
   for (int jk = 0; jk < njk; ++jk) {
     const std::vector<Term> &v = per[jk];
     if (v.empty()) continue;
+    const bool in_done = !d.sym || jk / Na <= jk % Na;
     act.push_back(jk);
     for (size_t a = 0; a < v.size();) {
       size_t b = a;
@@ -377,26 +397,28 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
       }
       rilm.push_back(v[a].ilm);
       rstart.push_back((int)til.size());
-      frad_in += d.h_rad_in[v[a].ilm];
-      frad_off += d.h_rad_off[v[a].ilm];
+      if (in_done) frad_in += d.h_rad_in[v[a].ilm];
+      frad_off += d.sym ? d.h_rad_low[v[a].ilm] : d.h_rad_off[v[a].ilm];
       a = b;
     }
     lstart.push_back((int)rilm.size());
     nterm += (int64_t)v.size();
+    if (in_done) nterm_in += (int64_t)v.size();
   }
   d.n_act = (int)act.size();
   d.n_runs = (int64_t)rilm.size();
   d.n_terms = nterm;
   // Gaunt-weighted density sums: 4 channels x 2 flops per term and tile entry
-  double sum_in = 0.0, sum_all = 0.0;
+  double sum_in = 0.0, sum_off = 0.0;
   for (int e = 0; e < S.Nel(); ++e) {
     sum_in += (double)S.rad.ni[e] * S.rad.ni[e];
-    for (int f = 0; f < S.Nel(); ++f) sum_all += (double)S.rad.ni[e] * S.rad.ni[f];
+    for (int f = 0; f < S.Nel(); ++f)
+      if (d.sym ? e > f : e != f) sum_off += (double)S.rad.ni[e] * S.rad.ni[f];
   }
+  d.flops_in = frad_in + 8.0 * (double)nterm_in * sum_in;
+  d.flops_off = frad_off + 8.0 * (double)nterm * sum_off;
+  d.flops_k_angular = 8.0 * ((double)nterm_in * sum_in + (double)nterm * sum_off);
   d.flops_k_radial = frad_in + frad_off;
-  d.flops_k_angular = 8.0 * (double)nterm * sum_all;
-  d.flops_in = frad_in + 8.0 * (double)nterm * sum_in;
-  d.flops_off = frad_off + 8.0 * (double)nterm * (sum_all - sum_in);
   // dense in-element path: runs grouped by (L,|M|); one workgroup per
   // (ilm, element, 32 runs), groups dealt round-robin to the 8 XCDs so the
   // workgroups sharing one operator table run on the same L2
@@ -441,13 +463,21 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.n_wg_kl = 0;
   if (!d.dense_in && d.max_rank <= KL_RANK_MAX && !rilm.empty()) {
     const int Nel = S.Nel(), nr = (int)rilm.size();
-    std::vector<int> run_act(nr), cnt(nlm + 1, 0), runs(nr);
+    std::vector<int> run_act(nr), cnt(nlm + 1, 0), runs;
+    std::vector<unsigned char> keep(nr, 1);
     for (size_t a = 0; a < act.size(); ++a)
-      for (int r = lstart[a]; r < lstart[a + 1]; ++r) run_act[r] = (int)a;
-    for (int r : rilm) ++cnt[r + 1];
+      for (int r = lstart[a]; r < lstart[a + 1]; ++r) {
+        run_act[r] = (int)a;
+        if (d.sym && act[a] / Na > act[a] % Na) keep[r] = 0;  // the mirror of block (k, j) covers it
+      }
+    for (int r = 0; r < nr; ++r)
+      if (keep[r]) ++cnt[rilm[r] + 1];
     for (int q = 0; q < nlm; ++q) cnt[q + 1] += cnt[q];
+    runs.resize(cnt[nlm]);
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (int r = 0; r < nr; ++r) runs[fill[rilm[r]]++] = r;
+    for (int r = 0; r < nr; ++r)
+      if (keep[r]) runs[fill[rilm[r]]++] = r;
+    if (runs.empty()) runs.push_back(0);
     std::vector<std::vector<int>> per_xcd(8);
     for (int q = 0; q < nlm; ++q)
       for (int e = 0; e < Nel; ++e)
@@ -497,15 +527,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
       tt[q] = x;
     }
     const double bytes = (double)src.size() * S.Nel() * S.Nel() * TILE2 * sizeof(double);
-    double budget;
-    if (const char *env = std::getenv("HFGPU_EXCHANGE_TILE_GIB")) {
-      budget = std::atof(env) * (double)(1ull << 30);
-    } else {
-      size_t free_b = 0, total_b = 0;
-      hip_check(hipMemGetInfo(&free_b, &total_b), "device memory info");
-      budget = 0.5 * (double)(free_b + d.ptile.n * sizeof(double));
-    }
-    d.tiled = bytes <= budget;
+    d.tiled = bytes <= exchange_tile_budget(d.ptile.n * sizeof(double) + d.ptile32.n * sizeof(float));
     d.n_src = (int)src.size();
     d.src_blk.upload(src, s);
     d.term_tile.upload(tt, s);
@@ -513,6 +535,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
   d.screen_mask = mask;
+  d.list_sym = d.sym;
   ++d.list_gen;
 }
 
@@ -528,7 +551,14 @@ void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, in
   diatomic_block_norms_launch(d, P, ldp, s);
   hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
             "copy block norms");
+  const bool try_sym = !d.dense_in && symmetric_exchange_allowed();
+  if (try_sym) {
+    block_asym_launch(P, ldp, d.Nrad, d.Nang, d.skip.p, d.poff.p, d.asym.p, s);
+    hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
+                             s), "copy block asymmetry");
+  }
   hip_check(hipStreamSynchronize(s), "block norms sync");
+  d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
   build_exchange_lists(h, s);
   diatomic_exchange_kernels(d, P, ldp, K, ldk, s, &h->ctx->timer, fp32);
 }

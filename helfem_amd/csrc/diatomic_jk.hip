@@ -179,6 +179,54 @@ __global__ void __launch_bounds__(NT) k_dia_j_expand(DJOutArgs A) {
 
 // --- Exchange ----------------------------------------------------------------
 
+// Symmetry test of P per shell block pair i <= l: out[2 il] = max |P_il -
+// P_li'|, out[2 il + 1] = max |P_il| (zero for i > l).  32x32 sub-tiles of
+// both blocks are read coalesced into LDS and compared transposed.  Shell a
+// spans rows poff[a] .. poff[a] + Nrad - skip[a] (a Nrad, no skip when the
+// tables are null: the atomic basis).
+__global__ void __launch_bounds__(NT) k_block_asym(const double *P, int64_t ldp, int Nrad, int Nang,
+                                                    const int *skip, const long long *poff, double *out) {
+  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
+  __shared__ double A[32][33], B[32][33];
+  __shared__ double red[8];
+  double amax = 0.0, pmax = 0.0;
+  if (i <= l) {
+    const int ri = Nrad - (skip ? skip[i] : 0), rl = Nrad - (skip ? skip[l] : 0);
+    const long long oi = poff ? poff[i] : (long long)i * Nrad, ol = poff ? poff[l] : (long long)l * Nrad;
+    const int t = threadIdx.x, x = t & 31, y0 = t >> 5;
+    for (int tr = 0; tr < ri; tr += 32)
+      for (int tc = 0; tc < rl; tc += 32) {
+        __syncthreads();
+        // A[x][y] = P_il(tr + x, tc + y), B[x][y] = P_li(tc + x, tr + y)
+        for (int y = y0; y < 32; y += NT / 32) {
+          const bool va = tr + x < ri && tc + y < rl, vb = tc + x < rl && tr + y < ri;
+          A[x][y] = va ? P[(oi + tr + x) + (ol + tc + y) * ldp] : 0.0;
+          B[x][y] = vb ? P[(ol + tc + x) + (oi + tr + y) * ldp] : 0.0;
+        }
+        __syncthreads();
+        for (int y = y0; y < 32; y += NT / 32) {
+          const double dlt = fabs(A[x][y] - B[y][x]);
+          amax = isnan(dlt) ? HUGE_VAL : fmax(amax, dlt);  // a NaN counts as asymmetric
+          pmax = fmax(pmax, fabs(A[x][y]));
+        }
+      }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    amax = fmax(amax, __shfl_xor(amax, o));
+    pmax = fmax(pmax, __shfl_xor(pmax, o));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = amax;
+    red[4 + w] = pmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[2 * il] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    out[2 * il + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+  }
+}
+
 __global__ void __launch_bounds__(NT) k_dia_block_norms(const double *P, int64_t ldp, int Nrad, int Nang,
                                                          const int *skip, const long long *poff, double *bnorm) {
   const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
@@ -384,6 +432,7 @@ struct DKArgs {
   const T *pt;         // tiled density (k_dia_ptile) or null: gather from P
   const int *ttile;    // per term: its tile in an element pair's slice
   int n_src;
+  int sym;  // symmetric density: e > f pairs (e == f: j <= k) also written to the mirror
 };
 
 // One workgroup per (active output block (j,k), element pair (e,f)), the
@@ -403,6 +452,7 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
   if (efi >= A.n_ef) return;
   const int ef = A.efl[efi], e = ef / A.Nel, f = ef - e * A.Nel;
   const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+  if (INSIDE && A.sym && j > k) return;  // the mirror of block (k, j) covers it
   const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
@@ -516,6 +566,14 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
     atomicAdd(dst, -v);
   else
     *dst = -v;
+  if (A.sym && (!INSIDE || j != k)) {
+    // mirror tile (k, j, f, e) = this tile transposed
+    double *mdst = A.K + col + row * A.ldk;
+    if (srow || scol)
+      atomicAdd(mdst, -v);
+    else
+      *mdst = -v;
+  }
 }
 
 // Cross-element launch held to 5 waves per SIMD (96 registers), which the
@@ -564,6 +622,7 @@ struct DKLArgs {
   const T *pt;         // tiled density (k_dia_ptile) or null: gather from P
   const int *ttile;
   int n_src;
+  int sym;  // symmetric density: runs of blocks j < k also written to the mirror
 };
 
 template <typename T, bool TILED>
@@ -637,7 +696,10 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
       const int a = kk[rr];
       if (a >= n || c >= n) continue;
       const long long row = pure_row(A.skip, A.poff, j, fe + a);
-      if (row >= 0 && col >= 0) atomicAdd(A.K + row + col * A.ldk, -(double)acc[rr]);
+      if (row >= 0 && col >= 0) {
+        atomicAdd(A.K + row + col * A.ldk, -(double)acc[rr]);
+        if (A.sym && j != k) atomicAdd(A.K + col + row * A.ldk, -(double)acc[rr]);
+      }
     }
   }
 }
@@ -801,9 +863,27 @@ void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64
   hip_check(hipGetLastError(), "diatomic_block_norms");
 }
 
+void block_asym_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
+                       double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_asym, dim3(Nang * Nang), dim3(NT), 0, s, P, ldp, Nrad, Nang, skip, poff, out);
+  hip_check(hipGetLastError(), "block_asym");
+}
+
+bool density_is_symmetric(const double *h_asym, int Nang) {
+  double amax = 0.0, pmax = 0.0;
+  for (int q = 0; q < Nang * Nang; ++q) {
+    amax = std::max(amax, h_asym[2 * q]);
+    pmax = std::max(pmax, h_asym[2 * q + 1]);
+  }
+  return amax <= 1e-14 * pmax;
+}
+
+bool symmetric_exchange_allowed() {
+  const char *env = std::getenv("HFGPU_EXCHANGE_SYMMETRIC");
+  return !(env && std::atoi(env) == 0);
+}
+
 namespace {
-// FP32 copy of the P/Q tiles with P kinds divided and Q kinds multiplied by
-// exp(lsig) of their (ilm, element).
 template <typename T>
 void launch_dia_k(int part, bool tiled, int64_t grid, hipStream_t s, const DKArgs<T> &a) {
   const dim3 gd((unsigned)grid), bd(NT);
@@ -817,6 +897,8 @@ void launch_dia_k(int part, bool tiled, int64_t grid, hipStream_t s, const DKArg
     hipLaunchKernelGGL((k_dia_k_in<T, false>), gd, bd, 0, s, a);
 }
 
+// FP32 copy of the P/Q tiles with P kinds divided and Q kinds multiplied by
+// exp(lsig) of their (ilm, element).
 __global__ void k_scale_pq_f32(const double *pq, const double *lsig, float *pq32, int n_x) {
   const int x = blockIdx.x;
   if (x >= n_x) return;
@@ -873,7 +955,7 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
   // in-element tiles (Cholesky factor); boundary rows/cols shared between
   // neighbouring tiles are accumulated atomically in both
   for (int part = 0; part < 2; ++part) {
-    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) : d.Nel;
+    const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) / (d.sym ? 2 : 1) : d.Nel;
     if (n_ef == 0) continue;
     const char *name = part == 0 ? (fp32 ? "diatomic_k_off_f32" : "diatomic_k_off")
                                  : (fp32 ? "diatomic_k_in_f32" : "diatomic_k_in");
@@ -896,7 +978,7 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
         DKLArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                          d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p,
                          d.term_pos.p, d.term_f32.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p,
-                         d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src};
+                         d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src, (int)d.sym};
         if (d.tiled)
           hipLaunchKernelGGL((k_dia_kin_lds<float, true>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
         else
@@ -905,7 +987,7 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
         DKLArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.chol.p,
                           d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.lrun_tstart.p, d.term_pos.p,
                           d.term_f.p, d.wg_kl.p, d.kl_runs.p, d.run_act.p,
-                          d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src};
+                          d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src, (int)d.sym};
         if (d.tiled)
           hipLaunchKernelGGL((k_dia_kin_lds<double, true>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
         else
@@ -917,19 +999,19 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;
     const int64_t grid = (int64_t)n_ef_pad * d.n_act;
     if (grid > 0x7fffffff) throw Error(-1, "exchange grid too large");
-    const int *efl = part == 0 ? d.ef_off.p : d.ef_in.p;
+    const int *efl = part == 0 ? (d.sym ? d.ef_low.p : d.ef_off.p) : d.ef_in.p;
     TimedLaunch tl(timer, name, s, flops);
     if (fp32) {
       DKArgs<float> a{d.P32.p, d.Nbf, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p,
                       d.pq32.p, d.chol32.p, d.sigma32.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p,
                       d.lrun_ilm.p, d.lrun_tstart.p, d.term_pos.p, d.term_f32.p, efl, n_ef, d.lsig.p,
-                      d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src};
+                      d.tiled ? d.ptile32.p : nullptr, d.term_tile.p, d.n_src, (int)d.sym};
       launch_dia_k<float>(part, d.tiled, grid, s, a);
     } else {
       DKArgs<double> a{P, ldp, K, ldk, d.Nel, d.Nang, d.n_act, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
                        d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.act_lstart.p, d.lrun_ilm.p,
                        d.lrun_tstart.p, d.term_pos.p, d.term_f.p, efl, n_ef, nullptr,
-                       d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src};
+                       d.tiled ? d.ptile.p : nullptr, d.term_tile.p, d.n_src, (int)d.sym};
       launch_dia_k<double>(part, d.tiled, grid, s, a);
     }
     hip_check(hipGetLastError(), "diatomic_k");

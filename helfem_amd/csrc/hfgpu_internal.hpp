@@ -84,6 +84,11 @@ struct TimedLaunch {
   }
 };
 
+// Bytes the tiled density copy of an exchange may take: the environment
+// variable HFGPU_EXCHANGE_TILE_GIB (GiB; 0 turns the tiles off), else half of
+// the free device memory plus what the handle's tiles already hold.
+double exchange_tile_budget(size_t held_bytes);
+
 // Device buffer owned by a handle.
 template <typename T>
 struct DevArray {
@@ -153,8 +158,16 @@ struct AtomicDevice {
   // that survive the reference's norm screen (TwoDBasis.cpp:949-952).
   std::vector<int> h_act, h_tstart, h_til;
   std::vector<double> h_tc, h_radL, h_radL_in;  // radial flops per coupled (block, L): all / in-element
+  std::vector<double> h_radL_low;         // ... over the element pairs e > f
   std::vector<int> h_ni;
   DevArray<int> ef_in, ef_off;            // element pairs e*Nel+f of the two exchange launches
+  DevArray<int> ef_low;                   // the pairs e > f (symmetric-density exchange)
+  // Symmetric densities (max |P - P'| <= 1e-14 max |P|) take the symmetric
+  // exchange: only tiles with e > f, or e == f and j <= k, are computed, each
+  // also written transposed to its mirror (k,j,f,e).
+  DevArray<double> asym;                  // [Nang^2][2]: max |P_il - P_li'|, max |P_il|
+  PinnedArray<double> h_asym;
+  bool sym = false, list_sym = false;     // this call's mode; the mode the lists were built for
   std::vector<unsigned char> screen_mask;  // mask the compacted lists were built for
   DevArray<int> act_jk;                   // active output blocks j*Nang+k
   DevArray<int> term_start;               // [n_act*N_L + 1]
@@ -185,6 +198,14 @@ struct AtomicDevice {
   int rs_kind = 0;
   DevArray<double> y_rs, y_rb, y_chol, e_T, rs_Lscale;
   DevArray<int> y_chol_off, y_chol_rank;
+  // Density tiles of the screened source blocks in gather order
+  // ([e*Nel+f][src][lane][4], see k_atomic_ptile), rebuilt per exchange call;
+  // used when they fit the tile budget, else the gathers read P.
+  DevArray<double> ptile;
+  DevArray<float> ptile32;
+  DevArray<int> src_blk, term_tile;
+  int n_src = 0;
+  bool tiled = false;
   // FP32 copies for the reduced-precision exchange (built on first use)
   DevArray<float> rs32, rb32, chol32, term_c32, P32;
   bool f32_tables = false;
@@ -219,6 +240,14 @@ struct DiatomicDevice {
   std::vector<unsigned char> screen_mask;
   std::vector<double> h_rad_in, h_rad_off; // radial flops per coupled (jk, ilm): in / across elements
   DevArray<int> ef_in, ef_off;            // element pairs e*Nel+f of the two exchange launches
+  DevArray<int> ef_low;                   // the pairs e > f (symmetric-density exchange)
+  // Symmetric densities (max |P - P'| <= 1e-14 max |P|, from k_dia_block_asym)
+  // take the symmetric exchange: only tiles with e > f, or e == f and j <= k,
+  // are computed, each also written transposed to its mirror (k,j,f,e).
+  DevArray<double> asym;                  // [Nang^2][2]: max |P_il - P_li'|, max |P_il|
+  PinnedArray<double> h_asym;
+  bool sym = false, list_sym = false;     // this call's mode; the mode the lists were built for
+  std::vector<double> h_rad_low;          // radial flops per coupled (jk, ilm) over the pairs e > f
   DevArray<int> act_jk, act_lstart;       // active output blocks; ilm runs per block
   DevArray<int> lrun_ilm, lrun_tstart;    // per run: ilm and first term
   DevArray<int> term_il;
@@ -317,6 +346,14 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                              hipStream_t s, KernelTimer *timer);
 void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s);
+// Per shell block pair i <= l: out[2 il] = max |P_il - P_li'|, out[2 il + 1] =
+// max |P_il| (skip/poff null: shell a is rows a*Nrad .. a*Nrad + Nrad).
+void block_asym_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
+                       double *out, hipStream_t s);
+// max |P - P'| <= 1e-14 max |P| over the block_asym output (a NaN fails)
+bool density_is_symmetric(const double *h_asym, int Nang);
+// false when the environment variable HFGPU_EXCHANGE_SYMMETRIC is 0
+bool symmetric_exchange_allowed();
 // fp32 as for atomic_exchange_launch.
 void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                                hipStream_t s, KernelTimer *timer, bool fp32 = false);

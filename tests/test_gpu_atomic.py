@@ -128,6 +128,58 @@ def test_dense_operator_inside_elements(gpu_ctx, monkeypatch):
     assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
 
 
+@pytest.mark.parametrize("name", ["Kr-l3m2-4el", "Ar-l2m1-3el"])
+def test_symmetric_and_general_density_paths(name, gpu_ctx, monkeypatch):
+    """A symmetric density (|P - P'| <= 1e-14 |P|) takes the symmetric exchange
+    (half the element pairs and blocks, mirrored); anything else the general
+    one.  Both meet the oracle bar: a non-symmetric P, a symmetric one, the
+    same with the symmetric path switched off (HFGPU_EXCHANGE_SYMMETRIC=0),
+    and a symmetric P with rounding-level asymmetry."""
+    import torch
+    print(f"[gpu] atomic K, symmetric vs general density: {name}", flush=True)
+    ob, hb = pair(name, gpu_ctx)
+    rng = np.random.default_rng(1009)
+    G = rng.standard_normal((ob.Nbf, ob.Nbf))
+    S = random_symmetric(ob.Nbf, seed=1010)
+    N = S * (1.0 + 1e-16 * rng.standard_normal(S.shape))
+    for P in (G, S, N):
+        Kg = hb.exchange(torch.as_tensor(P, device="cuda")).cpu().numpy()
+        assert relerr(Kg, ob.exchange(P)) <= TOL
+    monkeypatch.setenv("HFGPU_EXCHANGE_SYMMETRIC", "0")
+    Kg = hb.exchange(torch.as_tensor(S, device="cuda")).cpu().numpy()
+    assert relerr(Kg, ob.exchange(S)) <= TOL
+    monkeypatch.delenv("HFGPU_EXCHANGE_SYMMETRIC")
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name", ["Kr-l3m2-4el", "He-l1-1el"])
+def test_direct_gather_without_tile_budget(name, gpu_ctx, monkeypatch):
+    """With no memory granted for the tiled density (HFGPU_EXCHANGE_TILE_GIB=0)
+    the exchange gathers straight from P; FP64 still meets the oracle bar and
+    FP32 agrees with the tiled FP32 path to rounding."""
+    import torch
+    print(f"[gpu] atomic K, direct density gathers: {name}", flush=True)
+    cfg = CASES[name]
+    ob = O.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"])
+    ob.compute_tei()
+    P = random_symmetric(ob.Nbf, seed=1006)
+    Pd = torch.as_tensor(P, device="cuda")
+    out = {}
+    for budget in ("0", None):
+        if budget is None:
+            monkeypatch.delenv("HFGPU_EXCHANGE_TILE_GIB", raising=False)
+        else:
+            monkeypatch.setenv("HFGPU_EXCHANGE_TILE_GIB", budget)
+        hb = H.AtomicBasis(cfg["Z"], cfg["lmax"], cfg["mmax"], cfg["nelem"], ctx=gpu_ctx)
+        hb.compute_tei()
+        out[budget] = (hb.exchange(Pd).cpu().numpy(), hb.exchange(Pd, precision="fp32").cpu().numpy())
+    torch.cuda.synchronize()
+    Kr = ob.exchange(P)
+    assert relerr(out["0"][0], Kr) <= TOL
+    assert relerr(out[None][0], Kr) <= TOL
+    assert relerr(out["0"][1], out[None][1]) <= 1e-6
+
+
 def test_gpu_uhf_hydrogen(gpu_ctx):
     """atomic-H-hf-u: one alpha electron; the exchange cancels the self-Coulomb
     exactly, E = -0.5 Eh to the basis limit (tests/refs/ci.json)."""

@@ -144,6 +144,30 @@ def test_dense_operator_inside_elements(gpu_ctx, monkeypatch):
     assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
 
 
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_symmetric_and_general_density_paths(name, gpu_ctx, monkeypatch):
+    """A symmetric density (|P - P'| <= 1e-14 |P|) takes the symmetric exchange
+    (half the element pairs and blocks, mirrored); anything else the general
+    one.  Both meet the oracle bar: a non-symmetric P, a symmetric one, the
+    same with the symmetric path switched off (HFGPU_EXCHANGE_SYMMETRIC=0),
+    and a symmetric P with rounding-level asymmetry."""
+    import torch
+    print(f"[gpu] diatomic K, symmetric vs general density: {name}", flush=True)
+    ob, hb = pair(name, gpu_ctx)
+    rng = np.random.default_rng(1007)
+    G = rng.standard_normal((ob.Nbf, ob.Nbf))
+    S = random_symmetric(ob.Nbf, seed=1008)
+    N = S * (1.0 + 1e-16 * rng.standard_normal(S.shape))
+    for P in (G, S, N):
+        Kg = hb.exchange(torch.as_tensor(P, device="cuda")).cpu().numpy()
+        assert relerr(Kg, ob.exchange(P)) <= TOL
+    monkeypatch.setenv("HFGPU_EXCHANGE_SYMMETRIC", "0")
+    Kg = hb.exchange(torch.as_tensor(S, device="cuda")).cpu().numpy()
+    assert relerr(Kg, ob.exchange(S)) <= TOL
+    monkeypatch.delenv("HFGPU_EXCHANGE_SYMMETRIC")
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("name", ["N2-l432-3el", "HeH-l2-1el"])
 def test_direct_gather_without_tile_budget(name, gpu_ctx, monkeypatch):
     """With no memory granted for the tiled density (HFGPU_EXCHANGE_TILE_GIB=0)

@@ -51,6 +51,21 @@ def test_rs_exchange_matches_oracle(name, kind, gpu_ctx):
     assert relerr(hb.exchange(P), ob.exchange(P)) <= TOL
 
 
+@pytest.mark.parametrize("kind", ["yukawa", "erfc"])
+def test_rs_exchange_general_density(kind, gpu_ctx):
+    """A non-symmetric density takes the general (all element pairs) path."""
+    print(f"[gpu] range-separated exchange {kind}, non-symmetric P", flush=True)
+    c = CASES["Ne-l2-3el"]
+    ob = O.AtomicBasis(c["Z"], c["lmax"], c["mmax"], c["nelem"], nquad=c["nquad"])
+    hb = H.AtomicBasis(c["Z"], c["lmax"], c["mmax"], c["nelem"], nquad=c["nquad"], ctx=gpu_ctx)
+    ob.compute_tei()
+    hb.compute_tei()
+    getattr(ob, "compute_" + kind)(0.4)
+    getattr(hb, "compute_" + kind)(0.4)
+    P = np.random.default_rng(1502).standard_normal((ob.Nbf, ob.Nbf))
+    assert relerr(hb.rs_exchange(P), ob.rs_exchange(P)) <= TOL
+
+
 def test_rs_exchange_hydrogen_closed_form(gpu_ctx):
     """Yukawa self-exchange of the hydrogen 1s density through the GPU path."""
     from helfem_amd.scf import uhf
