@@ -160,20 +160,6 @@ __global__ void __launch_bounds__(NT) k_atomic_j_expand(JOutArgs A) {
 
 // --- Exchange ----------------------------------------------------------------
 
-__global__ void __launch_bounds__(NT) k_block_norms(const double *P, int64_t ldp, int Nrad, int Nang,
-                                                     double *bnorm) {
-  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
-  __shared__ double red[8];
-  double s = 0.0, dummy = 0.0;
-  const int64_t n = (int64_t)Nrad * Nrad;
-  for (int64_t q = threadIdx.x; q < n; q += NT) {
-    const int64_t c = q / Nrad, r = q - c * Nrad;
-    const double v = P[(int64_t)(i * Nrad) + r + ((int64_t)l * Nrad + c) * ldp];
-    s += v * v;
-  }
-  block_sum2(s, dummy, red);
-  if (threadIdx.x == 0) bnorm[il] = sqrt(s);
-}
 
 template <typename T>
 struct KArgs {
@@ -885,16 +871,13 @@ static bool exchange_prologue(AtomicDevice &d, const double *P, int64_t ldp, dou
     fill_zero_launch(K, nbf, nbf, ldk, s);
     return false;
   }
-  hipLaunchKernelGGL(k_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.bnorm.p);
-  hip_check(hipGetLastError(), "block_norms");
+  block_stats_launch(P, ldp, d.Nrad, d.Nang, nullptr, nullptr, d.bnorm.p, d.asym.p, s);
   hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
             "copy block norms");
   const bool try_sym = allow_sym && !d.dense_in && symmetric_exchange_allowed();
-  if (try_sym) {
-    block_asym_launch(P, ldp, d.Nrad, d.Nang, nullptr, nullptr, d.asym.p, s);
+  if (try_sym)
     hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
                              s), "copy block asymmetry");
-  }
   hip_check(hipStreamSynchronize(s), "block norms sync");
   d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
   compact_exchange_lists(d, s);

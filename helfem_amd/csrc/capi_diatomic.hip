@@ -548,15 +548,13 @@ void ensure_ready(hf_diatomic *h) {
 void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk, bool fp32 = false) {
   DiatomicDevice &d = h->dev;
   hipStream_t s = h->ctx->cur;
-  diatomic_block_norms_launch(d, P, ldp, s);
+  block_stats_launch(P, ldp, d.Nrad, d.Nang, d.skip.p, d.poff.p, d.bnorm.p, d.asym.p, s);
   hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
             "copy block norms");
   const bool try_sym = !d.dense_in && symmetric_exchange_allowed();
-  if (try_sym) {
-    block_asym_launch(P, ldp, d.Nrad, d.Nang, d.skip.p, d.poff.p, d.asym.p, s);
+  if (try_sym)
     hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
                              s), "copy block asymmetry");
-  }
   hip_check(hipStreamSynchronize(s), "block norms sync");
   d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
   build_exchange_lists(h, s);

@@ -179,69 +179,63 @@ __global__ void __launch_bounds__(NT) k_dia_j_expand(DJOutArgs A) {
 
 // --- Exchange ----------------------------------------------------------------
 
-// Symmetry test of P per shell block pair i <= l: out[2 il] = max |P_il -
-// P_li'|, out[2 il + 1] = max |P_il| (zero for i > l).  32x32 sub-tiles of
-// both blocks are read coalesced into LDS and compared transposed.  Shell a
-// spans rows poff[a] .. poff[a] + Nrad - skip[a] (a Nrad, no skip when the
-// tables are null: the atomic basis).
-__global__ void __launch_bounds__(NT) k_block_asym(const double *P, int64_t ldp, int Nrad, int Nang,
-                                                    const int *skip, const long long *poff, double *out) {
+// One pass over P per shell block pair i <= l: the Frobenius norms of P_il
+// and P_li (the reference's screen) into bnorm, and the symmetry test
+// asym[2 il] = max |P_il - P_li'|, asym[2 il + 1] = max |P_il|.  32x32
+// sub-tiles of both blocks are read coalesced into LDS and compared
+// transposed.  Shell a spans rows poff[a] .. poff[a] + Nrad - skip[a] (a Nrad,
+// no skip when the tables are null: the atomic basis).
+__global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp, int Nrad, int Nang,
+                                                    const int *skip, const long long *poff, double *bnorm,
+                                                    double *asym) {
   const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
+  if (i > l) return;  // covered by the workgroup of (l, i)
   __shared__ double A[32][33], B[32][33];
-  __shared__ double red[8];
-  double amax = 0.0, pmax = 0.0;
-  if (i <= l) {
-    const int ri = Nrad - (skip ? skip[i] : 0), rl = Nrad - (skip ? skip[l] : 0);
-    const long long oi = poff ? poff[i] : (long long)i * Nrad, ol = poff ? poff[l] : (long long)l * Nrad;
-    const int t = threadIdx.x, x = t & 31, y0 = t >> 5;
-    for (int tr = 0; tr < ri; tr += 32)
-      for (int tc = 0; tc < rl; tc += 32) {
-        __syncthreads();
-        // A[x][y] = P_il(tr + x, tc + y), B[x][y] = P_li(tc + x, tr + y)
-        for (int y = y0; y < 32; y += NT / 32) {
-          const bool va = tr + x < ri && tc + y < rl, vb = tc + x < rl && tr + y < ri;
-          A[x][y] = va ? P[(oi + tr + x) + (ol + tc + y) * ldp] : 0.0;
-          B[x][y] = vb ? P[(ol + tc + x) + (oi + tr + y) * ldp] : 0.0;
-        }
-        __syncthreads();
-        for (int y = y0; y < 32; y += NT / 32) {
-          const double dlt = fabs(A[x][y] - B[y][x]);
-          amax = isnan(dlt) ? HUGE_VAL : fmax(amax, dlt);  // a NaN counts as asymmetric
-          pmax = fmax(pmax, fabs(A[x][y]));
-        }
+  __shared__ double red[16];
+  double amax = 0.0, pmax = 0.0, ssa = 0.0, ssb = 0.0;
+  const int ri = Nrad - (skip ? skip[i] : 0), rl = Nrad - (skip ? skip[l] : 0);
+  const long long oi = poff ? poff[i] : (long long)i * Nrad, ol = poff ? poff[l] : (long long)l * Nrad;
+  const int t = threadIdx.x, x = t & 31, y0 = t >> 5;
+  for (int tr = 0; tr < ri; tr += 32)
+    for (int tc = 0; tc < rl; tc += 32) {
+      __syncthreads();
+      // A[x][y] = P_il(tr + x, tc + y), B[x][y] = P_li(tc + x, tr + y)
+      for (int y = y0; y < 32; y += NT / 32) {
+        const bool va = tr + x < ri && tc + y < rl, vb = tc + x < rl && tr + y < ri;
+        const double pa = va ? P[(oi + tr + x) + (ol + tc + y) * ldp] : 0.0;
+        const double pb = vb ? P[(ol + tc + x) + (oi + tr + y) * ldp] : 0.0;
+        A[x][y] = pa;
+        B[x][y] = pb;
+        ssa += pa * pa;
+        ssb += pb * pb;
       }
-  }
+      __syncthreads();
+      for (int y = y0; y < 32; y += NT / 32) {
+        const double dlt = fabs(A[x][y] - B[y][x]);
+        amax = isnan(dlt) ? HUGE_VAL : fmax(amax, dlt);  // a NaN counts as asymmetric
+        pmax = fmax(pmax, fabs(A[x][y]));
+      }
+    }
   for (int o = 32; o > 0; o >>= 1) {
     amax = fmax(amax, __shfl_xor(amax, o));
     pmax = fmax(pmax, __shfl_xor(pmax, o));
+    ssa += __shfl_xor(ssa, o);
+    ssb += __shfl_xor(ssb, o);
   }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
+  const int w = t >> 6;
+  if ((t & 63) == 0) {
     red[w] = amax;
     red[4 + w] = pmax;
+    red[8 + w] = ssa;
+    red[12 + w] = ssb;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    out[2 * il] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    out[2 * il + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+  if (t == 0) {
+    asym[2 * il] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    asym[2 * il + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    bnorm[il] = sqrt((red[8] + red[9]) + (red[10] + red[11]));
+    if (i != l) bnorm[l * Nang + i] = sqrt((red[12] + red[13]) + (red[14] + red[15]));
   }
-}
-
-__global__ void __launch_bounds__(NT) k_dia_block_norms(const double *P, int64_t ldp, int Nrad, int Nang,
-                                                         const int *skip, const long long *poff, double *bnorm) {
-  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
-  __shared__ double red[8];
-  const int ri = Nrad - skip[i], rl = Nrad - skip[l];
-  const long long r0 = poff[i], c0 = poff[l];
-  double s = 0.0, dummy = 0.0;
-  const int64_t n = (int64_t)ri * rl;
-  for (int64_t q = threadIdx.x; q < n; q += NT) {
-    const int64_t c = q / ri, r = q - c * ri;
-    const double v = P[(r0 + r) + (c0 + c) * ldp];
-    s += v * v;
-  }
-  block_sum2(s, dummy, red);
-  if (threadIdx.x == 0) bnorm[il] = sqrt(s);
 }
 
 // Two products sharing the Y operand, issued interleaved so the two
@@ -529,13 +523,17 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
           load_chol<T>(M, p + 1, kk, c, ny0, ny2, nx0, nx2);
           nsg = A.sigma[off + p + 1];
         }
-        // ya = R00 M0' - R02 M2', yb = -R20 M0' + R22 M2' (signs folded into r02, r20)
+        // ya = R00 M0' - R02 M2', yb = -R20 M0' + R22 M2' (signs folded into
+        // r02, r20); sigma_p = +-1 goes on the X operands, off the MFMA chain
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          x0[s] *= sg;
+          x2[s] *= sg;
+        }
         acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
         mma_pair<T>(r00, r20, y0, ya, yb);
         mma_pair<T>(r02, r22, y2, ya, yb);
-        ya *= sg;
-        yb *= sg;
-        mma_pair2<T>(x0, ya, acc, x2, yb, acc2);  // acc += M0' ya, acc2 += M2' yb
+        mma_pair2<T>(x0, ya, acc, x2, yb, acc2);  // acc += sigma M0' ya, acc2 += sigma M2' yb
         if (p + 1 < rank) {
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
@@ -670,22 +668,23 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
       r20[s] = -r20[s];
     }
     acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+    // sigma_p = +-1 is applied to the X operands (exact), so the products
+    // of consecutive p are independent chains the scheduler can interleave
+#pragma unroll 2
     for (int p = 0; p < rank; ++p) {
       const T *M0 = Mt + 2 * p * TL, *M2 = M0 + TL;
+      const T sp = sg[p];
       T y0[4], y2[4], x0[4], x2[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         y0[s] = M0[c + KL_LD * kk[s]];
         y2[s] = M2[c + KL_LD * kk[s]];
-        x0[s] = M0[kk[s] + KL_LD * c];
-        x2[s] = M2[kk[s] + KL_LD * c];
+        x0[s] = sp * M0[kk[s] + KL_LD * c];
+        x2[s] = sp * M2[kk[s] + KL_LD * c];
       }
       acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
       mma_pair<T>(r00, r20, y0, ya, yb);
       mma_pair<T>(r02, r22, y2, ya, yb);
-      const T sp = sg[p];
-      ya *= sp;
-      yb *= sp;
       mma_pair2<T>(x0, ya, acc, x2, yb, acc2);
     }
     acc += acc2;
@@ -857,24 +856,20 @@ void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t l
   }
 }
 
-void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s) {
-  hipLaunchKernelGGL(k_dia_block_norms, dim3(d.Nang * d.Nang), dim3(NT), 0, s, P, ldp, d.Nrad, d.Nang, d.skip.p,
-                     d.poff.p, d.bnorm.p);
-  hip_check(hipGetLastError(), "diatomic_block_norms");
-}
-
-void block_asym_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
-                       double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_block_asym, dim3(Nang * Nang), dim3(NT), 0, s, P, ldp, Nrad, Nang, skip, poff, out);
-  hip_check(hipGetLastError(), "block_asym");
+void block_stats_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
+                        double *bnorm, double *asym, hipStream_t s) {
+  hipLaunchKernelGGL(k_block_stats, dim3(Nang * Nang), dim3(NT), 0, s, P, ldp, Nrad, Nang, skip, poff, bnorm, asym);
+  hip_check(hipGetLastError(), "block_stats");
 }
 
 bool density_is_symmetric(const double *h_asym, int Nang) {
   double amax = 0.0, pmax = 0.0;
-  for (int q = 0; q < Nang * Nang; ++q) {
-    amax = std::max(amax, h_asym[2 * q]);
-    pmax = std::max(pmax, h_asym[2 * q + 1]);
-  }
+  for (int i = 0; i < Nang; ++i)
+    for (int l = i; l < Nang; ++l) {  // only the pairs i <= l are written
+      const int q = i * Nang + l;
+      amax = std::max(amax, h_asym[2 * q]);
+      pmax = std::max(pmax, h_asym[2 * q + 1]);
+    }
   return amax <= 1e-14 * pmax;
 }
 

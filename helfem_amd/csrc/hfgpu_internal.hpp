@@ -241,7 +241,7 @@ struct DiatomicDevice {
   std::vector<double> h_rad_in, h_rad_off; // radial flops per coupled (jk, ilm): in / across elements
   DevArray<int> ef_in, ef_off;            // element pairs e*Nel+f of the two exchange launches
   DevArray<int> ef_low;                   // the pairs e > f (symmetric-density exchange)
-  // Symmetric densities (max |P - P'| <= 1e-14 max |P|, from k_dia_block_asym)
+  // Symmetric densities (max |P - P'| <= 1e-14 max |P|, from k_block_stats)
   // take the symmetric exchange: only tiles with e > f, or e == f and j <= k,
   // are computed, each also written transposed to its mirror (k,j,f,e).
   DevArray<double> asym;                  // [Nang^2][2]: max |P_il - P_li'|, max |P_il|
@@ -345,12 +345,12 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                              hipStream_t s, KernelTimer *timer);
-void diatomic_block_norms_launch(const DiatomicDevice &d, const double *P, int64_t ldp, hipStream_t s);
-// Per shell block pair i <= l: out[2 il] = max |P_il - P_li'|, out[2 il + 1] =
+// One pass over P: Frobenius norm of every shell block into bnorm[Nang^2],
+// and per block pair i <= l asym[2 il] = max |P_il - P_li'|, asym[2 il + 1] =
 // max |P_il| (skip/poff null: shell a is rows a*Nrad .. a*Nrad + Nrad).
-void block_asym_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
-                       double *out, hipStream_t s);
-// max |P - P'| <= 1e-14 max |P| over the block_asym output (a NaN fails)
+void block_stats_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
+                        double *bnorm, double *asym, hipStream_t s);
+// max |P - P'| <= 1e-14 max |P| over the block_stats asym output (a NaN fails)
 bool density_is_symmetric(const double *h_asym, int Nang);
 // false when the environment variable HFGPU_EXCHANGE_SYMMETRIC is 0
 bool symmetric_exchange_allowed();
