@@ -268,7 +268,8 @@ def main():
             scf[prec] = {"solves_per_s": 1.0 / tsolve, "seconds": tsolve, "iterations": r.iterations,
                          "iterations_fp32_exchange": r.iterations_fp32, "energy": r.E,
                          "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
-                         "eigensolves": r.eigensolves, "reused_eigensolves": r.reused_solves}
+                         "eigensolves": r.eigensolves, "reused_eigensolves": r.reused_solves,
+                         "skipped_eigensolves": r.skipped_solves}
         scf["mixed_vs_fp64"] = {"speedup": scf["mixed"]["solves_per_s"] / scf["fp64"]["solves_per_s"],
                                 "energy_difference_Eh": scf["mixed"]["energy"] - scf["fp64"]["energy"]}
         scf["convergence"] = "dE < 1e-10 Eh and orbital gradient < 1e-6, core-Hamiltonian guess"

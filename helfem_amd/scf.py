@@ -82,9 +82,17 @@ class _Blocks:
         once.
     "No coupling" and "the same" are judged at 1e-12 of the block's largest
     element, so an open shell that breaks a symmetry falls back to the full
-    eigensolve of its block."""
+    eigensolve of its block.
+
+    A third, also exact: a sub-problem solved before whose lowest eigenvalue,
+    lowered by the Frobenius norm of the change of its matrix since (Weyl's
+    bound), stays above the previous highest occupied level by GAP is not
+    solved: it holds no occupied orbital.  After the aufbau over the solved
+    ones the bound is checked against the new highest occupied level, and any
+    sub-problem that fails it is solved after all."""
 
     TOL = 1e-12
+    GAP = 0.05
 
     def __init__(self, basis, S, symm, dev):
         import torch
@@ -117,6 +125,9 @@ class _Blocks:
         self.Scpl = [self._coupling(Sk, k) for k, Sk in enumerate(self.S)]
         self.solves = 0
         self.paired_solves = 0
+        self.skipped_solves = 0
+        self.memo = {}  # (slot, block, shells) -> (matrix last solved, its lowest eigenvalue)
+        self.homo = {}  # slot -> highest occupied level of the last aufbau
 
     def _coupling(self, M, k):
         """(ns x ns) host matrix of max |M_st| over shell pairs (None when the
@@ -200,19 +211,35 @@ class _Blocks:
         return self.torch.cat([self.torch.arange(sh[s_][1], sh[s_][1] + sh[s_][2], device=self.dev)
                                for s_ in comp])
 
-    def prepare(self, Fb):
+    def prepare(self, Fb, slot=0):
         """Independent eigenproblems X' F X of this Fock matrix (one per
-        distinct shell group), with the bookkeeping `finish` needs."""
+        distinct shell group and not excluded by the Weyl bound), with the
+        bookkeeping `finish` needs.  `slot` names the spin channel."""
+        torch = self.torch
         subs, loc, reuse = self._subproblems(Fb)
         uniq = [i for i in range(len(subs)) if reuse[i] == i]
-        X, mats = {}, []
+        X, mats = {}, {}
         for i in uniq:
             k, li = subs[i][0], loc[i]
             X[i] = self.X[k][li][:, li]
-            mats.append(X[i].T @ Fb[k][li][:, li] @ X[i])
-        self.solves += len(uniq)
+            mats[i] = X[i].T @ Fb[k][li][:, li] @ X[i]
+        keys = {i: (slot, subs[i][0], tuple(subs[i][1])) for i in uniq}
+        skipped = {}
+        homo = self.homo.get(slot)
+        cand = [i for i in uniq if homo is not None and keys[i] in self.memo
+                and self.memo[keys[i]][0].shape == mats[i].shape]
+        if cand:
+            dn = torch.stack([torch.linalg.matrix_norm(mats[i] - self.memo[keys[i]][0]) for i in cand]).cpu().numpy()
+            for i, d in zip(cand, dn):
+                lb = self.memo[keys[i]][1] - float(d)
+                if lb > homo + self.GAP:
+                    skipped[i] = lb
+        solve = [i for i in uniq if i not in skipped]
+        self.solves += len(solve)
+        self.skipped_solves += len(skipped)
         self.paired_solves += len(subs) - len(uniq)
-        return {"subs": subs, "loc": loc, "reuse": reuse, "uniq": uniq, "X": X, "mats": mats}
+        return {"subs": subs, "loc": loc, "reuse": reuse, "uniq": solve, "X": X, "mats": [mats[i] for i in solve],
+                "all": mats, "keys": keys, "skipped": skipped, "slot": slot}
 
     def finish(self, plan, sols, nocc, weight, like):
         """Aufbau over all sub-problems from their eigenpairs (`sols`, in the
@@ -220,10 +247,28 @@ class _Blocks:
         blocks and the occupied orbitals per block."""
         subs, loc, reuse = plan["subs"], plan["loc"], plan["reuse"]
         ev, vecs = {}, {}
-        for i, (e, v) in zip(plan["uniq"], sols):
-            ev[i] = e
-            vecs[i] = plan["X"][i] @ v
-        order = sorted((float(e), i, j) for i in range(len(subs)) for j, e in enumerate(ev[reuse[i]]))
+
+        def take(idx, solutions):
+            for i, (e, v) in zip(idx, solutions):
+                ev[i] = e
+                vecs[i] = plan["X"][i] @ v
+                self.memo[plan["keys"][i]] = (plan["all"][i], float(e[0]))
+
+        def aufbau():
+            return sorted((float(e), i, j) for i in range(len(subs)) if reuse[i] in ev
+                          for j, e in enumerate(ev[reuse[i]]))
+
+        take(plan["uniq"], sols)
+        order = aufbau()
+        homo = order[nocc - 1][0] if nocc > 0 else -np.inf
+        late = [i for i, lb in plan["skipped"].items() if not lb > homo]
+        if late:  # the occupied levels rose past a bound: solve those after all
+            self.solves += len(late)
+            self.skipped_solves -= len(late)
+            take(late, batched_eigh([plan["all"][i] for i in late]))
+            order = aufbau()
+            homo = order[nocc - 1][0] if nocc > 0 else -np.inf
+        self.homo[plan["slot"]] = homo
         nsub = [0] * len(subs)
         for _, i, _j in order[:nocc]:
             nsub[i] += 1
@@ -242,8 +287,8 @@ class _Blocks:
                 P[b[:, None], b[None, :]] = Pb[k]
         return P, Pb, nk
 
-    def density(self, Fb, nocc, weight, like):
-        plan = self.prepare(Fb)
+    def density(self, Fb, nocc, weight, like, slot=0):
+        plan = self.prepare(Fb, slot)
         return self.finish(plan, batched_eigh(plan["mats"]), nocc, weight, like)
 
     def commutator(self, Fb, Pb):
@@ -450,6 +495,7 @@ class SCFState:
                         Exx=float(t["Exx"]), Exc=float(t["Exc"]), Enucr=self.Enucr, iterations=self.it,
                         iterations_fp32=self.it_fp32, seconds=time.perf_counter() - self.t0, history=self.hist,
                         phases=self.clk.t, eigensolves=self.bs.solves, reused_solves=self.bs.paired_solves,
+                        skipped_solves=self.bs.skipped_solves,
                         tag=self.tag)
         if self.restricted:
             out.update(P=self.dens[0][0], occupations=self.dens[0][2])
@@ -480,7 +526,7 @@ class SCFEngine:
         reqs = []
         for st in self.active:
             for s, Fb, nocc in st.requests():
-                reqs.append((st, s, st.bs.prepare(Fb), nocc))
+                reqs.append((st, s, st.bs.prepare(Fb, s), nocc))
         sols = batched_eigh([m for r in reqs for m in r[2]["mats"]])
         off = 0
         for st, s, plan, nocc in reqs:

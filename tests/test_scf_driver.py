@@ -92,3 +92,29 @@ def test_uhf_hydrogen_on_cpu():
     r = uhf(OracleFock(1, 0, 5), 1, 0, device="cpu", conv=1e-12)
     assert abs(r.E - REF["atomic-H-hf-u"]["total"]) < 1e-9
     assert abs(r.Ecoul + r.Exx) < 1e-10
+
+
+def test_weyl_skip_of_unoccupied_blocks(monkeypatch):
+    """Ne at lmax 2: the d blocks hold no occupied orbital, so once their
+    lowest level minus the change of their matrix stays above the highest
+    occupied level they are not diagonalised; the energy equals the SCF that
+    solves every block (GAP = inf switches the bound off), and the bound is
+    not trusted blindly: UHF with an empty beta channel skips everything of
+    it after the first iteration and still reproduces the full solve."""
+    from helfem_amd import scf
+    from helfem_amd.scf import rhf, uhf
+    ne = OracleFock(10, 2, 4)
+    r = rhf(ne, 10, device="cpu", conv=1e-11)
+    assert r.skipped_solves > 0
+    monkeypatch.setattr(scf._Blocks, "GAP", np.inf)
+    full = rhf(ne, 10, device="cpu", conv=1e-11)
+    assert full.skipped_solves == 0
+    assert abs(r.E - full.E) < 1e-10
+    assert r.iterations == full.iterations
+    monkeypatch.setattr(scf._Blocks, "GAP", 0.05)
+    h = OracleFock(1, 1, 4)
+    u = uhf(h, 1, 0, device="cpu", conv=1e-12)
+    monkeypatch.setattr(scf._Blocks, "GAP", np.inf)
+    uf = uhf(h, 1, 0, device="cpu", conv=1e-12)
+    assert u.skipped_solves > 0 and uf.skipped_solves == 0
+    assert abs(u.E - uf.E) < 1e-10
