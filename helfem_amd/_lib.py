@@ -39,6 +39,7 @@ SIGNATURES = {
     "hf_atomic_commit_tei": (c_int, [c_vp]),
     "hf_atomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),
     "hf_atomic_one_electron": (c_int, [c_vp, c_int, c_dp, c_i64]),
+    "hf_atomic_one_electron_dev": (c_int, [c_vp, c_int, c_dp, c_i64]),
     "hf_atomic_coulomb": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
@@ -68,6 +69,7 @@ SIGNATURES = {
     "hf_diatomic_compute_tei": (c_int, [c_vp, c_int]),
     "hf_diatomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),
     "hf_diatomic_one_electron": (c_int, [c_vp, c_int, c_dp, c_i64]),
+    "hf_diatomic_one_electron_dev": (c_int, [c_vp, c_int, c_dp, c_i64]),
     "hf_diatomic_work": (c_int, [c_vp, c_vp]),
     "hf_diatomic_coulomb": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_diatomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),

@@ -675,6 +675,40 @@ void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStr
   hip_check(hipGetLastError(), "mat_to_f32");
 }
 
+namespace {
+// One workgroup per block: out(row0 + r, col0 + c) = vals[off + r + c rows].
+__global__ void __launch_bounds__(NT) k_scatter_blocks(double *out, int64_t ld, const long long *meta,
+                                                       const double *vals) {
+  const long long *m = meta + 5 * (size_t)blockIdx.x;
+  const long long rows = m[2], n = m[2] * m[3];
+  const double *v = vals + m[4];
+  for (long long q = threadIdx.x; q < n; q += NT) {
+    const long long c = q / rows, r = q - c * rows;
+    out[(m[0] + r) + (m[1] + c) * ld] = v[q];
+  }
+}
+}  // namespace
+
+void scatter_blocks_launch(double *out, int64_t ld, int64_t n, const std::vector<long long> &meta,
+                           const std::vector<double> &vals, hipStream_t s) {
+  fill_zero_launch(out, n, n, ld, s);
+  const size_t nblk = meta.size() / 5;
+  if (nblk == 0) return;
+  for (size_t b = 0; b < nblk; ++b) {
+    const long long *m = &meta[5 * b];
+    if (m[0] < 0 || m[1] < 0 || m[0] + m[2] > n || m[1] + m[3] > n || m[4] + m[2] * m[3] > (long long)vals.size())
+      throw Error(-1, "one-electron block outside the matrix");
+  }
+  if (nblk > 0x7fffffff) throw Error(-1, "too many one-electron blocks");
+  DevArray<long long> dm;
+  DevArray<double> dv;
+  dm.upload(meta, s);
+  dv.upload(vals, s);
+  hipLaunchKernelGGL(k_scatter_blocks, dim3((unsigned)nblk), dim3(NT), 0, s, out, ld, dm.p, dv.p);
+  hip_check(hipGetLastError(), "scatter_blocks");
+  hip_check(hipStreamSynchronize(s), "scatter_blocks sync");  // before the staging buffers go
+}
+
 double exchange_tile_budget(size_t held_bytes) {
   if (const char *env = std::getenv("HFGPU_EXCHANGE_TILE_GIB")) return std::atof(env) * (double)(1ull << 30);
   size_t free_b = 0, total_b = 0;

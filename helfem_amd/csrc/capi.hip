@@ -532,6 +532,19 @@ int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t l
   });
 }
 
+int hf_atomic_one_electron_dev(const hf_atomic *h, int which, double *dOut, int64_t ld) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    need(which >= 0 && which <= 2, "which must be 0 (overlap), 1 (kinetic) or 2 (nuclear)");
+    DeviceGuard g(h->ctx->device);
+    check_matrix(dOut, ld, h->host.Nbf(), "out");
+    BlockList b;
+    h->host.one_electron_blocks(which, b);
+    scatter_blocks_launch(dOut, ld, h->host.Nbf(), b.meta, b.vals, h->ctx->cur);
+  });
+}
+
 int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj) {
   return guarded([&] {
     need(h != nullptr, "NULL handle");

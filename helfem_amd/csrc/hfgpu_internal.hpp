@@ -89,6 +89,11 @@ struct TimedLaunch {
 // the free device memory plus what the handle's tiles already hold.
 double exchange_tile_budget(size_t held_bytes);
 
+// n x n device matrix from its nonzero blocks (meta: row0, col0, rows, cols,
+// offset per block; vals column-major), zero elsewhere; synchronises s.
+void scatter_blocks_launch(double *out, int64_t ld, int64_t n, const std::vector<long long> &meta,
+                           const std::vector<double> &vals, hipStream_t s);
+
 // Device buffer owned by a handle.
 template <typename T>
 struct DevArray {

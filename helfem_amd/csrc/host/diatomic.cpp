@@ -525,20 +525,30 @@ void DiatomicSetup::compute_tei() {
 // radial functions that survive remove_boundaries.
 namespace {
 template <class F>
-void fill_pure(const DiatomicSetup &s, double *out, int64_t ld, F block_value) {
+void collect_pure(const DiatomicSetup &s, BlockList &out, F block_value) {
   const int N = s.Nrad(), na = s.Nang();
-  const int64_t nbf = s.Nbf();
-#pragma omp parallel for schedule(static)
-  for (int64_t j = 0; j < nbf; ++j)
-    for (int64_t i = 0; i < nbf; ++i) out[i + j * ld] = 0.0;
-#pragma omp parallel for schedule(dynamic)
+  std::vector<std::pair<int, int>> pairs;
   for (int j = 0; j < na; ++j)
-    for (int i = 0; i < na; ++i) {
-      if (s.mval[i] != s.mval[j]) continue;
-      const int si = s.skip(i), sj = s.skip(j);
-      for (int b = sj; b < N; ++b)
-        for (int a = si; a < N; ++a)
-          out[(s.poff[i] + a - si) + (s.poff[j] + b - sj) * ld] = block_value(i, j, a, b);
+    for (int i = 0; i < na; ++i)
+      if (s.mval[i] == s.mval[j]) pairs.emplace_back(i, j);
+  std::vector<std::vector<double>> blk(pairs.size());
+#pragma omp parallel for schedule(dynamic)
+  for (size_t p = 0; p < pairs.size(); ++p) {
+    const int i = pairs[p].first, j = pairs[p].second, si = s.skip(i), sj = s.skip(j);
+    std::vector<double> v((size_t)(N - si) * (N - sj));
+    bool any = false;
+    for (int b = sj; b < N; ++b)
+      for (int a = si; a < N; ++a) {
+        const double x = block_value(i, j, a, b);
+        v[(size_t)(a - si) + (size_t)(b - sj) * (N - si)] = x;
+        any = any || x != 0.0;
+      }
+    if (any) blk[p] = std::move(v);
+  }
+  for (size_t p = 0; p < pairs.size(); ++p)
+    if (!blk[p].empty()) {
+      const int i = pairs[p].first, j = pairs[p].second;
+      out.add(s.poff[i], s.poff[j], N - s.skip(i), N - s.skip(j), blk[p].data());
     }
 }
 
@@ -553,47 +563,65 @@ std::vector<double> pair_table(const DiatomicSetup &s, G coef) {
 }
 }  // namespace
 
+void DiatomicSetup::one_electron_blocks(int which, BlockList &out) const {
+  if (which == 0) {
+    const Blk I10 = radial_integral(1, 0), I12 = radial_integral(1, 2);
+    const double f = std::pow(Rh, 3);
+    const std::vector<double> cc =
+        pair_table(*this, [&](int i, int j) { return gaunt_cos2(lval[j], mval[j], lval[i], mval[i]); });
+    collect_pure(*this, out, [&](int i, int j, int a, int b) {
+      double v = 0.0;
+      if (lval[i] == lval[j]) v += 1.0 * I12(a, b);
+      const double c = cc[(size_t)i * Nang() + j];
+      if (c != 0.0) v += -c * I10(a, b);
+      return v * f;
+    });
+  } else if (which == 1) {
+    const Blk Tr = radial_kinetic(), Ip1 = radial_integral(1, 0), Im1 = radial_integral(-1, 0);
+    const double f = Rh / 2.0;
+    collect_pure(*this, out, [&](int i, int j, int a, int b) {
+      if (i != j) return 0.0;
+      double v = 0.0;
+      v += 1.0 * Tr(a, b);
+      if (lval[i] != 0) v += double(lval[i] * (lval[i] + 1)) * Ip1(a, b);
+      if (mval[i] != 0) v += double(mval[i] * mval[i]) * Im1(a, b);
+      return v * f;
+    });
+  } else if (which == 2) {
+    const Blk I10 = radial_integral(1, 0), I11 = radial_integral(1, 1);
+    const double f = -std::pow(Rh, 2);
+    const std::vector<double> cc =
+        pair_table(*this, [&](int i, int j) { return gaunt_cos(lval[j], mval[j], lval[i], mval[i]); });
+    collect_pure(*this, out, [&](int i, int j, int a, int b) {
+      double v = 0.0;
+      if (lval[i] == lval[j]) v += double(Z1 + Z2) * I11(a, b);
+      if (Z1 != Z2) {
+        const double c = cc[(size_t)i * Nang() + j];
+        if (c != 0.0) v += (double(Z2 - Z1) * c) * I10(a, b);
+      }
+      return v * f;
+    });
+  } else {
+    throw std::invalid_argument("which must be 0, 1 or 2");
+  }
+}
+
 void DiatomicSetup::overlap(double *out, int64_t ld) const {
-  const Blk I10 = radial_integral(1, 0), I12 = radial_integral(1, 2);
-  const double f = std::pow(Rh, 3);
-  const std::vector<double> cc =
-      pair_table(*this, [&](int i, int j) { return gaunt_cos2(lval[j], mval[j], lval[i], mval[i]); });
-  fill_pure(*this, out, ld, [&](int i, int j, int a, int b) {
-    double v = 0.0;
-    if (lval[i] == lval[j]) v += 1.0 * I12(a, b);
-    const double c = cc[(size_t)i * Nang() + j];
-    if (c != 0.0) v += -c * I10(a, b);
-    return v * f;
-  });
+  BlockList b;
+  one_electron_blocks(0, b);
+  b.to_dense(out, ld, Nbf());
 }
 
 void DiatomicSetup::kinetic(double *out, int64_t ld) const {
-  const Blk Tr = radial_kinetic(), Ip1 = radial_integral(1, 0), Im1 = radial_integral(-1, 0);
-  const double f = Rh / 2.0;
-  fill_pure(*this, out, ld, [&](int i, int j, int a, int b) {
-    if (i != j) return 0.0;
-    double v = 0.0;
-    v += 1.0 * Tr(a, b);
-    if (lval[i] != 0) v += double(lval[i] * (lval[i] + 1)) * Ip1(a, b);
-    if (mval[i] != 0) v += double(mval[i] * mval[i]) * Im1(a, b);
-    return v * f;
-  });
+  BlockList b;
+  one_electron_blocks(1, b);
+  b.to_dense(out, ld, Nbf());
 }
 
 void DiatomicSetup::nuclear(double *out, int64_t ld) const {
-  const Blk I10 = radial_integral(1, 0), I11 = radial_integral(1, 1);
-  const double f = -std::pow(Rh, 2);
-  const std::vector<double> cc =
-      pair_table(*this, [&](int i, int j) { return gaunt_cos(lval[j], mval[j], lval[i], mval[i]); });
-  fill_pure(*this, out, ld, [&](int i, int j, int a, int b) {
-    double v = 0.0;
-    if (lval[i] == lval[j]) v += double(Z1 + Z2) * I11(a, b);
-    if (Z1 != Z2) {
-      const double c = cc[(size_t)i * Nang() + j];
-      if (c != 0.0) v += (double(Z2 - Z1) * c) * I10(a, b);
-    }
-    return v * f;
-  });
+  BlockList b;
+  one_electron_blocks(2, b);
+  b.to_dense(out, ld, Nbf());
 }
 
 }  // namespace hfg

@@ -71,6 +71,8 @@ struct DiatomicSetup {
   void overlap(double *out, int64_t ld) const;
   void kinetic(double *out, int64_t ld) const;
   void nuclear(double *out, int64_t ld) const;
+  // the same as their nonzero shell-pair blocks: which 0 overlap, 1 kinetic, 2 nuclear
+  void one_electron_blocks(int which, BlockList &out) const;
 
   Blk radial_integral(int m, int n) const;  // int B_i B_j sinh^m cosh^n
   Blk radial_kinetic() const;               // int B_i' B_j' sinh

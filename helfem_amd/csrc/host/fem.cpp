@@ -451,46 +451,64 @@ void radial_diag(const RadialFE &rad, const std::function<Blk(int)> &k, std::vec
       for (int i = 0; i < b.r; ++i) M[(size_t)(f + i) + (size_t)(f + j) * n] += b(i, j);
   }
 }
-void place(double *out, int64_t ld, int64_t nbf, int nrad, int nang, const std::vector<double> &blk,
-           const std::function<double(int)> &fac, const std::vector<double> *extra = nullptr,
-           const std::function<double(int)> *efac = nullptr) {
-  for (int64_t j = 0; j < nbf; ++j)
-    for (int64_t i = 0; i < nbf; ++i) out[i + j * ld] = 0.0;
-  for (int a = 0; a < nang; ++a) {
-    const double f = fac(a);
-    const double g = (extra && efac) ? (*efac)(a) : 0.0;
-    for (int j = 0; j < nrad; ++j)
-      for (int i = 0; i < nrad; ++i) {
-        double v = f * blk[(size_t)i + (size_t)j * nrad];
-        if (g != 0.0) v += g * (*extra)[(size_t)i + (size_t)j * nrad];
-        out[(int64_t)(a * nrad + i) + (int64_t)(a * nrad + j) * ld] = v;
-      }
-  }
-}
 }  // namespace
 
-// TwoDBasis.cpp:321-333
+void BlockList::to_dense(double *out, int64_t ld, int64_t n) const {
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t i = 0; i < n; ++i) out[i + j * ld] = 0.0;
+#pragma omp parallel for schedule(dynamic)
+  for (size_t b = 0; b < count(); ++b) {
+    const long long *m = &meta[5 * b];
+    const double *v = &vals[(size_t)m[4]];
+    for (long long c = 0; c < m[3]; ++c)
+      for (long long r = 0; r < m[2]; ++r) out[(m[0] + r) + (m[1] + c) * ld] = v[r + c * m[2]];
+  }
+}
+
+// TwoDBasis.cpp:321-392: every one-electron matrix is block diagonal over the
+// angular shells; overlap S, kinetic T + l(l+1) T_l, nuclear -Z <1/r> (point
+// nucleus at the origin) on each diagonal block.
+void AtomicSetup::one_electron_blocks(int which, BlockList &out) const {
+  if (which < 0 || which > 2) throw std::invalid_argument("which must be 0, 1 or 2");
+  std::vector<double> M, X;
+  if (which == 0)
+    radial_diag(rad, [&](int e) { return rad.rpow(0, e); }, M);
+  else if (which == 1) {
+    radial_diag(rad, [&](int e) { return rad.kin(e); }, M);
+    radial_diag(rad, [&](int e) { return rad.kin_l(e); }, X);
+  } else
+    radial_diag(rad, [&](int e) { return rad.rpow(-1, e); }, M);
+  const int n = Nrad();
+  std::vector<double> blk((size_t)n * n);
+  for (int a = 0; a < Nang(); ++a) {
+    const double f = which == 2 ? -double(Z) : 1.0;
+    const double g = which == 1 ? double(lval[a] * (lval[a] + 1)) : 0.0;
+    for (size_t q = 0; q < blk.size(); ++q) {
+      double v = f * M[q];
+      if (g != 0.0) v += g * X[q];
+      blk[q] = v;
+    }
+    out.add((long long)a * n, (long long)a * n, n, n, blk.data());
+  }
+}
+
 void AtomicSetup::overlap(double *out, int64_t ld) const {
-  std::vector<double> S;
-  radial_diag(rad, [&](int e) { return rad.rpow(0, e); }, S);
-  place(out, ld, Nbf(), Nrad(), Nang(), S, [](int) { return 1.0; });
+  BlockList b;
+  one_electron_blocks(0, b);
+  b.to_dense(out, ld, Nbf());
 }
 
-// TwoDBasis.cpp:354-374: T + l(l+1) T_l on each angular diagonal block
 void AtomicSetup::kinetic(double *out, int64_t ld) const {
-  std::vector<double> T, Tl;
-  radial_diag(rad, [&](int e) { return rad.kin(e); }, T);
-  radial_diag(rad, [&](int e) { return rad.kin_l(e); }, Tl);
-  const std::function<double(int)> lfac = [&](int a) { return double(lval[a] * (lval[a] + 1)); };
-  place(out, ld, Nbf(), Nrad(), Nang(), T, [](int) { return 1.0; }, &Tl, &lfac);
+  BlockList b;
+  one_electron_blocks(1, b);
+  b.to_dense(out, ld, Nbf());
 }
 
-// TwoDBasis.cpp:377-392, point nucleus at the origin
 void AtomicSetup::nuclear(double *out, int64_t ld) const {
-  std::vector<double> V;
-  radial_diag(rad, [&](int e) { return rad.rpow(-1, e); }, V);
-  const double z = -double(Z);
-  place(out, ld, Nbf(), Nrad(), Nang(), V, [z](int) { return z; });
+  BlockList b;
+  one_electron_blocks(2, b);
+  b.to_dense(out, ld, Nbf());
 }
 
 }  // namespace hfg

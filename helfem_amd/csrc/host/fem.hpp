@@ -53,6 +53,19 @@ Blk lagrange_prim(const std::vector<double> &x, const std::vector<double> &nodes
 
 // Finite-element radial basis with zero value at both ends
 // (FiniteElementBasisT + FEMRadialBasisT).
+// Dense matrix given by its nonzero blocks: per block (row0, col0, rows,
+// cols, offset into vals), values column-major.  to_dense zeroes the rest.
+struct BlockList {
+  std::vector<long long> meta;
+  std::vector<double> vals;
+  void add(long long row0, long long col0, int rows, int cols, const double *v) {
+    meta.insert(meta.end(), {row0, col0, (long long)rows, (long long)cols, (long long)vals.size()});
+    vals.insert(vals.end(), v, v + (size_t)rows * cols);
+  }
+  size_t count() const { return meta.size() / 5; }
+  void to_dense(double *out, int64_t ld, int64_t n) const;
+};
+
 struct RadialFE {
   int nnodes = 0, nquad = 0;
   bool drop_left = true;       // zero value at the left end (r = 0)
@@ -133,6 +146,8 @@ struct AtomicSetup {
   void overlap(double *out, int64_t ld) const;
   void kinetic(double *out, int64_t ld) const;
   void nuclear(double *out, int64_t ld) const;
+  // the same as their nonzero blocks: which 0 overlap, 1 kinetic, 2 nuclear
+  void one_electron_blocks(int which, BlockList &out) const;
 };
 
 }  // namespace hfg

@@ -102,6 +102,18 @@ class DiatomicBasis:
         check(lib().hf_diatomic_one_electron(self._h, which, out.ctypes.data, self.Nbf))
         return out
 
+    def one_electron_dev(self, which):
+        """Overlap (0), kinetic (1) or nuclear attraction (2) as a float64 CUDA
+        tensor, assembled on the device from its nonzero blocks (same values
+        as the host form)."""
+        import torch
+        if self.ctx is None:
+            raise RuntimeError("basis was created without a GPU context")
+        self.ctx.bind_torch_stream()
+        out = torch.empty((self.Nbf, self.Nbf), dtype=torch.float64, device="cuda")
+        check(lib().hf_diatomic_one_electron_dev(self._h, which, out.data_ptr(), self.Nbf))
+        return out.T  # column-major storage: the transpose view is the matrix
+
     def overlap(self):
         return self._one_e(0)
 

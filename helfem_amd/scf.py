@@ -26,10 +26,13 @@ import numpy as np
 
 
 def _one_electron(basis, dev):
-    """S, T, V on the device.  The host matrices are Fortran-ordered, so their
-    transposes are C-contiguous views: those are sent and transposed back on
-    the device, avoiding a host-side copy of each Nbf x Nbf matrix."""
+    """S, T, V on the device: assembled there from their nonzero blocks when
+    the basis has a GPU context, else built on the host and sent (the host
+    matrices are Fortran-ordered, so their transposes are C-contiguous views:
+    those are sent and transposed back, avoiding a host-side copy)."""
     import torch
+    if dev.type == "cuda" and getattr(basis, "ctx", None) is not None and hasattr(basis, "one_electron_dev"):
+        return [basis.one_electron_dev(w).contiguous() for w in (0, 1, 2)]
     out = []
     for M in (basis.overlap(), basis.kinetic(), basis.nuclear()):
         out.append(torch.from_numpy(np.asarray(M).T).to(dev, dtype=torch.float64).T.contiguous())
@@ -58,6 +61,8 @@ class _Clock:
 
 
 def _invh(S):
+    """D (D S D)^-1/2 with D = diag(S)^-1/2 (invh with chol = false,
+    libhelfem/src/utils.cpp:121-159)."""
     import torch
     d = 1.0 / torch.sqrt(torch.diagonal(S))
     Sn = d[:, None] * S * d[None, :]
@@ -93,6 +98,7 @@ class _Blocks:
 
     TOL = 1e-12
     GAP = 0.05
+    DEGEN = 1e-9
 
     def __init__(self, basis, S, symm, dev):
         import torch
@@ -100,7 +106,7 @@ class _Blocks:
         idx = basis.get_sym_idx(symm)
         self.b = [torch.as_tensor(b, dtype=torch.long, device=dev) for b in idx]
         self.S = [S[b][:, b] for b in self.b]
-        self.X = [_invh(Sk) for Sk in self.S]
+        self.X = [None] * len(self.S)
         self.dev = dev
         # shells of each block, in block order: (l, local start, size)
         lval = np.asarray(basis.lval).tolist()
@@ -123,11 +129,49 @@ class _Blocks:
                 self.shells[k] = [(-1, 0, len(idx[k]))]
         # overlap couplings between shells (fixed)
         self.Scpl = [self._coupling(Sk, k) for k, Sk in enumerate(self.S)]
+        self.X = [self._invh_block(k) for k in range(len(self.S))]
         self.solves = 0
         self.paired_solves = 0
         self.skipped_solves = 0
         self.memo = {}  # (slot, block, shells) -> (matrix last solved, its lowest eigenvalue)
         self.homo = {}  # slot -> highest occupied level of the last aufbau
+
+    def _invh_block(self, k):
+        """invh of overlap block k, taken over the groups of shells that S
+        couples (S is block diagonal over them, and so is its half inverse);
+        groups with identical overlap blocks share one eigensolve."""
+        torch = self.torch
+        Sk = self.S[k]
+        if self.Scpl[k] is None:
+            return _invh(Sk)
+        host = self.Scpl[k].cpu().numpy()
+        ns = len(self.shells[k])
+        parent = list(range(ns))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+
+        for s_ in range(ns):
+            for t in range(s_ + 1, ns):
+                if host[s_, t] != 0.0:
+                    parent[find(s_)] = find(t)
+        comps = {}
+        for s_ in range(ns):
+            comps.setdefault(find(s_), []).append(s_)
+        X = torch.zeros_like(Sk)
+        done = []
+        for comp in sorted(comps.values()):
+            li = self._local(k, comp)
+            Ssub = Sk[li][:, li]
+            Xsub = next((Xp for Sp, Xp in done if Sp.shape == Ssub.shape and torch.equal(Sp, Ssub)), None)
+            if Xsub is None:
+                Xsub = _invh(Ssub)
+                done.append((Ssub, Xsub))
+            X[li[:, None], li[None, :]] = Xsub
+        return X
 
     def _coupling(self, M, k):
         """(ns x ns) host matrix of max |M_st| over shell pairs (None when the
@@ -255,8 +299,17 @@ class _Blocks:
                 self.memo[plan["keys"][i]] = (plan["all"][i], float(e[0]))
 
         def aufbau():
-            return sorted((float(e), i, j) for i in range(len(subs)) if reuse[i] in ev
-                          for j, e in enumerate(ev[reuse[i]]))
+            # levels within DEGEN of each other (a chain of such gaps) count as
+            # degenerate and are filled in sub-problem order, so the occupied
+            # set does not hinge on rounding (open shells: which m fills first)
+            lv = sorted((float(e), i, j) for i in range(len(subs)) if reuse[i] in ev
+                        for j, e in enumerate(ev[reuse[i]]))
+            out, start = [], 0
+            for q in range(1, len(lv) + 1):
+                if q == len(lv) or lv[q][0] - lv[q - 1][0] > self.DEGEN * max(1.0, abs(lv[q][0])):
+                    out += sorted(lv[start:q], key=lambda x: (x[1], x[2]))
+                    start = q
+            return out
 
         take(plan["uniq"], sols)
         order = aufbau()
@@ -300,29 +353,56 @@ class _Blocks:
         return out
 
 
+PAD_RATIO = 0.75
+
+
 def batched_eigh(mats):
-    """Eigenpairs of symmetric matrices; equal sizes share one batched call
-    and all eigenvalues come to the host in one transfer."""
+    """Eigenpairs of symmetric matrices, ascending.  Matrices whose sizes lie
+    within PAD_RATIO of the largest of a group share one batched eigensolve:
+    each is padded to the group size with a diagonal c = 2 |A|_F + 1 above its
+    spectrum and no coupling, so its eigenpairs are the first n of the padded
+    matrix (the Householder reduction keeps the zero coupling exact).  The
+    eigensolves are launch-bound at these sizes, so fewer, larger batches
+    win.  All eigenvalues come to the host in one transfer."""
     import torch
     if not mats:
         return []
-    by_n = {}
-    for j, A in enumerate(mats):
-        by_n.setdefault(A.shape[0], []).append(j)
+    order = sorted(range(len(mats)), key=lambda j: -mats[j].shape[0])
+    groups = []
+    for j in order:
+        if groups and mats[j].shape[0] >= PAD_RATIO * mats[groups[-1][0]].shape[0]:
+            groups[-1].append(j)
+        else:
+            groups.append([j])
     vals, vecs = [None] * len(mats), [None] * len(mats)
     evs = []
-    for n, members in by_n.items():
-        e, v = torch.linalg.eigh(torch.stack([mats[j] for j in members]))
+    for members in groups:
+        N = mats[members[0]].shape[0]
+        A0 = mats[members[0]]
+        if all(mats[j].shape[0] == N for j in members):
+            stack = torch.stack([mats[j] for j in members])
+        else:
+            stack = A0.new_zeros((len(members), N, N))
+            for q, j in enumerate(members):
+                A = mats[j]
+                n = A.shape[0]
+                stack[q, :n, :n] = A
+                if n < N:
+                    idx = torch.arange(n, N, device=A.device)
+                    stack[q, idx, idx] = 2.0 * torch.linalg.matrix_norm(A) + 1.0
+        e, v = torch.linalg.eigh(stack)
         for q, j in enumerate(members):
-            vecs[j] = v[q]
+            n = mats[j].shape[0]
+            vecs[j] = v[q, :n, :n]
         evs.append((members, e))
     host = torch.cat([e.reshape(-1) for _, e in evs]).cpu().numpy()
     off = 0
     for members, e in evs:
-        n = e.shape[1]
-        for j in members:
-            vals[j] = host[off:off + n]
-            off += n
+        N = e.shape[1]
+        for q, j in enumerate(members):
+            n = mats[j].shape[0]
+            vals[j] = host[off + q * N:off + q * N + n]
+        off += len(members) * N
     return list(zip(vals, vecs))
 
 
@@ -522,23 +602,32 @@ class SCFEngine:
         """One iteration of every active state; returns the states that
         finished in it."""
         import torch
+        devs = {st.dev for st in self.active}
         t = time.perf_counter()
         reqs = []
         for st in self.active:
             for s, Fb, nocc in st.requests():
                 reqs.append((st, s, st.bs.prepare(Fb, s), nocc))
+        for dev in devs:
+            _sync(dev)
+        t1 = time.perf_counter()
         sols = batched_eigh([m for r in reqs for m in r[2]["mats"]])
+        for dev in devs:
+            _sync(dev)
+        t2 = time.perf_counter()
         off = 0
         for st, s, plan, nocc in reqs:
             n = len(plan["mats"])
             st.deliver(s, st.bs.finish(plan, sols[off:off + n], nocc, st.weight, st.H))
             off += n
-        for dev in {st.dev for st in self.active}:
+        for dev in devs:
             _sync(dev)
-        dt = time.perf_counter() - t
-        self.eig_seconds += dt
+        t3 = time.perf_counter()
+        self.eig_seconds += t2 - t1
         for st in self.active:
-            st.clk.t["density"] = st.clk.t.get("density", 0.0) + dt
+            ph = st.clk.t
+            ph["eigensolve"] = ph.get("eigensolve", 0.0) + (t2 - t1)
+            ph["density"] = ph.get("density", 0.0) + (t1 - t) + (t3 - t2)
             st.clk.last = time.perf_counter()
             st.advance()
         self.steps += 1

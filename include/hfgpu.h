@@ -118,6 +118,9 @@ int hf_atomic_get_cache(const hf_atomic *h, int kind, int L, int iel, int dims[2
 /* One-electron matrices (Nbf x Nbf): which 0 overlap (TwoDBasis.cpp:321),
  * 1 kinetic (:354), 2 nuclear attraction (:377). */
 int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t ld);
+/* The same into a device matrix on the handle's stream (needs a context):
+ * only the nonzero angular blocks are built on the host and sent. */
+int hf_atomic_one_electron_dev(const hf_atomic *h, int which, double *dOut, int64_t ld);
 
 /* Range-separated two-electron caches: the Yukawa kernel exp(-lambda r12)/r12
  * (TwoDBasisT::compute_yukawa, TwoDBasis.h:183, TwoDBasis.cpp:738-760) or
@@ -244,6 +247,9 @@ int hf_diatomic_get_cache(const hf_diatomic *h, int kind, int ilm, int iel, int 
 /* Pure-basis one-electron matrices: which 0 overlap, 1 kinetic, 2 nuclear
  * attraction (basis.h:280-284, basis.cpp:1032-1156). */
 int hf_diatomic_one_electron(const hf_diatomic *h, int which, double *out, int64_t ld);
+/* The same into a device matrix on the handle's stream (needs a context):
+ * only the nonzero shell-pair blocks are built on the host and sent. */
+int hf_diatomic_one_electron_dev(const hf_diatomic *h, int which, double *dOut, int64_t ld);
 /* Work of the most recent exchange build, as hf_atomic_work
  * (basis.cpp:1818-2089). */
 int hf_diatomic_work(hf_diatomic *h, double out[4]);

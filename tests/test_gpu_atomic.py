@@ -190,3 +190,16 @@ def test_gpu_uhf_hydrogen(gpu_ctx):
     r = uhf(hb, 1, 0, conv=1e-12)
     assert abs(r.E - REF["atomic-H-hf-u"]["total"]) < 1e-9, r.E
     assert abs(r.Ecoul + r.Exx) < 1e-10
+
+
+def test_one_electron_on_device(gpu_ctx):
+    """hf_atomic_one_electron_dev assembles S, T, V on the device from their
+    angular blocks: equal to the host matrices, which equal the oracle's."""
+    import torch
+    print("[gpu] atomic one-electron matrices on the device", flush=True)
+    ob, hb = pair("Kr-l3m2-4el", gpu_ctx)
+    for w, host in ((0, hb.overlap()), (1, hb.kinetic()), (2, hb.nuclear())):
+        dev = hb.one_electron_dev(w).cpu().numpy()
+        torch.cuda.synchronize()
+        assert np.array_equal(dev, host), w
+    assert np.array_equal(hb.overlap(), ob.overlap())

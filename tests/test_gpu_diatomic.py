@@ -212,3 +212,18 @@ def test_nh_triplet_uhf_on_gpu(absm, gpu_ctx):
     assert abs(r.E - ORC["diatomic-NH-hf-u"]["total"]) < 1e-10, r.E
     name = "diatomic-NH-hf-absm" if absm else "diatomic-NH-hf-u"
     assert abs(r.E - REF[name]["total"]) < 3e-9, (r.E, REF[name]["total"])
+
+
+@pytest.mark.parametrize("name", ["LiH-l32-4el", "N2-l432-3el"])
+def test_one_electron_on_device(name, gpu_ctx):
+    """hf_diatomic_one_electron_dev assembles S, T, V on the device from their
+    nonzero shell-pair blocks: equal to the host matrices and the oracle's."""
+    import torch
+    print(f"[gpu] diatomic one-electron matrices on the device: {name}", flush=True)
+    ob, hb = pair(name, gpu_ctx)
+    for w, host, orc in ((0, hb.overlap(), ob.overlap()), (1, hb.kinetic(), ob.kinetic()),
+                         (2, hb.nuclear(), ob.nuclear())):
+        dev = hb.one_electron_dev(w).cpu().numpy()
+        torch.cuda.synchronize()
+        assert np.array_equal(dev, host), w
+        assert np.array_equal(host, orc), w

@@ -118,3 +118,38 @@ def test_weyl_skip_of_unoccupied_blocks(monkeypatch):
     uf = uhf(h, 1, 0, device="cpu", conv=1e-12)
     assert u.skipped_solves > 0 and uf.skipped_solves == 0
     assert abs(u.E - uf.E) < 1e-10
+
+
+def test_blockwise_invh_equals_full():
+    """The half inverse taken over the shell groups S couples (with shared
+    eigensolves for identical groups) equals invh of the whole symmetry block
+    (libhelfem/src/utils.cpp:121-159) and orthonormalises it."""
+    from helfem_amd.scf import _Blocks, _invh, _one_electron
+    ne = OracleFock(10, 2, 4)
+    S, _, _ = _one_electron(ne, torch.device("cpu"))
+    bs = _Blocks(ne, S, 1, torch.device("cpu"))
+    for Sk, Xk in zip(bs.S, bs.X):
+        full = _invh(Sk)
+        assert float(torch.max(torch.abs(Xk - full))) <= 1e-10 * float(torch.max(torch.abs(full)))
+        eye = torch.eye(Sk.shape[0], dtype=Sk.dtype)
+        assert float(torch.max(torch.abs(Xk.T @ Sk @ Xk - eye))) <= 1e-12
+
+
+def test_padded_batched_eigh():
+    """Mixed sizes share padded batched eigensolves: the eigenvalues equal
+    each matrix's own and the eigenvectors span the same vectors (up to sign),
+    also for a spectrum spanning 1e5."""
+    from helfem_amd.scf import batched_eigh
+    g = torch.Generator().manual_seed(7)
+    mats = []
+    for n in (40, 37, 31, 40, 12, 29):
+        A = torch.randn(n, n, generator=g, dtype=torch.float64)
+        A = A + A.T
+        A[0, 0] += 1e5
+        mats.append(A)
+    for A, (e, v) in zip(mats, batched_eigh(mats)):
+        e0, v0 = torch.linalg.eigh(A)
+        assert v.shape == A.shape and len(e) == A.shape[0]
+        assert np.max(np.abs(e - e0.numpy())) <= 1e-10
+        s = torch.sign((v * v0).sum(0))
+        assert float(torch.max(torch.abs(v * s - v0))) <= 1e-10
