@@ -27,6 +27,15 @@
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
 
+// terms in flight per lane in the cross-element gathers, and that launch's
+// waves per SIMD
+#ifndef HFG_DIA_OFF_UNROLL
+#define HFG_DIA_OFF_UNROLL 4
+#endif
+#ifndef HFG_DIA_OFF_WAVES
+#define HFG_DIA_OFF_WAVES 5
+#endif
+
 namespace hfg {
 namespace {
 
@@ -470,7 +479,8 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
     const int ilm = A.lrun_ilm[run];
     T r00[4], r02[4], r20[4], r22[4];
     if constexpr (TILED)
-      gather_tiles<T>(pt, A.ttile, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], r00, r02, r20, r22);
+      gather_tiles<T, INSIDE ? 2 : HFG_DIA_OFF_UNROLL>(pt, A.ttile, A.term_f, A.lrun_tstart[run],
+                                                       A.lrun_tstart[run + 1], r00, r02, r20, r22);
     else
       gather_terms<T>(A.P, A.ldp, A.tpos, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], rloc, cloc, cmask,
                       r00, r02, r20, r22);
@@ -577,7 +587,7 @@ __device__ inline void dia_k_body(const DKArgs<T> &A) {
 // Cross-element launch held to 5 waves per SIMD (96 registers), which the
 // two-term gather fits; the in-element fallback keeps the compiler's choice.
 template <typename T, bool TILED>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) k_dia_k_off(DKArgs<T> A) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(HFG_DIA_OFF_WAVES))) k_dia_k_off(DKArgs<T> A) {
   dia_k_body<false, TILED, T>(A);
 }
 template <typename T, bool TILED>

@@ -129,48 +129,60 @@ class _Blocks:
                 self.shells[k] = [(-1, 0, len(idx[k]))]
         # overlap couplings between shells (fixed)
         self.Scpl = [self._coupling(Sk, k) for k, Sk in enumerate(self.S)]
-        self.X = [self._invh_block(k) for k in range(len(self.S))]
+        self.X = self._invh_blocks()
         self.solves = 0
         self.paired_solves = 0
         self.skipped_solves = 0
         self.memo = {}  # (slot, block, shells) -> (matrix last solved, its lowest eigenvalue)
         self.homo = {}  # slot -> highest occupied level of the last aufbau
 
-    def _invh_block(self, k):
-        """invh of overlap block k, taken over the groups of shells that S
-        couples (S is block diagonal over them, and so is its half inverse);
-        groups with identical overlap blocks share one eigensolve."""
+    def _invh_blocks(self):
+        """invh of every overlap block, taken over the groups of shells that S
+        couples (S is block diagonal over them, and so is its half inverse):
+        one eigensolve per distinct group, all in padded batched calls."""
         torch = self.torch
-        Sk = self.S[k]
-        if self.Scpl[k] is None:
-            return _invh(Sk)
-        host = self.Scpl[k].cpu().numpy()
-        ns = len(self.shells[k])
-        parent = list(range(ns))
+        groups = []  # (block, local indices, index of the distinct group)
+        uniq = []    # distinct overlap sub-blocks
+        for k, Sk in enumerate(self.S):
+            if self.Scpl[k] is None:
+                comps = [list(range(len(self.shells[k])))]
+            else:
+                host = self.Scpl[k].cpu().numpy()
+                ns = len(self.shells[k])
+                parent = list(range(ns))
 
-        def find(x):
-            while parent[x] != x:
-                parent[x] = parent[parent[x]]
-                x = parent[x]
-            return x
+                def find(x):
+                    while parent[x] != x:
+                        parent[x] = parent[parent[x]]
+                        x = parent[x]
+                    return x
 
-        for s_ in range(ns):
-            for t in range(s_ + 1, ns):
-                if host[s_, t] != 0.0:
-                    parent[find(s_)] = find(t)
-        comps = {}
-        for s_ in range(ns):
-            comps.setdefault(find(s_), []).append(s_)
-        X = torch.zeros_like(Sk)
-        done = []
-        for comp in sorted(comps.values()):
-            li = self._local(k, comp)
-            Ssub = Sk[li][:, li]
-            Xsub = next((Xp for Sp, Xp in done if Sp.shape == Ssub.shape and torch.equal(Sp, Ssub)), None)
-            if Xsub is None:
-                Xsub = _invh(Ssub)
-                done.append((Ssub, Xsub))
-            X[li[:, None], li[None, :]] = Xsub
+                for s_ in range(ns):
+                    for t in range(s_ + 1, ns):
+                        if host[s_, t] != 0.0:
+                            parent[find(s_)] = find(t)
+                cc = {}
+                for s_ in range(ns):
+                    cc.setdefault(find(s_), []).append(s_)
+                comps = sorted(cc.values())
+            for comp in comps:
+                li = self._local(k, comp)
+                Ssub = Sk[li][:, li]
+                u = next((q for q, Sp in enumerate(uniq) if Sp.shape == Ssub.shape and torch.equal(Sp, Ssub)), None)
+                if u is None:
+                    u = len(uniq)
+                    uniq.append(Ssub)
+                groups.append((k, li, u))
+        # D (D S D)^-1/2 per distinct group (invh, utils.cpp:121-159)
+        ds = [1.0 / torch.sqrt(torch.diagonal(Sp)) for Sp in uniq]
+        sols = batched_eigh([d[:, None] * Sp * d[None, :] for d, Sp in zip(ds, uniq)])
+        Xu = []
+        for d, (e, v) in zip(ds, sols):
+            ev = torch.as_tensor(e, dtype=v.dtype, device=v.device)
+            Xu.append(d[:, None] * ((v * (1.0 / torch.sqrt(ev))) @ v.T))
+        X = [torch.zeros_like(Sk) for Sk in self.S]
+        for k, li, u in groups:
+            X[k][li[:, None], li[None, :]] = Xu[u]
         return X
 
     def _coupling(self, M, k):
