@@ -297,10 +297,12 @@ class _Blocks:
         return {"subs": subs, "loc": loc, "reuse": reuse, "uniq": solve, "X": X, "mats": [mats[i] for i in solve],
                 "all": mats, "keys": keys, "skipped": skipped, "slot": slot}
 
-    def finish(self, plan, sols, nocc, weight, like):
+    def finish(self, plan, sols, nocc, weight, like, frozen=None):
         """Aufbau over all sub-problems from their eigenpairs (`sols`, in the
         order of plan["mats"]); returns the full P (for J/K), its symmetry
-        blocks and the occupied orbitals per block."""
+        blocks and the occupied orbitals per block.  `frozen` (orbitals per
+        block) replaces the aufbau's choice of blocks: each block then takes
+        its own lowest levels."""
         subs, loc, reuse = plan["subs"], plan["loc"], plan["reuse"]
         ev, vecs = {}, {}
 
@@ -326,17 +328,26 @@ class _Blocks:
         take(plan["uniq"], sols)
         order = aufbau()
         homo = order[nocc - 1][0] if nocc > 0 else -np.inf
-        late = [i for i, lb in plan["skipped"].items() if not lb > homo]
+        late = [i for i, lb in plan["skipped"].items() if frozen is not None or not lb > homo]
         if late:  # the occupied levels rose past a bound: solve those after all
             self.solves += len(late)
             self.skipped_solves -= len(late)
             take(late, batched_eigh([plan["all"][i] for i in late]))
             order = aufbau()
             homo = order[nocc - 1][0] if nocc > 0 else -np.inf
-        self.homo[plan["slot"]] = homo
         nsub = [0] * len(subs)
-        for _, i, _j in order[:nocc]:
-            nsub[i] += 1
+        if frozen is None:
+            self.homo[plan["slot"]] = homo
+            for _, i, _j in order[:nocc]:
+                nsub[i] += 1
+        else:
+            self.homo[plan["slot"]] = np.inf  # no Weyl skipping under fixed occupations
+            left = list(frozen)
+            for _, i, _j in order:
+                k = subs[i][0]
+                if left[k] > 0:
+                    left[k] -= 1
+                    nsub[i] += 1
         P = like.new_zeros(like.shape)
         Pb = [like.new_zeros((len(b), len(b))) for b in self.b]
         nk = [0] * len(self.b)
@@ -496,6 +507,10 @@ class SCFState:
         z = torch.zeros((), dtype=torch.float64, device=self.dev)
         self.terms = dict(Ekin=z, Enuc=z, Ecoul=z, Exx=z, Exc=z)
         self.dens = [None] * self.nspin
+        # occupations per block of every delivered density; frozen once the
+        # aufbau flip-flops between patterns (near-degenerate frontier levels)
+        self.occ_hist = [[] for _ in range(self.nspin)]
+        self.frozen = [None] * self.nspin
         Hb = self.bs.blocks_of(self.H)
         self.pending = [(s, Hb) for s in range(self.nspin)]
         self.done = False
@@ -506,6 +521,26 @@ class SCFState:
 
     def deliver(self, s, dens):
         self.dens[s] = dens
+        self.occ_hist[s].append(tuple(dens[2]))
+
+    FLIPS = 3       # occupation changes within the last WINDOW iterations that freeze them
+    WINDOW = 10
+
+    def _check_flips(self):
+        """Aufbau oscillating between occupation patterns (a frontier level
+        crossing back and forth) never converges: after FLIPS changes within
+        WINDOW iterations keep the pattern of the lowest energy seen so far."""
+        if self.frozen[0] is not None or self.it < self.WINDOW:
+            return
+        n = min(len(self.hist), *(len(h) for h in self.occ_hist))
+        pats = [tuple(self.occ_hist[s][t] for s in range(self.nspin)) for t in range(n)]
+        recent = pats[-self.WINDOW:]
+        if sum(1 for a, b in zip(recent, recent[1:]) if a != b) < self.FLIPS:
+            return
+        best = min(range(n), key=lambda t: self.hist[t])
+        self.frozen = [list(pats[best][s]) for s in range(self.nspin)]
+        self.diis.reset()
+        self.Eold = None
 
     def advance(self):
         """One Fock build, energy and convergence test on the delivered
@@ -556,6 +591,7 @@ class SCFState:
             err += bs.commutator(Fb[s], self.dens[s][1])
         emax = float(torch.max(torch.stack([torch.max(torch.abs(x)) for x in err])))
         self.hist.append(E)
+        self._check_flips()
         clk.tick("energy+diis")
         if self.verbose:
             print(f"it {self.it:3d} E {E:.12f} err {emax:.3e}{' (fp32 K)' if self.fp32 else ''}", flush=True)
@@ -587,7 +623,7 @@ class SCFState:
                         Exx=float(t["Exx"]), Exc=float(t["Exc"]), Enucr=self.Enucr, iterations=self.it,
                         iterations_fp32=self.it_fp32, seconds=time.perf_counter() - self.t0, history=self.hist,
                         phases=self.clk.t, eigensolves=self.bs.solves, reused_solves=self.bs.paired_solves,
-                        skipped_solves=self.bs.skipped_solves,
+                        skipped_solves=self.bs.skipped_solves, frozen_occupations=self.frozen[0] is not None,
                         tag=self.tag)
         if self.restricted:
             out.update(P=self.dens[0][0], occupations=self.dens[0][2])
@@ -630,7 +666,7 @@ class SCFEngine:
         off = 0
         for st, s, plan, nocc in reqs:
             n = len(plan["mats"])
-            st.deliver(s, st.bs.finish(plan, sols[off:off + n], nocc, st.weight, st.H))
+            st.deliver(s, st.bs.finish(plan, sols[off:off + n], nocc, st.weight, st.H, st.frozen[s]))
             off += n
         for dev in devs:
             _sync(dev)

@@ -153,3 +153,29 @@ def test_padded_batched_eigh():
         assert np.max(np.abs(e - e0.numpy())) <= 1e-10
         s = torch.sign((v * v0).sum(0))
         assert float(torch.max(torch.abs(v * s - v0))) <= 1e-10
+
+
+def test_frozen_occupations_in_aufbau():
+    """finish() with per-block orbital counts puts exactly that many of each
+    block's lowest orbitals in the density; with the aufbau's own counts it
+    reproduces the aufbau density."""
+    from helfem_amd.scf import _Blocks, _one_electron
+    be = OracleFock(4, 1, 4)
+    S, T, V = _one_electron(be, torch.device("cpu"))
+    H = T + V
+    bs = _Blocks(be, S, 1, torch.device("cpu"))
+    Hb = bs.blocks_of(H)
+    P0, Pb0, nk0 = bs.density(Hb, 2, 2.0, H)
+    assert sum(nk0) == 2
+    plan = bs.prepare(Hb, 1)
+    from helfem_amd.scf import batched_eigh
+    sols = batched_eigh(plan["mats"])
+    P1, _, nk1 = bs.finish(plan, sols, 2, 2.0, H, frozen=list(nk0))
+    assert nk1 == nk0 and float(torch.max(torch.abs(P1 - P0))) < 1e-12
+    want = [1 if k == 0 else 0 for k in range(len(nk0))]
+    want[len(nk0) // 2] += 1
+    plan = bs.prepare(Hb, 2)
+    P2, _, nk2 = bs.finish(plan, batched_eigh(plan["mats"]), 2, 2.0, H, frozen=want)
+    assert nk2 == want
+    Sd = S.numpy()
+    assert abs(float(np.sum(P2.numpy() * Sd)) - 4.0) < 1e-10
