@@ -31,12 +31,11 @@ namespace {
 
 using namespace dev;
 
+// Columns over blockIdx.y, rows over blockIdx.x (no index division).
 __global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
-  const int64_t n = rows * cols;
-  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n; q += (int64_t)gridDim.x * NT) {
-    const int64_t j = q / rows, i = q - j * rows;
-    A[i + j * ld] = 0.0;
-  }
+  for (int64_t j = blockIdx.y; j < cols; j += gridDim.y)
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < rows; i += (int64_t)gridDim.x * NT)
+      A[i + j * ld] = 0.0;
 }
 
 // --- Coulomb -----------------------------------------------------------------
@@ -719,9 +718,9 @@ double exchange_tile_budget(size_t held_bytes) {
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   const int64_t n = rows * cols;
   if (n == 0) return;
-  int64_t blocks = (n + NT - 1) / NT;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(NT), 0, s, A, rows, cols, ld);
+  const unsigned bx = (unsigned)std::min<int64_t>((rows + NT - 1) / NT, 64);
+  const unsigned by = (unsigned)std::min<int64_t>(cols, 65535);
+  hipLaunchKernelGGL(k_fill_zero, dim3(bx, by), dim3(NT), 0, s, A, rows, cols, ld);
   hip_check(hipGetLastError(), "fill_zero");
 }
 
@@ -905,15 +904,10 @@ static bool exchange_prologue(AtomicDevice &d, const double *P, int64_t ldp, dou
     fill_zero_launch(K, nbf, nbf, ldk, s);
     return false;
   }
-  block_stats_launch(P, ldp, d.Nrad, d.Nang, nullptr, nullptr, d.bnorm.p, d.asym.p, s);
-  hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
-            "copy block norms");
-  const bool try_sym = allow_sym && !d.dense_in && symmetric_exchange_allowed();
-  if (try_sym)
-    hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
-                             s), "copy block asymmetry");
-  hip_check(hipStreamSynchronize(s), "block norms sync");
-  d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
+  block_stats_launch(P, ldp, d.Nrad, d.Nang, nullptr, nullptr, d.stats.p, d.h_stats.p, s);
+  hip_check(hipStreamSynchronize(s), "block statistics sync");
+  const bool symmetric = block_stats_reduce(d.h_stats.p, d.Nrad, d.Nang, d.h_bnorm.p);
+  d.sym = allow_sym && !d.dense_in && symmetric_exchange_allowed() && symmetric;
   compact_exchange_lists(d, s);
   fill_zero_launch(K, nbf, nbf, ldk, s);
   return d.n_act > 0;

@@ -256,14 +256,14 @@ void atomic_upload(hf_atomic *h) {
   d.n_act = 0;
   d.n_terms = 0;
   d.h_bnorm.alloc((size_t)Na * Na);
-  d.h_asym.alloc((size_t)Na * Na * 2);
-  d.asym.alloc((size_t)Na * Na * 2);
+  d.h_stats.alloc((size_t)Na * Na * 4 * block_stats_chunks(d.Nrad));
+  d.stats.alloc((size_t)Na * Na * 4 * block_stats_chunks(d.Nrad));
   d.Jin.alloc((size_t)std::max(nlm, 1) * Nel * TILE2);
   d.jsm.alloc((size_t)std::max(nlm, 1) * Nel);
   d.jbg.alloc((size_t)std::max(nlm, 1) * Nel);
   d.preA.alloc((size_t)std::max(nlm, 1) * Nel);
   d.preB.alloc((size_t)std::max(nlm, 1) * Nel);
-  d.bnorm.alloc((size_t)Na * Na);
+
   hip_check(hipStreamSynchronize(s), "upload sync");
   atomic_build_operator(S, d, s);
   h->uploaded = true;

@@ -293,9 +293,9 @@ void diatomic_upload(hf_diatomic *h) {
   d.screen_mask.clear();
   d.n_act = 0;
   d.h_bnorm.alloc((size_t)Na * Na);
-  d.bnorm.alloc((size_t)Na * Na);
-  d.h_asym.alloc((size_t)Na * Na * 2);
-  d.asym.alloc((size_t)Na * Na * 2);
+
+  d.h_stats.alloc((size_t)Na * Na * 4 * block_stats_chunks(d.Nrad));
+  d.stats.alloc((size_t)Na * Na * 4 * block_stats_chunks(d.Nrad));
   d.Jin.alloc((size_t)std::max(nLM, 1) * Nel * 2 * TILE2);
   d.ds.alloc((size_t)std::max(nLM, 1) * Nel);
   d.db.alloc((size_t)std::max(nLM, 1) * Nel);
@@ -548,15 +548,10 @@ void ensure_ready(hf_diatomic *h) {
 void exchange_launch(hf_diatomic *h, const double *P, int64_t ldp, double *K, int64_t ldk, bool fp32 = false) {
   DiatomicDevice &d = h->dev;
   hipStream_t s = h->ctx->cur;
-  block_stats_launch(P, ldp, d.Nrad, d.Nang, d.skip.p, d.poff.p, d.bnorm.p, d.asym.p, s);
-  hip_check(hipMemcpyAsync(d.h_bnorm.p, d.bnorm.p, sizeof(double) * d.Nang * d.Nang, hipMemcpyDeviceToHost, s),
-            "copy block norms");
-  const bool try_sym = !d.dense_in && symmetric_exchange_allowed();
-  if (try_sym)
-    hip_check(hipMemcpyAsync(d.h_asym.p, d.asym.p, sizeof(double) * 2 * d.Nang * d.Nang, hipMemcpyDeviceToHost,
-                             s), "copy block asymmetry");
-  hip_check(hipStreamSynchronize(s), "block norms sync");
-  d.sym = try_sym && density_is_symmetric(d.h_asym.p, d.Nang);
+  block_stats_launch(P, ldp, d.Nrad, d.Nang, d.skip.p, d.poff.p, d.stats.p, d.h_stats.p, s);
+  hip_check(hipStreamSynchronize(s), "block statistics sync");
+  const bool symmetric = block_stats_reduce(d.h_stats.p, d.Nrad, d.Nang, d.h_bnorm.p);
+  d.sym = !d.dense_in && symmetric_exchange_allowed() && symmetric;
   build_exchange_lists(h, s);
   diatomic_exchange_kernels(d, P, ldp, K, ldk, s, &h->ctx->timer, fp32);
 }

@@ -188,24 +188,24 @@ __global__ void __launch_bounds__(NT) k_dia_j_expand(DJOutArgs A) {
 
 // --- Exchange ----------------------------------------------------------------
 
-// One pass over P per shell block pair i <= l: the Frobenius norms of P_il
-// and P_li (the reference's screen) into bnorm, and the symmetry test
-// asym[2 il] = max |P_il - P_li'|, asym[2 il + 1] = max |P_il|.  32x32
-// sub-tiles of both blocks are read coalesced into LDS and compared
-// transposed.  Shell a spans rows poff[a] .. poff[a] + Nrad - skip[a] (a Nrad,
-// no skip when the tables are null: the atomic basis).
+// One pass over P per shell block pair i <= l and 32-row chunk ch of P_il:
+// partial sums for the Frobenius norms of P_il and P_li (the reference's
+// screen) and partial maxima for the symmetry test, part[(il * nch + ch) *
+// 4 + {0..3}] = (sum P_il^2, sum P_li^2, max |P_il - P_li'|, max |P_il|),
+// reduced on the host.  32x32 sub-tiles of both blocks are read coalesced
+// into LDS and compared transposed.  Shell a spans rows poff[a] .. poff[a] +
+// Nrad - skip[a] (a Nrad, no skip when the tables are null: atomic basis).
 __global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp, int Nrad, int Nang,
-                                                    const int *skip, const long long *poff, double *bnorm,
-                                                    double *asym) {
-  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang;
-  if (i > l) return;  // covered by the workgroup of (l, i)
+                                                    const int *skip, const long long *poff, double *part) {
+  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang, ch = blockIdx.y, nch = gridDim.y;
+  if (i > l) return;  // covered by the workgroups of (l, i)
   __shared__ double A[32][33], B[32][33];
   __shared__ double red[16];
   double amax = 0.0, pmax = 0.0, ssa = 0.0, ssb = 0.0;
   const int ri = Nrad - (skip ? skip[i] : 0), rl = Nrad - (skip ? skip[l] : 0);
   const long long oi = poff ? poff[i] : (long long)i * Nrad, ol = poff ? poff[l] : (long long)l * Nrad;
   const int t = threadIdx.x, x = t & 31, y0 = t >> 5;
-  for (int tr = 0; tr < ri; tr += 32)
+  for (int tr = 32 * ch; tr < ri; tr += 32 * nch)
     for (int tc = 0; tc < rl; tc += 32) {
       __syncthreads();
       // A[x][y] = P_il(tr + x, tc + y), B[x][y] = P_li(tc + x, tr + y)
@@ -240,10 +240,11 @@ __global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp
   }
   __syncthreads();
   if (t == 0) {
-    asym[2 * il] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    asym[2 * il + 1] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
-    bnorm[il] = sqrt((red[8] + red[9]) + (red[10] + red[11]));
-    if (i != l) bnorm[l * Nang + i] = sqrt((red[12] + red[13]) + (red[14] + red[15]));
+    double *o = part + ((size_t)il * nch + ch) * 4;
+    o[0] = (red[8] + red[9]) + (red[10] + red[11]);
+    o[1] = (red[12] + red[13]) + (red[14] + red[15]);
+    o[2] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    o[3] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
   }
 }
 
@@ -866,19 +867,32 @@ void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t l
   }
 }
 
+int block_stats_chunks(int Nrad) { return (Nrad + 31) / 32; }
+
 void block_stats_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
-                        double *bnorm, double *asym, hipStream_t s) {
-  hipLaunchKernelGGL(k_block_stats, dim3(Nang * Nang), dim3(NT), 0, s, P, ldp, Nrad, Nang, skip, poff, bnorm, asym);
+                        double *d_part, double *h_part, hipStream_t s) {
+  const int nch = block_stats_chunks(Nrad);
+  hipLaunchKernelGGL(k_block_stats, dim3(Nang * Nang, nch), dim3(NT), 0, s, P, ldp, Nrad, Nang, skip, poff, d_part);
   hip_check(hipGetLastError(), "block_stats");
+  hip_check(hipMemcpyAsync(h_part, d_part, sizeof(double) * 4 * nch * Nang * Nang, hipMemcpyDeviceToHost, s),
+            "copy block statistics");
 }
 
-bool density_is_symmetric(const double *h_asym, int Nang) {
+bool block_stats_reduce(const double *h_part, int Nrad, int Nang, double *bnorm) {
+  const int nch = block_stats_chunks(Nrad);
   double amax = 0.0, pmax = 0.0;
   for (int i = 0; i < Nang; ++i)
     for (int l = i; l < Nang; ++l) {  // only the pairs i <= l are written
-      const int q = i * Nang + l;
-      amax = std::max(amax, h_asym[2 * q]);
-      pmax = std::max(pmax, h_asym[2 * q + 1]);
+      const double *o = h_part + (size_t)(i * Nang + l) * nch * 4;
+      double sa = 0.0, sb = 0.0;
+      for (int c = 0; c < nch; ++c) {
+        sa += o[4 * c];
+        sb += o[4 * c + 1];
+        amax = std::max(amax, o[4 * c + 2]);
+        pmax = std::max(pmax, o[4 * c + 3]);
+      }
+      bnorm[i * Nang + l] = std::sqrt(sa);
+      if (i != l) bnorm[l * Nang + i] = std::sqrt(sb);
     }
   return amax <= 1e-14 * pmax;
 }

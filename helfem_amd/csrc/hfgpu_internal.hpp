@@ -170,8 +170,8 @@ struct AtomicDevice {
   // Symmetric densities (max |P - P'| <= 1e-14 max |P|) take the symmetric
   // exchange: only tiles with e > f, or e == f and j <= k, are computed, each
   // also written transposed to its mirror (k,j,f,e).
-  DevArray<double> asym;                  // [Nang^2][2]: max |P_il - P_li'|, max |P_il|
-  PinnedArray<double> h_asym;
+  DevArray<double> stats;                 // block statistics of k_block_stats, and their host copy
+  PinnedArray<double> h_stats;
   bool sym = false, list_sym = false;     // this call's mode; the mode the lists were built for
   std::vector<unsigned char> screen_mask;  // mask the compacted lists were built for
   DevArray<int> act_jk;                   // active output blocks j*Nang+k
@@ -181,7 +181,7 @@ struct AtomicDevice {
   DevArray<double> term_c;                // 4pi/(2L+1) * G_j * G_k
   PinnedArray<double> h_bnorm;
   // work buffers
-  DevArray<double> Jin, jsm, jbg, preA, preB, bnorm;
+  DevArray<double> Jin, jsm, jbg, preA, preB;
   DevArray<double> hostP, hostOut;        // staging for host-pointer entry points
   int max_rank = 0;
   int64_t n_terms = 0, n_pairs = 0;
@@ -249,8 +249,8 @@ struct DiatomicDevice {
   // Symmetric densities (max |P - P'| <= 1e-14 max |P|, from k_block_stats)
   // take the symmetric exchange: only tiles with e > f, or e == f and j <= k,
   // are computed, each also written transposed to its mirror (k,j,f,e).
-  DevArray<double> asym;                  // [Nang^2][2]: max |P_il - P_li'|, max |P_il|
-  PinnedArray<double> h_asym;
+  DevArray<double> stats;                 // block statistics of k_block_stats, and their host copy
+  PinnedArray<double> h_stats;
   bool sym = false, list_sym = false;     // this call's mode; the mode the lists were built for
   std::vector<double> h_rad_low;          // radial flops per coupled (jk, ilm) over the pairs e > f
   DevArray<int> act_jk, act_lstart;       // active output blocks; ilm runs per block
@@ -261,7 +261,7 @@ struct DiatomicDevice {
   DevArray<int> mirror;                   // absm: pairs (dst jk, src jk)
   int n_mirror = 0;
   PinnedArray<double> h_bnorm;
-  DevArray<double> Jin, ds, db, preA, preB, bnorm;
+  DevArray<double> Jin, ds, db, preA, preB;
   DevArray<double> hostP, hostOut;
   int64_t n_terms = 0, n_runs = 0, n_pairs = 0;
   double flops_k_radial = 0.0, flops_k_angular = 0.0;
@@ -350,13 +350,16 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                              hipStream_t s, KernelTimer *timer);
-// One pass over P: Frobenius norm of every shell block into bnorm[Nang^2],
-// and per block pair i <= l asym[2 il] = max |P_il - P_li'|, asym[2 il + 1] =
-// max |P_il| (skip/poff null: shell a is rows a*Nrad .. a*Nrad + Nrad).
+// One pass over P (k_block_stats): per shell block pair i <= l and 32-row
+// chunk, partial squared norms of P_il and P_li and partial maxima of |P_il -
+// P_li'| and |P_il|, copied to h_part (4 * chunks * Nang^2 doubles, pinned)
+// on stream s (skip/poff null: shell a is rows a*Nrad .. a*Nrad + Nrad).
+int block_stats_chunks(int Nrad);
 void block_stats_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
-                        double *bnorm, double *asym, hipStream_t s);
-// max |P - P'| <= 1e-14 max |P| over the block_stats asym output (a NaN fails)
-bool density_is_symmetric(const double *h_asym, int Nang);
+                        double *d_part, double *h_part, hipStream_t s);
+// After the copy: the Frobenius norm of every shell block into bnorm[Nang^2];
+// returns max |P - P'| <= 1e-14 max |P| (a NaN fails).
+bool block_stats_reduce(const double *h_part, int Nrad, int Nang, double *bnorm);
 // false when the environment variable HFGPU_EXCHANGE_SYMMETRIC is 0
 bool symmetric_exchange_allowed();
 // fp32 as for atomic_exchange_launch.
