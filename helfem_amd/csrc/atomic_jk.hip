@@ -31,11 +31,20 @@ namespace {
 
 using namespace dev;
 
-// Columns over blockIdx.y, rows over blockIdx.x (no index division).
+// Zero a rows x cols matrix; a contiguous one (ld == rows) as 16-byte stores.
 __global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
-  for (int64_t j = blockIdx.y; j < cols; j += gridDim.y)
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < rows; i += (int64_t)gridDim.x * NT)
-      A[i + j * ld] = 0.0;
+  const int64_t n = rows * cols, stride = (int64_t)gridDim.x * NT;
+  if (ld == rows && (reinterpret_cast<uintptr_t>(A) & 15) == 0) {
+    double2 *A2 = reinterpret_cast<double2 *>(A);
+    const int64_t n2 = n / 2;
+    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n2; q += stride) A2[q] = make_double2(0.0, 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) A[n - 1] = 0.0;
+    return;
+  }
+  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+    const int64_t j = q / rows, i = q - j * rows;
+    A[i + j * ld] = 0.0;
+  }
 }
 
 // --- Coulomb -----------------------------------------------------------------
@@ -718,9 +727,9 @@ double exchange_tile_budget(size_t held_bytes) {
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
   const int64_t n = rows * cols;
   if (n == 0) return;
-  const unsigned bx = (unsigned)std::min<int64_t>((rows + NT - 1) / NT, 64);
-  const unsigned by = (unsigned)std::min<int64_t>(cols, 65535);
-  hipLaunchKernelGGL(k_fill_zero, dim3(bx, by), dim3(NT), 0, s, A, rows, cols, ld);
+  int64_t blocks = (n + NT - 1) / NT;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(NT), 0, s, A, rows, cols, ld);
   hip_check(hipGetLastError(), "fill_zero");
 }
 
