@@ -5,6 +5,8 @@
 #include <cstdlib>
 #include <cmath>
 #include <memory>
+#include <map>
+#include <set>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -306,6 +308,214 @@ void diatomic_upload(hf_diatomic *h) {
   h->uploaded = true;
 }
 
+// Lists of the two-step cross-element exchange (k_dia_k2_off).  The terms of
+// output block (j,k) and (L,M) factor as fac(L,M) u(j,i) v(k,l) with u = (mod
+// Gaunt, plain Gaunt) of (j,i) and v the same of (k,l) (basis.cpp:1905-1960),
+// so R(j,k) = fac sum_i u(j,i) (x) V(i) with V_b(i) = sum_l v_b(k,l) P(i,l):
+// the V tiles of one (k, (L,|M|)) group serve every row j.  l_j + L + l_i is
+// even for every coupling, so a group splits by the parity of l_i into two
+// with disjoint rows.  A slot is one (M, i) of a group; its lt entries are the
+// (tile of (i,l), v0, v1) that pass the screen; a row's jt entries are (slot,
+// fac u0, fac u1).  Leaves d.k2 off when a column has more than K2_JMAX rows
+// or a group more than K2_SLOTS slots.
+void build_twostep(hf_diatomic *h, const std::vector<int> &act, const std::vector<int> &tile_of,
+                   const std::vector<std::vector<int>> &bydm, const std::vector<int> &ilm_of,
+                   const std::vector<double> &fac_of, int Mext, int dmoff, hipStream_t s) {
+  const DiatomicSetup &S = h->host;
+  DiatomicDevice &d = h->dev;
+  const int Na = S.Nang(), Nel = S.Nel();
+  const auto &lv = S.lval;
+  const auto &mv = S.mval;
+  d.k2 = false;
+  d.n_wg_k2 = 0;
+  if (!d.tiled || Nel < 2 || act.empty()) return;
+  // rows of every column shell k
+  std::vector<std::vector<int>> rows(Na);
+  for (int jk : act) rows[jk % Na].push_back(jk / Na);
+  std::vector<int> ks;
+  for (int k = 0; k < Na; ++k) {
+    if (rows[k].empty()) continue;
+    if ((int)rows[k].size() > K2_JMAX) return;
+    ks.push_back(k);
+  }
+  struct Group {
+    int ilm, par;
+    std::vector<std::pair<int, int>> slots;               // (M, i)
+    std::vector<std::vector<std::pair<int, std::pair<double, double>>>> lt;  // per slot: (tile, v)
+    std::vector<std::vector<std::pair<int, std::pair<double, double>>>> jt;  // per row: (slot, c)
+  };
+  std::vector<std::vector<Group>> groups(ks.size());
+  bool fits = true;
+#pragma omp parallel for schedule(dynamic)
+  for (size_t kq = 0; kq < ks.size(); ++kq) {
+    const int k = ks[kq], lk = lv[k], mk = mv[k];
+    const std::vector<int> &js = rows[k];
+    std::map<std::pair<int, int>, int> gid;  // (ilm, parity of l_i) -> group
+    std::vector<Group> &G = groups[kq];
+    std::vector<std::map<std::pair<int, int>, int>> slot_of;
+    std::vector<std::vector<std::set<int>>> lseen;
+    std::vector<std::vector<std::map<int, int>>> jseen;
+    for (size_t q = 0; q < js.size(); ++q) {
+      const int j = js[q], lj = lv[j], mj = mv[j], dm = mj - mk;
+      if (std::abs(dm) > 2 * (dmoff / 2)) continue;
+      for (int il : bydm[dm + dmoff]) {
+        const int i = il / Na, l = il - i * Na;
+        const int li = lv[i], mi = mv[i], ll = lv[l], M = mj - mi;
+        const int Lmin = std::max(std::max(std::abs(li - lj), std::abs(lk - ll)) - 2, std::abs(M));
+        const int Lmax = std::min(li + lj, lk + ll) + 2;
+        const size_t gji = ((size_t)j * Na + i) * d.gL, gkl = ((size_t)k * Na + l) * d.gL;
+        for (int L = Lmin; L <= Lmax; ++L) {
+          const double u0 = d.h_g0[gji + L], u1 = d.h_g2[gji + L];
+          const double v0 = d.h_g0[gkl + L], v1 = d.h_g2[gkl + L];
+          if ((u0 == 0.0 && u1 == 0.0) || (v0 == 0.0 && v1 == 0.0)) continue;
+          const size_t lm = (size_t)L * Mext + M + dmoff;
+          const int ilm = ilm_of[lm];
+          const double fac = fac_of[lm];
+          const auto gk = std::make_pair(ilm, li & 1);
+          auto it = gid.find(gk);
+          int g;
+          if (it == gid.end()) {
+            g = (int)G.size();
+            gid[gk] = g;
+            G.push_back(Group{ilm, li & 1, {}, {},
+                              std::vector<std::vector<std::pair<int, std::pair<double, double>>>>(js.size())});
+            slot_of.emplace_back();
+            lseen.emplace_back();
+            jseen.emplace_back(js.size());
+          } else {
+            g = it->second;
+          }
+          Group &gr = G[g];
+          auto sk = std::make_pair(M, i);
+          auto sit = slot_of[g].find(sk);
+          int sl;
+          if (sit == slot_of[g].end()) {
+            sl = (int)gr.slots.size();
+            slot_of[g][sk] = sl;
+            gr.slots.push_back(sk);
+            gr.lt.emplace_back();
+            lseen[g].emplace_back();
+          } else {
+            sl = sit->second;
+          }
+          if (lseen[g][sl].insert(l).second) gr.lt[sl].push_back({tile_of[(size_t)i * Na + l], {v0, v1}});
+          if (jseen[g][q].find(sl) == jseen[g][q].end()) {
+            jseen[g][q][sl] = 1;
+            gr.jt[q].push_back({sl, {fac * u0, fac * u1}});
+          }
+        }
+      }
+    }
+    std::sort(G.begin(), G.end(),
+              [](const Group &a, const Group &b) { return a.ilm < b.ilm || (a.ilm == b.ilm && a.par < b.par); });
+    for (const Group &gr : G)
+      if ((int)gr.slots.size() > K2_SLOTS) {
+#pragma omp critical
+        fits = false;
+      }
+  }
+  if (!fits) return;
+  // flatten
+  std::vector<int> kshell, kj(1, 0), jshell, kg(1, 0), gilm, gslot(1, 0), gjt, sl(1, 0), lt, jto, jts;
+  std::vector<double> ltv, jtc;
+  std::vector<double> kcost(ks.size(), 0.0);
+  double n_lt = 0.0, n_jt = 0.0, frad = 0.0;
+  for (size_t kq = 0; kq < ks.size(); ++kq) {
+    kshell.push_back(ks[kq]);
+    for (int j : rows[ks[kq]]) jshell.push_back(j);
+    kj.push_back((int)jshell.size());
+    for (const Group &gr : groups[kq]) {
+      gilm.push_back(gr.ilm);
+      gjt.push_back((int)jto.size());
+      for (size_t x = 0; x < gr.slots.size(); ++x) {
+        for (auto &e : gr.lt[x]) {
+          lt.push_back(e.first);
+          ltv.push_back(e.second.first);
+          ltv.push_back(e.second.second);
+        }
+        sl.push_back((int)lt.size());
+        n_lt += (double)gr.lt[x].size();
+        kcost[kq] += 2.0 * (double)gr.lt[x].size();
+      }
+      gslot.push_back((int)sl.size() - 1);
+      jto.push_back((int)jts.size());
+      for (auto &rowv : gr.jt) {
+        for (auto &e : rowv) {
+          jts.push_back(e.first);
+          jtc.push_back(e.second.first);
+          jtc.push_back(e.second.second);
+        }
+        jto.push_back((int)jts.size());
+        n_jt += (double)rowv.size();
+        if (!rowv.empty()) {
+          frad += d.sym ? d.h_rad_low[gr.ilm] : d.h_rad_off[gr.ilm];
+          kcost[kq] += (double)rowv.size() + 24.0;
+        }
+      }
+    }
+    kg.push_back((int)gilm.size());
+  }
+  // workgroups: element pairs of the launch dealt to XCDs (pair index % 8),
+  // columns in decreasing cost within each pair
+  std::vector<int> efs;
+  for (int e = 0; e < Nel; ++e)
+    for (int f = 0; f < Nel; ++f)
+      if (d.sym ? e > f : e != f) efs.push_back(e * Nel + f);
+  std::vector<int> korder(ks.size());
+  for (size_t q = 0; q < ks.size(); ++q) korder[q] = (int)q;
+  std::sort(korder.begin(), korder.end(), [&](int a, int b) { return kcost[a] > kcost[b]; });
+  std::vector<std::vector<int>> per_xcd(8);
+  for (size_t p = 0; p < efs.size(); ++p)
+    for (int kq : korder) {
+      auto &v = per_xcd[p % 8];
+      v.push_back(kq);
+      v.push_back(efs[p]);
+    }
+  size_t mx = 0;
+  for (auto &v : per_xcd) mx = std::max(mx, v.size() / 2);
+  std::vector<int> wg(mx * 8 * 2, -1);
+  for (int x = 0; x < 8; ++x)
+    for (size_t q = 0; q < per_xcd[x].size() / 2; ++q) {
+      wg[(q * 8 + x) * 2] = per_xcd[x][2 * q];
+      wg[(q * 8 + x) * 2 + 1] = per_xcd[x][2 * q + 1];
+    }
+  if (lt.empty()) {
+    lt.push_back(0);
+    ltv.assign(2, 0.0);
+  }
+  if (jts.empty()) {
+    jts.push_back(0);
+    jtc.assign(2, 0.0);
+  }
+  if (gilm.empty()) {
+    gilm.push_back(0);
+    gjt.push_back(0);
+  }
+  d.k2_wg.upload(wg, s);
+  d.k2_kshell.upload(kshell, s);
+  d.k2_kj.upload(kj, s);
+  d.k2_jshell.upload(jshell, s);
+  d.k2_kg.upload(kg, s);
+  d.k2_gilm.upload(gilm, s);
+  d.k2_gslot.upload(gslot, s);
+  d.k2_gjt.upload(gjt, s);
+  d.k2_sl.upload(sl, s);
+  d.k2_lt.upload(lt, s);
+  d.k2_ltv.upload(ltv, s);
+  d.k2_jto.upload(jto, s);
+  d.k2_jts.upload(jts, s);
+  d.k2_jtc.upload(jtc, s);
+  d.n_wg_k2 = (int)(mx * 8);
+  // per element pair: 4 flops per lt entry (two channels) and 8 per jt entry
+  // (four channels) on every tile entry, plus the radial products
+  double sum_off = 0.0;
+  for (int e = 0; e < Nel; ++e)
+    for (int f = 0; f < Nel; ++f)
+      if (d.sym ? e > f : e != f) sum_off += (double)S.rad.ni[e] * S.rad.ni[f];
+  d.flops_off_k2 = frad + (4.0 * n_lt + 8.0 * n_jt) * sum_off;
+  d.k2 = std::getenv("HFGPU_EXCHANGE_TWOSTEP") == nullptr || std::atoi(std::getenv("HFGPU_EXCHANGE_TWOSTEP")) != 0;
+}
+
 // Exchange coupling lists for the density blocks that pass the norm screen
 // (|P_il|_F >= 10 eps, basis.cpp:1858-1875).  For each output block (j,k):
 // runs of terms per (L,|M|), each term a density block (i,l) with
@@ -531,6 +741,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
     d.n_src = (int)src.size();
     d.src_blk.upload(src, s);
     d.term_tile.upload(tt, s);
+    build_twostep(h, act, id, bydm, ilm_of, fac_of, Mext, dmoff, s);
   }
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");

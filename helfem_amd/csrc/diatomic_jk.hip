@@ -596,6 +596,158 @@ __global__ void __launch_bounds__(NT) k_dia_k_in(DKArgs<T> A) {
   dia_k_body<true, TILED, T>(A);
 }
 
+// --- Cross-element exchange in two steps ----------------------------------------
+//
+// One workgroup per (column shell k, element pair (e,f)); each wave owns up to
+// three rows j of the column and keeps their output tiles in registers over
+// all coupled (L,|M|) groups of k.  Per group (and parity class of l_i): the
+// slot tiles V_b(i) = sum_l v_b(k,l) P(i,l) of the pair are formed in LDS
+// (waves split the slots), then each owned row gathers its four channel tiles
+// R00 = sum c0 V0, R02 = -sum c0 V1, R20 = -sum c1 V0, R22 = sum c1 V1 from LDS
+// and applies the disjoint P/Q factors as k_dia_k_off does.  The rows share
+// each V tile, so the L2 gathers drop from |S_j| |S_k| per row and group to
+// |S_k| per slot.
+
+template <typename T>
+struct DK2Args {
+  const T *pt;  // tiled density
+  int n_src;
+  double *K;
+  int64_t ldk;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const T *pq;
+  const int *wg, *kshell, *kj, *jshell, *kg, *gilm, *gslot, *gjt, *sl, *lt, *jto, *jts;
+  const T *ltv, *jtc;
+  int sym;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(K2_THREADS) k_dia_k2_off(DK2Args<T> A) {
+  typedef Mma<T> MM;
+  typedef typename MM::acc_t acc_t;
+  typedef acc_t vec;
+  constexpr int NW = K2_THREADS / 64, JPW = K2_JMAX / NW;
+  __shared__ T V[K2_SLOTS * 2 * TILE2];  // slot tiles V0, V1 in lane-major operand order
+  __shared__ T F[4 * TILE2];             // i0, i2 (row side), j0, j2 (column side) of the group
+  const int w = blockIdx.x;
+  const int kq = A.wg[2 * w], ef = A.wg[2 * w + 1];
+  if (kq < 0) return;
+  const int e = ef / A.Nel, f = ef - e * A.Nel;
+  const int k = A.kshell[kq], r0 = A.kj[kq], nr = A.kj[kq + 1] - r0;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  int kk[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) kk[s] = MM::row(g, s);
+  acc_t acc[JPW];  // one accumulator per owned row (registers: three rows per wave)
+#pragma unroll
+  for (int r = 0; r < JPW; ++r) acc[r] = acc_t{0, 0, 0, 0};
+  const T *pt = A.pt + (size_t)ef * A.n_src * TILE2 + lane * 4;
+  const bool up = e > f;
+  for (int gi = A.kg[kq]; gi < A.kg[kq + 1]; ++gi) {
+    const int ilm = A.gilm[gi], s0 = A.gslot[gi], ns = A.gslot[gi + 1] - s0;
+    __syncthreads();  // the previous group's LDS tiles are no longer read
+    {
+      // row side: Q of e if e > f else P of e; column side the other kind
+      const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2, *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
+      const T *src[4] = {te + (up ? 2 : 0) * TILE2, te + (up ? 3 : 1) * TILE2, tf + (up ? 0 : 2) * TILE2,
+                         tf + (up ? 1 : 3) * TILE2};
+      for (int u = t; u < 4 * TILE2; u += K2_THREADS) F[u] = src[u / TILE2][u % TILE2];
+    }
+    for (int sq = wv; sq < ns; sq += NW) {
+      vec v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+      const int x1 = A.sl[s0 + sq + 1];
+      int x = A.sl[s0 + sq];
+      for (; x + 2 <= x1; x += 2) {
+        const vec ta = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x] * TILE2);
+        const vec tb = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x + 1] * TILE2);
+        v0 += A.ltv[2 * x] * ta + A.ltv[2 * x + 2] * tb;
+        v1 += A.ltv[2 * x + 1] * ta + A.ltv[2 * x + 3] * tb;
+      }
+      if (x < x1) {
+        const vec ta = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x] * TILE2);
+        v0 += A.ltv[2 * x] * ta;
+        v1 += A.ltv[2 * x + 1] * ta;
+      }
+      *reinterpret_cast<vec *>(V + (size_t)(2 * sq) * TILE2 + lane * 4) = v0;
+      *reinterpret_cast<vec *>(V + (size_t)(2 * sq + 1) * TILE2 + lane * 4) = v1;
+    }
+    __syncthreads();
+    T a0[4], a2[4], b0[4], b2[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int x = c + TILE * kk[s];
+      a0[s] = F[x];
+      a2[s] = F[TILE2 + x];
+      b0[s] = F[2 * TILE2 + x];
+      b2[s] = F[3 * TILE2 + x];
+    }
+    const int *jo = A.jto + A.gjt[gi];
+#pragma unroll
+    for (int r = 0; r < JPW; ++r) {
+      const int q = wv + NW * r;
+      if (q >= nr) break;
+      const int y0 = jo[q], y1 = jo[q + 1];
+      if (y0 == y1) continue;
+      T r00[4] = {0, 0, 0, 0}, r02[4] = {0, 0, 0, 0}, r20[4] = {0, 0, 0, 0}, r22[4] = {0, 0, 0, 0};
+      for (int y = y0; y < y1; ++y) {
+        const int slq = A.jts[y];
+        const T c0 = A.jtc[2 * y], c1 = A.jtc[2 * y + 1];
+        const vec V0 = *reinterpret_cast<const vec *>(V + (size_t)(2 * slq) * TILE2 + lane * 4);
+        const vec V1 = *reinterpret_cast<const vec *>(V + (size_t)(2 * slq + 1) * TILE2 + lane * 4);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          r00[s] += c0 * V0[s];
+          r02[s] -= c0 * V1[s];
+          r20[s] -= c1 * V0[s];
+          r22[s] += c1 * V1[s];
+        }
+      }
+      acc_t ya = {0, 0, 0, 0}, yb = {0, 0, 0, 0};
+      mma_pair<T>(r00, r20, b0, ya, yb);
+      mma_pair<T>(r02, r22, b2, ya, yb);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc[r] = MM::step(a0[s], ya[s], acc[r]);  // acc += i0 ya
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc[r] = MM::step(a2[s], yb[s], acc[r]);  // acc += i2 yb
+    }
+  }
+  // the owned rows' tiles (lane holds entry (row(g,s), c)) and their mirrors
+  const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
+  const int b = c;
+  const long long col = pure_row(A.skip, A.poff, k, fj + min(b, nj - 1));
+  const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+#pragma unroll
+  for (int r = 0; r < JPW; ++r) {
+    const int q = wv + NW * r;
+    if (q >= nr) break;
+    const acc_t v = acc[r];
+    const int j = A.jshell[r0 + q];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int a = kk[rr];
+      if (a >= ni || b >= nj || col < 0) continue;
+      const long long row = pure_row(A.skip, A.poff, j, fi + a);
+      if (row < 0) continue;
+      const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
+      double *dst = A.K + row + col * A.ldk;
+      if (srow || scol)
+        atomicAdd(dst, -(double)v[rr]);
+      else
+        *dst = -(double)v[rr];
+      if (A.sym) {
+        double *mdst = A.K + col + row * A.ldk;
+        if (srow || scol)
+          atomicAdd(mdst, -(double)v[rr]);
+        else
+          *mdst = -(double)v[rr];
+      }
+    }
+  }
+}
+
 // --- In-element exchange with the Cholesky factors resident in LDS -------------
 //
 // One 1024-thread workgroup per ((L,|M|), element e, batch of runs of that
@@ -1013,6 +1165,16 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
           hipLaunchKernelGGL((k_dia_kin_lds<double, false>), dim3(d.n_wg_kl), dim3(KL_THREADS), 0, s, a);
       }
       hip_check(hipGetLastError(), "diatomic_kin_lds");
+      continue;
+    }
+    if (part == 0 && d.k2 && !fp32) {
+      TimedLaunch tl(timer, name, s, d.flops_off_k2);
+      DK2Args<double> a{d.ptile.p, d.n_src, K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.pq.p,
+                        d.k2_wg.p, d.k2_kshell.p, d.k2_kj.p, d.k2_jshell.p, d.k2_kg.p, d.k2_gilm.p, d.k2_gslot.p,
+                        d.k2_gjt.p, d.k2_sl.p, d.k2_lt.p, d.k2_jto.p, d.k2_jts.p, d.k2_ltv.p, d.k2_jtc.p,
+                        (int)d.sym};
+      hipLaunchKernelGGL((k_dia_k2_off<double>), dim3(d.n_wg_k2), dim3(K2_THREADS), 0, s, a);
+      hip_check(hipGetLastError(), "diatomic_k2_off");
       continue;
     }
     const int n_ef_pad = ((n_ef + 7) / 8) * 8;

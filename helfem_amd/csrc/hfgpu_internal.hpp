@@ -15,11 +15,15 @@ namespace hfg {
 // functions), the MFMA f64 tile edge on CDNA.
 constexpr int TILE = 16;
 constexpr int TILE2 = TILE * TILE;
+// operator extent per channel: c + 15 d for c, d < 15, padded to 4
 constexpr int KC = 228;
 // largest Cholesky rank the LDS-resident in-element diatomic exchange holds
 constexpr int KL_RANK_MAX = 32;
 // runs per workgroup of that kernel
-constexpr int KL_BATCH = 64;  // operator extent per channel: c + 15 d for c, d < 15, padded to 4
+constexpr int KL_BATCH = 64;
+// two-step cross-element diatomic exchange: slot tiles per (L,|M|) group in
+// LDS, rows per column shell (three per wave), threads per workgroup
+constexpr int K2_SLOTS = 17, K2_JMAX = 24, K2_THREADS = 512;
 
 struct Error : std::runtime_error {
   int code;
@@ -294,6 +298,20 @@ struct DiatomicDevice {
   bool f32_tables = false;
   int64_t list_gen = 0, term_f32_gen = -1; // exchange-list version, and the one term_f32 holds
   int n_wg_in = 0;
+  // Cross-element exchange in two steps (k_dia_k2_off), used when every
+  // column block has at most K2_JMAX coupled rows and every group at most
+  // K2_SLOTS slots: one workgroup per (column shell k, element pair); per
+  // coupled ((L,|M|), parity of l_i) group of k, slot tiles V_b(i) = sum_l
+  // v_b(k,l) P(i,l) (b: mod / plain Gaunt channel) in LDS, then each row j's
+  // four channel tiles R = fac sum_i u(j,i) (x) V(i) and its radial products.
+  bool k2 = false;
+  int n_wg_k2 = 0;
+  DevArray<int> k2_wg;                    // per workgroup: (k index, element pair) or (-1, -1)
+  DevArray<int> k2_kshell, k2_kj, k2_jshell, k2_kg;  // per k: shell, rows (offsets + shells), groups
+  DevArray<int> k2_gilm, k2_gslot, k2_gjt;           // per group: ilm, slot range, offset into k2_jto
+  DevArray<int> k2_sl, k2_lt, k2_jto, k2_jts;        // slot lt ranges; lt tile ids; per (group,row) jt ranges; jt slots
+  DevArray<double> k2_ltv, k2_jtc;                   // (v0, v1) per lt; (fac u0, fac u1) per jt
+  double flops_off_k2 = 0.0;
 };
 
 

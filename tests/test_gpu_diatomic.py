@@ -168,6 +168,27 @@ def test_symmetric_and_general_density_paths(name, gpu_ctx, monkeypatch):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_single_step_cross_element_kernel(name, gpu_ctx, monkeypatch):
+    """HFGPU_EXCHANGE_TWOSTEP=0 keeps the one-step cross-element kernel (one
+    gather per term and element pair) instead of the two-step one (shared slot
+    tiles per column and (L,|M|)); both meet the oracle bar for symmetric and
+    general densities."""
+    import torch
+    print(f"[gpu] diatomic K, one-step cross-element kernel: {name}", flush=True)
+    monkeypatch.setenv("HFGPU_EXCHANGE_TWOSTEP", "0")
+    Z1, Z2, R, lmm, nel = CASES[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    ob.compute_tei()
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    hb.compute_tei()
+    rng = np.random.default_rng(1011)
+    for P in (random_symmetric(ob.Nbf, seed=1012), rng.standard_normal((ob.Nbf, ob.Nbf))):
+        Kg = hb.exchange(torch.as_tensor(P, device="cuda")).cpu().numpy()
+        assert relerr(Kg, ob.exchange(P)) <= TOL
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("name", ["N2-l432-3el", "HeH-l2-1el"])
 def test_direct_gather_without_tile_budget(name, gpu_ctx, monkeypatch):
     """With no memory granted for the tiled density (HFGPU_EXCHANGE_TILE_GIB=0)
