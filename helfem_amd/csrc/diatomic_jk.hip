@@ -630,7 +630,6 @@ __global__ void __launch_bounds__(K2_THREADS) k_dia_k2_off(DK2Args<T> A) {
   typedef acc_t vec;
   constexpr int NW = K2_THREADS / 64, JPW = K2_JMAX / NW;
   __shared__ T V[K2_SLOTS * 2 * TILE2];  // slot tiles V0, V1 in lane-major operand order
-  __shared__ T F[4 * TILE2];             // i0, i2 (row side), j0, j2 (column side) of the group
   const int w = blockIdx.x;
   const int kq = A.wg[2 * w], ef = A.wg[2 * w + 1];
   if (kq < 0) return;
@@ -648,25 +647,38 @@ __global__ void __launch_bounds__(K2_THREADS) k_dia_k2_off(DK2Args<T> A) {
   const bool up = e > f;
   for (int gi = A.kg[kq]; gi < A.kg[kq + 1]; ++gi) {
     const int ilm = A.gilm[gi], s0 = A.gslot[gi], ns = A.gslot[gi + 1] - s0;
-    __syncthreads();  // the previous group's LDS tiles are no longer read
+    // the group's factor operands are fetched first and used after the slot
+    // phase: row side Q of e if e > f else P of e, column side the other kind
+    T a0[4], a2[4], b0[4], b2[4];
     {
-      // row side: Q of e if e > f else P of e; column side the other kind
       const T *te = A.pq + ((size_t)ilm * A.Nel + e) * 4 * TILE2, *tf = A.pq + ((size_t)ilm * A.Nel + f) * 4 * TILE2;
-      const T *src[4] = {te + (up ? 2 : 0) * TILE2, te + (up ? 3 : 1) * TILE2, tf + (up ? 0 : 2) * TILE2,
-                         tf + (up ? 1 : 3) * TILE2};
-      for (int u = t; u < 4 * TILE2; u += K2_THREADS) F[u] = src[u / TILE2][u % TILE2];
+      const T *i0 = te + (up ? 2 : 0) * TILE2, *i2 = te + (up ? 3 : 1) * TILE2;
+      const T *j0 = tf + (up ? 0 : 2) * TILE2, *j2 = tf + (up ? 1 : 3) * TILE2;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int x = c + TILE * kk[s];
+        a0[s] = i0[x];
+        a2[s] = i2[x];
+        b0[s] = j0[x];
+        b2[s] = j2[x];
+      }
     }
+    __syncthreads();  // the previous group's LDS tiles are no longer read
     for (int sq = wv; sq < ns; sq += NW) {
       vec v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
       const int x1 = A.sl[s0 + sq + 1];
       int x = A.sl[s0 + sq];
-      for (; x + 2 <= x1; x += 2) {
-        const vec ta = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x] * TILE2);
-        const vec tb = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x + 1] * TILE2);
-        v0 += A.ltv[2 * x] * ta + A.ltv[2 * x + 2] * tb;
-        v1 += A.ltv[2 * x + 1] * ta + A.ltv[2 * x + 3] * tb;
+      for (; x + 4 <= x1; x += 4) {
+        vec tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tv[u] = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x + u] * TILE2);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v0 += A.ltv[2 * (x + u)] * tv[u];
+          v1 += A.ltv[2 * (x + u) + 1] * tv[u];
+        }
       }
-      if (x < x1) {
+      for (; x < x1; ++x) {
         const vec ta = *reinterpret_cast<const vec *>(pt + (size_t)A.lt[x] * TILE2);
         v0 += A.ltv[2 * x] * ta;
         v1 += A.ltv[2 * x + 1] * ta;
@@ -675,15 +687,6 @@ __global__ void __launch_bounds__(K2_THREADS) k_dia_k2_off(DK2Args<T> A) {
       *reinterpret_cast<vec *>(V + (size_t)(2 * sq + 1) * TILE2 + lane * 4) = v1;
     }
     __syncthreads();
-    T a0[4], a2[4], b0[4], b2[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int x = c + TILE * kk[s];
-      a0[s] = F[x];
-      a2[s] = F[TILE2 + x];
-      b0[s] = F[2 * TILE2 + x];
-      b2[s] = F[3 * TILE2 + x];
-    }
     const int *jo = A.jto + A.gjt[gi];
 #pragma unroll
     for (int r = 0; r < JPW; ++r) {
