@@ -312,7 +312,7 @@ def main():
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k_off_f64 + k_atomic_k_in_f64",
-                                                 "diatomic": "k_dia_k_off<double> + k_dia_kin_lds<double>"}[kind] +
+                                                 "diatomic": "k_dia_k2_off<double> + k_dia_kin_lds<double>"}[kind] +
                                                 " (exchange: cross- and in-element launches of one build)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),

@@ -5,8 +5,6 @@
 #include <cstdlib>
 #include <cmath>
 #include <memory>
-#include <map>
-#include <set>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -319,7 +317,7 @@ void diatomic_upload(hf_diatomic *h) {
 // fac u0, fac u1).  Leaves d.k2 off when a column has more than K2_JMAX rows
 // or a group more than K2_SLOTS slots.
 void build_twostep(hf_diatomic *h, const std::vector<int> &act, const std::vector<int> &tile_of,
-                   const std::vector<std::vector<int>> &bydm, const std::vector<int> &ilm_of,
+                   const std::vector<unsigned char> &mask, const std::vector<int> &ilm_of,
                    const std::vector<double> &fac_of, int Mext, int dmoff, hipStream_t s) {
   const DiatomicSetup &S = h->host;
   DiatomicDevice &d = h->dev;
@@ -344,68 +342,70 @@ void build_twostep(hf_diatomic *h, const std::vector<int> &act, const std::vecto
     std::vector<std::vector<std::pair<int, std::pair<double, double>>>> lt;  // per slot: (tile, v)
     std::vector<std::vector<std::pair<int, std::pair<double, double>>>> jt;  // per row: (slot, c)
   };
+  const int nlm = d.n_lm, gL = d.gL;
+  std::vector<std::vector<int>> sector(2 * dmoff + 1);  // shells by m + dmoff
+  for (int a = 0; a < Na; ++a) sector[mv[a] + dmoff].push_back(a);
   std::vector<std::vector<Group>> groups(ks.size());
   bool fits = true;
 #pragma omp parallel for schedule(dynamic)
   for (size_t kq = 0; kq < ks.size(); ++kq) {
-    const int k = ks[kq], lk = lv[k], mk = mv[k];
+    const int k = ks[kq], mk = mv[k];
     const std::vector<int> &js = rows[k];
-    std::map<std::pair<int, int>, int> gid;  // (ilm, parity of l_i) -> group
     std::vector<Group> &G = groups[kq];
-    std::vector<std::map<std::pair<int, int>, int>> slot_of;
-    std::vector<std::vector<std::set<int>>> lseen;
-    std::vector<std::vector<std::map<int, int>>> jseen;
-    for (size_t q = 0; q < js.size(); ++q) {
-      const int j = js[q], lj = lv[j], mj = mv[j], dm = mj - mk;
-      if (std::abs(dm) > 2 * (dmoff / 2)) continue;
-      for (int il : bydm[dm + dmoff]) {
-        const int i = il / Na, l = il - i * Na;
-        const int li = lv[i], mi = mv[i], ll = lv[l], M = mj - mi;
-        const int Lmin = std::max(std::max(std::abs(li - lj), std::abs(lk - ll)) - 2, std::abs(M));
-        const int Lmax = std::min(li + lj, lk + ll) + 2;
-        const size_t gji = ((size_t)j * Na + i) * d.gL, gkl = ((size_t)k * Na + l) * d.gL;
-        for (int L = Lmin; L <= Lmax; ++L) {
-          const double u0 = d.h_g0[gji + L], u1 = d.h_g2[gji + L];
-          const double v0 = d.h_g0[gkl + L], v1 = d.h_g2[gkl + L];
-          if ((u0 == 0.0 && u1 == 0.0) || (v0 == 0.0 && v1 == 0.0)) continue;
-          const size_t lm = (size_t)L * Mext + M + dmoff;
-          const int ilm = ilm_of[lm];
-          const double fac = fac_of[lm];
-          const auto gk = std::make_pair(ilm, li & 1);
-          auto it = gid.find(gk);
-          int g;
-          if (it == gid.end()) {
-            g = (int)G.size();
-            gid[gk] = g;
-            G.push_back(Group{ilm, li & 1, {}, {},
-                              std::vector<std::vector<std::pair<int, std::pair<double, double>>>>(js.size())});
-            slot_of.emplace_back();
-            lseen.emplace_back();
-            jseen.emplace_back(js.size());
-          } else {
-            g = it->second;
-          }
-          Group &gr = G[g];
-          auto sk = std::make_pair(M, i);
-          auto sit = slot_of[g].find(sk);
-          int sl;
-          if (sit == slot_of[g].end()) {
-            sl = (int)gr.slots.size();
-            slot_of[g][sk] = sl;
-            gr.slots.push_back(sk);
-            gr.lt.emplace_back();
-            lseen[g].emplace_back();
-          } else {
-            sl = sit->second;
-          }
-          if (lseen[g][sl].insert(l).second) gr.lt[sl].push_back({tile_of[(size_t)i * Na + l], {v0, v1}});
-          if (jseen[g][q].find(sl) == jseen[g][q].end()) {
-            jseen[g][q][sl] = 1;
+    std::vector<int> gidx((size_t)2 * nlm, -1);
+    std::vector<int> Lk;
+    for (int L = 0; L < gL; ++L)
+      for (int M = -std::min(L, dmoff); M <= std::min(L, dmoff); ++M) {
+        const int ml = mk - M;
+        if (std::abs(ml) > dmoff / 2) continue;
+        Lk.clear();
+        for (int l : sector[ml + dmoff]) {
+          const size_t x = ((size_t)k * Na + l) * gL + L;
+          if (d.h_g0[x] != 0.0 || d.h_g2[x] != 0.0) Lk.push_back(l);
+        }
+        if (Lk.empty()) continue;
+        const size_t lm = (size_t)L * Mext + M + dmoff;
+        const int ilm = ilm_of[lm];
+        const double fac = fac_of[lm];
+        for (size_t q = 0; q < js.size(); ++q) {
+          const int j = js[q], mi = mv[j] - M;
+          if (std::abs(mi) > dmoff / 2) continue;
+          for (int i : sector[mi + dmoff]) {
+            const size_t xu = ((size_t)j * Na + i) * gL + L;
+            const double u0 = d.h_g0[xu], u1 = d.h_g2[xu];
+            if (u0 == 0.0 && u1 == 0.0) continue;
+            const int par = lv[i] & 1;
+            int &g = gidx[(size_t)2 * ilm + par];
+            if (g < 0) {
+              g = (int)G.size();
+              G.push_back(Group{ilm, par, {}, {},
+                                std::vector<std::vector<std::pair<int, std::pair<double, double>>>>(js.size())});
+            }
+            Group &gr = G[g];
+            int sl = -1;
+            for (size_t z = 0; z < gr.slots.size(); ++z)
+              if (gr.slots[z].first == M && gr.slots[z].second == i) {
+                sl = (int)z;
+                break;
+              }
+            if (sl < 0) {
+              // the slot's density blocks (i,l) that pass the screen
+              std::vector<std::pair<int, std::pair<double, double>>> ltv;
+              for (int l : Lk) {
+                const int tl = mask[(size_t)i * Na + l] ? tile_of[(size_t)i * Na + l] : -1;
+                if (tl < 0) continue;
+                const size_t x = ((size_t)k * Na + l) * gL + L;
+                ltv.push_back({tl, {d.h_g0[x], d.h_g2[x]}});
+              }
+              if (ltv.empty()) continue;
+              sl = (int)gr.slots.size();
+              gr.slots.push_back({M, i});
+              gr.lt.push_back(std::move(ltv));
+            }
             gr.jt[q].push_back({sl, {fac * u0, fac * u1}});
           }
         }
       }
-    }
     std::sort(G.begin(), G.end(),
               [](const Group &a, const Group &b) { return a.ilm < b.ilm || (a.ilm == b.ilm && a.par < b.par); });
     for (const Group &gr : G)
@@ -741,7 +741,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
     d.n_src = (int)src.size();
     d.src_blk.upload(src, s);
     d.term_tile.upload(tt, s);
-    build_twostep(h, act, id, bydm, ilm_of, fac_of, Mext, dmoff, s);
+    build_twostep(h, act, id, mask, ilm_of, fac_of, Mext, dmoff, s);
   }
   d.term_f.upload(tf, s);
   hip_check(hipStreamSynchronize(s), "exchange list upload");
