@@ -376,20 +376,16 @@ void build_twostep(hf_diatomic *h, const std::vector<int> &act, const std::vecto
             if (u0 == 0.0 && u1 == 0.0) continue;
             const int par = lv[i] & 1;
             int &g = gidx[(size_t)2 * ilm + par];
-            if (g < 0) {
-              g = (int)G.size();
-              G.push_back(Group{ilm, par, {}, {},
-                                std::vector<std::vector<std::pair<int, std::pair<double, double>>>>(js.size())});
-            }
-            Group &gr = G[g];
             int sl = -1;
-            for (size_t z = 0; z < gr.slots.size(); ++z)
-              if (gr.slots[z].first == M && gr.slots[z].second == i) {
-                sl = (int)z;
-                break;
-              }
+            if (g >= 0)
+              for (size_t z = 0; z < G[g].slots.size(); ++z)
+                if (G[g].slots[z].first == M && G[g].slots[z].second == i) {
+                  sl = (int)z;
+                  break;
+                }
             if (sl < 0) {
-              // the slot's density blocks (i,l) that pass the screen
+              // the slot's density blocks (i,l) that pass the screen; a group
+              // is opened only with its first non-empty slot
               std::vector<std::pair<int, std::pair<double, double>>> ltv;
               for (int l : Lk) {
                 const int tl = mask[(size_t)i * Na + l] ? tile_of[(size_t)i * Na + l] : -1;
@@ -398,10 +394,16 @@ void build_twostep(hf_diatomic *h, const std::vector<int> &act, const std::vecto
                 ltv.push_back({tl, {d.h_g0[x], d.h_g2[x]}});
               }
               if (ltv.empty()) continue;
-              sl = (int)gr.slots.size();
-              gr.slots.push_back({M, i});
-              gr.lt.push_back(std::move(ltv));
+              if (g < 0) {
+                g = (int)G.size();
+                G.push_back(Group{ilm, par, {}, {},
+                                  std::vector<std::vector<std::pair<int, std::pair<double, double>>>>(js.size())});
+              }
+              sl = (int)G[g].slots.size();
+              G[g].slots.push_back({M, i});
+              G[g].lt.push_back(std::move(ltv));
             }
+            Group &gr = G[g];
             gr.jt[q].push_back({sl, {fac * u0, fac * u1}});
           }
         }
