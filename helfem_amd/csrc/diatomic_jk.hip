@@ -834,9 +834,9 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
       r20[s] = -r20[s];
     }
     acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
-    // sigma_p = +-1 is applied to the X operands (exact), so the products
-    // of consecutive p are independent chains the scheduler can interleave
-#pragma unroll 2
+    // sigma_p = +-1 is applied to the X operands (exact).  The loop stays
+    // rolled: the 16 waves of the workgroup keep the matrix pipe busy, and
+    // two vectors per step measured no faster.
     for (int p = 0; p < rank; ++p) {
       const T *M0 = Mt + 2 * p * TL, *M2 = M0 + TL;
       const T sp = sg[p];
