@@ -406,28 +406,20 @@ class DiatomicBasis:
         _check(lib().orc_diatomic_exchange(self._h, _ptr(P), _ptr(K)))
         return K
 
+    # The reference's diatomic basis has no range-separated exchange
+    # (src/diatomic/basis.h declares none); the atomic oracle entry points
+    # take an atomic handle, so these must not forward to them.
     def compute_yukawa(self, lam):
-        """Yukawa range-separation caches (TwoDBasis.cpp:738-760)."""
-        _check(lib().orc_atomic_compute_yukawa(self._h, float(lam)))
+        raise NotImplementedError("range-separated exchange exists for atomic bases only")
 
     def compute_erfc(self, mu):
-        """erfc range-separation caches (TwoDBasis.cpp:762-771)."""
-        _check(lib().orc_atomic_compute_erfc(self._h, float(mu)))
+        raise NotImplementedError("range-separated exchange exists for atomic bases only")
 
     def rs_exchange(self, P):
-        """-K of the short-range kernel (TwoDBasis.cpp:1002-1131)."""
-        P = np.asfortranarray(P, dtype=np.float64)
-        K = np.zeros((self.Nbf, self.Nbf), order="F")
-        _check(lib().orc_atomic_rs_exchange(self._h, _ptr(P), _ptr(K)))
-        return K
+        raise NotImplementedError("range-separated exchange exists for atomic bases only")
 
     def rs_cache(self, kind, index):
-        dims = np.zeros(2, dtype=np.int32)
-        _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, None))
-        out = np.zeros((int(dims[0]), int(dims[1])), order="F")
-        if out.size:
-            _check(lib().orc_atomic_rs_cache(self._h, kind, index, dims.ctypes.data, _ptr(out)))
-        return out
+        raise NotImplementedError("range-separated exchange exists for atomic bases only")
 
     def exchange_blocks(self, P, jk):
         """-K restricted to the output blocks j*Nang+k listed in jk."""
