@@ -43,6 +43,7 @@ SIGNATURES = {
     "hf_atomic_coulomb": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_atomic_coulomb_dev_noalloc": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_compute_yukawa": (c_int, [c_vp, c_dbl]),
     "hf_atomic_compute_erfc": (c_int, [c_vp, c_dbl]),

@@ -733,23 +733,35 @@ void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStre
   hip_check(hipGetLastError(), "fill_zero");
 }
 
+size_t atomic_coulomb_work_doubles(const AtomicDevice &d) {
+  return (size_t)std::max(d.n_lm, 1) * d.Nel * (TILE2 + 4);
+}
+
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
-                           hipStream_t s, KernelTimer *timer) {
+                           hipStream_t s, KernelTimer *timer, double *work) {
   const int64_t nbf = (int64_t)d.Nrad * d.Nang;
   fill_zero_launch(J, nbf, nbf, ldj, s);
   if (d.n_lm == 0) return;
+  // work space: the handle's buffers, or one caller-given block laid out as
+  // Jin [n_lm][Nel][TILE2], then jsm, jbg, preA, preB [n_lm][Nel] each
+  const size_t nq = (size_t)d.n_lm * d.Nel;
+  double *Jin = work ? work : d.Jin.p;
+  double *jsm = work ? work + nq * TILE2 : d.jsm.p;
+  double *jbg = work ? jsm + nq : d.jbg.p;
+  double *preA = work ? jbg + nq : d.preA.p;
+  double *preB = work ? preA + nq : d.preB.p;
   JArgs a{P, ldp, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.rs.p, d.rb.p, d.chol.p, d.chol_off.p,
-          d.chol_rank.p, d.lm_L.p, d.lm_pstart.p, d.pair_kl.p, d.pair_c.p, d.Jin.p, d.jsm.p, d.jbg.p};
+          d.chol_rank.p, d.lm_L.p, d.lm_pstart.p, d.pair_kl.p, d.pair_c.p, Jin, jsm, jbg};
   {
     TimedLaunch tl(timer, "atomic_j_gather", s);
     hipLaunchKernelGGL(k_atomic_j_gather, dim3(d.n_lm * d.Nel), dim3(NT), 0, s, a);
     hip_check(hipGetLastError(), "atomic_j_gather");
   }
-  hipLaunchKernelGGL(k_atomic_j_prefix, dim3((d.n_lm + 63) / 64), dim3(64), 0, s, d.n_lm, d.Nel, d.jsm.p,
-                     d.jbg.p, d.preA.p, d.preB.p);
+  hipLaunchKernelGGL(k_atomic_j_prefix, dim3((d.n_lm + 63) / 64), dim3(64), 0, s, d.n_lm, d.Nel, jsm, jbg,
+                     preA, preB);
   hip_check(hipGetLastError(), "atomic_j_prefix");
-  JOutArgs o{J, ldj, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.lm_L.p, d.rs.p, d.rb.p, d.Jin.p,
-             d.preA.p, d.preB.p, d.out_start.p, d.out_lm.p, d.out_c.p};
+  JOutArgs o{J, ldj, d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.lm_L.p, d.rs.p, d.rb.p, Jin,
+             preA, preB, d.out_start.p, d.out_lm.p, d.out_c.p};
   {
     TimedLaunch tl(timer, "atomic_j_expand", s);
     hipLaunchKernelGGL(k_atomic_j_expand, dim3(d.Nang * d.Nang * d.Nel), dim3(NT), 0, s, o);

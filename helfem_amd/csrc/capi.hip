@@ -305,6 +305,7 @@ int hf_ctx_create(int device, unsigned flags, hf_ctx **out) {
     DeviceGuard g(device);
     hip_check(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking), "hipStreamCreate");
     c->cur = c->own;
+    noalloc_scratch_init(device);
     *out = c.release();
   });
 }

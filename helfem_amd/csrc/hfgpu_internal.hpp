@@ -343,8 +343,14 @@ struct XCGridDevice {
 };
 
 // launchers (atomic_jk.hip)
+// work: null (the handle's buffers) or atomic_coulomb_work_doubles(d)
+// doubles of caller-owned device memory.
 void atomic_coulomb_launch(const AtomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
-                           hipStream_t s, KernelTimer *timer);
+                           hipStream_t s, KernelTimer *timer, double *work = nullptr);
+size_t atomic_coulomb_work_doubles(const AtomicDevice &d);
+// Device address of the allocation-free entry point's scratch array, looked
+// up for `device` (the current device); once per device, at context creation.
+void noalloc_scratch_init(int device);
 // fp32: gathers and matrix products in FP32 (the result stays FP64, with
 // about 1e-6 relative error); otherwise the reference's FP64 arithmetic.
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,

@@ -145,6 +145,16 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
 /* Device-pointer forms of the two above, asynchronous on the context stream. */
 int hf_atomic_coulomb_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 int hf_atomic_exchange_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+/* Allocation-free form of hf_atomic_coulomb_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops).  Its work space (the in-element J
+ * bands and multipole traces, n_lm * Nel * 260 doubles) comes from a
+ * 16 MiB array every GPU holds from module load to process end, so the call
+ * makes no allocation; a basis needing more returns HF_EINVAL.  The handle's
+ * tables must already be on the device (one ordinary J or K call uploads
+ * them; HF_ESTATE before).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (two streams or host threads) race. */
+int hf_atomic_coulomb_dev_noalloc(hf_atomic *h, const double *dP, int64_t ldp, double *dJ, int64_t ldj);
 /* Reduced-precision form of hf_atomic_exchange_dev for the early iterations
  * of a mixed-precision SCF: the density, the Gaunt weights and the radial
  * factors are rounded to FP32, gathered and multiplied on the FP32 matrix
