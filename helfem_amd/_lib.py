@@ -32,6 +32,7 @@ SIGNATURES = {
     "hf_atomic_work": (c_int, [c_vp, c_vp]),
     "hf_atomic_create": (c_int, [c_vp, c_int, c_int, c_int, c_dp, c_int, c_vp, c_vp, c_int, ctypes.POINTER(c_vp)]),
     "hf_atomic_destroy": (c_int, [c_vp]),
+    "hf_atomic_clone": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "hf_atomic_sizes": (c_int, [c_vp, c_vp]),
     "hf_atomic_elements": (c_int, [c_vp, c_vp, c_vp]),
     "hf_atomic_compute_tei": (c_int, [c_vp, c_int]),
@@ -44,6 +45,12 @@ SIGNATURES = {
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev_noalloc": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
+    "hf_atomic_batch_create": (c_int, [c_vp, c_vp, c_int, c_vp]),
+    "hf_atomic_batch_destroy": (c_int, [c_vp]),
+    "hf_atomic_batch_set_options": (c_int, [c_vp, c_int, c_int]),
+    "hf_atomic_batch_tuning": (c_int, [c_vp, c_vp, c_vp]),
+    "hf_atomic_batch_coulomb_dev": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "hf_atomic_batch_exchange_dev": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "hf_atomic_exchange_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_compute_yukawa": (c_int, [c_vp, c_dbl]),
     "hf_atomic_compute_erfc": (c_int, [c_vp, c_dbl]),

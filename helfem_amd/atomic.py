@@ -110,6 +110,16 @@ class AtomicBasis:
             _lib._LIB.hf_atomic_destroy(h)
             self._h = ctypes.c_void_p()
 
+    def clone(self):
+        """A second handle on this basis (hf_atomic_clone): same host setup
+        and caches, device tables of its own."""
+        h = ctypes.c_void_p()
+        check(lib().hf_atomic_clone(self._h, ctypes.byref(h)))
+        c = object.__new__(AtomicBasis)
+        c.__dict__.update({k: v for k, v in self.__dict__.items() if k != "_h"})
+        c._h = h
+        return c
+
     # --- setup ---------------------------------------------------------
     def compute_tei(self, exchange: bool = True):
         check(lib().hf_atomic_compute_tei(self._h, int(exchange)))
@@ -220,3 +230,75 @@ class AtomicBasis:
         if precision != "fp64":
             raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
         return self._apply(P, lib().hf_atomic_exchange, lib().hf_atomic_exchange_dev, out)
+
+
+def _ptr_array(tensors):
+    return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+
+
+def _ld_array(tensors):
+    return (ctypes.c_int64 * len(tensors))(*[t.shape[0] for t in tensors])
+
+
+class AtomicBatch:
+    """Coulomb and exchange builds of many atomic bases as single launches
+    (hf_atomic_batch: one launch per kernel for all systems, the launch
+    sequence replayed from a HIP graph, workgroup shapes autotuned).
+
+    The bases must share one Context and have compute_tei() done; a basis may
+    appear once (use AtomicBasis.clone() for a second density of the same
+    basis).  Matrices are float64 CUDA tensors on torch's current stream."""
+
+    def __init__(self, bases, ctx: Context):
+        self.bases = list(bases)
+        self.ctx = ctx
+        arr = (ctypes.c_void_p * len(self.bases))(*[b._h.value for b in self.bases])
+        h = ctypes.c_void_p()
+        ctx.bind_torch_stream()
+        check(lib().hf_atomic_batch_create(ctx.handle, arr, len(self.bases), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._LIB is not None:
+            _lib._LIB.hf_atomic_batch_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def set_options(self, use_graphs: bool = True, autotune: bool = True):
+        check(lib().hf_atomic_batch_set_options(self._h, int(use_graphs), int(autotune)))
+
+    def tuning(self):
+        """Autotuner state: waves per workgroup per (class, part), and the
+        measured launch times (ms) of the 1, 2, 4-wave shapes."""
+        nw = np.zeros(8, np.int32)
+        ms = np.zeros(24)
+        check(lib().hf_atomic_batch_tuning(self._h, nw.ctypes.data, ms.ctypes.data))
+        return nw.reshape(4, 2), ms.reshape(4, 2, 3)
+
+    def _check(self, Ps, outs):
+        if len(Ps) != len(self.bases) or len(outs) != len(self.bases):
+            raise ValueError("one density and one output per basis")
+        for b, P, O in zip(self.bases, Ps, outs):
+            for X in (P, O):
+                if X.shape != (b.Nbf, b.Nbf) or not X.is_contiguous() or not X.is_cuda:
+                    raise ValueError(f"matrices must be contiguous {b.Nbf}x{b.Nbf} CUDA tensors")
+
+    def coulomb(self, Ps, outs):
+        """outs[s] = J(Ps[s]) for every basis s."""
+        self._check(Ps, outs)
+        self.ctx.bind_torch_stream()
+        check(lib().hf_atomic_batch_coulomb_dev(self._h, _ptr_array(Ps), _ld_array(Ps), _ptr_array(outs),
+                                                _ld_array(outs)))
+        return outs
+
+    def exchange(self, Ps, outs, fp32=None):
+        """outs[s] = -K(Ps[s]) for every basis s; fp32[s] true selects the
+        reduced-precision form for that system."""
+        self._check(Ps, outs)
+        self.ctx.bind_torch_stream()
+        f = None
+        if fp32 is not None:
+            f = (ctypes.c_int * len(self.bases))(*[int(bool(x)) for x in fp32])
+        check(lib().hf_atomic_batch_exchange_dev(self._h, _ptr_array(Ps), _ld_array(Ps), _ptr_array(outs),
+                                                 _ld_array(outs), f))
+        return outs

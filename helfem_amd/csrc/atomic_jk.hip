@@ -24,6 +24,8 @@
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
 #include <cstdlib>
+#include <cstring>
+#include <type_traits>
 #include <cmath>
 
 namespace hfg {
@@ -32,19 +34,24 @@ namespace {
 using namespace dev;
 
 // Zero a rows x cols matrix; a contiguous one (ld == rows) as 16-byte stores.
-__global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
-  const int64_t n = rows * cols, stride = (int64_t)gridDim.x * NT;
+// Workgroup blk of nblk zeroes its grid-stride share.
+__device__ inline void fill_zero_body(double *A, int64_t rows, int64_t cols, int64_t ld, int64_t blk, int64_t nblk) {
+  const int64_t n = rows * cols, stride = nblk * NT;
   if (ld == rows && (reinterpret_cast<uintptr_t>(A) & 15) == 0) {
     double2 *A2 = reinterpret_cast<double2 *>(A);
     const int64_t n2 = n / 2;
-    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n2; q += stride) A2[q] = make_double2(0.0, 0.0);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) A[n - 1] = 0.0;
+    for (int64_t q = blk * NT + threadIdx.x; q < n2; q += stride) A2[q] = make_double2(0.0, 0.0);
+    if (blk == 0 && threadIdx.x == 0 && (n & 1)) A[n - 1] = 0.0;
     return;
   }
-  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < n; q += stride) {
+  for (int64_t q = blk * NT + threadIdx.x; q < n; q += stride) {
     const int64_t j = q / rows, i = q - j * rows;
     A[i + j * ld] = 0.0;
   }
+}
+
+__global__ void __launch_bounds__(NT) k_fill_zero(double *A, int64_t rows, int64_t cols, int64_t ld) {
+  fill_zero_body(A, rows, cols, ld, blockIdx.x, gridDim.x);
 }
 
 // --- Coulomb -----------------------------------------------------------------
@@ -61,9 +68,9 @@ struct JArgs {
   double *Jin, *jsm, *jbg;
 };
 
-// One workgroup per ((L,M), element).
-__global__ void __launch_bounds__(NT) k_atomic_j_gather(JArgs A) {
-  const int lm = blockIdx.x / A.Nel, e = blockIdx.x - lm * A.Nel;
+// One workgroup per ((L,M), element): blk = lm * Nel + e.
+__device__ inline void j_gather_body(const JArgs &A, int blk) {
+  const int lm = blk / A.Nel, e = blk - lm * A.Nel;
   const int t = threadIdx.x, a = t & 15, b = t >> 4;
   const int ni = A.ni[e], f0 = A.first[e], L = A.lm_L[lm];
   __shared__ double sP[TILE2];
@@ -107,10 +114,11 @@ __global__ void __launch_bounds__(NT) k_atomic_j_gather(JArgs A) {
   A.Jin[((size_t)lm * A.Nel + e) * TILE2 + t] = jin;
 }
 
+__global__ void __launch_bounds__(NT) k_atomic_j_gather(JArgs A) { j_gather_body(A, blockIdx.x); }
+
 // Cross-element weights: pre_a(lm,e) = sum_{j>e} jbig(j), pre_b = sum_{j<e} jsmall(j)
-__global__ void k_atomic_j_prefix(int n_lm, int Nel, const double *jsm, const double *jbg, double *preA,
-                                  double *preB) {
-  const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void j_prefix_body(int lm, int n_lm, int Nel, const double *jsm, const double *jbg,
+                                     double *preA, double *preB) {
   if (lm >= n_lm) return;
   const double *s = jsm + (size_t)lm * Nel;
   const double *g = jbg + (size_t)lm * Nel;
@@ -124,6 +132,11 @@ __global__ void k_atomic_j_prefix(int n_lm, int Nel, const double *jsm, const do
     preA[(size_t)lm * Nel + e] = acc;
     acc += g[e];
   }
+}
+
+__global__ void k_atomic_j_prefix(int n_lm, int Nel, const double *jsm, const double *jbg, double *preA,
+                                  double *preB) {
+  j_prefix_body(blockIdx.x * blockDim.x + threadIdx.x, n_lm, Nel, jsm, jbg, preA, preB);
 }
 
 struct JOutArgs {
@@ -145,9 +158,9 @@ __device__ inline double jband(const JOutArgs &A, int lm, int e, int t) {
   return v;
 }
 
-// One workgroup per (angular output block (i,j), element).
-__global__ void __launch_bounds__(NT) k_atomic_j_expand(JOutArgs A) {
-  const int ij = blockIdx.x / A.Nel, e = blockIdx.x - ij * A.Nel;
+// One workgroup per (angular output block (i,j), element): blk = ij * Nel + e.
+__device__ inline void j_expand_body(const JOutArgs &A, int blk) {
+  const int ij = blk / A.Nel, e = blk - ij * A.Nel;
   const int q0 = A.out_start[ij], q1 = A.out_start[ij + 1];
   if (q0 == q1) return;
   const int t = threadIdx.x, a = t & 15, b = t >> 4;
@@ -165,6 +178,8 @@ __global__ void __launch_bounds__(NT) k_atomic_j_expand(JOutArgs A) {
   const int i = ij / A.Nang, j = ij - i * A.Nang;
   A.J[(int64_t)(i * A.Nrad + f0 + a) + (int64_t)(j * A.Nrad + f0 + b) * A.ldj] = v;
 }
+
+__global__ void __launch_bounds__(NT) k_atomic_j_expand(JOutArgs A) { j_expand_body(A, blockIdx.x); }
 
 // --- Exchange ----------------------------------------------------------------
 
@@ -195,12 +210,12 @@ struct KArgs {
 // block of P between element e's radial functions of shell i and element f's
 // of shell l (src = i*Nang+l from the source list), lane-major in the
 // matrix-core operand order (entry lane*4 + s is P(c, row(g,s)) for lane
-// (g,c)), zero outside the elements.  One wave per tile.
+// (g,c)), zero outside the elements.  One wave per tile; wg is the
+// workgroup index (four tiles per workgroup).
 template <typename T>
-__global__ void __launch_bounds__(NT) k_atomic_ptile(const double *P, int64_t ldp, int Nel, int Nrad,
-                                                     const int *first, const int *ni, int Nang, const int *src_blk,
-                                                     int n_src, T *out) {
-  const int64_t tile = (int64_t)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+__device__ inline void ptile_body(const double *P, int64_t ldp, int Nel, int Nrad, const int *first, const int *ni,
+                                  int Nang, const int *src_blk, int n_src, T *out, int64_t wg) {
+  const int64_t tile = wg * (NT / 64) + (threadIdx.x >> 6);
   if (tile >= (int64_t)Nel * Nel * n_src) return;
   const int ef = (int)(tile / n_src), src = (int)(tile - (int64_t)ef * n_src);
   const int e = ef / Nel, f = ef - e * Nel;
@@ -217,6 +232,13 @@ __global__ void __launch_bounds__(NT) k_atomic_ptile(const double *P, int64_t ld
   *reinterpret_cast<typename Mma<T>::acc_t *>(out + tile * TILE2 + lane * 4) = v;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(NT) k_atomic_ptile(const double *P, int64_t ldp, int Nel, int Nrad,
+                                                     const int *first, const int *ni, int Nang, const int *src_blk,
+                                                     int n_src, T *out) {
+  ptile_body<T>(P, ldp, Nel, Nrad, first, ni, Nang, src_blk, n_src, out, blockIdx.x);
+}
+
 // One workgroup per (coupled output block (j,k), element pair (e,f)); the
 // four waves split the multipoles L.  For each L a wave forms the
 // Gaunt-weighted density tile R_L(e,f) = sum_terms c * P_il(e,f) directly in
@@ -229,12 +251,14 @@ __global__ void __launch_bounds__(NT) k_atomic_ptile(const double *P, int64_t ld
 // waves' tiles in FP64.  Workgroups sharing (e,f) run on one XCD.  The
 // density tiles come from the tiled copy (k_atomic_ptile: one aligned 2 KB
 // tile per (element pair, density block)) when it fits in memory, else
-// straight from P.
-template <bool INSIDE, bool TILED, typename T>
-__device__ inline void atomic_k_body(const KArgs<T> &A) {
+// straight from P.  NW (waves per workgroup, 1, 2 or 4) is the workgroup
+// shape: fewer waves suit bases with few multipoles (N_L < 4 leaves waves of
+// a four-wave group idle); w is the workgroup's index within its system's
+// grid (a multiple-of-8 offset of the launch's, so w & 7 is its XCD).
+template <bool INSIDE, bool TILED, typename T, int NW = 4>
+__device__ inline void atomic_k_body(const KArgs<T> &A, int w) {
   typedef Mma<T> MM;
   typedef typename MM::acc_t acc_t;
-  const int w = blockIdx.x;
   const int xcd = w & 7, loc = w >> 3;
   const int efi = (loc / A.n_act) * 8 + xcd;
   const int act = loc - (loc / A.n_act) * A.n_act;
@@ -245,7 +269,7 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
   const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int ni = A.ni[e], nj = A.ni[f], fi = A.first[e], fj = A.first[f];
-  __shared__ double red[4 * TILE2];
+  __shared__ double red[NW * TILE2];
   acc_t acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
   // Lanes outside the element pair read a clamped entry of the same tile and
   // are masked once per tile, so the term loop has no branches and its loads
@@ -261,7 +285,7 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
   }
   typedef typename MM::acc_t vec;
   const T *pt = TILED ? A.pt + (size_t)ef * A.n_src * TILE2 + lane * 4 : nullptr;
-  for (int L = wv; L < A.N_L; L += 4) {
+  for (int L = wv; L < A.N_L; L += NW) {
     const int q0 = A.term_start[act * A.N_L + L], q1 = A.term_start[act * A.N_L + L + 1];
     if (q0 == q1) continue;
     T r[4] = {0, 0, 0, 0};
@@ -356,29 +380,33 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
       }
     }
   }
-  // combine the four waves' partial tiles (lane holds K(row(g,r), c)) in FP64
+  // combine the waves' partial tiles (lane holds K(row(g,r), c)) in FP64
   acc += acc2;
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) red[wv * TILE2 + kk[rr] + TILE * c] = (double)acc[rr];
   __syncthreads();
-  const int a = t & 15, b = t >> 4;
-  if (a >= ni || b >= nj) return;
-  const double v = red[t] + red[TILE2 + t] + red[2 * TILE2 + t] + red[3 * TILE2 + t];
-  const int64_t row = (int64_t)j * A.Nrad + fi + a, col = (int64_t)k * A.Nrad + fj + b;
-  const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
-  const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
-  double *dst = A.K + row + col * A.ldk;
-  if (srow || scol)
-    atomicAdd(dst, -v);
-  else
-    *dst = -v;
-  if (A.sym && (!INSIDE || j != k)) {
-    // mirror tile (k, j, f, e) = this tile transposed
-    double *mdst = A.K + col + row * A.ldk;
+  for (int tt = t; tt < TILE2; tt += 64 * NW) {
+    const int a = tt & 15, b = tt >> 4;
+    if (a >= ni || b >= nj) continue;
+    double v = red[tt];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) v += red[q * TILE2 + tt];
+    const int64_t row = (int64_t)j * A.Nrad + fi + a, col = (int64_t)k * A.Nrad + fj + b;
+    const bool srow = (a == 0 && e > 0) || (a == ni - 1 && e + 1 < A.Nel);
+    const bool scol = (b == 0 && f > 0) || (b == nj - 1 && f + 1 < A.Nel);
+    double *dst = A.K + row + col * A.ldk;
     if (srow || scol)
-      atomicAdd(mdst, -v);
+      atomicAdd(dst, -v);
     else
-      *mdst = -v;
+      *dst = -v;
+    if (A.sym && (!INSIDE || j != k)) {
+      // mirror tile (k, j, f, e) = this tile transposed
+      double *mdst = A.K + col + row * A.ldk;
+      if (srow || scol)
+        atomicAdd(mdst, -v);
+      else
+        *mdst = -v;
+    }
   }
 }
 
@@ -387,19 +415,19 @@ __device__ inline void atomic_k_body(const KArgs<T> &A) {
 // compiler's register choice.
 template <bool TILED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_atomic_k_off_f64(KArgs<double> A) {
-  atomic_k_body<false, TILED, double>(A);
+  atomic_k_body<false, TILED, double>(A, blockIdx.x);
 }
 template <bool TILED>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_atomic_k_off_f32(KArgs<float> A) {
-  atomic_k_body<false, TILED, float>(A);
+  atomic_k_body<false, TILED, float>(A, blockIdx.x);
 }
 template <bool TILED>
 __global__ void __launch_bounds__(NT) k_atomic_k_in_f64(KArgs<double> A) {
-  atomic_k_body<true, TILED, double>(A);
+  atomic_k_body<true, TILED, double>(A, blockIdx.x);
 }
 template <bool TILED>
 __global__ void __launch_bounds__(NT) k_atomic_k_in_f32(KArgs<float> A) {
-  atomic_k_body<true, TILED, float>(A);
+  atomic_k_body<true, TILED, float>(A, blockIdx.x);
 }
 
 void launch_atomic_k(int part, bool tiled, int64_t grid, hipStream_t s, const KArgs<double> &a) {
@@ -1051,6 +1079,473 @@ void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, doubl
     }
     hip_check(hipGetLastError(), "atomic_k");
   }
+}
+
+
+// --- Batched launches over many atomic systems ------------------------------
+//
+// Each kernel of the Coulomb and exchange builds runs once for all systems of
+// an AtomicBatch: the grid is the concatenation of the systems' grids, a
+// device table holds every system's argument record and the first workgroup
+// of each system, and a workgroup finds its system by binary search
+// (batch_find).  Every system's grid is padded to a multiple of 8 workgroups
+// where the kernel maps workgroups to XCDs, so the XCD mapping inside a
+// system is the one of its own launch.
+
+namespace {
+
+struct FillArgs {
+  double *A;
+  int64_t rows, cols, ld;
+  int nblk;
+};
+
+struct PrefixArgs {
+  int n_lm, Nel;
+  const double *jsm, *jbg;
+  double *preA, *preB;
+};
+
+template <typename T>
+struct PtileArgs {
+  const double *P;
+  int64_t ldp;
+  int Nel, Nrad, Nang, n_src;
+  const int *first, *ni, *src_blk;
+  T *out;
+};
+
+__global__ void __launch_bounds__(NT) k_fill_zero_batch(const FillArgs *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  const FillArgs &a = args[s];
+  fill_zero_body(a.A, a.rows, a.cols, a.ld, blockIdx.x - start[s], a.nblk);
+}
+
+__global__ void __launch_bounds__(NT) k_atomic_j_gather_batch(const JArgs *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  j_gather_body(args[s], blockIdx.x - start[s]);
+}
+
+__global__ void __launch_bounds__(64) k_atomic_j_prefix_batch(const PrefixArgs *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  const PrefixArgs &a = args[s];
+  j_prefix_body((blockIdx.x - start[s]) * 64 + threadIdx.x, a.n_lm, a.Nel, a.jsm, a.jbg, a.preA, a.preB);
+}
+
+__global__ void __launch_bounds__(NT) k_atomic_j_expand_batch(const JOutArgs *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  j_expand_body(args[s], blockIdx.x - start[s]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) k_atomic_ptile_batch(const PtileArgs<T> *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  const PtileArgs<T> &a = args[s];
+  ptile_body<T>(a.P, a.ldp, a.Nel, a.Nrad, a.first, a.ni, a.Nang, a.src_blk, a.n_src, a.out, blockIdx.x - start[s]);
+}
+
+// The exchange kernels in three workgroup shapes; the cross-element ones keep
+// the occupancy hints of the single-system launches.
+#define HFG_K_BATCH(NAME, INSIDE, T, ATTR)                                                              \
+  template <bool TILED, int NW>                                                                         \
+  __global__ void __launch_bounds__(64 * NW) ATTR NAME(const KArgs<T> *args, const int *start, int n) { \
+    const int s = batch_find(start, n, blockIdx.x);                                                     \
+    atomic_k_body<INSIDE, TILED, T, NW>(args[s], blockIdx.x - start[s]);                                 \
+  }
+HFG_K_BATCH(k_atomic_k_off_f64_batch, false, double, __attribute__((amdgpu_waves_per_eu(6))))
+HFG_K_BATCH(k_atomic_k_off_f32_batch, false, float, __attribute__((amdgpu_waves_per_eu(8))))
+HFG_K_BATCH(k_atomic_k_in_f64_batch, true, double, )
+HFG_K_BATCH(k_atomic_k_in_f32_batch, true, float, )
+#undef HFG_K_BATCH
+
+template <typename T, bool INSIDE, bool TILED, int NW>
+void launch_k_batch_nw(const void *args, const int *start, int n, int grid, hipStream_t s) {
+  const KArgs<T> *a = static_cast<const KArgs<T> *>(args);
+  if constexpr (std::is_same<T, double>::value) {
+    if constexpr (INSIDE)
+      hipLaunchKernelGGL((k_atomic_k_in_f64_batch<TILED, NW>), dim3(grid), dim3(64 * NW), 0, s, a, start, n);
+    else
+      hipLaunchKernelGGL((k_atomic_k_off_f64_batch<TILED, NW>), dim3(grid), dim3(64 * NW), 0, s, a, start, n);
+  } else {
+    if constexpr (INSIDE)
+      hipLaunchKernelGGL((k_atomic_k_in_f32_batch<TILED, NW>), dim3(grid), dim3(64 * NW), 0, s, a, start, n);
+    else
+      hipLaunchKernelGGL((k_atomic_k_off_f32_batch<TILED, NW>), dim3(grid), dim3(64 * NW), 0, s, a, start, n);
+  }
+}
+
+// cls = fp32 * 2 + tiled; part 0 cross-element, 1 in-element; nw in {1, 2, 4}
+void launch_k_batch(int cls, int part, int nw, const void *args, const int *start, int n, int grid, hipStream_t s) {
+  if (grid <= 0) return;
+#define HFG_KB(T, IN, TL)                                                     \
+  do {                                                                        \
+    if (nw == 1)                                                              \
+      launch_k_batch_nw<T, IN, TL, 1>(args, start, n, grid, s);               \
+    else if (nw == 2)                                                         \
+      launch_k_batch_nw<T, IN, TL, 2>(args, start, n, grid, s);               \
+    else                                                                      \
+      launch_k_batch_nw<T, IN, TL, 4>(args, start, n, grid, s);               \
+  } while (0)
+  const bool f32 = cls >= 2, tiled = cls & 1;
+  if (!f32 && !part && !tiled) HFG_KB(double, false, false);
+  else if (!f32 && !part && tiled) HFG_KB(double, false, true);
+  else if (!f32 && part && !tiled) HFG_KB(double, true, false);
+  else if (!f32 && part && tiled) HFG_KB(double, true, true);
+  else if (f32 && !part && !tiled) HFG_KB(float, false, false);
+  else if (f32 && !part && tiled) HFG_KB(float, false, true);
+  else if (f32 && part && !tiled) HFG_KB(float, true, false);
+  else HFG_KB(float, true, true);
+#undef HFG_KB
+  hip_check(hipGetLastError(), "atomic_k_batch");
+}
+
+// Host image of the device argument tables: sections aligned to 256 bytes.
+struct TabImage {
+  std::vector<char> buf;
+  template <typename X>
+  size_t add(const std::vector<X> &v) {
+    const size_t off = (buf.size() + 255) & ~(size_t)255;
+    buf.resize(off + std::max<size_t>(v.size(), 1) * sizeof(X), 0);
+    if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
+    return off;
+  }
+};
+
+// Copy a table image to the device through the pinned staging buffer (after
+// the previous copy out of it has finished); returns the device base.
+char *upload_table(const TabImage &img, DevArray<char> &dtab, PinnedArray<char> &htab, hipEvent_t &ev,
+                   hipStream_t s) {
+  if (ev) hip_check(hipEventSynchronize(ev), "table staging wait");
+  else hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+  htab.alloc(img.buf.size());
+  dtab.alloc(img.buf.size());
+  std::memcpy(htab.p, img.buf.data(), img.buf.size());
+  hip_check(hipMemcpyAsync(dtab.p, htab.p, img.buf.size(), hipMemcpyHostToDevice, s), "table upload");
+  hip_check(hipEventRecord(ev, s), "hipEventRecord");
+  return dtab.p;
+}
+
+int fill_blocks(int64_t rows, int64_t cols) {
+  const int64_t n = rows * cols;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 8 * NT - 1) / (8 * NT)));
+}
+
+// Run `launches` as a graph captured for `sig` (re-captured when sig
+// changes), or directly when graphs are off or the stream cannot capture
+// (the legacy null stream).
+template <typename F>
+void run_graphed(bool use, hipGraphExec_t &gexec, std::vector<long long> &cur, const std::vector<long long> &sig,
+                 hipStream_t s, F launches) {
+  if (!use || s == nullptr) {
+    launches();
+    return;
+  }
+  if (!gexec || cur != sig) {
+    if (gexec) {
+      (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(e, "hipGraphInstantiate");
+    cur = sig;
+  }
+  hip_check(hipGraphLaunch(gexec, s), "hipGraphLaunch");
+}
+
+}  // namespace
+
+AtomicBatch::~AtomicBatch() {
+  if (gexec_j) (void)hipGraphExecDestroy(gexec_j);
+  if (gexec_k) (void)hipGraphExecDestroy(gexec_k);
+  for (hipEvent_t e : {ev_j, ev_k, ev_s})
+    if (e) (void)hipEventDestroy(e);
+}
+
+void atomic_batch_coulomb(AtomicBatch &b, const double *const *P, const int64_t *ldp, double *const *J,
+                          const int64_t *ldj, hipStream_t s, KernelTimer *timer) {
+  const int n = (int)b.sys.size();
+  std::vector<FillArgs> fills(n);
+  std::vector<JArgs> gat(n);
+  std::vector<PrefixArgs> pre(n);
+  std::vector<JOutArgs> outs(n);
+  std::vector<int> sF(n + 1, 0), sG(n + 1, 0), sP(n + 1, 0), sE(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    AtomicDevice &d = *b.sys[i];
+    const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+    fills[i] = FillArgs{J[i], nbf, nbf, ldj[i], fill_blocks(nbf, nbf)};
+    sF[i + 1] = sF[i] + fills[i].nblk;
+    gat[i] = JArgs{P[i], ldp[i], d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.rs.p, d.rb.p, d.chol.p,
+                   d.chol_off.p, d.chol_rank.p, d.lm_L.p, d.lm_pstart.p, d.pair_kl.p, d.pair_c.p,
+                   d.Jin.p, d.jsm.p, d.jbg.p};
+    pre[i] = PrefixArgs{d.n_lm, d.Nel, d.jsm.p, d.jbg.p, d.preA.p, d.preB.p};
+    outs[i] = JOutArgs{J[i], ldj[i], d.Nel, d.Nrad, d.Nang, d.first.p, d.ni.p, d.lm_L.p, d.rs.p, d.rb.p,
+                       d.Jin.p, d.preA.p, d.preB.p, d.out_start.p, d.out_lm.p, d.out_c.p};
+    const bool any = d.n_lm > 0;
+    sG[i + 1] = sG[i] + (any ? d.n_lm * d.Nel : 0);
+    sP[i + 1] = sP[i] + (any ? (d.n_lm + 63) / 64 : 0);
+    sE[i + 1] = sE[i] + (any ? d.Nang * d.Nang * d.Nel : 0);
+  }
+  TabImage img;
+  const size_t oF = img.add(fills), oG = img.add(gat), oP = img.add(pre), oE = img.add(outs);
+  const size_t osF = img.add(sF), osG = img.add(sG), osP = img.add(sP), osE = img.add(sE);
+  char *base = upload_table(img, b.dtab_j, b.htab_j, b.ev_j, s);
+  const std::vector<long long> sig{sF[n], sG[n], sP[n], sE[n]};
+  auto launches = [&](KernelTimer *t) {
+    const int *dsF = (const int *)(base + osF), *dsG = (const int *)(base + osG);
+    const int *dsP = (const int *)(base + osP), *dsE = (const int *)(base + osE);
+    hipLaunchKernelGGL(k_fill_zero_batch, dim3(sF[n]), dim3(NT), 0, s, (const FillArgs *)(base + oF), dsF, n);
+    hip_check(hipGetLastError(), "fill_zero_batch");
+    if (sG[n] == 0) return;
+    {
+      TimedLaunch tl(t, "batch_j_gather", s);
+      hipLaunchKernelGGL(k_atomic_j_gather_batch, dim3(sG[n]), dim3(NT), 0, s, (const JArgs *)(base + oG), dsG, n);
+      hip_check(hipGetLastError(), "atomic_j_gather_batch");
+    }
+    hipLaunchKernelGGL(k_atomic_j_prefix_batch, dim3(sP[n]), dim3(64), 0, s, (const PrefixArgs *)(base + oP), dsP,
+                       n);
+    hip_check(hipGetLastError(), "atomic_j_prefix_batch");
+    {
+      TimedLaunch tl(t, "batch_j_expand", s);
+      hipLaunchKernelGGL(k_atomic_j_expand_batch, dim3(sE[n]), dim3(NT), 0, s, (const JOutArgs *)(base + oE), dsE,
+                         n);
+      hip_check(hipGetLastError(), "atomic_j_expand_batch");
+    }
+  };
+  if (timer && timer->on)
+    launches(timer);
+  else
+    run_graphed(b.use_graphs, b.gexec_j, b.sig_j, sig, s, [&] { launches(nullptr); });
+}
+
+void atomic_batch_exchange(AtomicBatch &b, const double *const *P, const int64_t *ldp, double *const *K,
+                           const int64_t *ldk, const int *fp32, hipStream_t s, KernelTimer *timer) {
+  const int n = (int)b.sys.size();
+  // 1. block statistics of every density in one launch, one copy, one sync
+  {
+    std::vector<StatsArgs> st(n);
+    std::vector<int> sS(n + 1, 0);
+    b.stats_off.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+      AtomicDevice &d = *b.sys[i];
+      const int nch = block_stats_chunks(d.Nrad);
+      const size_t cnt = d.h_act.empty() ? 0 : (size_t)d.Nang * d.Nang * 4 * nch;
+      b.stats_off[i + 1] = b.stats_off[i] + cnt;
+      st[i] = StatsArgs{P[i], ldp[i], d.Nrad, d.Nang, nch, nullptr, nullptr, nullptr};
+      sS[i + 1] = sS[i] + (cnt ? d.Nang * d.Nang * nch : 0);
+    }
+    b.stats.alloc(std::max<size_t>(b.stats_off[n], 1));
+    b.h_stats.alloc(std::max<size_t>(b.stats_off[n], 1));
+    for (int i = 0; i < n; ++i) st[i].part = b.stats.p + b.stats_off[i];
+    TabImage img;
+    const size_t oA = img.add(st), oS = img.add(sS);
+    char *base = upload_table(img, b.dtab_s, b.htab_s, b.ev_s, s);
+    {
+      TimedLaunch tl(timer, "batch_stats", s);
+      block_stats_batch_launch((const StatsArgs *)(base + oA), (const int *)(base + oS), n, sS[n], s);
+    }
+    if (b.stats_off[n])
+      hip_check(hipMemcpyAsync(b.h_stats.p, b.stats.p, sizeof(double) * b.stats_off[n], hipMemcpyDeviceToHost, s),
+                "copy block statistics");
+    hip_check(hipStreamSynchronize(s), "block statistics sync");
+  }
+  // 2. per system: screen, symmetry mode, compacted lists, reduced-precision tables
+  for (int i = 0; i < n; ++i) {
+    AtomicDevice &d = *b.sys[i];
+    if (d.h_act.empty()) continue;
+    const bool symmetric = block_stats_reduce(b.h_stats.p + b.stats_off[i], d.Nrad, d.Nang, d.h_bnorm.p);
+    d.sym = !d.dense_in && symmetric_exchange_allowed() && symmetric;
+    compact_exchange_lists(d, s);
+    if (fp32[i] && d.n_act > 0) {
+      if (!d.f32_tables) {
+        d.rs32.alloc(d.rs.n);
+        d.rb32.alloc(d.rb.n);
+        d.chol32.alloc(d.chol.n);
+        to_f32_launch(d.rs.p, d.rs32.p, d.rs.n, s);
+        to_f32_launch(d.rb.p, d.rb32.p, d.rb.n, s);
+        to_f32_launch(d.chol.p, d.chol32.p, d.chol.n, s);
+        d.f32_tables = true;
+      }
+      if (d.term_c32_gen != d.list_gen) {
+        d.term_c32.alloc(d.term_c.n);
+        to_f32_launch(d.term_c.p, d.term_c32.p, d.term_c.n, s);
+        d.term_c32_gen = d.list_gen;
+      }
+      const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+      if (!d.tiled) {
+        d.P32.alloc((size_t)nbf * nbf);
+        mat_to_f32_launch(P[i], ldp[i], d.P32.p, nbf, s);
+      }
+    }
+    if (d.tiled && d.n_act > 0) {
+      const size_t nt = (size_t)d.Nel * d.Nel * d.n_src * TILE2;
+      if (fp32[i])
+        d.ptile32.alloc(nt);
+      else
+        d.ptile.alloc(nt);
+    }
+  }
+  // 3. argument tables per class (fp32 * 2 + tiled)
+  std::vector<FillArgs> fills(n);
+  std::vector<int> sF(n + 1, 0);
+  std::vector<KArgs<double>> kd[2][2];  // [tiled][part]
+  std::vector<KArgs<float>> kf[2][2];
+  std::vector<int> sk[4][2];
+  std::vector<PtileArgs<double>> pd;
+  std::vector<PtileArgs<float>> pf;
+  std::vector<int> spd(1, 0), spf(1, 0);
+  for (int c = 0; c < 4; ++c)
+    for (int p = 0; p < 2; ++p) sk[c][p].assign(1, 0);
+  double flops[4][2] = {};
+  for (int i = 0; i < n; ++i) {
+    AtomicDevice &d = *b.sys[i];
+    const int64_t nbf = (int64_t)d.Nrad * d.Nang;
+    fills[i] = FillArgs{K[i], nbf, nbf, ldk[i], fill_blocks(nbf, nbf)};
+    sF[i + 1] = sF[i] + fills[i].nblk;
+    if (d.h_act.empty() || d.n_act == 0) continue;
+    const bool f = fp32[i] != 0, tl = d.tiled;
+    const int cls = (f ? 2 : 0) + (tl ? 1 : 0);
+    if (tl) {
+      const int64_t n_tiles = (int64_t)d.Nel * d.Nel * d.n_src, wg = (n_tiles + NT / 64 - 1) / (NT / 64);
+      if (f) {
+        pf.push_back(PtileArgs<float>{P[i], ldp[i], d.Nel, d.Nrad, d.Nang, d.n_src, d.first.p, d.ni.p,
+                                      d.src_blk.p, d.ptile32.p});
+        spf.push_back(spf.back() + (int)wg);
+      } else {
+        pd.push_back(PtileArgs<double>{P[i], ldp[i], d.Nel, d.Nrad, d.Nang, d.n_src, d.first.p, d.ni.p,
+                                       d.src_blk.p, d.ptile.p});
+        spd.push_back(spd.back() + (int)wg);
+      }
+    }
+    for (int part = 0; part < 2; ++part) {
+      const int n_ef = part == 0 ? d.Nel * (d.Nel - 1) / (d.sym ? 2 : 1) : d.Nel;
+      const int n_ef_pad = ((n_ef + 7) / 8) * 8;
+      const int64_t grid = n_ef == 0 ? 0 : (int64_t)n_ef_pad * d.n_act;
+      if ((int64_t)sk[cls][part].back() + grid > 0x7fffffff) throw Error(-1, "batched exchange grid too large");
+      const int *efl = part == 0 ? (d.sym ? d.ef_low.p : d.ef_off.p) : d.ef_in.p;
+      if (f)
+        kf[tl][part].push_back(KArgs<float>{tl ? nullptr : d.P32.p, nbf, K[i], ldk[i], d.Nel, d.Nrad, d.Nang, d.N_L,
+                                            d.n_act, d.first.p, d.ni.p, d.rs32.p, d.rb32.p, d.chol32.p,
+                                            d.chol_off.p, d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_off.p,
+                                            d.term_c32.p, efl, n_ef, nullptr, tl ? d.ptile32.p : nullptr,
+                                            d.term_tile.p, d.n_src, (int)d.sym});
+      else
+        kd[tl][part].push_back(KArgs<double>{P[i], ldp[i], K[i], ldk[i], d.Nel, d.Nrad, d.Nang, d.N_L, d.n_act,
+                                             d.first.p, d.ni.p, d.rs.p, d.rb.p, d.chol.p, d.chol_off.p,
+                                             d.chol_rank.p, d.act_jk.p, d.term_start.p, d.term_off.p, d.term_c.p,
+                                             efl, n_ef, nullptr, tl ? d.ptile.p : nullptr, d.term_tile.p, d.n_src,
+                                             (int)d.sym});
+      sk[cls][part].push_back(sk[cls][part].back() + (int)grid);
+      flops[cls][part] += part == 0 ? d.flops_off : d.flops_in;
+    }
+  }
+  TabImage img;
+  const size_t oF = img.add(fills), osF = img.add(sF);
+  const size_t opd = img.add(pd), ospd = img.add(spd), opf = img.add(pf), ospf = img.add(spf);
+  size_t ok[4][2], osk[4][2];
+  for (int c = 0; c < 4; ++c)
+    for (int p = 0; p < 2; ++p) {
+      const bool f = c >= 2, tl = c & 1;
+      ok[c][p] = f ? img.add(kf[tl][p]) : img.add(kd[tl][p]);
+      osk[c][p] = img.add(sk[c][p]);
+    }
+  char *base = upload_table(img, b.dtab_k, b.htab_k, b.ev_k, s);
+  auto nsys = [&](int c, int p) { return (int)sk[c][p].size() - 1; };
+  auto fill_all = [&] {
+    hipLaunchKernelGGL(k_fill_zero_batch, dim3(sF[n]), dim3(NT), 0, s, (const FillArgs *)(base + oF),
+                       (const int *)(base + osF), n);
+    hip_check(hipGetLastError(), "fill_zero_batch");
+  };
+  auto ptiles = [&](KernelTimer *t) {
+    TimedLaunch tl(t, "batch_ptile", s);
+    if (spd.back() > 0)
+      hipLaunchKernelGGL(k_atomic_ptile_batch<double>, dim3(spd.back()), dim3(NT), 0, s,
+                         (const PtileArgs<double> *)(base + opd), (const int *)(base + ospd), (int)pd.size());
+    if (spf.back() > 0)
+      hipLaunchKernelGGL(k_atomic_ptile_batch<float>, dim3(spf.back()), dim3(NT), 0, s,
+                         (const PtileArgs<float> *)(base + opf), (const int *)(base + ospf), (int)pf.size());
+    hip_check(hipGetLastError(), "atomic_ptile_batch");
+  };
+  auto kpart = [&](int c, int p, int nw) {
+    launch_k_batch(c, p, nw, base + ok[c][p], (const int *)(base + osk[c][p]), nsys(c, p), sk[c][p].back(), s);
+  };
+  // 4. autotune the workgroup shape of each class whose composition changed
+  for (int c = 0; c < 4; ++c) {
+    const std::vector<long long> csig{sk[c][0].back(), sk[c][1].back(), nsys(c, 0), nsys(c, 1)};
+    if (sk[c][0].back() + sk[c][1].back() == 0) continue;
+    if (b.nw[c][0] && b.tuned_sig[c] == csig) continue;
+    if (!b.autotune) {
+      b.nw[c][0] = b.nw[c][1] = 4;
+      b.tuned_sig[c] = csig;
+      continue;
+    }
+    ptiles(nullptr);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    const int shapes[3] = {1, 2, 4};
+    for (int p = 0; p < 2; ++p) {
+      if (sk[c][p].back() == 0) {
+        b.nw[c][p] = 4;
+        continue;
+      }
+      float best = 1e30f;
+      for (int q = 0; q < 3; ++q) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+          fill_all();
+          hip_check(hipEventRecord(e0, s), "hipEventRecord");
+          kpart(c, p, shapes[q]);
+          hip_check(hipEventRecord(e1, s), "hipEventRecord");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        }
+        b.tune_ms[c][p][q] = ms;
+        if (ms < best) {
+          best = ms;
+          b.nw[c][p] = shapes[q];
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b.tuned_sig[c] = csig;
+  }
+  // 5. the launch sequence: zero K, density tiles, cross- and in-element kernels
+  std::vector<long long> sig{sF[n], spd.back(), spf.back()};
+  for (int c = 0; c < 4; ++c)
+    for (int p = 0; p < 2; ++p) {
+      sig.push_back(sk[c][p].back());
+      sig.push_back(b.nw[c][p]);
+    }
+  static const char *names[4][2] = {{"batch_k_off", "batch_k_in"},
+                                    {"batch_k_off", "batch_k_in"},
+                                    {"batch_k_off_f32", "batch_k_in_f32"},
+                                    {"batch_k_off_f32", "batch_k_in_f32"}};
+  auto launches = [&](KernelTimer *t) {
+    fill_all();
+    ptiles(t);
+    for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < 4; ++c) {
+        if (sk[c][p].back() == 0) continue;
+        TimedLaunch tl(t, names[c][p], s, flops[c][p]);
+        kpart(c, p, b.nw[c][p] ? b.nw[c][p] : 4);
+      }
+  };
+  if (timer && timer->on)
+    launches(timer);
+  else
+    run_graphed(b.use_graphs, b.gexec_k, b.sig_k, sig, s, [&] { launches(nullptr); });
 }
 
 }  // namespace hfg

@@ -385,6 +385,16 @@ int hf_atomic_create(hf_ctx *ctx, int Z, int nnodes, int nquad, const double *bv
   });
 }
 
+int hf_atomic_clone(const hf_atomic *h, hf_atomic **out) {
+  return guarded([&] {
+    need(h != nullptr && out != nullptr, "NULL argument");
+    std::unique_ptr<hf_atomic> c(new hf_atomic);
+    c->ctx = h->ctx;
+    c->host = h->host;
+    *out = c.release();
+  });
+}
+
 int hf_atomic_destroy(hf_atomic *h) {
   return guarded([&] {
     if (!h) return;
@@ -639,6 +649,102 @@ int hf_atomic_exchange(hf_atomic *h, const double *P, int64_t ldp, double *K, in
     host_roundtrip(h->ctx->cur, h->host.Nbf(), h->dev.hostP, h->dev.hostOut, P, ldp, K, ldk, [&](const double *dp, int64_t lp, double *dout, int64_t lo) {
       atomic_exchange_launch(h->dev, dp, lp, dout, lo, h->ctx->cur, &h->ctx->timer);
     });
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ atomic batch
+
+struct hf_atomic_batch {
+  hf_ctx *ctx = nullptr;
+  std::vector<hf_atomic *> handles;
+  hfg::AtomicBatch b;
+};
+
+extern "C" {
+
+int hf_atomic_batch_create(hf_ctx *ctx, hf_atomic *const *handles, int n, hf_atomic_batch **out) {
+  return guarded([&] {
+    need(ctx != nullptr && out != nullptr, "NULL context or output");
+    need(n >= 1 && handles != nullptr, "a batch needs at least one handle");
+    DeviceGuard g(ctx->device);
+    std::unique_ptr<hf_atomic_batch> b(new hf_atomic_batch);
+    b->ctx = ctx;
+    for (int i = 0; i < n; ++i) {
+      hf_atomic *h = handles[i];
+      need(h != nullptr, "NULL handle in the batch");
+      need(h->ctx == ctx, "every handle of a batch must use the batch's context");
+      ensure_ready(h);
+      need(!h->dev.dense_in, "the dense in-element operator is not supported in a batch");
+      b->handles.push_back(h);
+      b->b.sys.push_back(&h->dev);
+    }
+    *out = b.release();
+  });
+}
+
+int hf_atomic_batch_destroy(hf_atomic_batch *b) {
+  return guarded([&] {
+    if (!b) return;
+    DeviceGuard g(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->cur);
+    delete b;
+  });
+}
+
+int hf_atomic_batch_set_options(hf_atomic_batch *b, int use_graphs, int autotune) {
+  return guarded([&] {
+    need(b != nullptr, "NULL batch");
+    b->b.use_graphs = use_graphs != 0;
+    b->b.autotune = autotune != 0;
+    for (auto &t : b->b.tuned_sig) t.clear();
+  });
+}
+
+int hf_atomic_batch_tuning(const hf_atomic_batch *b, int nw[8], double ms[24]) {
+  return guarded([&] {
+    need(b != nullptr, "NULL batch");
+    for (int c = 0; c < 4; ++c)
+      for (int p = 0; p < 2; ++p) {
+        if (nw) nw[c * 2 + p] = b->b.nw[c][p];
+        for (int q = 0; q < 3; ++q)
+          if (ms) ms[(c * 2 + p) * 3 + q] = b->b.tune_ms[c][p][q];
+      }
+  });
+}
+
+namespace {
+void check_batch_mats(hf_atomic_batch *b, const double *const *P, const int64_t *ldp, double *const *O,
+                      const int64_t *ldo) {
+  need(P != nullptr && ldp != nullptr && O != nullptr && ldo != nullptr, "NULL pointer array");
+  for (size_t i = 0; i < b->handles.size(); ++i) {
+    check_matrix(P[i], ldp[i], b->handles[i]->host.Nbf(), "P");
+    check_matrix(O[i], ldo[i], b->handles[i]->host.Nbf(), "output");
+  }
+}
+}  // namespace
+
+int hf_atomic_batch_coulomb_dev(hf_atomic_batch *b, const double *const *dP, const int64_t *ldp, double *const *dJ,
+                                const int64_t *ldj) {
+  return guarded([&] {
+    need(b != nullptr, "NULL batch");
+    DeviceGuard g(b->ctx->device);
+    check_batch_mats(b, dP, ldp, dJ, ldj);
+    atomic_batch_coulomb(b->b, dP, ldp, dJ, ldj, b->ctx->cur, &b->ctx->timer);
+  });
+}
+
+int hf_atomic_batch_exchange_dev(hf_atomic_batch *b, const double *const *dP, const int64_t *ldp, double *const *dK,
+                                 const int64_t *ldk, const int *fp32) {
+  return guarded([&] {
+    need(b != nullptr, "NULL batch");
+    DeviceGuard g(b->ctx->device);
+    check_batch_mats(b, dP, ldp, dK, ldk);
+    std::vector<int> f(b->handles.size(), 0);
+    if (fp32)
+      for (size_t i = 0; i < f.size(); ++i) f[i] = fp32[i] != 0;
+    atomic_batch_exchange(b->b, dP, ldp, dK, ldk, f.data(), b->ctx->cur, &b->ctx->timer);
   });
 }
 

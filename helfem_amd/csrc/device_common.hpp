@@ -86,5 +86,20 @@ __device__ inline typename Mma<T>::acc_t mma16(const T xs[4], typename Mma<T>::a
   return d;
 }
 
+// Index s of the system owning workgroup b of a launch that concatenates
+// the grids of n systems: the largest s with start[s] <= b (systems with no
+// workgroups have start[s] == start[s+1] and are skipped).
+__device__ inline int batch_find(const int *start, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 }  // namespace dev
 }  // namespace hfg

@@ -195,9 +195,9 @@ __global__ void __launch_bounds__(NT) k_dia_j_expand(DJOutArgs A) {
 // reduced on the host.  32x32 sub-tiles of both blocks are read coalesced
 // into LDS and compared transposed.  Shell a spans rows poff[a] .. poff[a] +
 // Nrad - skip[a] (a Nrad, no skip when the tables are null: atomic basis).
-__global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp, int Nrad, int Nang,
-                                                    const int *skip, const long long *poff, double *part) {
-  const int il = blockIdx.x, i = il / Nang, l = il - i * Nang, ch = blockIdx.y, nch = gridDim.y;
+__device__ inline void block_stats_body(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip,
+                                        const long long *poff, double *part, int il, int ch, int nch) {
+  const int i = il / Nang, l = il - i * Nang;
   if (i > l) return;  // covered by the workgroups of (l, i)
   __shared__ double A[32][33], B[32][33];
   __shared__ double red[16];
@@ -246,6 +246,20 @@ __global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp
     o[2] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
     o[3] = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
   }
+}
+
+__global__ void __launch_bounds__(NT) k_block_stats(const double *P, int64_t ldp, int Nrad, int Nang,
+                                                    const int *skip, const long long *poff, double *part) {
+  block_stats_body(P, ldp, Nrad, Nang, skip, poff, part, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+// Many matrices in one launch: workgroup b of system s (start[s] <= b <
+// start[s+1]) takes shell pair il = local / nch and chunk local % nch.
+__global__ void __launch_bounds__(NT) k_block_stats_batch(const StatsArgs *args, const int *start, int n) {
+  const int s = batch_find(start, n, blockIdx.x);
+  const StatsArgs &a = args[s];
+  const int loc = blockIdx.x - start[s], nch = a.nch;
+  block_stats_body(a.P, a.ldp, a.Nrad, a.Nang, a.skip, a.poff, a.part, loc / nch, loc - (loc / nch) * nch, nch);
 }
 
 // Two products sharing the Y operand, issued interleaved so the two
@@ -1023,6 +1037,12 @@ void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t l
 }
 
 int block_stats_chunks(int Nrad) { return (Nrad + 31) / 32; }
+
+void block_stats_batch_launch(const StatsArgs *d_args, const int *d_start, int n, int total_wg, hipStream_t s) {
+  if (total_wg <= 0) return;
+  hipLaunchKernelGGL(k_block_stats_batch, dim3(total_wg), dim3(NT), 0, s, d_args, d_start, n);
+  hip_check(hipGetLastError(), "block_stats_batch");
+}
 
 void block_stats_launch(const double *P, int64_t ldp, int Nrad, int Nang, const int *skip, const long long *poff,
                         double *d_part, double *h_part, hipStream_t s) {

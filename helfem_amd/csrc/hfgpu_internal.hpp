@@ -364,6 +364,53 @@ void mat_to_f32_launch(const double *A, int64_t lda, float *B, int64_t n, hipStr
 void fill_zero_launch(double *A, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
 void atomic_build_operator(const AtomicSetup &S, AtomicDevice &d, hipStream_t s);
 
+// Arguments of one matrix of a batched k_block_stats launch.
+struct StatsArgs {
+  const double *P;
+  int64_t ldp;
+  int Nrad, Nang, nch;
+  const int *skip;
+  const long long *poff;
+  double *part;
+};
+void block_stats_batch_launch(const StatsArgs *d_args, const int *d_start, int n, int total_wg, hipStream_t s);
+
+// Many atomic systems whose Coulomb and exchange builds run as one launch
+// per kernel (atomic_jk.hip): per-system argument records and workgroup
+// offsets live in device tables rewritten each call; the launch sequence
+// after the host-side screening is captured once in a HIP graph and replayed
+// while the grid totals and workgroup shapes stay the same.  The cross- and
+// in-element exchange kernels come in three workgroup shapes (1, 2 or 4
+// waves, splitting the multipoles L); an autotuner times each on the batch
+// and keeps the fastest per (precision, tiled) class and composition.
+struct AtomicBatch {
+  std::vector<AtomicDevice *> sys;
+  // device argument tables, written from a pinned staging copy
+  DevArray<char> dtab_j, dtab_k, dtab_s;
+  PinnedArray<char> htab_j, htab_k, htab_s;
+  hipEvent_t ev_j = nullptr, ev_k = nullptr, ev_s = nullptr;  // the last copy out of each staging buffer
+  DevArray<double> stats;
+  PinnedArray<double> h_stats;
+  std::vector<size_t> stats_off;
+  // captured launch sequences and the grid signature each was captured for
+  hipGraphExec_t gexec_j = nullptr, gexec_k = nullptr;
+  std::vector<long long> sig_j, sig_k;
+  bool use_graphs = true;
+  // autotuned waves per workgroup of the exchange kernels, per class
+  // (fp32 * 2 + tiled) and launch part (0 cross-element, 1 in-element);
+  // the signature each choice was made for
+  int nw[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  std::vector<long long> tuned_sig[4];
+  bool autotune = true;
+  double tune_ms[4][2][3] = {};  // measured launch times of the 1, 2, 4-wave shapes
+  ~AtomicBatch();
+};
+void atomic_batch_coulomb(AtomicBatch &b, const double *const *P, const int64_t *ldp, double *const *J,
+                          const int64_t *ldj, hipStream_t s, KernelTimer *timer);
+// fp32[s] != 0: system s in reduced precision (as atomic_exchange_launch).
+void atomic_batch_exchange(AtomicBatch &b, const double *const *P, const int64_t *ldp, double *const *K,
+                           const int64_t *ldk, const int *fp32, hipStream_t s, KernelTimer *timer);
+
 // launchers (xc_grid.hip); exc/nel summed on the host from h_part
 // Pb == nullptr: restricted (P the total density, H the potential); else
 // spin-polarized with Pa, Pb in and Ha, Hb out.

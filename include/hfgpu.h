@@ -14,9 +14,12 @@
  *     thread-local message.  No C++ exception crosses this boundary.
  *   - One handle is stream-ordered and must not be used from two host
  *     threads at once; different handles may be used concurrently.
- *   - Functions ending in _dev take device pointers and are asynchronous on
- *     the context stream; all others take host pointers and return when the
- *     result is in host memory.
+ *   - Functions ending in _dev take device pointers and queue their work on
+ *     the context stream.  The exchange forms first read the density's block
+ *     norms back to the host (the reference's screen decides which blocks
+ *     are contracted), so they synchronise the stream once before queueing
+ *     the contraction; the others do not synchronise.  All other functions
+ *     take host pointers and return when the result is in host memory.
  */
 #ifndef HFGPU_H
 #define HFGPU_H
@@ -84,6 +87,11 @@ int hf_element_grid(double rmax, int nel, int igrid, double zexp, double *out);
 int hf_atomic_create(hf_ctx *ctx, int Z, int nnodes, int nquad, const double *bval, int nbval,
                      const int *lval, const int *mval, int nang, hf_atomic **out);
 int hf_atomic_destroy(hf_atomic *h);
+/* A second handle on the same basis and context, with a copy of the host-side
+ * setup (two-electron caches included) and device tables of its own, so two
+ * densities of one basis (the alpha and beta spins of an unrestricted SCF)
+ * can sit in one batch.  No counterpart in the reference (a TwoDBasis copy). */
+int hf_atomic_clone(const hf_atomic *h, hf_atomic **out);
 
 /* Sizes: out[0]=Nbf, out[1]=Nrad, out[2]=Nang, out[3]=Nel, out[4]=N_L. */
 int hf_atomic_sizes(const hf_atomic *h, int64_t out[5]);
@@ -162,6 +170,43 @@ int hf_atomic_coulomb_dev_noalloc(hf_atomic *h, const double *dP, int64_t ldp, d
  * relative error of about 1e-6.  Same screening and output as the FP64 form
  * of TwoDBasisT::exchange (TwoDBasis.cpp:880-999). */
 int hf_atomic_exchange_fp32_dev(hf_atomic *h, const double *dP, int64_t ldp, double *dK, int64_t ldk);
+
+/* ----------------------------------------------------------- atomic batch */
+
+/* Many atomic bases whose Fock builds run together: each kernel of the
+ * Coulomb and of the exchange build is one launch over every system (the
+ * systems' workgroups concatenated, a device table of per-system arguments),
+ * and the launch sequence after the density screen is captured in a HIP
+ * graph and replayed while the grid sizes stay the same (graphs need a
+ * context stream other than the legacy null stream).  Batched forms of
+ * TwoDBasisT::coulomb / exchange (src/atomic/TwoDBasis.h:223-225,
+ * TwoDBasis.cpp:774-999); the reference builds one system at a time and has
+ * no counterpart of the batch itself.  The handles must share ctx, have
+ * their two-electron caches computed, stay alive while the batch exists and
+ * not be used by other calls concurrently. */
+typedef struct hf_atomic_batch hf_atomic_batch;
+int hf_atomic_batch_create(hf_ctx *ctx, hf_atomic *const *handles, int n, hf_atomic_batch **out);
+int hf_atomic_batch_destroy(hf_atomic_batch *b);
+/* use_graphs: capture and replay the launch sequences (default 1);
+ * autotune: time the 1-, 2- and 4-wave workgroup shapes of the exchange
+ * kernels on the batch and keep the fastest (default 1), re-tuned when the
+ * batch's exchange grid changes; 0 fixes the 4-wave shape. */
+int hf_atomic_batch_set_options(hf_atomic_batch *b, int use_graphs, int autotune);
+/* The autotuner's state: nw[c*2 + p] waves per workgroup chosen for class c
+ * (fp32 * 2 + tiled density) and part p (0 across elements, 1 inside), and
+ * ms[(c*2 + p)*3 + q] the launch times it measured for 1, 2, 4 waves. */
+int hf_atomic_batch_tuning(const hf_atomic_batch *b, int nw[8], double ms[24]);
+/* J_s = J(P_s) for every system s (device pointers, asynchronous on the
+ * context stream).  Arrays of n pointers and leading dimensions. */
+int hf_atomic_batch_coulomb_dev(hf_atomic_batch *b, const double *const *dP, const int64_t *ldp, double *const *dJ,
+                                const int64_t *ldj);
+/* K_s = -K(P_s) for every system s, fp32[s] != 0 selecting the
+ * reduced-precision form for that system (fp32 may be NULL: all FP64).  The
+ * density screen of every system is decided on the host from one batched
+ * statistics launch: the call returns after one stream synchronisation,
+ * with the exchange launches queued. */
+int hf_atomic_batch_exchange_dev(hf_atomic_batch *b, const double *const *dP, const int64_t *ldp, double *const *dK,
+                                 const int64_t *ldk, const int *fp32);
 
 /* ------------------------------------------------------- atomic DFT grid */
 

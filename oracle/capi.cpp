@@ -4,6 +4,7 @@
 #include "diatomic.hpp"
 #include "dftgrid.hpp"
 #include <cstring>
+#include <omp.h>
 #include <stdexcept>
 #include <string>
 
@@ -386,6 +387,14 @@ int orc_diatomic_cache(void *h, int kind, int ilm, int iel, int *dims, double *o
     dims[1] = m.c;
     if (out) put(m, out);
   });
+}
+
+// Thread count of the oracle's OpenMP loops (the CPU baseline is timed at
+// all host threads and at one).
+int orc_set_threads(int n) {
+  if (n < 1) return -1;
+  omp_set_num_threads(n);
+  return 0;
 }
 
 }  // extern "C"

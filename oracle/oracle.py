@@ -82,6 +82,11 @@ def lib():
     return _LIB
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's loops."""
+    _check(lib().orc_set_threads(int(n)))
+
+
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
