@@ -33,6 +33,8 @@ SIGNATURES = {
     "hf_atomic_create": (c_int, [c_vp, c_int, c_int, c_int, c_dp, c_int, c_vp, c_vp, c_int, ctypes.POINTER(c_vp)]),
     "hf_atomic_destroy": (c_int, [c_vp]),
     "hf_atomic_clone": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "hf_atomic_dftgrid_create_rule": (c_int, [c_vp, c_int, c_int, c_int, ctypes.POINTER(c_vp)]),
+    "hf_quadrature": (c_int, [c_int, c_int, c_dp, c_dp]),
     "hf_atomic_sizes": (c_int, [c_vp, c_vp]),
     "hf_atomic_elements": (c_int, [c_vp, c_vp, c_vp]),
     "hf_atomic_compute_tei": (c_int, [c_vp, c_int]),

@@ -12,6 +12,7 @@ struct hf_dftgrid {
   const hfg::DiatomicSetup *diatomic = nullptr;
   int lang = 0, mang = 0;
   bool full3d = false;  // diatomic: the general (mu, nu, phi) grid instead of the pure-m one
+  int rule = 0;         // atomic: 0 the reference's rule, 1 Gauss-Legendre cos(theta) x Gauss-Lobatto r
   hfg::XCGridDevice dev;
   bool uploaded = false;
   int64_t Nbf() const { return atomic ? atomic->Nbf() : diatomic->Nbf(); }
@@ -89,8 +90,13 @@ void xc_upload(hf_dftgrid *g) {
   d.lds_pol_ok = 2 * lds <= 64 * 1024;
   if (sizeof(double) * ((size_t)nrad * TILE + 4 * TILE2) > 64 * 1024) throw Error(HF_EINVAL, "radial quadrature too large");
   std::vector<double> xq, wq, xt, wt;
-  gauss_chebyshev2(nrad, xq, wq);
-  gauss_chebyshev2(g->lang, xt, wt);
+  if (g->rule == 1) {
+    gauss_lobatto(nrad, xq, wq);
+    gauss_legendre(g->lang, xt, wt);
+  } else {
+    gauss_chebyshev2(nrad, xq, wq);
+    gauss_chebyshev2(g->lang, xt, wt);
+  }
   const double dphi = 2.0 * kPi / g->mang;
   std::vector<double> R((size_t)Nel * nrad * TILE, 0.0), wpt((size_t)Nel * nrad * g->lang);
   std::vector<double> th((size_t)g->lang * Na), cs((size_t)nd * g->mang);
@@ -187,16 +193,41 @@ hf_dftgrid *ready(hf_dftgrid *g) {
 
 extern "C" {
 
-int hf_atomic_dftgrid_create(hf_atomic *h, int lang, int mang, hf_dftgrid **out) {
+int hf_atomic_dftgrid_create_rule(hf_atomic *h, int lang, int mang, int rule, hf_dftgrid **out) {
   return guarded([&] {
     need(h && out, "NULL argument");
     need(lang >= 1 && mang >= 1, "grid orders must be positive");
+    need(rule == 0 || rule == 1, "rule must be 0 (reference) or 1 (Gauss-Legendre x Gauss-Lobatto)");
+    need(rule == 0 || h->host.rad.nquad >= 2, "a Gauss-Lobatto radial rule needs at least 2 points");
     std::unique_ptr<hf_dftgrid> g(new hf_dftgrid);
     g->ctx = h->ctx;
     g->atomic = &h->host;
     g->lang = lang;
     g->mang = mang;
+    g->rule = rule;
     *out = g.release();
+  });
+}
+
+int hf_atomic_dftgrid_create(hf_atomic *h, int lang, int mang, hf_dftgrid **out) {
+  return hf_atomic_dftgrid_create_rule(h, lang, mang, 0, out);
+}
+
+int hf_quadrature(int kind, int n, double *x, double *w) {
+  return guarded([&] {
+    need(x && w, "NULL argument");
+    need(n >= 1, "n must be positive");
+    std::vector<double> xv, wv;
+    if (kind == 0)
+      gauss_chebyshev2(n, xv, wv);
+    else if (kind == 1)
+      gauss_lobatto(n, xv, wv);
+    else if (kind == 2)
+      gauss_legendre(n, xv, wv);
+    else
+      throw Error(HF_EINVAL, "kind must be 0 (Chebyshev), 1 (Gauss-Lobatto) or 2 (Gauss-Legendre)");
+    std::copy(xv.begin(), xv.end(), x);
+    std::copy(wv.begin(), wv.end(), w);
   });
 }
 

@@ -105,6 +105,42 @@ void gauss_lobatto(int n, std::vector<double> &x, std::vector<double> &w) {
   }
 }
 
+// Gauss-Legendre rule on [-1, 1] (exact for polynomials of degree 2n - 1):
+// Newton on P_n from the Chebyshev guesses cos(pi (i - 1/4) / (n + 1/2)),
+// weights 2 / ((1 - x^2) P_n'(x)^2); nodes increasing.
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
+  if (n < 1) throw std::runtime_error("Gauss-Legendre rule needs n >= 1");
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const double pi = std::acos(-1.0);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(pi * (double(i) + 0.75) / (double(n) + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p0 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p2 = p0;
+        p0 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p0 - (j - 1.0) * p2) / double(j);
+      }
+      dp = double(n) * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::abs(dz) <= 4.0 * std::numeric_limits<double>::epsilon()) break;
+    }
+    {  // derivative at the converged node
+      double p1 = 1.0, p0 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p2 = p0;
+        p0 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p0 - (j - 1.0) * p2) / double(j);
+      }
+      dp = double(n) * (z * p1 - p0) / (z * z - 1.0);
+    }
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
 void gauss_chebyshev2(int n, std::vector<double> &x, std::vector<double> &w) {
   x.assign(n, 0.0);
   w.assign(n, 0.0);

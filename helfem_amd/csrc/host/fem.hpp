@@ -34,6 +34,7 @@ Blk blk_tn(const Blk &A, const Blk &B);  // A' * B
 void gauss_lobatto(int n, std::vector<double> &x, std::vector<double> &w);
 // Modified Gauss-Chebyshev rule of the 2nd kind (libhelfem/include/chebyshev.h:31-55)
 void gauss_chebyshev2(int n, std::vector<double> &x, std::vector<double> &w);
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
 // Element boundaries (libhelfem/include/grid.h:37-101), igrid 1..4
 std::vector<double> element_grid(double rmax, int num_el, int igrid, double zexp);
 

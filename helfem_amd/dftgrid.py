@@ -37,7 +37,7 @@ class DFTGrid:
     for a DiatomicBasis the pure-m grid (purem=True, mang unused) or the
     general 3D grid (purem=False), as the diatomic driver's --purem switch."""
 
-    def __init__(self, basis, lang: int | None = None, mang: int | None = None, purem: bool = True):
+    def __init__(self, basis, lang: int | None = None, mang: int | None = None, purem: bool = True, rule: int = 0):
         from .diatomic import DiatomicBasis
         lmax = int(np.max(basis.lval))
         mmax = int(np.max(np.abs(basis.mval)))
@@ -52,7 +52,7 @@ class DFTGrid:
             check(lib().hf_diatomic_dftgrid3d_create(basis._h, self.lang, self.mang, ctypes.byref(h)))
         else:
             self.mang = 4 * mmax + 12 if mang is None else int(mang)
-            check(lib().hf_atomic_dftgrid_create(basis._h, self.lang, self.mang, ctypes.byref(h)))
+            check(lib().hf_atomic_dftgrid_create_rule(basis._h, self.lang, self.mang, int(rule), ctypes.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -112,3 +112,13 @@ class DFTGrid:
                                                 Ha.data_ptr(), n, Hb.data_ptr(), n, int(beta), thr,
                                                 res.ctypes.data))
         return Ha, Hb, float(res[0]), float(res[1]), float(res[2])
+
+
+def quadrature(kind: str, n: int):
+    """(x, w) of the Chebyshev ("chebyshev", the reference's radial and
+    cos(theta) rule), Gauss-Lobatto ("lobatto") or Gauss-Legendre ("legendre")
+    rule on [-1, 1] (hf_quadrature)."""
+    k = {"chebyshev": 0, "lobatto": 1, "legendre": 2}[kind]
+    x, w = np.zeros(n), np.zeros(n)
+    check(lib().hf_quadrature(k, int(n), x.ctypes.data, w.ctypes.data))
+    return x, w

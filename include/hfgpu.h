@@ -217,6 +217,16 @@ int hf_atomic_batch_exchange_dev(hf_atomic_batch *b, const double *const *dP, co
  * 4 lmax + 12, mang = 4 mmax + 12, atomic/main.cpp:329-330).  The grid keeps
  * a pointer to h: destroy it before the basis. */
 int hf_atomic_dftgrid_create(hf_atomic *h, int lang, int mang, hf_dftgrid **out);
+/* The same grid sizes on another rule: rule 0 is the reference's (above);
+ * rule 1 takes Gauss-Legendre nodes in cos(theta) and Gauss-Lobatto radial
+ * points per element (the same counts), with uniform phi.  No counterpart in
+ * the reference, which has only rule 0 (src/general/angular.cpp:63-70,
+ * src/atomic/dftgrid.cpp:470-576). */
+int hf_atomic_dftgrid_create_rule(hf_atomic *h, int lang, int mang, int rule, hf_dftgrid **out);
+/* Quadrature rules on [-1, 1], nodes increasing: kind 0 the Chebyshev rule of
+ * the radial and cos(theta) grids (libhelfem/include/chebyshev.h:47-50),
+ * 1 Gauss-Lobatto (lobatto.h), 2 Gauss-Legendre. */
+int hf_quadrature(int kind, int n, double *x, double *w);
 int hf_dftgrid_destroy(hf_dftgrid *g);
 /* Pure-m DFT quadrature of a diatomic basis: the mu Chebyshev points of every
  * element x `lang` nu Chebyshev nodes, the phi integral done analytically
