@@ -1,21 +1,32 @@
 #!/usr/bin/env python3
-"""Fock-build benchmark on one MI355X.
+"""Fock-build and SCF benchmark on one MI355X.
 
-A step is one Fock build of the two-electron part: J(P) and K(P/2) on a
-seeded density resident in HBM (FP64 plain mode, the drop-in path).  The
-default workload is diatomic N2 at R = 2.074 bohr in prolate spheroidal
-coordinates: 8 mu-elements to mu_max = arcosh(40 / Rhalf), LIP-15, lmax 16
-for m = 0..3 (Nbf 11894), 14 electrons in sigma/pi blocks.  Atomic
-workloads (Xe lmax=mmax=6, Ne) are selectable with --workload.
+The headline line (`value`) is the N2 diatomic Fock build: a step is one
+build of the two-electron part, J(P) and K(P/2), on a seeded density resident
+in HBM (FP64 plain mode, the drop-in path).  N2 at R = 2.074 bohr in prolate
+spheroidal coordinates: 8 mu-elements to mu_max = arcosh(40 / Rhalf), LIP-15,
+lmax 16 for m = 0..3 (Nbf 11894), 14 electrons in sigma/pi blocks.
 
-Prints one JSON line (rank 0):
-  value        Fock builds/s over the timed steps (inputs already in HBM)
-  roofline     dominant kernel (exchange): algorithmic FP64 flops per launch
-               / average launch time from HIP events on its stream, against
-               the 78.6 TFLOP/s FP64 matrix peak
-  cpu_baseline the CPU oracle (same loop structure as the reference) on the
-               host cores, on a bounded sample of the same workload
-  latency_ms   p50/p99 of per-step Fock-build latency
+The same JSON line carries, under "configs":
+  n2_scf    N2 SCF solves/s, FP64 and mixed (FP32 exchange while the orbital
+            gradient is above 1e-3), after a warm-up solve, setup excluded,
+            legs alternated; the energy difference between them
+  xe_hf     Xe (lmax = mmax = 6, 25 elements, Nbf 17101) HF Fock builds/s
+            (J + K) with the exchange roofline
+  xe_lda    Xe LDA (Slater + VWN5, ldft = mdft = 36) Fock builds/s (J +
+            Vxc) with the HBM roofline of the exchange-correlation kernels
+  xe_scf    Xe HF SCF solves/s, FP64 and mixed, as n2_scf
+  atoms     random atoms drawn like the 512-atom batch (Z 1..86, 4..24
+            elements of order 10 or 15, lmax = mmax 0..4): the vectorised
+            cohort in mixed precision against the per-system FP64 driver
+            solving them one after another (the plain drop-in way)
+
+  roofline     dominant kernel of the headline (the exchange): executed FP64
+               flops per launch / average launch time from HIP events on its
+               stream, against the 78.6 TFLOP/s FP64 matrix peak
+  cpu_baseline the CPU oracle (the reference's loop structure) on the host
+               cores, on a bounded sample of the headline workload
+Other workloads (--workload) print their own line.
 """
 from __future__ import annotations
 
@@ -45,6 +56,9 @@ WORKLOADS = {
                                 "m=0..3, Nbf 11894"),
     "xe-hf-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003,
                          desc="Xe HF atomic, lmax=mmax=6, 25 LIP-15 elements to 40 bohr (exp grid), Nbf 17101"),
+    "xe-lda-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, lda=True,
+                          desc="Xe LDA (Slater + VWN5) atomic, lmax=mmax=6, 25 LIP-15 elements, ldft=mdft=36 "
+                               "(97,200 points per element), Nbf 17101"),
     "ne-hf-atomic": dict(kind="atomic", Z=10, lmax=0, mmax=0, nelem=10, seed=1001,
                          desc="Ne HF atomic, lmax=0, 10 LIP-15 elements to 40 bohr (exp grid), Nbf 139"),
     "ne-hf-atomic-l1": dict(kind="atomic", Z=10, lmax=1, mmax=1, nelem=10, seed=1001,
@@ -54,6 +68,7 @@ WORKLOADS = {
 KERNELS = {"atomic": (("atomic_k_off", "atomic_k_in"), ("atomic_ptile", "atomic_j_gather", "atomic_j_expand")),
            "diatomic": (("diatomic_k_off", "diatomic_k_in"),
                         ("diatomic_ptile", "diatomic_j_gather", "diatomic_j_expand"))}
+XC_KERNELS = ("xc_rad_in", "xc_point", "xc_rad_out")
 
 
 def occupied_shells(Z: int):
@@ -78,8 +93,9 @@ def block_occupations(Z, mvals, lmax):
 
 def measured_traffic(workload):
     """HBM bytes per exchange launch from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE x 2 and WRITE_SIZE x 1, the factors
-    a plain device copy of known size gives), or None when none is committed."""
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by what a plain
+    device copy of known size reads and writes), or None when none is
+    committed."""
     f = ROOT / "profiles" / f"traffic_{workload}.json"
     if not f.exists():
         return None
@@ -95,10 +111,13 @@ def parse():
     ap.add_argument("--workload", default="n2-hf-diatomic", choices=sorted(WORKLOADS) + ["batch-atoms", "serve-diatomic"],
                     help="a Fock-build workload, or batch-atoms / serve-diatomic (many SCF solves on one GPU)")
     ap.add_argument("--systems", type=int, default=0,
-                    help="systems in batch-atoms (default 32) or requests in serve-diatomic (default 16)")
+                    help="systems in batch-atoms and the atoms sub-benchmark (default 64) or requests in "
+                         "serve-diatomic (default 16)")
     ap.add_argument("--rate", type=float, default=2.0, help="serve-diatomic: Poisson arrival rate (1/s)")
+    ap.add_argument("--only", action="store_true",
+                    help="the headline workload alone (no configs sub-benchmarks, no SCF)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-scf", action="store_true", help="skip the timed full SCF solve")
+    ap.add_argument("--no-scf", action="store_true", help="skip the timed SCF solves")
     ap.add_argument("--cpu-sample-stride", type=int, default=0,
                     help="time every n-th exchange output block on the CPU and extrapolate "
                          "(0: 24 for atomic, 64 for diatomic workloads)")
@@ -125,8 +144,240 @@ def seeded_occupations(cfg, hb):
     return [cfg["occ"].get(m, 0) for m in ms]
 
 
+def kernel_stats(ctx, names):
+    from helfem_amd import _lib
+    out = {}
+    for name in names:
+        v = np.zeros(3)
+        _lib.check(_lib.lib().hf_ctx_kernel_stats(ctx.handle, name.encode(), v.ctypes.data))
+        out[name] = v
+    return out
+
+
+def xc_band_bytes(hb):
+    """Algorithmic HBM bytes of one exchange-correlation evaluation: the
+    element-diagonal bands of P read and of H written, over every shell pair
+    the atomic grid couples (all of them), 8 bytes per entry; the dense zero
+    fill of H is reported apart."""
+    per_pair = float(sum(int(n) ** 2 for n in hb.nprim) - (hb.Nel - 1))  # shared corners counted once
+    return 2.0 * 8.0 * hb.Nang * hb.Nang * per_pair
+
+
+def fock_bench(H, cfg, ctx, steps, warmup, hb=None, P=None):
+    """Time `steps` Fock builds (J + K, or J + Vxc for an LDA workload) on the
+    seeded density; per-kernel times from HIP events on the context stream."""
+    import torch
+    from helfem_amd import _lib
+    from helfem_amd.synthetic import seeded_density
+    kind = cfg["kind"]
+    lda = cfg.get("lda", False)
+    knames, jnames = KERNELS[kind]
+    t = time.perf_counter()
+    if hb is None:
+        hb = make_basis(H, cfg, ctx)
+        hb.compute_tei(exchange=not lda)
+    setup_s = time.perf_counter() - t
+    log(f"{cfg['desc']}: Nbf={hb.Nbf}; host setup {setup_s:.1f}s")
+    S = hb.overlap()
+    if P is None:
+        P = seeded_density(S, hb.get_sym_idx(1), seeded_occupations(cfg, hb), seed=cfg["seed"])
+    nelec = float(np.sum(P * S))
+    dev = torch.device("cuda:0")
+    Pd = torch.as_tensor(P, device=dev)
+    Pa = 0.5 * Pd
+    J = torch.empty_like(Pd)
+    K = torch.empty_like(Pd)
+    grid = None
+    if lda:
+        from helfem_amd.dftgrid import DFTGrid
+        grid = DFTGrid(hb, lang=36, mang=36)
+    ctx.bind_torch_stream()
+    xc_out = {}
+
+    def build():
+        hb.coulomb(Pd, out=J)
+        if lda:
+            _, xc_out["exc"], xc_out["nel"], _ = grid.eval_Fxc(1, 7, Pd, out=K)
+        else:
+            hb.exchange(Pa, out=K)
+
+    t = time.perf_counter()
+    for _ in range(warmup):
+        build()
+    torch.cuda.synchronize()
+    log(f"warmup {warmup} steps {time.perf_counter() - t:.2f}s (includes device upload of the caches)")
+    _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 1))
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record()
+        build()
+        ends[i].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
+    names = (XC_KERNELS if lda else knames) + jnames
+    stats = kernel_stats(ctx, names)
+    _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 0))
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    Ecoul = 0.5 * float(torch.sum(Pd * J))
+    out = {"workload": cfg["desc"], "Nbf": hb.Nbf, "fock_builds_per_s": steps / (t1 - t0),
+           "ms_per_build": ms_per_step, "steps": steps,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "kernels_ms": {k: (v[1] / max(v[0], 1)) for k, v in stats.items()},
+           "density": f"seeded SplitMix64 (seed {cfg['seed']}), P = 2CC' per m-block, tr(PS) = {nelec:.3f}",
+           "host_setup_s": setup_s, "energies": {"coulomb": Ecoul}}
+    if lda:
+        x_ms = sum(stats[n][1] / max(stats[n][0], 1) for n in XC_KERNELS)
+        nbytes = xc_band_bytes(hb)
+        out["energies"].update(exc=xc_out["exc"], nel=xc_out["nel"])
+        out["roofline"] = {"bound": "hbm", "kernel": "k_xc_rad_in + k_xc_point + k_xc_rad_out (one Vxc evaluation)",
+                           "achieved": nbytes / (x_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": nbytes / (x_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "avg_launch_ms": x_ms, "bytes_per_launch": nbytes,
+                           "note": "bytes: element-diagonal P bands in + H bands out over all shell pairs"}
+        log(f"{steps} LDA builds {ms_per_step:.3f} ms/step; XC kernels {x_ms:.3f} ms -> "
+            f"{out['roofline']['achieved']:.0f} GB/s")
+    else:
+        k_ms = sum(stats[n][1] / max(stats[n][0], 1) for n in knames)
+        k_flops = sum(stats[n][2] / max(stats[n][0], 1) for n in knames)
+        achieved = k_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+        out["energies"]["exchange"] = float(torch.sum(Pa * K))
+        out["kernels_tflops"] = {k: (v[2] / max(v[1], 1e-30) / 1e9) for k, v in stats.items() if v[2] > 0}
+        out["exchange_work"] = hb.work()
+        out["roofline"] = {"bound": "mfma", "kernel": {"atomic": "k_atomic_k_off_f64 + k_atomic_k_in_f64",
+                                                       "diatomic": "k_dia_k2_off<double> + k_dia_kin_lds<double>"}[kind] +
+                                                      " (exchange: cross- and in-element launches of one build)",
+                           "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                           "avg_launch_ms": k_ms, "flops_per_launch": k_flops}
+        log(f"{steps} steps {ms_per_step:.3f} ms/step; exchange kernel {k_ms:.3f} ms, "
+            f"{k_flops / 1e12:.4f} TFLOP/launch -> {achieved:.2f} TFLOP/s")
+        out["split_exchange"] = split_exchange(hb, Pd, S, cfg)
+    return out, hb, P
+
+
+def split_exchange(hb, Pd, S, cfg, frac=1e-3, reps=3):
+    """The mixed SCF's late-iteration exchange K64(P_ref) + K32(P - P_ref)
+    on a density that moved a `frac` fraction from P_ref: its error against
+    the FP64 build of P, and the time of its per-iteration part (one FP32
+    build) against one FP64 build (K64(P_ref) is kept from an earlier
+    iteration)."""
+    import torch
+    from helfem_amd.synthetic import seeded_density
+    X = torch.as_tensor(seeded_density(S, hb.get_sym_idx(1), seeded_occupations(cfg, hb), seed=cfg["seed"] + 1),
+                        device=Pd.device)
+    Pa = 0.5 * Pd
+    Pn = Pa + frac * torch.max(torch.abs(Pa)) / torch.max(torch.abs(X)) * X
+    Kref = hb.exchange(Pa)
+    dP = Pn - Pa
+    times = {}
+    for name, fn in (("fp64_ms", lambda: hb.exchange(Pn)), ("fp32_increment_ms", lambda: hb.exchange(dP, precision="fp32"))):
+        fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        times[name] = ev[0].elapsed_time(ev[1]) / reps
+    K64 = hb.exchange(Pn)
+    Ks = Kref + hb.exchange(dP, precision="fp32")
+    err = float(torch.max(torch.abs(Ks - K64)) / torch.max(torch.abs(K64)))
+    K32 = hb.exchange(Pn, precision="fp32")
+    err32 = float(torch.max(torch.abs(K32 - K64)) / torch.max(torch.abs(K64)))
+    log(f"split exchange: FP64 {times['fp64_ms']:.3f} ms, FP32 increment {times['fp32_increment_ms']:.3f} ms, "
+        f"rel err {err:.2e} (plain FP32 {err32:.2e})")
+    return dict(times, density_change=frac, rel_err_vs_fp64=err, plain_fp32_rel_err_vs_fp64=err32,
+                speedup=times["fp64_ms"] / times["fp32_increment_ms"],
+                note="per-iteration cost of K64(P_ref) + K32(P - P_ref) is one FP32 build plus an add; "
+                     "errors against the FP64 GPU build, which meets the oracle to 1e-12")
+
+
+def scf_compare(hb, nel, xc=None, legs=2):
+    """FP64 and mixed-precision SCF solves of one basis: a warm-up solve
+    first (device uploads, allocator), then `legs` solves of each precision in
+    alternating order (fp64, mixed, mixed, fp64, ...); solve time excludes the
+    setup phase (one-electron matrices and orthogonalisers)."""
+    from helfem_amd.scf import rhf
+    rhf(hb, nel, conv=1e-10, precision="fp64", xc=xc)
+    order = []
+    for q in range(legs):
+        order += ["fp64", "mixed"] if q % 2 == 0 else ["mixed", "fp64"]
+    res = {"fp64": [], "mixed": []}
+    for prec in order:
+        r = rhf(hb, nel, conv=1e-10, precision=prec, xc=xc)
+        solve = r.seconds - r.phases.get("setup", 0.0)
+        res[prec].append((solve, r))
+        log(f"SCF {prec}: E = {r.E:.10f} in {r.iterations} iterations ({r.iterations_fp32} with FP32 exchange), "
+            f"{solve:.3f}s without setup ({r.seconds:.3f}s total)")
+    out = {}
+    for prec, rs in res.items():
+        ts = [t for t, _ in rs]
+        r = rs[-1][1]
+        out[prec] = {"solves_per_s": 1.0 / float(np.mean(ts)), "seconds_per_solve": [round(t, 4) for t in ts],
+                     "iterations": r.iterations, "iterations_fp32_exchange": r.iterations_fp32, "energy": r.E,
+                     "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
+                     "eigensolves": r.eigensolves, "skipped_eigensolves": r.skipped_solves}
+    out["mixed_vs_fp64"] = {"speedup": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"],
+                            "energy_difference_Eh": out["mixed"]["energy"] - out["fp64"]["energy"]}
+    out["convergence"] = "dE < 1e-10 Eh and orbital gradient < 1e-6, core-Hamiltonian guess"
+    out["timing"] = f"warm-up solve, then {legs} solves per precision in alternating order; setup excluded"
+    return out
+
+
+def atoms_bench(H, ctx, n, seed=1004, nseq=None):
+    """Random atoms drawn like the 512-atom batch: the vectorised cohort in
+    mixed precision (and FP64) against the per-system FP64 driver solving the
+    same atoms one after another."""
+    from helfem_amd import batch
+    from helfem_amd.cohort import Cohort
+    t = time.perf_counter()
+    systems = batch.random_atoms(n, seed=seed, ctx=ctx, zmax=86, nel_range=(4, 24), orders=(10, 15),
+                                 lmax_range=(0, 4), setup_threads=16)
+    setup = time.perf_counter() - t
+    nbf = [s.info["Nbf"] for s in systems]
+    log(f"{n} atoms, Nbf {min(nbf)}..{max(nbf)}; host setup {setup:.1f}s")
+    Cohort(systems[:2], ctx, conv=1e-8).solve()  # warm-up: device uploads of two systems' tables
+    pinned = sum(s.occ is not None for s in systems)
+    out = {"workload": f"{n} random neutral atoms (seed {seed}): Z 1..86, 4..24 LIP elements of order 10 or 15, "
+                       f"lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh", "host_setup_s": setup,
+           "Nbf_range": [min(nbf), max(nbf)],
+           "occupations": f"{pinned} of {n} with the fixed per-block occupations of their Madelung/Hund "
+                          f"configuration (the reference's --readocc); the rest (lmax below an occupied l) "
+                          f"by aufbau"}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        t = time.perf_counter()
+        co = Cohort(systems, ctx, precision=prec, conv=1e-8, occupations=[s.occ for s in systems])
+        r = co.solve()
+        dt = time.perf_counter() - t
+        res[prec] = r
+        out[f"cohort_{prec}"] = {"solves_per_s": n / dt, "seconds": dt,
+                                 "converged": int(sum(x["converged"] for x in r)),
+                                 "iterations_mean": float(np.mean([x["iterations"] for x in r])),
+                                 "iterations_max": int(max(x["iterations"] for x in r)),
+                                 "phases_s": {k: round(v, 3) for k, v in co.phases.items()},
+                                 "eigensolves": r[0]["eigensolves"], "skipped_eigensolves": r[0]["skipped_eigensolves"],
+                                 "autotuned_waves": co.kbatch.tuning()[0].tolist()}
+        log(f"cohort {prec}: {n / dt:.2f} solves/s ({dt:.2f}s)")
+    nseq = n if nseq is None else nseq
+    seq, dt = batch.solve_sequential(systems[:nseq], precision="fp64", conv=1e-8)
+    out["sequential_fp64"] = {"solves_per_s": nseq / dt, "seconds": dt, "systems": nseq,
+                              "engine": "per-system FP64 driver (helfem_amd.scf), one system after another"}
+    log(f"sequential FP64: {nseq / dt:.2f} solves/s ({dt:.2f}s)")
+    out["mixed_cohort_vs_sequential_fp64"] = out["cohort_mixed"]["solves_per_s"] / out["sequential_fp64"]["solves_per_s"]
+    out["max_abs_dE_mixed_vs_fp64_cohort_Eh"] = max(abs(a["E"] - b["E"]) for a, b in zip(res["mixed"], res["fp64"]))
+    out["max_abs_dE_cohort_vs_sequential_Eh"] = max(abs(a["E"] - b.E) for a, b in zip(res["fp64"][:nseq], seq))
+    return out
+
+
 def cpu_baseline(cfg, P, nblocks_total, stride):
-    """Oracle J (full) and K (strided sample of output blocks, extrapolated)."""
+    """Oracle J (full) and K (strided sample of output blocks, extrapolated),
+    on all host threads, plus the K sample on one thread."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -146,44 +397,48 @@ def cpu_baseline(cfg, P, nblocks_total, stride):
     tKs = time.perf_counter() - t
     tK = tKs * nblocks_total / sample.size
     log(f"oracle J {tJ:.2f}s, K sample {sample.size}/{nblocks_total} blocks {tKs:.2f}s -> K {tK:.1f}s")
-    return {"value": 1.0 / (tJ + tK), "unit": "Fock builds/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/ (C++ -O3, OpenMP {threads} threads): one full coulomb + exchange over "
-                      f"{sample.size} of {nblocks_total} output blocks (every {stride}th), "
-                      f"extrapolated; measured {tJ + tKs:.1f}s of CPU work",
-            "seconds": {"coulomb": tJ, "exchange_estimated": tK}}
+    out = {"value": 1.0 / (tJ + tK), "unit": "Fock builds/s", "cores": threads, "kind": "port",
+           "sample": f"oracle/ (C++ -O3, OpenMP {threads} threads): one full coulomb + exchange over "
+                     f"{sample.size} of {nblocks_total} output blocks (every {stride}th), "
+                     f"extrapolated; measured {tJ + tKs:.1f}s of CPU work",
+           "seconds": {"coulomb": tJ, "exchange_estimated": tK}}
+    # one thread: a smaller sample of K; J scaled by the K sample's thread speedup
+    one = sample[::8]
+    O.set_threads(1)
+    try:
+        t = time.perf_counter()
+        ob.exchange_blocks(0.5 * P, one)
+        tK1s = time.perf_counter() - t
+    finally:
+        O.set_threads(threads)
+    t = time.perf_counter()
+    ob.exchange_blocks(0.5 * P, one)
+    tKns = time.perf_counter() - t
+    speed = tK1s / max(tKns, 1e-9)
+    tK1 = tK1s * nblocks_total / one.size
+    tJ1 = tJ * speed
+    out["one_thread"] = {"value": 1.0 / (tJ1 + tK1), "unit": "Fock builds/s", "cores": 1,
+                         "sample": f"exchange over {one.size} of {nblocks_total} blocks on 1 thread, extrapolated; "
+                                   f"coulomb estimated as the {threads}-thread time x the measured thread speedup "
+                                   f"of that sample ({speed:.1f})"}
+    log(f"oracle 1 thread: K sample {one.size} blocks {tK1s:.2f}s (speedup {speed:.1f} on {threads} threads)")
+    return out
 
 
 def main_batch(args):
-    """Many independent atomic SCF solves through one lockstep engine (shared
-    padded batched eigensolves, admission by HBM footprint), against the same
-    systems solved one after another."""
+    """Random atoms through the vectorised cohort against the per-system
+    driver one after another (the atoms sub-benchmark as its own line)."""
     import helfem_amd as H
-    from helfem_amd import batch
-    n = args.systems or 32
     ctx = H.Context(0)
-    t = time.perf_counter()
-    systems = batch.random_atoms(n, seed=1004, ctx=ctx, zmax=86, nel_range=(4, 24), orders=(10, 15),
-                                 lmax_range=(0, 4), setup_threads=16)
-    log(f"{n} atoms, Nbf {min(s.info['Nbf'] for s in systems)}..{max(s.info['Nbf'] for s in systems)}; "
-        f"host setup {time.perf_counter() - t:.1f}s")
-    out = {}
-    for prec in ("mixed", "fp64"):
-        rb, tb = batch.solve_batched(systems, precision=prec, conv=1e-8)
-        log(f"{prec}: batched {tb:.1f}s")
-        rs, ts = batch.solve_sequential(systems, precision=prec, conv=1e-8)
-        out[prec] = {"batched_solves_per_s": n / tb, "sequential_solves_per_s": n / ts,
-                     "iterations": sum(r.iterations for r in rb),
-                     "max_abs_dE_batched_vs_sequential": max(abs(a.E - b.E) for a, b in zip(rb, rs))}
-        log(f"{prec}: batched {n / tb:.2f} solves/s, sequential {n / ts:.2f}")
+    n = args.systems or 64
+    out = atoms_bench(H, ctx, n)
+    v = out["cohort_mixed"]["solves_per_s"]
     print(json.dumps({
-        "metric": METRIC, "value": out["mixed"]["batched_solves_per_s"], "unit": "SCF solves/s",
-        "n_gpus": 1, "steps": n, "warmup": 0, "ms_per_step": 1e3 / out["mixed"]["batched_solves_per_s"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (mixed: f32 exchange early)",
-        "data": "synthetic",
-        "config": {"workload": f"{n} random neutral atoms (seed 1004): Z 1..86, 4..24 LIP elements of order 10 or "
-                               f"15, lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh",
-                   "parallelism": "single GPU, one lockstep engine"},
-        "solves": out, "cpu_baseline": None}), flush=True)
+        "metric": METRIC, "value": v, "unit": "SCF solves/s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": 1e3 / v, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (mixed: f32 exchange early)", "data": "synthetic",
+        "config": {"workload": out["workload"], "parallelism": "single GPU, one vectorised cohort"},
+        "atoms": out, "cpu_baseline": None}), flush=True)
 
 
 def main_serve(args):
@@ -224,128 +479,67 @@ def main():
         return main_serve(args)
     import torch
     import helfem_amd as H
-    from helfem_amd import _lib
-    from helfem_amd.synthetic import seeded_density
 
     cfg = WORKLOADS[args.workload]
-    kind = cfg["kind"]
-    knames, jnames = KERNELS[kind]
     ctx = H.Context(0)
-    t = time.perf_counter()
-    hb = make_basis(H, cfg, ctx)
-    hb.compute_tei()
-    log(f"basis Nbf={hb.Nbf} Nrad={hb.Nrad} Nang={hb.Nang}; host setup {time.perf_counter() - t:.1f}s")
-    S = hb.overlap()
-    blocks = hb.get_sym_idx(1)
-    nocc = seeded_occupations(cfg, hb)
-    P = seeded_density(S, blocks, nocc, seed=cfg["seed"])
-    nelec = float(np.sum(P * S))
-    log(f"seeded density: tr(PS) = {nelec:.6f}, occupied orbitals per m-block {nocc}")
-
-    dev = torch.device("cuda:0")
-    Pd = torch.as_tensor(P, device=dev)
-    Pa = 0.5 * Pd
-    J = torch.empty_like(Pd)
-    K = torch.empty_like(Pd)
-    ctx.bind_torch_stream()
-    t = time.perf_counter()
-    for _ in range(args.warmup):
-        hb.coulomb(Pd, out=J)
-        hb.exchange(Pa, out=K)
-    torch.cuda.synchronize()
-    log(f"warmup {args.warmup} steps {time.perf_counter() - t:.2f}s (includes device upload of the caches)")
-    _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 1))
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record()
-        hb.coulomb(Pd, out=J)
-        hb.exchange(Pa, out=K)
-        ends[i].record()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
-    stats = {}
-    for name in knames + jnames:
-        out = np.zeros(3)
-        _lib.check(_lib.lib().hf_ctx_kernel_stats(ctx.handle, name.encode(), out.ctypes.data))
-        stats[name] = out
-    _lib.check(_lib.lib().hf_ctx_set_timing(ctx.handle, 0))
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    k_ms = sum(stats[n][1] / max(stats[n][0], 1) for n in knames)
-    k_flops = sum(stats[n][2] / max(stats[n][0], 1) for n in knames)
-    achieved = k_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
-    log(f"{args.steps} steps {ms_per_step:.3f} ms/step; exchange kernel {k_ms:.3f} ms, "
-        f"{k_flops / 1e12:.4f} TFLOP/launch -> {achieved:.2f} TFLOP/s")
-
+    head, hb, P = fock_bench(H, cfg, ctx, args.steps, args.warmup)
     if args.copy_calibration:
         nbytes = 1 << 30
-        a = torch.empty(nbytes // 8, dtype=torch.float64, device=dev)
+        a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda:0")
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"copy calibration: 3 x {nbytes} bytes copied")
-
-    Ecoul = 0.5 * float(torch.sum(Pd * J))
-    Exx = float(torch.sum(Pa * K))
-    log(f"Ecoul {Ecoul:.10f} Exx {Exx:.10f}")
-
+    roof = dict(head["roofline"])
+    if roof["bound"] == "mfma":
+        roof["traffic"] = measured_traffic(args.workload)
     result = {
         "metric": METRIC,
-        "value": args.steps / (t1 - t0),
+        "value": head["fock_builds_per_s"],
         "unit": "Fock builds/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_build"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": cfg["desc"], "mode": "plain FP64 J+K (HF Fock build)", "Nbf": hb.Nbf,
-                   "density": f"seeded SplitMix64 (seed {cfg['seed']}), P = 2CC' per m-block, "
-                              f"tr(PS) = {nelec:.3f}",
-                   "parallelism": "single GPU"},
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "roofline": {"bound": "mfma", "kernel": {"atomic": "k_atomic_k_off_f64 + k_atomic_k_in_f64",
-                                                 "diatomic": "k_dia_k2_off<double> + k_dia_kin_lds<double>"}[kind] +
-                                                " (exchange: cross- and in-element launches of one build)",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args.workload),
-                     "avg_launch_ms": k_ms, "flops_per_launch": k_flops},
-        "kernels_ms": {k: (v[1] / max(v[0], 1)) for k, v in stats.items()},
-        "kernels_tflops": {k: (v[2] / max(v[1], 1e-30) / 1e9) for k, v in stats.items() if v[2] > 0},
-        "energies": {"coulomb": Ecoul, "exchange": Exx},
-        "exchange_work": hb.work(),
+        "config": {"workload": cfg["desc"],
+                   "mode": "plain FP64 J+Vxc (LDA Fock build)" if cfg.get("lda") else "plain FP64 J+K (HF Fock build)",
+                   "Nbf": hb.Nbf, "density": head["density"], "parallelism": "single GPU"},
+        "latency_ms": head["latency_ms"],
+        "roofline": roof,
+        "kernels_ms": head["kernels_ms"],
+        "energies": head["energies"],
     }
-    if not args.no_scf:
-        from helfem_amd.scf import rhf
-        nel = int(round(nelec))
-        scf = {}
-        for prec in ("fp64", "mixed"):
-            t = time.perf_counter()
-            r = rhf(hb, nel, conv=1e-10, precision=prec)
-            tsolve = time.perf_counter() - t
-            log(f"SCF {prec}: E = {r.E:.10f} in {r.iterations} iterations ({r.iterations_fp32} with FP32 "
-                f"exchange), {tsolve:.2f}s")
-            scf[prec] = {"solves_per_s": 1.0 / tsolve, "seconds": tsolve, "iterations": r.iterations,
-                         "iterations_fp32_exchange": r.iterations_fp32, "energy": r.E,
-                         "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
-                         "eigensolves": r.eigensolves, "reused_eigensolves": r.reused_solves,
-                         "skipped_eigensolves": r.skipped_solves}
-        scf["mixed_vs_fp64"] = {"speedup": scf["mixed"]["solves_per_s"] / scf["fp64"]["solves_per_s"],
-                                "energy_difference_Eh": scf["mixed"]["energy"] - scf["fp64"]["energy"]}
-        scf["convergence"] = "dE < 1e-10 Eh and orbital gradient < 1e-6, core-Hamiltonian guess"
-        scf["precision"] = ("fp64: FP64 throughout; mixed: FP32 exchange until the orbital gradient is below "
-                            "1e-3, FP64 after (overlap, eigensolves and energies always FP64)")
-        result["scf"] = scf
+    for k in ("kernels_tflops", "exchange_work"):
+        if k in head:
+            result[k] = head[k]
+    configs = {}
+    nel = int(round(float(np.sum(P * hb.overlap()))))
+    nang = hb.Nang
+    if not args.no_scf and not args.only and not cfg.get("lda"):
+        configs[args.workload.split("-")[0] + "_scf"] = scf_compare(hb, nel)
+    del hb
+    torch.cuda.empty_cache()
+    if not args.only and args.workload == "n2-hf-diatomic":
+        xe, xb, xP = fock_bench(H, WORKLOADS["xe-hf-atomic"], ctx, max(5, args.steps // 2), 1)
+        configs["xe_hf"] = xe
+        lda, _, _ = fock_bench(H, WORKLOADS["xe-lda-atomic"], ctx, 3, 1, hb=xb, P=xP)
+        configs["xe_lda"] = lda
+        if not args.no_scf:
+            configs["xe_scf"] = scf_compare(xb, 54)
+        del xb, xP
+        torch.cuda.empty_cache()
+        configs["atoms"] = atoms_bench(H, ctx, args.systems or 64)
+    if configs:
+        result["configs"] = configs
     if not args.no_cpu_baseline:
-        stride = args.cpu_sample_stride or (24 if kind == "atomic" else 64)
-        result["cpu_baseline"] = cpu_baseline(cfg, P, hb.Nang * hb.Nang, stride)
+        stride = args.cpu_sample_stride or (24 if cfg["kind"] == "atomic" else 64)
+        result["cpu_baseline"] = cpu_baseline(cfg, P, nang * nang, stride)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

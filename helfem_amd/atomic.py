@@ -166,10 +166,22 @@ class AtomicBasis:
         return self._one_e(2)
 
     def get_sym_idx(self, symm=1):
-        """TwoDBasis.cpp:202-228: one index list per unique m (symm=1)."""
+        """TwoDBasis.cpp:200-227: one index list per unique m (symm=1) or per
+        (l, m) shell (symm=2).  symm=3 splits each m by the parity of l (even
+        before odd): a density of definite-parity orbitals never couples the
+        two."""
         if symm == 0:
             return [np.arange(self.Nbf)]
+        if symm == 2:  # one block per (l, m) shell, in shell order
+            return [np.arange(a * self.Nrad, (a + 1) * self.Nrad) for a in range(self.Nang)]
         out = []
+        if symm == 3:
+            for m in sorted(set(self.mval.tolist())):
+                for par in (0, 1):
+                    sh = [a for a in range(self.Nang) if self.mval[a] == m and self.lval[a] % 2 == par]
+                    if sh:
+                        out.append(np.concatenate([np.arange(a * self.Nrad, (a + 1) * self.Nrad) for a in sh]))
+            return out
         for m in sorted(set(self.mval.tolist())):
             out.append(np.concatenate([np.arange(a * self.Nrad, (a + 1) * self.Nrad)
                                        for a in range(self.Nang) if self.mval[a] == m]))

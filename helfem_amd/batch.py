@@ -34,6 +34,9 @@ class System:
     grid: object = None
     arrival: float = 0.0
     info: dict = field(default_factory=dict)
+    # fixed occupations (alpha, beta) per (l, m) shell block of the ground-state
+    # configuration (occupations.block_counts, symmetry 2), or None: aufbau
+    occ: tuple = None
 
     @property
     def restricted(self):
@@ -65,8 +68,16 @@ def state_for(sys_, precision="fp64", conv=1e-10, maxit=200):
     """SCFState of a system: RHF/RKS with nela doubly occupied orbitals when
     the spins pair up, UHF/UKS otherwise."""
     nel = (sys_.nela,) if sys_.restricted else (sys_.nela, sys_.nelb)
-    return SCFState(sys_.basis, nel, sys_.restricted, conv=conv, maxit=maxit, xc=_xc_for(sys_),
-                    precision=precision, tag=sys_.tag)
+    from .atomic import AtomicBasis
+    # atoms: orbitals of pure (l, m) (the reference's symmetry=2), with the
+    # configuration's fixed occupations when the basis holds them (as the
+    # vectorised cohort)
+    atomic = isinstance(sys_.basis, AtomicBasis)
+    occ = None
+    if atomic and sys_.occ is not None:
+        occ = sys_.occ[:1] if sys_.restricted else sys_.occ
+    return SCFState(sys_.basis, nel, sys_.restricted, symm=2 if atomic else 1, conv=conv, maxit=maxit,
+                    xc=_xc_for(sys_), precision=precision, tag=sys_.tag, occupations=occ)
 
 
 def solve_batched(systems, precision="fp64", conv=1e-10, budget_bytes=64 << 30):
@@ -113,7 +124,8 @@ _MADELUNG = sorted(((n, l) for n in range(1, 8) for l in range(0, n)), key=lambd
 
 def _spins(Z):
     """Ground-state-like spin counts: Hund's rule on the last open subshell of
-    the Madelung filling (nela - nelb unpaired electrons)."""
+    the Madelung filling (nela - nelb unpaired electrons); equal to
+    occupations.spins."""
     left = Z
     for n, l in _MADELUNG:
         cap = 2 * (2 * l + 1)
@@ -146,12 +158,18 @@ def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax
 
     with ThreadPoolExecutor(setup_threads) as ex:
         bases = list(ex.map(build, specs))
-    out = []
-    for spec, b in zip(specs, bases):
-        na, nb = _spins(spec[0])
-        out.append(System(b, na, nb, tag=spec, info=dict(Z=spec[0], nelem=spec[1], nnodes=spec[2], lmax=spec[3],
-                                                          Nbf=b.Nbf)))
-    return out
+    return [atom_system(b, spec[0], tag=spec, info=dict(Z=spec[0], nelem=spec[1], nnodes=spec[2], lmax=spec[3],
+                                                        Nbf=b.Nbf)) for spec, b in zip(specs, bases)]
+
+
+def atom_system(basis, Z, tag=None, info=None):
+    """A neutral atom with Hund's-rule spins and the fixed ground-state
+    occupations of its Madelung configuration where the basis holds every
+    occupied (l, m) (else None: aufbau)."""
+    from . import occupations as oc
+    na, nb = oc.spins(Z)
+    return System(basis, na, nb, tag=tag, info=dict(info or {}),
+                  occ=oc.block_counts(Z, basis.lval, basis.mval, symm=2))
 
 
 def random_diatomic_specs(n, seed, zmax=18, r_range=(1.0, 6.0), nel_range=(4, 12), lmax_range=(8, 24),

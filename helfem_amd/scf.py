@@ -122,6 +122,16 @@ class _Blocks:
                     sh.append((lval[a], off, n))
                     off += n
                 self.shells.append(sh)
+        elif symm == 3:  # (m, parity of l) blocks of an atomic basis
+            for m in sorted(set(mval)):
+                for par in (0, 1):
+                    sh, off = [], 0
+                    for a in range(len(lval)):
+                        if mval[a] == m and lval[a] % 2 == par:
+                            sh.append((lval[a], off, basis.Nrad))
+                            off += basis.Nrad
+                    if sh:
+                        self.shells.append(sh)
         else:
             self.shells = [[(-1, 0, len(b))] for b in idx]
         for k, sh in enumerate(self.shells):
@@ -267,10 +277,11 @@ class _Blocks:
         return self.torch.cat([self.torch.arange(sh[s_][1], sh[s_][1] + sh[s_][2], device=self.dev)
                                for s_ in comp])
 
-    def prepare(self, Fb, slot=0):
+    def prepare(self, Fb, slot=0, frozen=None):
         """Independent eigenproblems X' F X of this Fock matrix (one per
         distinct shell group and not excluded by the Weyl bound), with the
-        bookkeeping `finish` needs.  `slot` names the spin channel."""
+        bookkeeping `finish` needs.  `slot` names the spin channel; with
+        `frozen` (orbitals per block) blocks holding none are not solved."""
         torch = self.torch
         subs, loc, reuse = self._subproblems(Fb)
         uniq = [i for i in range(len(subs)) if reuse[i] == i]
@@ -281,8 +292,14 @@ class _Blocks:
             mats[i] = X[i].T @ Fb[k][li][:, li] @ X[i]
         keys = {i: (slot, subs[i][0], tuple(subs[i][1])) for i in uniq}
         skipped = {}
+        if frozen is not None:
+            # solutions another sub-problem reuses stay needed
+            needed = {reuse[j] for j in range(len(subs)) if frozen[subs[j][0]] > 0}
+            for i in uniq:
+                if frozen[subs[i][0]] == 0 and i not in needed:
+                    skipped[i] = np.inf
         homo = self.homo.get(slot)
-        cand = [i for i in uniq if homo is not None and keys[i] in self.memo
+        cand = [i for i in uniq if homo is not None and i not in skipped and keys[i] in self.memo
                 and self.memo[keys[i]][0].shape == mats[i].shape]
         if cand:
             dn = torch.stack([torch.linalg.matrix_norm(mats[i] - self.memo[keys[i]][0]) for i in cand]).cpu().numpy()
@@ -328,7 +345,8 @@ class _Blocks:
         take(plan["uniq"], sols)
         order = aufbau()
         homo = order[nocc - 1][0] if nocc > 0 else -np.inf
-        late = [i for i, lb in plan["skipped"].items() if frozen is not None or not lb > homo]
+        late = [i for i, lb in plan["skipped"].items()
+                if (frozen[subs[i][0]] > 0 if frozen is not None else not lb > homo)]
         if late:  # the occupied levels rose past a bound: solve those after all
             self.solves += len(late)
             self.skipped_solves -= len(late)
@@ -482,7 +500,7 @@ class SCFState:
     nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
 
     def __init__(self, basis, nel_spin, restricted, symm=1, conv=1e-10, maxit=100, diis_len=10, verbose=False,
-                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None):
+                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
@@ -501,7 +519,16 @@ class SCFState:
         self.nspin = 1 if restricted else 2
         self.weight = 2.0 if restricted else 1.0
         self.fp32 = precision == "mixed"
+        self.mixed = precision == "mixed"
         self.it_fp32 = 0
+        # incremental exchange of the mixed mode after the FP32 phase:
+        # K(P) = K64(P_ref) + K32(P - P_ref) against the last FP64 build, while
+        # max|P - P_ref| <= INC_MAX max|P|; convergence is confirmed by a full
+        # FP64 build
+        self.kref = [None] * (1 if restricted else 2)
+        self.final_check = False
+        self.it_inc = 0
+        self.inc_now = False
         self.diis = _DIIS(diis_len)
         self.Eold, self.E, self.it, self.hist = None, 0.0, 0, []
         z = torch.zeros((), dtype=torch.float64, device=self.dev)
@@ -511,9 +538,14 @@ class SCFState:
         # aufbau flip-flops between patterns (near-degenerate frontier levels)
         self.occ_hist = [[] for _ in range(self.nspin)]
         self.frozen = [None] * self.nspin
+        if occupations is not None:
+            # fixed orbitals per block and spin (the reference's --readocc)
+            self.frozen = [list(occupations[sp]) for sp in range(self.nspin)]
         Hb = self.bs.blocks_of(self.H)
         self.pending = [(s, Hb) for s in range(self.nspin)]
         self.done = False
+        self.stream = None  # the stream of this state's Fock builds in an overlapping engine
+        self._built = None
 
     def requests(self):
         """(spin, Fock blocks, occupied orbitals) still to be diagonalised."""
@@ -523,6 +555,7 @@ class SCFState:
         self.dens[s] = dens
         self.occ_hist[s].append(tuple(dens[2]))
 
+    INC_MAX = 1e-2  # largest max|P - P_ref| / max|P| of an incremental exchange build
     FLIPS = 3       # occupation changes within the last WINDOW iterations that freeze them
     WINDOW = 10
 
@@ -545,41 +578,98 @@ class SCFState:
     def advance(self):
         """One Fock build, energy and convergence test on the delivered
         densities; leaves the next eigenproblems in `pending` or sets done."""
+        if self.launch():
+            self.complete()
+
+    def launch(self, stream=None):
+        """Queue this iteration's Fock build (J, and K or Vxc) -- on `stream`
+        when given, so the builds of several states overlap on the device
+        (the exchange's screen synchronises only that stream).  False when
+        the state is done."""
+        import contextlib
         torch = self.torch
         self.pending = []
         if self.it >= self.maxit:
             self.done = True
-            return
+            return False
         self.it += 1
-        clk, bs, nspin = self.clk, self.bs, self.nspin
-        f64 = torch.float64
+        clk, nspin = self.clk, self.nspin
+        tick = clk.tick if stream is None else (lambda name: None)
         Ps = [d[0] for d in self.dens]
         P = Ps[0] if self.restricted else Ps[0] + Ps[1]
-        J = self.basis.coulomb(P)
-        clk.tick("coulomb")
+        scope = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+        made = []
+        with scope:
+            if stream is not None:
+                for X in Ps + [P]:
+                    X.record_stream(stream)
+            J = self.basis.coulomb(P)
+            made.append(J)
+            tick("coulomb")
+            Ks, Vs, Exc = [None] * nspin, [None] * nspin, None
+            if self.xc is None:
+                spins = [s for s in range(nspin) if self.nel_spin[s] > 0]
+                Psps = {s: (0.5 * Ps[s] if self.restricted else Ps[s]) for s in spins}
+                # incremental only when every spin's change from its reference is small
+                inc = (self.mixed and not self.fp32 and not self.final_check
+                       and all(self.kref[s] is not None for s in spins))
+                if inc:
+                    rel = torch.stack([torch.max(torch.abs(Psps[s] - self.kref[s][0])) / torch.max(torch.abs(Psps[s]))
+                                       for s in spins])
+                    inc = bool(torch.max(rel) <= self.INC_MAX)
+                self.inc_now = inc
+                for s in spins:
+                    Psp = Psps[s]
+                    if self.fp32:
+                        K = self.basis.exchange(Psp, precision="fp32")
+                    elif inc:
+                        K = self.kref[s][1] + self.basis.exchange(Psp - self.kref[s][0], precision="fp32")
+                    else:
+                        K = self.basis.exchange(Psp, precision="fp64")
+                        if self.mixed:
+                            self.kref[s] = (Psp, K)
+                    Ks[s] = (Psp, K)
+                    made += list(Ks[s])
+                if inc:
+                    self.it_inc += 1
+                elif not self.fp32:
+                    self.final_check = False  # a full FP64 build: the next test is final
+                tick("exchange")
+            else:
+                if self.restricted:
+                    Vs[0], Exc = self.xc(P)
+                else:
+                    Vs[0], Vs[1], Exc = self.xc(Ps[0], Ps[1])
+                made += [V for V in Vs if V is not None]
+                tick("xc")
+        self._built = (Ps, P, J, Ks, Vs, Exc, stream, made)
+        return True
+
+    def complete(self):
+        """Energy, orbital gradient, DIIS and convergence test of the Fock
+        build queued by launch()."""
+        torch = self.torch
+        Ps, P, J, Ks, Vs, Exc, stream, made = self._built
+        self._built = None
+        if stream is not None:
+            cur = torch.cuda.current_stream(self.dev)
+            cur.wait_stream(stream)
+            for X in made:
+                X.record_stream(cur)
+        clk, bs, nspin = self.clk, self.bs, self.nspin
+        f64 = torch.float64
         Ecoul = 0.5 * torch.sum(P * J)
         Fs = [self.H + J for _ in range(nspin)]
         Exx = torch.zeros((), dtype=f64, device=self.dev)
-        Exc = torch.zeros((), dtype=f64, device=self.dev)
-        if self.xc is None:
-            kp = "fp32" if self.fp32 else "fp64"
-            for s in range(nspin):
-                Psp = 0.5 * Ps[s] if self.restricted else Ps[s]
-                if self.nel_spin[s] == 0:
-                    continue
-                Ks = self.basis.exchange(Psp, precision=kp)
-                Exx = Exx + (2.0 if self.restricted else 1.0) * 0.5 * torch.sum(Psp * Ks)
-                Fs[s] = Fs[s] + Ks
-            clk.tick("exchange")
-        else:
-            if self.restricted:
-                Vxc, Exc = self.xc(P)
-                Fs[0] = Fs[0] + Vxc
-            else:
-                Va, Vb, Exc = self.xc(Ps[0], Ps[1])
-                Fs[0] = Fs[0] + Va
-                Fs[1] = Fs[1] + Vb
-            clk.tick("xc")
+        if Exc is None:
+            Exc = torch.zeros((), dtype=f64, device=self.dev)
+        for s in range(nspin):
+            if Ks[s] is not None:
+                Psp, K = Ks[s]
+                Exx = Exx + (2.0 if self.restricted else 1.0) * 0.5 * torch.sum(Psp * K)
+                Fs[s] = Fs[s] + K
+            if Vs[s] is not None:
+                Fs[s] = Fs[s] + Vs[s]
         Ekin = torch.sum(P * self.T)
         Enuc = torch.sum(P * self.V)
         self.terms = dict(Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc)
@@ -605,10 +695,12 @@ class SCFState:
                 self.Eold = None
                 self.pending = [(s, Fb[s]) for s in range(nspin)]
                 return
-        elif self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6:
+        elif self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6 and not self.inc_now:
             self.done = True
             return
         else:
+            if self.inc_now and self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6:
+                self.final_check = True  # converged on an incremental build: confirm with a full FP64 one
             self.Eold = E
         nb = len(bs.b)
         self.diis.push([x for F in Fb for x in F], err)
@@ -621,7 +713,8 @@ class SCFState:
         t = self.terms
         out = SCFResult(E=self.E, Ekin=float(t["Ekin"]), Enuc=float(t["Enuc"]), Ecoul=float(t["Ecoul"]),
                         Exx=float(t["Exx"]), Exc=float(t["Exc"]), Enucr=self.Enucr, iterations=self.it,
-                        iterations_fp32=self.it_fp32, seconds=time.perf_counter() - self.t0, history=self.hist,
+                        iterations_fp32=self.it_fp32, iterations_incremental=self.it_inc,
+                        seconds=time.perf_counter() - self.t0, history=self.hist,
                         phases=self.clk.t, eigensolves=self.bs.solves, reused_solves=self.bs.paired_solves,
                         skipped_solves=self.bs.skipped_solves, frozen_occupations=self.frozen[0] is not None,
                         tag=self.tag)
@@ -636,12 +729,15 @@ class SCFEngine:
     """Advances many SCFStates in lockstep: every step diagonalises all the
     pending Fock blocks of all states together (equal-size eigenproblems in
     one batched call), then runs each state's Fock build.  States can join
-    between steps and leave when they converge (continuous batching)."""
+    between steps and leave when they converge (continuous batching).  With
+    `overlap` (default) the Fock builds of several states are queued on one
+    stream each before any is completed, so they run concurrently."""
 
-    def __init__(self):
+    def __init__(self, overlap=True):
         self.active = []
         self.steps = 0
         self.eig_seconds = 0.0
+        self.overlap = overlap
 
     def add(self, state):
         self.active.append(state)
@@ -655,7 +751,7 @@ class SCFEngine:
         reqs = []
         for st in self.active:
             for s, Fb, nocc in st.requests():
-                reqs.append((st, s, st.bs.prepare(Fb, s), nocc))
+                reqs.append((st, s, st.bs.prepare(Fb, s, st.frozen[s]), nocc))
         for dev in devs:
             _sync(dev)
         t1 = time.perf_counter()
@@ -677,15 +773,33 @@ class SCFEngine:
             ph["eigensolve"] = ph.get("eigensolve", 0.0) + (t2 - t1)
             ph["density"] = ph.get("density", 0.0) + (t1 - t) + (t3 - t2)
             st.clk.last = time.perf_counter()
-            st.advance()
+        if self.overlap and len(self.active) > 1 and all(st.dev.type == "cuda" for st in self.active):
+            # every state's Fock build queued on a stream of its own before
+            # any is completed, so the builds of the states overlap on the
+            # device; the densities were made on the current stream
+            cur = torch.cuda.current_stream()
+            launched = []
+            for st in self.active:
+                if st.stream is None:
+                    st.stream = torch.cuda.Stream(st.dev)
+                st.stream.wait_stream(cur)
+                if st.launch(st.stream):
+                    launched.append(st)
+            for st in launched:
+                st.complete()
+        else:
+            for st in self.active:
+                st.advance()
         self.steps += 1
         done = [st for st in self.active if st.done]
         self.active = [st for st in self.active if not st.done]
         return done
 
 
-def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch):
-    st = SCFState(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch)
+def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
+         occupations=None):
+    st = SCFState(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
+                  occupations=occupations)
     eng = SCFEngine()
     eng.add(st)
     while eng.active:
@@ -694,23 +808,33 @@ def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, 
 
 
 def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100, diis_len: int = 10,
-        verbose: bool = False, xc=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3):
+        verbose: bool = False, xc=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
+        occupations=None):
     """Closed-shell SCF; returns energies, iterations and the final density.
 
     xc(P) -> (Vxc, Exc) adds a density functional (None: Hartree-Fock).
     precision="mixed" builds the exchange in FP32 while the orbital gradient
     (the largest DIIS commutator element) is above `switch`, then restarts
-    DIIS and finishes in FP64; overlap, orthogonalisation, eigensolves and
-    energy sums are FP64 throughout."""
-    return _scf(basis, (nelec // 2,), True, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch)
+    DIIS on FP64 and split-precision builds: a full FP64 exchange K64(P_ref)
+    is kept and later iterations add the FP32 exchange of the density's
+    change, K64(P_ref) + K32(P - P_ref) (a fresh FP64 build whenever
+    max|P - P_ref| exceeds 1e-2 max|P|); convergence reached on such a build
+    is confirmed by a full FP64 one, so the converged energy carries no FP32
+    error.  Overlap, orthogonalisation, eigensolves and
+    energy sums are FP64 throughout.  occupations: fixed doubly occupied
+    orbitals per symmetry block ([counts]) instead of the aufbau."""
+    return _scf(basis, (nelec // 2,), True, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
+                occupations)
 
 
 def uhf(basis, nela: int, nelb: int, symm: int = 1, conv: float = 1e-10, maxit: int = 200, diis_len: int = 10,
-        verbose: bool = False, xc_pol=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3):
+        verbose: bool = False, xc_pol=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
+        occupations=None):
     """Unrestricted SCF (the reference's fock builders with Pa != Pb,
     atomic/main.cpp:369-470, diatomic/main.cpp:370-479).
 
     xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects spin-polarized Kohn-Sham DFT;
     None is UHF with Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb)).  precision="mixed"
-    as for rhf."""
-    return _scf(basis, (nela, nelb), False, symm, conv, maxit, diis_len, verbose, xc_pol, device, precision, switch)
+    and occupations ([alpha counts, beta counts]) as for rhf."""
+    return _scf(basis, (nela, nelb), False, symm, conv, maxit, diis_len, verbose, xc_pol, device, precision, switch,
+                occupations)
