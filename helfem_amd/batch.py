@@ -64,9 +64,10 @@ def _xc_for(sys_):
     return xc_pol
 
 
-def state_for(sys_, precision="fp64", conv=1e-10, maxit=200):
+def state_for(sys_, precision="fp64", conv=1e-10, maxit=200, gconv=1e-6):
     """SCFState of a system: RHF/RKS with nela doubly occupied orbitals when
-    the spins pair up, UHF/UKS otherwise."""
+    the spins pair up, UHF/UKS otherwise; converged when the energy changes by
+    less than conv and the largest orbital-gradient entry is below gconv."""
     nel = (sys_.nela,) if sys_.restricted else (sys_.nela, sys_.nelb)
     from .atomic import AtomicBasis
     # atoms: orbitals of pure (l, m) (the reference's symmetry=2), with the
@@ -77,7 +78,7 @@ def state_for(sys_, precision="fp64", conv=1e-10, maxit=200):
     if atomic and sys_.occ is not None:
         occ = sys_.occ[:1] if sys_.restricted else sys_.occ
     return SCFState(sys_.basis, nel, sys_.restricted, symm=2 if atomic else 1, conv=conv, maxit=maxit,
-                    xc=_xc_for(sys_), precision=precision, tag=sys_.tag, occupations=occ)
+                    xc=_xc_for(sys_), precision=precision, tag=sys_.tag, occupations=occ, gconv=gconv)
 
 
 def solve_batched(systems, precision="fp64", conv=1e-10, budget_bytes=64 << 30):
@@ -103,13 +104,13 @@ def solve_batched(systems, precision="fp64", conv=1e-10, budget_bytes=64 << 30):
     return results, time.perf_counter() - t0
 
 
-def solve_sequential(systems, precision="fp64", conv=1e-10):
+def solve_sequential(systems, precision="fp64", conv=1e-10, gconv=1e-6):
     """One engine per system, one after another (the plain drop-in way)."""
     t0 = time.perf_counter()
     out = []
     for i, s in enumerate(systems):
         eng = SCFEngine()
-        st = state_for(s, precision, conv)
+        st = state_for(s, precision, conv, gconv=gconv)
         eng.add(st)
         while eng.active:
             eng.step()
@@ -204,7 +205,7 @@ def build_diatomic(spec, ctx):
 
 
 def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=64 << 30, setup_threads=8,
-          build=build_diatomic):
+          build=build_diatomic, gconv=1e-6):
     """Poisson arrivals at `rate_hz` of the requests in `specs`; each request's
     basis is prepared on a host thread pool as it arrives and admitted into
     the running engine when ready and when its footprint fits the budget.
@@ -231,7 +232,7 @@ def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=
         used = sum(in_use.values())
         while ready and (not eng.active or used + ready[0][2].footprint() <= budget_bytes):
             _, i, s = heapq.heappop(ready)
-            st = state_for(s, precision, conv)
+            st = state_for(s, precision, conv, gconv=gconv)
             st.tag = (i, s)
             eng.add(st)
             in_use[i] = s.footprint()

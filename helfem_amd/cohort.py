@@ -680,3 +680,49 @@ class Cohort:
                     converged=bool(self.done[q] and self.it[q] < self.maxit), orbital_gradient=float(self.last_emax[q]),
                     frozen_occupations=self.frozen[q] is not None, phases=dict(self.phases),
                     eigensolves=self.nsolves, skipped_eigensolves=self.nskipped)
+
+
+def cohort_bytes(system, diis_len=10, symm=2):
+    """Device bytes a system holds in a cohort: its flat matrices (S, T, V, J,
+    total density; density and exchange per spin channel) and its padded
+    block stacks (overlap, orthogonaliser, core Hamiltonian, last solved
+    matrix, DIIS history of Fock matrices and errors, and the transient Fock
+    and gathered J, K stacks of an iteration)."""
+    b = system.basis
+    nch = 1 if system.nela == system.nelb else (2 if system.nelb > 0 else 1)
+    flat = 8 * b.Nbf ** 2 * (5 + 2 * nch)
+    blk = sum(len(i) ** 2 for i in b.get_sym_idx(symm))
+    return flat + 8 * nch * blk * (2 * diis_len + 10)
+
+
+def cohort_groups(systems, budget_bytes, diis_len=10, symm=2):
+    """Consecutive runs of system indices whose cohort_bytes sum stays within
+    `budget_bytes` (a system larger than the budget gets a cohort of its own)."""
+    groups, chunk, used = [], [], 0
+    for i, s in enumerate(systems):
+        need = cohort_bytes(s, diis_len, symm)
+        if chunk and used + need > budget_bytes:
+            groups.append(chunk)
+            chunk, used = [], 0
+        chunk.append(i)
+        used += need
+    if chunk:
+        groups.append(chunk)
+    return groups
+
+
+def solve_cohorts(systems, ctx, budget_bytes=96 << 30, **kw):
+    """All systems through cohorts of at most `budget_bytes` of device memory
+    each, one after another (input order); returns the results in input order
+    (each with the index of its cohort under "cohort") and the wall time."""
+    t0 = time.perf_counter()
+    occ = kw.pop("occupations", None)
+    groups = cohort_groups(systems, budget_bytes, kw.get("diis_len", 10), kw.get("symm", 2))
+    res = [None] * len(systems)
+    for c, g in enumerate(groups):
+        co = Cohort([systems[i] for i in g], ctx, occupations=None if occ is None else [occ[i] for i in g], **kw)
+        for i, r in zip(g, co.solve()):
+            r["cohort"] = c
+            res[i] = r
+        del co
+    return res, time.perf_counter() - t0

@@ -500,13 +500,14 @@ class SCFState:
     nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
 
     def __init__(self, basis, nel_spin, restricted, symm=1, conv=1e-10, maxit=100, diis_len=10, verbose=False,
-                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None):
+                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None, gconv=1e-6):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.torch = torch
         self.basis, self.nel_spin, self.restricted = basis, tuple(nel_spin), restricted
         self.conv, self.maxit, self.verbose, self.xc, self.switch = conv, maxit, verbose, xc, switch
+        self.gconv = gconv  # largest orbital-gradient (commutator) entry at convergence
         self.tag = tag
         self.dev = torch.device(device)
         self.t0 = time.perf_counter()
@@ -695,11 +696,11 @@ class SCFState:
                 self.Eold = None
                 self.pending = [(s, Fb[s]) for s in range(nspin)]
                 return
-        elif self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6 and not self.inc_now:
+        elif self.Eold is not None and abs(E - self.Eold) < self.conv and emax < self.gconv and not self.inc_now:
             self.done = True
             return
         else:
-            if self.inc_now and self.Eold is not None and abs(E - self.Eold) < self.conv and emax < 1e-6:
+            if self.inc_now and self.Eold is not None and abs(E - self.Eold) < self.conv and emax < self.gconv:
                 self.final_check = True  # converged on an incremental build: confirm with a full FP64 one
             self.Eold = E
         nb = len(bs.b)

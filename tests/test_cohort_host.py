@@ -56,3 +56,47 @@ def test_frozen_occupations_take_each_block_lowest():
 def test_too_few_levels_is_an_error():
     with pytest.raises(RuntimeError):
         aufbau_levels(np.array([[-1.0, np.inf]]), 2)
+
+
+class _FakeBasis:
+    def __init__(self, nbf, blocks):
+        self.Nbf = nbf
+        self._blocks = blocks
+
+    def get_sym_idx(self, symm):
+        out, o = [], 0
+        for b in self._blocks:
+            out.append(np.arange(o, o + b))
+            o += b
+        return out
+
+
+class _FakeSystem:
+    def __init__(self, nbf, blocks, nela, nelb):
+        self.basis, self.nela, self.nelb = _FakeBasis(nbf, blocks), nela, nelb
+
+
+def test_cohort_groups_respect_budget():
+    """Memory-bounded cohorts: consecutive runs under the budget, every
+    system once, in order; an oversized system alone."""
+    from helfem_amd.cohort import cohort_bytes, cohort_groups
+    rng = np.random.default_rng(7)
+    systems = []
+    for _ in range(50):
+        blocks = [int(x) for x in rng.integers(5, 60, size=int(rng.integers(1, 6)))]
+        na = int(rng.integers(1, 10))
+        systems.append(_FakeSystem(sum(blocks), blocks, na, na - int(rng.integers(0, 2))))
+    sizes = [cohort_bytes(s) for s in systems]
+    # restricted systems hold one channel, open-shell ones two
+    r = _FakeSystem(100, [100], 3, 3)
+    u = _FakeSystem(100, [100], 3, 2)
+    assert cohort_bytes(u) - cohort_bytes(r) == 8 * 100 * 100 * (2 + 30)
+    budget = 3 * max(sizes)
+    groups = cohort_groups(systems, budget)
+    assert [i for g in groups for i in g] == list(range(len(systems)))
+    for g in groups:
+        assert sum(sizes[i] for i in g) <= budget
+    for a, b in zip(groups, groups[1:]):
+        assert sum(sizes[i] for i in a) + sizes[b[0]] > budget  # greedy: the next one did not fit
+    assert cohort_groups(systems, 1) == [[i] for i in range(len(systems))]
+    assert cohort_groups(systems, 1 << 62) == [list(range(len(systems)))]

@@ -83,3 +83,16 @@ def test_mixed_precision_cohort(systems, together, gpu_ctx):
     for s, a, b in zip(systems, mixed, together):
         assert abs(a["E"] - b["E"]) < 1e-9, (s.info, a["E"], b["E"])
     assert any(r["iterations_fp32"] > 0 for r in mixed)
+
+
+def test_memory_bounded_cohorts_equal_one_cohort(systems, together, gpu_ctx):
+    """solve_cohorts under a budget that splits the systems into several
+    cohorts gives each system the energy of the single cohort."""
+    from helfem_amd.cohort import cohort_bytes, solve_cohorts
+    print("[gpu] memory-bounded cohorts vs one cohort", flush=True)
+    budget = 2 * max(cohort_bytes(s) for s in systems)
+    res, _ = solve_cohorts(systems, gpu_ctx, budget_bytes=budget, conv=1e-10, occupations=_occ(systems))
+    assert len({r["cohort"] for r in res}) > 1
+    for s, a, b in zip(systems, res, together):
+        assert a["converged"], s.info
+        assert abs(a["E"] - b["E"]) < 1e-10, (s.info, a["E"], b["E"])

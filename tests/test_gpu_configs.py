@@ -228,16 +228,17 @@ def test_served_diatomic_distribution_equals_alone(gpu_ctx):
     HF or LDA), Poisson arrivals, continuous batching with the requests'
     Fock builds overlapping on the device: each served energy equals the
     request solved alone to 1e-8 Eh (both converged to dE < 1e-11 Eh and an
-    orbital gradient below 1e-6, whose residual energy error reaches 1e-9 Eh
-    for the larger LDA requests)."""
+    orbital gradient below 1e-7: at 1e-6 the open-shell LDA requests, whose
+    soft orbital rotations leave an energy error of order g^2 / 3e-5, still
+    differed by 3e-8 Eh)."""
     from helfem_amd import batch
     print("[gpu] six served diatomic requests vs alone", flush=True)
     specs = batch.random_diatomic_specs(6, seed=1005)
     assert {s["lda"] for s in specs} == {True, False}
-    res, wall = batch.serve(specs, rate_hz=2.0, seed=1006, ctx=gpu_ctx, conv=1e-11, setup_threads=16)
+    res, wall = batch.serve(specs, rate_hz=2.0, seed=1006, ctx=gpu_ctx, conv=1e-11, setup_threads=16, gconv=1e-7)
     for spec, r in zip(specs, res):
-        print(f"[gpu]   {spec} latency {r['latency_s']:.1f}s E {r['E']:.10f}", flush=True)
-        alone, _ = batch.solve_sequential([batch.build_diatomic(spec, gpu_ctx)], conv=1e-11)
+        print(f"[gpu]   {spec} latency {r['latency_s']:.1f}s E {r['E']:.10f} in {r['iterations']} it", flush=True)
+        alone, _ = batch.solve_sequential([batch.build_diatomic(spec, gpu_ctx)], conv=1e-11, gconv=1e-7)
         assert abs(r["E"] - alone[0].E) < 1e-8, (spec, r["E"], alone[0].E)
 
 
