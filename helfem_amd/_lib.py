@@ -38,6 +38,7 @@ SIGNATURES = {
     "hf_atomic_sizes": (c_int, [c_vp, c_vp]),
     "hf_atomic_elements": (c_int, [c_vp, c_vp, c_vp]),
     "hf_atomic_compute_tei": (c_int, [c_vp, c_int]),
+    "hf_atomic_compute_tei_device": (c_int, [c_vp, c_vp]),
     "hf_atomic_set_cache": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp]),
     "hf_atomic_commit_tei": (c_int, [c_vp]),
     "hf_atomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),

@@ -121,8 +121,16 @@ class AtomicBasis:
         return c
 
     # --- setup ---------------------------------------------------------
-    def compute_tei(self, exchange: bool = True):
-        check(lib().hf_atomic_compute_tei(self._h, int(exchange)))
+    def compute_tei(self, exchange: bool = True, device: bool = False):
+        """The two-electron caches (TwoDBasis.cpp:709-735).  device=True builds
+        the in-element Slater tensors and their Cholesky factors on the GPU
+        (needs a context) and returns the wall time in seconds."""
+        if not device:
+            check(lib().hf_atomic_compute_tei(self._h, int(exchange)))
+            return None
+        sec = ctypes.c_double(0.0)
+        check(lib().hf_atomic_compute_tei_device(self._h, ctypes.byref(sec)))
+        return sec.value
 
     def set_cache(self, kind, L, iel, mat):
         mat = _f64_fortran(mat)

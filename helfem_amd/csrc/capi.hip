@@ -452,6 +452,16 @@ int hf_atomic_compute_tei(hf_atomic *h, int exchange) {
   });
 }
 
+int hf_atomic_compute_tei_device(hf_atomic *h, double *seconds) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    atomic_tei_device(h->host, h->ctx->cur, seconds);
+    h->uploaded = false;
+  });
+}
+
 int hf_atomic_compute_yukawa(hf_atomic *h, double lambda) {
   return guarded([&] {
     need(h != nullptr, "NULL handle");

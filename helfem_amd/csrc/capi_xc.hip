@@ -76,13 +76,18 @@ void xc_upload(hf_dftgrid *g) {
     rstart[a + 1] = (int)rb.size();
   }
   d.npair = np;
-  std::vector<int> dstart(nd + 1, 0), da, db, dp;
+  // pairs renumbered by class d (x = position in the d-sorted order): the
+  // work arrays D, W are indexed by x, and rx maps each row-list entry to it
+  std::vector<int> dstart(nd + 1, 0), da, db, px(np), xd;
   for (int dd = 0; dd < nd; ++dd) {
+    for (size_t j = 0; j < byd_p[dd].size(); ++j) px[byd_p[dd][j]] = (int)da.size() + (int)j;
     da.insert(da.end(), byd_a[dd].begin(), byd_a[dd].end());
     db.insert(db.end(), byd_b[dd].begin(), byd_b[dd].end());
-    dp.insert(dp.end(), byd_p[dd].begin(), byd_p[dd].end());
+    xd.insert(xd.end(), byd_a[dd].size(), dd);
     dstart[dd + 1] = (int)da.size();
   }
+  std::vector<int> rx(rp.size());
+  for (size_t q = 0; q < rp.size(); ++q) rx[q] = px[rp[q]];
   const size_t lds = sizeof(double) * ((size_t)np + (size_t)g->lang * nd + (size_t)g->lang * g->mang);
   if (lds > 64 * 1024)
     throw Error(HF_EINVAL, "angular basis and DFT grid too large for one workgroup's LDS (need shell pairs + "
@@ -104,6 +109,14 @@ void xc_upload(hf_dftgrid *g) {
     for (int a = 0; a < Na; ++a) th[(size_t)t * Na + a] = theta_value(lv[a], mv[a], xt[t]);
   for (int dd = 0; dd < nd; ++dd)
     for (int k = 0; k < g->mang; ++k) cs[(size_t)dd * g->mang + k] = std::cos(double(dd - d.dmax) * (k * dphi));
+  // Theta_a(t) Theta_b(t) per pair x, in both orders ([t][x] and [x][t])
+  std::vector<double> thx((size_t)g->lang * np), thxt((size_t)np * g->lang);
+  for (int t = 0; t < g->lang; ++t)
+    for (int x = 0; x < np; ++x) {
+      const double v = th[(size_t)t * Na + da[x]] * th[(size_t)t * Na + db[x]];
+      thx[(size_t)t * np + x] = v;
+      thxt[(size_t)x * g->lang + t] = v;
+    }
   for (int e = 0; e < Nel; ++e) {
     const Blk v = g->atomic ? rad.rvalues(xq, e) : rad.values(0, xq, e);
     const double h = rad.half_len(e), mid = rad.mid(e);
@@ -135,15 +148,14 @@ void xc_upload(hf_dftgrid *g) {
   d.poff.upload(poff, s);
   d.R.upload(R, s);
   d.wpt.upload(wpt, s);
-  d.th.upload(th, s);
   d.cs.upload(cs, s);
   d.rstart.upload(rstart, s);
   d.rb.upload(rb, s);
-  d.rp.upload(rp, s);
+  d.rx.upload(rx, s);
   d.dstart.upload(dstart, s);
-  d.da.upload(da, s);
-  d.db.upload(db, s);
-  d.dp.upload(dp, s);
+  d.xd.upload(xd, s);
+  d.thx.upload(thx, s);
+  d.thxt.upload(thxt, s);
   d.D.alloc((size_t)Nel * np * nrad);
   d.W.alloc((size_t)Nel * np * nrad);
   d.part.alloc((size_t)2 * Nel * nrad);

@@ -331,11 +331,11 @@ struct XCGridDevice {
   DevArray<long long> poff;
   DevArray<double> R;        // [Nel][nrad][TILE]
   DevArray<double> wpt;      // [Nel][nrad][lang] quadrature weight (phi weight included)
-  DevArray<double> th;       // [lang][Nang]
   DevArray<double> cs;       // [2 dmax + 1][mang] cos(d phi_k)
-  DevArray<int> rstart, rb, rp;       // pairs of row a: b and pair index
-  DevArray<int> dstart, da, db, dp;   // pairs bucketed by d = m_a - m_b + dmax
-  DevArray<double> D, W;     // [Nel][npair][nrad]
+  DevArray<int> rstart, rb, rx;       // pairs of row a: b and pair index x
+  DevArray<int> dstart, xd;           // pairs x bucketed by d = m_a - m_b + dmax; d of each x
+  DevArray<double> thx, thxt;         // Theta_a(t) Theta_b(t): [lang][npair] and [npair][lang]
+  DevArray<double> D, W;     // D [Nel][nrad][npair], W [Nel][npair][nrad]
   DevArray<double> Db, Wb;   // second spin of a polarized evaluation
   DevArray<double> part;     // [Nel*nrad][2]: Exc and Nel partial sums
   PinnedArray<double> h_part;
@@ -351,6 +351,11 @@ size_t atomic_coulomb_work_doubles(const AtomicDevice &d);
 // Device address of the allocation-free entry point's scratch array, looked
 // up for `device` (the current device); once per device, at context creation.
 void noalloc_scratch_init(int device);
+
+// compute_tei with the in-element Slater tensors and their Cholesky factors
+// built on the device (tei_atomic.hip); fills the same host caches.
+// seconds (optional) gets the wall time.  Synchronises s.
+void atomic_tei_device(AtomicSetup &S, hipStream_t s, double *seconds);
 // fp32: gathers and matrix products in FP32 (the result stays FP64, with
 // about 1e-6 relative error); otherwise the reference's FP64 arithmetic.
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,

@@ -476,6 +476,19 @@ void AtomicSetup::compute_tei() {
   have_tei = true;
 }
 
+void AtomicSetup::compute_radial_factors() {
+  const int nel = rad.Nel();
+  rsmall.assign((size_t)N_L * nel, Blk());
+  rbig.assign((size_t)N_L * nel, Blk());
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int L = 0; L < N_L; ++L)
+    for (int e = 0; e < nel; ++e) {
+      const size_t i = (size_t)L * nel + e;
+      rsmall[i] = rad.rpow(L, e);
+      if (e > 0) rbig[i] = rad.rpow(-L - 1, e);
+    }
+}
+
 namespace {
 void radial_diag(const RadialFE &rad, const std::function<Blk(int)> &k, std::vector<double> &M) {
   const int n = rad.Nrad();

@@ -143,6 +143,9 @@ struct AtomicSetup {
   int Nang() const { return (int)lval.size(); }
   int64_t Nbf() const { return (int64_t)Nrad() * Nang(); }
   void compute_tei();
+  // the r^L and r^-L-1 element matrices alone (the part of compute_tei that
+  // stays on the host when the Slater tensors are built on the device)
+  void compute_radial_factors();
   // dense one-electron matrices, column-major Nbf x Nbf with leading dim ld
   void overlap(double *out, int64_t ld) const;
   void kinetic(double *out, int64_t ld) const;

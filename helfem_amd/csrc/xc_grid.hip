@@ -118,92 +118,107 @@ struct XCArgs {
   double thr;
   const int *first, *ni, *skip;
   const long long *poff;
-  const int *rstart, *rb, *rp, *dstart, *da, *db, *dp;
-  const double *R, *wpt, *th, *cs;
+  const int *rstart, *rb, *rx, *dstart, *xd;
+  const double *R, *wpt, *thx, *thxt, *cs;
   double *D, *W, *part;
   double *Db, *Wb;  // second spin (polarized evaluation), else null
 };
 
-// One workgroup per (element, shell a); the waves take the pairs (a, b) of
-// row a in turn.
+// One workgroup per (element, shell a).  The pairs (a, b) of row a go in
+// groups of J = 256 / nrad (1 to 8, xc_pairs_per_group): their P tiles are staged in LDS, and
+// thread t takes (pair t / nrad, radial point t % nrad), so every thread of
+// the workgroup has a point (J nrad of 256 busy).  A thread keeps R_i in
+// registers; the R rows sit in LDS at an odd stride (17), so neighbouring
+// points read different banks, and the tile reads of a wave touch at most two
+// tiles (broadcast).  R and the tiles are zero outside the element.
+constexpr int XC_RS = TILE + 1;
+
+__host__ __device__ inline int xc_pairs_per_group(int nrad) { return min(8, max(1, NT / nrad)); }
+
 __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
   const int e = blockIdx.x / A.Nang, a = blockIdx.x - e * A.Nang;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x;
   const int ni = A.ni[e], f0 = A.first[e];
+  const int J = xc_pairs_per_group(A.nrad);
   extern __shared__ double sm[];
-  double *sR = sm;                    // [nrad][TILE]
-  double *sP = sm + A.nrad * TILE;    // [4][TILE2]
-  for (int q = t; q < A.nrad * TILE; q += NT) sR[q] = A.R[(size_t)e * A.nrad * TILE + q];
+  double *sR = sm;                       // [nrad][XC_RS]
+  double *sP = sm + A.nrad * XC_RS;      // [J][TILE2]
+  for (int q = t; q < A.nrad * TILE; q += NT)
+    sR[(q >> 4) * XC_RS + (q & 15)] = A.R[(size_t)e * A.nrad * TILE + q];
   const int q0 = A.rstart[a], q1 = A.rstart[a + 1];
-  for (int qb = q0; qb < q1; qb += 4) {
-    const int q = qb + wv;
+  for (int qb = q0; qb < q1; qb += J) {
     __syncthreads();
-    if (q < q1) {
-      const int b = A.rb[q];
-      for (int u = 0; u < 4; ++u) {
-        const int x = lane + 64 * u, r = x & 15, s = x >> 4;
-        double v = 0.0;
-        if (r < ni && s < ni) {
-          const long long row = grid_row(A.skip, A.poff, a, f0 + r), col = grid_row(A.skip, A.poff, b, f0 + s);
-          if (row >= 0 && col >= 0) v = A.P[row + col * A.ldp];
-        }
-        sP[wv * TILE2 + x] = v;
+    for (int x = t; x < J * TILE2; x += NT) {
+      const int j = x >> 8, y = x & 255, r = y & 15, s = y >> 4, q = qb + j;
+      double v = 0.0;
+      if (q < q1 && r < ni && s < ni) {
+        const long long row = grid_row(A.skip, A.poff, a, f0 + r), col = grid_row(A.skip, A.poff, A.rb[q], f0 + s);
+        if (row >= 0 && col >= 0) v = A.P[row + col * A.ldp];
       }
+      sP[x] = v;
     }
     __syncthreads();
-    if (q >= q1) continue;
-    const int p = A.rp[q];
-    const double *Pt = sP + wv * TILE2;
-    for (int i = lane; i < A.nrad; i += 64) {
-      const double *Ri = sR + i * TILE;
+    for (int item = t; item < J * A.nrad; item += NT) {
+      const int j = item / A.nrad, i = item - j * A.nrad, q = qb + j;
+      if (q >= q1) continue;
+      const double *Pt = sP + j * TILE2;
+      const double *Ri = sR + i * XC_RS;
+      double ri[TILE];
+#pragma unroll
+      for (int s = 0; s < TILE; ++s) ri[s] = Ri[s];
       double d = 0.0;
-      for (int r = 0; r < ni; ++r) {
+#pragma unroll
+      for (int r = 0; r < TILE; ++r) {
         double tr = 0.0;
-        for (int s = 0; s < ni; ++s) tr += Pt[r + TILE * s] * Ri[s];
-        d += Ri[r] * tr;
+#pragma unroll
+        for (int s = 0; s < TILE; ++s) tr += Pt[r + TILE * s] * ri[s];
+        d += ri[r] * tr;
       }
-      A.D[((size_t)e * A.npair + p) * A.nrad + i] = d;
+      A.D[((size_t)e * A.nrad + i) * A.npair + A.rx[q]] = d;
     }
   }
 }
 
 // One workgroup per (element, radial point).  With A.Db set the density of
 // the second spin is contracted alongside and the polarized functional used.
+// The pairs x are numbered by class d = m_a - m_b, so the class sums read
+// contiguous runs of D and of the Theta_a Theta_b table: thread (d, t) forms
+// C_d(t) from the [x][t] table (lanes of one d read consecutive t), and
+// thread x forms W(x) from the [t][x] table (lanes read consecutive x).
 __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   const int e = blockIdx.x / A.nrad, i = blockIdx.x - e * A.nrad;
   const int t = threadIdx.x;
-  const int Na = A.Nang, nd = A.nd, nang = A.lang * A.mang;
+  const int np = A.npair, nd = A.nd, lang = A.lang, mang = A.mang, nang = lang * mang;
   const bool pol = A.Db != nullptr;
   const int ns = pol ? 2 : 1;
   extern __shared__ double sm[];
-  double *sD = sm;                          // [ns][npair]
-  double *sC = sD + ns * A.npair;           // [ns][lang][nd]
-  double *sV = sC + ns * A.lang * nd;       // [ns][lang][mang]
+  double *sD = sm;                    // [ns][np]
+  double *sC = sD + ns * np;          // [ns][nd][lang]
+  double *sV = sC + ns * nd * lang;   // [ns][lang][mang]
   __shared__ double red[8];
-  const size_t eb = (size_t)e * A.npair * A.nrad;
-  for (int p = t; p < A.npair; p += NT) {
-    sD[p] = A.D[eb + (size_t)p * A.nrad + i];
-    if (pol) sD[A.npair + p] = A.Db[eb + (size_t)p * A.nrad + i];
+  const size_t row = ((size_t)e * A.nrad + i) * np;
+  for (int x = t; x < np; x += NT) {
+    sD[x] = A.D[row + x];
+    if (pol) sD[np + x] = A.Db[row + x];
   }
   __syncthreads();
-  for (int q = t; q < ns * A.lang * nd; q += NT) {
-    const int sp = q / (A.lang * nd), qq = q - sp * A.lang * nd;
-    const int tt = qq / nd, d = qq - tt * nd;
-    const double *th = A.th + (size_t)tt * Na;
-    const double *D = sD + sp * A.npair;
+  for (int q = t; q < ns * nd * lang; q += NT) {
+    const int sp = q / (nd * lang), qq = q - sp * nd * lang;
+    const int d = qq / lang, tt = qq - d * lang;
+    const double *D = sD + sp * np;
     double c = 0.0;
-    for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) c += th[A.da[x]] * th[A.db[x]] * D[A.dp[x]];
+    for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) c += A.thxt[(size_t)x * lang + tt] * D[x];
     sC[q] = c;
   }
   __syncthreads();
-  const double *wp = A.wpt + ((size_t)e * A.nrad + i) * A.lang;
+  const double *wp = A.wpt + ((size_t)e * A.nrad + i) * lang;
   double ex = 0.0, ne = 0.0;
   for (int q = t; q < nang; q += NT) {
-    const int tt = q / A.mang, k = q - tt * A.mang;
+    const int tt = q / mang, k = q - tt * mang;
     double ra = 0.0, rb = 0.0;
-    for (int d = 0; d < nd; ++d) ra += sC[tt * nd + d] * A.cs[(size_t)d * A.mang + k];
+    for (int d = 0; d < nd; ++d) ra += sC[d * lang + tt] * A.cs[(size_t)d * mang + k];
     if (pol)
-      for (int d = 0; d < nd; ++d) rb += sC[A.lang * nd + tt * nd + d] * A.cs[(size_t)d * A.mang + k];
+      for (int d = 0; d < nd; ++d) rb += sC[nd * lang + d * lang + tt] * A.cs[(size_t)d * mang + k];
     const double w = wp[tt];
     double exc, va, vb = 0.0;
     if (pol)
@@ -221,25 +236,24 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
     A.part[2 * ((size_t)e * A.nrad + i) + 1] = ne;
   }
   __syncthreads();
-  for (int q = t; q < ns * A.lang * nd; q += NT) {
-    const int sp = q / (A.lang * nd), qq = q - sp * A.lang * nd;
-    const int tt = qq / nd, d = qq - tt * nd;
-    const double *V = sV + sp * nang;
+  for (int q = t; q < ns * nd * lang; q += NT) {
+    const int sp = q / (nd * lang), qq = q - sp * nd * lang;
+    const int d = qq / lang, tt = qq - d * lang;
+    const double *V = sV + sp * nang + tt * mang;
     double s = 0.0;
-    for (int k = 0; k < A.mang; ++k) s += V[tt * A.mang + k] * A.cs[(size_t)d * A.mang + k];
+    for (int k = 0; k < mang; ++k) s += V[k] * A.cs[(size_t)d * mang + k];
     sC[q] = s;
   }
   __syncthreads();
   for (int sp = 0; sp < ns; ++sp) {
     double *Wout = sp == 0 ? A.W : A.Wb;
-    const double *C = sC + sp * A.lang * nd;
-    for (int d = 0; d < nd; ++d)
-      for (int x = A.dstart[d] + t; x < A.dstart[d + 1]; x += NT) {
-        const int a = A.da[x], b = A.db[x];
-        double w = 0.0;
-        for (int tt = 0; tt < A.lang; ++tt) w += A.th[(size_t)tt * Na + a] * A.th[(size_t)tt * Na + b] * C[tt * nd + d];
-        Wout[eb + (size_t)A.dp[x] * A.nrad + i] = w;
-      }
+    const double *C = sC + sp * nd * lang;
+    for (int x = t; x < np; x += NT) {
+      const double *Cd = C + A.xd[x] * lang;
+      double w = 0.0;
+      for (int tt = 0; tt < lang; ++tt) w += A.thx[(size_t)tt * np + x] * Cd[tt];
+      Wout[((size_t)e * np + x) * A.nrad + i] = w;  // [element][x][point]: rad_out reads runs of points
+    }
   }
 }
 
@@ -263,17 +277,16 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
   const long long row = grid_row(A.skip, A.poff, a, f0 + r);
   if (row < 0) return;
   const bool corner = has_next && r == ni - 1 && s == ni - 1;
-  const size_t eb = (size_t)e * A.npair * A.nrad;
-  const size_t nb = (size_t)(e + 1) * A.npair * A.nrad;
+  const size_t np = A.npair;
   for (int q = A.rstart[a]; q < A.rstart[a + 1]; ++q) {
-    const int b = A.rb[q], p = A.rp[q];
+    const int b = A.rb[q], x = A.rx[q];
     const long long col = grid_row(A.skip, A.poff, b, f0 + s);
     if (col < 0) continue;
-    const double *Wp = A.W + eb + (size_t)p * A.nrad;
+    const double *Wp = A.W + ((size_t)e * np + x) * A.nrad;
     double v = 0.0;
     for (int i = 0; i < A.nrad; ++i) v += sR[i * TILE + r] * sR[i * TILE + s] * Wp[i];
     if (corner) {
-      const double *Wn = A.W + nb + (size_t)p * A.nrad;
+      const double *Wn = A.W + ((size_t)(e + 1) * np + x) * A.nrad;
       for (int i = 0; i < A.nrad; ++i) v += sN[i] * sN[i] * Wn[i];
     }
     A.H[row + col * A.ldh] = v;
@@ -290,8 +303,9 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
   if (pol) fill_zero_launch(Hb, g.Nbf, g.Nbf, ldhb, s);
   const int nd = 2 * g.dmax + 1;
   XCArgs a{Pa, ldpa, Ha, ldha, g.Nel, g.Nang, g.nrad, g.lang, g.mang, nd, xf, cf, g.npair, thr,
-           g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rp.p, g.dstart.p, g.da.p, g.db.p, g.dp.p,
-           g.R.p, g.wpt.p, g.th.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr, pol ? g.Wb.p : nullptr};
+           g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rx.p, g.dstart.p, g.xd.p,
+           g.R.p, g.wpt.p, g.thx.p, g.thxt.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr,
+           pol ? g.Wb.p : nullptr};
   XCArgs b = a;  // second spin: its density in, its matrix out
   if (pol) {
     b.P = Pb;
@@ -302,7 +316,7 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
     b.W = g.Wb.p;
   }
   const int ns = pol ? 2 : 1;
-  const size_t sh_in = sizeof(double) * ((size_t)g.nrad * TILE + 4 * TILE2);
+  const size_t sh_in = sizeof(double) * ((size_t)g.nrad * XC_RS + (size_t)xc_pairs_per_group(g.nrad) * TILE2);
   const size_t sh_pt =
       sizeof(double) * ns * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
   const size_t sh_out = sizeof(double) * ((size_t)g.nrad * TILE + g.nrad);

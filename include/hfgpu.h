@@ -113,6 +113,14 @@ int hf_atomic_work(hf_atomic *h, double out[4]);
  * (src/atomic/TwoDBasis.h:181, TwoDBasis.cpp:709-735). */
 int hf_atomic_compute_tei(hf_atomic *h, int exchange);
 
+/* The same caches with the in-element Slater tensors (Lobatto order
+ * refinement of the split-at-r=r' quadrature) and their pivoted Cholesky
+ * factors built on the GPU; the r^L / r^-L-1 element matrices stay on the
+ * host.  Replaces TwoDBasisT::compute_tei like hf_atomic_compute_tei
+ * (RadialBasis.cpp:644-716 for the tensor and factor); needs a context;
+ * synchronises its stream.  seconds (may be NULL) gets the wall time. */
+int hf_atomic_compute_tei_device(hf_atomic *h, double *seconds);
+
 /* Supply one cached factor computed elsewhere (e.g. by HelFEM's own
  * compute_tei), instead of hf_atomic_compute_tei.  kind 0: disjoint_L
  * (r^L, Ni x Ni), 1: disjoint_m1L (r^-L-1, Ni x Ni, ignored for iel 0),

@@ -142,6 +142,23 @@ def test_xe_full_size_coulomb_exchange(density, gpu_ctx):
     assert err <= TOL
 
 
+def test_xe_device_setup_coulomb_exchange(gpu_ctx):
+    """The Xe two-electron setup built on the device (Slater tensors and
+    Cholesky factors of all 13 x 25 (L, element) pairs): J and the sampled K
+    from those caches meet the same bar against the oracle."""
+    import time
+    print("[gpu] Xe device two-electron setup: J and sampled K vs oracle", flush=True)
+    ob, hb, P, shell = _xe(gpu_ctx)
+    t = time.perf_counter()
+    db = H.AtomicBasis(XE["Z"], XE["lmax"], XE["mmax"], XE["nel"], ctx=gpu_ctx)
+    dev = db.compute_tei(device=True)
+    total = time.perf_counter() - t
+    err = _coulomb_and_sampled_exchange(ob, db, shell, P)
+    print(f"[gpu]   device setup {dev:.3f} s ({total:.3f} s with the basis); sampled K rel err {err:.2e}",
+          flush=True)
+    assert err <= TOL
+
+
 def test_xe_full_size_fp32_exchange_error(gpu_ctx):
     """The reduced-precision exchange at the Xe size, stated against the
     oracle on the sampled blocks."""
