@@ -56,7 +56,7 @@ WORKLOADS = {
                                 "m=0..3, Nbf 11894"),
     "xe-hf-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003,
                          desc="Xe HF atomic, lmax=mmax=6, 25 LIP-15 elements to 40 bohr (exp grid), Nbf 17101"),
-    "xe-lda-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, lda=True,
+    "xe-lda-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, lda=True, name="xe-lda-atomic",
                           desc="Xe LDA (Slater + VWN5) atomic, lmax=mmax=6, 25 LIP-15 elements, ldft=mdft=36 "
                                "(97,200 points per element), Nbf 17101"),
     "ne-hf-atomic": dict(kind="atomic", Z=10, lmax=0, mmax=0, nelem=10, seed=1001,
@@ -154,6 +154,24 @@ def kernel_stats(ctx, names):
     return out
 
 
+def xc_flops(hb, lang, mang):
+    """FP64 operations of one exchange-correlation evaluation on the atomic
+    grid (the three kernels' sums with the element sizes, LDA point formulas
+    not counted): radial contraction 2 (Ni^2 + Ni) per (pair, point), the
+    pair-class and cos(d phi) transforms both ways per radial point, the
+    band assembly 3 Ni^2 per (pair, point)."""
+    mval = np.asarray(hb.mval)
+    dmax = 2 * int(np.max(np.abs(mval)))
+    npair = int(np.sum(np.abs(mval[:, None] - mval[None, :]) <= dmax))
+    nd = 2 * dmax + 1
+    nrad = 5 * 15
+    ni = np.asarray(hb.nprim, dtype=float)
+    rad_in = npair * nrad * float(np.sum(2.0 * (ni * ni + ni)))
+    point = hb.Nel * nrad * (4.0 * lang * npair + 4.0 * lang * mang * nd)
+    rad_out = npair * nrad * float(np.sum(3.0 * ni * ni))
+    return rad_in + point + rad_out
+
+
 def xc_band_bytes(hb):
     """Algorithmic HBM bytes of one exchange-correlation evaluation: the
     element-diagonal bands of P read and of H written, over every shell pair
@@ -233,11 +251,20 @@ def fock_bench(H, cfg, ctx, steps, warmup, hb=None, P=None):
         x_ms = sum(stats[n][1] / max(stats[n][0], 1) for n in XC_KERNELS)
         nbytes = xc_band_bytes(hb)
         out["energies"].update(exc=xc_out["exc"], nel=xc_out["nel"])
+        traffic = None
+        tf = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
+        if tf.exists():
+            traffic = json.loads(tf.read_text()).get("xc_bytes_per_evaluation")
+        flops = xc_flops(hb, 36, 36)
         out["roofline"] = {"bound": "hbm", "kernel": "k_xc_rad_in + k_xc_point + k_xc_rad_out (one Vxc evaluation)",
                            "achieved": nbytes / (x_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": nbytes / (x_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "frac": nbytes / (x_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                            "avg_launch_ms": x_ms, "bytes_per_launch": nbytes,
-                           "note": "bytes: element-diagonal P bands in + H bands out over all shell pairs"}
+                           "note": "bytes: element-diagonal P bands in + H bands out over all shell pairs; "
+                                   "traffic: rocprofv3 FETCH/WRITE of the three kernels (profiles/)",
+                           "fp64_vector": {"flops_per_launch": flops,
+                                           "achieved_tflops": flops / (x_ms * 1e-3) / 1e12,
+                                           "frac": flops / (x_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
         log(f"{steps} LDA builds {ms_per_step:.3f} ms/step; XC kernels {x_ms:.3f} ms -> "
             f"{out['roofline']['achieved']:.0f} GB/s")
     else:
@@ -329,6 +356,37 @@ def scf_compare(hb, nel, xc=None, legs=2):
     return out
 
 
+def cohort_scf(hb, Z, ctx, legs=2):
+    """One atom through the vectorised cohort (per-(l, m) blocks, fixed
+    ground-state occupations), FP64 and mixed, after a warm-up solve, legs
+    alternated; setup excluded."""
+    from helfem_amd import batch
+    from helfem_amd.cohort import Cohort
+    s = batch.atom_system(hb, Z)
+    Cohort([s], ctx, conv=1e-10, occupations=[s.occ]).solve()
+    order = []
+    for q in range(legs):
+        order += ["fp64", "mixed"] if q % 2 == 0 else ["mixed", "fp64"]
+    res = {"fp64": [], "mixed": []}
+    for prec in order:
+        co = Cohort([s], ctx, precision=prec, conv=1e-10, occupations=[s.occ])
+        t = time.perf_counter()
+        r = co.solve()[0]
+        dt = time.perf_counter() - t
+        res[prec].append((dt, r))
+        log(f"cohort SCF {prec}: E = {r['E']:.10f} in {r['iterations']} iterations, {dt:.3f}s")
+    out = {}
+    for prec, rs in res.items():
+        out[prec] = {"solves_per_s": 1.0 / float(np.mean([t for t, _ in rs])),
+                     "seconds_per_solve": [round(t, 4) for t, _ in rs], "iterations": rs[-1][1]["iterations"],
+                     "energy": rs[-1][1]["E"]}
+    out["mixed_vs_fp64"] = {"speedup": out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"],
+                            "energy_difference_Eh": out["mixed"]["energy"] - out["fp64"]["energy"]}
+    out["engine"] = ("vectorised cohort of one atom: per-(l, m) blocks with the ground-state occupations, "
+                     "batched eigensolves; dE < 1e-10 Eh")
+    return out
+
+
 def atoms_bench(H, ctx, n, seed=1004, nseq=None, budget_bytes=96 << 30):
     """Random atoms drawn like the 512-atom batch: the vectorised cohort in
     mixed precision (and FP64) against the per-system FP64 driver solving the
@@ -341,11 +399,12 @@ def atoms_bench(H, ctx, n, seed=1004, nseq=None, budget_bytes=96 << 30):
                                  lmax_range=(0, 4), setup_threads=16)
     setup = time.perf_counter() - t
     nbf = [s.info["Nbf"] for s in systems]
-    log(f"{n} atoms, Nbf {min(nbf)}..{max(nbf)}; host setup {setup:.1f}s")
+    log(f"{n} atoms, Nbf {min(nbf)}..{max(nbf)}; setup {setup:.1f}s")
     Cohort(systems[:2], ctx, conv=1e-8).solve()  # warm-up: device uploads of two systems' tables
     pinned = sum(s.occ is not None for s in systems)
     out = {"workload": f"{n} random neutral atoms (seed {seed}): Z 1..86, 4..24 LIP elements of order 10 or 15, "
-                       f"lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh", "host_setup_s": setup,
+                       f"lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh", "setup_s": setup,
+           "setup": "bases on 16 host threads, two-electron setups on the GPU (hf_atomic_compute_tei_device)",
            "Nbf_range": [min(nbf), max(nbf)],
            "occupations": f"{pinned} of {n} with the fixed per-block occupations of their Madelung/Hund "
                           f"configuration (the reference's --readocc); the rest (lmax below an occupied l) "
@@ -534,6 +593,7 @@ def main():
         configs["xe_lda"] = lda
         if not args.no_scf:
             configs["xe_scf"] = scf_compare(xb, 54)
+            configs["xe_scf_cohort"] = cohort_scf(xb, 54, ctx)
         del xb, xP
         torch.cuda.empty_cache()
         configs["atoms"] = atoms_bench(H, ctx, args.systems or 64)

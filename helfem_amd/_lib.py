@@ -78,6 +78,7 @@ SIGNATURES = {
     "hf_diatomic_maps": (c_int, [c_vp, c_vp, c_vp]),
     "hf_diatomic_set_absm_symmetric": (c_int, [c_vp, c_int]),
     "hf_diatomic_compute_tei": (c_int, [c_vp, c_int]),
+    "hf_diatomic_compute_tei_device": (c_int, [c_vp, c_vp]),
     "hf_diatomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),
     "hf_diatomic_one_electron": (c_int, [c_vp, c_int, c_dp, c_i64]),
     "hf_diatomic_one_electron_dev": (c_int, [c_vp, c_int, c_dp, c_i64]),

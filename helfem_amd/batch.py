@@ -137,10 +137,13 @@ def _spins(Z):
     return (Z + 1) // 2, Z // 2
 
 
-def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax_range=(0, 4), setup_threads=8):
+def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax_range=(0, 4), setup_threads=8,
+                 device_setup=True):
     """Seeded atomic systems: Z ~ U{1..zmax}, elements ~ U{nel_range}, LIP
     order from `orders`, lmax = mmax ~ U{lmax_range}; neutral, spins from
-    Hund's rule.  Host two-electron setups run on a thread pool."""
+    Hund's rule.  The bases are built on a thread pool, their two-electron
+    setups on the GPU (device_setup, the default when ctx is given) or on
+    the host."""
     from .atomic import AtomicBasis
     rng = np.random.default_rng(seed)
     specs = []
@@ -154,7 +157,7 @@ def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax
     def build(spec):
         Z, nel, nnodes, lmax = spec
         b = AtomicBasis(Z, lmax, lmax, nel, nnodes=nnodes, ctx=ctx)
-        b.compute_tei()
+        b.compute_tei(device=device_setup and ctx is not None)
         return b
 
     with ThreadPoolExecutor(setup_threads) as ex:

@@ -303,6 +303,7 @@ void diatomic_upload(hf_diatomic *h) {
   d.preB.alloc((size_t)std::max(nLM, 1) * Nel);
   hip_check(hipStreamSynchronize(s), "upload sync");
   diatomic_build_operator(S, d, s);
+  diatomic_op_build(d, s);
   h->uploaded = true;
 }
 
@@ -673,6 +674,7 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
   // in-element workgroups of the LDS-resident kernel: runs grouped by ilm,
   // batches of KL_BATCH per (ilm, element), element e's batches on XCD e % 8
   d.n_wg_kl = 0;
+  d.n_wg_op = 0;
   if (!d.dense_in && d.max_rank <= KL_RANK_MAX && !rilm.empty()) {
     const int Nel = S.Nel(), nr = (int)rilm.size();
     std::vector<int> run_act(nr), cnt(nlm + 1, 0), runs;
@@ -707,6 +709,27 @@ void build_exchange_lists(hf_diatomic *h, hipStream_t s) {
     d.kl_runs.upload(runs, s);
     d.run_act.upload(run_act, s);
     d.n_wg_kl = (int)(mx * 8);
+    // the matrix-core operator form: OP_RUNS runs per workgroup, the groups
+    // of one (ilm, element) dealt to one XCD so they share its operator in L2
+    if (d.op_in) {
+      std::vector<std::vector<int>> px(8);
+      int grp = 0;
+      for (int q = 0; q < nlm; ++q)
+        for (int e = 0; e < Nel; ++e) {
+          if (cnt[q + 1] == cnt[q]) continue;
+          auto &v = px[grp++ % 8];
+          for (int b = cnt[q]; b < cnt[q + 1]; b += OP_RUNS)
+            v.insert(v.end(), {q, e, b, std::min(OP_RUNS, cnt[q + 1] - b)});
+        }
+      size_t mo = 0;
+      for (auto &v : px) mo = std::max(mo, v.size() / 4);
+      std::vector<int> wo(mo * 8 * 4, -1);
+      for (int xc = 0; xc < 8; ++xc)
+        for (size_t j = 0; j < px[xc].size() / 4; ++j)
+          for (int u = 0; u < 4; ++u) wo[(j * 8 + xc) * 4 + u] = px[xc][4 * j + u];
+      d.wg_op.upload(wo, s);
+      d.n_wg_op = (int)(mo * 8);
+    }
   }
   if (act.empty()) act.push_back(0);
   if (rilm.empty()) rilm.push_back(0);
@@ -864,6 +887,16 @@ int hf_diatomic_compute_tei(hf_diatomic *h, int exchange) {
   return guarded([&] {
     need(h != nullptr, "NULL handle");
     h->host.compute_tei();
+    h->uploaded = false;
+  });
+}
+
+int hf_diatomic_compute_tei_device(hf_diatomic *h, double *seconds) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    diatomic_tei_device(h->host, h->ctx->cur, seconds);
     h->uploaded = false;
   });
 }

@@ -883,6 +883,157 @@ __global__ void __launch_bounds__(KL_THREADS) k_dia_kin_lds(DKLArgs<T> A) {
   }
 }
 
+// --- In-element exchange as a dense operator on the matrix cores ---------------
+//
+// For one ((L,|M|), element e) every coupled output block's in-element tile
+// is Y(a,b) = sum_ch sum_cd T_ch((a,b),(c,d)) R_ch(c,d), the four channel
+// tiles R_ch the run's Gaunt-weighted density sums and T_ch = sign_ch W~_ch
+// with W~ = B diag(sigma) B' (basis.cpp:1992-2033 contracts the same
+// factor).  Applied as a GEMM Y (225 x runs) = T (225 x 912) X (912 x runs) it
+// costs 2 n^4 per channel and run against 12 r n^3 / 4 of the Cholesky form
+// (r ~ 30: 3.4x fewer matrix operations).
+//
+// k_dia_op_build (at upload) writes T in the matrix-core operand order: for
+// k-step ks (inner indices kappa = 4 ks + g, kappa = ch * 228 + c + 15 d) and
+// row tile rt (rows rho = 16 rt + c, rho = a + 15 b), lane (g, c) holds
+// T(rho, kappa); one aligned 512-byte read per wave and (ks, rt).
+// k_dia_kin_op: one 256-thread workgroup per ((L,|M|), e, up to 16 runs):
+// the waves gather the runs' four channel tiles from the tiled density into
+// LDS (X[kappa][run], rows padded to 17), then each wave multiplies its row
+// tiles of T (streamed from L2: the workgroups of one (L,|M|), e) run on one
+// XCD) by X on the FP64 matrix cores and adds the tiles into K atomically
+// (boundary rows shared with other launches; other (L,|M|) of the block in
+// other workgroups), with the mirror of a symmetric density as k_dia_kin_lds.
+
+constexpr int OP_NC = 16, OP_KC = 228, OP_K = 4 * OP_KC, OP_KS = OP_K / 4, OP_RT = 15, OP_XLD = 17;
+constexpr size_t OP_TSIZE = (size_t)OP_KS * OP_RT * 64;  // doubles of one (ilm, element) operator
+
+struct DOpBuildArgs {
+  const double *chol, *sigma;
+  const int *chol_off, *chol_rank, *ni;
+  int Nel;
+  double *T;  // [ilm*Nel + e][OP_TSIZE]
+};
+
+__global__ void __launch_bounds__(NT) k_dia_op_build(DOpBuildArgs A) {
+  const int x = blockIdx.x / OP_KS, ks = blockIdx.x - x * OP_KS;
+  const int e = x % A.Nel, n = A.ni[e];
+  const int rank = A.chol_rank[x], off = A.chol_off[x];
+  const double *M = A.chol + (size_t)off * 2 * TILE2;
+  double *out = A.T + (size_t)x * OP_TSIZE + (size_t)ks * OP_RT * 64;
+  for (int q = threadIdx.x; q < OP_RT * 64; q += NT) {
+    const int rt = q >> 6, lane = q & 63, g = lane >> 4, c = lane & 15;
+    const int rho = 16 * rt + c, kap = 4 * ks + g;
+    const int ch = kap / OP_KC, kc = kap - ch * OP_KC;
+    const int cc = kc % 15, dd = kc / 15, a = rho % 15, b = rho / 15;
+    double v = 0.0;
+    if (rho < 225 && kc < 225 && a < n && b < n && cc < n && dd < n) {
+      // B(xo + cc + a n, p) B(yo + b + dd n, p): channel block x of M_p at (cc, a), y at (b, dd)
+      const int xo = ch >> 1, yo = ch & 1;
+      for (int p = 0; p < rank; ++p) {
+        const double *Mp = M + (size_t)p * 2 * TILE2;
+        v += A.sigma[off + p] * Mp[xo * TILE2 + cc + TILE * a] * Mp[yo * TILE2 + b + TILE * dd];
+      }
+      if (ch == 1 || ch == 2) v = -v;
+    }
+    out[q] = v;
+  }
+}
+
+struct DOpArgs {
+  double *K;
+  int64_t ldk;
+  int Nel, Nang;
+  const int *first, *ni, *skip;
+  const long long *poff;
+  const double *T;
+  const int *act_jk, *lrun_tstart;
+  const double *term_f;
+  const int *wg;       // per workgroup: ilm, e, first entry in runs, count (or ilm = -1)
+  const int *runs, *run_act;
+  const double *pt;    // tiled density
+  const int *ttile;
+  int n_src;
+  int sym;
+};
+
+__global__ void __launch_bounds__(NT) k_dia_kin_op(DOpArgs A) {
+  __shared__ double X[OP_K * OP_XLD];
+  const int w = blockIdx.x;
+  const int ilm = A.wg[4 * w], e = A.wg[4 * w + 1], b0 = A.wg[4 * w + 2], nb = A.wg[4 * w + 3];
+  if (ilm < 0) return;
+  const int t = threadIdx.x, lane = t & 63, g = lane >> 4, c = lane & 15;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int n = A.ni[e], fe = A.first[e];
+  for (int q = t; q < OP_K * OP_XLD; q += NT) X[q] = 0.0;
+  __syncthreads();
+  const double *pt = A.pt + (size_t)(e * A.Nel + e) * A.n_src * TILE2 + lane * 4;
+  for (int i = wv; i < nb; i += NT / 64) {
+    const int run = A.runs[b0 + i];
+    double r[4][4];
+    gather_tiles<double>(pt, A.ttile, A.term_f, A.lrun_tstart[run], A.lrun_tstart[run + 1], r[0], r[1], r[2],
+                         r[3]);
+    // lane (g, c) holds R_ch(c, kk) in entry s, kk = row(g, s): kappa = ch * 228 + c + 15 kk
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kk = Mma<double>::row(g, s);
+      if (c < n && kk < n)
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch) X[(ch * OP_KC + c + 15 * kk) * OP_XLD + i] = r[ch][s];
+    }
+  }
+  __syncthreads();
+  // row tiles wv, wv + 4, wv + 8, wv + 12 (< 15)
+  const int nrt = wv < 3 ? 4 : 3;
+  v4d acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  // One wave per SIMD (the X tile takes the LDS): the operator reads are
+  // issued OP_PF k-steps ahead so their L2 latency hides behind the matrix work.
+  constexpr int OP_PF = 4;
+  static_assert(OP_KS % OP_PF == 0, "k-steps in whole prefetch rounds");
+  const double *Tx = A.T + (size_t)(ilm * A.Nel + e) * OP_TSIZE + lane + (size_t)wv * 64;
+  double abuf[OP_PF][4];
+#pragma unroll
+  for (int u = 0; u < OP_PF; ++u)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) abuf[u][j] = j < nrt ? Tx[(size_t)u * OP_RT * 64 + 4 * j * 64] : 0.0;
+  for (int ks0 = 0; ks0 < OP_KS; ks0 += OP_PF) {
+#pragma unroll
+    for (int u = 0; u < OP_PF; ++u) {
+      const int ks = ks0 + u;
+      const double bv = X[(4 * ks + g) * OP_XLD + c];
+      double av[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) av[j] = abuf[u][j];
+      if (ks + OP_PF < OP_KS) {
+        const double *Tn = Tx + (size_t)(ks + OP_PF) * OP_RT * 64;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) abuf[u][j] = j < nrt ? Tn[4 * j * 64] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < nrt) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv, acc[j], 0, 0, 0);
+    }
+  }
+  // lane (g, c) holds Y(rho = 16 rt + g + 4 r, run c) in acc[j][r]
+  if (c >= nb) return;
+  const int run = A.runs[b0 + c], act = A.run_act[run];
+  const int jk = A.act_jk[act], j = jk / A.Nang, k = jk - j * A.Nang;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    if (jj >= nrt) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rho = 16 * (wv + 4 * jj) + g + 4 * r;
+      const int a = rho % 15, b = rho / 15;
+      if (rho >= 225 || a >= n || b >= n) continue;
+      const long long row = pure_row(A.skip, A.poff, j, fe + a), col = pure_row(A.skip, A.poff, k, fe + b);
+      if (row < 0 || col < 0) continue;
+      atomicAdd(A.K + row + col * A.ldk, -acc[jj][r]);
+      if (A.sym && j != k) atomicAdd(A.K + col + row * A.ldk, -acc[jj][r]);
+    }
+  }
+}
+
 // --- In-element exchange as a dense operator ----------------------------------
 //
 // For one ((L,|M|), element e) the in-element exchange of every output block
@@ -1011,6 +1162,26 @@ __global__ void __launch_bounds__(NT) k_dia_mirror(double *K, int64_t ldk, int N
 }
 
 }  // namespace
+
+void diatomic_op_build(DiatomicDevice &d, hipStream_t s) {
+  d.op_in = false;
+  d.Top2.release();
+  const size_t nx = (size_t)d.n_lm * d.Nel;
+  if (nx == 0 || d.max_rank > 2 * TILE2) return;
+  // opt-in: on N2 (lmax 16) the Cholesky-form kernel is faster (13.1 against
+  // 16.2 ms): each workgroup streams its 1.75 MB operator for 16 runs
+  const char *env = std::getenv("HFGPU_EXCHANGE_OP");
+  if (!env || std::atoi(env) == 0) return;
+  size_t fr = 0, tot = 0;
+  hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  const size_t bytes = nx * OP_TSIZE * sizeof(double);
+  if (bytes > fr / 4) return;  // keep three quarters of the free memory for densities and tiles
+  d.Top2.alloc(nx * OP_TSIZE);
+  DOpBuildArgs a{d.chol.p, d.sigma.p, d.chol_off.p, d.chol_rank.p, d.ni.p, d.Nel, d.Top2.p};
+  hipLaunchKernelGGL(k_dia_op_build, dim3((unsigned)(nx * OP_KS)), dim3(NT), 0, s, a);
+  hip_check(hipGetLastError(), "diatomic_op_build");
+  d.op_in = true;
+}
 
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,
                              hipStream_t s, KernelTimer *timer) {
@@ -1164,6 +1335,17 @@ void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, 
       hip_check(hipGetLastError(), "diatomic_kin_gemm");
       hipLaunchKernelGGL(k_dia_kin_reduce, dim3(d.n_act * d.Nel), dim3(NT), 0, s, r);
       hip_check(hipGetLastError(), "diatomic_kin_reduce");
+      continue;
+    }
+    if (part == 1 && d.op_in && d.tiled && !fp32 && d.n_wg_op > 0) {
+      // timed against the Cholesky-form flop count of the same results (the
+      // reference's algorithm), so the rate compares with the other form
+      TimedLaunch tl(timer, name, s, flops);
+      DOpArgs a{K, ldk, d.Nel, d.Nang, d.first.p, d.ni.p, d.skip.p, d.poff.p, d.Top2.p, d.act_jk.p,
+                d.lrun_tstart.p, d.term_f.p, d.wg_op.p, d.kl_runs.p, d.run_act.p, d.ptile.p, d.term_tile.p,
+                d.n_src, (int)d.sym};
+      hipLaunchKernelGGL(k_dia_kin_op, dim3(d.n_wg_op), dim3(NT), 0, s, a);
+      hip_check(hipGetLastError(), "diatomic_kin_op");
       continue;
     }
     if (part == 1 && d.n_wg_kl > 0) {

@@ -283,6 +283,14 @@ struct DiatomicDevice {
   // first entry in kl_runs, count); runs grouped by ilm; output block per run
   DevArray<int> wg_kl, kl_runs, run_act;
   int n_wg_kl = 0;
+  // in-element exchange as a dense operator on the matrix cores
+  // (k_dia_kin_op): the operators of every (ilm, element) in operand order,
+  // built on the device at upload when they fit the budget; per workgroup
+  // (ilm, e, first entry in kl_runs, count <= 16)
+  bool op_in = false;
+  DevArray<double> Top2;
+  DevArray<int> wg_op;
+  int n_wg_op = 0;
   // Density tiles of the screened source blocks in gather order
   // ([e*Nel+f][src][lane][4], see k_dia_ptile), rebuilt per exchange call;
   // src_blk lists the blocks i*Nang+l, term_tile maps each term to its tile.
@@ -356,6 +364,10 @@ void noalloc_scratch_init(int device);
 // built on the device (tei_atomic.hip); fills the same host caches.
 // seconds (optional) gets the wall time.  Synchronises s.
 void atomic_tei_device(AtomicSetup &S, hipStream_t s, double *seconds);
+// the diatomic compute_tei with the two-channel kernels and their signed
+// Cholesky factors built on the device (tei_diatomic.hip); P/Q integrals,
+// rule sizes and outer-node Q values on the host.  Synchronises s.
+void diatomic_tei_device(DiatomicSetup &S, hipStream_t s, double *seconds);
 // fp32: gathers and matrix products in FP32 (the result stays FP64, with
 // about 1e-6 relative error); otherwise the reference's FP64 arithmetic.
 void atomic_exchange_launch(AtomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
@@ -442,5 +454,9 @@ bool symmetric_exchange_allowed();
 void diatomic_exchange_kernels(DiatomicDevice &d, const double *P, int64_t ldp, double *K, int64_t ldk,
                                hipStream_t s, KernelTimer *timer, bool fp32 = false);
 void diatomic_build_operator(const DiatomicSetup &S, DiatomicDevice &d, hipStream_t s);
+// the matrix-core operators of k_dia_kin_op from the device Cholesky tiles
+// (opt-in with HFGPU_EXCHANGE_OP=1; sets d.op_in when they fit the budget)
+void diatomic_op_build(DiatomicDevice &d, hipStream_t s);
+constexpr int OP_RUNS = 16;  // runs per k_dia_kin_op workgroup
 
 }  // namespace hfg

@@ -471,17 +471,11 @@ int DiatomicSetup::twoe_order(int iel) const {
   return nconv;
 }
 
-// compute_tei, basis.cpp:1382-1547
-void DiatomicSetup::compute_tei() {
-  const int ne = Nel(), nlm = (int)lm_map.size();
-  P0.assign((size_t)ne * nlm, Blk());
-  P2 = P0;
-  Q0 = P0;
-  Q2 = P0;
-  cdB = P0;
-  cdS.assign((size_t)ne * nlm, std::vector<double>());
-  // runs of equal |M| in lm_map, with the largest L of each run
-  std::vector<int> run_lo, run_hi, run_Lhi;
+namespace {
+// runs of equal |M| in lm_map, with the largest L of each run
+void m_runs(const std::vector<std::pair<int, int>> &lm_map, std::vector<int> &run_lo, std::vector<int> &run_hi,
+            std::vector<int> &run_Lhi) {
+  const int nlm = (int)lm_map.size();
   for (int i = 0; i < nlm;) {
     int j = i, Lhi = 0;
     while (j < nlm && lm_map[j].second == lm_map[i].second) Lhi = std::max(Lhi, lm_map[j++].first);
@@ -490,6 +484,34 @@ void DiatomicSetup::compute_tei() {
     run_Lhi.push_back(Lhi);
     i = j;
   }
+}
+}  // namespace
+
+std::vector<double> DiatomicSetup::outer_legendre_q(int iel, int n, int Mabs, int Lmax) const {
+  // the outer nodes of element_rule
+  std::vector<double> x, w;
+  gauss_chebyshev2(n, x, w);
+  const double lo = rad.bval[iel], hi = rad.bval[iel + 1];
+  const double mid = 0.5 * (hi + lo), len = 0.5 * (hi - lo);
+  LegendreCache leg(Mabs, Lmax);
+  std::vector<double> out((size_t)n * (Lmax + 1));
+  for (int i = 0; i < n; ++i) {
+    const double ch = std::cosh(mid + len * x[i]);
+    for (int L = 0; L <= Lmax; ++L) out[(size_t)i * (Lmax + 1) + L] = leg.Q(L, ch);
+  }
+  return out;
+}
+
+void DiatomicSetup::compute_pq() {
+  const int ne = Nel(), nlm = (int)lm_map.size();
+  P0.assign((size_t)ne * nlm, Blk());
+  P2 = P0;
+  Q0 = P0;
+  Q2 = P0;
+  cdB = P0;
+  cdS.assign((size_t)ne * nlm, std::vector<double>());
+  std::vector<int> run_lo, run_hi, run_Lhi;
+  m_runs(lm_map, run_lo, run_hi, run_Lhi);
   const int nrun = (int)run_lo.size();
 #pragma omp parallel for schedule(dynamic)
   for (int t = 0; t < nrun * ne; ++t) {
@@ -504,6 +526,22 @@ void DiatomicSetup::compute_tei() {
       Q2[i] = pq_integral(true, 2, e, L, leg);
     }
   }
+}
+
+std::vector<int> DiatomicSetup::twoe_orders() const {
+  std::vector<int> order(Nel());
+#pragma omp parallel for schedule(dynamic)
+  for (int e = 0; e < Nel(); ++e) order[e] = twoe_order(e);
+  return order;
+}
+
+// compute_tei, basis.cpp:1382-1547
+void DiatomicSetup::compute_tei() {
+  compute_pq();
+  const int ne = Nel();
+  std::vector<int> run_lo, run_hi, run_Lhi;
+  m_runs(lm_map, run_lo, run_hi, run_Lhi);
+  const int nrun = (int)run_lo.size();
   std::vector<int> order(ne);
 #pragma omp parallel for schedule(dynamic)
   for (int e = 0; e < ne; ++e) order[e] = twoe_order(e);

@@ -67,6 +67,13 @@ struct DiatomicSetup {
   int LMind(int L, int M) const;
   double LMfac(int L, int M) const;
   void compute_tei();
+  // the disjoint P/Q integrals alone (the first half of compute_tei) and the
+  // sizing of the in-element factor caches
+  void compute_pq();
+  // Q_L^|M|(cosh mu_i) at the n outer Chebyshev nodes of element iel, the
+  // values the in-element kernel's outer contraction weights with, as
+  // [i * (Lmax + 1) + L] (the device setup takes them from here)
+  std::vector<double> outer_legendre_q(int iel, int n, int Mabs, int Lmax) const;
   // dense pure-basis one-electron matrices, column-major Nbf x Nbf
   void overlap(double *out, int64_t ld) const;
   void kinetic(double *out, int64_t ld) const;
@@ -78,6 +85,7 @@ struct DiatomicSetup {
   Blk radial_kinetic() const;               // int B_i' B_j' sinh
   Blk pq_integral(bool q, int k, int iel, int L, LegendreCache &leg) const;
   int twoe_order(int iel) const;
+  std::vector<int> twoe_orders() const;  // twoe_order of every element (OpenMP)
 };
 
 }  // namespace hfg

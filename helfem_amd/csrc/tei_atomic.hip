@@ -289,6 +289,18 @@ __global__ void __launch_bounds__(256) k_tei_chol(CholArgs C) {
   if (i == 0) C.rank[T.z] = rank;
 }
 
+// Packs each task's rank columns (n2 rows each) contiguously at off[task].
+__global__ void __launch_bounds__(256) k_tei_pack(const double *F, const TeiElem *el, const int4 *task,
+                                                  const int *rank, const long long *off, double *dst) {
+  const int4 T = task[blockIdx.x];
+  const int n2 = el[T.y].nb * el[T.y].nb, r = rank[T.z];
+  const double *src = F + (size_t)T.z * TSTRIDE;
+  for (int q = threadIdx.x; q < r * n2; q += 256) {
+    const int p = q / n2, a = q - p * n2;
+    dst[off[T.z] + q] = src[(size_t)p * NB2S + a];
+  }
+}
+
 }  // namespace
 
 namespace hfg {
@@ -391,18 +403,28 @@ void atomic_tei_device(AtomicSetup &S, hipStream_t s, double *seconds) {
   CholArgs C{d_el.p, d_all.p, buf[0].p, buf[1].p, fac.p, d_rank.p, S.chol_tol};
   hipLaunchKernelGGL(k_tei_chol, dim3(ntask), dim3(256), 0, s, C);
   hip_check(hipGetLastError(), "device pivoted Cholesky");
+  // ranks first, then only the factors' rank columns come back (packed)
   std::vector<int> rk(ntask);
   hip_check(hipMemcpyAsync(rk.data(), d_rank.p, ntask * sizeof(int), hipMemcpyDeviceToHost, s), "ranks");
-  std::vector<double> F((size_t)ntask * TSTRIDE);
-  hip_check(hipMemcpyAsync(F.data(), fac.p, F.size() * sizeof(double), hipMemcpyDeviceToHost, s), "factors");
+  hip_check(hipStreamSynchronize(s), "device two-electron setup");
+  std::vector<long long> off(ntask + 1, 0);
+  for (int i = 0; i < ntask; ++i) off[i + 1] = off[i] + (long long)rk[i] * el[i % nel].nb * el[i % nel].nb;
+  DevArray<long long> d_off;
+  DevArray<double> packed;
+  PinnedArray<double> hp;
+  d_off.upload(off, s);
+  packed.alloc((size_t)std::max<long long>(off[ntask], 1));
+  hp.alloc((size_t)std::max<long long>(off[ntask], 1));
+  hipLaunchKernelGGL(k_tei_pack, dim3(ntask), dim3(256), 0, s, fac.p, d_el.p, d_all.p, d_rank.p, d_off.p, packed.p);
+  hip_check(hipGetLastError(), "device factor pack");
+  hip_check(hipMemcpyAsync(hp.p, packed.p, off[ntask] * sizeof(double), hipMemcpyDeviceToHost, s), "factors");
   hip_check(hipStreamSynchronize(s), "device two-electron setup");
   S.compute_radial_factors();
   S.chol.assign((size_t)NL * nel, Blk());
   for (int i = 0; i < ntask; ++i) {
     const int e = i % nel, n2 = el[e].nb * el[e].nb, r = rk[i];
     Blk Lf(n2, r);
-    for (int j = 0; j < r; ++j)
-      for (int q = 0; q < n2; ++q) Lf(q, j) = F[(size_t)i * TSTRIDE + (size_t)j * NB2S + q];
+    std::copy(hp.p + off[i], hp.p + off[i + 1], Lf.v.begin());
     S.chol[i] = std::move(Lf);
   }
   S.have_tei = true;

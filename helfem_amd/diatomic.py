@@ -77,8 +77,16 @@ class DiatomicBasis:
             self._h = ctypes.c_void_p()
 
     # --- setup ---------------------------------------------------------
-    def compute_tei(self, exchange: bool = True):
-        check(lib().hf_diatomic_compute_tei(self._h, int(exchange)))
+    def compute_tei(self, exchange: bool = True, device: bool = False):
+        """The two-electron caches (basis.cpp:1382-1547).  device=True builds
+        the two-channel in-element kernels and their signed Cholesky factors
+        on the GPU (needs a context) and returns the wall time in seconds."""
+        if not device:
+            check(lib().hf_diatomic_compute_tei(self._h, int(exchange)))
+            return None
+        sec = ctypes.c_double(0.0)
+        check(lib().hf_diatomic_compute_tei_device(self._h, ctypes.byref(sec)))
+        return sec.value
 
     def set_absm_symmetric(self, sym: bool):
         check(lib().hf_diatomic_set_absm_symmetric(self._h, int(bool(sym))))

@@ -313,6 +313,14 @@ int hf_diatomic_set_absm_symmetric(hf_diatomic *h, int sym);
  * two-channel kernel.  Replaces TwoDBasis::compute_tei (basis.h:265,
  * basis.cpp:1382-1547). */
 int hf_diatomic_compute_tei(hf_diatomic *h, int exchange);
+/* The same caches with the two-channel in-element kernels (the Chebyshev
+ * quadrature of quadrature.cpp:162-258, symmetrised as basis.cpp:1470-1490)
+ * and their signed pivoted Cholesky factors (basis.cpp:1492-1537) built on
+ * the GPU; the P/Q integrals, the per-element rule sizes and the Q values at
+ * the outer nodes stay on the host.  Replaces TwoDBasis::compute_tei like
+ * hf_diatomic_compute_tei; needs a context; synchronises its stream.
+ * seconds (may be NULL) gets the wall time. */
+int hf_diatomic_compute_tei_device(hf_diatomic *h, double *seconds);
 /* Read a cache back: kind 0 P0, 1 P2, 2 Q0, 3 Q2 (Ni x Ni), 4 the Cholesky
  * vectors B (2 Ni^2 x rank), 5 their signs (rank x 1).  dims gets
  * (rows, cols); data may be NULL. */

@@ -142,6 +142,24 @@ def test_xe_full_size_coulomb_exchange(density, gpu_ctx):
     assert err <= TOL
 
 
+def test_n2_device_setup_coulomb_exchange(gpu_ctx):
+    """The N2 two-electron setup with the two-channel kernels and their
+    signed Cholesky factors of all 224 x 8 ((L,|M|), element) pairs built on
+    the device: J and the sampled K from those caches meet the same bar
+    against the oracle."""
+    import time
+    print("[gpu] N2 device two-electron setup: J and sampled K vs oracle", flush=True)
+    ob, hb, P, shell = _n2(gpu_ctx)
+    t = time.perf_counter()
+    db = H.DiatomicBasis(N2["Z1"], N2["Z2"], N2["R"], N2["lmm"], N2["nel"], ctx=gpu_ctx)
+    dev = db.compute_tei(device=True)
+    total = time.perf_counter() - t
+    err = _coulomb_and_sampled_exchange(ob, db, shell, P)
+    print(f"[gpu]   device setup {dev:.3f} s ({total:.3f} s with the basis); sampled K rel err {err:.2e}",
+          flush=True)
+    assert err <= TOL
+
+
 def test_xe_device_setup_coulomb_exchange(gpu_ctx):
     """The Xe two-electron setup built on the device (Slater tensors and
     Cholesky factors of all 13 x 25 (L, element) pairs): J and the sampled K

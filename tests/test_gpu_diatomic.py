@@ -248,3 +248,30 @@ def test_one_electron_on_device(name, gpu_ctx):
         torch.cuda.synchronize()
         assert np.array_equal(dev, host), w
         assert np.array_equal(host, orc), w
+
+
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_inelement_operator_and_cholesky_forms_agree(name, gpu_ctx, monkeypatch):
+    """The in-element exchange as dense operators on the matrix cores
+    (opt-in, HFGPU_EXCHANGE_OP=1 at upload) and in the Cholesky form (the
+    default): both within the oracle bar, for a symmetric and a general
+    density."""
+    import torch
+    print(f"[gpu] in-element operator vs Cholesky form: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = CASES[name]
+    ob, _ = pair(name, gpu_ctx)
+    P = random_symmetric(ob.Nbf, seed=1012)
+    G = P + 0.3 * np.triu(random_symmetric(ob.Nbf, seed=1013))
+    # the device tables (and the form) are fixed at a handle's first build
+    ch = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    ch.compute_tei()
+    Kc = [ch.exchange(torch.as_tensor(D, device="cuda")).cpu().numpy() for D in (P, G)]
+    monkeypatch.setenv("HFGPU_EXCHANGE_OP", "1")
+    op = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    op.compute_tei()
+    Ko = [op.exchange(torch.as_tensor(D, device="cuda")).cpu().numpy() for D in (P, G)]
+    for D, kc, ko in zip((P, G), Kc, Ko):
+        Kr = ob.exchange(D)
+        assert relerr(ko, Kr) <= TOL, relerr(ko, Kr)
+        assert relerr(kc, Kr) <= TOL, relerr(kc, Kr)
+        assert not np.array_equal(ko, kc)  # two forms ran: their roundings differ
