@@ -402,13 +402,15 @@ def atoms_bench(H, ctx, n, seed=1004, nseq=None, budget_bytes=96 << 30):
     log(f"{n} atoms, Nbf {min(nbf)}..{max(nbf)}; setup {setup:.1f}s")
     Cohort(systems[:2], ctx, conv=1e-8).solve()  # warm-up: device uploads of two systems' tables
     pinned = sum(s.occ is not None for s in systems)
+    restricted = sum("configuration" in s.info for s in systems)
     out = {"workload": f"{n} random neutral atoms (seed {seed}): Z 1..86, 4..24 LIP elements of order 10 or 15, "
                        f"lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh", "setup_s": setup,
            "setup": "bases on 16 host threads, two-electron setups on the GPU (hf_atomic_compute_tei_device)",
            "Nbf_range": [min(nbf), max(nbf)],
-           "occupations": f"{pinned} of {n} with the fixed per-block occupations of their Madelung/Hund "
-                          f"configuration (the reference's --readocc); the rest (lmax below an occupied l) "
-                          f"by aufbau"}
+           "occupations": f"{pinned} of {n} with fixed per-(l, m) occupations (the reference's --readocc): "
+                          f"{pinned - restricted} of their Madelung/Hund ground-state configuration, {restricted} "
+                          f"(lmax below an occupied l) of the Madelung/Hund filling of the subshells the basis "
+                          f"holds; {n - pinned} by aufbau"}
     res = {}
     for prec in ("mixed", "fp64"):
         r, dt = solve_cohorts(systems, ctx, budget_bytes=budget_bytes, precision=prec, conv=1e-8,
@@ -503,14 +505,19 @@ def main_batch(args):
 
 
 def main_serve(args):
-    """A Poisson stream of diatomic SCF requests served by one engine: host
-    setup on a thread pool as requests arrive, admission by HBM footprint."""
+    """A Poisson stream of diatomic SCF requests served by one engine: each
+    request's basis built on a host thread pool as it arrives (two-electron
+    setup on the GPU), admission by HBM footprint."""
     import helfem_amd as H
     from helfem_amd import batch
     n = args.systems or 16
     ctx = H.Context(0)
     specs = batch.random_diatomic_specs(n, seed=1005)
-    res, wall = batch.serve(specs, args.rate, seed=1006, ctx=ctx, precision="mixed", conv=1e-8, setup_threads=16)
+    def done(i, r):
+        log(f"request {i} done at {r['done_s']:.1f}s: latency {r['latency_s']:.1f}s, {r['iterations']} iterations")
+
+    res, wall = batch.serve(specs, args.rate, seed=1006, ctx=ctx, precision="mixed", conv=1e-8, setup_threads=16,
+                            on_done=done)
     lat = batch.latency_summary(res)
     log(f"{n} requests at {args.rate}/s: {n / wall:.2f} solves/s, latency p50 {lat['p50_s']:.2f}s p99 "
         f"{lat['p99_s']:.2f}s")

@@ -167,13 +167,22 @@ def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax
 
 
 def atom_system(basis, Z, tag=None, info=None):
-    """A neutral atom with Hund's-rule spins and the fixed ground-state
-    occupations of its Madelung configuration where the basis holds every
-    occupied (l, m) (else None: aufbau)."""
+    """A neutral atom with the fixed occupations (and Hund's-rule spins) of
+    its Madelung ground-state configuration where the basis holds every
+    occupied (l, m); in a basis without some occupied l, of the Madelung/Hund
+    filling of the subshells it holds (what aufbau reaches there, pinned so
+    the state does not hinge on rounding); None (aufbau) if neither fits."""
     from . import occupations as oc
+    occ = oc.block_counts(Z, basis.lval, basis.mval, symm=2)
     na, nb = oc.spins(Z)
-    return System(basis, na, nb, tag=tag, info=dict(info or {}),
-                  occ=oc.block_counts(Z, basis.lval, basis.mval, symm=2))
+    info = dict(info or {})
+    if occ is None:
+        lmax = int(max(basis.lval))
+        occ = oc.block_counts(Z, basis.lval, basis.mval, symm=2, lmax=lmax)
+        if occ is not None:
+            na, nb = oc.spins(Z, lmax)
+            info["configuration"] = f"Madelung/Hund over l <= {lmax}"
+    return System(basis, na, nb, tag=tag, info=info, occ=occ)
 
 
 def random_diatomic_specs(n, seed, zmax=18, r_range=(1.0, 6.0), nel_range=(4, 12), lmax_range=(8, 24),
@@ -195,11 +204,13 @@ def random_diatomic_specs(n, seed, zmax=18, r_range=(1.0, 6.0), nel_range=(4, 12
     return out
 
 
-def build_diatomic(spec, ctx):
+def build_diatomic(spec, ctx, device_setup=True):
+    """A served request's system: the basis with its two-electron setup (on
+    the GPU by default) and, for LDA, its pure-m DFT grid."""
     from .diatomic import DiatomicBasis
     from .dftgrid import DFTGrid
     b = DiatomicBasis(spec["Z1"], spec["Z2"], spec["R"], [spec["lmax"]] * (spec["mmax"] + 1), spec["nelem"], ctx=ctx)
-    b.compute_tei(exchange=not spec["lda"])
+    b.compute_tei(exchange=not spec["lda"], device=device_setup and ctx is not None)
     ne = spec["Z1"] + spec["Z2"]
     s = System(b, (ne + 1) // 2, ne // 2, lda=spec["lda"], tag=spec, info=dict(spec, Nbf=b.Nbf))
     if spec["lda"]:
@@ -208,11 +219,12 @@ def build_diatomic(spec, ctx):
 
 
 def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=64 << 30, setup_threads=8,
-          build=build_diatomic, gconv=1e-6):
+          build=build_diatomic, gconv=1e-6, on_done=None):
     """Poisson arrivals at `rate_hz` of the requests in `specs`; each request's
     basis is prepared on a host thread pool as it arrives and admitted into
     the running engine when ready and when its footprint fits the budget.
-    Returns per-request results with arrival, admission and completion times."""
+    Returns per-request results with arrival, admission and completion times;
+    on_done(index, result) is called as each request completes."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, size=len(specs))
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -255,6 +267,8 @@ def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=
             r["latency_s"] = r["done_s"] - s.arrival
             r["spec"] = s.info
             results[i] = r
+            if on_done is not None:
+                on_done(i, r)
             del in_use[i]
     ex.shutdown()
     return [results[i] for i in range(len(specs))], time.perf_counter() - t0

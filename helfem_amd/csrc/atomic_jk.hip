@@ -1152,8 +1152,10 @@ __global__ void __launch_bounds__(NT) k_atomic_ptile_batch(const PtileArgs<T> *a
     const int s = batch_find(start, n, blockIdx.x);                                                     \
     atomic_k_body<INSIDE, TILED, T, NW>(args[s], blockIdx.x - start[s]);                                 \
   }
-HFG_K_BATCH(k_atomic_k_off_f64_batch, false, double, __attribute__((amdgpu_waves_per_eu(6))))
-HFG_K_BATCH(k_atomic_k_off_f32_batch, false, float, __attribute__((amdgpu_waves_per_eu(8))))
+// one wave per SIMD fewer than the single-system kernels: the per-system
+// argument struct read from memory needs the registers (no spills)
+HFG_K_BATCH(k_atomic_k_off_f64_batch, false, double, __attribute__((amdgpu_waves_per_eu(5))))
+HFG_K_BATCH(k_atomic_k_off_f32_batch, false, float, __attribute__((amdgpu_waves_per_eu(7))))
 HFG_K_BATCH(k_atomic_k_in_f64_batch, true, double, )
 HFG_K_BATCH(k_atomic_k_in_f32_batch, true, float, )
 #undef HFG_K_BATCH

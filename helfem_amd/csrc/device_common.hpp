@@ -98,7 +98,9 @@ __device__ inline int batch_find(const int *start, int n, int b) {
     else
       hi = mid - 1;
   }
-  return lo;
+  // the same for every lane: kept in a scalar register, so the system's
+  // argument struct is read with scalar loads
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 }  // namespace dev

@@ -18,14 +18,21 @@ from __future__ import annotations
 import numpy as np
 
 MADELUNG = sorted(((n, l) for n in range(1, 8) for l in range(0, n)), key=lambda s: (s[0] + s[1], s[0]))
+# the same order continued to high n, for fillings restricted to low l (an
+# s-only basis holds Rn's 86 electrons in 43 s orbitals)
+MADELUNG_LONG = sorted(((n, l) for n in range(1, 61) for l in range(0, n)), key=lambda s: (s[0] + s[1], s[0]))
 
 
-def configuration(Z):
-    """[(n, l, alpha m list, beta m list)] of the Madelung/Hund ground state."""
+def configuration(Z, lmax=None):
+    """[(n, l, alpha m list, beta m list)] of the Madelung/Hund ground state;
+    with lmax, of the Madelung/Hund filling of the subshells with l <= lmax
+    alone (the configuration aufbau reaches in a basis without the higher l)."""
     left, out = Z, []
-    for n, l in MADELUNG:
+    for n, l in (MADELUNG if lmax is None else MADELUNG_LONG):
         if left <= 0:
             break
+        if lmax is not None and l > lmax:
+            continue
         cap = 2 * (2 * l + 1)
         k = min(left, cap)
         na = min(k, 2 * l + 1)
@@ -36,19 +43,20 @@ def configuration(Z):
     return out
 
 
-def spins(Z):
+def spins(Z, lmax=None):
     """(nela, nelb) of the configuration."""
-    cfg = configuration(Z)
+    cfg = configuration(Z, lmax)
     return sum(len(a) for _, _, a, _ in cfg), sum(len(b) for _, _, _, b in cfg)
 
 
-def block_counts(Z, lval, mval, symm):
+def block_counts(Z, lval, mval, symm, lmax=None):
     """Occupied orbitals per spin and symmetry block of an atomic basis with
     shells (lval, mval), blocks ordered as AtomicBasis.get_sym_idx(symm):
     symm 2 one per shell (shell order), 1 one per m (ascending), 3 one per m
     and parity of l (even before odd).  Returns (alpha counts, beta counts),
     or None when a subshell of the configuration has no (l, m) shell in the
-    basis (lmax or mmax too small)."""
+    basis (lmax or mmax too small).  lmax restricts the configuration to the
+    subshells with l <= lmax (configuration(Z, lmax))."""
     lval, mval = np.asarray(lval).tolist(), np.asarray(mval).tolist()
     have = set(zip(lval, mval))
     if symm == 2:
@@ -65,7 +73,7 @@ def block_counts(Z, lval, mval, symm):
         raise ValueError("symm must be 1, 2 or 3")
     index = {k: i for i, k in enumerate(keys)}
     ca, cb = [0] * len(keys), [0] * len(keys)
-    for _, l, ma, mb in configuration(Z):
+    for _, l, ma, mb in configuration(Z, lmax):
         for counts, ms in ((ca, ma), (cb, mb)):
             for m in ms:
                 if (l, m) not in have:
