@@ -219,7 +219,7 @@ def build_diatomic(spec, ctx, device_setup=True):
 
 
 def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=64 << 30, setup_threads=8,
-          build=build_diatomic, gconv=1e-6, on_done=None):
+          build=build_diatomic, gconv=1e-6, on_done=None, overlap=True):
     """Poisson arrivals at `rate_hz` of the requests in `specs`; each request's
     basis is prepared on a host thread pool as it arrives and admitted into
     the running engine when ready and when its footprint fits the budget.
@@ -235,6 +235,7 @@ def serve(specs, rate_hz, seed, ctx, precision="fp64", conv=1e-10, budget_bytes=
     eng = SCFEngine()
     in_use, admitted_at, results = {}, {}, {}
     nxt = 0
+    eng.overlap = overlap
     while len(results) < len(specs):
         now = time.perf_counter() - t0
         while nxt < len(specs) and arrivals[nxt] <= now:

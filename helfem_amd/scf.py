@@ -519,8 +519,11 @@ class SCFState:
         self.clk.tick("setup")
         self.nspin = 1 if restricted else 2
         self.weight = 2.0 if restricted else 1.0
-        self.fp32 = precision == "mixed"
-        self.mixed = precision == "mixed"
+        # reduced precision applies to the exchange; a pure DFT build (xc set,
+        # no exchange) has none, so its mixed mode is the FP64 one (no switch,
+        # no DIIS restart)
+        self.mixed = precision == "mixed" and xc is None
+        self.fp32 = self.mixed
         self.it_fp32 = 0
         # incremental exchange of the mixed mode after the FP32 phase:
         # K(P) = K64(P_ref) + K32(P - P_ref) against the last FP64 build, while
@@ -597,10 +600,15 @@ class SCFState:
         clk, nspin = self.clk, self.nspin
         tick = clk.tick if stream is None else (lambda name: None)
         Ps = [d[0] for d in self.dens]
-        P = Ps[0] if self.restricted else Ps[0] + Ps[1]
         scope = torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
         made = []
         with scope:
+            # the total density is formed on the build's own stream: `stream`
+            # waited for the densities (made on the caller's stream) before
+            # this call, not for work queued on the caller's stream after it
+            P = Ps[0] if self.restricted else Ps[0] + Ps[1]
+            if not self.restricted:
+                made.append(P)  # made on this stream, used by complete() on the caller's
             if stream is not None:
                 for X in Ps + [P]:
                     X.record_stream(stream)

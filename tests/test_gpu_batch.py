@@ -44,3 +44,31 @@ def test_served_stream_matches_alone(gpu_ctx):
         assert r["done_s"] >= r["admitted_s"] >= r["arrival_s"] - 1e-9
     lat = batch.latency_summary(res)
     assert lat["p99_s"] >= lat["p50_s"] > 0
+
+
+def test_overlapped_open_shell_builds(gpu_ctx):
+    """Open-shell (unrestricted) LDA and HF states whose Fock builds overlap on
+    per-state streams: each converges to its energy alone.  Regression for the
+    total density of an unrestricted state being formed on the caller's stream
+    after the build stream had synchronised with it (the build read a density
+    still being written and some states never converged)."""
+    from helfem_amd.scf import SCFEngine
+    print("[gpu] overlapped open-shell Fock builds vs alone", flush=True)
+    specs = [dict(Z1=7, Z2=2, R=2.0, nelem=4, lmax=8, mmax=1, lda=True),
+             dict(Z1=3, Z2=2, R=3.0, nelem=4, lmax=6, mmax=1, lda=True),
+             dict(Z1=5, Z2=2, R=2.5, nelem=4, lmax=6, mmax=1, lda=False),
+             dict(Z1=1, Z2=2, R=1.5, nelem=3, lmax=6, mmax=0, lda=True)]
+    systems = [batch.build_diatomic(s, gpu_ctx) for s in specs]
+    assert all(not s.restricted for s in systems)
+    for rep in range(3):
+        eng = SCFEngine(overlap=True)
+        states = [batch.state_for(s, "fp64", 1e-10) for s in systems]
+        for st in states:
+            eng.add(st)
+        while eng.active:
+            eng.step()
+        for s, st in zip(systems, states):
+            alone, _ = batch.solve_sequential([s], conv=1e-10)
+            r = st.result()
+            assert r.iterations < 200, (rep, s.info)
+            assert abs(r.E - alone[0].E) < 1e-8, (rep, s.info, r.E, alone[0].E)
