@@ -113,7 +113,9 @@ def parse():
     ap.add_argument("--systems", type=int, default=0,
                     help="systems in batch-atoms (default 128), the atoms sub-benchmark (default 64) or requests in "
                          "serve-diatomic (default 16)")
-    ap.add_argument("--rate", type=float, default=2.0, help="serve-diatomic: Poisson arrival rate (1/s)")
+    ap.add_argument("--rate", type=float, default=0.2,
+                    help="serve-diatomic: Poisson arrival rate (1/s); the default is about 0.8x the measured "
+                         "capacity of one GPU on this request mix (0.26 solves/s when saturated)")
     ap.add_argument("--only", action="store_true",
                     help="the headline workload alone (no configs sub-benchmarks, no SCF)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
