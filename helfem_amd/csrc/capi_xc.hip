@@ -1,6 +1,7 @@
 // C-ABI for the DFT grids (include/hfgpu.h): grid tables and the LDA
 // exchange-correlation matrix (kernels in xc_grid.hip).
 #include "capi_common.hpp"
+#include "xc_func.hpp"
 #include <cmath>
 #include <memory>
 
@@ -29,6 +30,17 @@ double theta_value(int l, int m, double x) {
   double y = std::sph_legendre((unsigned)l, (unsigned)ma, std::acos(x));
   if (m < 0 && (ma & 1)) y = -y;
   return y;
+}
+
+// dTheta_a/dtheta at x = cos(theta): m cot(theta) Theta_lm + sqrt((l - m)(l + m + 1))
+// Theta_l,m+1, the phi = 0 value of the reference's d/dtheta Y_lm
+// (TwoDBasis.cpp:1181-1199).
+double theta_deriv(int l, int m, double x) {
+  const double sinth = std::sqrt(std::max((1.0 - x) * (1.0 + x), 0.0));
+  const double cotth = sinth > 0.0 ? x / sinth : 0.0;
+  double v = m * cotth * theta_value(l, m, x);
+  if (m < l) v += std::sqrt(double((l - m) * (l + m + 1))) * theta_value(l, m + 1, x);
+  return v;
 }
 
 // Grid tables shared by both bases.  Atomic (dftgrid.cpp:470-576): R = B/r
@@ -156,6 +168,48 @@ void xc_upload(hf_dftgrid *g) {
   d.xd.upload(xd, s);
   d.thx.upload(thx, s);
   d.thxt.upload(thxt, s);
+  if (g->atomic) {
+    // gradient tables (GGA): d/dr R, 1/r, 1/sin(theta), sin(d phi), d/dtheta
+    // (Theta_a Theta_b), and the transposed pair of each row-list entry
+    std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), rinv((size_t)Nel * nrad), isin(g->lang);
+    std::vector<double> sn((size_t)nd * g->mang), thd((size_t)g->lang * np), thdt((size_t)np * g->lang);
+    for (int e = 0; e < Nel; ++e) {
+      const Blk v = rad.rderiv(xq, e);
+      const double h = rad.half_len(e), mid = rad.mid(e);
+      for (int i = 0; i < nrad; ++i) {
+        for (int c = 0; c < v.c; ++c) dR[((size_t)e * nrad + i) * TILE + c] = v(i, c);
+        rinv[(size_t)e * nrad + i] = 1.0 / (mid + h * xq[i]);
+      }
+    }
+    for (int t = 0; t < g->lang; ++t) isin[t] = 1.0 / std::sqrt((1.0 - xt[t]) * (1.0 + xt[t]));
+    for (int dd = 0; dd < nd; ++dd)
+      for (int k = 0; k < g->mang; ++k) sn[(size_t)dd * g->mang + k] = std::sin(double(dd - d.dmax) * (k * dphi));
+    std::vector<double> dth((size_t)g->lang * Na);
+    for (int t = 0; t < g->lang; ++t)
+      for (int a = 0; a < Na; ++a) dth[(size_t)t * Na + a] = theta_deriv(lv[a], mv[a], xt[t]);
+    for (int t = 0; t < g->lang; ++t)
+      for (int x = 0; x < np; ++x) {
+        const size_t A = (size_t)t * Na + da[x], B = (size_t)t * Na + db[x];
+        const double v = dth[A] * th[B] + th[A] * dth[B];
+        thd[(size_t)t * np + x] = v;
+        thdt[(size_t)x * g->lang + t] = v;
+      }
+    std::vector<int> xid((size_t)Na * Na, -1), rxT(rb.size());
+    for (int a = 0; a < Na; ++a)
+      for (int q = rstart[a]; q < rstart[a + 1]; ++q) xid[(size_t)a * Na + rb[q]] = rx[q];
+    for (int a = 0; a < Na; ++a)
+      for (int q = rstart[a]; q < rstart[a + 1]; ++q) {
+        rxT[q] = xid[(size_t)rb[q] * Na + a];
+        need(rxT[q] >= 0, "pair list is not symmetric");
+      }
+    d.dR.upload(dR, s);
+    d.rinv.upload(rinv, s);
+    d.isin.upload(isin, s);
+    d.sn.upload(sn, s);
+    d.thd.upload(thd, s);
+    d.thdt.upload(thdt, s);
+    d.rxT.upload(rxT, s);
+  }
   d.D.alloc((size_t)Nel * np * nrad);
   d.W.alloc((size_t)Nel * np * nrad);
   d.part.alloc((size_t)2 * Nel * nrad);
@@ -164,9 +218,16 @@ void xc_upload(hf_dftgrid *g) {
   g->uploaded = true;
 }
 
-void check_functionals(int x_func, int c_func) {
-  if (x_func != 0 && x_func != 1) throw Error(HF_EINVAL, "unsupported exchange functional (0 or 1 = lda_x)");
-  if (c_func != 0 && c_func != 7) throw Error(HF_EINVAL, "unsupported correlation functional (0 or 7 = lda_c_vwn)");
+// lda_x (1) with lda_c_vwn (7) everywhere; on atomic grids also PBE exchange
+// (101), PW92 (12, 13) and PBE (130) correlation, through the general point
+// kernel (xc_func.hpp)
+void check_functionals(const hf_dftgrid *g, int x_func, int c_func) {
+  if (!xcf::exchange_ok(x_func)) throw Error(HF_EINVAL, "unsupported exchange functional (0, 1 = lda_x, 101 = gga_x_pbe)");
+  if (!xcf::correlation_ok(c_func))
+    throw Error(HF_EINVAL, "unsupported correlation functional (0, 7 = lda_c_vwn, 12 = lda_c_pw, 13 = lda_c_pw_mod, "
+                           "130 = gga_c_pbe)");
+  if (xc_generic(x_func, c_func) && !g->atomic)
+    throw Error(HF_EINVAL, "the diatomic grids evaluate lda_x and lda_c_vwn only; GGA and PW92 run on atomic grids");
 }
 
 // Pb == nullptr: restricted; else spin-polarized (Hb gets the beta potential,
@@ -175,10 +236,21 @@ void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ld
               double *H, int64_t ldh, double *Hb, int64_t ldhb, bool beta, double thr, double *out) {
   hipStream_t s = g->ctx->cur;
   XCGridDevice &d = g->dev;
+  const bool gen = xc_generic(x_func, c_func);
+  if (gen) {
+    if (xc_point_lds(d, x_func, c_func, Pb != nullptr) > 160 * 1024)
+      throw Error(HF_EINVAL, "angular basis and DFT grid too large for one workgroup's LDS in the GGA evaluation");
+    d.E.alloc(d.D.n);
+    d.Y.alloc(d.W.n);
+  }
   if (Pb) {
-    if (!d.lds_pol_ok) throw Error(HF_EINVAL, "DFT grid too large for the spin-polarized evaluation");
+    if (!gen && !d.lds_pol_ok) throw Error(HF_EINVAL, "DFT grid too large for the spin-polarized evaluation");
     d.Db.alloc(d.D.n);
     d.Wb.alloc(d.W.n);
+    if (gen) {
+      d.Eb.alloc(d.D.n);
+      d.Yb.alloc(d.W.n);
+    }
   }
   xc_grid_launch(d, x_func, c_func, thr, P, ldp, Pb, ldpb, H, ldh, Hb, ldhb, s, &g->ctx->timer);
   if (Pb && !beta) fill_zero_launch(Hb, d.Nbf, d.Nbf, ldhb, s);
@@ -286,7 +358,7 @@ int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double 
   return guarded([&] {
     ready(g);
     need(out != nullptr, "out is NULL");
-    check_functionals(x_func, c_func);
+    check_functionals(g, x_func, c_func);
     DeviceGuard dg(g->ctx->device);
     const int64_t n = g->Nbf();
     check_matrix(dP, ldp, n, "P");
@@ -301,7 +373,7 @@ int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, 
   return guarded([&] {
     ready(g);
     need(out != nullptr, "out is NULL");
-    check_functionals(x_func, c_func);
+    check_functionals(g, x_func, c_func);
     DeviceGuard dg(g->ctx->device);
     if (!g->uploaded) xc_upload(g);
     host_roundtrip(g->ctx->cur, g->Nbf(), g->dev.hostP, g->dev.hostOut, P, ldp, H, ldh,
@@ -317,7 +389,7 @@ int hf_dftgrid_eval_fxc_pol_dev(hf_dftgrid *g, int x_func, int c_func, const dou
   return guarded([&] {
     ready(g);
     need(out != nullptr, "out is NULL");
-    check_functionals(x_func, c_func);
+    check_functionals(g, x_func, c_func);
     DeviceGuard dg(g->ctx->device);
     const int64_t n = g->Nbf();
     check_matrix(dPa, ldpa, n, "Pa");
@@ -335,7 +407,7 @@ int hf_dftgrid_eval_fxc_pol(hf_dftgrid *g, int x_func, int c_func, const double 
   return guarded([&] {
     ready(g);
     need(out != nullptr, "out is NULL");
-    check_functionals(x_func, c_func);
+    check_functionals(g, x_func, c_func);
     DeviceGuard dg(g->ctx->device);
     const int64_t n = g->Nbf();
     check_matrix(Pa, ldpa, n, "Pa");

@@ -345,6 +345,16 @@ struct XCGridDevice {
   DevArray<double> thx, thxt;         // Theta_a(t) Theta_b(t): [lang][npair] and [npair][lang]
   DevArray<double> D, W;     // D [Nel][nrad][npair], W [Nel][npair][nrad]
   DevArray<double> Db, Wb;   // second spin of a polarized evaluation
+  // gradient (GGA) tables and work arrays, built on the first GGA evaluation
+  bool gga_ready = false;
+  DevArray<double> dR;       // [Nel][nrad][TILE] d/dr of R
+  DevArray<double> rinv;     // [Nel][nrad] 1/r
+  DevArray<double> isin;     // [lang] 1/sin(theta_t)
+  DevArray<double> sn;       // [2 dmax + 1][mang] sin(d phi_k)
+  DevArray<double> thd, thdt;  // d/dtheta (Theta_a Theta_b): [lang][npair] and [npair][lang]
+  DevArray<int> rxT;         // per row-list entry: pair index of the transposed pair (b, a)
+  DevArray<double> E, Y;     // E [Nel][nrad][npair] (d/dr on the row), Y [Nel][npair][nrad]
+  DevArray<double> Eb, Yb;
   DevArray<double> part;     // [Nel*nrad][2]: Exc and Nel partial sums
   PinnedArray<double> h_part;
   DevArray<double> hostP, hostOut;
@@ -434,6 +444,11 @@ void atomic_batch_exchange(AtomicBatch &b, const double *const *P, const int64_t
 void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *Pa, int64_t ldpa, const double *Pb,
                     int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
                     KernelTimer *timer);
+// true when (xf, cf) is not the plain lda_x + lda_c_vwn pair: the general
+// point kernel (GGA terms, any supported functional) runs, on atomic grids
+bool xc_generic(int xf, int cf);
+// dynamic LDS of the point kernel that xc_grid_launch will use
+size_t xc_point_lds(const XCGridDevice &g, int xf, int cf, bool pol);
 
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,

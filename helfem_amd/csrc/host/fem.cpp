@@ -231,11 +231,13 @@ Blk LipShape::eval(const std::vector<double> &x, int deriv, double h) const {
   return out;
 }
 
-Blk LipShape::eval_over_r(const std::vector<double> &x, double h) const {
+Blk LipShape::eval_over_r(const std::vector<double> &x, double h, int deriv) const {
   if (lo < 1) throw std::logic_error("B/r needs the r=0 function dropped");
+  // B_k(x) = (x + 1)/(x_k + 1) L_k(x) with L_k the Lagrange polynomial on the
+  // nodes without x = -1, and r = h (x + 1) on an element that starts at 0
   const std::vector<double> red(nodes.begin() + 1, nodes.end());
-  const Blk p = lagrange_prim(x, red, 0);
-  const double scale = 1.0 / std::pow(h, 1);
+  const Blk p = lagrange_prim(x, red, deriv);
+  const double scale = 1.0 / std::pow(h, deriv + 1);
   Blk out((int)x.size(), nbf());
   for (int k = 0; k < nbf(); ++k) {
     const int full = lo + k;
@@ -274,6 +276,20 @@ Blk RadialFE::rvalues(const std::vector<double> &x, int iel) const {
   const double m = mid(iel), h = half_len(iel);
   for (int f = 0; f < v.c; ++f)
     for (int i = 0; i < v.r; ++i) v(i, f) /= (m + h * x[i]);
+  return v;
+}
+
+Blk RadialFE::rderiv(const std::vector<double> &x, int iel) const {
+  const LipShape s = shape(iel);
+  const double h = half_len(iel), m = mid(iel);
+  if (iel == 0 && drop_left) return s.eval_over_r(x, h, 1);
+  const Blk f = s.eval(x, 0, h), d = s.eval(x, 1, h);
+  Blk v(f.r, f.c);
+  for (int c = 0; c < f.c; ++c)
+    for (int i = 0; i < f.r; ++i) {
+      const double ir = 1.0 / (m + h * x[i]);
+      v(i, c) = (d(i, c) - f(i, c) * ir) * ir;
+    }
   return v;
 }
 

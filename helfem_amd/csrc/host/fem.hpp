@@ -47,8 +47,9 @@ struct LipShape {
   int nbf() const { return hi - lo; }
   // values (deriv 0/1/2) at x, scaled by 1/h^deriv; (npts x nbf)
   Blk eval(const std::vector<double> &x, int deriv, double h) const;
-  // B(x)/r on the first element, where B(-1) = 0 (LIPBasis.h:83-111)
-  Blk eval_over_r(const std::vector<double> &x, double h) const;
+  // d^deriv/dr^deriv of B(x)/r on the first element, where B(-1) = 0
+  // (LIPBasis.h:83-111)
+  Blk eval_over_r(const std::vector<double> &x, double h, int deriv = 0) const;
 };
 Blk lagrange_prim(const std::vector<double> &x, const std::vector<double> &nodes, int deriv);
 
@@ -84,6 +85,8 @@ struct RadialFE {
   double mid(int iel) const { return 0.5 * (bval[iel] + bval[iel + 1]); }
   // radial function values R = B/r at element coordinates x
   Blk rvalues(const std::vector<double> &x, int iel) const;
+  // their r derivatives d/dr (B/r) (RadialBasis.cpp:890-905)
+  Blk rderiv(const std::vector<double> &x, int iel) const;
   // kind 0: B, 1: dB/dr, 10: R = B/r
   Blk values(int kind, const std::vector<double> &x, int iel) const;
   // auto-converged <lkind| f |rkind> over element iel

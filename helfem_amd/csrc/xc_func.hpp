@@ -1,0 +1,213 @@
+// Exchange-correlation functionals by libxc id, for the device (and host):
+//   1 lda_x, 7 lda_c_vwn (VWN5), 12 lda_c_pw (PW92 as printed), 13 lda_c_pw_mod
+//   (libxc's more precise PW92 constants), 101 gga_x_pbe, 130 gga_c_pbe
+// from the published formulas (Slater; Vosko-Wilk-Nusair, Can. J. Phys. 58,
+// 1200 (1980); Perdew-Wang, PRB 45, 13244 (1992); Perdew-Burke-Ernzerhof,
+// PRL 77, 3865 (1996)) with libxc's constants and thresholds: no output below
+// the density threshold, 1 +- zeta floored at DBL_EPSILON, sigma_aa and
+// sigma_bb floored at 1e-40, sigma_ab clamped to +-(sigma_aa + sigma_bb)/2; a
+// polarized spin below the threshold adds no PBE exchange.
+//
+// Each functional is one energy density per volume f(rho_a, rho_b, sigma_aa,
+// sigma_ab, sigma_bb) written once over a scalar type; the potentials come
+// from forward-mode differentiation (a value and one directional derivative,
+// one pass per input), so the derivative formulas are never written by hand.
+#pragma once
+#include <cfloat>
+#include <cmath>
+
+namespace hfg {
+namespace xcf {
+
+// value and derivative along one input
+struct Dl {
+  double v, d;
+};
+__host__ __device__ inline Dl mk(double v, double d = 0.0) { return Dl{v, d}; }
+__host__ __device__ inline Dl operator+(Dl a, Dl b) { return {a.v + b.v, a.d + b.d}; }
+__host__ __device__ inline Dl operator-(Dl a, Dl b) { return {a.v - b.v, a.d - b.d}; }
+__host__ __device__ inline Dl operator-(Dl a) { return {-a.v, -a.d}; }
+__host__ __device__ inline Dl operator*(Dl a, Dl b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__host__ __device__ inline Dl operator/(Dl a, Dl b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__host__ __device__ inline Dl operator+(Dl a, double b) { return {a.v + b, a.d}; }
+__host__ __device__ inline Dl operator+(double a, Dl b) { return {a + b.v, b.d}; }
+__host__ __device__ inline Dl operator-(Dl a, double b) { return {a.v - b, a.d}; }
+__host__ __device__ inline Dl operator-(double a, Dl b) { return {a - b.v, -b.d}; }
+__host__ __device__ inline Dl operator*(Dl a, double b) { return {a.v * b, a.d * b}; }
+__host__ __device__ inline Dl operator*(double a, Dl b) { return {a * b.v, a * b.d}; }
+__host__ __device__ inline Dl operator/(Dl a, double b) { return {a.v / b, a.d / b}; }
+__host__ __device__ inline Dl operator/(double a, Dl b) {
+  const double q = a / b.v;
+  return {q, -q * b.d / b.v};
+}
+__host__ __device__ inline Dl sqrt(Dl a) {
+  const double s = ::sqrt(a.v);
+  return {s, 0.5 * a.d / s};
+}
+__host__ __device__ inline Dl cbrt(Dl a) {
+  const double c = ::cbrt(a.v);
+  return {c, a.d / (3.0 * c * c)};
+}
+__host__ __device__ inline Dl log(Dl a) { return {::log(a.v), a.d / a.v}; }
+__host__ __device__ inline Dl exp(Dl a) {
+  const double e = ::exp(a.v);
+  return {e, e * a.d};
+}
+__host__ __device__ inline Dl atan(Dl a) { return {::atan(a.v), a.d / (1.0 + a.v * a.v)}; }
+__host__ __device__ inline double val(Dl a) { return a.v; }
+__host__ __device__ inline double val(double a) { return a; }
+
+template <class T> __host__ __device__ inline T floor_at(T x, double lo) { return val(x) < lo ? T(x * 0.0 + lo) : x; }
+__host__ __device__ inline Dl floor_at(Dl x, double lo) { return x.v < lo ? mk(lo) : x; }
+
+constexpr double kPi = 3.14159265358979323846;
+
+// x^(4/3) and x^(2/3)
+template <class T> __host__ __device__ inline T p43(T x) { return x * cbrt(x); }
+template <class T> __host__ __device__ inline T p23(T x) {
+  const T c = cbrt(x);
+  return c * c;
+}
+
+// VWN interpolation formula in x = sqrt(rs)
+template <class T> __host__ __device__ inline T vwn_F(double A, double b, double c, double x0, T x) {
+  const double Q = ::sqrt(4.0 * c - b * b), X0 = x0 * x0 + b * x0 + c;
+  const double f1 = 2.0 * b / Q, f2 = b * x0 / X0, f3 = 2.0 * (2.0 * x0 + b) / Q;
+  const T X = x * x + b * x + c;
+  const T xm = x - x0;
+  return A * (log(x * x / X) + (f1 - f2 * f3) * atan(Q / (2.0 * x + b)) - f2 * log(xm * xm / X));
+}
+
+// spin interpolation f(zeta) from 1 + zeta and 1 - zeta
+template <class T> __host__ __device__ inline T fzeta(T opz, T omz) {
+  return (p43(opz) + p43(omz) - 2.0) / (2.0 * ::cbrt(2.0) - 2.0);
+}
+
+// PW92: G(rs) per spin channel k (paramagnetic, ferromagnetic, -spin stiffness)
+template <class T> __host__ __device__ inline T pw92_G(int k, bool mod, T rs, T srs) {
+  const double A = mod ? (k == 0 ? 0.0310907 : k == 1 ? 0.01554535 : 0.0168869)
+                       : (k == 0 ? 0.031091 : k == 1 ? 0.015545 : 0.016887);
+  const double a1 = k == 0 ? 0.21370 : k == 1 ? 0.20548 : 0.11125;
+  const double b1 = k == 0 ? 7.5957 : k == 1 ? 14.1189 : 10.357;
+  const double b2 = k == 0 ? 3.5876 : k == 1 ? 6.1977 : 3.6231;
+  const double b3 = k == 0 ? 1.6382 : k == 1 ? 3.3662 : 0.88026;
+  const double b4 = k == 0 ? 0.49294 : k == 1 ? 0.62517 : 0.49671;
+  const T den = srs * (b1 + b3 * rs) + rs * (b2 + b4 * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * log(1.0 + 1.0 / (2.0 * A * den));
+}
+
+// PW92 correlation energy per particle
+template <class T> __host__ __device__ inline T pw92(bool mod, T rs, T z, T opz, T omz) {
+  const double fz20 = mod ? 1.709920934161365617563962776245 : 1.709921;
+  const T srs = sqrt(rs);
+  const T g0 = pw92_G(0, mod, rs, srs), g1 = pw92_G(1, mod, rs, srs), g2 = pw92_G(2, mod, rs, srs);
+  const T fz = fzeta(opz, omz), z2 = z * z;
+  return g0 + z2 * z2 * fz * (g1 - g0 + g2 / fz20) - fz * g2 / fz20;
+}
+
+// PBE exchange energy per volume of an unpolarized density n, |grad n|^2 = s2n
+template <class T> __host__ __device__ inline T pbe_x(T n, T s2n) {
+  const double kappa = 0.8040, mu = 0.06672455060314922 * kPi * kPi / 3.0;
+  const double c2 = 4.0 * ::cbrt(9.0 * kPi * kPi * kPi * kPi);  // 4 (3 pi^2)^(2/3)
+  const T n43 = p43(n);
+  const T s2 = s2n / (c2 * n43 * n43);
+  return -0.75 * ::cbrt(3.0 / kPi) * n43 * (1.0 + kappa - kappa / (1.0 + (mu / kappa) * s2));
+}
+
+// energy per volume of functional `id` (0: none)
+template <class T>
+__host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
+  if (id == 0) return ra * 0.0;
+  if (id == 1) {
+    T e = ra * 0.0;
+    if (val(ra) > 0.0) e = e + p43(ra);
+    if (val(rb) > 0.0) e = e + p43(rb);
+    return -0.75 * ::cbrt(6.0 / kPi) * e;
+  }
+  if (id == 101) {
+    T e = ra * 0.0;
+    if (!pol || val(ra) >= thr) e = e + 0.5 * pbe_x(2.0 * ra, 4.0 * floor_at(saa, 1e-40));
+    if (!pol || val(rb) >= thr) e = e + 0.5 * pbe_x(2.0 * rb, 4.0 * floor_at(sbb, 1e-40));
+    return e;
+  }
+  const T rho = ra + rb;
+  const T z = (ra - rb) / rho;
+  const T opz = floor_at(1.0 + z, DBL_EPSILON), omz = floor_at(1.0 - z, DBL_EPSILON);
+  const T rs = cbrt(3.0 / (4.0 * kPi * rho));
+  if (id == 7) {
+    const T x = sqrt(rs);
+    const T eP = vwn_F(0.0310907, 3.72744, 12.9352, -0.10498, x);
+    const T eF = vwn_F(0.01554535, 7.06042, 18.0578, -0.32500, x);
+    const T eA = vwn_F(-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584, x);
+    const double fpp = 4.0 / (9.0 * (::cbrt(2.0) - 1.0));
+    const T fz = fzeta(opz, omz), z2 = z * z, z4 = z2 * z2;
+    return rho * (eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4);
+  }
+  if (id == 12 || id == 13) return rho * pw92(id == 13, rs, z, opz, omz);
+  // 130: PBE correlation
+  const double beta = 0.06672455060314922, gamma = (1.0 - 0.69314718055994530942) / (kPi * kPi);
+  saa = floor_at(saa, 1e-40);
+  sbb = floor_at(sbb, 1e-40);
+  const T sav = 0.5 * (saa + sbb);
+  if (val(sab) > val(sav)) sab = sav;
+  if (val(sab) < -val(sav)) sab = -sav;
+  const T ec = pw92(true, rs, z, opz, omz);
+  const T phi = 0.5 * (p23(opz) + p23(omz)), phi3 = phi * phi * phi;
+  const T ks2 = (4.0 / kPi) * cbrt(3.0 * kPi * kPi * rho);
+  const T t2 = (saa + 2.0 * sab + sbb) / (4.0 * phi * phi * ks2 * rho * rho);
+  const T A = (beta / gamma) / (exp(-ec / (gamma * phi3)) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * phi3 * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+__host__ __device__ inline bool is_gga(int id) { return id == 101 || id == 130; }
+__host__ __device__ inline bool exchange_ok(int id) { return id == 0 || id == 1 || id == 101; }
+__host__ __device__ inline bool correlation_ok(int id) {
+  return id == 0 || id == 7 || id == 12 || id == 13 || id == 130;
+}
+
+template <class T>
+__host__ __device__ inline T total(int xf, int cf, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
+  return energy(xf, ra, rb, saa, sab, sbb, thr, pol) + energy(cf, ra, rb, saa, sab, sbb, thr, pol);
+}
+
+// Unpolarized: rho, sigma = |grad rho|^2 -> energy per volume f, vrho, vsigma.
+__host__ __device__ inline void eval_unpol(int xf, int cf, double rho, double sigma, double thr, bool gga, double &f,
+                                           double &vrho, double &vsigma) {
+  f = vrho = vsigma = 0.0;
+  if (!(rho >= thr)) return;
+  Dl r = mk(rho, 1.0), s = mk(sigma);
+  Dl e = total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, thr, false);
+  f = e.v;
+  vrho = e.d;
+  if (gga) {
+    r = mk(rho);
+    s = mk(sigma, 1.0);
+    e = total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, thr, false);
+    vsigma = e.d;
+  }
+}
+
+// Polarized: x = (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb) -> f and
+// v[k] = df/dx_k (the sigma derivatives only with gga).
+__host__ __device__ inline void eval_pol(int xf, int cf, const double x[5], double thr, bool gga, double &f,
+                                         double v[5]) {
+  f = 0.0;
+  for (int k = 0; k < 5; ++k) v[k] = 0.0;
+  if (!(x[0] + x[1] >= thr)) return;
+  const int nk = gga ? 5 : 2;
+  for (int k = 0; k < nk; ++k) {
+    Dl y[5];
+    for (int j = 0; j < 5; ++j) y[j] = mk(x[j], j == k ? 1.0 : 0.0);
+    const Dl e = total(xf, cf, y[0], y[1], y[2], y[3], y[4], thr, true);
+    f = e.v;
+    v[k] = e.d;
+  }
+}
+
+}  // namespace xcf
+}  // namespace hfg

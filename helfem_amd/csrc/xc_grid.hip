@@ -17,8 +17,26 @@
 // which is the reference's rho = sum_uv P_uv Re(chi_u chi_v*) and
 // H += (a .* w v) a' + (b .* w v) b' (dftgrid.cpp:51-128, 304-359;
 // dftgrid_purem.cpp:212-260, 468-500) with the sums reordered.
+//
+// GGA (atomic grid; dftgrid.cpp:75-86, 166-180, 320-334, 382-405,
+// dftgrid_common.h:188-216).  The gradient in spherical components,
+// g = 2 Re sum P (d chi) chi* / (1, r, r sin theta), factorises the same way:
+//   g_r     = 2 sum_d G_d cos(d phi),  G_d(t) = sum_{x in d} Theta_a Theta_b E(x),
+//             E(x) = sum_rs R'_r P_(a,r),(b,s) R_s (k_xc_rad_in alongside D)
+//   g_theta = (1/r) sum_d T_d cos(d phi),  T_d(t) = sum_{x in d} (Theta_a Theta_b)' D(x)
+//   g_phi   = -(1/(r sin theta)) sum_d d C_d sin(d phi)
+// (the last two use D(a,b) = D(b,a), P symmetric, as the reference's factor
+// 2 does).  With U = 2 w vsigma grad rho (restricted) the potential term
+// H += X + X', X = sum U . Re(grad chi_u) chi_v*, becomes per pair x = (a,b)
+//   H_(a,r),(b,s) += sum_i R_r R_s W2(x) + R'_r R_s Y(x) + R_r R'_s Y(x')
+//   W2(x) = sum_t Theta_a Theta_b [V_d - d/(r sin theta) Vphi_d] + (1/r) (Theta_a Theta_b)' Vtheta_d
+//   Y(x)  = sum_t Theta_a Theta_b Vr_d,   x' = (b, a)
+// with V_d, Vr_d, Vtheta_d = sum_k (w vrho, U_r, U_theta) cos(d phi_k) and
+// Vphi_d = sum_k U_phi sin(d phi_k).
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
+#include "xc_func.hpp"
+#include <algorithm>
 
 namespace hfg {
 namespace {
@@ -122,6 +140,10 @@ struct XCArgs {
   const double *R, *wpt, *thx, *thxt, *cs;
   double *D, *W, *part;
   double *Db, *Wb;  // second spin (polarized evaluation), else null
+  // gradient terms (k_xc_rad_in<true>, k_xc_point_gga, k_xc_rad_out<true>)
+  const double *dR, *rinv, *isin, *sn, *thd, *thdt;
+  const int *rxT;
+  double *E, *Y, *Eb, *Yb;
 };
 
 // One workgroup per (element, shell a).  The pairs (a, b) of row a go in
@@ -135,16 +157,20 @@ constexpr int XC_RS = TILE + 1;
 
 __host__ __device__ inline int xc_pairs_per_group(int nrad) { return min(8, max(1, NT / nrad)); }
 
+template <bool GGA>
 __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
   const int e = blockIdx.x / A.Nang, a = blockIdx.x - e * A.Nang;
   const int t = threadIdx.x;
   const int ni = A.ni[e], f0 = A.first[e];
   const int J = xc_pairs_per_group(A.nrad);
   extern __shared__ double sm[];
-  double *sR = sm;                       // [nrad][XC_RS]
-  double *sP = sm + A.nrad * XC_RS;      // [J][TILE2]
-  for (int q = t; q < A.nrad * TILE; q += NT)
+  double *sR = sm;                                    // [nrad][XC_RS]
+  double *sdR = sm + A.nrad * XC_RS;                  // [nrad][XC_RS] with GGA
+  double *sP = sm + (GGA ? 2 : 1) * A.nrad * XC_RS;   // [J][TILE2]
+  for (int q = t; q < A.nrad * TILE; q += NT) {
     sR[(q >> 4) * XC_RS + (q & 15)] = A.R[(size_t)e * A.nrad * TILE + q];
+    if (GGA) sdR[(q >> 4) * XC_RS + (q & 15)] = A.dR[(size_t)e * A.nrad * TILE + q];
+  }
   const int q0 = A.rstart[a], q1 = A.rstart[a + 1];
   for (int qb = q0; qb < q1; qb += J) {
     __syncthreads();
@@ -162,19 +188,22 @@ __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
       const int j = item / A.nrad, i = item - j * A.nrad, q = qb + j;
       if (q >= q1) continue;
       const double *Pt = sP + j * TILE2;
-      const double *Ri = sR + i * XC_RS;
+      const double *Ri = sR + i * XC_RS, *dRi = sdR + i * XC_RS;
       double ri[TILE];
 #pragma unroll
       for (int s = 0; s < TILE; ++s) ri[s] = Ri[s];
-      double d = 0.0;
+      double d = 0.0, eg = 0.0;
 #pragma unroll
       for (int r = 0; r < TILE; ++r) {
         double tr = 0.0;
 #pragma unroll
         for (int s = 0; s < TILE; ++s) tr += Pt[r + TILE * s] * ri[s];
         d += ri[r] * tr;
+        if (GGA) eg += dRi[r] * tr;
       }
-      A.D[((size_t)e * A.nrad + i) * A.npair + A.rx[q]] = d;
+      const size_t o = ((size_t)e * A.nrad + i) * A.npair + A.rx[q];
+      A.D[o] = d;
+      if (GGA) A.E[o] = eg;
     }
   }
 }
@@ -257,20 +286,167 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   }
 }
 
+// GGA point kernel: one workgroup per (element, radial point), both spins
+// when polarized.  LDS: region A holds D and E of each spin, then the point
+// values (w vrho, U_r, U_theta, U_phi); region B holds the class sums (C, G,
+// T), then the class transforms (V_d - d/(r sin) Vphi_d, Vr_d, Vtheta_d).
+// Any supported functional pair runs here (vsigma = 0 without a GGA part).
+size_t xc_gga_point_lds(int np, int nd, int lang, int mang, int ns) {
+  const size_t na = (size_t)std::max(2 * np, 4 * lang * mang);
+  return sizeof(double) * ((size_t)ns * na + (size_t)ns * 3 * nd * lang);
+}
+
+__global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
+  const int e = blockIdx.x / A.nrad, i = blockIdx.x - e * A.nrad;
+  const int t = threadIdx.x;
+  const int np = A.npair, nd = A.nd, lang = A.lang, mang = A.mang, nang = lang * mang, dmax = (nd - 1) / 2;
+  const bool pol = A.Db != nullptr;
+  const int ns = pol ? 2 : 1;
+  const bool gga = xcf::is_gga(A.xf) || xcf::is_gga(A.cf);
+  const int na = max(2 * np, 4 * nang), ncl = nd * lang;
+  extern __shared__ double sm[];
+  double *sA = sm;            // [ns][na]
+  double *sB = sm + ns * na;  // [ns][3][nd][lang]
+  __shared__ double red[8];
+  const size_t row = ((size_t)e * A.nrad + i) * np;
+  for (int x = t; x < np; x += NT) {
+    sA[x] = A.D[row + x];
+    sA[np + x] = A.E[row + x];
+    if (pol) {
+      sA[na + x] = A.Db[row + x];
+      sA[na + np + x] = A.Eb[row + x];
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < ns * 3 * ncl; q += NT) {
+    const int sp = q / (3 * ncl), r1 = q - sp * 3 * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
+    const int d = r2 / lang, tt = r2 - d * lang;
+    const double *src = sA + sp * na + (c == 1 ? np : 0);
+    const double *tab = c == 2 ? A.thdt : A.thxt;
+    double acc = 0.0;
+    for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) acc += tab[(size_t)x * lang + tt] * src[x];
+    sB[q] = acc;
+  }
+  __syncthreads();
+  const double ri = A.rinv[(size_t)e * A.nrad + i];
+  const double *wp = A.wpt + ((size_t)e * A.nrad + i) * lang;
+  double ex = 0.0, ne = 0.0;
+  for (int q = t; q < nang; q += NT) {
+    const int tt = q / mang, k = q - tt * mang;
+    const double ris = ri * A.isin[tt];
+    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int sp = 0; sp < ns; ++sp) {
+      const double *C = sB + sp * 3 * ncl, *G = C + ncl, *T = G + ncl;
+      double r0 = 0.0, gr = 0.0, gt = 0.0, gp = 0.0;
+      for (int d = 0; d < nd; ++d) {
+        const double c = A.cs[(size_t)d * mang + k], cd = C[d * lang + tt];
+        r0 += cd * c;
+        gr += G[d * lang + tt] * c;
+        gt += T[d * lang + tt] * c;
+        gp += (double)(d - dmax) * cd * A.sn[(size_t)d * mang + k];
+      }
+      rho[sp] = r0;
+      g[sp][0] = 2.0 * gr;
+      g[sp][1] = ri * gt;
+      g[sp][2] = -ris * gp;
+    }
+    const double w = wp[tt];
+    double f, va, vb = 0.0, u[2][3];
+    if (pol) {
+      const double x5[5] = {rho[0], rho[1], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
+                            g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2],
+                            g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2]};
+      double v[5];
+      xcf::eval_pol(A.xf, A.cf, x5, A.thr, gga, f, v);
+      va = v[0];
+      vb = v[1];
+      for (int c = 0; c < 3; ++c) {
+        u[0][c] = w * (2.0 * v[2] * g[0][c] + v[3] * g[1][c]);
+        u[1][c] = w * (2.0 * v[4] * g[1][c] + v[3] * g[0][c]);
+      }
+    } else {
+      double vs;
+      xcf::eval_unpol(A.xf, A.cf, rho[0], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2], A.thr, gga,
+                      f, va, vs);
+      for (int c = 0; c < 3; ++c) u[0][c] = 2.0 * w * vs * g[0][c];
+    }
+    ex += w * f;
+    ne += w * (rho[0] + rho[1]);
+    sA[q] = w * va;
+    sA[nang + q] = u[0][0];
+    sA[2 * nang + q] = u[0][1];
+    sA[3 * nang + q] = u[0][2];
+    if (pol) {
+      sA[na + q] = w * vb;
+      sA[na + nang + q] = u[1][0];
+      sA[na + 2 * nang + q] = u[1][1];
+      sA[na + 3 * nang + q] = u[1][2];
+    }
+  }
+  block_sum2(ex, ne, red);
+  if (t == 0) {
+    A.part[2 * ((size_t)e * A.nrad + i)] = ex;
+    A.part[2 * ((size_t)e * A.nrad + i) + 1] = ne;
+  }
+  __syncthreads();
+  for (int q = t; q < ns * 3 * ncl; q += NT) {
+    const int sp = q / (3 * ncl), r1 = q - sp * 3 * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
+    const int d = r2 / lang, tt = r2 - d * lang;
+    const double *V = sA + sp * na + tt * mang;
+    const double *cs = A.cs + (size_t)d * mang, *snd = A.sn + (size_t)d * mang;
+    double acc = 0.0;
+    if (c == 0) {
+      const double fphi = (double)(d - dmax) * ri * A.isin[tt];
+      const double *Vp = V + 3 * nang;
+      for (int k = 0; k < mang; ++k) acc += V[k] * cs[k] - fphi * Vp[k] * snd[k];
+    } else {
+      const double *Vc = V + (c == 1 ? 1 : 2) * nang;
+      for (int k = 0; k < mang; ++k) acc += Vc[k] * cs[k];
+    }
+    sB[q] = acc;
+  }
+  __syncthreads();
+  for (int sp = 0; sp < ns; ++sp) {
+    double *Wout = sp == 0 ? A.W : A.Wb, *Yout = sp == 0 ? A.Y : A.Yb;
+    const double *Q = sB + sp * 3 * ncl, *Vr = Q + ncl, *Vt = Vr + ncl;
+    for (int x = t; x < np; x += NT) {
+      const int d = A.xd[x];
+      const double *Qd = Q + d * lang, *Vrd = Vr + d * lang, *Vtd = Vt + d * lang;
+      double w2 = 0.0, wt = 0.0, y = 0.0;
+      for (int tt = 0; tt < lang; ++tt) {
+        const double th = A.thx[(size_t)tt * np + x];
+        w2 += th * Qd[tt];
+        y += th * Vrd[tt];
+        wt += A.thd[(size_t)tt * np + x] * Vtd[tt];
+      }
+      Wout[((size_t)e * np + x) * A.nrad + i] = w2 + ri * wt;
+      Yout[((size_t)e * np + x) * A.nrad + i] = y;
+    }
+  }
+}
+
 // One workgroup per (element, shell a); thread (r,s) of the element band,
 // looping over the pairs (a, b) of row a.  The corner shared with element
 // e+1 is written by e with both contributions; e > 0 skips its own (0,0).
+template <bool GGA>
 __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
   const int e = blockIdx.x / A.Nang, a = blockIdx.x - e * A.Nang;
   const int t = threadIdx.x, r = t & 15, s = t >> 4;
   const int ni = A.ni[e], f0 = A.first[e];
   extern __shared__ double sm[];
-  double *sR = sm;  // [nrad][TILE] of e, then column 0 of e+1
+  double *sR = sm;  // [nrad][TILE] of e, then column 0 of e+1; with GGA the same of d/dr R
   double *sN = sm + A.nrad * TILE;
-  for (int q = t; q < A.nrad * TILE; q += NT) sR[q] = A.R[(size_t)e * A.nrad * TILE + q];
+  double *sdR = sN + A.nrad, *sdN = sdR + A.nrad * TILE;
+  for (int q = t; q < A.nrad * TILE; q += NT) {
+    sR[q] = A.R[(size_t)e * A.nrad * TILE + q];
+    if (GGA) sdR[q] = A.dR[(size_t)e * A.nrad * TILE + q];
+  }
   const bool has_next = e + 1 < A.Nel;
   if (has_next)
-    for (int q = t; q < A.nrad; q += NT) sN[q] = A.R[((size_t)(e + 1) * A.nrad + q) * TILE];
+    for (int q = t; q < A.nrad; q += NT) {
+      sN[q] = A.R[((size_t)(e + 1) * A.nrad + q) * TILE];
+      if (GGA) sdN[q] = A.dR[((size_t)(e + 1) * A.nrad + q) * TILE];
+    }
   __syncthreads();
   if (r >= ni || s >= ni) return;
   if (e > 0 && r == 0 && s == 0) return;
@@ -284,10 +460,24 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
     if (col < 0) continue;
     const double *Wp = A.W + ((size_t)e * np + x) * A.nrad;
     double v = 0.0;
-    for (int i = 0; i < A.nrad; ++i) v += sR[i * TILE + r] * sR[i * TILE + s] * Wp[i];
-    if (corner) {
-      const double *Wn = A.W + ((size_t)(e + 1) * np + x) * A.nrad;
-      for (int i = 0; i < A.nrad; ++i) v += sN[i] * sN[i] * Wn[i];
+    if (GGA) {
+      const int xT = A.rxT[q];
+      const double *Yp = A.Y + ((size_t)e * np + x) * A.nrad, *YT = A.Y + ((size_t)e * np + xT) * A.nrad;
+      for (int i = 0; i < A.nrad; ++i) {
+        const double Rr = sR[i * TILE + r], Rs = sR[i * TILE + s];
+        v += Rr * Rs * Wp[i] + sdR[i * TILE + r] * Rs * Yp[i] + Rr * sdR[i * TILE + s] * YT[i];
+      }
+      if (corner) {
+        const double *Wn = A.W + ((size_t)(e + 1) * np + x) * A.nrad;
+        const double *Yn = A.Y + ((size_t)(e + 1) * np + x) * A.nrad, *YTn = A.Y + ((size_t)(e + 1) * np + xT) * A.nrad;
+        for (int i = 0; i < A.nrad; ++i) v += sN[i] * (sN[i] * Wn[i] + sdN[i] * (Yn[i] + YTn[i]));
+      }
+    } else {
+      for (int i = 0; i < A.nrad; ++i) v += sR[i * TILE + r] * sR[i * TILE + s] * Wp[i];
+      if (corner) {
+        const double *Wn = A.W + ((size_t)(e + 1) * np + x) * A.nrad;
+        for (int i = 0; i < A.nrad; ++i) v += sN[i] * sN[i] * Wn[i];
+      }
     }
     A.H[row + col * A.ldh] = v;
   }
@@ -295,17 +485,27 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
 
 }  // namespace
 
+bool xc_generic(int xf, int cf) { return !((xf == 0 || xf == 1) && (cf == 0 || cf == 7)); }
+
+size_t xc_point_lds(const XCGridDevice &g, int xf, int cf, bool pol) {
+  const int ns = pol ? 2 : 1, nd = 2 * g.dmax + 1;
+  if (xc_generic(xf, cf)) return xc_gga_point_lds(g.npair, nd, g.lang, g.mang, ns);
+  return sizeof(double) * ns * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
+}
+
 void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *Pa, int64_t ldpa, const double *Pb,
                     int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
                     KernelTimer *timer) {
   const bool pol = Pb != nullptr;
+  const bool gen = xc_generic(xf, cf);
   fill_zero_launch(Ha, g.Nbf, g.Nbf, ldha, s);
   if (pol) fill_zero_launch(Hb, g.Nbf, g.Nbf, ldhb, s);
   const int nd = 2 * g.dmax + 1;
   XCArgs a{Pa, ldpa, Ha, ldha, g.Nel, g.Nang, g.nrad, g.lang, g.mang, nd, xf, cf, g.npair, thr,
            g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rx.p, g.dstart.p, g.xd.p,
            g.R.p, g.wpt.p, g.thx.p, g.thxt.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr,
-           pol ? g.Wb.p : nullptr};
+           pol ? g.Wb.p : nullptr, g.dR.p, g.rinv.p, g.isin.p, g.sn.p, g.thd.p, g.thdt.p, g.rxT.p,
+           g.E.p, g.Y.p, pol ? g.Eb.p : nullptr, pol ? g.Yb.p : nullptr};
   XCArgs b = a;  // second spin: its density in, its matrix out
   if (pol) {
     b.P = Pb;
@@ -314,27 +514,47 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
     b.ldh = ldhb;
     b.D = g.Db.p;
     b.W = g.Wb.p;
+    b.E = g.Eb.p;
+    b.Y = g.Yb.p;
   }
-  const int ns = pol ? 2 : 1;
-  const size_t sh_in = sizeof(double) * ((size_t)g.nrad * XC_RS + (size_t)xc_pairs_per_group(g.nrad) * TILE2);
-  const size_t sh_pt =
-      sizeof(double) * ns * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
-  const size_t sh_out = sizeof(double) * ((size_t)g.nrad * TILE + g.nrad);
+  const int rf = gen ? 2 : 1;
+  const size_t sh_in =
+      sizeof(double) * ((size_t)rf * g.nrad * XC_RS + (size_t)xc_pairs_per_group(g.nrad) * TILE2);
+  const size_t sh_pt = xc_point_lds(g, xf, cf, pol);
+  const size_t sh_out = sizeof(double) * rf * ((size_t)g.nrad * TILE + g.nrad);
   {
     TimedLaunch tl(timer, "xc_rad_in", s);
-    hipLaunchKernelGGL(k_xc_rad_in, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
-    if (pol) hipLaunchKernelGGL(k_xc_rad_in, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
+    if (gen) {
+      hipLaunchKernelGGL(k_xc_rad_in<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
+      if (pol) hipLaunchKernelGGL(k_xc_rad_in<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
+    } else {
+      hipLaunchKernelGGL(k_xc_rad_in<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
+      if (pol) hipLaunchKernelGGL(k_xc_rad_in<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
+    }
     hip_check(hipGetLastError(), "xc_rad_in");
   }
   {
     TimedLaunch tl(timer, "xc_point", s);
-    hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+    if (gen) {
+      if (sh_pt > 64 * 1024)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xc_point_gga),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh_pt),
+                  "xc_point_gga LDS size");
+      hipLaunchKernelGGL(k_xc_point_gga, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+    } else {
+      hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+    }
     hip_check(hipGetLastError(), "xc_point");
   }
   {
     TimedLaunch tl(timer, "xc_rad_out", s);
-    hipLaunchKernelGGL(k_xc_rad_out, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
-    if (pol) hipLaunchKernelGGL(k_xc_rad_out, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
+    if (gen) {
+      hipLaunchKernelGGL(k_xc_rad_out<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
+      if (pol) hipLaunchKernelGGL(k_xc_rad_out<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
+    } else {
+      hipLaunchKernelGGL(k_xc_rad_out<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
+      if (pol) hipLaunchKernelGGL(k_xc_rad_out<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
+    }
     hip_check(hipGetLastError(), "xc_rad_out");
   }
 }

@@ -290,6 +290,26 @@ int orc_lda_pol(int xf, int cf, int n, const double *ra, const double *rb, doubl
     for (int i = 0; i < n; ++i) lda_polarized(xf, cf, ra[i], rb[i], thr, exc[i], va[i], vb[i]);
   });
 }
+// Any supported functional pair at n points: unpolarized (rho, sigma) ->
+// (exc, vrho, vsigma) when pol = 0; polarized rho (2n: a, b), sigma (3n:
+// aa, ab, bb) -> exc, vrho (2n), vsigma (3n) when pol = 1
+int orc_xc_eval(int xf, int cf, int pol, int n, const double *rho, const double *sigma, double thr, double *exc,
+                double *vrho, double *vsigma) {
+  return guard([&] {
+    for (int i = 0; i < n; ++i) {
+      if (pol) {
+        const double r[2] = {rho[i], rho[n + i]}, s[3] = {sigma[i], sigma[n + i], sigma[2 * n + i]};
+        double vr[2], vs[3];
+        xc_polarized(xf, cf, r, s, thr, exc[i], vr, vs);
+        vrho[i] = vr[0];
+        vrho[n + i] = vr[1];
+        for (int k = 0; k < 3; ++k) vsigma[k * n + i] = vs[k];
+      } else {
+        xc_unpolarized(xf, cf, rho[i], sigma[i], thr, exc[i], vrho[i], vsigma[i]);
+      }
+    }
+  });
+}
 // Atomic DFT grid: H (Nbf x Nbf) and out = (Exc, Nel)
 int orc_atomic_xc(void *h, int lang, int mang, int xf, int cf, const double *P, double thr, double *H, double *out) {
   return guard([&] {

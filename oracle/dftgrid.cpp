@@ -151,9 +151,20 @@ void AtomicXC::eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha, M
 }
 
 // Shared element loop; Pb == nullptr selects the restricted form with P the total density.
+// GGA functionals add the density gradient in spherical components
+// (update_density, dftgrid.cpp:75-86, 166-180: g_c = 2 Re sum P (d_c chi) chi* /
+// scale_c with scale = 1, r, r sin(theta)) from the basis derivatives of
+// eval_df (TwoDBasis.cpp:1157-1200: d/dr, d/dtheta, d/dphi of R Y), and the
+// potential term H += X + X', X = Re(sum_c gn_c d_c chi) chi*', gn_c = w (2
+// vsigma g_c) / scale_c restricted, w (2 vsigma_ss g_s,c + vsigma_ab g_s',c) /
+// scale_c polarized (eval_Fxc, dftgrid.cpp:320-334, 382-405;
+// increment_gga_split, dftgrid_common.h:188-216).
 void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &Hb, double &Exc, double &Nel,
                     double thr) const {
+  xc_check(xf, cf);
   const bool pol = Pbp != nullptr;
+  const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const int Na = b.Nang(), Nr = rad.Nbf();
   H = Mat(P.r, P.c);
@@ -164,32 +175,66 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
   for (int iel = 0; iel < rad.Nel(); ++iel) {
     // compute_bf, dftgrid.cpp:470-576
     const Mat R = rad.get_bf(rad.xq, iel);  // (nrad x Ni)
+    const Mat dR = gga ? rad.get_df(rad.xq, iel) : Mat();
     const std::vector<double> r = rad.eval_coord(rad.xq, iel);
     const double h = rad.scaling_factor(iel);
     const int nrad = (int)rad.xq.size(), Ni = R.c, nbf = Na * Ni, npt = nrad * nang;
     std::vector<int> idx(nbf);
     for (int a = 0; a < Na; ++a)
       for (int j = 0; j < Ni; ++j) idx[a * Ni + j] = a * Nr + rad.first[iel] + j;
-    std::vector<double> wtot(npt);
-    for (int ia = 0; ia < nang; ++ia)
-      for (int ir = 0; ir < nrad; ++ir) wtot[ia * nrad + ir] = wang[ia] * (h * rad.wq[ir]) * std::pow(r[ir], 2);
+    std::vector<double> wtot(npt), sc_th(npt), sc_ph(npt);
+    for (int ia = 0; ia < nang; ++ia) {
+      const double sth = std::sqrt(1.0 - cth[ia] * cth[ia]);
+      for (int ir = 0; ir < nrad; ++ir) {
+        wtot[ia * nrad + ir] = wang[ia] * (h * rad.wq[ir]) * std::pow(r[ir], 2);
+        sc_th[ia * nrad + ir] = r[ir];
+        sc_ph[ia * nrad + ir] = r[ir] * sth;
+      }
+    }
     Mat are(nbf, npt), aim(nbf, npt);  // real and imaginary parts of conj(chi)
+    Mat dre[3], dim[3];                // the same of conj(d_c chi), c = r, theta, phi
+    if (gga)
+      for (int c = 0; c < 3; ++c) dre[c] = dim[c] = Mat(nbf, npt);
     for (int ia = 0; ia < nang; ++ia)
       for (int a = 0; a < Na; ++a) {
-        const std::complex<double> y = std::conj(spherical_harmonic(b.lval[a], b.mval[a], cth[ia], phi[ia]));
+        const int l = b.lval[a], m = b.mval[a];
+        const std::complex<double> Y = spherical_harmonic(l, m, cth[ia], phi[ia]);
+        const std::complex<double> y = std::conj(Y);
+        std::complex<double> yc[3];
+        if (gga) {
+          const double sinth = std::sqrt(std::max((1.0 - cth[ia]) * (1.0 + cth[ia]), 0.0));
+          const double cotth = sinth > 0.0 ? cth[ia] / sinth : 0.0;
+          std::complex<double> angfac = double(m) * cotth * Y;
+          if (m < l)
+            angfac += std::sqrt(double((l - m) * (l + m + 1))) * std::exp(std::complex<double>(0.0, -phi[ia])) *
+                      spherical_harmonic(l, m + 1, cth[ia], phi[ia]);
+          yc[0] = y;
+          yc[1] = std::conj(angfac);
+          yc[2] = std::conj(std::complex<double>(0.0, m) * Y);
+        }
         for (int j = 0; j < Ni; ++j)
           for (int ir = 0; ir < nrad; ++ir) {
-            are(a * Ni + j, ia * nrad + ir) = y.real() * R(ir, j);
-            aim(a * Ni + j, ia * nrad + ir) = y.imag() * R(ir, j);
+            const int u = a * Ni + j, p = ia * nrad + ir;
+            are(u, p) = y.real() * R(ir, j);
+            aim(u, p) = y.imag() * R(ir, j);
+            if (gga) {
+              dre[0](u, p) = yc[0].real() * dR(ir, j);
+              dim[0](u, p) = yc[0].imag() * dR(ir, j);
+              for (int c = 1; c < 3; ++c) {
+                dre[c](u, p) = yc[c].real() * R(ir, j);
+                dim[c](u, p) = yc[c].imag() * R(ir, j);
+              }
+            }
           }
       }
-    // update_density: rho = colwise (P a).a + (P b).b, per spin when polarized
-    auto density = [&](const Mat &Pm) {
+    // update_density: rho = colwise (P a).a + (P b).b, per spin when polarized;
+    // with GGA also g_c = 2 ((P a).da_c + (P b).db_c) / scale_c
+    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3]) {
       Mat Pe(nbf, nbf);
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
       const Mat PA = matmul(Pe, are), PB = matmul(Pe, aim);
-      std::vector<double> rho(npt);
+      rho.assign(npt, 0.0);
       for (int p = 0; p < npt; ++p) {
         double s = 0.0, t = 0.0;
         for (int u = 0; u < nbf; ++u) {
@@ -198,25 +243,72 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
         }
         rho[p] = s + t;
       }
-      return rho;
+      if (!gga) return;
+      for (int c = 0; c < 3; ++c) {
+        g[c].assign(npt, 0.0);
+        for (int p = 0; p < npt; ++p) {
+          double s = 0.0, t = 0.0;
+          for (int u = 0; u < nbf; ++u) {
+            s += PA(u, p) * dre[c](u, p);
+            t += PB(u, p) * dim[c](u, p);
+          }
+          const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
+          g[c][p] = 2.0 * (s + t) / sc;
+        }
+      }
     };
-    const std::vector<double> ra = density(P);
-    const std::vector<double> rb = pol ? density(*Pbp) : std::vector<double>();
-    std::vector<double> va(npt), vb(npt);
+    std::vector<double> ra, rb, ga[3], gb[3];
+    density(P, ra, ga);
+    if (pol) density(*Pbp, rb, gb);
+    std::vector<double> va(npt), vb(npt), gna[3], gnb[3];
+    if (gga)
+      for (int c = 0; c < 3; ++c) {
+        gna[c].assign(npt, 0.0);
+        if (pol) gnb[c].assign(npt, 0.0);
+      }
     for (int p = 0; p < npt; ++p) {
       double exc;
-      if (pol) {
-        lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
-        Nel += wtot[p] * (ra[p] + rb[p]);
-        Exc += wtot[p] * exc * (ra[p] + rb[p]);
+      const double rt = pol ? ra[p] + rb[p] : ra[p];
+      if (plain_lda) {
+        if (pol)
+          lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
+        else
+          lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
+      } else if (pol) {
+        const double rho[2] = {ra[p], rb[p]};
+        double sig[3] = {0.0, 0.0, 0.0}, vr[2], vs[3];
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            sig[0] += ga[c][p] * ga[c][p];
+            sig[1] += ga[c][p] * gb[c][p];
+            sig[2] += gb[c][p] * gb[c][p];
+          }
+        xc_polarized(xf, cf, rho, sig, thr, exc, vr, vs);
+        va[p] = vr[0];
+        vb[p] = vr[1];
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
+            gna[c][p] = wtot[p] * (2.0 * vs[0] * ga[c][p] + vs[1] * gb[c][p]) / sc;
+            gnb[c][p] = wtot[p] * (2.0 * vs[2] * gb[c][p] + vs[1] * ga[c][p]) / sc;
+          }
       } else {
-        lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
-        Nel += wtot[p] * ra[p];
-        Exc += wtot[p] * exc * ra[p];
+        double sig = 0.0, vs;
+        if (gga)
+          for (int c = 0; c < 3; ++c) sig += ga[c][p] * ga[c][p];
+        xc_unpolarized(xf, cf, ra[p], sig, thr, exc, va[p], vs);
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
+            gna[c][p] = 2.0 * wtot[p] * vs * ga[c][p] / sc;
+          }
       }
+      Nel += wtot[p] * rt;
+      Exc += wtot[p] * exc * rt;
     }
-    // eval_Fxc + increment_lda_split: H += (a .* w v) a' + (b .* w v) b'
-    auto potential = [&](const std::vector<double> &vr, Mat &Hm) {
+    // eval_Fxc + increment_lda_split: H += (a .* w v) a' + (b .* w v) b';
+    // increment_gga_split: X = (sum_c gn_c .* da_c) a' + (sum_c gn_c .* db_c) b', H += X + X'
+    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, Mat &Hm) {
       Mat av = are, bv = aim;
       for (int p = 0; p < npt; ++p) {
         const double f = vr[p] * wtot[p];
@@ -225,12 +317,28 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
           bv(u, p) *= f;
         }
       }
-      const Mat Ha = matmul_nt(av, are), Hb2 = matmul_nt(bv, aim);
+      Mat Hs = matmul_nt(av, are);
+      const Mat H2 = matmul_nt(bv, aim);
+      for (size_t q = 0; q < Hs.a.size(); ++q) Hs.a[q] += H2.a[q];
+      if (gga) {
+        Mat gre(nbf, npt), gim(nbf, npt);
+        for (int c = 0; c < 3; ++c)
+          for (int p = 0; p < npt; ++p)
+            for (int u = 0; u < nbf; ++u) {
+              gre(u, p) += gn[c][p] * dre[c](u, p);
+              gim(u, p) += gn[c][p] * dim[c](u, p);
+            }
+        Mat X = matmul_nt(gre, are);
+        const Mat X2 = matmul_nt(gim, aim);
+        for (size_t q = 0; q < X.a.size(); ++q) X.a[q] += X2.a[q];
+        for (int v = 0; v < nbf; ++v)
+          for (int u = 0; u < nbf; ++u) Hs(u, v) += X(u, v) + X(v, u);
+      }
       for (int v = 0; v < nbf; ++v)
-        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Ha(u, v) + Hb2(u, v);
+        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Hs(u, v);
     };
-    potential(va, H);
-    if (pol) potential(vb, Hb);
+    potential(va, gna, H);
+    if (pol) potential(vb, gnb, Hb);
   }
 }
 

@@ -8,6 +8,15 @@ namespace orc {
 // lda_x (1) + lda_c_vwn (7, VWN5), spin-unpolarized; thr as libxc's density threshold
 void lda_unpolarized(int xf, int cf, double rho, double thr, double &exc, double &vrho);
 void lda_polarized(int xf, int cf, double ra, double rb, double thr, double &exc, double &va, double &vb);
+// Functionals by id (lda_x 1, lda_c_vwn 7, lda_c_pw 12, lda_c_pw_mod 13,
+// gga_x_pbe 101, gga_c_pbe 130; xcfunc.cpp): energy per particle exc,
+// vrho = d(rho exc)/drho and vsigma = d(rho exc)/dsigma; polarized sigma is
+// (aa, ab, bb) and vsigma likewise.
+bool xc_is_gga(int id);
+void xc_check(int xf, int cf);
+void xc_unpolarized(int xf, int cf, double rho, double sigma, double thr, double &exc, double &vrho, double &vsigma);
+void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], double thr, double &exc,
+                  double vrho[2], double vsigma[3]);
 void angular_chebyshev(int l, int m, std::vector<double> &cth, std::vector<double> &phi, std::vector<double> &w);
 std::complex<double> spherical_harmonic(int l, int m, double cth, double phi);
 

@@ -294,6 +294,21 @@ Mat RadialBasis::get_bf(const std::vector<double> &x, int iel) const {
   return v;
 }
 
+// d/dr (B/r): the analytic deflated form on the first element, else
+// (B' - B/r)/r (RadialBasis.cpp:890-905)
+Mat RadialBasis::get_df(const std::vector<double> &x, int iel) const {
+  if (iel == 0) return get_basis(0).eval_over_r(x, 1, scaling_factor(0));
+  const Mat f = eval_dnf(x, 0, iel), d = eval_dnf(x, 1, iel);
+  const std::vector<double> r = eval_coord(x, iel);
+  Mat out(f.r, f.c);
+  for (int j = 0; j < f.c; ++j)
+    for (int i = 0; i < f.r; ++i) {
+      const double ir = 1.0 / r[i];
+      out(i, j) = (-f(i, j) * ir + d(i, j)) * ir;
+    }
+  return out;
+}
+
 Mat RadialBasis::eval_kind(int kind, const std::vector<double> &x, int iel) const {
   if (kind == 0) return eval_dnf(x, 0, iel);
   if (kind == 1) return eval_dnf(x, 1, iel);

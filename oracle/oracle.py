@@ -57,6 +57,8 @@ def lib():
             getattr(L, name).argtypes = None
         L.orc_lda.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
                               ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_xc_eval.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double] + \
+            [ctypes.c_void_p] * 3
         L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_lda_pol.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -134,6 +136,27 @@ def lda_pol(ra, rb, x_func=1, c_func=7, thr=1e-12):
     exc, va, vb = np.zeros_like(ra), np.zeros_like(ra), np.zeros_like(ra)
     _check(lib().orc_lda_pol(x_func, c_func, ra.size, _ptr(ra), _ptr(rb), thr, _ptr(exc), _ptr(va), _ptr(vb)))
     return exc, va, vb
+
+
+FUNCTIONALS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "gga_x_pbe": 101,
+               "gga_c_pbe": 130}
+
+
+def xc_eval(rho, sigma, x_func, c_func, thr=1e-12):
+    """Functional values at points: unpolarized rho (n,), sigma (n,) -> (exc,
+    vrho, vsigma); polarized rho (2, n) [a, b], sigma (3, n) [aa, ab, bb] ->
+    (exc (n,), vrho (2, n), vsigma (3, n)).  Energies per particle, potentials
+    d(rho exc)/drho and d(rho exc)/dsigma (complex-step derivatives)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    pol = rho.ndim == 2
+    n = rho.shape[-1]
+    exc = np.zeros(n)
+    vr = np.zeros_like(rho)
+    vs = np.zeros_like(sigma)
+    _check(lib().orc_xc_eval(x_func, c_func, int(pol), n, _ptr(rho), _ptr(sigma), thr, _ptr(exc), _ptr(vr),
+                             _ptr(vs)))
+    return exc, vr, vs
 
 
 def grid(rmax, nel, igrid=4, zexp=2.0):

@@ -16,6 +16,9 @@
 //   gaunt ........................ 550 golden doubles of src/general/gaunt_test.cpp + sympy exact
 //   atomic coulomb/exchange ...... He/Be/Ne HF energies of tests/refs/ci.json through oracle/scf.py;
 //                                  Exx = -Coulomb/2 (He closed shell), J symmetric, K symmetric
+//   PBE (xcfunc.cpp) + GGA grid .. uniform-gas and gradient-expansion limits, Lieb-Oxford saturation,
+//                                  finite-difference potentials and dExc/dP, He PBE energies
+//                                  (restricted and unrestricted) of tests/refs/ci.json
 #pragma once
 #include <vector>
 #include <cmath>
@@ -100,6 +103,7 @@ struct RadialBasis {
   std::vector<double> eval_coord(const std::vector<double> &x, int iel) const;
   Mat eval_dnf(const std::vector<double> &x, int n, int iel) const;
   Mat get_bf(const std::vector<double> &x, int iel) const;   // RadialBasis.cpp:881-890 (R = B/r)
+  Mat get_df(const std::vector<double> &x, int iel) const;   // RadialBasis.cpp:890-905 (d/dr of B/r)
   // kind: 0 = B0, 1 = B1, 10 = R0
   Mat eval_kind(int kind, const std::vector<double> &x, int iel) const;
   // matrix_element_auto with no breakpoints (FiniteElementBasis.cpp:556-604)

@@ -1,0 +1,175 @@
+"""Pin the oracle's GGA exchange-correlation path: PBE exchange (libxc
+gga_x_pbe, id 101) and correlation (gga_c_pbe, id 130) on the atomic DFT
+grid (src/atomic/dftgrid.cpp:51-128, 304-359, 470-576;
+src/general/dftgrid_common.h:188-216).
+
+libxc is absent, so the functionals are pinned by what their construction
+fixes (Perdew, Burke, Ernzerhof, PRL 77, 3865 (1996)): the uniform-gas limit
+(PBE exchange = Slater, PBE correlation = PW92 at zero gradient), the
+gradient-expansion coefficients mu and beta at small gradients, the
+Lieb-Oxford saturation 1 + kappa of the exchange enhancement and the
+vanishing correlation of rapidly varying densities; the potentials by finite
+differences of the energy; the grid's gradient terms by the exchange-
+correlation matrix being the derivative of Exc with respect to P; and the
+whole path by the reference's PBE energies of He, restricted and
+unrestricted (tests/refs/ci.json atomic-He-gga-r/-u, -2.8929348668).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+REF = json.loads((Path(__file__).parent / "golden" / "reference_energies.json").read_text())
+KAPPA, MU, BETA = 0.804, 0.06672455060314922 * np.pi ** 2 / 3, 0.06672455060314922
+
+
+def test_zero_gradient_is_the_uniform_gas():
+    rho = np.geomspace(1e-6, 1e3, 9)
+    z = np.zeros_like(rho)
+    ex, vx, _ = O.xc_eval(rho, z, 101, 0)
+    e0, v0 = O.lda(rho, 1, 0)
+    assert np.max(np.abs(ex / e0 - 1)) < 1e-14 and np.max(np.abs(vx / v0 - 1)) < 1e-14
+    ec, vc, _ = O.xc_eval(rho, z, 0, 130)
+    ep, vp, _ = O.xc_eval(rho, z, 0, 13)
+    assert np.max(np.abs(ec / ep - 1)) < 1e-13 and np.max(np.abs(vc / vp - 1)) < 1e-13
+    # polarized, zeta = 1/3 and fully polarized
+    for f in (2 / 3, 1.0):
+        r2 = np.stack([f * rho, (1 - f) * rho])
+        s3 = np.zeros((3, rho.size))
+        ec, _, _ = O.xc_eval(r2, s3, 0, 130)
+        ep, _, _ = O.xc_eval(r2, s3, 0, 13)
+        assert np.max(np.abs(ec / ep - 1)) < 1e-13, f
+
+
+def test_exchange_enhancement_limits():
+    rho = np.full(3, 0.3)
+    e0, _ = O.lda(rho, 1, 0)
+    # small s: F = 1 + mu s^2 + O(s^4)
+    s = np.array([1e-3, 2e-3, 4e-3])
+    sigma = (s * 2 * (3 * np.pi ** 2) ** (1 / 3) * rho ** (4 / 3)) ** 2
+    ex, _, _ = O.xc_eval(rho, sigma, 101, 0)
+    assert np.allclose((ex / e0 - 1) / s ** 2, MU, rtol=1e-4)
+    # large s: F -> 1 + kappa (Lieb-Oxford)
+    ex, _, _ = O.xc_eval(rho, np.array([1e6, 1e8, 1e10]), 101, 0)
+    F = ex / e0
+    assert np.all(np.diff(F) > 0) and abs(F[-1] - (1 + KAPPA)) < 1e-6
+
+
+def test_correlation_gradient_limits():
+    rho = np.array([1e-3, 0.05, 2.0])
+    ep, _, _ = O.xc_eval(rho, np.zeros(3), 0, 13)
+    # small t: H = beta t^2 + O(t^4) (phi = 1); t^2 = sigma / (4 ks^2 rho^2), ks^2 = 4 kF / pi
+    ks2 = 4 * (3 * np.pi ** 2 * rho) ** (1 / 3) / np.pi
+    t2 = 1e-6
+    ec, _, _ = O.xc_eval(rho, t2 * 4 * ks2 * rho ** 2, 0, 130)
+    assert np.allclose((ec - ep) / t2, BETA, rtol=1e-4)
+    # t -> infinity: the correlation energy vanishes
+    ec, _, _ = O.xc_eval(rho, 1e12 * 4 * ks2 * rho ** 2, 0, 130)
+    assert np.max(np.abs(ec / ep)) < 1e-5
+
+
+def test_pw92_near_vwn5():
+    """PW92 and VWN5 fit the same Ceperley-Alder energies: within 1 mEh per
+    electron over rs 0.5..20, unpolarized and fully polarized; the modified
+    parameters differ from the printed ones by less than 1e-6 Eh."""
+    rs = np.geomspace(0.5, 20, 12)
+    rho = 3 / (4 * np.pi * rs ** 3)
+    e7, _, _ = O.xc_eval(rho, np.zeros_like(rho), 0, 7)
+    e12, _, _ = O.xc_eval(rho, np.zeros_like(rho), 0, 12)
+    e13, _, _ = O.xc_eval(rho, np.zeros_like(rho), 0, 13)
+    assert np.max(np.abs(e12 - e7)) < 1e-3 and np.max(np.abs(e13 - e12)) < 1e-6
+    r2, s3 = np.stack([rho, np.zeros_like(rho)]), np.zeros((3, rho.size))
+    f7, _, _ = O.xc_eval(r2, s3, 0, 7)
+    f13, _, _ = O.xc_eval(r2, s3, 0, 13)
+    assert np.max(np.abs(f13 - f7)) < 1e-3
+
+
+def _fd(f, x, h):
+    """fourth-order central difference of f at x (step h, elementwise)"""
+    return (8 * (f(x + h) - f(x - h)) - (f(x + 2 * h) - f(x - 2 * h))) / (12 * h)
+
+
+@pytest.mark.parametrize("xf,cf", [(101, 0), (0, 130), (1, 0), (0, 7), (0, 12), (0, 13)])
+def test_potentials_are_energy_derivatives(xf, cf):
+    """vrho and vsigma against fourth-order finite differences of rho*exc, one
+    functional at a time (PBE exchange and correlation cancel in vsigma to
+    three digits, which a difference quotient of their sum cannot resolve)."""
+    rng = np.random.default_rng(11)
+    rho = np.geomspace(1e-4, 50, 15)
+    sigma = rng.uniform(0.01, 3.0, rho.size) * rho ** (8 / 3)
+    e, vr, vs = O.xc_eval(rho, sigma, xf, cf)
+    fr = _fd(lambda x: x * O.xc_eval(x, sigma, xf, cf)[0], rho, 1e-3 * rho)
+    assert np.max(np.abs(fr - vr) / np.abs(vr)) < 1e-8
+    if xc_gga := xf == 101 or cf == 130:
+        fs = _fd(lambda x: rho * O.xc_eval(rho, x, xf, cf)[0], sigma, 1e-3 * sigma)
+        assert np.max(np.abs(fs - vs) / np.abs(vs)) < 1e-8
+    else:
+        assert np.all(vs == 0.0)
+    # polarized: each of the five inputs
+    r2 = np.stack([0.7 * rho, 0.3 * rho])
+    s3 = np.stack([0.49 * sigma, 0.15 * sigma, 0.09 * sigma])
+    e, vr, vs = O.xc_eval(r2, s3, xf, cf)
+    for k in range(5 if xc_gga else 2):
+        def f(x, k=k):
+            a, b = r2.copy(), s3.copy()
+            (a if k < 2 else b)[k if k < 2 else k - 2] = x
+            return (a[0] + a[1]) * O.xc_eval(a, b, xf, cf)[0]
+        x = r2[k] if k < 2 else s3[k - 2]
+        v = vr[k] if k < 2 else vs[k - 2]
+        fd = _fd(f, x, 1e-3 * x)
+        assert np.max(np.abs(fd - v) / (np.abs(v) + 1e-300)) < 1e-8, k
+
+
+@pytest.fixture(scope="module")
+def ne_basis():
+    b = O.AtomicBasis(10, 2, 1, 3, nnodes=8)
+    b.compute_tei()
+    return b
+
+
+def _density(b, seed, k=6):
+    rng = np.random.default_rng(seed)
+    C = 0.3 * rng.standard_normal((b.Nbf, k))
+    D = rng.standard_normal((b.Nbf, b.Nbf))
+    return C @ C.T, D + D.T
+
+
+def test_gga_matrix_is_energy_derivative(ne_basis):
+    """<H, D> = dExc(P + h D)/dh: the gradient terms (d/dr, d/dtheta, d/dphi
+    of the basis, the 1/r and 1/(r sin theta) scales and the X + X' form) carry
+    their factors correctly on a non-spherical density (lmax 2, mmax 1)."""
+    b = ne_basis
+    P, D = _density(b, 5)
+    H, E, _ = b.xc(P, x_func=101, c_func=130)
+    h = 1e-5
+    fd = (b.xc(P + h * D, x_func=101, c_func=130)[1] - b.xc(P - h * D, x_func=101, c_func=130)[1]) / (2 * h)
+    assert abs(fd / np.sum(H * D) - 1) < 1e-8
+    Pa, Pb = 0.6 * P, 0.4 * P
+    Ha, Hb, _, _ = b.xc_pol(Pa, Pb, x_func=101, c_func=130)
+    for which, Hs in ((0, Ha), (1, Hb)):
+        e = []
+        for sgn in (1, -1):
+            A, B = (Pa + sgn * h * D, Pb) if which == 0 else (Pa, Pb + sgn * h * D)
+            e.append(b.xc_pol(A, B, x_func=101, c_func=130)[2])
+        assert abs((e[0] - e[1]) / (2 * h) / np.sum(Hs * D) - 1) < 1e-8, which
+
+
+def test_polarized_equal_spins_is_restricted(ne_basis):
+    b = ne_basis
+    P, _ = _density(b, 6)
+    H, E, N = b.xc(P, x_func=101, c_func=130)
+    Ha, Hb, E2, N2 = b.xc_pol(0.5 * P, 0.5 * P, x_func=101, c_func=130)
+    s = np.max(np.abs(H))
+    assert abs(E2 - E) < 1e-13 * abs(E) and np.max(np.abs(Ha - H)) < 1e-13 * s and np.max(np.abs(Hb - H)) < 1e-13 * s
+
+
+def test_he_pbe_energies():
+    b = O.AtomicBasis(2, 0, 0, 5)
+    b.compute_tei()
+    r = O.scf_rhf(b, 2, xc=lambda P: b.xc(P, x_func=101, c_func=130)[:2])
+    assert abs(r["E"] - REF["atomic-He-gga-r"]["total"]) < 1e-9, r["E"]
+    u = O.scf_uhf(b, 1, 1, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb, x_func=101, c_func=130)[:3])
+    assert abs(u["E"] - REF["atomic-He-gga-u"]["total"]) < 1e-9, u["E"]
