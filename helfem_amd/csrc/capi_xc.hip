@@ -168,20 +168,34 @@ void xc_upload(hf_dftgrid *g) {
   d.xd.upload(xd, s);
   d.thx.upload(thx, s);
   d.thxt.upload(thxt, s);
-  if (g->atomic) {
-    // gradient tables (GGA): d/dr R, 1/r, 1/sin(theta), sin(d phi), d/dtheta
-    // (Theta_a Theta_b), and the transposed pair of each row-list entry
-    std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), rinv((size_t)Nel * nrad), isin(g->lang);
+  {
+    // gradient tables (GGA): d/dr R (atomic, R = B/r) or d/dmu R (diatomic,
+    // R = B), the inverse scale factors of the three gradient components per
+    // point, sin(d phi), d/dtheta (Theta_a Theta_b), and the transposed pair
+    // of each row-list entry
+    std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), gsc((size_t)Nel * nrad * g->lang * 3);
     std::vector<double> sn((size_t)nd * g->mang), thd((size_t)g->lang * np), thdt((size_t)np * g->lang);
     for (int e = 0; e < Nel; ++e) {
-      const Blk v = rad.rderiv(xq, e);
+      const Blk v = g->atomic ? rad.rderiv(xq, e) : rad.values(1, xq, e);
       const double h = rad.half_len(e), mid = rad.mid(e);
       for (int i = 0; i < nrad; ++i) {
         for (int c = 0; c < v.c; ++c) dR[((size_t)e * nrad + i) * TILE + c] = v(i, c);
-        rinv[(size_t)e * nrad + i] = 1.0 / (mid + h * xq[i]);
+        const double x = mid + h * xq[i];
+        for (int t = 0; t < g->lang; ++t) {
+          const double sth = std::sqrt(std::max((1.0 - xt[t]) * (1.0 + xt[t]), 0.0));
+          double *c3 = &gsc[(((size_t)e * nrad + i) * g->lang + t) * 3];
+          if (g->atomic) {
+            c3[0] = 1.0;
+            c3[1] = 1.0 / x;
+            c3[2] = 1.0 / (x * sth);
+          } else {
+            const double Rh = g->diatomic->Rh, shmu = std::sinh(x);
+            c3[0] = c3[1] = 1.0 / (Rh * std::sqrt(shmu * shmu + sth * sth));
+            c3[2] = g->full3d ? 1.0 / (Rh * shmu * sth) : 0.0;
+          }
+        }
       }
     }
-    for (int t = 0; t < g->lang; ++t) isin[t] = 1.0 / std::sqrt((1.0 - xt[t]) * (1.0 + xt[t]));
     for (int dd = 0; dd < nd; ++dd)
       for (int k = 0; k < g->mang; ++k) sn[(size_t)dd * g->mang + k] = std::sin(double(dd - d.dmax) * (k * dphi));
     std::vector<double> dth((size_t)g->lang * Na);
@@ -203,8 +217,7 @@ void xc_upload(hf_dftgrid *g) {
         need(rxT[q] >= 0, "pair list is not symmetric");
       }
     d.dR.upload(dR, s);
-    d.rinv.upload(rinv, s);
-    d.isin.upload(isin, s);
+    d.gsc.upload(gsc, s);
     d.sn.upload(sn, s);
     d.thd.upload(thd, s);
     d.thdt.upload(thdt, s);
@@ -218,16 +231,15 @@ void xc_upload(hf_dftgrid *g) {
   g->uploaded = true;
 }
 
-// lda_x (1) with lda_c_vwn (7) everywhere; on atomic grids also PBE exchange
-// (101), PW92 (12, 13) and PBE (130) correlation, through the general point
-// kernel (xc_func.hpp)
+// lda_x (1) with lda_c_vwn (7) through the LDA point kernel; PBE exchange
+// (101), PW92 (12, 13) and PBE (130) correlation, alone or mixed with those,
+// through the general point kernel (xc_func.hpp)
 void check_functionals(const hf_dftgrid *g, int x_func, int c_func) {
+  (void)g;
   if (!xcf::exchange_ok(x_func)) throw Error(HF_EINVAL, "unsupported exchange functional (0, 1 = lda_x, 101 = gga_x_pbe)");
   if (!xcf::correlation_ok(c_func))
     throw Error(HF_EINVAL, "unsupported correlation functional (0, 7 = lda_c_vwn, 12 = lda_c_pw, 13 = lda_c_pw_mod, "
                            "130 = gga_c_pbe)");
-  if (xc_generic(x_func, c_func) && !g->atomic)
-    throw Error(HF_EINVAL, "the diatomic grids evaluate lda_x and lda_c_vwn only; GGA and PW92 run on atomic grids");
 }
 
 // Pb == nullptr: restricted; else spin-polarized (Hb gets the beta potential,

@@ -345,11 +345,9 @@ struct XCGridDevice {
   DevArray<double> thx, thxt;         // Theta_a(t) Theta_b(t): [lang][npair] and [npair][lang]
   DevArray<double> D, W;     // D [Nel][nrad][npair], W [Nel][npair][nrad]
   DevArray<double> Db, Wb;   // second spin of a polarized evaluation
-  // gradient (GGA) tables and work arrays, built on the first GGA evaluation
-  bool gga_ready = false;
-  DevArray<double> dR;       // [Nel][nrad][TILE] d/dr of R
-  DevArray<double> rinv;     // [Nel][nrad] 1/r
-  DevArray<double> isin;     // [lang] 1/sin(theta_t)
+  // gradient (GGA) tables and work arrays
+  DevArray<double> dR;       // [Nel][nrad][TILE] d/dr (atomic) or d/dmu (diatomic) of R
+  DevArray<double> gsc;      // [Nel][nrad][lang][3] inverse scale factors 1/h of the gradient components
   DevArray<double> sn;       // [2 dmax + 1][mang] sin(d phi_k)
   DevArray<double> thd, thdt;  // d/dtheta (Theta_a Theta_b): [lang][npair] and [npair][lang]
   DevArray<int> rxT;         // per row-list entry: pair index of the transposed pair (b, a)
@@ -445,7 +443,7 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
                     int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
                     KernelTimer *timer);
 // true when (xf, cf) is not the plain lda_x + lda_c_vwn pair: the general
-// point kernel (GGA terms, any supported functional) runs, on atomic grids
+// point kernel (GGA terms, any supported functional) runs
 bool xc_generic(int xf, int cf);
 // dynamic LDS of the point kernel that xc_grid_launch will use
 size_t xc_point_lds(const XCGridDevice &g, int xf, int cf, bool pol);

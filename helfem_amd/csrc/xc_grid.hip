@@ -18,21 +18,25 @@
 // H += (a .* w v) a' + (b .* w v) b' (dftgrid.cpp:51-128, 304-359;
 // dftgrid_purem.cpp:212-260, 468-500) with the sums reordered.
 //
-// GGA (atomic grid; dftgrid.cpp:75-86, 166-180, 320-334, 382-405,
-// dftgrid_common.h:188-216).  The gradient in spherical components,
-// g = 2 Re sum P (d chi) chi* / (1, r, r sin theta), factorises the same way:
-//   g_r     = 2 sum_d G_d cos(d phi),  G_d(t) = sum_{x in d} Theta_a Theta_b E(x),
-//             E(x) = sum_rs R'_r P_(a,r),(b,s) R_s (k_xc_rad_in alongside D)
-//   g_theta = (1/r) sum_d T_d cos(d phi),  T_d(t) = sum_{x in d} (Theta_a Theta_b)' D(x)
-//   g_phi   = -(1/(r sin theta)) sum_d d C_d sin(d phi)
+// GGA (atomic: dftgrid.cpp:75-86, 166-180, 320-334, 382-405; pure-m
+// diatomic: dftgrid_purem.cpp:248-260, 382-395, 468-500; 3D diatomic:
+// src/diatomic/dftgrid.cpp:75-100, 292-305; dftgrid_common.h:188-216).  The
+// gradient in the grid's orthogonal components, g = 2 Re sum P (d chi) chi* /
+// (h_1, h_2, h_3), factorises the same way, with c_k = 1/h_k per point
+// (atomic (1, 1/r, 1/(r sin theta)); diatomic 1/h_mu = 1/h_nu =
+// 1/(Rh sqrt(sinh^2 mu + sin^2 nu)), 1/h_phi = 1/(Rh sinh mu sin nu)):
+//   g_1 = 2 c_1 sum_d G_d cos(d phi),  G_d(t) = sum_{x in d} Theta_a Theta_b E(x),
+//         E(x) = sum_rs R'_r P_(a,r),(b,s) R_s (k_xc_rad_in alongside D)
+//   g_2 = c_2 sum_d T_d cos(d phi),  T_d(t) = sum_{x in d} (Theta_a Theta_b)' D(x)
+//   g_3 = -c_3 sum_d d C_d sin(d phi)      (zero on the pure-m grid: d = 0)
 // (the last two use D(a,b) = D(b,a), P symmetric, as the reference's factor
 // 2 does).  With U = 2 w vsigma grad rho (restricted) the potential term
 // H += X + X', X = sum U . Re(grad chi_u) chi_v*, becomes per pair x = (a,b)
 //   H_(a,r),(b,s) += sum_i R_r R_s W2(x) + R'_r R_s Y(x) + R_r R'_s Y(x')
-//   W2(x) = sum_t Theta_a Theta_b [V_d - d/(r sin theta) Vphi_d] + (1/r) (Theta_a Theta_b)' Vtheta_d
+//   W2(x) = sum_t Theta_a Theta_b [V_d - d Vphi_d] + (Theta_a Theta_b)' Vtheta_d
 //   Y(x)  = sum_t Theta_a Theta_b Vr_d,   x' = (b, a)
-// with V_d, Vr_d, Vtheta_d = sum_k (w vrho, U_r, U_theta) cos(d phi_k) and
-// Vphi_d = sum_k U_phi sin(d phi_k).
+// with V_d, Vr_d, Vtheta_d = sum_k (w vrho, c_1 U_1, c_2 U_2) cos(d phi_k) and
+// Vphi_d = sum_k c_3 U_3 sin(d phi_k).
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
 #include "xc_func.hpp"
@@ -140,8 +144,9 @@ struct XCArgs {
   const double *R, *wpt, *thx, *thxt, *cs;
   double *D, *W, *part;
   double *Db, *Wb;  // second spin (polarized evaluation), else null
-  // gradient terms (k_xc_rad_in<true>, k_xc_point_gga, k_xc_rad_out<true>)
-  const double *dR, *rinv, *isin, *sn, *thd, *thdt;
+  // gradient terms (k_xc_rad_in<true>, k_xc_point_gga, k_xc_rad_out<true>);
+  // gsc [Nel][nrad][lang][3] = 1/(h_1, h_2, h_3) per point
+  const double *dR, *gsc, *sn, *thd, *thdt;
   const int *rxT;
   double *E, *Y, *Eb, *Yb;
 };
@@ -328,12 +333,12 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
     sB[q] = acc;
   }
   __syncthreads();
-  const double ri = A.rinv[(size_t)e * A.nrad + i];
   const double *wp = A.wpt + ((size_t)e * A.nrad + i) * lang;
+  const double *gs = A.gsc + ((size_t)e * A.nrad + i) * lang * 3;
   double ex = 0.0, ne = 0.0;
   for (int q = t; q < nang; q += NT) {
     const int tt = q / mang, k = q - tt * mang;
-    const double ris = ri * A.isin[tt];
+    const double c1 = gs[3 * tt], c2 = gs[3 * tt + 1], c3 = gs[3 * tt + 2];
     double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     for (int sp = 0; sp < ns; ++sp) {
       const double *C = sB + sp * 3 * ncl, *G = C + ncl, *T = G + ncl;
@@ -346,9 +351,9 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
         gp += (double)(d - dmax) * cd * A.sn[(size_t)d * mang + k];
       }
       rho[sp] = r0;
-      g[sp][0] = 2.0 * gr;
-      g[sp][1] = ri * gt;
-      g[sp][2] = -ris * gp;
+      g[sp][0] = 2.0 * c1 * gr;
+      g[sp][1] = c2 * gt;
+      g[sp][2] = -c3 * gp;
     }
     const double w = wp[tt];
     double f, va, vb = 0.0, u[2][3];
@@ -373,14 +378,14 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
     ex += w * f;
     ne += w * (rho[0] + rho[1]);
     sA[q] = w * va;
-    sA[nang + q] = u[0][0];
-    sA[2 * nang + q] = u[0][1];
-    sA[3 * nang + q] = u[0][2];
+    sA[nang + q] = c1 * u[0][0];
+    sA[2 * nang + q] = c2 * u[0][1];
+    sA[3 * nang + q] = c3 * u[0][2];
     if (pol) {
       sA[na + q] = w * vb;
-      sA[na + nang + q] = u[1][0];
-      sA[na + 2 * nang + q] = u[1][1];
-      sA[na + 3 * nang + q] = u[1][2];
+      sA[na + nang + q] = c1 * u[1][0];
+      sA[na + 2 * nang + q] = c2 * u[1][1];
+      sA[na + 3 * nang + q] = c3 * u[1][2];
     }
   }
   block_sum2(ex, ne, red);
@@ -396,7 +401,7 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
     const double *cs = A.cs + (size_t)d * mang, *snd = A.sn + (size_t)d * mang;
     double acc = 0.0;
     if (c == 0) {
-      const double fphi = (double)(d - dmax) * ri * A.isin[tt];
+      const double fphi = (double)(d - dmax);
       const double *Vp = V + 3 * nang;
       for (int k = 0; k < mang; ++k) acc += V[k] * cs[k] - fphi * Vp[k] * snd[k];
     } else {
@@ -419,7 +424,7 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
         y += th * Vrd[tt];
         wt += A.thd[(size_t)tt * np + x] * Vtd[tt];
       }
-      Wout[((size_t)e * np + x) * A.nrad + i] = w2 + ri * wt;
+      Wout[((size_t)e * np + x) * A.nrad + i] = w2 + wt;
       Yout[((size_t)e * np + x) * A.nrad + i] = y;
     }
   }
@@ -504,7 +509,7 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
   XCArgs a{Pa, ldpa, Ha, ldha, g.Nel, g.Nang, g.nrad, g.lang, g.mang, nd, xf, cf, g.npair, thr,
            g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rx.p, g.dstart.p, g.xd.p,
            g.R.p, g.wpt.p, g.thx.p, g.thxt.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr,
-           pol ? g.Wb.p : nullptr, g.dR.p, g.rinv.p, g.isin.p, g.sn.p, g.thd.p, g.thdt.p, g.rxT.p,
+           pol ? g.Wb.p : nullptr, g.dR.p, g.gsc.p, g.sn.p, g.thd.p, g.thdt.p, g.rxT.p,
            g.E.p, g.Y.p, pol ? g.Eb.p : nullptr, pol ? g.Yb.p : nullptr};
   XCArgs b = a;  // second spin: its density in, its matrix out
   if (pol) {

@@ -258,12 +258,13 @@ int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid 
  * density, out[2] = kinetic energy density integral (0 for LDA and GGA).
  * x_func / c_func are libxc functional ids: exchange 0 (none), 1 (lda_x),
  * 101 (gga_x_pbe); correlation 0, 7 (lda_c_vwn, VWN5), 12 (lda_c_pw),
- * 13 (lda_c_pw_mod), 130 (gga_c_pbe).  The diatomic grids take 1 and 7
- * only; others return HF_EINVAL.  Densities below thr are skipped.  Replaces
- * DFTGrid::eval_Fxc (src/atomic/dftgrid.h:156, dftgrid.cpp:591-653, with the
- * GGA gradient terms of dftgrid.cpp:75-86, 320-334) and
- * PureMDFTGrid::eval_Fxc (src/diatomic/dftgrid_purem.h:154,
- * dftgrid_purem.cpp:675-703) for functionals without external parameters. */
+ * 13 (lda_c_pw_mod), 130 (gga_c_pbe), on every grid; others return
+ * HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
+ * (src/atomic/dftgrid.h:156, dftgrid.cpp:591-653, with the GGA gradient
+ * terms of dftgrid.cpp:75-86, 320-334), PureMDFTGrid::eval_Fxc
+ * (src/diatomic/dftgrid_purem.h:154, dftgrid_purem.cpp:675-703, GGA terms
+ * :248-260, 468-500) and the 3D diatomic DFTGrid::eval_Fxc for functionals
+ * without external parameters. */
 int hf_dftgrid_eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ldp, double *H,
                         int64_t ldh, double thr, double out[3]);
 /* Spin-polarized form: densities Pa, Pb in, potentials Ha, Hb out (Hb zeroed

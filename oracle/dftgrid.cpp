@@ -369,7 +369,10 @@ void DiatomicXC::eval_Fxc(int xf, int cf, const Mat &Pa, const Mat &Pb, Mat &Ha,
 // density and potential as in the atomic grid (update_density, increment_lda).
 void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout, Mat &Hbout, double &Exc,
                         double &Nel, double thr) const {
+  xc_check(xf, cf);
   const bool pol = Pbin != nullptr;
+  const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);
   const Mat Pbx = pol ? b.expand_boundaries(*Pbin) : Mat();
@@ -380,36 +383,69 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
   const double Rh3 = std::pow(b.Rhalf, 3);
   for (int iel = 0; iel < rad.Nel(); ++iel) {
     const Mat Bv = rad.eval_dnf(rad.xq, 0, iel);  // (nq x Ni)
+    const Mat Bd = gga ? rad.eval_dnf(rad.xq, 1, iel) : Mat();
     const std::vector<double> mu = rad.eval_coord(rad.xq, iel);
     const double h = rad.scaling_factor(iel);
     const int nrad = (int)mu.size(), Ni = Bv.c, nbf = Na * Ni, npt = nrad * nang;
     std::vector<int> idx(nbf);
     for (int a = 0; a < Na; ++a)
       for (int j = 0; j < Ni; ++j) idx[a * Ni + j] = a * Nr + rad.first[iel] + j;
-    std::vector<double> wtot(npt);
+    // weights and the scale factors h_mu = h_nu = Rh sqrt(sinh^2 mu + sin^2 nu),
+    // h_phi = Rh sinh(mu) sin(nu) (compute_bf, dftgrid.cpp:414-450)
+    std::vector<double> wtot(npt), sc[3];
+    for (int c = 0; c < 3; ++c) sc[c].assign(npt, 1.0);
     for (int ia = 0; ia < nang; ++ia) {
       const double sth = std::sqrt(std::max((1.0 - cth3[ia]) * (1.0 + cth3[ia]), 0.0));
       for (int ir = 0; ir < nrad; ++ir) {
         const double shmu = std::sinh(mu[ir]);
-        wtot[ia * nrad + ir] = wang3[ia] * (h * rad.wq[ir]) * Rh3 * shmu * (shmu * shmu + sth * sth);
+        const int p = ia * nrad + ir;
+        wtot[p] = wang3[ia] * (h * rad.wq[ir]) * Rh3 * shmu * (shmu * shmu + sth * sth);
+        sc[0][p] = sc[1][p] = b.Rhalf * std::sqrt(shmu * shmu + sth * sth);
+        sc[2][p] = b.Rhalf * shmu * sth;
       }
     }
     Mat are(nbf, npt), aim(nbf, npt);  // real and imaginary parts of conj(chi)
+    Mat dre[3], dim[3];                // the same of conj(d_c chi), c = mu, nu, phi (eval_df, basis.cpp:2204-2243)
+    if (gga)
+      for (int c = 0; c < 3; ++c) dre[c] = dim[c] = Mat(nbf, npt);
     for (int ia = 0; ia < nang; ++ia)
       for (int a = 0; a < Na; ++a) {
-        const std::complex<double> y = std::conj(spherical_harmonic(b.lval[a], b.mval[a], cth3[ia], phi3[ia]));
+        const int l = b.lval[a], m = b.mval[a];
+        const std::complex<double> Y = spherical_harmonic(l, m, cth3[ia], phi3[ia]);
+        const std::complex<double> y = std::conj(Y);
+        std::complex<double> yc[3];
+        if (gga) {
+          const double sinth = std::sqrt(std::max((1.0 - cth3[ia]) * (1.0 + cth3[ia]), 0.0));
+          const double cotth = sinth > 0.0 ? cth3[ia] / sinth : 0.0;
+          std::complex<double> angfac = double(m) * cotth * Y;
+          if (m < l)
+            angfac += std::sqrt(double((l - m) * (l + m + 1))) * std::exp(std::complex<double>(0.0, -phi3[ia])) *
+                      spherical_harmonic(l, m + 1, cth3[ia], phi3[ia]);
+          yc[0] = y;
+          yc[1] = std::conj(angfac);
+          yc[2] = std::conj(std::complex<double>(0.0, m) * Y);
+        }
         for (int j = 0; j < Ni; ++j)
           for (int ir = 0; ir < nrad; ++ir) {
-            are(a * Ni + j, ia * nrad + ir) = y.real() * Bv(ir, j);
-            aim(a * Ni + j, ia * nrad + ir) = y.imag() * Bv(ir, j);
+            const int u = a * Ni + j, p = ia * nrad + ir;
+            are(u, p) = y.real() * Bv(ir, j);
+            aim(u, p) = y.imag() * Bv(ir, j);
+            if (gga) {
+              dre[0](u, p) = yc[0].real() * Bd(ir, j);
+              dim[0](u, p) = yc[0].imag() * Bd(ir, j);
+              for (int c = 1; c < 3; ++c) {
+                dre[c](u, p) = yc[c].real() * Bv(ir, j);
+                dim[c](u, p) = yc[c].imag() * Bv(ir, j);
+              }
+            }
           }
       }
-    auto density = [&](const Mat &Pm) {
+    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3]) {
       Mat Pe(nbf, nbf);
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
       const Mat PA = matmul(Pe, are), PB = matmul(Pe, aim);
-      std::vector<double> rho(npt);
+      rho.assign(npt, 0.0);
       for (int p = 0; p < npt; ++p) {
         double s = 0.0, t = 0.0;
         for (int u = 0; u < nbf; ++u) {
@@ -418,24 +454,65 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
         }
         rho[p] = s + t;
       }
-      return rho;
+      if (!gga) return;
+      for (int c = 0; c < 3; ++c) {
+        g[c].assign(npt, 0.0);
+        for (int p = 0; p < npt; ++p) {
+          double s = 0.0, t = 0.0;
+          for (int u = 0; u < nbf; ++u) {
+            s += PA(u, p) * dre[c](u, p);
+            t += PB(u, p) * dim[c](u, p);
+          }
+          g[c][p] = 2.0 * (s + t) / sc[c][p];
+        }
+      }
     };
-    const std::vector<double> ra = density(P);
-    const std::vector<double> rb = pol ? density(Pbx) : std::vector<double>();
-    std::vector<double> va(npt), vb(npt);
+    std::vector<double> ra, rb, ga[3], gb[3];
+    density(P, ra, ga);
+    if (pol) density(Pbx, rb, gb);
+    std::vector<double> va(npt), vb(npt), gna[3], gnb[3];
+    if (gga)
+      for (int c = 0; c < 3; ++c) {
+        gna[c].assign(npt, 0.0);
+        if (pol) gnb[c].assign(npt, 0.0);
+      }
     for (int p = 0; p < npt; ++p) {
       double exc;
-      if (pol) {
-        lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
-        Nel += wtot[p] * (ra[p] + rb[p]);
-        Exc += wtot[p] * exc * (ra[p] + rb[p]);
+      const double rt = pol ? ra[p] + rb[p] : ra[p];
+      if (plain_lda) {
+        if (pol)
+          lda_polarized(xf, cf, ra[p], rb[p], thr, exc, va[p], vb[p]);
+        else
+          lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
+      } else if (pol) {
+        const double rho[2] = {ra[p], rb[p]};
+        double sig[3] = {0.0, 0.0, 0.0}, vr[2], vs[3];
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            sig[0] += ga[c][p] * ga[c][p];
+            sig[1] += ga[c][p] * gb[c][p];
+            sig[2] += gb[c][p] * gb[c][p];
+          }
+        xc_polarized(xf, cf, rho, sig, thr, exc, vr, vs);
+        va[p] = vr[0];
+        vb[p] = vr[1];
+        if (gga)
+          for (int c = 0; c < 3; ++c) {
+            gna[c][p] = wtot[p] * (2.0 * vs[0] * ga[c][p] + vs[1] * gb[c][p]) / sc[c][p];
+            gnb[c][p] = wtot[p] * (2.0 * vs[2] * gb[c][p] + vs[1] * ga[c][p]) / sc[c][p];
+          }
       } else {
-        lda_unpolarized(xf, cf, ra[p], thr, exc, va[p]);
-        Nel += wtot[p] * ra[p];
-        Exc += wtot[p] * exc * ra[p];
+        double sig = 0.0, vs;
+        if (gga)
+          for (int c = 0; c < 3; ++c) sig += ga[c][p] * ga[c][p];
+        xc_unpolarized(xf, cf, ra[p], sig, thr, exc, va[p], vs);
+        if (gga)
+          for (int c = 0; c < 3; ++c) gna[c][p] = 2.0 * wtot[p] * vs * ga[c][p] / sc[c][p];
       }
+      Nel += wtot[p] * rt;
+      Exc += wtot[p] * exc * rt;
     }
-    auto potential = [&](const std::vector<double> &vr, Mat &Hm) {
+    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, Mat &Hm) {
       Mat av = are, bv = aim;
       for (int p = 0; p < npt; ++p) {
         const double f = vr[p] * wtot[p];
@@ -444,21 +521,45 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
           bv(u, p) *= f;
         }
       }
-      const Mat Ha = matmul_nt(av, are), Hb2 = matmul_nt(bv, aim);
+      Mat Hs = matmul_nt(av, are);
+      const Mat H2s = matmul_nt(bv, aim);
+      for (size_t q = 0; q < Hs.a.size(); ++q) Hs.a[q] += H2s.a[q];
+      if (gga) {
+        Mat gre(nbf, npt), gim(nbf, npt);
+        for (int c = 0; c < 3; ++c)
+          for (int p = 0; p < npt; ++p)
+            for (int u = 0; u < nbf; ++u) {
+              gre(u, p) += gn[c][p] * dre[c](u, p);
+              gim(u, p) += gn[c][p] * dim[c](u, p);
+            }
+        Mat X = matmul_nt(gre, are);
+        const Mat X2 = matmul_nt(gim, aim);
+        for (size_t q = 0; q < X.a.size(); ++q) X.a[q] += X2.a[q];
+        for (int v = 0; v < nbf; ++v)
+          for (int u = 0; u < nbf; ++u) Hs(u, v) += X(u, v) + X(v, u);
+      }
       for (int v = 0; v < nbf; ++v)
-        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Ha(u, v) + Hb2(u, v);
+        for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Hs(u, v);
     };
-    potential(va, H);
-    if (pol) potential(vb, H2);
+    potential(va, gna, H);
+    if (pol) potential(vb, gnb, H2);
   }
   Hout = b.remove_boundaries(H);
   if (pol) Hbout = b.remove_boundaries(H2);
 }
 
+// GGA on the pure-m grid (dftgrid_purem.cpp:87-199, 248-260, 382-395,
+// 468-500): per m block the mu and nu derivatives of B_j(mu) Y_lm(nu, 0)
+// (dr_m = Y B', dth_m = dY/dnu B), gradient g_c = 2 sum (P f) d_c f / h with
+// h = h_mu = h_nu = Rh sqrt(sinh^2 mu + sin^2 nu) (no phi component: the
+// density is phi-independent), potential X = (sum_c gn_c d_c) f', H += X + X'.
 void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout, Mat &Hbout, double &Exc,
                       double &Nel, double thr) const {
   if (mang > 0) return eval3d(xf, cf, Pin, Pbin, Hout, Hbout, Exc, Nel, thr);
+  xc_check(xf, cf);
   const bool pol = Pbin != nullptr;
+  const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);
   const Mat Pbx = pol ? b.expand_boundaries(*Pbin) : Mat();
@@ -472,75 +573,137 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
   const double Rh3 = std::pow(b.Rhalf, 3);
   for (int iel = 0; iel < rad.Nel(); ++iel) {
     const Mat Bv = rad.eval_dnf(rad.xq, 0, iel);  // (nq x Ni)
+    const Mat Bd = gga ? rad.eval_dnf(rad.xq, 1, iel) : Mat();
     const std::vector<double> mu = rad.eval_coord(rad.xq, iel);
     const double h = rad.scaling_factor(iel);
     const int Ni = Bv.c;
     for (int ir = 0; ir < (int)mu.size(); ++ir) {
       const double wr = h * rad.wq[ir], shmu = std::sinh(mu[ir]);
-      std::vector<double> wtot(nang);
+      std::vector<double> wtot(nang), hs(nang);
       for (int ia = 0; ia < nang; ++ia) {
         const double sth = std::sqrt(std::max((1.0 - cth[ia]) * (1.0 + cth[ia]), 0.0));
         wtot[ia] = 2.0 * kPi * wang[ia] * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
+        hs[ia] = b.Rhalf * std::sqrt(shmu * shmu + sth * sth);
       }
       std::vector<std::vector<int>> idx(mlist.size());
-      std::vector<Mat> f(mlist.size());
+      std::vector<Mat> f(mlist.size()), fr(mlist.size()), ft(mlist.size());
       std::vector<double> rho(nang, 0.0), rhob(nang, 0.0);
+      std::vector<double> ga[2] = {std::vector<double>(nang, 0.0), std::vector<double>(nang, 0.0)};
+      std::vector<double> gb[2] = {std::vector<double>(nang, 0.0), std::vector<double>(nang, 0.0)};
       for (size_t im = 0; im < mlist.size(); ++im) {
         for (int a = 0; a < Na; ++a)
           if (b.mval[a] == mlist[im])
             for (int j = 0; j < Ni; ++j) idx[im].push_back(a * Nr + rad.first[iel] + j);
         const int nbf = (int)idx[im].size();
         f[im] = Mat(nbf, nang);
+        if (gga) fr[im] = ft[im] = Mat(nbf, nang);
         int row = 0;
         for (int a = 0; a < Na; ++a) {
           if (b.mval[a] != mlist[im]) continue;
+          const int l = b.lval[a], m = b.mval[a];
           for (int ia = 0; ia < nang; ++ia) {
-            const double Y = std::real(spherical_harmonic(b.lval[a], b.mval[a], cth[ia], 0.0));
-            for (int j = 0; j < Ni; ++j) f[im](row + j, ia) = Y * Bv(ir, j);
+            const double Y = std::real(spherical_harmonic(l, m, cth[ia], 0.0));
+            double dY = 0.0;
+            if (gga) {
+              const double sinth = std::sqrt(std::max((1.0 - cth[ia]) * (1.0 + cth[ia]), 0.0));
+              const double cotth = sinth > 0.0 ? cth[ia] / sinth : 0.0;
+              dY = m * cotth * Y;
+              if (m < l) dY += std::sqrt(double((l - m) * (l + m + 1))) * std::real(spherical_harmonic(l, m + 1, cth[ia], 0.0));
+            }
+            for (int j = 0; j < Ni; ++j) {
+              f[im](row + j, ia) = Y * Bv(ir, j);
+              if (gga) {
+                fr[im](row + j, ia) = Y * Bd(ir, j);
+                ft[im](row + j, ia) = dY * Bv(ir, j);
+              }
+            }
           }
           row += Ni;
         }
-        auto block_density = [&](const Mat &Pm, std::vector<double> &r) {
+        auto block_density = [&](const Mat &Pm, std::vector<double> &r, std::vector<double> *g) {
           Mat Pblk(nbf, nbf);
           for (int v = 0; v < nbf; ++v)
             for (int u = 0; u < nbf; ++u) Pblk(u, v) = Pm(idx[im][u], idx[im][v]);
           const Mat Pv = matmul(Pblk, f[im]);
           for (int ia = 0; ia < nang; ++ia) {
-            double s = 0.0;
-            for (int u = 0; u < nbf; ++u) s += Pv(u, ia) * f[im](u, ia);
+            double s = 0.0, s0 = 0.0, s1 = 0.0;
+            for (int u = 0; u < nbf; ++u) {
+              s += Pv(u, ia) * f[im](u, ia);
+              if (gga) {
+                s0 += Pv(u, ia) * fr[im](u, ia);
+                s1 += Pv(u, ia) * ft[im](u, ia);
+              }
+            }
             r[ia] += s;
+            if (gga) {
+              g[0][ia] += 2.0 * s0;
+              g[1][ia] += 2.0 * s1;
+            }
           }
         };
-        block_density(P, rho);
-        if (pol) block_density(Pbx, rhob);
+        block_density(P, rho, ga);
+        if (pol) block_density(Pbx, rhob, gb);
       }
-      std::vector<double> wv(nang), wvb(nang);
+      std::vector<double> wv(nang), wvb(nang), gna[2], gnb[2];
+      for (int c = 0; c < 2; ++c) {
+        gna[c].assign(nang, 0.0);
+        gnb[c].assign(nang, 0.0);
+      }
       for (int ia = 0; ia < nang; ++ia) {
         double exc, v, vb2 = 0.0;
-        if (pol) {
-          lda_polarized(xf, cf, rho[ia], rhob[ia], thr, exc, v, vb2);
-          Nel += wtot[ia] * (rho[ia] + rhob[ia]);
-          Exc += wtot[ia] * exc * (rho[ia] + rhob[ia]);
-        } else {
-          lda_unpolarized(xf, cf, rho[ia], thr, exc, v);
-          Nel += wtot[ia] * rho[ia];
-          Exc += wtot[ia] * exc * rho[ia];
+        for (int c = 0; c < 2 && gga; ++c) {
+          ga[c][ia] /= hs[ia];
+          gb[c][ia] /= hs[ia];
         }
+        if (plain_lda) {
+          if (pol)
+            lda_polarized(xf, cf, rho[ia], rhob[ia], thr, exc, v, vb2);
+          else
+            lda_unpolarized(xf, cf, rho[ia], thr, exc, v);
+        } else if (pol) {
+          const double rr[2] = {rho[ia], rhob[ia]};
+          const double sig[3] = {ga[0][ia] * ga[0][ia] + ga[1][ia] * ga[1][ia],
+                                 ga[0][ia] * gb[0][ia] + ga[1][ia] * gb[1][ia],
+                                 gb[0][ia] * gb[0][ia] + gb[1][ia] * gb[1][ia]};
+          double vr[2], vs[3];
+          xc_polarized(xf, cf, rr, sig, thr, exc, vr, vs);
+          v = vr[0];
+          vb2 = vr[1];
+          for (int c = 0; c < 2 && gga; ++c) {
+            gna[c][ia] = wtot[ia] * (2.0 * vs[0] * ga[c][ia] + vs[1] * gb[c][ia]) / hs[ia];
+            gnb[c][ia] = wtot[ia] * (2.0 * vs[2] * gb[c][ia] + vs[1] * ga[c][ia]) / hs[ia];
+          }
+        } else {
+          double vs;
+          xc_unpolarized(xf, cf, rho[ia], ga[0][ia] * ga[0][ia] + ga[1][ia] * ga[1][ia], thr, exc, v, vs);
+          for (int c = 0; c < 2 && gga; ++c) gna[c][ia] = 2.0 * wtot[ia] * vs * ga[c][ia] / hs[ia];
+        }
+        const double rt = pol ? rho[ia] + rhob[ia] : rho[ia];
+        Nel += wtot[ia] * rt;
+        Exc += wtot[ia] * exc * rt;
         wv[ia] = v * wtot[ia];
         wvb[ia] = vb2 * wtot[ia];
       }
-      auto potential = [&](const std::vector<double> &w, Mat &Hm) {
+      auto potential = [&](const std::vector<double> &w, const std::vector<double> *gn, Mat &Hm) {
         for (size_t im = 0; im < mlist.size(); ++im) {
           Mat fw = f[im];
           for (int ia = 0; ia < nang; ++ia)
             for (int u = 0; u < fw.r; ++u) fw(u, ia) *= w[ia];
-          const Mat Hblk = matmul_nt(fw, f[im]);
+          Mat Hblk = matmul_nt(fw, f[im]);
+          if (gga) {
+            Mat gr(fw.r, nang);
+            for (int ia = 0; ia < nang; ++ia)
+              for (int u = 0; u < fw.r; ++u) gr(u, ia) = gn[0][ia] * fr[im](u, ia) + gn[1][ia] * ft[im](u, ia);
+            const Mat X = matmul_nt(gr, f[im]);
+            for (int v = 0; v < X.c; ++v)
+              for (int u = 0; u < X.r; ++u) Hblk(u, v) += X(u, v) + X(v, u);
+          }
           for (int v = 0; v < Hblk.c; ++v)
             for (int u = 0; u < Hblk.r; ++u) Hm(idx[im][u], idx[im][v]) += Hblk(u, v);
         }
       };
-      potential(wv, H);
-      if (pol) potential(wvb, H2);
+      potential(wv, gna, H);
+      if (pol) potential(wvb, gnb, H2);
     }
   }
   Hout = b.remove_boundaries(H);

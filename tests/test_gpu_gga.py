@@ -1,14 +1,18 @@
 """GPU parity of the GGA exchange-correlation path (PBE, and the other
-functional pairs the general point kernel evaluates) against the oracle.
+functional pairs the general point kernel evaluates) against the oracle, on
+the atomic, pure-m diatomic and 3D diatomic grids.
 
-The oracle restates the reference's dense atomic grid with the gradient terms
-(src/atomic/dftgrid.cpp:51-128, 304-359, 470-576; dftgrid_common.h:188-216)
+The oracle restates the reference's dense grids with the gradient terms
+(src/atomic/dftgrid.cpp:51-128, 304-359, 470-576; src/diatomic/
+dftgrid_purem.cpp:87-500; src/diatomic/dftgrid.cpp:75-100, 292-305, 414-450;
+dftgrid_common.h:188-216)
 and differentiates the functionals by complex step; the kernels contract the
 factorised basis one index at a time and differentiate by forward-mode dual
 numbers, so the two agree to rounding.  Bar: max|H_gpu - H_oracle| /
 max|H_oracle| <= 1e-12, Exc and Nel within 1e-12 relative; the PBE SCF
 energies of He (restricted and unrestricted) within 1e-9 Eh of the
-reference's -2.8929348668 (tests/refs/ci.json atomic-He-gga-r/-u).
+reference's -2.8929348668 (tests/refs/ci.json atomic-He-gga-r/-u) and of Ne
+through the diatomic code, -128.866426259 (diatomic-Ne-dummy-gga).
 """
 import json
 from pathlib import Path
@@ -82,12 +86,72 @@ def test_polarized_gga_matrix_matches_oracle(name, gpu_ctx):
     assert np.max(np.abs(Hbg - Hbr)) <= TOL * np.max(np.abs(Har))
 
 
-def test_diatomic_grid_refuses_gga(gpu_ctx):
-    hb = H.DiatomicBasis(1, 1, 1.4, [4], 3, ctx=gpu_ctx)
+DIA = {
+    "N2-l432-3el": (7, 7, 2.074, [4, 3, 2], 3),
+    "LiH-l32-4el": (3, 1, 3.0, [3, 2], 4),
+    "HeH-l21-1el": (2, 1, 1.5, [2, 1], 1),
+}
+
+
+@pytest.mark.parametrize("name", list(DIA))
+def test_purem_gga_matrix_matches_oracle(name, gpu_ctx):
+    """Pure-m diatomic grid (dftgrid_purem.cpp:248-260, 382-395, 468-500):
+    mu and nu gradient components with h_mu = h_nu, restricted and polarized."""
+    print(f"[gpu] diatomic pure-m GGA Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    P, Pb = _density(ob, 2205), _density(ob, 2206, 0.5)
     grid = H.DFTGrid(hb)
-    P = np.eye(hb.Nbf) * 1e-3
-    with pytest.raises(RuntimeError):
-        grid.eval_Fxc(101, 130, P)
+    Hr, Er, Nr = ob.xc(P, x_func=101, c_func=130)
+    Hg, Eg, Ng, _ = grid.eval_Fxc(101, 130, P)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+    Har, Hbr, Er, Nr = ob.xc_pol(P, Pb, x_func=101, c_func=130)
+    Hag, Hbg, Eg, Ng, _ = grid.eval_Fxc_pol(101, 130, P, Pb)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL, (relerr(Hag, Har), relerr(Hbg, Hbr))
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+
+
+@pytest.mark.parametrize("name", ["LiH-l32-4el", "HeH-l21-1el"])
+def test_3d_gga_matrix_matches_oracle(name, gpu_ctx):
+    """General 3D diatomic grid (src/diatomic/dftgrid.cpp:75-100, 292-305):
+    mu, nu and phi components on a density that mixes m."""
+    from helfem_amd.synthetic import random_symmetric
+    print(f"[gpu] diatomic 3D-grid GGA Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    S = ob.overlap()
+    X = random_symmetric(ob.Nbf, seed=1411)
+    P = X @ S @ X.T
+    P *= 4.0 / np.sum(P * S)
+    grid = H.DFTGrid(hb, purem=False)
+    Hr, Er, Nr = ob.xc3d(P, x_func=101, c_func=130)
+    Hg, Eg, Ng, _ = grid.eval_Fxc(101, 130, P)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+    Pb = 0.3 * P
+    Har, Hbr, Er, Nr = ob.xc3d_pol(P, Pb, x_func=101, c_func=130)
+    Hag, Hbg, Eg, Ng, _ = grid.eval_Fxc_pol(101, 130, P, Pb)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL, (relerr(Hag, Har), relerr(Hbg, Hbr))
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+
+
+def test_ne_dummy_centre_pbe_scf_on_gpu(gpu_ctx):
+    """diatomic-Ne-dummy-gga: PBE on the pure-m grid, Z1=10, Z2=0, R=1 bohr, 3
+    elements, lmax 13 (m=0), 9 (|m|=1); reference -128.866426259."""
+    import torch
+    from helfem_amd.scf import rhf
+    print("[gpu] diatomic Ne dummy-centre PBE SCF", flush=True)
+    hb = H.DiatomicBasis(10, 0, 1.0, [13, 9], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    r = rhf(hb, 10, xc=lambda P: grid.eval_Fxc(101, 130, P)[:2], conv=1e-12)
+    torch.cuda.synchronize()
+    ref = REF["diatomic-Ne-dummy-gga"]["total"]
+    print(f"[gpu]   E = {r.E:.10f} (reference {ref})", flush=True)
+    assert abs(r.E - ref) < 1e-9, (r.E, ref)
 
 
 @pytest.mark.parametrize("restricted", [True, False], ids=["r", "u"])

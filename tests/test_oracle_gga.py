@@ -173,3 +173,67 @@ def test_he_pbe_energies():
     assert abs(r["E"] - REF["atomic-He-gga-r"]["total"]) < 1e-9, r["E"]
     u = O.scf_uhf(b, 1, 1, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb, x_func=101, c_func=130)[:3])
     assert abs(u["E"] - REF["atomic-He-gga-u"]["total"]) < 1e-9, u["E"]
+
+
+# ---- diatomic grids: pure-m (dftgrid_purem.cpp:87-199, 248-260, 382-395,
+# 468-500) and the general 3D grid (src/diatomic/dftgrid.cpp:75-100, 292-305,
+# 414-450; basis.cpp:2204-2243)
+
+@pytest.fixture(scope="module")
+def nh_basis():
+    b = O.DiatomicBasis(7, 1, 1.958, [5, 4, 3], 3, nnodes=8)
+    b.compute_tei()
+    return b
+
+
+def _m_diagonal(b, P):
+    Pd = np.zeros_like(P)
+    for blk in b.sym_idx(1):
+        Pd[np.ix_(blk, blk)] = P[np.ix_(blk, blk)]
+    return Pd
+
+
+@pytest.mark.parametrize("grid", ["purem", "3d"])
+def test_diatomic_gga_matrix_is_energy_derivative(nh_basis, grid):
+    """<H, D> = dExc(P + h D)/dh on both diatomic grids, restricted and both
+    spins: the mu and nu derivatives, the scale factor h_mu = h_nu and, on the
+    3D grid, the phi component carry their factors correctly."""
+    b = nh_basis
+    P, D = _density(b, 21)
+    if grid == "purem":
+        P, D = _m_diagonal(b, P), _m_diagonal(b, D)
+        xc = lambda A: b.xc(A, x_func=101, c_func=130)
+        xcp = lambda A, B: b.xc_pol(A, B, x_func=101, c_func=130)
+    else:
+        xc = lambda A: b.xc3d(A, x_func=101, c_func=130)
+        xcp = lambda A, B: b.xc3d_pol(A, B, x_func=101, c_func=130)
+    H, E, _ = xc(P)
+    h = 1e-5
+    fd = (xc(P + h * D)[1] - xc(P - h * D)[1]) / (2 * h)
+    assert abs(fd / np.sum(H * D) - 1) < 1e-8
+    Pa, Pb = 0.6 * P, 0.4 * P
+    Ha, Hb, _, _ = xcp(Pa, Pb)
+    for which, Hs in ((0, Ha), (1, Hb)):
+        e = [xcp(Pa + s * h * D, Pb)[2] if which == 0 else xcp(Pa, Pb + s * h * D)[2] for s in (1, -1)]
+        assert abs((e[0] - e[1]) / (2 * h) / np.sum(Hs * D) - 1) < 1e-8, which
+    Ha, Hb, E2, _ = xcp(0.5 * P, 0.5 * P)
+    assert abs(E2 - E) < 1e-13 * abs(E) and np.max(np.abs(Ha - H)) < 1e-13 * np.max(np.abs(H))
+
+
+def test_3d_grid_gga_equals_purem_on_m_diagonal_density(nh_basis):
+    b = nh_basis
+    P = _m_diagonal(b, _density(b, 22)[0])
+    H3, E3, N3 = b.xc3d(P, x_func=101, c_func=130)
+    Hp, Ep, Np = b.xc(P, x_func=101, c_func=130)
+    assert np.max(np.abs(H3 - Hp)) < 1e-11 * np.max(np.abs(Hp))
+    assert abs(E3 - Ep) < 1e-12 * abs(Ep) and abs(N3 - Np) < 1e-12 * abs(Np)
+
+
+def test_ne_dummy_centre_pbe_energy():
+    """diatomic-Ne-dummy-gga (tests/refs/ci.json, -128.866426259): Ne through
+    the diatomic code with a zero-charge centre at 1 bohr, lmax 13 (m=0) and 9
+    (|m|=1), 3 elements, PBE on the pure-m grid."""
+    b = O.DiatomicBasis(10, 0, 1.0, [13, 9], 3)
+    b.compute_tei()
+    r = O.scf_rhf(b, 10, xc=lambda P: b.xc(P, x_func=101, c_func=130)[:2])
+    assert abs(r["E"] - REF["diatomic-Ne-dummy-gga"]["total"]) < 1e-9, r["E"]
