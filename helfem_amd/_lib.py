@@ -44,6 +44,8 @@ SIGNATURES = {
     "hf_atomic_get_cache": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dp]),
     "hf_atomic_one_electron": (c_int, [c_vp, c_int, c_dp, c_i64]),
     "hf_atomic_one_electron_dev": (c_int, [c_vp, c_int, c_dp, c_i64]),
+    "hf_atomic_one_electron_prepare": (c_int, [c_vp]),
+    "hf_eigh_batched": (c_int, [c_vp, c_int, c_dp, c_i64, c_i64, c_dp, c_dp, c_vp, c_int]),
     "hf_atomic_coulomb": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_exchange": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),
     "hf_atomic_coulomb_dev": (c_int, [c_vp, c_dp, c_i64, c_dp, c_i64]),

@@ -164,6 +164,14 @@ class AtomicBasis:
         check(lib().hf_atomic_one_electron_dev(self._h, which, out.data_ptr(), self.Nbf))
         return out.T  # column-major storage: the transpose view is the matrix
 
+    def prepare_one_electron(self):
+        """Build and send the radial one-electron factors now (host work,
+        safe to run for different bases on different threads), so later
+        one_electron_dev calls only expand them on the device."""
+        if self.ctx is None:
+            raise RuntimeError("basis was created without a GPU context")
+        check(lib().hf_atomic_one_electron_prepare(self._h))
+
     def overlap(self):
         return self._one_e(0)
 

@@ -42,6 +42,16 @@ class System:
     def restricted(self):
         return self.nela == self.nelb
 
+    def beta_basis(self):
+        """A second handle on the basis for the beta density of an
+        unrestricted solve, made once and kept (its device tables with it),
+        so later solves of this system reuse it."""
+        b = self.__dict__.get("_beta")
+        if b is None:
+            b = self.basis.clone()
+            self.__dict__["_beta"] = b
+        return b
+
     def footprint(self):
         """Device bytes held while in flight: S, T, V, H, P (per spin), J, K
         or Vxc, F and its DIIS blocks, about 14 Nbf^2 doubles."""
@@ -158,6 +168,8 @@ def random_atoms(n, seed, ctx, zmax=86, nel_range=(4, 24), orders=(10, 15), lmax
         Z, nel, nnodes, lmax = spec
         b = AtomicBasis(Z, lmax, lmax, nel, nnodes=nnodes, ctx=ctx)
         b.compute_tei(device=device_setup and ctx is not None)
+        if ctx is not None:
+            b.prepare_one_electron()
         return b
 
     with ThreadPoolExecutor(setup_threads) as ex:

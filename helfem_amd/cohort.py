@@ -35,6 +35,9 @@ import time
 
 import numpy as np
 
+from . import linalg
+from .scf import EIGH_MIN_N
+
 PAD_RATIO = 0.75
 DEGEN = 1e-9
 DEGEN_FP32 = 1e-5  # levels from an FP32 exchange carry ~1e-6 relative error
@@ -117,6 +120,7 @@ class Cohort:
         self.t0 = time.perf_counter()
         self._last = self.t0
         self.phases = {}
+        self._infos = []
         with torch.cuda.stream(self.stream):
             self._setup(precision)
             self._batches()
@@ -158,7 +162,8 @@ class Cohort:
         for q, s in enumerate(self.systems):
             if not self.restricted[q] and s.nelb > 0:
                 self.beta_of[q] = len(self.chan)
-                self.chan.append(_Channel(q, 1, 1.0, s.nelb, s.basis.clone(), off))
+                beta = s.beta_basis() if hasattr(s, "beta_basis") else s.basis.clone()
+                self.chan.append(_Channel(q, 1, 1.0, s.nelb, beta, off))
                 off += int(sz[q])
         ctot = off
         self.ntot, self.ctot = ntot, ctot
@@ -198,7 +203,9 @@ class Cohort:
         self.kmax = max(1, max(c.nocc for c in self.chan))
         self.cls = []
         for members in groups:
-            N = blocks[members[0]][1].size
+            # stacks below EIGH_MIN_N are padded up to it: the batched
+            # eigensolver is an order of magnitude slower below that size
+            N = max(blocks[members[0]][1].size, EIGH_MIN_N)
             B = len(members)
             rows = np.full((B, N), -1, dtype=np.int64)
             for p, b in enumerate(members):
@@ -223,7 +230,7 @@ class Cohort:
             d = torch.diagonal(C["S"], dim1=1, dim2=2)
             d = torch.where(d > 0, 1.0 / torch.sqrt(d.clamp(min=1e-300)), torch.zeros_like(d))
             Sn = d[:, :, None] * C["S"] * d[:, None, :] + torch.diag_embed(C["pad"])
-            e, v = torch.linalg.eigh(Sn)
+            e, v = self._eigh(Sn)
             X = d[:, :, None] * ((v * (1.0 / torch.sqrt(e))[:, None, :]) @ v.transpose(1, 2))
             C["X"] = X * (1.0 - C["pad"])[:, :, None] * (1.0 - C["pad"])[:, None, :]
             C["kc"] = kc = min(self.kmax, N)
@@ -378,7 +385,7 @@ class Cohort:
             pt = torch.as_tensor(pos, device=self.dev)
             As_ = A.index_select(0, pt) if pos.size < len(C["members"]) else A
             c = 2.0 * torch.linalg.matrix_norm(As_) + 1.0
-            e, v = torch.linalg.eigh(As_ + torch.diag_embed(C["pad"].index_select(0, pt) * c[:, None]))
+            e, v = self._eigh(As_ + torch.diag_embed(C["pad"].index_select(0, pt) * c[:, None]))
             kc = C["kc"]
             C["ev"].index_copy_(0, pt, e[:, :kc])
             C["V"].index_copy_(0, pt, v[:, :, :kc])
@@ -386,10 +393,22 @@ class Cohort:
             self.solved[C["members"][pos]] = True
             self.nsolves += int(pos.size)
 
+    def _eigh(self, A):
+        """Batched eigensolve (hf_eigh_batched, rocSOLVER dsyevd); its status
+        is checked with the next transfer of the levels."""
+        e, v, info = linalg.eigh(A, self.ctx)
+        self._infos.append(info)
+        return e, v
+
     def _levels(self):
         """Stored lowest levels of every block, on the host ([nblk, kmax])."""
         torch = self.torch
-        host = torch.cat([C["ev"].reshape(-1) for C in self.cls]).cpu().numpy()
+        parts = [C["ev"].reshape(-1) for C in self.cls]
+        nval = sum(p.numel() for p in parts)
+        host = torch.cat(parts + [i.to(torch.float64) for i in self._infos]).cpu().numpy()
+        if np.any(host[nval:] != 0):
+            raise RuntimeError("batched eigensolver did not converge")
+        self._infos = []
         out = np.full((self.nblk, self.kmax), np.inf)
         o = 0
         for C in self.cls:
@@ -719,10 +738,18 @@ def solve_cohorts(systems, ctx, budget_bytes=96 << 30, **kw):
     occ = kw.pop("occupations", None)
     groups = cohort_groups(systems, budget_bytes, kw.get("diis_len", 10), kw.get("symm", 2))
     res = [None] * len(systems)
+    import gc
+    import torch
     for c, g in enumerate(groups):
         co = Cohort([systems[i] for i in g], ctx, occupations=None if occ is None else [occ[i] for i in g], **kw)
         for i, r in zip(g, co.solve()):
             r["cohort"] = c
             res[i] = r
+        # hand the cohort's memory back: a cohort holds up to the budget in
+        # padded stacks, and blocks cached from the last one would leave the
+        # next one's allocations to fail, free the whole cache and retry
+        # (seconds per cohort)
         del co
+        gc.collect()
+        torch.cuda.empty_cache()
     return res, time.perf_counter() - t0

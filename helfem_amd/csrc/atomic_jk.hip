@@ -27,6 +27,9 @@
 #include <cstring>
 #include <type_traits>
 #include <cmath>
+#include <array>
+#include <map>
+#include <mutex>
 
 namespace hfg {
 namespace {
@@ -725,6 +728,40 @@ __global__ void __launch_bounds__(NT) k_scatter_blocks(double *out, int64_t ld, 
 }
 }  // namespace
 
+namespace {
+// Block-diagonal one-electron matrix of an atomic basis: shell a's diagonal
+// block is f M + l_a (l_a + 1) X (X null: no l term), zero elsewhere; the
+// products and the sum are rounded separately, as the host assembly does.
+__global__ void __launch_bounds__(NT) k_shell_diag_expand(double *out, int64_t ld, int64_t nbf, int n,
+                                                          const double *M, const double *X, const int *lval,
+                                                          double f) {
+#pragma clang fp contract(off)  // two roundings, as the host assembly: no fused multiply-add
+  const int64_t total = nbf * nbf;
+  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * NT) {
+    const int64_t j = q / nbf, i = q - j * nbf;
+    const int64_t a = i / n;
+    double v = 0.0;
+    if (a == j / n) {
+      const int64_t m = (i - a * n) + (j - a * n) * n;
+      v = f * M[m];
+      if (X) {
+        const int l = lval[a];
+        if (l != 0) v += double(l * (l + 1)) * X[m];
+      }
+    }
+    out[i + j * ld] = v;
+  }
+}
+}  // namespace
+
+void shell_diag_launch(double *out, int64_t ld, int64_t nbf, int n, const double *M, const double *X,
+                       const int *lval, double f, hipStream_t s) {
+  const int64_t total = nbf * nbf;
+  const unsigned blocks = (unsigned)std::min<int64_t>((total + NT - 1) / NT, 8192);
+  hipLaunchKernelGGL(k_shell_diag_expand, dim3(std::max(1u, blocks)), dim3(NT), 0, s, out, ld, nbf, n, M, X, lval, f);
+  hip_check(hipGetLastError(), "shell_diag_expand");
+}
+
 void scatter_blocks_launch(double *out, int64_t ld, int64_t n, const std::vector<long long> &meta,
                            const std::vector<double> &vals, hipStream_t s) {
   fill_zero_launch(out, n, n, ld, s);
@@ -1232,6 +1269,22 @@ int fill_blocks(int64_t rows, int64_t cols) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 8 * NT - 1) / (8 * NT)));
 }
 
+// Autotuned workgroup shapes, per (device, class, part, size bucket), shared
+// by every batch of the process.
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, int> g_tuned;
+
+int tuned_shape(int dev, int c, int p, int bucket) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const auto it = g_tuned.find({dev, c, p, bucket});
+  return it == g_tuned.end() ? 0 : it->second;
+}
+
+void remember_shape(int dev, int c, int p, int bucket, int nw) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[{dev, c, p, bucket}] = nw;
+}
+
 // Run `launches` as a graph captured for `sig` (re-captured when sig
 // changes), or directly when graphs are off or the stream cannot capture
 // (the legacy null stream).
@@ -1481,26 +1534,35 @@ void atomic_batch_exchange(AtomicBatch &b, const double *const *P, const int64_t
   auto kpart = [&](int c, int p, int nw) {
     launch_k_batch(c, p, nw, base + ok[c][p], (const int *)(base + osk[c][p]), nsys(c, p), sk[c][p].back(), s);
   };
-  // 4. autotune the workgroup shape of each class whose composition changed
-  for (int c = 0; c < 4; ++c) {
-    const std::vector<long long> csig{sk[c][0].back(), sk[c][1].back(), nsys(c, 0), nsys(c, 1)};
-    if (sk[c][0].back() + sk[c][1].back() == 0) continue;
-    if (b.nw[c][0] && b.tuned_sig[c] == csig) continue;
-    if (!b.autotune) {
-      b.nw[c][0] = b.nw[c][1] = 4;
-      b.tuned_sig[c] = csig;
-      continue;
-    }
-    ptiles(nullptr);
-    hipEvent_t e0, e1;
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
-    const int shapes[3] = {1, 2, 4};
+  // 4. the workgroup shape of each class and part: tuned once per size bucket
+  // (log2 of the launch's workgroup count) on this device and kept for the
+  // process, so batches whose composition changes every iteration (systems
+  // converging, switching precision) reuse the choice instead of re-timing
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  bool tiles_ready = false;
+  for (int c = 0; c < 4; ++c)
     for (int p = 0; p < 2; ++p) {
-      if (sk[c][p].back() == 0) {
+      const int wgs = sk[c][p].back();
+      if (wgs == 0) continue;
+      if (!b.autotune) {
         b.nw[c][p] = 4;
         continue;
       }
+      const int bucket = 31 - __builtin_clz((unsigned)wgs);
+      const int known = tuned_shape(dev, c, p, bucket);
+      if (known > 0) {
+        b.nw[c][p] = known;
+        continue;
+      }
+      if (!tiles_ready) {
+        ptiles(nullptr);
+        tiles_ready = true;
+      }
+      hipEvent_t e0, e1;
+      hip_check(hipEventCreate(&e0), "hipEventCreate");
+      hip_check(hipEventCreate(&e1), "hipEventCreate");
+      const int shapes[3] = {1, 2, 4};
       float best = 1e30f;
       for (int q = 0; q < 3; ++q) {
         float ms = 0.f;
@@ -1518,11 +1580,10 @@ void atomic_batch_exchange(AtomicBatch &b, const double *const *P, const int64_t
           b.nw[c][p] = shapes[q];
         }
       }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      remember_shape(dev, c, p, bucket, b.nw[c][p]);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    b.tuned_sig[c] = csig;
-  }
   // 5. the launch sequence: zero K, density tiles, cross- and in-element kernels
   std::vector<long long> sig{sF[n], spd.back(), spf.back()};
   for (int c = 0; c < 4; ++c)

@@ -315,6 +315,7 @@ int hf_ctx_destroy(hf_ctx *ctx) {
     if (!ctx) return;
     DeviceGuard g(ctx->device);
     (void)hipStreamSynchronize(ctx->cur);
+    hf_ctx_release_blas(ctx);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
   });
@@ -553,6 +554,25 @@ int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t l
   });
 }
 
+namespace {
+// the radial one-electron matrices of kind `which` (and the shells' l
+// values) on the device, computed and uploaded on first use
+void prepare_one_electron(const hf_atomic *hc, int which) {
+  hf_atomic *h = const_cast<hf_atomic *>(hc);
+  if (h->lval_dev.p && h->one_M[which].p) return;
+  // the host copies stay alive until the stream has consumed them
+  std::vector<int> l(h->host.lval.begin(), h->host.lval.end());
+  std::vector<double> M, X;
+  if (!h->lval_dev.p) h->lval_dev.upload(l, h->ctx->cur);
+  if (!h->one_M[which].p) {
+    h->host.radial_one_electron(which, M, X);
+    h->one_M[which].upload(M, h->ctx->cur);
+    if (which == 1) h->one_X.upload(X, h->ctx->cur);
+  }
+  hip_check(hipStreamSynchronize(h->ctx->cur), "one-electron upload");
+}
+}  // namespace
+
 int hf_atomic_one_electron_dev(const hf_atomic *h, int which, double *dOut, int64_t ld) {
   return guarded([&] {
     need(h != nullptr, "NULL handle");
@@ -560,9 +580,19 @@ int hf_atomic_one_electron_dev(const hf_atomic *h, int which, double *dOut, int6
     need(which >= 0 && which <= 2, "which must be 0 (overlap), 1 (kinetic) or 2 (nuclear)");
     DeviceGuard g(h->ctx->device);
     check_matrix(dOut, ld, h->host.Nbf(), "out");
-    BlockList b;
-    h->host.one_electron_blocks(which, b);
-    scatter_blocks_launch(dOut, ld, h->host.Nbf(), b.meta, b.vals, h->ctx->cur);
+    prepare_one_electron(h, which);
+    shell_diag_launch(dOut, ld, h->host.Nbf(), h->host.Nrad(), h->one_M[which].p,
+                      which == 1 ? h->one_X.p : nullptr, h->lval_dev.p, which == 2 ? -double(h->host.Z) : 1.0,
+                      h->ctx->cur);
+  });
+}
+
+int hf_atomic_one_electron_prepare(hf_atomic *h) {
+  return guarded([&] {
+    need(h != nullptr, "NULL handle");
+    need(h->ctx != nullptr, "handle was created without a device context");
+    DeviceGuard g(h->ctx->device);
+    for (int w = 0; w < 3; ++w) prepare_one_electron(h, w);
   });
 }
 
@@ -708,7 +738,6 @@ int hf_atomic_batch_set_options(hf_atomic_batch *b, int use_graphs, int autotune
     need(b != nullptr, "NULL batch");
     b->b.use_graphs = use_graphs != 0;
     b->b.autotune = autotune != 0;
-    for (auto &t : b->b.tuned_sig) t.clear();
   });
 }
 

@@ -11,7 +11,10 @@ struct hf_ctx {
   hipStream_t own = nullptr;
   hipStream_t cur = nullptr;
   hfg::KernelTimer timer;
+  void *blas = nullptr;  // rocblas_handle of the batched eigensolver, created on first use
 };
+// destroys ctx->blas (capi_eig.hip)
+void hf_ctx_release_blas(hf_ctx *ctx);
 
 struct hf_atomic {
   hf_ctx *ctx = nullptr;
@@ -19,6 +22,10 @@ struct hf_atomic {
   hfg::AtomicDevice dev;
   bool uploaded = false;
   bool rs_uploaded = false;  // range-separated tables on the device
+  // radial one-electron matrices (Nrad x Nrad) on the device, per kind, and
+  // the kinetic l(l+1) factor's X; the shells' l values
+  hfg::DevArray<double> one_M[3], one_X;
+  hfg::DevArray<int> lval_dev;
 };
 
 struct hf_diatomic {

@@ -168,13 +168,14 @@ void xc_upload(hf_dftgrid *g) {
   d.xd.upload(xd, s);
   d.thx.upload(thx, s);
   d.thxt.upload(thxt, s);
+  // gradient tables (GGA): d/dr R (atomic, R = B/r) or d/dmu R (diatomic,
+  // R = B), the inverse scale factors of the three gradient components per
+  // point, sin(d phi), d/dtheta (Theta_a Theta_b), and the transposed pair of
+  // each row-list entry; the host vectors live until the stream sync below
+  std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), gsc((size_t)Nel * nrad * g->lang * 3);
+  std::vector<double> sn((size_t)nd * g->mang), thd((size_t)g->lang * np), thdt((size_t)np * g->lang);
+  std::vector<int> rxT(rb.size());
   {
-    // gradient tables (GGA): d/dr R (atomic, R = B/r) or d/dmu R (diatomic,
-    // R = B), the inverse scale factors of the three gradient components per
-    // point, sin(d phi), d/dtheta (Theta_a Theta_b), and the transposed pair
-    // of each row-list entry
-    std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), gsc((size_t)Nel * nrad * g->lang * 3);
-    std::vector<double> sn((size_t)nd * g->mang), thd((size_t)g->lang * np), thdt((size_t)np * g->lang);
     for (int e = 0; e < Nel; ++e) {
       const Blk v = g->atomic ? rad.rderiv(xq, e) : rad.values(1, xq, e);
       const double h = rad.half_len(e), mid = rad.mid(e);
@@ -208,7 +209,7 @@ void xc_upload(hf_dftgrid *g) {
         thd[(size_t)t * np + x] = v;
         thdt[(size_t)x * g->lang + t] = v;
       }
-    std::vector<int> xid((size_t)Na * Na, -1), rxT(rb.size());
+    std::vector<int> xid((size_t)Na * Na, -1);
     for (int a = 0; a < Na; ++a)
       for (int q = rstart[a]; q < rstart[a + 1]; ++q) xid[(size_t)a * Na + rb[q]] = rx[q];
     for (int a = 0; a < Na; ++a)

@@ -95,6 +95,10 @@ double exchange_tile_budget(size_t held_bytes);
 
 // n x n device matrix from its nonzero blocks (meta: row0, col0, rows, cols,
 // offset per block; vals column-major), zero elsewhere; synchronises s.
+// block-diagonal f M + l (l + 1) X per shell of n radial functions (atomic
+// one-electron matrices; X and lval may be null)
+void shell_diag_launch(double *out, int64_t ld, int64_t nbf, int n, const double *M, const double *X,
+                       const int *lval, double f, hipStream_t s);
 void scatter_blocks_launch(double *out, int64_t ld, int64_t n, const std::vector<long long> &meta,
                            const std::vector<double> &vals, hipStream_t s);
 
@@ -421,13 +425,12 @@ struct AtomicBatch {
   hipGraphExec_t gexec_j = nullptr, gexec_k = nullptr;
   std::vector<long long> sig_j, sig_k;
   bool use_graphs = true;
-  // autotuned waves per workgroup of the exchange kernels, per class
-  // (fp32 * 2 + tiled) and launch part (0 cross-element, 1 in-element);
-  // the signature each choice was made for
+  // waves per workgroup of the exchange kernels used by the last call, per
+  // class (fp32 * 2 + tiled) and launch part (0 cross-element, 1
+  // in-element); autotuned once per size bucket and device for the process
   int nw[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  std::vector<long long> tuned_sig[4];
   bool autotune = true;
-  double tune_ms[4][2][3] = {};  // measured launch times of the 1, 2, 4-wave shapes
+  double tune_ms[4][2][3] = {};  // launch times of the 1, 2, 4-wave shapes, last tuning of this batch
   ~AtomicBatch();
 };
 void atomic_batch_coulomb(AtomicBatch &b, const double *const *P, const int64_t *ldp, double *const *J,

@@ -534,9 +534,9 @@ void BlockList::to_dense(double *out, int64_t ld, int64_t n) const {
 // TwoDBasis.cpp:321-392: every one-electron matrix is block diagonal over the
 // angular shells; overlap S, kinetic T + l(l+1) T_l, nuclear -Z <1/r> (point
 // nucleus at the origin) on each diagonal block.
-void AtomicSetup::one_electron_blocks(int which, BlockList &out) const {
+void AtomicSetup::radial_one_electron(int which, std::vector<double> &M, std::vector<double> &X) const {
   if (which < 0 || which > 2) throw std::invalid_argument("which must be 0, 1 or 2");
-  std::vector<double> M, X;
+  X.clear();
   if (which == 0)
     radial_diag(rad, [&](int e) { return rad.rpow(0, e); }, M);
   else if (which == 1) {
@@ -544,6 +544,11 @@ void AtomicSetup::one_electron_blocks(int which, BlockList &out) const {
     radial_diag(rad, [&](int e) { return rad.kin_l(e); }, X);
   } else
     radial_diag(rad, [&](int e) { return rad.rpow(-1, e); }, M);
+}
+
+void AtomicSetup::one_electron_blocks(int which, BlockList &out) const {
+  std::vector<double> M, X;
+  radial_one_electron(which, M, X);
   const int n = Nrad();
   std::vector<double> blk((size_t)n * n);
   for (int a = 0; a < Nang(); ++a) {

@@ -155,6 +155,9 @@ struct AtomicSetup {
   void nuclear(double *out, int64_t ld) const;
   // the same as their nonzero blocks: which 0 overlap, 1 kinetic, 2 nuclear
   void one_electron_blocks(int which, BlockList &out) const;
+  // the radial factors of those blocks: shell a's block is f M + l_a (l_a + 1) X
+  // with f = 1 (overlap, kinetic) or -Z (nuclear); X only for the kinetic energy
+  void radial_one_electron(int which, std::vector<double> &M, std::vector<double> &X) const;
 };
 
 }  // namespace hfg

@@ -108,6 +108,7 @@ class _Blocks:
         self.S = [S[b][:, b] for b in self.b]
         self.X = [None] * len(self.S)
         self.dev = dev
+        self.ctx = getattr(basis, "ctx", None)
         # shells of each block, in block order: (l, local start, size)
         lval = np.asarray(basis.lval).tolist()
         mval = np.asarray(basis.mval).tolist()
@@ -185,7 +186,7 @@ class _Blocks:
                 groups.append((k, li, u))
         # D (D S D)^-1/2 per distinct group (invh, utils.cpp:121-159)
         ds = [1.0 / torch.sqrt(torch.diagonal(Sp)) for Sp in uniq]
-        sols = batched_eigh([d[:, None] * Sp * d[None, :] for d, Sp in zip(ds, uniq)])
+        sols = batched_eigh([d[:, None] * Sp * d[None, :] for d, Sp in zip(ds, uniq)], self.ctx)
         Xu = []
         for d, (e, v) in zip(ds, sols):
             ev = torch.as_tensor(e, dtype=v.dtype, device=v.device)
@@ -350,7 +351,7 @@ class _Blocks:
         if late:  # the occupied levels rose past a bound: solve those after all
             self.solves += len(late)
             self.skipped_solves -= len(late)
-            take(late, batched_eigh([plan["all"][i] for i in late]))
+            take(late, batched_eigh([plan["all"][i] for i in late], self.ctx))
             order = aufbau()
             homo = order[nocc - 1][0] if nocc > 0 else -np.inf
         nsub = [0] * len(subs)
@@ -383,7 +384,7 @@ class _Blocks:
 
     def density(self, Fb, nocc, weight, like, slot=0):
         plan = self.prepare(Fb, slot)
-        return self.finish(plan, batched_eigh(plan["mats"]), nocc, weight, like)
+        return self.finish(plan, batched_eigh(plan["mats"], self.ctx), nocc, weight, like)
 
     def commutator(self, Fb, Pb):
         """Orthonormal-basis commutators X_k' (F P S - S P F) X_k per block."""
@@ -395,17 +396,26 @@ class _Blocks:
 
 
 PAD_RATIO = 0.75
+# Smallest stack size handed to the batched eigensolver.  The rocSOLVER build
+# that ships with this PyTorch (and that the library binds to once torch is
+# loaded, the two sharing a soname) solves batches of n <= 64 matrices an
+# order of magnitude slower than n = 72 (178 blocks: 50 ms at n = 53, 2 ms at
+# n = 71), so smaller blocks are padded up.
+EIGH_MIN_N = 72
 
 
-def batched_eigh(mats):
+def batched_eigh(mats, ctx=None):
     """Eigenpairs of symmetric matrices, ascending.  Matrices whose sizes lie
     within PAD_RATIO of the largest of a group share one batched eigensolve:
     each is padded to the group size with a diagonal c = 2 |A|_F + 1 above its
     spectrum and no coupling, so its eigenpairs are the first n of the padded
     matrix (the Householder reduction keeps the zero coupling exact).  The
     eigensolves are launch-bound at these sizes, so fewer, larger batches
-    win.  All eigenvalues come to the host in one transfer."""
+    win.  With a Context the stacks go to hf_eigh_batched (rocSOLVER's
+    batched dsyevd), else to torch.linalg.eigh.  All eigenvalues (and the
+    solver statuses) come to the host in one transfer."""
     import torch
+    from . import linalg
     if not mats:
         return []
     order = sorted(range(len(mats)), key=lambda j: -mats[j].shape[0])
@@ -416,9 +426,11 @@ def batched_eigh(mats):
         else:
             groups.append([j])
     vals, vecs = [None] * len(mats), [None] * len(mats)
-    evs = []
+    evs, infos = [], []
     for members in groups:
-        N = mats[members[0]].shape[0]
+        # stacks below EIGH_MIN_N are padded up to it: the batched eigensolver
+        # is an order of magnitude slower below that size
+        N = max(mats[members[0]].shape[0], EIGH_MIN_N)
         A0 = mats[members[0]]
         if all(mats[j].shape[0] == N for j in members):
             stack = torch.stack([mats[j] for j in members])
@@ -431,12 +443,20 @@ def batched_eigh(mats):
                 if n < N:
                     idx = torch.arange(n, N, device=A.device)
                     stack[q, idx, idx] = 2.0 * torch.linalg.matrix_norm(A) + 1.0
-        e, v = torch.linalg.eigh(stack)
+        if ctx is not None and stack.is_cuda:
+            e, v, info = linalg.eigh(stack, ctx)
+            infos.append(info)
+        else:
+            e, v = torch.linalg.eigh(stack)
         for q, j in enumerate(members):
             n = mats[j].shape[0]
             vecs[j] = v[q, :n, :n]
         evs.append((members, e))
-    host = torch.cat([e.reshape(-1) for _, e in evs]).cpu().numpy()
+    parts = [e.reshape(-1) for _, e in evs]
+    nval = sum(p.numel() for p in parts)
+    host = torch.cat(parts + [i.to(torch.float64) for i in infos]).cpu().numpy()
+    if np.any(host[nval:] != 0):
+        raise RuntimeError("batched eigensolver did not converge")
     off = 0
     for members, e in evs:
         N = e.shape[1]
@@ -764,7 +784,7 @@ class SCFEngine:
         for dev in devs:
             _sync(dev)
         t1 = time.perf_counter()
-        sols = batched_eigh([m for r in reqs for m in r[2]["mats"]])
+        sols = batched_eigh([m for r in reqs for m in r[2]["mats"]], self.active[0].bs.ctx if self.active else None)
         for dev in devs:
             _sync(dev)
         t2 = time.perf_counter()

@@ -135,8 +135,15 @@ int hf_atomic_get_cache(const hf_atomic *h, int kind, int L, int iel, int dims[2
  * 1 kinetic (:354), 2 nuclear attraction (:377). */
 int hf_atomic_one_electron(const hf_atomic *h, int which, double *out, int64_t ld);
 /* The same into a device matrix on the handle's stream (needs a context):
- * only the nonzero angular blocks are built on the host and sent. */
+ * the radial factors (Nrad x Nrad) are built on the host and sent once per
+ * handle and kind, then every call expands them into the block-diagonal
+ * matrix on the device. */
 int hf_atomic_one_electron_dev(const hf_atomic *h, int which, double *dOut, int64_t ld);
+/* Build and send those radial factors for all three kinds now (host work
+ * that callers preparing many bases can spread over threads, one handle per
+ * thread); synchronises the handle's stream.  No counterpart in the
+ * reference, which assembles the matrices on the host when asked. */
+int hf_atomic_one_electron_prepare(hf_atomic *h);
 
 /* Range-separated two-electron caches: the Yukawa kernel exp(-lambda r12)/r12
  * (TwoDBasisT::compute_yukawa, TwoDBasis.h:183, TwoDBasis.cpp:738-760) or
@@ -197,12 +204,15 @@ int hf_atomic_batch_create(hf_ctx *ctx, hf_atomic *const *handles, int n, hf_ato
 int hf_atomic_batch_destroy(hf_atomic_batch *b);
 /* use_graphs: capture and replay the launch sequences (default 1);
  * autotune: time the 1-, 2- and 4-wave workgroup shapes of the exchange
- * kernels on the batch and keep the fastest (default 1), re-tuned when the
- * batch's exchange grid changes; 0 fixes the 4-wave shape. */
+ * kernels and keep the fastest (default 1), once per size bucket (log2 of
+ * the launch's workgroup count), class and device for the whole process, so
+ * batches whose composition changes reuse the choice; 0 fixes the 4-wave
+ * shape. */
 int hf_atomic_batch_set_options(hf_atomic_batch *b, int use_graphs, int autotune);
-/* The autotuner's state: nw[c*2 + p] waves per workgroup chosen for class c
- * (fp32 * 2 + tiled density) and part p (0 across elements, 1 inside), and
- * ms[(c*2 + p)*3 + q] the launch times it measured for 1, 2, 4 waves. */
+/* The autotuner's state: nw[c*2 + p] waves per workgroup the last call used
+ * for class c (fp32 * 2 + tiled density) and part p (0 across elements, 1
+ * inside), and ms[(c*2 + p)*3 + q] the launch times this batch last measured
+ * for 1, 2, 4 waves (zero when every shape it needed was already tuned). */
 int hf_atomic_batch_tuning(const hf_atomic_batch *b, int nw[8], double ms[24]);
 /* J_s = J(P_s) for every system s (device pointers, asynchronous on the
  * context stream).  Arrays of n pointers and leading dimensions. */
@@ -281,6 +291,18 @@ int hf_dftgrid_eval_fxc_pol_dev(hf_dftgrid *g, int x_func, int c_func, const dou
 /* Device-pointer form; synchronises the context stream to return out. */
 int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double *dP, int64_t ldp, double *dH,
                             int64_t ldh, double thr, double out[3]);
+
+/* ------------------------------------------------- batched eigensolver */
+/* Eigenvalues (ascending, dW[b*n + i]) and eigenvectors (overwriting each
+ * column-major A_b = dA + b*strideA with leading dimension lda) of a batch of
+ * symmetric n x n matrices (lower triangles read), on the context stream
+ * with rocSOLVER's dsyevd_strided_batched; dE is n*batch doubles of
+ * workspace, dinfo batch ints (0: converged).  Asynchronous; the SCF drivers
+ * stack their Fock blocks into one call.  Replaces the per-block host
+ * eigensolves of eig_gsym / eig_gsym_sub (src/general/scf_helpers.cpp:41-80),
+ * applied to the orthogonalised blocks. */
+int hf_eigh_batched(hf_ctx *ctx, int n, double *dA, int64_t lda, int64_t strideA, double *dW, double *dE,
+                    int *dinfo, int batch);
 
 /* --------------------------------------------------------- diatomic basis */
 
