@@ -402,12 +402,12 @@ def atoms_bench(H, ctx, n, seed=1004, nseq=None, budget_bytes=96 << 30):
     setup = time.perf_counter() - t
     nbf = [s.info["Nbf"] for s in systems]
     log(f"{n} atoms, Nbf {min(nbf)}..{max(nbf)}; setup {setup:.1f}s")
-    # warm-up, untimed, in both precisions over the first cohort's worth of
-    # atoms: the one-time costs (workgroup shapes tuned per size bucket, the
-    # beta-spin handles and their device tables, graph captures) fall here
-    # and not on whichever timed leg runs first
+    # warm-up, untimed, in both precisions over the same atoms: the one-time
+    # costs (workgroup shapes tuned per size bucket, the beta-spin handles and
+    # their device tables, graph captures) fall here and not on whichever
+    # timed leg runs first
     t = time.perf_counter()
-    warm = systems[:min(n, 128)]
+    warm = systems
     for prec in ("mixed", "fp64"):
         solve_cohorts(warm, ctx, budget_bytes=budget_bytes, precision=prec, conv=1e-8,
                       occupations=[s.occ for s in warm])
@@ -419,7 +419,7 @@ def atoms_bench(H, ctx, n, seed=1004, nseq=None, budget_bytes=96 << 30):
                        f"lmax = mmax 0..4, Hund spins; SCF to dE < 1e-8 Eh", "setup_s": setup,
            "setup": "bases on 16 host threads, two-electron setups on the GPU (hf_atomic_compute_tei_device)",
            "warmup_s": warm_s,
-           "warmup": f"untimed: the first {len(warm)} atoms solved once in each precision before the timed legs",
+           "warmup": "untimed: every atom solved once in each precision before the timed legs",
            "Nbf_range": [min(nbf), max(nbf)],
            "occupations": f"{pinned} of {n} with fixed per-(l, m) occupations (the reference's --readocc): "
                           f"{pinned - restricted} of their Madelung/Hund ground-state configuration, {restricted} "
