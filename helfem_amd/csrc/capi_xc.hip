@@ -174,6 +174,8 @@ void xc_upload(hf_dftgrid *g) {
   // each row-list entry; the host vectors live until the stream sync below
   std::vector<double> dR((size_t)Nel * nrad * TILE, 0.0), gsc((size_t)Nel * nrad * g->lang * 3);
   std::vector<double> sn((size_t)nd * g->mang), thd((size_t)g->lang * np), thdt((size_t)np * g->lang);
+  std::vector<double> thdd((size_t)g->lang * np), thddt((size_t)np * g->lang), thmm((size_t)g->lang * np),
+      thmmt((size_t)np * g->lang);
   std::vector<int> rxT(rb.size());
   {
     for (int e = 0; e < Nel; ++e) {
@@ -192,7 +194,9 @@ void xc_upload(hf_dftgrid *g) {
           } else {
             const double Rh = g->diatomic->Rh, shmu = std::sinh(x);
             c3[0] = c3[1] = 1.0 / (Rh * std::sqrt(shmu * shmu + sth * sth));
-            c3[2] = g->full3d ? 1.0 / (Rh * shmu * sth) : 0.0;
+            // 1/h_phi: the phi gradient of a pure-m density vanishes (d = 0), but
+            // its kinetic energy density keeps the m^2 / h_phi^2 term
+            c3[2] = 1.0 / (Rh * shmu * sth);
           }
         }
       }
@@ -208,6 +212,11 @@ void xc_upload(hf_dftgrid *g) {
         const double v = dth[A] * th[B] + th[A] * dth[B];
         thd[(size_t)t * np + x] = v;
         thdt[(size_t)x * g->lang + t] = v;
+        const double vd = dth[A] * dth[B], vm = double(mv[da[x]] * mv[db[x]]) * th[A] * th[B];
+        thdd[(size_t)t * np + x] = vd;
+        thddt[(size_t)x * g->lang + t] = vd;
+        thmm[(size_t)t * np + x] = vm;
+        thmmt[(size_t)x * g->lang + t] = vm;
       }
     std::vector<int> xid((size_t)Na * Na, -1);
     for (int a = 0; a < Na; ++a)
@@ -223,11 +232,15 @@ void xc_upload(hf_dftgrid *g) {
     d.thd.upload(thd, s);
     d.thdt.upload(thdt, s);
     d.rxT.upload(rxT, s);
+    d.thdd.upload(thdd, s);
+    d.thddt.upload(thddt, s);
+    d.thmm.upload(thmm, s);
+    d.thmmt.upload(thmmt, s);
   }
   d.D.alloc((size_t)Nel * np * nrad);
   d.W.alloc((size_t)Nel * np * nrad);
-  d.part.alloc((size_t)2 * Nel * nrad);
-  d.h_part.alloc((size_t)2 * Nel * nrad);
+  d.part.alloc((size_t)3 * Nel * nrad);
+  d.h_part.alloc((size_t)3 * Nel * nrad);
   hip_check(hipStreamSynchronize(s), "dft grid upload");
   g->uploaded = true;
 }
@@ -237,10 +250,11 @@ void xc_upload(hf_dftgrid *g) {
 // through the general point kernel (xc_func.hpp)
 void check_functionals(const hf_dftgrid *g, int x_func, int c_func) {
   (void)g;
-  if (!xcf::exchange_ok(x_func)) throw Error(HF_EINVAL, "unsupported exchange functional (0, 1 = lda_x, 101 = gga_x_pbe)");
+  if (!xcf::exchange_ok(x_func))
+    throw Error(HF_EINVAL, "unsupported exchange functional (0, 1 = lda_x, 101 = gga_x_pbe, 202 = mgga_x_tpss)");
   if (!xcf::correlation_ok(c_func))
     throw Error(HF_EINVAL, "unsupported correlation functional (0, 7 = lda_c_vwn, 12 = lda_c_pw, 13 = lda_c_pw_mod, "
-                           "130 = gga_c_pbe)");
+                           "130 = gga_c_pbe, 231 = mgga_c_tpss)");
 }
 
 // Pb == nullptr: restricted; else spin-polarized (Hb gets the beta potential,
@@ -255,6 +269,10 @@ void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ld
       throw Error(HF_EINVAL, "angular basis and DFT grid too large for one workgroup's LDS in the GGA evaluation");
     d.E.alloc(d.D.n);
     d.Y.alloc(d.W.n);
+    if (xc_tau(x_func, c_func)) {
+      d.F.alloc(d.D.n);
+      d.Z.alloc(d.W.n);
+    }
   }
   if (Pb) {
     if (!gen && !d.lds_pol_ok) throw Error(HF_EINVAL, "DFT grid too large for the spin-polarized evaluation");
@@ -264,20 +282,25 @@ void eval_fxc(hf_dftgrid *g, int x_func, int c_func, const double *P, int64_t ld
       d.Eb.alloc(d.D.n);
       d.Yb.alloc(d.W.n);
     }
+    if (xc_tau(x_func, c_func)) {
+      d.Fb.alloc(d.D.n);
+      d.Zb.alloc(d.W.n);
+    }
   }
   xc_grid_launch(d, x_func, c_func, thr, P, ldp, Pb, ldpb, H, ldh, Hb, ldhb, s, &g->ctx->timer);
   if (Pb && !beta) fill_zero_launch(Hb, d.Nbf, d.Nbf, ldhb, s);
-  hip_check(hipMemcpyAsync(d.h_part.p, d.part.p, sizeof(double) * 2 * d.Nel * d.nrad, hipMemcpyDeviceToHost, s),
+  hip_check(hipMemcpyAsync(d.h_part.p, d.part.p, sizeof(double) * 3 * d.Nel * d.nrad, hipMemcpyDeviceToHost, s),
             "copy XC partial sums");
   hip_check(hipStreamSynchronize(s), "XC sync");
-  double exc = 0.0, nel = 0.0;
+  double exc = 0.0, nel = 0.0, ekin = 0.0;
   for (int q = 0; q < d.Nel * d.nrad; ++q) {
-    exc += d.h_part.p[2 * q];
-    nel += d.h_part.p[2 * q + 1];
+    exc += d.h_part.p[3 * q];
+    nel += d.h_part.p[3 * q + 1];
+    ekin += d.h_part.p[3 * q + 2];
   }
   out[0] = exc;
   out[1] = nel;
-  out[2] = 0.0;  // kinetic energy density integral: zero for LDA
+  out[2] = ekin;  // integral of the kinetic energy density: zero without a meta-GGA
 }
 
 hf_dftgrid *ready(hf_dftgrid *g) {

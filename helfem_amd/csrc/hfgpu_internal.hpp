@@ -357,7 +357,11 @@ struct XCGridDevice {
   DevArray<int> rxT;         // per row-list entry: pair index of the transposed pair (b, a)
   DevArray<double> E, Y;     // E [Nel][nrad][npair] (d/dr on the row), Y [Nel][npair][nrad]
   DevArray<double> Eb, Yb;
-  DevArray<double> part;     // [Nel*nrad][2]: Exc and Nel partial sums
+  // kinetic energy density (meta-GGA): Theta'_a Theta'_b and m_a m_b Theta_a Theta_b
+  // ([lang][npair] and [npair][lang]), F = R' P R' [Nel][nrad][npair], Z [Nel][npair][nrad]
+  DevArray<double> thdd, thddt, thmm, thmmt;
+  DevArray<double> F, Z, Fb, Zb;
+  DevArray<double> part;     // [Nel*nrad][3]: Exc, Nel and Ekin (integral of tau) partial sums
   PinnedArray<double> h_part;
   DevArray<double> hostP, hostOut;
 };
@@ -450,6 +454,8 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
 bool xc_generic(int xf, int cf);
 // dynamic LDS of the point kernel that xc_grid_launch will use
 size_t xc_point_lds(const XCGridDevice &g, int xf, int cf, bool pol);
+// true when the pair needs the kinetic energy density (meta-GGA)
+bool xc_tau(int xf, int cf);
 
 // launchers (diatomic_jk.hip)
 void diatomic_coulomb_launch(const DiatomicDevice &d, const double *P, int64_t ldp, double *J, int64_t ldj,

@@ -1,17 +1,21 @@
 // Exchange-correlation functionals by libxc id, for the device (and host):
 //   1 lda_x, 7 lda_c_vwn (VWN5), 12 lda_c_pw (PW92 as printed), 13 lda_c_pw_mod
-//   (libxc's more precise PW92 constants), 101 gga_x_pbe, 130 gga_c_pbe
+//   (libxc's more precise PW92 constants), 101 gga_x_pbe, 130 gga_c_pbe,
+//   202 mgga_x_tpss, 231 mgga_c_tpss
 // from the published formulas (Slater; Vosko-Wilk-Nusair, Can. J. Phys. 58,
 // 1200 (1980); Perdew-Wang, PRB 45, 13244 (1992); Perdew-Burke-Ernzerhof,
-// PRL 77, 3865 (1996)) with libxc's constants and thresholds: no output below
-// the density threshold, 1 +- zeta floored at DBL_EPSILON, sigma_aa and
-// sigma_bb floored at 1e-40, sigma_ab clamped to +-(sigma_aa + sigma_bb)/2; a
-// polarized spin below the threshold adds no PBE exchange.
+// PRL 77, 3865 (1996); Tao-Perdew-Staroverov-Scuseria, PRL 91, 146401 (2003))
+// with libxc's constants and thresholds: no output below the density
+// threshold, 1 +- zeta floored at DBL_EPSILON, sigma_aa and sigma_bb floored
+// at 1e-40, sigma_ab clamped to +-(sigma_aa + sigma_bb)/2; a polarized spin
+// below the threshold adds no PBE or TPSS exchange; for TPSS tau floored at
+// 1e-20 and sigma_ss capped at 8 rho_s tau_s (z <= 1).
 //
 // Each functional is one energy density per volume f(rho_a, rho_b, sigma_aa,
-// sigma_ab, sigma_bb) written once over a scalar type; the potentials come
-// from forward-mode differentiation (a value and one directional derivative,
-// one pass per input), so the derivative formulas are never written by hand.
+// sigma_ab, sigma_bb, tau_a, tau_b) written once over a scalar type; the
+// potentials come from forward-mode differentiation (a value and one
+// directional derivative, one pass per input), so the derivative formulas
+// are never written by hand.
 #pragma once
 #include <cfloat>
 #include <cmath>
@@ -117,10 +121,101 @@ template <class T> __host__ __device__ inline T pbe_x(T n, T s2n) {
   return -0.75 * ::cbrt(3.0 / kPi) * n43 * (1.0 + kappa - kappa / (1.0 + (mu / kappa) * s2));
 }
 
+// PBE correlation per volume on PW92 (modified constants)
+template <class T> __host__ __device__ inline T pbe_c(T ra, T rb, T saa, T sab, T sbb) {
+  const double beta = 0.06672455060314922, gamma = (1.0 - 0.69314718055994530942) / (kPi * kPi);
+  saa = floor_at(saa, 1e-40);
+  sbb = floor_at(sbb, 1e-40);
+  const T sav = 0.5 * (saa + sbb);
+  if (val(sab) > val(sav)) sab = sav;
+  if (val(sab) < -val(sav)) sab = -sav;
+  const T rho = ra + rb;
+  const T z = (ra - rb) / rho;
+  const T opz = floor_at(1.0 + z, DBL_EPSILON), omz = floor_at(1.0 - z, DBL_EPSILON);
+  const T rs = cbrt(3.0 / (4.0 * kPi * rho));
+  const T ec = pw92(true, rs, z, opz, omz);
+  const T phi = 0.5 * (p23(opz) + p23(omz)), phi3 = phi * phi * phi;
+  const T ks2 = (4.0 / kPi) * cbrt(3.0 * kPi * kPi * rho);
+  const T t2 = (saa + 2.0 * sab + sbb) / (4.0 * phi * phi * ks2 * rho * rho);
+  const T A = (beta / gamma) / (exp(-ec / (gamma * phi3)) - 1.0);
+  const T At2 = A * t2;
+  const T H = gamma * phi3 * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+// TPSS exchange per volume of an unpolarized density n, |grad n|^2 = s2n,
+// kinetic energy density tn
+template <class T> __host__ __device__ inline T tpss_x(T n, T s2n, T tn) {
+  const double kappa = 0.804, b = 0.40, c = 1.59096, e = 1.537, mu = 0.21951;
+  const double ckf = ::cbrt(9.0 * kPi * kPi * kPi * kPi);  // (3 pi^2)^(2/3)
+  tn = floor_at(tn, 1e-20);
+  if (val(s2n) > val(8.0 * n * tn)) s2n = 8.0 * n * tn;
+  const T n43 = p43(n);
+  const T p = s2n / (4.0 * ckf * n43 * n43);
+  const T tw = s2n / (8.0 * n);
+  const T z = tw / tn;
+  const T alpha = (tn - tw) / (0.3 * ckf * n43 * cbrt(n));
+  const T qb = 0.45 * (alpha - 1.0) / sqrt(1.0 + b * alpha * (alpha - 1.0)) + (2.0 / 3.0) * p;
+  const T z35 = 0.6 * z, zz = 1.0 + z * z, se = ::sqrt(e) * p + 1.0;
+  const T x = ((10.0 / 81.0 + c * z * z / (zz * zz)) * p + (146.0 / 2025.0) * qb * qb -
+               (73.0 / 405.0) * qb * sqrt(0.5 * z35 * z35 + 0.5 * p * p) + (100.0 / 6561.0 / kappa) * p * p +
+               (20.0 / 81.0) * ::sqrt(e) * z35 * z35 + e * mu * p * p * p) /
+              (se * se);
+  return -0.75 * ::cbrt(3.0 / kPi) * n43 * (1.0 + kappa - kappa / (1.0 + x / kappa));
+}
+
+// TPSS correlation per volume: revPKZB built on PBE correlation
+template <class T>
+__host__ __device__ inline T tpss_c(T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr) {
+  const double d = 2.8;
+  ta = floor_at(ta, 1e-20);
+  tb = floor_at(tb, 1e-20);
+  saa = floor_at(saa, 1e-40);
+  sbb = floor_at(sbb, 1e-40);
+  if (val(saa) > val(8.0 * ra * ta)) saa = 8.0 * ra * ta;
+  if (val(sbb) > val(8.0 * rb * tb)) sbb = 8.0 * rb * tb;
+  const T sav = 0.5 * (saa + sbb);
+  if (val(sab) > val(sav)) sab = sav;
+  if (val(sab) < -val(sav)) sab = -sav;
+  const T zero = ra * 0.0;
+  const T n = ra + rb;
+  const T z = (saa + 2.0 * sab + sbb) / (8.0 * n * (ta + tb));
+  const T zeta = (ra - rb) / n;
+  const T opz = floor_at(1.0 + zeta, DBL_EPSILON), omz = floor_at(1.0 - zeta, DBL_EPSILON);
+  const T eps = pbe_c(ra, rb, saa, sab, sbb) / n;
+  T sum = zero;
+  if (val(ra) >= thr) {
+    T ea = pbe_c(ra, zero, saa, zero, zero) / ra;
+    if (val(ea) < val(eps)) ea = eps;
+    sum = sum + ra / n * ea;
+  }
+  if (val(rb) >= thr) {
+    T eb = pbe_c(rb, zero, sbb, zero, zero) / rb;
+    if (val(eb) < val(eps)) eb = eps;
+    sum = sum + rb / n * eb;
+  }
+  const T gz2 = (omz * omz * saa - 2.0 * omz * opz * sab + opz * opz * sbb) / (n * n);
+  const T xi2 = gz2 / (4.0 * p23(3.0 * kPi * kPi * n));
+  const T z2 = zeta * zeta;
+  const T C0 = 0.53 + z2 * (0.87 + z2 * (0.50 + 2.26 * z2));
+  const T den = 1.0 + 0.5 * xi2 * (1.0 / p43(opz) + 1.0 / p43(omz));
+  const T den2 = den * den;
+  const T C = C0 / (den2 * den2);
+  const T erev = eps * (1.0 + C * z * z) - (1.0 + C) * z * z * sum;
+  return n * erev * (1.0 + d * erev * z * z * z);
+}
+
 // energy per volume of functional `id` (0: none)
 template <class T>
-__host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
+__host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol) {
   if (id == 0) return ra * 0.0;
+  if (id == 202) {
+    T e = ra * 0.0;
+    if (!pol || val(ra) >= thr) e = e + 0.5 * tpss_x(2.0 * ra, 4.0 * floor_at(saa, 1e-40), 2.0 * ta);
+    if (!pol || val(rb) >= thr) e = e + 0.5 * tpss_x(2.0 * rb, 4.0 * floor_at(sbb, 1e-40), 2.0 * tb);
+    return e;
+  }
+  if (id == 231) return tpss_c(ra, rb, saa, sab, sbb, ta, tb, thr);
   if (id == 1) {
     T e = ra * 0.0;
     if (val(ra) > 0.0) e = e + p43(ra);
@@ -148,62 +243,49 @@ __host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, dou
   }
   if (id == 12 || id == 13) return rho * pw92(id == 13, rs, z, opz, omz);
   // 130: PBE correlation
-  const double beta = 0.06672455060314922, gamma = (1.0 - 0.69314718055994530942) / (kPi * kPi);
-  saa = floor_at(saa, 1e-40);
-  sbb = floor_at(sbb, 1e-40);
-  const T sav = 0.5 * (saa + sbb);
-  if (val(sab) > val(sav)) sab = sav;
-  if (val(sab) < -val(sav)) sab = -sav;
-  const T ec = pw92(true, rs, z, opz, omz);
-  const T phi = 0.5 * (p23(opz) + p23(omz)), phi3 = phi * phi * phi;
-  const T ks2 = (4.0 / kPi) * cbrt(3.0 * kPi * kPi * rho);
-  const T t2 = (saa + 2.0 * sab + sbb) / (4.0 * phi * phi * ks2 * rho * rho);
-  const T A = (beta / gamma) / (exp(-ec / (gamma * phi3)) - 1.0);
-  const T At2 = A * t2;
-  const T H = gamma * phi3 * log(1.0 + (beta / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
-  return rho * (ec + H);
+  return pbe_c(ra, rb, saa, sab, sbb);
 }
 
-__host__ __device__ inline bool is_gga(int id) { return id == 101 || id == 130; }
-__host__ __device__ inline bool exchange_ok(int id) { return id == 0 || id == 1 || id == 101; }
+__host__ __device__ inline bool is_gga(int id) { return id == 101 || id == 130 || id == 202 || id == 231; }
+__host__ __device__ inline bool is_mgga(int id) { return id == 202 || id == 231; }
+__host__ __device__ inline bool exchange_ok(int id) { return id == 0 || id == 1 || id == 101 || id == 202; }
 __host__ __device__ inline bool correlation_ok(int id) {
-  return id == 0 || id == 7 || id == 12 || id == 13 || id == 130;
+  return id == 0 || id == 7 || id == 12 || id == 13 || id == 130 || id == 231;
 }
 
 template <class T>
-__host__ __device__ inline T total(int xf, int cf, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
-  return energy(xf, ra, rb, saa, sab, sbb, thr, pol) + energy(cf, ra, rb, saa, sab, sbb, thr, pol);
+__host__ __device__ inline T total(int xf, int cf, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr,
+                                   bool pol) {
+  return energy(xf, ra, rb, saa, sab, sbb, ta, tb, thr, pol) + energy(cf, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
 }
 
-// Unpolarized: rho, sigma = |grad rho|^2 -> energy per volume f, vrho, vsigma.
-__host__ __device__ inline void eval_unpol(int xf, int cf, double rho, double sigma, double thr, bool gga, double &f,
-                                           double &vrho, double &vsigma) {
-  f = vrho = vsigma = 0.0;
-  if (!(rho >= thr)) return;
-  Dl r = mk(rho, 1.0), s = mk(sigma);
-  Dl e = total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, thr, false);
-  f = e.v;
-  vrho = e.d;
-  if (gga) {
-    r = mk(rho);
-    s = mk(sigma, 1.0);
-    e = total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, thr, false);
-    vsigma = e.d;
+// Unpolarized: x = (rho, sigma = |grad rho|^2, tau) -> energy per volume f and
+// v[k] = df/dx_k (sigma only with gga, tau only with mgga).
+__host__ __device__ inline void eval_unpol(int xf, int cf, const double x[3], double thr, bool gga, bool mgga, double &f,
+                                           double v[3]) {
+  f = v[0] = v[1] = v[2] = 0.0;
+  if (!(x[0] >= thr)) return;
+  const int nk = mgga ? 3 : gga ? 2 : 1;
+  for (int k = 0; k < nk; ++k) {
+    const Dl r = mk(x[0], k == 0 ? 1.0 : 0.0), s = mk(x[1], k == 1 ? 1.0 : 0.0), t = mk(x[2], k == 2 ? 1.0 : 0.0);
+    const Dl e = total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, 0.5 * t, 0.5 * t, thr, false);
+    f = e.v;
+    v[k] = e.d;
   }
 }
 
-// Polarized: x = (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb) -> f and
-// v[k] = df/dx_k (the sigma derivatives only with gga).
-__host__ __device__ inline void eval_pol(int xf, int cf, const double x[5], double thr, bool gga, double &f,
-                                         double v[5]) {
+// Polarized: x = (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb, tau_a, tau_b)
+// -> f and v[k] = df/dx_k (the sigma derivatives only with gga, tau with mgga).
+__host__ __device__ inline void eval_pol(int xf, int cf, const double x[7], double thr, bool gga, bool mgga,
+                                         double &f, double v[7]) {
   f = 0.0;
-  for (int k = 0; k < 5; ++k) v[k] = 0.0;
+  for (int k = 0; k < 7; ++k) v[k] = 0.0;
   if (!(x[0] + x[1] >= thr)) return;
-  const int nk = gga ? 5 : 2;
+  const int nk = mgga ? 7 : gga ? 5 : 2;
   for (int k = 0; k < nk; ++k) {
-    Dl y[5];
-    for (int j = 0; j < 5; ++j) y[j] = mk(x[j], j == k ? 1.0 : 0.0);
-    const Dl e = total(xf, cf, y[0], y[1], y[2], y[3], y[4], thr, true);
+    Dl y[7];
+    for (int j = 0; j < 7; ++j) y[j] = mk(x[j], j == k ? 1.0 : 0.0);
+    const Dl e = total(xf, cf, y[0], y[1], y[2], y[3], y[4], y[5], y[6], thr, true);
     f = e.v;
     v[k] = e.d;
   }

@@ -37,6 +37,15 @@
 //   Y(x)  = sum_t Theta_a Theta_b Vr_d,   x' = (b, a)
 // with V_d, Vr_d, Vtheta_d = sum_k (w vrho, c_1 U_1, c_2 U_2) cos(d phi_k) and
 // Vphi_d = sum_k c_3 U_3 sin(d phi_k).
+//
+// Meta-GGA (tau; dftgrid.cpp:89-107, 336-349; dftgrid_purem.cpp:287-318,
+// 482-494): tau = 1/2 sum_c c_c^2 Re sum P d_c chi d_c chi*, per pair
+//   radial  F(x) = sum_rs R'_r P R'_s  with Theta_a Theta_b,
+//   theta   D(x) with Theta'_a Theta'_b,   phi  D(x) with m_a m_b Theta_a Theta_b,
+// and H += sum_c (w vtau/2) c_c^2 Re d_c chi_u d_c chi_v*: with
+// S_d = sum_k (w vtau / 2) cos(d phi_k), per pair
+//   H += sum_i R'_r R'_s Z(x) + R_r R_s sum_t (c_2^2 Theta'_a Theta'_b + c_3^2 m_a m_b Theta_a Theta_b) S_d,
+//   Z(x) = sum_t c_1^2 Theta_a Theta_b S_d.
 #include "hfgpu_internal.hpp"
 #include "device_common.hpp"
 #include "xc_func.hpp"
@@ -149,6 +158,9 @@ struct XCArgs {
   const double *dR, *gsc, *sn, *thd, *thdt;
   const int *rxT;
   double *E, *Y, *Eb, *Yb;
+  // kinetic energy density terms (meta-GGA)
+  const double *thdd, *thddt, *thmm, *thmmt;
+  double *F, *Z, *Fb, *Zb;
 };
 
 // One workgroup per (element, shell a).  The pairs (a, b) of row a go in
@@ -162,19 +174,20 @@ constexpr int XC_RS = TILE + 1;
 
 __host__ __device__ inline int xc_pairs_per_group(int nrad) { return min(8, max(1, NT / nrad)); }
 
-template <bool GGA>
+// MODE 0: D only (LDA); 1: also E (GGA); 2: also F = sum R'_r P R'_s (meta-GGA)
+template <int MODE>
 __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
   const int e = blockIdx.x / A.Nang, a = blockIdx.x - e * A.Nang;
   const int t = threadIdx.x;
   const int ni = A.ni[e], f0 = A.first[e];
   const int J = xc_pairs_per_group(A.nrad);
   extern __shared__ double sm[];
-  double *sR = sm;                                    // [nrad][XC_RS]
-  double *sdR = sm + A.nrad * XC_RS;                  // [nrad][XC_RS] with GGA
-  double *sP = sm + (GGA ? 2 : 1) * A.nrad * XC_RS;   // [J][TILE2]
+  double *sR = sm;                                        // [nrad][XC_RS]
+  double *sdR = sm + A.nrad * XC_RS;                      // [nrad][XC_RS] with GGA
+  double *sP = sm + (MODE ? 2 : 1) * A.nrad * XC_RS;      // [J][TILE2]
   for (int q = t; q < A.nrad * TILE; q += NT) {
     sR[(q >> 4) * XC_RS + (q & 15)] = A.R[(size_t)e * A.nrad * TILE + q];
-    if (GGA) sdR[(q >> 4) * XC_RS + (q & 15)] = A.dR[(size_t)e * A.nrad * TILE + q];
+    if (MODE) sdR[(q >> 4) * XC_RS + (q & 15)] = A.dR[(size_t)e * A.nrad * TILE + q];
   }
   const int q0 = A.rstart[a], q1 = A.rstart[a + 1];
   for (int qb = q0; qb < q1; qb += J) {
@@ -197,18 +210,23 @@ __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
       double ri[TILE];
 #pragma unroll
       for (int s = 0; s < TILE; ++s) ri[s] = Ri[s];
-      double d = 0.0, eg = 0.0;
+      double d = 0.0, eg = 0.0, fk = 0.0;
 #pragma unroll
       for (int r = 0; r < TILE; ++r) {
-        double tr = 0.0;
+        double tr = 0.0, trd = 0.0;
 #pragma unroll
-        for (int s = 0; s < TILE; ++s) tr += Pt[r + TILE * s] * ri[s];
+        for (int s = 0; s < TILE; ++s) {
+          tr += Pt[r + TILE * s] * ri[s];
+          if (MODE == 2) trd += Pt[r + TILE * s] * dRi[s];
+        }
         d += ri[r] * tr;
-        if (GGA) eg += dRi[r] * tr;
+        if (MODE) eg += dRi[r] * tr;
+        if (MODE == 2) fk += dRi[r] * trd;
       }
       const size_t o = ((size_t)e * A.nrad + i) * A.npair + A.rx[q];
       A.D[o] = d;
-      if (GGA) A.E[o] = eg;
+      if (MODE) A.E[o] = eg;
+      if (MODE == 2) A.F[o] = fk;
     }
   }
 }
@@ -266,8 +284,9 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   }
   block_sum2(ex, ne, red);
   if (t == 0) {
-    A.part[2 * ((size_t)e * A.nrad + i)] = ex;
-    A.part[2 * ((size_t)e * A.nrad + i) + 1] = ne;
+    A.part[3 * ((size_t)e * A.nrad + i)] = ex;
+    A.part[3 * ((size_t)e * A.nrad + i) + 1] = ne;
+    A.part[3 * ((size_t)e * A.nrad + i) + 2] = 0.0;
   }
   __syncthreads();
   for (int q = t; q < ns * nd * lang; q += NT) {
@@ -291,43 +310,59 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   }
 }
 
-// GGA point kernel: one workgroup per (element, radial point), both spins
-// when polarized.  LDS: region A holds D and E of each spin, then the point
-// values (w vrho, U_r, U_theta, U_phi); region B holds the class sums (C, G,
-// T), then the class transforms (V_d - d/(r sin) Vphi_d, Vr_d, Vtheta_d).
-// Any supported functional pair runs here (vsigma = 0 without a GGA part).
-size_t xc_gga_point_lds(int np, int nd, int lang, int mang, int ns) {
-  const size_t na = (size_t)std::max(2 * np, 4 * lang * mang);
-  return sizeof(double) * ((size_t)ns * na + (size_t)ns * 3 * nd * lang);
+// General point kernel: one workgroup per (element, radial point), both spins
+// when polarized.  LDS: region A holds each spin's radial contractions (D, E
+// and with tau F), then its point values (w vrho, c_1 U_1, c_2 U_2, c_3 U_3
+// and with tau w vtau/2); region B holds the class sums (C, G, T and with tau
+// the three tau sums), then the class transforms (V_d - d Vphi_d, Vr_d,
+// Vtheta_d and with tau S_d).  Any supported functional pair runs here
+// (vsigma = 0 without a GGA part, vtau = 0 without a meta-GGA part).
+template <int MODE> struct GgaShape {
+  static constexpr int nin = MODE == 2 ? 3 : 2;  // radial contractions per spin
+  static constexpr int ncs = MODE == 2 ? 6 : 3;  // class sums per spin
+  static constexpr int npv = MODE == 2 ? 5 : 4;  // point arrays per spin
+};
+
+size_t xc_gga_point_lds(int np, int nd, int lang, int mang, int ns, bool tau) {
+  const int nin = tau ? 3 : 2, ncs = tau ? 6 : 3, npv = tau ? 5 : 4;
+  const size_t na = (size_t)std::max(nin * np, npv * lang * mang);
+  return sizeof(double) * ((size_t)ns * na + (size_t)ns * ncs * nd * lang);
 }
 
+template <int MODE>
 __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
+  typedef GgaShape<MODE> G;
   const int e = blockIdx.x / A.nrad, i = blockIdx.x - e * A.nrad;
   const int t = threadIdx.x;
   const int np = A.npair, nd = A.nd, lang = A.lang, mang = A.mang, nang = lang * mang, dmax = (nd - 1) / 2;
   const bool pol = A.Db != nullptr;
   const int ns = pol ? 2 : 1;
   const bool gga = xcf::is_gga(A.xf) || xcf::is_gga(A.cf);
-  const int na = max(2 * np, 4 * nang), ncl = nd * lang;
+  const bool tau = MODE == 2;
+  const int na = max(G::nin * np, G::npv * nang), ncl = nd * lang;
   extern __shared__ double sm[];
   double *sA = sm;            // [ns][na]
-  double *sB = sm + ns * na;  // [ns][3][nd][lang]
+  double *sB = sm + ns * na;  // [ns][ncs][nd][lang]
   __shared__ double red[8];
   const size_t row = ((size_t)e * A.nrad + i) * np;
   for (int x = t; x < np; x += NT) {
     sA[x] = A.D[row + x];
     sA[np + x] = A.E[row + x];
+    if (tau) sA[2 * np + x] = A.F[row + x];
     if (pol) {
       sA[na + x] = A.Db[row + x];
       sA[na + np + x] = A.Eb[row + x];
+      if (tau) sA[na + 2 * np + x] = A.Fb[row + x];
     }
   }
   __syncthreads();
-  for (int q = t; q < ns * 3 * ncl; q += NT) {
-    const int sp = q / (3 * ncl), r1 = q - sp * 3 * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
+  // class sums: 0 C (Theta Theta D), 1 G (Theta Theta E), 2 T ((Theta Theta)' D),
+  // with tau 3 (Theta Theta F), 4 (Theta' Theta' D), 5 (m m Theta Theta D)
+  for (int q = t; q < ns * G::ncs * ncl; q += NT) {
+    const int sp = q / (G::ncs * ncl), r1 = q - sp * G::ncs * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
     const int d = r2 / lang, tt = r2 - d * lang;
-    const double *src = sA + sp * na + (c == 1 ? np : 0);
-    const double *tab = c == 2 ? A.thdt : A.thxt;
+    const double *src = sA + sp * na + (c == 1 ? np : c == 3 ? 2 * np : 0);
+    const double *tab = c == 2 ? A.thdt : c == 4 ? A.thddt : c == 5 ? A.thmmt : A.thxt;
     double acc = 0.0;
     for (int x = A.dstart[d]; x < A.dstart[d + 1]; ++x) acc += tab[(size_t)x * lang + tt] * src[x];
     sB[q] = acc;
@@ -335,67 +370,87 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
   __syncthreads();
   const double *wp = A.wpt + ((size_t)e * A.nrad + i) * lang;
   const double *gs = A.gsc + ((size_t)e * A.nrad + i) * lang * 3;
-  double ex = 0.0, ne = 0.0;
+  double ex = 0.0, ne = 0.0, ek = 0.0;
   for (int q = t; q < nang; q += NT) {
     const int tt = q / mang, k = q - tt * mang;
     const double c1 = gs[3 * tt], c2 = gs[3 * tt + 1], c3 = gs[3 * tt + 2];
-    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double rho[2] = {0.0, 0.0}, g[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}, tk[2] = {0.0, 0.0};
     for (int sp = 0; sp < ns; ++sp) {
-      const double *C = sB + sp * 3 * ncl, *G = C + ncl, *T = G + ncl;
-      double r0 = 0.0, gr = 0.0, gt = 0.0, gp = 0.0;
+      const double *C = sB + sp * G::ncs * ncl;
+      double r0 = 0.0, gr = 0.0, gt = 0.0, gp = 0.0, kf = 0.0, kt = 0.0, km = 0.0;
       for (int d = 0; d < nd; ++d) {
-        const double c = A.cs[(size_t)d * mang + k], cd = C[d * lang + tt];
-        r0 += cd * c;
-        gr += G[d * lang + tt] * c;
-        gt += T[d * lang + tt] * c;
+        const double cc = A.cs[(size_t)d * mang + k], cd = C[d * lang + tt];
+        r0 += cd * cc;
+        gr += C[ncl + d * lang + tt] * cc;
+        gt += C[2 * ncl + d * lang + tt] * cc;
         gp += (double)(d - dmax) * cd * A.sn[(size_t)d * mang + k];
+        if (tau) {
+          kf += C[3 * ncl + d * lang + tt] * cc;
+          kt += C[4 * ncl + d * lang + tt] * cc;
+          km += C[5 * ncl + d * lang + tt] * cc;
+        }
       }
       rho[sp] = r0;
       g[sp][0] = 2.0 * c1 * gr;
       g[sp][1] = c2 * gt;
       g[sp][2] = -c3 * gp;
+      if (tau) tk[sp] = 0.5 * (c1 * c1 * kf + c2 * c2 * kt + c3 * c3 * km);
     }
     const double w = wp[tt];
-    double f, va, vb = 0.0, u[2][3];
+    double f, va, vb = 0.0, vta = 0.0, vtb = 0.0, u[2][3];
     if (pol) {
-      const double x5[5] = {rho[0], rho[1], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
+      const double x7[7] = {rho[0], rho[1], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2],
                             g[0][0] * g[1][0] + g[0][1] * g[1][1] + g[0][2] * g[1][2],
-                            g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2]};
-      double v[5];
-      xcf::eval_pol(A.xf, A.cf, x5, A.thr, gga, f, v);
+                            g[1][0] * g[1][0] + g[1][1] * g[1][1] + g[1][2] * g[1][2], tk[0], tk[1]};
+      double v[7];
+      xcf::eval_pol(A.xf, A.cf, x7, A.thr, gga, tau, f, v);
       va = v[0];
       vb = v[1];
+      vta = v[5];
+      vtb = v[6];
       for (int c = 0; c < 3; ++c) {
         u[0][c] = w * (2.0 * v[2] * g[0][c] + v[3] * g[1][c]);
         u[1][c] = w * (2.0 * v[4] * g[1][c] + v[3] * g[0][c]);
       }
     } else {
-      double vs;
-      xcf::eval_unpol(A.xf, A.cf, rho[0], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2], A.thr, gga,
-                      f, va, vs);
-      for (int c = 0; c < 3; ++c) u[0][c] = 2.0 * w * vs * g[0][c];
+      const double x3[3] = {rho[0], g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[0][2] * g[0][2], tk[0]};
+      double v[3];
+      xcf::eval_unpol(A.xf, A.cf, x3, A.thr, gga, tau, f, v);
+      va = v[0];
+      vta = v[2];
+      for (int c = 0; c < 3; ++c) u[0][c] = 2.0 * w * v[1] * g[0][c];
     }
     ex += w * f;
     ne += w * (rho[0] + rho[1]);
+    ek += w * (tk[0] + tk[1]);
     sA[q] = w * va;
     sA[nang + q] = c1 * u[0][0];
     sA[2 * nang + q] = c2 * u[0][1];
     sA[3 * nang + q] = c3 * u[0][2];
+    if (tau) sA[4 * nang + q] = 0.5 * w * vta;
     if (pol) {
       sA[na + q] = w * vb;
       sA[na + nang + q] = c1 * u[1][0];
       sA[na + 2 * nang + q] = c2 * u[1][1];
       sA[na + 3 * nang + q] = c3 * u[1][2];
+      if (tau) sA[na + 4 * nang + q] = 0.5 * w * vtb;
     }
   }
   block_sum2(ex, ne, red);
+  __syncthreads();
+  ek = wave_sum(ek);
+  if ((t & 63) == 0) red[t >> 6] = ek;
+  __syncthreads();
   if (t == 0) {
-    A.part[2 * ((size_t)e * A.nrad + i)] = ex;
-    A.part[2 * ((size_t)e * A.nrad + i) + 1] = ne;
+    A.part[3 * ((size_t)e * A.nrad + i)] = ex;
+    A.part[3 * ((size_t)e * A.nrad + i) + 1] = ne;
+    A.part[3 * ((size_t)e * A.nrad + i) + 2] = red[0] + red[1] + red[2] + red[3];
   }
   __syncthreads();
-  for (int q = t; q < ns * 3 * ncl; q += NT) {
-    const int sp = q / (3 * ncl), r1 = q - sp * 3 * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
+  // class transforms: 0 Q = V_d - d Vphi_d, 1 Vr_d, 2 Vtheta_d, with tau 3 S_d
+  const int ntr = tau ? 4 : 3;
+  for (int q = t; q < ns * ntr * ncl; q += NT) {
+    const int sp = q / (ntr * ncl), r1 = q - sp * ntr * ncl, c = r1 / ncl, r2 = r1 - c * ncl;
     const int d = r2 / lang, tt = r2 - d * lang;
     const double *V = sA + sp * na + tt * mang;
     const double *cs = A.cs + (size_t)d * mang, *snd = A.sn + (size_t)d * mang;
@@ -405,27 +460,32 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
       const double *Vp = V + 3 * nang;
       for (int k = 0; k < mang; ++k) acc += V[k] * cs[k] - fphi * Vp[k] * snd[k];
     } else {
-      const double *Vc = V + (c == 1 ? 1 : 2) * nang;
+      const double *Vc = V + (c == 1 ? 1 : c == 2 ? 2 : 4) * nang;
       for (int k = 0; k < mang; ++k) acc += Vc[k] * cs[k];
     }
-    sB[q] = acc;
+    sB[sp * G::ncs * ncl + c * ncl + d * lang + tt] = acc;
   }
   __syncthreads();
   for (int sp = 0; sp < ns; ++sp) {
-    double *Wout = sp == 0 ? A.W : A.Wb, *Yout = sp == 0 ? A.Y : A.Yb;
-    const double *Q = sB + sp * 3 * ncl, *Vr = Q + ncl, *Vt = Vr + ncl;
+    double *Wout = sp == 0 ? A.W : A.Wb, *Yout = sp == 0 ? A.Y : A.Yb, *Zout = sp == 0 ? A.Z : A.Zb;
+    const double *Q = sB + sp * G::ncs * ncl, *Vr = Q + ncl, *Vt = Vr + ncl, *S = Vt + ncl;
     for (int x = t; x < np; x += NT) {
       const int d = A.xd[x];
-      const double *Qd = Q + d * lang, *Vrd = Vr + d * lang, *Vtd = Vt + d * lang;
-      double w2 = 0.0, wt = 0.0, y = 0.0;
+      const double *Qd = Q + d * lang, *Vrd = Vr + d * lang, *Vtd = Vt + d * lang, *Sd = S + d * lang;
+      double w2 = 0.0, y = 0.0, z = 0.0;
       for (int tt = 0; tt < lang; ++tt) {
         const double th = A.thx[(size_t)tt * np + x];
-        w2 += th * Qd[tt];
+        w2 += th * Qd[tt] + A.thd[(size_t)tt * np + x] * Vtd[tt];
         y += th * Vrd[tt];
-        wt += A.thd[(size_t)tt * np + x] * Vtd[tt];
+        if (tau) {
+          const double c1 = gs[3 * tt], c2 = gs[3 * tt + 1], c3 = gs[3 * tt + 2];
+          w2 += (c2 * c2 * A.thdd[(size_t)tt * np + x] + c3 * c3 * A.thmm[(size_t)tt * np + x]) * Sd[tt];
+          z += c1 * c1 * th * Sd[tt];
+        }
       }
-      Wout[((size_t)e * np + x) * A.nrad + i] = w2 + wt;
+      Wout[((size_t)e * np + x) * A.nrad + i] = w2;
       Yout[((size_t)e * np + x) * A.nrad + i] = y;
+      if (tau) Zout[((size_t)e * np + x) * A.nrad + i] = z;
     }
   }
 }
@@ -433,7 +493,7 @@ __global__ void __launch_bounds__(NT) k_xc_point_gga(XCArgs A) {
 // One workgroup per (element, shell a); thread (r,s) of the element band,
 // looping over the pairs (a, b) of row a.  The corner shared with element
 // e+1 is written by e with both contributions; e > 0 skips its own (0,0).
-template <bool GGA>
+template <int MODE>
 __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
   const int e = blockIdx.x / A.Nang, a = blockIdx.x - e * A.Nang;
   const int t = threadIdx.x, r = t & 15, s = t >> 4;
@@ -444,13 +504,13 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
   double *sdR = sN + A.nrad, *sdN = sdR + A.nrad * TILE;
   for (int q = t; q < A.nrad * TILE; q += NT) {
     sR[q] = A.R[(size_t)e * A.nrad * TILE + q];
-    if (GGA) sdR[q] = A.dR[(size_t)e * A.nrad * TILE + q];
+    if (MODE) sdR[q] = A.dR[(size_t)e * A.nrad * TILE + q];
   }
   const bool has_next = e + 1 < A.Nel;
   if (has_next)
     for (int q = t; q < A.nrad; q += NT) {
       sN[q] = A.R[((size_t)(e + 1) * A.nrad + q) * TILE];
-      if (GGA) sdN[q] = A.dR[((size_t)(e + 1) * A.nrad + q) * TILE];
+      if (MODE) sdN[q] = A.dR[((size_t)(e + 1) * A.nrad + q) * TILE];
     }
   __syncthreads();
   if (r >= ni || s >= ni) return;
@@ -465,17 +525,23 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
     if (col < 0) continue;
     const double *Wp = A.W + ((size_t)e * np + x) * A.nrad;
     double v = 0.0;
-    if (GGA) {
+    if (MODE) {
       const int xT = A.rxT[q];
       const double *Yp = A.Y + ((size_t)e * np + x) * A.nrad, *YT = A.Y + ((size_t)e * np + xT) * A.nrad;
+      const double *Zp = MODE == 2 ? A.Z + ((size_t)e * np + x) * A.nrad : nullptr;
       for (int i = 0; i < A.nrad; ++i) {
-        const double Rr = sR[i * TILE + r], Rs = sR[i * TILE + s];
-        v += Rr * Rs * Wp[i] + sdR[i * TILE + r] * Rs * Yp[i] + Rr * sdR[i * TILE + s] * YT[i];
+        const double Rr = sR[i * TILE + r], Rs = sR[i * TILE + s], Dr = sdR[i * TILE + r], Ds = sdR[i * TILE + s];
+        v += Rr * Rs * Wp[i] + Dr * Rs * Yp[i] + Rr * Ds * YT[i];
+        if (MODE == 2) v += Dr * Ds * Zp[i];
       }
       if (corner) {
         const double *Wn = A.W + ((size_t)(e + 1) * np + x) * A.nrad;
         const double *Yn = A.Y + ((size_t)(e + 1) * np + x) * A.nrad, *YTn = A.Y + ((size_t)(e + 1) * np + xT) * A.nrad;
         for (int i = 0; i < A.nrad; ++i) v += sN[i] * (sN[i] * Wn[i] + sdN[i] * (Yn[i] + YTn[i]));
+        if (MODE == 2) {
+          const double *Zn = A.Z + ((size_t)(e + 1) * np + x) * A.nrad;
+          for (int i = 0; i < A.nrad; ++i) v += sdN[i] * sdN[i] * Zn[i];
+        }
       }
     } else {
       for (int i = 0; i < A.nrad; ++i) v += sR[i * TILE + r] * sR[i * TILE + s] * Wp[i];
@@ -491,18 +557,51 @@ __global__ void __launch_bounds__(NT) k_xc_rad_out(XCArgs A) {
 }  // namespace
 
 bool xc_generic(int xf, int cf) { return !((xf == 0 || xf == 1) && (cf == 0 || cf == 7)); }
+bool xc_tau(int xf, int cf) { return xcf::is_mgga(xf) || xcf::is_mgga(cf); }
 
 size_t xc_point_lds(const XCGridDevice &g, int xf, int cf, bool pol) {
   const int ns = pol ? 2 : 1, nd = 2 * g.dmax + 1;
-  if (xc_generic(xf, cf)) return xc_gga_point_lds(g.npair, nd, g.lang, g.mang, ns);
+  if (xc_generic(xf, cf)) return xc_gga_point_lds(g.npair, nd, g.lang, g.mang, ns, xc_tau(xf, cf));
   return sizeof(double) * ns * ((size_t)g.npair + (size_t)g.lang * nd + (size_t)g.lang * g.mang);
 }
+
+namespace {
+template <int MODE>
+void launch_general(const XCGridDevice &g, const XCArgs &a, const XCArgs &b, bool pol, size_t sh_in, size_t sh_pt,
+                    size_t sh_out, hipStream_t s, KernelTimer *timer) {
+  {
+    TimedLaunch tl(timer, "xc_rad_in", s);
+    hipLaunchKernelGGL(k_xc_rad_in<MODE>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
+    if (pol) hipLaunchKernelGGL(k_xc_rad_in<MODE>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
+    hip_check(hipGetLastError(), "xc_rad_in");
+  }
+  {
+    TimedLaunch tl(timer, "xc_point", s);
+    if (MODE == 0) {
+      hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+    } else {
+      if (sh_pt > 64 * 1024)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xc_point_gga<MODE>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh_pt),
+                  "xc_point_gga LDS size");
+      hipLaunchKernelGGL(k_xc_point_gga<MODE>, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+    }
+    hip_check(hipGetLastError(), "xc_point");
+  }
+  {
+    TimedLaunch tl(timer, "xc_rad_out", s);
+    hipLaunchKernelGGL(k_xc_rad_out<MODE>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
+    if (pol) hipLaunchKernelGGL(k_xc_rad_out<MODE>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
+    hip_check(hipGetLastError(), "xc_rad_out");
+  }
+}
+}  // namespace
 
 void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *Pa, int64_t ldpa, const double *Pb,
                     int64_t ldpb, double *Ha, int64_t ldha, double *Hb, int64_t ldhb, hipStream_t s,
                     KernelTimer *timer) {
   const bool pol = Pb != nullptr;
-  const bool gen = xc_generic(xf, cf);
+  const int mode = xc_tau(xf, cf) ? 2 : xc_generic(xf, cf) ? 1 : 0;
   fill_zero_launch(Ha, g.Nbf, g.Nbf, ldha, s);
   if (pol) fill_zero_launch(Hb, g.Nbf, g.Nbf, ldhb, s);
   const int nd = 2 * g.dmax + 1;
@@ -510,7 +609,8 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
            g.first.p, g.ni.p, g.skip.p, g.poff.p, g.rstart.p, g.rb.p, g.rx.p, g.dstart.p, g.xd.p,
            g.R.p, g.wpt.p, g.thx.p, g.thxt.p, g.cs.p, g.D.p, g.W.p, g.part.p, pol ? g.Db.p : nullptr,
            pol ? g.Wb.p : nullptr, g.dR.p, g.gsc.p, g.sn.p, g.thd.p, g.thdt.p, g.rxT.p,
-           g.E.p, g.Y.p, pol ? g.Eb.p : nullptr, pol ? g.Yb.p : nullptr};
+           g.E.p, g.Y.p, pol ? g.Eb.p : nullptr, pol ? g.Yb.p : nullptr,
+           g.thdd.p, g.thddt.p, g.thmm.p, g.thmmt.p, g.F.p, g.Z.p, pol ? g.Fb.p : nullptr, pol ? g.Zb.p : nullptr};
   XCArgs b = a;  // second spin: its density in, its matrix out
   if (pol) {
     b.P = Pb;
@@ -521,47 +621,20 @@ void xc_grid_launch(XCGridDevice &g, int xf, int cf, double thr, const double *P
     b.W = g.Wb.p;
     b.E = g.Eb.p;
     b.Y = g.Yb.p;
+    b.F = g.Fb.p;
+    b.Z = g.Zb.p;
   }
-  const int rf = gen ? 2 : 1;
+  const int rf = mode ? 2 : 1;
   const size_t sh_in =
       sizeof(double) * ((size_t)rf * g.nrad * XC_RS + (size_t)xc_pairs_per_group(g.nrad) * TILE2);
   const size_t sh_pt = xc_point_lds(g, xf, cf, pol);
   const size_t sh_out = sizeof(double) * rf * ((size_t)g.nrad * TILE + g.nrad);
-  {
-    TimedLaunch tl(timer, "xc_rad_in", s);
-    if (gen) {
-      hipLaunchKernelGGL(k_xc_rad_in<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
-      if (pol) hipLaunchKernelGGL(k_xc_rad_in<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
-    } else {
-      hipLaunchKernelGGL(k_xc_rad_in<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, a);
-      if (pol) hipLaunchKernelGGL(k_xc_rad_in<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_in, s, b);
-    }
-    hip_check(hipGetLastError(), "xc_rad_in");
-  }
-  {
-    TimedLaunch tl(timer, "xc_point", s);
-    if (gen) {
-      if (sh_pt > 64 * 1024)
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xc_point_gga),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh_pt),
-                  "xc_point_gga LDS size");
-      hipLaunchKernelGGL(k_xc_point_gga, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
-    } else {
-      hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
-    }
-    hip_check(hipGetLastError(), "xc_point");
-  }
-  {
-    TimedLaunch tl(timer, "xc_rad_out", s);
-    if (gen) {
-      hipLaunchKernelGGL(k_xc_rad_out<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
-      if (pol) hipLaunchKernelGGL(k_xc_rad_out<true>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
-    } else {
-      hipLaunchKernelGGL(k_xc_rad_out<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, a);
-      if (pol) hipLaunchKernelGGL(k_xc_rad_out<false>, dim3(g.Nel * g.Nang), dim3(NT), sh_out, s, b);
-    }
-    hip_check(hipGetLastError(), "xc_rad_out");
-  }
+  if (mode == 2)
+    launch_general<2>(g, a, b, pol, sh_in, sh_pt, sh_out, s, timer);
+  else if (mode == 1)
+    launch_general<1>(g, a, b, pol, sh_in, sh_pt, sh_out, s, timer);
+  else
+    launch_general<0>(g, a, b, pol, sh_in, sh_pt, sh_out, s, timer);
 }
 
 }  // namespace hfg

@@ -8,8 +8,8 @@ restricted exchange-correlation matrix with Exc, the integrated electron
 count and the kinetic-energy-density integral, like DFTGrid::eval_Fxc
 (dftgrid.cpp:591-653); `eval_Fxc_pol(x_func, c_func, Pa, Pb)` is the
 unrestricted overload (dftgrid.cpp:655-712) returning (Ha, Hb, Exc, Nel, Ekin).  Functionals are libxc ids; parse_method turns the
-driver's --method string (e.g. "lda_x-lda_c_vwn", "gga_x_pbe-gga_c_pbe") into them.  lda_x with lda_c_vwn runs on
-every grid; PBE exchange and correlation and PW92 correlation run on the atomic grid.
+driver's --method string (e.g. "lda_x-lda_c_vwn", "gga_x_pbe-gga_c_pbe", "mgga_x_tpss-mgga_c_tpss") into them;
+every grid evaluates all of them.
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ from ._lib import check, lib
 from .atomic import _f64_fortran
 
 FUNCTIONALS = {"none": 0, "lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "gga_x_pbe": 101,
-               "gga_c_pbe": 130}
+               "gga_c_pbe": 130, "mgga_x_tpss": 202, "mgga_c_tpss": 231}
 
 
 def parse_method(method: str):

@@ -290,33 +290,42 @@ int orc_lda_pol(int xf, int cf, int n, const double *ra, const double *rb, doubl
     for (int i = 0; i < n; ++i) lda_polarized(xf, cf, ra[i], rb[i], thr, exc[i], va[i], vb[i]);
   });
 }
-// Any supported functional pair at n points: unpolarized (rho, sigma) ->
-// (exc, vrho, vsigma) when pol = 0; polarized rho (2n: a, b), sigma (3n:
-// aa, ab, bb) -> exc, vrho (2n), vsigma (3n) when pol = 1
-int orc_xc_eval(int xf, int cf, int pol, int n, const double *rho, const double *sigma, double thr, double *exc,
-                double *vrho, double *vsigma) {
+// Any supported functional pair at n points: unpolarized (rho, sigma, tau) ->
+// (exc, vrho, vsigma, vtau) when pol = 0; polarized rho (2n: a, b), sigma
+// (3n: aa, ab, bb), tau (2n) -> exc, vrho (2n), vsigma (3n), vtau (2n) when
+// pol = 1; tau and vtau may be NULL (zero kinetic energy density)
+int orc_xc_eval(int xf, int cf, int pol, int n, const double *rho, const double *sigma, const double *tau, double thr,
+                double *exc, double *vrho, double *vsigma, double *vtau) {
   return guard([&] {
     for (int i = 0; i < n; ++i) {
       if (pol) {
         const double r[2] = {rho[i], rho[n + i]}, s[3] = {sigma[i], sigma[n + i], sigma[2 * n + i]};
-        double vr[2], vs[3];
-        xc_polarized(xf, cf, r, s, thr, exc[i], vr, vs);
+        const double t[2] = {tau ? tau[i] : 0.0, tau ? tau[n + i] : 0.0};
+        double vr[2], vs[3], vt[2];
+        xc_polarized(xf, cf, r, s, t, thr, exc[i], vr, vs, vt);
+        if (vtau) {
+          vtau[i] = vt[0];
+          vtau[n + i] = vt[1];
+        }
         vrho[i] = vr[0];
         vrho[n + i] = vr[1];
         for (int k = 0; k < 3; ++k) vsigma[k * n + i] = vs[k];
       } else {
-        xc_unpolarized(xf, cf, rho[i], sigma[i], thr, exc[i], vrho[i], vsigma[i]);
+        double vt;
+        xc_unpolarized(xf, cf, rho[i], sigma[i], tau ? tau[i] : 0.0, thr, exc[i], vrho[i], vsigma[i], vt);
+        if (vtau) vtau[i] = vt;
       }
     }
   });
 }
-// Atomic DFT grid: H (Nbf x Nbf) and out = (Exc, Nel)
+// Atomic DFT grid: H (Nbf x Nbf) and out = (Exc, Nel, Ekin = integral of tau, meta-GGA)
 int orc_atomic_xc(void *h, int lang, int mang, int xf, int cf, const double *P, double thr, double *H, double *out) {
   return guard([&] {
     AtomicBasis *b = (AtomicBasis *)h;
     AtomicXC g(*b, lang, mang);
     Mat Hm;
     g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(Hm, H);
   });
 }
@@ -327,6 +336,7 @@ int orc_diatomic_xc(void *h, int lang, int xf, int cf, const double *P, double t
     DiatomicXC g(*b, lang);
     Mat Hm;
     g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(Hm, H);
   });
 }
@@ -338,6 +348,7 @@ int orc_atomic_xc_pol(void *h, int lang, int mang, int xf, int cf, const double 
     AtomicXC g(*b, lang, mang);
     Mat A, B;
     g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(A, Ha);
     put(B, Hb);
   });
@@ -349,6 +360,7 @@ int orc_diatomic_xc_pol(void *h, int lang, int xf, int cf, const double *Pa, con
     DiatomicXC g(*b, lang);
     Mat A, B;
     g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(A, Ha);
     put(B, Hb);
   });
@@ -362,6 +374,7 @@ int orc_diatomic_xc3d(void *h, int lang, int mang, int xf, int cf, const double 
     DiatomicXC g(*b, lang, mang);
     Mat Hm;
     g.eval_Fxc(xf, cf, get(P, b->Nbf(), b->Nbf()), Hm, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(Hm, H);
   });
 }
@@ -373,6 +386,7 @@ int orc_diatomic_xc3d_pol(void *h, int lang, int mang, int xf, int cf, const dou
     DiatomicXC g(*b, lang, mang);
     Mat A, B;
     g.eval_Fxc(xf, cf, get(Pa, b->Nbf(), b->Nbf()), get(Pb, b->Nbf(), b->Nbf()), A, B, out[0], out[1], thr);
+    out[2] = g.ekin;
     put(A, Ha);
     put(B, Hb);
   });

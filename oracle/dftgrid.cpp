@@ -164,6 +164,8 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
   xc_check(xf, cf);
   const bool pol = Pbp != nullptr;
   const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool mgga = xc_is_mgga(xf) || xc_is_mgga(cf);
+  ekin = 0.0;
   const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const int Na = b.Nang(), Nr = rad.Nbf();
@@ -229,7 +231,7 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
       }
     // update_density: rho = colwise (P a).a + (P b).b, per spin when polarized;
     // with GGA also g_c = 2 ((P a).da_c + (P b).db_c) / scale_c
-    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3]) {
+    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3], std::vector<double> &tau) {
       Mat Pe(nbf, nbf);
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
@@ -256,11 +258,24 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
           g[c][p] = 2.0 * (s + t) / sc;
         }
       }
+      // kinetic energy density tau = 1/2 sum_c ((P da_c).da_c + (P db_c).db_c) / scale_c^2
+      // (update_density, dftgrid.cpp:89-107, 186-214)
+      if (!mgga) return;
+      tau.assign(npt, 0.0);
+      for (int c = 0; c < 3; ++c) {
+        const Mat QA = matmul(Pe, dre[c]), QB = matmul(Pe, dim[c]);
+        for (int p = 0; p < npt; ++p) {
+          double s = 0.0;
+          for (int u = 0; u < nbf; ++u) s += QA(u, p) * dre[c](u, p) + QB(u, p) * dim[c](u, p);
+          const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
+          tau[p] += 0.5 * s / (sc * sc);
+        }
+      }
     };
-    std::vector<double> ra, rb, ga[3], gb[3];
-    density(P, ra, ga);
-    if (pol) density(*Pbp, rb, gb);
-    std::vector<double> va(npt), vb(npt), gna[3], gnb[3];
+    std::vector<double> ra, rb, ga[3], gb[3], ta, tb;
+    density(P, ra, ga, ta);
+    if (pol) density(*Pbp, rb, gb, tb);
+    std::vector<double> va(npt), vb(npt), gna[3], gnb[3], vta(npt, 0.0), vtb(npt, 0.0);
     if (gga)
       for (int c = 0; c < 3; ++c) {
         gna[c].assign(npt, 0.0);
@@ -283,9 +298,13 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
             sig[1] += ga[c][p] * gb[c][p];
             sig[2] += gb[c][p] * gb[c][p];
           }
-        xc_polarized(xf, cf, rho, sig, thr, exc, vr, vs);
+        const double tt[2] = {mgga ? ta[p] : 0.0, mgga ? tb[p] : 0.0};
+        double vt[2];
+        xc_polarized(xf, cf, rho, sig, tt, thr, exc, vr, vs, vt);
         va[p] = vr[0];
         vb[p] = vr[1];
+        vta[p] = vt[0];
+        vtb[p] = vt[1];
         if (gga)
           for (int c = 0; c < 3; ++c) {
             const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
@@ -296,7 +315,7 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
         double sig = 0.0, vs;
         if (gga)
           for (int c = 0; c < 3; ++c) sig += ga[c][p] * ga[c][p];
-        xc_unpolarized(xf, cf, ra[p], sig, thr, exc, va[p], vs);
+        xc_unpolarized(xf, cf, ra[p], sig, mgga ? ta[p] : 0.0, thr, exc, va[p], vs, vta[p]);
         if (gga)
           for (int c = 0; c < 3; ++c) {
             const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
@@ -305,10 +324,12 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
       }
       Nel += wtot[p] * rt;
       Exc += wtot[p] * exc * rt;
+      if (mgga) ekin += wtot[p] * (pol ? ta[p] + tb[p] : ta[p]);
     }
     // eval_Fxc + increment_lda_split: H += (a .* w v) a' + (b .* w v) b';
     // increment_gga_split: X = (sum_c gn_c .* da_c) a' + (sum_c gn_c .* db_c) b', H += X + X'
-    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, Mat &Hm) {
+    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, const std::vector<double> &vt,
+                         Mat &Hm) {
       Mat av = are, bv = aim;
       for (int p = 0; p < npt; ++p) {
         const double f = vr[p] * wtot[p];
@@ -334,11 +355,27 @@ void AtomicXC::eval(int xf, int cf, const Mat &P, const Mat *Pbp, Mat &H, Mat &H
         for (int v = 0; v < nbf; ++v)
           for (int u = 0; u < nbf; ++u) Hs(u, v) += X(u, v) + X(v, u);
       }
+      // meta-GGA: H += sum_c (da_c .* w vtau/2 / scale_c^2) da_c' + (db_c ...) db_c'
+      // (eval_Fxc, dftgrid.cpp:336-349, increment_lda_split per component)
+      if (mgga)
+        for (int c = 0; c < 3; ++c) {
+          Mat dv = dre[c], ev = dim[c];
+          for (int p = 0; p < npt; ++p) {
+            const double sc = c == 0 ? 1.0 : (c == 1 ? sc_th[p] : sc_ph[p]);
+            const double f = 0.5 * vt[p] * wtot[p] / (sc * sc);
+            for (int u = 0; u < nbf; ++u) {
+              dv(u, p) *= f;
+              ev(u, p) *= f;
+            }
+          }
+          const Mat T1 = matmul_nt(dv, dre[c]), T2 = matmul_nt(ev, dim[c]);
+          for (size_t q = 0; q < Hs.a.size(); ++q) Hs.a[q] += T1.a[q] + T2.a[q];
+        }
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Hs(u, v);
     };
-    potential(va, gna, H);
-    if (pol) potential(vb, gnb, Hb);
+    potential(va, gna, vta, H);
+    if (pol) potential(vb, gnb, vtb, Hb);
   }
 }
 
@@ -372,6 +409,8 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
   xc_check(xf, cf);
   const bool pol = Pbin != nullptr;
   const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool mgga = xc_is_mgga(xf) || xc_is_mgga(cf);
+  ekin = 0.0;
   const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);
@@ -440,7 +479,7 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
             }
           }
       }
-    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3]) {
+    auto density = [&](const Mat &Pm, std::vector<double> &rho, std::vector<double> g[3], std::vector<double> &tau) {
       Mat Pe(nbf, nbf);
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Pe(u, v) = Pm(idx[u], idx[v]);
@@ -466,11 +505,24 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
           g[c][p] = 2.0 * (s + t) / sc[c][p];
         }
       }
+      // kinetic energy density tau = 1/2 sum_c ((P da_c).da_c + (P db_c).db_c) / scale_c^2
+      // (update_density, dftgrid.cpp:89-107, 186-214)
+      if (!mgga) return;
+      tau.assign(npt, 0.0);
+      for (int c = 0; c < 3; ++c) {
+        const Mat QA = matmul(Pe, dre[c]), QB = matmul(Pe, dim[c]);
+        for (int p = 0; p < npt; ++p) {
+          double s = 0.0;
+          for (int u = 0; u < nbf; ++u) s += QA(u, p) * dre[c](u, p) + QB(u, p) * dim[c](u, p);
+          const double hc = sc[c][p];
+          tau[p] += 0.5 * s / (hc * hc);
+        }
+      }
     };
-    std::vector<double> ra, rb, ga[3], gb[3];
-    density(P, ra, ga);
-    if (pol) density(Pbx, rb, gb);
-    std::vector<double> va(npt), vb(npt), gna[3], gnb[3];
+    std::vector<double> ra, rb, ga[3], gb[3], ta, tb;
+    density(P, ra, ga, ta);
+    if (pol) density(Pbx, rb, gb, tb);
+    std::vector<double> va(npt), vb(npt), gna[3], gnb[3], vta(npt, 0.0), vtb(npt, 0.0);
     if (gga)
       for (int c = 0; c < 3; ++c) {
         gna[c].assign(npt, 0.0);
@@ -493,9 +545,13 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
             sig[1] += ga[c][p] * gb[c][p];
             sig[2] += gb[c][p] * gb[c][p];
           }
-        xc_polarized(xf, cf, rho, sig, thr, exc, vr, vs);
+        const double tt[2] = {mgga ? ta[p] : 0.0, mgga ? tb[p] : 0.0};
+        double vt[2];
+        xc_polarized(xf, cf, rho, sig, tt, thr, exc, vr, vs, vt);
         va[p] = vr[0];
         vb[p] = vr[1];
+        vta[p] = vt[0];
+        vtb[p] = vt[1];
         if (gga)
           for (int c = 0; c < 3; ++c) {
             gna[c][p] = wtot[p] * (2.0 * vs[0] * ga[c][p] + vs[1] * gb[c][p]) / sc[c][p];
@@ -505,14 +561,16 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
         double sig = 0.0, vs;
         if (gga)
           for (int c = 0; c < 3; ++c) sig += ga[c][p] * ga[c][p];
-        xc_unpolarized(xf, cf, ra[p], sig, thr, exc, va[p], vs);
+        xc_unpolarized(xf, cf, ra[p], sig, mgga ? ta[p] : 0.0, thr, exc, va[p], vs, vta[p]);
         if (gga)
           for (int c = 0; c < 3; ++c) gna[c][p] = 2.0 * wtot[p] * vs * ga[c][p] / sc[c][p];
       }
       Nel += wtot[p] * rt;
       Exc += wtot[p] * exc * rt;
+      if (mgga) ekin += wtot[p] * (pol ? ta[p] + tb[p] : ta[p]);
     }
-    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, Mat &Hm) {
+    auto potential = [&](const std::vector<double> &vr, const std::vector<double> *gn, const std::vector<double> &vt,
+                         Mat &Hm) {
       Mat av = are, bv = aim;
       for (int p = 0; p < npt; ++p) {
         const double f = vr[p] * wtot[p];
@@ -538,11 +596,27 @@ void DiatomicXC::eval3d(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Ho
         for (int v = 0; v < nbf; ++v)
           for (int u = 0; u < nbf; ++u) Hs(u, v) += X(u, v) + X(v, u);
       }
+      // meta-GGA: H += sum_c (da_c .* w vtau/2 / scale_c^2) da_c' + (db_c ...) db_c'
+      // (eval_Fxc, dftgrid.cpp:336-349, increment_lda_split per component)
+      if (mgga)
+        for (int c = 0; c < 3; ++c) {
+          Mat dv = dre[c], ev = dim[c];
+          for (int p = 0; p < npt; ++p) {
+            const double hc = sc[c][p];
+            const double f = 0.5 * vt[p] * wtot[p] / (hc * hc);
+            for (int u = 0; u < nbf; ++u) {
+              dv(u, p) *= f;
+              ev(u, p) *= f;
+            }
+          }
+          const Mat T1 = matmul_nt(dv, dre[c]), T2 = matmul_nt(ev, dim[c]);
+          for (size_t q = 0; q < Hs.a.size(); ++q) Hs.a[q] += T1.a[q] + T2.a[q];
+        }
       for (int v = 0; v < nbf; ++v)
         for (int u = 0; u < nbf; ++u) Hm(idx[u], idx[v]) += Hs(u, v);
     };
-    potential(va, gna, H);
-    if (pol) potential(vb, gnb, H2);
+    potential(va, gna, vta, H);
+    if (pol) potential(vb, gnb, vtb, H2);
   }
   Hout = b.remove_boundaries(H);
   if (pol) Hbout = b.remove_boundaries(H2);
@@ -559,6 +633,8 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
   xc_check(xf, cf);
   const bool pol = Pbin != nullptr;
   const bool gga = xc_is_gga(xf) || xc_is_gga(cf);
+  const bool mgga = xc_is_mgga(xf) || xc_is_mgga(cf);
+  ekin = 0.0;
   const bool plain_lda = !gga && (xf == 0 || xf == 1) && (cf == 0 || cf == 7);
   const RadialBasis &rad = b.radial;
   const Mat P = b.expand_boundaries(Pin);
@@ -579,15 +655,16 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
     const int Ni = Bv.c;
     for (int ir = 0; ir < (int)mu.size(); ++ir) {
       const double wr = h * rad.wq[ir], shmu = std::sinh(mu[ir]);
-      std::vector<double> wtot(nang), hs(nang);
+      std::vector<double> wtot(nang), hs(nang), hphi(nang);
       for (int ia = 0; ia < nang; ++ia) {
         const double sth = std::sqrt(std::max((1.0 - cth[ia]) * (1.0 + cth[ia]), 0.0));
         wtot[ia] = 2.0 * kPi * wang[ia] * wr * Rh3 * shmu * (shmu * shmu + sth * sth);
         hs[ia] = b.Rhalf * std::sqrt(shmu * shmu + sth * sth);
+        hphi[ia] = b.Rhalf * shmu * sth;
       }
       std::vector<std::vector<int>> idx(mlist.size());
       std::vector<Mat> f(mlist.size()), fr(mlist.size()), ft(mlist.size());
-      std::vector<double> rho(nang, 0.0), rhob(nang, 0.0);
+      std::vector<double> rho(nang, 0.0), rhob(nang, 0.0), taua(nang, 0.0), taub(nang, 0.0);
       std::vector<double> ga[2] = {std::vector<double>(nang, 0.0), std::vector<double>(nang, 0.0)};
       std::vector<double> gb[2] = {std::vector<double>(nang, 0.0), std::vector<double>(nang, 0.0)};
       for (size_t im = 0; im < mlist.size(); ++im) {
@@ -620,7 +697,8 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
           }
           row += Ni;
         }
-        auto block_density = [&](const Mat &Pm, std::vector<double> &r, std::vector<double> *g) {
+        auto block_density = [&](const Mat &Pm, std::vector<double> &r, std::vector<double> *g,
+                                 std::vector<double> &tau) {
           Mat Pblk(nbf, nbf);
           for (int v = 0; v < nbf; ++v)
             for (int u = 0; u < nbf; ++u) Pblk(u, v) = Pm(idx[im][u], idx[im][v]);
@@ -640,11 +718,27 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
               g[1][ia] += 2.0 * s1;
             }
           }
+          // tau = 1/2 [((P dr).dr + (P dth).dth) / h^2 + m^2 rho_m / h_phi^2]
+          // (update_density, dftgrid_purem.cpp:287-318)
+          if (mgga) {
+            const Mat Qr = matmul(Pblk, fr[im]), Qt = matmul(Pblk, ft[im]);
+            const double m2 = double(mlist[im]) * mlist[im];
+            for (int ia = 0; ia < nang; ++ia) {
+              double kr = 0.0, kt = 0.0, rm = 0.0;
+              for (int u = 0; u < nbf; ++u) {
+                kr += Qr(u, ia) * fr[im](u, ia);
+                kt += Qt(u, ia) * ft[im](u, ia);
+                rm += Pv(u, ia) * f[im](u, ia);
+              }
+              const double kp = hphi[ia] > 0.0 ? m2 * rm / (hphi[ia] * hphi[ia]) : 0.0;
+              tau[ia] += 0.5 * ((kr + kt) / (hs[ia] * hs[ia]) + kp);
+            }
+          }
         };
-        block_density(P, rho, ga);
-        if (pol) block_density(Pbx, rhob, gb);
+        block_density(P, rho, ga, taua);
+        if (pol) block_density(Pbx, rhob, gb, taub);
       }
-      std::vector<double> wv(nang), wvb(nang), gna[2], gnb[2];
+      std::vector<double> wv(nang), wvb(nang), gna[2], gnb[2], vta(nang, 0.0), vtb(nang, 0.0);
       for (int c = 0; c < 2; ++c) {
         gna[c].assign(nang, 0.0);
         gnb[c].assign(nang, 0.0);
@@ -665,26 +759,32 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
           const double sig[3] = {ga[0][ia] * ga[0][ia] + ga[1][ia] * ga[1][ia],
                                  ga[0][ia] * gb[0][ia] + ga[1][ia] * gb[1][ia],
                                  gb[0][ia] * gb[0][ia] + gb[1][ia] * gb[1][ia]};
-          double vr[2], vs[3];
-          xc_polarized(xf, cf, rr, sig, thr, exc, vr, vs);
+          double vr[2], vs[3], vt[2];
+          const double tt[2] = {taua[ia], taub[ia]};
+          xc_polarized(xf, cf, rr, sig, tt, thr, exc, vr, vs, vt);
           v = vr[0];
           vb2 = vr[1];
+          vta[ia] = vt[0];
+          vtb[ia] = vt[1];
           for (int c = 0; c < 2 && gga; ++c) {
             gna[c][ia] = wtot[ia] * (2.0 * vs[0] * ga[c][ia] + vs[1] * gb[c][ia]) / hs[ia];
             gnb[c][ia] = wtot[ia] * (2.0 * vs[2] * gb[c][ia] + vs[1] * ga[c][ia]) / hs[ia];
           }
         } else {
           double vs;
-          xc_unpolarized(xf, cf, rho[ia], ga[0][ia] * ga[0][ia] + ga[1][ia] * ga[1][ia], thr, exc, v, vs);
+          xc_unpolarized(xf, cf, rho[ia], ga[0][ia] * ga[0][ia] + ga[1][ia] * ga[1][ia], taua[ia], thr, exc, v, vs,
+                         vta[ia]);
           for (int c = 0; c < 2 && gga; ++c) gna[c][ia] = 2.0 * wtot[ia] * vs * ga[c][ia] / hs[ia];
         }
         const double rt = pol ? rho[ia] + rhob[ia] : rho[ia];
         Nel += wtot[ia] * rt;
         Exc += wtot[ia] * exc * rt;
+        if (mgga) ekin += wtot[ia] * (pol ? taua[ia] + taub[ia] : taua[ia]);
         wv[ia] = v * wtot[ia];
         wvb[ia] = vb2 * wtot[ia];
       }
-      auto potential = [&](const std::vector<double> &w, const std::vector<double> *gn, Mat &Hm) {
+      auto potential = [&](const std::vector<double> &w, const std::vector<double> *gn, const std::vector<double> &vt,
+                           Mat &Hm) {
         for (size_t im = 0; im < mlist.size(); ++im) {
           Mat fw = f[im];
           for (int ia = 0; ia < nang; ++ia)
@@ -698,12 +798,30 @@ void DiatomicXC::eval(int xf, int cf, const Mat &Pin, const Mat *Pbin, Mat &Hout
             for (int v = 0; v < X.c; ++v)
               for (int u = 0; u < X.r; ++u) Hblk(u, v) += X(u, v) + X(v, u);
           }
+          // meta-GGA: w vtau/2 / h^2 on dr and dth, m^2 w vtau/2 / h_phi^2 on f
+          // (eval_Fxc, dftgrid_purem.cpp:482-494)
+          if (mgga) {
+            const double m2 = double(mlist[im]) * mlist[im];
+            Mat a = fr[im], c = ft[im], e = f[im];
+            for (int ia = 0; ia < nang; ++ia) {
+              const double vtl = 0.5 * vt[ia] * wtot[ia];
+              const double wr = vtl / (hs[ia] * hs[ia]);
+              const double wp = hphi[ia] > 0.0 ? m2 * vtl / (hphi[ia] * hphi[ia]) : 0.0;
+              for (int u = 0; u < a.r; ++u) {
+                a(u, ia) *= wr;
+                c(u, ia) *= wr;
+                e(u, ia) *= wp;
+              }
+            }
+            const Mat T1 = matmul_nt(a, fr[im]), T2 = matmul_nt(c, ft[im]), T3 = matmul_nt(e, f[im]);
+            for (size_t q = 0; q < Hblk.a.size(); ++q) Hblk.a[q] += T1.a[q] + T2.a[q] + T3.a[q];
+          }
           for (int v = 0; v < Hblk.c; ++v)
             for (int u = 0; u < Hblk.r; ++u) Hm(idx[im][u], idx[im][v]) += Hblk(u, v);
         }
       };
-      potential(wv, gna, H);
-      if (pol) potential(wvb, gnb, H2);
+      potential(wv, gna, vta, H);
+      if (pol) potential(wvb, gnb, vtb, H2);
     }
   }
   Hout = b.remove_boundaries(H);

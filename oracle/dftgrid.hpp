@@ -12,11 +12,15 @@ void lda_polarized(int xf, int cf, double ra, double rb, double thr, double &exc
 // gga_x_pbe 101, gga_c_pbe 130; xcfunc.cpp): energy per particle exc,
 // vrho = d(rho exc)/drho and vsigma = d(rho exc)/dsigma; polarized sigma is
 // (aa, ab, bb) and vsigma likewise.
-bool xc_is_gga(int id);
+// mgga_x_tpss 202 and mgga_c_tpss 231 also take the kinetic energy density
+// tau (per spin when polarized) and give vtau = d(rho exc)/dtau.
+bool xc_is_gga(int id);   // needs the density gradient (GGA or meta-GGA)
+bool xc_is_mgga(int id);  // needs tau
 void xc_check(int xf, int cf);
-void xc_unpolarized(int xf, int cf, double rho, double sigma, double thr, double &exc, double &vrho, double &vsigma);
-void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], double thr, double &exc,
-                  double vrho[2], double vsigma[3]);
+void xc_unpolarized(int xf, int cf, double rho, double sigma, double tau, double thr, double &exc, double &vrho,
+                    double &vsigma, double &vtau);
+void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], const double tau[2], double thr,
+                  double &exc, double vrho[2], double vsigma[3], double vtau[2]);
 void angular_chebyshev(int l, int m, std::vector<double> &cth, std::vector<double> &phi, std::vector<double> &w);
 std::complex<double> spherical_harmonic(int l, int m, double cth, double phi);
 
@@ -24,6 +28,7 @@ std::complex<double> spherical_harmonic(int l, int m, double cth, double phi);
 struct AtomicXC {
   const AtomicBasis &b;
   int lang, mang;
+  mutable double ekin = 0.0;  // integral of tau over the grid, meta-GGA (compute_Ekin)
   std::vector<double> cth, phi, wang;
   AtomicXC(const AtomicBasis &b, int lang, int mang);
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;
@@ -40,6 +45,7 @@ struct DiatomicBasis;
 struct DiatomicXC {
   const DiatomicBasis &b;
   int lang, mang;
+  mutable double ekin = 0.0;  // integral of tau over the grid, meta-GGA (compute_Ekin)
   std::vector<double> cth, wang, phi3, cth3, wang3;
   DiatomicXC(const DiatomicBasis &b, int lang, int mang = 0);
   void eval_Fxc(int xf, int cf, const Mat &P, Mat &H, double &Exc, double &Nel, double thr) const;

@@ -57,8 +57,8 @@ def lib():
             getattr(L, name).argtypes = None
         L.orc_lda.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
                               ctypes.c_void_p, ctypes.c_void_p]
-        L.orc_xc_eval.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double] + \
-            [ctypes.c_void_p] * 3
+        L.orc_xc_eval.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3 + [ctypes.c_double] + \
+            [ctypes.c_void_p] * 4
         L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_lda_pol.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -139,14 +139,16 @@ def lda_pol(ra, rb, x_func=1, c_func=7, thr=1e-12):
 
 
 FUNCTIONALS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "gga_x_pbe": 101,
-               "gga_c_pbe": 130}
+               "gga_c_pbe": 130, "mgga_x_tpss": 202, "mgga_c_tpss": 231}
 
 
-def xc_eval(rho, sigma, x_func, c_func, thr=1e-12):
-    """Functional values at points: unpolarized rho (n,), sigma (n,) -> (exc,
-    vrho, vsigma); polarized rho (2, n) [a, b], sigma (3, n) [aa, ab, bb] ->
-    (exc (n,), vrho (2, n), vsigma (3, n)).  Energies per particle, potentials
-    d(rho exc)/drho and d(rho exc)/dsigma (complex-step derivatives)."""
+def xc_eval(rho, sigma, x_func, c_func, thr=1e-12, tau=None):
+    """Functional values at points: unpolarized rho (n,), sigma (n,), tau (n,)
+    -> (exc, vrho, vsigma); polarized rho (2, n) [a, b], sigma (3, n) [aa, ab,
+    bb], tau (2, n) -> (exc (n,), vrho (2, n), vsigma (3, n)).  Energies per
+    particle, potentials d(rho exc)/drho and d(rho exc)/dsigma (complex-step
+    derivatives).  With tau given (meta-GGA), vtau = d(rho exc)/dtau is
+    returned fourth."""
     rho = np.ascontiguousarray(rho, dtype=np.float64)
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
     pol = rho.ndim == 2
@@ -154,9 +156,11 @@ def xc_eval(rho, sigma, x_func, c_func, thr=1e-12):
     exc = np.zeros(n)
     vr = np.zeros_like(rho)
     vs = np.zeros_like(sigma)
-    _check(lib().orc_xc_eval(x_func, c_func, int(pol), n, _ptr(rho), _ptr(sigma), thr, _ptr(exc), _ptr(vr),
-                             _ptr(vs)))
-    return exc, vr, vs
+    t = None if tau is None else np.ascontiguousarray(tau, dtype=np.float64)
+    vt = np.zeros_like(rho)
+    _check(lib().orc_xc_eval(x_func, c_func, int(pol), n, _ptr(rho), _ptr(sigma), None if t is None else _ptr(t), thr,
+                             _ptr(exc), _ptr(vr), _ptr(vs), _ptr(vt)))
+    return (exc, vr, vs) if tau is None else (exc, vr, vs, vt)
 
 
 def grid(rmax, nel, igrid=4, zexp=2.0):
@@ -297,8 +301,9 @@ class AtomicBasis:
         mang = 4 * mmax + 12 if mang is None else mang
         P = np.asfortranarray(P, dtype=np.float64)
         H = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_atomic_xc(self._h, lang, mang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        self.last_ekin = float(out[2])
         return H, float(out[0]), float(out[1])
 
     def xc_pol(self, Pa, Pb, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
@@ -310,9 +315,10 @@ class AtomicBasis:
         Pb = np.asfortranarray(Pb, dtype=np.float64)
         Ha = np.zeros((self.Nbf, self.Nbf), order="F")
         Hb = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_atomic_xc_pol(self._h, lang, mang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha),
                                        _ptr(Hb), _ptr(out)))
+        self.last_ekin = float(out[2])
         return Ha, Hb, float(out[0]), float(out[1])
 
     def cache(self, kind, L, iel):
@@ -464,8 +470,9 @@ class DiatomicBasis:
         lang = 4 * int(np.max(self.lval)) + 12 if lang is None else lang
         P = np.asfortranarray(P, dtype=np.float64)
         H = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_diatomic_xc(self._h, lang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        self.last_ekin = float(out[2])
         return H, float(out[0]), float(out[1])
 
     def xc_pol(self, Pa, Pb, lang=None, x_func=1, c_func=7, thr=1e-12):
@@ -475,9 +482,10 @@ class DiatomicBasis:
         Pb = np.asfortranarray(Pb, dtype=np.float64)
         Ha = np.zeros((self.Nbf, self.Nbf), order="F")
         Hb = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_diatomic_xc_pol(self._h, lang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha), _ptr(Hb),
                                          _ptr(out)))
+        self.last_ekin = float(out[2])
         return Ha, Hb, float(out[0]), float(out[1])
 
     def xc3d(self, P, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
@@ -486,8 +494,9 @@ class DiatomicBasis:
         mang = 4 * int(np.max(np.abs(self.mval))) + 12 if mang is None else mang
         P = np.asfortranarray(P, dtype=np.float64)
         H = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_diatomic_xc3d(self._h, lang, mang, x_func, c_func, _ptr(P), thr, _ptr(H), _ptr(out)))
+        self.last_ekin = float(out[2])
         return H, float(out[0]), float(out[1])
 
     def xc3d_pol(self, Pa, Pb, lang=None, mang=None, x_func=1, c_func=7, thr=1e-12):
@@ -498,9 +507,10 @@ class DiatomicBasis:
         Pb = np.asfortranarray(Pb, dtype=np.float64)
         Ha = np.zeros((self.Nbf, self.Nbf), order="F")
         Hb = np.zeros((self.Nbf, self.Nbf), order="F")
-        out = np.zeros(2)
+        out = np.zeros(3)
         _check(lib().orc_diatomic_xc3d_pol(self._h, lang, mang, x_func, c_func, _ptr(Pa), _ptr(Pb), thr, _ptr(Ha),
                                            _ptr(Hb), _ptr(out)))
+        self.last_ekin = float(out[2])
         return Ha, Hb, float(out[0]), float(out[1])
 
     def maps(self):

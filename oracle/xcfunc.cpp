@@ -9,10 +9,17 @@
 //                    kappa = 0.804, mu = beta pi^2 / 3
 //   130 gga_c_pbe    PBE correlation on lda_c_pw_mod, beta = 0.06672455060314922,
 //                    gamma = (1 - ln 2)/pi^2
+//   202 mgga_x_tpss  TPSS exchange (Tao, Perdew, Staroverov, Scuseria, PRL 91,
+//                    146401 (2003)): kappa 0.804, b 0.40, c 1.59096, e 1.537,
+//                    mu 0.21951, by spin scaling
+//   231 mgga_c_tpss  TPSS correlation (same paper): revPKZB on PBE correlation,
+//                    C(zeta, xi) with 0.53, 0.87, 0.50, 2.26, d = 2.8
 // Each functional is one energy density per volume, f(rho_a, rho_b, sigma_aa,
-// sigma_ab, sigma_bb), written once as a template; the potentials
-// (vrho = df/drho, vsigma = df/dsigma) are its complex-step derivatives,
-// Im f(x + i h)/h, exact to rounding for these analytic formulas.
+// sigma_ab, sigma_bb, tau_a, tau_b), written once as a template; the
+// potentials (vrho = df/drho, vsigma = df/dsigma, vtau = df/dtau) are its
+// complex-step derivatives, Im f(x + i h)/h, exact to rounding for these
+// analytic formulas.  Meta-GGAs also follow libxc's tau floor (1e-20) and
+// its Fermi-hole constraint sigma_ss <= 8 rho_s tau_s (z <= 1).
 // Thresholds follow libxc's: a point whose total density is below thr gives
 // zero; 1 +- zeta is floored at DBL_EPSILON; sigma_ab is clamped to
 // +-(sigma_aa + sigma_bb)/2 and the sigmas floored at 1e-40; in a polarized
@@ -77,10 +84,88 @@ template <class T> T pbe_x(const T &n, const T &s2n) {
   return -0.75 * std::cbrt(3.0 / kPi) * p43(n) * F;
 }
 
-// f(rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb) per volume of functional id
+// TPSS exchange of an unpolarized density n, |grad n|^2 = s2n, kinetic energy
+// density tn (per volume)
+template <class T> T tpss_x(const T &n, T s2n, T tn) {
+  const double kappa = 0.804, b = 0.40, c = 1.59096, e = 1.537, mu = 0.21951;
+  const double ckf = std::pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  tn = floor_at(tn, 1e-20);
+  if (re(s2n) > re(8.0 * n * tn)) s2n = 8.0 * n * tn;
+  const T n83 = p43(n) * p43(n);
+  const T p = s2n / (4.0 * ckf * n83);
+  const T tw = s2n / (8.0 * n);
+  const T z = tw / tn;
+  const T tunif = 0.3 * ckf * p43(n) * cbrt_(n);
+  const T alpha = (tn - tw) / tunif;
+  const T qb = 0.45 * (alpha - 1.0) / sqrt(1.0 + b * alpha * (alpha - 1.0)) + 2.0 * p / 3.0;
+  const T z35 = 0.6 * z;
+  const T x = ((10.0 / 81.0 + c * z * z / ((1.0 + z * z) * (1.0 + z * z))) * p + (146.0 / 2025.0) * qb * qb -
+               (73.0 / 405.0) * qb * sqrt(0.5 * z35 * z35 + 0.5 * p * p) + (1.0 / kappa) * (100.0 / 6561.0) * p * p +
+               2.0 * std::sqrt(e) * (10.0 / 81.0) * z35 * z35 + e * mu * p * p * p) /
+              ((1.0 + std::sqrt(e) * p) * (1.0 + std::sqrt(e) * p));
+  return -0.75 * std::cbrt(3.0 / kPi) * p43(n) * (1.0 + kappa - kappa / (1.0 + x / kappa));
+}
+
+template <class T> T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol);
+
+// PBE correlation energy per particle
+template <class T> T pbe_c_eps(const T &ra, const T &rb, const T &saa, const T &sab, const T &sbb, double thr) {
+  return energy(130, ra, rb, saa, sab, sbb, T(0.0), T(0.0), thr, true) / (ra + rb);
+}
+
+// TPSS correlation energy per volume
+template <class T> T tpss_c(T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr) {
+  const double d = 2.8;
+  ta = floor_at(ta, 1e-20);
+  tb = floor_at(tb, 1e-20);
+  saa = floor_at(saa, 1e-40);
+  sbb = floor_at(sbb, 1e-40);
+  if (re(saa) > re(8.0 * ra * ta)) saa = 8.0 * ra * ta;
+  if (re(sbb) > re(8.0 * rb * tb)) sbb = 8.0 * rb * tb;
+  const T sav = 0.5 * (saa + sbb);
+  if (re(sab) > re(sav)) sab = sav;
+  if (re(sab) < -re(sav)) sab = -sav;
+  const T n = ra + rb, sig = saa + 2.0 * sab + sbb, tau = ta + tb;
+  const T z = sig / (8.0 * n * tau);
+  const T zeta = (ra - rb) / n;
+  const T opz = floor_at(T(1.0) + zeta, DBL_EPSILON), omz = floor_at(T(1.0) - zeta, DBL_EPSILON);
+  const T eps = pbe_c_eps(ra, rb, saa, sab, sbb, thr);
+  // epsilon~_s = max(eps_PBE(n_s, 0, grad n_s, 0), eps_PBE(n_up, n_dn, ...)); a spin
+  // below the density threshold carries no weight
+  T sum = T(0.0);
+  if (re(ra) >= thr) {
+    T ea = pbe_c_eps(ra, T(0.0), saa, T(0.0), T(0.0), thr);
+    if (re(ea) < re(eps)) ea = eps;
+    sum = sum + ra / n * ea;
+  }
+  if (re(rb) >= thr) {
+    T eb = pbe_c_eps(rb, T(0.0), sbb, T(0.0), T(0.0), thr);
+    if (re(eb) < re(eps)) eb = eps;
+    sum = sum + rb / n * eb;
+  }
+  const T gz2 = (omz * omz * saa - 2.0 * omz * opz * sab + opz * opz * sbb) / (n * n);
+  const T xi2 = gz2 / (4.0 * p23(3.0 * kPi * kPi * n));
+  const T z2 = zeta * zeta;
+  const T C0 = 0.53 + 0.87 * z2 + 0.50 * z2 * z2 + 2.26 * z2 * z2 * z2;
+  const T den = 1.0 + 0.5 * xi2 * (1.0 / p43(opz) + 1.0 / p43(omz));
+  const T C = C0 / (den * den * den * den);
+  const T erev = eps * (1.0 + C * z * z) - (1.0 + C) * z * z * sum;
+  return n * erev * (1.0 + d * erev * z * z * z);
+}
+
+// f(rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb, tau_a, tau_b) per volume of functional id
 template <class T>
-T energy(int id, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
+T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol) {
   const T rho = ra + rb;
+  if (id == 202) {
+    saa = floor_at(saa, 1e-40);
+    sbb = floor_at(sbb, 1e-40);
+    T e = T(0.0);
+    if (!pol || re(ra) >= thr) e = e + 0.5 * tpss_x(2.0 * ra, 4.0 * saa, 2.0 * ta);
+    if (!pol || re(rb) >= thr) e = e + 0.5 * tpss_x(2.0 * rb, 4.0 * sbb, 2.0 * tb);
+    return e;
+  }
+  if (id == 231) return tpss_c(ra, rb, saa, sab, sbb, ta, tb, thr);
   if (id == 1) {
     T e = T(0.0);
     if (re(ra) > 0.0) e = e + p43(ra);
@@ -128,52 +213,57 @@ T energy(int id, T ra, T rb, T saa, T sab, T sbb, double thr, bool pol) {
 }
 
 template <class T>
-T total(int xf, int cf, const T &ra, const T &rb, const T &saa, const T &sab, const T &sbb, double thr, bool pol) {
+T total(int xf, int cf, const T &ra, const T &rb, const T &saa, const T &sab, const T &sbb, const T &ta, const T &tb,
+        double thr, bool pol) {
   T f = T(0.0);
-  if (xf) f = f + energy(xf, ra, rb, saa, sab, sbb, thr, pol);
-  if (cf) f = f + energy(cf, ra, rb, saa, sab, sbb, thr, pol);
+  if (xf) f = f + energy(xf, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
+  if (cf) f = f + energy(cf, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
   return f;
 }
 
 double step(double x) { return std::fabs(x) > 1e-250 ? 1e-20 * std::fabs(x) : 1e-270; }
 }  // namespace
 
-bool xc_is_gga(int id) { return id == 101 || id == 130; }
+bool xc_is_gga(int id) { return id == 101 || id == 130 || id == 202 || id == 231; }
+bool xc_is_mgga(int id) { return id == 202 || id == 231; }
 
 void xc_check(int xf, int cf) {
-  if (!(xf == 0 || xf == 1 || xf == 101)) throw std::invalid_argument("unsupported exchange functional");
-  if (!(cf == 0 || cf == 7 || cf == 12 || cf == 13 || cf == 130))
+  if (!(xf == 0 || xf == 1 || xf == 101 || xf == 202)) throw std::invalid_argument("unsupported exchange functional");
+  if (!(cf == 0 || cf == 7 || cf == 12 || cf == 13 || cf == 130 || cf == 231))
     throw std::invalid_argument("unsupported correlation functional");
 }
 
-void xc_unpolarized(int xf, int cf, double rho, double sigma, double thr, double &exc, double &vrho,
-                    double &vsigma) {
+void xc_unpolarized(int xf, int cf, double rho, double sigma, double tau, double thr, double &exc, double &vrho,
+                    double &vsigma, double &vtau) {
   xc_check(xf, cf);
-  exc = vrho = vsigma = 0.0;
+  exc = vrho = vsigma = vtau = 0.0;
   if (!(rho >= thr)) return;
-  auto f = [&](auto r, auto s) { return total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, thr, false); };
-  exc = f(rho, sigma) / rho;
-  const double hr = step(rho), hs = step(sigma);
-  vrho = f(cd(rho, hr), cd(sigma, 0.0)).imag() / hr;
-  vsigma = f(cd(rho, 0.0), cd(sigma, hs)).imag() / hs;
+  auto f = [&](auto r, auto s, auto t) {
+    return total(xf, cf, 0.5 * r, 0.5 * r, 0.25 * s, 0.25 * s, 0.25 * s, 0.5 * t, 0.5 * t, thr, false);
+  };
+  exc = f(rho, sigma, tau) / rho;
+  const double hr = step(rho), hs = step(sigma), ht = step(tau);
+  vrho = f(cd(rho, hr), cd(sigma, 0.0), cd(tau, 0.0)).imag() / hr;
+  vsigma = f(cd(rho, 0.0), cd(sigma, hs), cd(tau, 0.0)).imag() / hs;
+  vtau = f(cd(rho, 0.0), cd(sigma, 0.0), cd(tau, ht)).imag() / ht;
 }
 
-void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], double thr, double &exc,
-                  double vrho[2], double vsigma[3]) {
+void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], const double tau[2], double thr,
+                  double &exc, double vrho[2], double vsigma[3], double vtau[2]) {
   xc_check(xf, cf);
   exc = 0.0;
-  vrho[0] = vrho[1] = vsigma[0] = vsigma[1] = vsigma[2] = 0.0;
+  vrho[0] = vrho[1] = vsigma[0] = vsigma[1] = vsigma[2] = vtau[0] = vtau[1] = 0.0;
   const double rt = rho[0] + rho[1];
   if (!(rt >= thr)) return;
-  double x[5] = {rho[0], rho[1], sigma[0], sigma[1], sigma[2]};
-  exc = total(xf, cf, x[0], x[1], x[2], x[3], x[4], thr, true) / rt;
-  double *out[5] = {&vrho[0], &vrho[1], &vsigma[0], &vsigma[1], &vsigma[2]};
-  for (int k = 0; k < 5; ++k) {
-    cd y[5];
-    for (int j = 0; j < 5; ++j) y[j] = cd(x[j], 0.0);
+  double x[7] = {rho[0], rho[1], sigma[0], sigma[1], sigma[2], tau[0], tau[1]};
+  exc = total(xf, cf, x[0], x[1], x[2], x[3], x[4], x[5], x[6], thr, true) / rt;
+  double *out[7] = {&vrho[0], &vrho[1], &vsigma[0], &vsigma[1], &vsigma[2], &vtau[0], &vtau[1]};
+  for (int k = 0; k < 7; ++k) {
+    cd y[7];
+    for (int j = 0; j < 7; ++j) y[j] = cd(x[j], 0.0);
     const double h = step(x[k]);
     y[k] = cd(x[k], h);
-    *out[k] = total(xf, cf, y[0], y[1], y[2], y[3], y[4], thr, true).imag() / h;
+    *out[k] = total(xf, cf, y[0], y[1], y[2], y[3], y[4], y[5], y[6], thr, true).imag() / h;
   }
 }
 

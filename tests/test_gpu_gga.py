@@ -47,10 +47,10 @@ def _bases(name, ctx):
     return O.AtomicBasis(*args, **c), H.AtomicBasis(*args, ctx=ctx, **c)
 
 
-def _density(ob, seed, scale=1.0):
+def _density(ob, seed, scale=1.0, nocc=1):
     S = ob.overlap()
     blocks = ob.sym_idx(1)
-    return scale * seeded_density(S, blocks, [1] * len(blocks), seed=seed)
+    return scale * seeded_density(S, blocks, [nocc] * len(blocks), seed=seed)
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -173,3 +173,96 @@ def test_he_pbe_scf_on_gpu(restricted, gpu_ctx):
     torch.cuda.synchronize()
     print(f"[gpu]   E = {r.E:.10f} (reference {REF[name]['total']})", flush=True)
     assert abs(r.E - REF[name]["total"]) < 1e-9, (r.E, REF[name]["total"])
+
+
+# ---- meta-GGA (TPSS, mgga_x_tpss 202 + mgga_c_tpss 231): kinetic energy
+# density and the vtau terms on every grid; Ekin (the integral of tau) is the
+# third output, as DFTGrid::eval_Fxc returns it.  The densities hold two
+# orbitals per block: a one-orbital density has tau = tau_W exactly, z = 1, on
+# libxc's sigma <= 8 rho tau cap, where the split of the potential between
+# vsigma and vtau follows the rounding (equal energies, different matrices)
+
+@pytest.mark.parametrize("name", ["He-l0-5el", "Ne-l2m1-3el-n8", "Kr-l3m3-2el-n10"])
+def test_tpss_matrix_matches_oracle(name, gpu_ctx):
+    print(f"[gpu] atomic TPSS Vxc: {name}", flush=True)
+    ob, hb = _bases(name, gpu_ctx)
+    P = _density(ob, 2403, nocc=2)
+    grid = H.DFTGrid(hb)
+    Hr, Er, Nr = ob.xc(P, x_func=202, c_func=231)
+    Kr = ob.last_ekin
+    Hg, Eg, Ng, Kg = grid.eval_Fxc(202, 231, P)
+    print(f"[gpu]   H {relerr(Hg, Hr):.2e} Exc {abs(Eg - Er) / abs(Er):.2e} Ekin {abs(Kg - Kr) / abs(Kr):.2e}", flush=True)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr) and abs(Kg - Kr) <= TOL * abs(Kr)
+    Pb = _density(ob, 2404, 0.6, nocc=2)
+    Har, Hbr, Er, Nr = ob.xc_pol(P, Pb, x_func=202, c_func=231)
+    Kr = ob.last_ekin
+    Hag, Hbg, Eg, Ng, Kg = grid.eval_Fxc_pol(202, 231, P, Pb)
+    print(f"[gpu]   Ha {relerr(Hag, Har):.2e} Hb {relerr(Hbg, Hbr):.2e}", flush=True)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Kg - Kr) <= TOL * abs(Kr)
+
+
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_diatomic_tpss_matrix_matches_oracle(name, gpu_ctx):
+    """Pure-m grid (tau's m^2 / h_phi^2 term, dftgrid_purem.cpp:300-318) and the
+    3D grid, restricted and polarized."""
+    from helfem_amd.synthetic import random_symmetric
+    print(f"[gpu] diatomic TPSS Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    P, Pb = _density(ob, 2405, nocc=2), _density(ob, 2406, 0.5, nocc=2)
+    grid = H.DFTGrid(hb)
+    Hr, Er, _ = ob.xc(P, x_func=202, c_func=231)
+    Kr = ob.last_ekin
+    Hg, Eg, _, Kg = grid.eval_Fxc(202, 231, P)
+    assert relerr(Hg, Hr) <= TOL and abs(Eg - Er) <= TOL * abs(Er) and abs(Kg - Kr) <= TOL * abs(Kr)
+    Har, Hbr, Er, _ = ob.xc_pol(P, Pb, x_func=202, c_func=231)
+    Hag, Hbg, Eg, _, _ = grid.eval_Fxc_pol(202, 231, P, Pb)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL and abs(Eg - Er) <= TOL * abs(Er)
+    S = ob.overlap()
+    X = random_symmetric(ob.Nbf, seed=1511)
+    P3 = X @ S @ X.T
+    P3 *= 4.0 / np.sum(P3 * S)
+    g3 = H.DFTGrid(hb, purem=False)
+    Hr, Er, _ = ob.xc3d(P3, x_func=202, c_func=231)
+    Hg, Eg, _, _ = g3.eval_Fxc(202, 231, P3)
+    print(f"[gpu]   3D grid H {relerr(Hg, Hr):.2e}", flush=True)
+    assert relerr(Hg, Hr) <= TOL and abs(Eg - Er) <= TOL * abs(Er)
+    Har, Hbr, Er, _ = ob.xc3d_pol(P3, 0.3 * P3, x_func=202, c_func=231)
+    Hag, Hbg, Eg, _, _ = g3.eval_Fxc_pol(202, 231, P3, 0.3 * P3)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL and abs(Eg - Er) <= TOL * abs(Er)
+
+
+@pytest.mark.parametrize("restricted", [True, False], ids=["r", "u"])
+def test_he_tpss_scf_on_gpu(restricted, gpu_ctx):
+    import torch
+    from helfem_amd.scf import rhf, uhf
+    name = "atomic-He-mgga-" + ("r" if restricted else "u")
+    print(f"[gpu] atomic TPSS SCF: {name}", flush=True)
+    hb = H.AtomicBasis(2, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    if restricted:
+        r = rhf(hb, 2, xc=lambda P: grid.eval_Fxc(202, 231, P)[:2], conv=1e-12)
+    else:
+        r = uhf(hb, 1, 1, xc_pol=lambda Pa, Pb: grid.eval_Fxc_pol(202, 231, Pa, Pb)[:3], conv=1e-12)
+    torch.cuda.synchronize()
+    print(f"[gpu]   E = {r.E:.10f} (reference {REF[name]['total']})", flush=True)
+    assert abs(r.E - REF[name]["total"]) < 1e-9, (r.E, REF[name]["total"])
+
+
+def test_ne_dummy_centre_tpss_scf_on_gpu(gpu_ctx):
+    """diatomic-Ne-dummy-mgga: TPSS on the pure-m grid; reference -128.981103006."""
+    import torch
+    from helfem_amd.scf import rhf
+    print("[gpu] diatomic Ne dummy-centre TPSS SCF", flush=True)
+    hb = H.DiatomicBasis(10, 0, 1.0, [13, 9], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    r = rhf(hb, 10, xc=lambda P: grid.eval_Fxc(202, 231, P)[:2], conv=1e-12)
+    torch.cuda.synchronize()
+    ref = REF["diatomic-Ne-dummy-mgga"]["total"]
+    print(f"[gpu]   E = {r.E:.10f} (reference {ref})", flush=True)
+    assert abs(r.E - ref) < 1e-9, (r.E, ref)

@@ -245,3 +245,120 @@ def test_ne_dummy_centre_pbe_energy():
     b.compute_tei()
     r = O.scf_rhf(b, 10, xc=lambda P: b.xc(P, x_func=101, c_func=130)[:2])
     assert abs(r["E"] - REF["diatomic-Ne-dummy-gga"]["total"]) < 1e-9, r["E"]
+
+
+# ---- meta-GGA: TPSS exchange (mgga_x_tpss, 202) and correlation (mgga_c_tpss,
+# 231), Tao, Perdew, Staroverov, Scuseria, PRL 91, 146401 (2003); the grids'
+# kinetic energy density (dftgrid.cpp:89-107, 336-349; dftgrid_purem.cpp:
+# 287-318, 482-494)
+
+def _tau_unif(rho):
+    return 0.3 * (3 * np.pi ** 2) ** (2 / 3) * rho ** (5 / 3)
+
+
+def test_tpss_uniform_gas_limits():
+    """zero gradient and tau = tau_unif (alpha = 1, z = 0, p = 0): TPSS exchange
+    is Slater's and TPSS correlation is PW92 (modified constants)."""
+    rho = np.geomspace(1e-5, 1e3, 9)
+    z, tu = np.zeros_like(rho), _tau_unif(rho)
+    ex, vx, _, _ = O.xc_eval(rho, z, 202, 0, tau=tu)
+    e0, v0 = O.lda(rho, 1, 0)
+    assert np.max(np.abs(ex / e0 - 1)) < 1e-13
+    ec, _, _, _ = O.xc_eval(rho, z, 0, 231, tau=tu)
+    ep, _, _ = O.xc_eval(rho, z, 0, 13)
+    assert np.max(np.abs(ec / ep - 1)) < 1e-13
+
+
+def test_tpss_correlation_vanishes_for_one_electron_densities():
+    """A fully polarized density with tau = tau_W (one orbital) has no TPSS
+    correlation energy: revPKZB's self-interaction correction cancels PBE's."""
+    rho = np.geomspace(1e-4, 10, 8)
+    sigma = np.random.default_rng(3).uniform(0.1, 2.0, rho.size) * rho ** (8 / 3)
+    r2 = np.stack([rho, np.zeros_like(rho)])
+    s3 = np.stack([sigma, np.zeros_like(rho), np.zeros_like(rho)])
+    t2 = np.stack([sigma / (8 * rho), np.zeros_like(rho)])
+    ec, _, _, _ = O.xc_eval(r2, s3, 0, 231, tau=t2)
+    ep, _, _ = O.xc_eval(r2, s3, 0, 130)
+    assert np.max(np.abs(ec)) < 1e-10 * np.max(np.abs(ep)), ec  # the 1e-20 tau floor moves z off 1 by ~1e-13
+
+
+@pytest.mark.parametrize("xf,cf", [(202, 0), (0, 231)])
+def test_tpss_potentials_are_energy_derivatives(xf, cf):
+    rng = np.random.default_rng(12)
+    rho = np.geomspace(1e-3, 20, 12)
+    sigma = rng.uniform(0.05, 1.0, rho.size) * rho ** (8 / 3)
+    tau = sigma / (8 * rho) * rng.uniform(1.2, 3.0, rho.size)   # z = tau_W / tau in (1/3, 5/6)
+    e, vr, vs, vt = O.xc_eval(rho, sigma, xf, cf, tau=tau)
+    for k, (x, v) in enumerate(((rho, vr), (sigma, vs), (tau, vt))):
+        def f(y, k=k):
+            a = [rho, sigma, tau]
+            a[k] = y
+            return a[0] * O.xc_eval(a[0], a[1], xf, cf, tau=a[2])[0]
+        fd = _fd(f, x, 1e-4 * x)
+        assert np.max(np.abs(fd - v) / np.abs(v)) < 1e-7, k
+    r2 = np.stack([0.6 * rho, 0.4 * rho])
+    s3 = np.stack([0.36 * sigma, 0.2 * sigma, 0.16 * sigma])
+    t2 = np.stack([0.55 * tau, 0.45 * tau])
+    e, vr, vs, vt = O.xc_eval(r2, s3, xf, cf, tau=t2)
+    for k in range(7):
+        def f(y, k=k):
+            a, b, c = r2.copy(), s3.copy(), t2.copy()
+            (a if k < 2 else b if k < 5 else c)[k if k < 2 else k - 2 if k < 5 else k - 5] = y
+            return (a[0] + a[1]) * O.xc_eval(a, b, xf, cf, tau=c)[0]
+        x = r2[k] if k < 2 else s3[k - 2] if k < 5 else t2[k - 5]
+        v = vr[k] if k < 2 else vs[k - 2] if k < 5 else vt[k - 5]
+        fd = _fd(f, x, 1e-4 * x)
+        assert np.max(np.abs(fd - v) / (np.abs(v) + 1e-300)) < 1e-7, k
+
+
+@pytest.mark.parametrize("grid", ["atomic", "purem", "3d"])
+def test_tpss_matrix_is_energy_derivative(ne_basis, nh_basis, nh_small, grid):
+    """<H, D> = dExc(P + h D)/dh with TPSS on every grid: the kinetic energy
+    density (radial, angular and, on the diatomic grids, the m^2/h_phi^2 term)
+    and the vtau/2 potential terms carry their factors correctly."""
+    if grid == "atomic":
+        b, P, D = ne_basis, *_density(ne_basis, 31)
+        xc = lambda A: b.xc(A, x_func=202, c_func=231)
+        xcp = lambda A, B: b.xc_pol(A, B, x_func=202, c_func=231)
+    elif grid == "purem":
+        b = nh_basis
+        P, D = (_m_diagonal(b, M) for M in _density(b, 32))
+        xc = lambda A: b.xc(A, x_func=202, c_func=231)
+        xcp = lambda A, B: b.xc_pol(A, B, x_func=202, c_func=231)
+    else:
+        b, P, D = nh_small, *_density(nh_small, 33)
+        xc = lambda A: b.xc3d(A, x_func=202, c_func=231)
+        xcp = lambda A, B: b.xc3d_pol(A, B, x_func=202, c_func=231)
+    # TPSS is only piecewise smooth (the z <= 1 cap of libxc), so the bound is
+    # looser than for PBE; a missing factor would show at order one
+    H, E, _ = xc(P)
+    h = 1e-5
+    fd = (xc(P + h * D)[1] - xc(P - h * D)[1]) / (2 * h)
+    assert abs(fd / np.sum(H * D) - 1) < 1e-6
+    Pa, Pb = 0.6 * P, 0.4 * P
+    Ha, Hb, _, _ = xcp(Pa, Pb)
+    for which, Hs in ((0, Ha), (1, Hb)):
+        e = [xcp(Pa + s * h * D, Pb)[2] if which == 0 else xcp(Pa, Pb + s * h * D)[2] for s in (1, -1)]
+        assert abs((e[0] - e[1]) / (2 * h) / np.sum(Hs * D) - 1) < 1e-6, which
+
+
+def test_he_tpss_energies():
+    b = O.AtomicBasis(2, 0, 0, 5)
+    b.compute_tei()
+    r = O.scf_rhf(b, 2, xc=lambda P: b.xc(P, x_func=202, c_func=231)[:2])
+    assert abs(r["E"] - REF["atomic-He-mgga-r"]["total"]) < 1e-9, r["E"]
+    u = O.scf_uhf(b, 1, 1, xc_pol=lambda Pa, Pb: b.xc_pol(Pa, Pb, x_func=202, c_func=231)[:3])
+    assert abs(u["E"] - REF["atomic-He-mgga-u"]["total"]) < 1e-9, u["E"]
+    # the kinetic energy density integrates to the kinetic energy of the orbitals
+    _, _, _ = b.xc(r["P"], x_func=202, c_func=231)
+    T = np.sum(r["P"] * b.kinetic())
+    assert abs(b.last_ekin - T) < 1e-6 * T, (b.last_ekin, T)
+
+
+def test_ne_dummy_centre_tpss_energy():
+    """diatomic-Ne-dummy-mgga (tests/refs/ci.json, -128.981103006): TPSS on the
+    pure-m grid, Z1 = 10, Z2 = 0, R = 1 bohr, lmax 13 (m=0) and 9 (|m|=1)."""
+    b = O.DiatomicBasis(10, 0, 1.0, [13, 9], 3)
+    b.compute_tei()
+    r = O.scf_rhf(b, 10, xc=lambda P: b.xc(P, x_func=202, c_func=231)[:2])
+    assert abs(r["E"] - REF["diatomic-Ne-dummy-mgga"]["total"]) < 1e-9, r["E"]
