@@ -265,13 +265,14 @@ int hf_diatomic_dftgrid_create(hf_diatomic *h, int lang, hf_dftgrid **out);
 int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid **out);
 /* Restricted exchange-correlation matrix H (Nbf x Nbf, overwritten) of the
  * total density P, with out[0] = Exc, out[1] = Nel = integral of the
- * density, out[2] = kinetic energy density integral (0 for LDA and GGA).
- * x_func / c_func are libxc functional ids: exchange 0 (none), 1 (lda_x),
- * 101 (gga_x_pbe); correlation 0, 7 (lda_c_vwn, VWN5), 12 (lda_c_pw),
- * 13 (lda_c_pw_mod), 130 (gga_c_pbe), on every grid; others return
- * HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
+ * density, out[2] = Ekin = integral of the kinetic energy density tau (0
+ * without a meta-GGA).  x_func / c_func are libxc functional ids: exchange
+ * 0 (none), 1 (lda_x), 101 (gga_x_pbe), 202 (mgga_x_tpss); correlation 0,
+ * 7 (lda_c_vwn, VWN5), 12 (lda_c_pw), 13 (lda_c_pw_mod), 130 (gga_c_pbe),
+ * 231 (mgga_c_tpss), on every grid; others return HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
  * (src/atomic/dftgrid.h:156, dftgrid.cpp:591-653, with the GGA gradient
- * terms of dftgrid.cpp:75-86, 320-334), PureMDFTGrid::eval_Fxc
+ * terms of dftgrid.cpp:75-86, 320-334 and the meta-GGA tau terms of
+ * :89-107, 336-349), PureMDFTGrid::eval_Fxc
  * (src/diatomic/dftgrid_purem.h:154, dftgrid_purem.cpp:675-703, GGA terms
  * :248-260, 468-500) and the 3D diatomic DFTGrid::eval_Fxc for functionals
  * without external parameters. */

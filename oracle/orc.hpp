@@ -18,7 +18,9 @@
 //                                  Exx = -Coulomb/2 (He closed shell), J symmetric, K symmetric
 //   PBE (xcfunc.cpp) + GGA grid .. uniform-gas and gradient-expansion limits, Lieb-Oxford saturation,
 //                                  finite-difference potentials and dExc/dP, He PBE energies
-//                                  (restricted and unrestricted) of tests/refs/ci.json
+//                                  (restricted and unrestricted) of tests/refs/ci.json, Ne (diatomic)
+//   TPSS (xcfunc.cpp) + tau ...... uniform-gas limits, vanishing one-electron correlation,
+//                                  finite-difference potentials and dExc/dP, He and Ne TPSS energies
 #pragma once
 #include <vector>
 #include <cmath>
