@@ -66,6 +66,7 @@ SIGNATURES = {
     "hf_diatomic_dftgrid_create": (c_int, [c_vp, c_int, ctypes.POINTER(c_vp)]),
     "hf_diatomic_dftgrid3d_create": (c_int, [c_vp, c_int, c_int, c_vp]),
     "hf_dftgrid_destroy": (c_int, [c_vp]),
+    "hf_xc_exact_exchange": (c_int, [c_int, c_vp]),
     "hf_dftgrid_eval_fxc": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
     "hf_dftgrid_eval_fxc_dev": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dbl, c_vp]),
     "hf_dftgrid_eval_fxc_pol": (c_int, [c_vp, c_int, c_int, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64, c_dp, c_i64,

@@ -1,10 +1,16 @@
 // Exchange-correlation functionals by libxc id, for the device (and host):
-//   1 lda_x, 7 lda_c_vwn (VWN5), 12 lda_c_pw (PW92 as printed), 13 lda_c_pw_mod
-//   (libxc's more precise PW92 constants), 101 gga_x_pbe, 130 gga_c_pbe,
-//   202 mgga_x_tpss, 231 mgga_c_tpss
+//   1 lda_x, 7 lda_c_vwn (VWN5), 8 lda_c_vwn_rpa, 12 lda_c_pw (PW92 as
+//   printed), 13 lda_c_pw_mod (libxc's more precise PW92 constants),
+//   101 gga_x_pbe, 106 gga_x_b88, 130 gga_c_pbe, 131 gga_c_lyp,
+//   202 mgga_x_tpss, 231 mgga_c_tpss, and the global hybrids
+//   402 hyb_gga_xc_b3lyp (0.08 lda_x + 0.72 gga_x_b88 + 0.19 lda_c_vwn_rpa +
+//   0.81 gga_c_lyp; 0.20 exact exchange) and 406 hyb_gga_xc_pbeh (PBE0:
+//   0.75 gga_x_pbe + gga_c_pbe; 0.25 exact exchange)
 // from the published formulas (Slater; Vosko-Wilk-Nusair, Can. J. Phys. 58,
-// 1200 (1980); Perdew-Wang, PRB 45, 13244 (1992); Perdew-Burke-Ernzerhof,
-// PRL 77, 3865 (1996); Tao-Perdew-Staroverov-Scuseria, PRL 91, 146401 (2003))
+// 1200 (1980), QMC and RPA fits; Perdew-Wang, PRB 45, 13244 (1992);
+// Becke, PRA 38, 3098 (1988); Lee-Yang-Parr in the closed form of Miehlich
+// et al., CPL 157, 200 (1989); Perdew-Burke-Ernzerhof, PRL 77, 3865 (1996);
+// Tao-Perdew-Staroverov-Scuseria, PRL 91, 146401 (2003))
 // with libxc's constants and thresholds: no output below the density
 // threshold, 1 +- zeta floored at DBL_EPSILON, sigma_aa and sigma_bb floored
 // at 1e-40, sigma_ab clamped to +-(sigma_aa + sigma_bb)/2; a polarized spin
@@ -61,6 +67,7 @@ __host__ __device__ inline Dl exp(Dl a) {
   return {e, e * a.d};
 }
 __host__ __device__ inline Dl atan(Dl a) { return {::atan(a.v), a.d / (1.0 + a.v * a.v)}; }
+__host__ __device__ inline Dl asinh(Dl a) { return {::asinh(a.v), a.d / ::sqrt(1.0 + a.v * a.v)}; }
 __host__ __device__ inline double val(Dl a) { return a.v; }
 __host__ __device__ inline double val(double a) { return a; }
 
@@ -205,10 +212,76 @@ __host__ __device__ inline T tpss_c(T ra, T rb, T saa, T sab, T sbb, T ta, T tb,
   return n * erev * (1.0 + d * erev * z * z * z);
 }
 
+// Slater exchange per volume, spin-scaled
+template <class T> __host__ __device__ inline T slater_x(T ra, T rb) {
+  T e = ra * 0.0;
+  if (val(ra) > 0.0) e = e + p43(ra);
+  if (val(rb) > 0.0) e = e + p43(rb);
+  return -0.75 * ::cbrt(6.0 / kPi) * e;
+}
+
+// Becke 88 exchange per volume of one spin channel (density r, sigma s):
+// -r^(4/3) [C_x + beta x^2 / (1 + 6 beta x asinh x)], x = sqrt(s) / r^(4/3)
+template <class T> __host__ __device__ inline T b88_x(T r, T s) {
+  const double beta = 0.0042, cx = 0.75 * ::cbrt(6.0 / kPi);
+  const T r43 = p43(r), x = sqrt(s) / r43;
+  return -1.0 * r43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * asinh(x)));
+}
+
+template <class T> __host__ __device__ inline T b88_pair(T ra, T rb, T saa, T sbb, double thr, bool pol) {
+  T e = ra * 0.0;
+  if (!pol || val(ra) >= thr) e = e + b88_x(ra, floor_at(saa, 1e-40));
+  if (!pol || val(rb) >= thr) e = e + b88_x(rb, floor_at(sbb, 1e-40));
+  return e;
+}
+
+// PBE exchange of both spin channels by spin scaling
+template <class T> __host__ __device__ inline T pbe_x_pair(T ra, T rb, T saa, T sbb, double thr, bool pol) {
+  T e = ra * 0.0;
+  if (!pol || val(ra) >= thr) e = e + 0.5 * pbe_x(2.0 * ra, 4.0 * floor_at(saa, 1e-40));
+  if (!pol || val(rb) >= thr) e = e + 0.5 * pbe_x(2.0 * rb, 4.0 * floor_at(sbb, 1e-40));
+  return e;
+}
+
+// VWN RPA fit per volume: paramagnetic and ferromagnetic curves mixed by f(zeta)
+template <class T> __host__ __device__ inline T vwn_rpa_c(T rho, T rs, T opz, T omz) {
+  const T x = sqrt(rs);
+  const T eP = vwn_F(0.0310907, 13.0720, 42.7198, -0.409286, x);
+  const T eF = vwn_F(0.01554535, 20.1231, 101.578, -0.743294, x);
+  return rho * (eP + (eF - eP) * fzeta(opz, omz));
+}
+
+// LYP correlation per volume (Miehlich et al. closed form) on the spin
+// densities rebuilt from rho and the floored 1 +- zeta, sigmas floored and
+// sigma_ab clamped as for every GGA
+template <class T> __host__ __device__ inline T lyp_c(T rho, T opz, T omz, T saa, T sab, T sbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double cf = 0.3 * ::cbrt(9.0 * kPi * kPi * kPi * kPi), c113 = 12.699208415745595;  // 2^(11/3)
+  saa = floor_at(saa, 1e-40);
+  sbb = floor_at(sbb, 1e-40);
+  const T sav = 0.5 * (saa + sbb);
+  if (val(sab) > val(sav)) sab = sav;
+  if (val(sab) < -val(sav)) sab = -1.0 * sav;
+  const T ra = 0.5 * rho * opz, rb = 0.5 * rho * omz;
+  const T rm = 1.0 / cbrt(rho), den = 1.0 + d * rm;
+  const T rm3 = rm * rm * rm;
+  const T omega = exp(-c * rm) / den * (rm3 * rm3 * rm3 * rm * rm);  // ... rho^(-11/3)
+  const T delta = c * rm + d * rm / den;
+  const T sig = saa + 2.0 * sab + sbb;
+  const T ra43 = p43(ra), rb43 = p43(rb);
+  const T br = ra * rb *
+               (c113 * cf * (ra43 * ra43 + rb43 * rb43) + (47.0 / 18.0 - 7.0 / 18.0 * delta) * sig -
+                (2.5 - delta / 18.0) * (saa + sbb) - (delta - 11.0) / 9.0 * (ra * saa + rb * sbb) / rho);
+  const T r2 = (2.0 / 3.0) * rho * rho;
+  return -4.0 * a * ra * rb / (rho * den) -
+         a * b * omega * (br - r2 * sig + (r2 - ra * ra) * sbb + (r2 - rb * rb) * saa);
+}
+
 // energy per volume of functional `id` (0: none)
 template <class T>
 __host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol) {
   if (id == 0) return ra * 0.0;
+  if (id == 106) return b88_pair(ra, rb, saa, sbb, thr, pol);
   if (id == 202) {
     T e = ra * 0.0;
     if (!pol || val(ra) >= thr) e = e + 0.5 * tpss_x(2.0 * ra, 4.0 * floor_at(saa, 1e-40), 2.0 * ta);
@@ -216,18 +289,9 @@ __host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, T t
     return e;
   }
   if (id == 231) return tpss_c(ra, rb, saa, sab, sbb, ta, tb, thr);
-  if (id == 1) {
-    T e = ra * 0.0;
-    if (val(ra) > 0.0) e = e + p43(ra);
-    if (val(rb) > 0.0) e = e + p43(rb);
-    return -0.75 * ::cbrt(6.0 / kPi) * e;
-  }
-  if (id == 101) {
-    T e = ra * 0.0;
-    if (!pol || val(ra) >= thr) e = e + 0.5 * pbe_x(2.0 * ra, 4.0 * floor_at(saa, 1e-40));
-    if (!pol || val(rb) >= thr) e = e + 0.5 * pbe_x(2.0 * rb, 4.0 * floor_at(sbb, 1e-40));
-    return e;
-  }
+  if (id == 1) return slater_x(ra, rb);
+  if (id == 101) return pbe_x_pair(ra, rb, saa, sbb, thr, pol);
+  if (id == 406) return 0.75 * pbe_x_pair(ra, rb, saa, sbb, thr, pol) + pbe_c(ra, rb, saa, sab, sbb);
   const T rho = ra + rb;
   const T z = (ra - rb) / rho;
   const T opz = floor_at(1.0 + z, DBL_EPSILON), omz = floor_at(1.0 - z, DBL_EPSILON);
@@ -242,15 +306,31 @@ __host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, T t
     return rho * (eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4);
   }
   if (id == 12 || id == 13) return rho * pw92(id == 13, rs, z, opz, omz);
+  if (id == 8) return vwn_rpa_c(rho, rs, opz, omz);
+  if (id == 131) return lyp_c(rho, opz, omz, saa, sab, sbb);
+  if (id == 402)
+    return 0.08 * slater_x(ra, rb) + 0.72 * b88_pair(ra, rb, saa, sbb, thr, pol) +
+           0.19 * vwn_rpa_c(rho, rs, opz, omz) + 0.81 * lyp_c(rho, opz, omz, saa, sab, sbb);
   // 130: PBE correlation
   return pbe_c(ra, rb, saa, sab, sbb);
 }
 
-__host__ __device__ inline bool is_gga(int id) { return id == 101 || id == 130 || id == 202 || id == 231; }
+__host__ __device__ inline bool is_gga(int id) {
+  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406;
+}
 __host__ __device__ inline bool is_mgga(int id) { return id == 202 || id == 231; }
-__host__ __device__ inline bool exchange_ok(int id) { return id == 0 || id == 1 || id == 101 || id == 202; }
+__host__ __device__ inline bool exchange_ok(int id) {
+  return id == 0 || id == 1 || id == 101 || id == 106 || id == 202 || id == 402 || id == 406;
+}
 __host__ __device__ inline bool correlation_ok(int id) {
-  return id == 0 || id == 7 || id == 12 || id == 13 || id == 130 || id == 231;
+  return id == 0 || id == 7 || id == 8 || id == 12 || id == 13 || id == 130 || id == 131 || id == 231;
+}
+
+// exact-exchange fractions of a hybrid (full range, short range) and its
+// range-separation parameter; zero for the pure functionals
+__host__ __device__ inline void exact_exchange(int id, double &kfrac, double &kshort, double &omega) {
+  kfrac = id == 402 ? 0.20 : id == 406 ? 0.25 : 0.0;
+  kshort = omega = 0.0;
 }
 
 template <class T>

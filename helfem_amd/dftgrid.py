@@ -8,8 +8,9 @@ restricted exchange-correlation matrix with Exc, the integrated electron
 count and the kinetic-energy-density integral, like DFTGrid::eval_Fxc
 (dftgrid.cpp:591-653); `eval_Fxc_pol(x_func, c_func, Pa, Pb)` is the
 unrestricted overload (dftgrid.cpp:655-712) returning (Ha, Hb, Exc, Nel, Ekin).  Functionals are libxc ids; parse_method turns the
-driver's --method string (e.g. "lda_x-lda_c_vwn", "gga_x_pbe-gga_c_pbe", "mgga_x_tpss-mgga_c_tpss") into them;
-every grid evaluates all of them.
+driver's --method string (e.g. "lda_x-lda_c_vwn", "gga_x_pbe-gga_c_pbe", "mgga_x_tpss-mgga_c_tpss",
+"hyb_gga_xc_b3lyp") into them; every grid evaluates all of them, and exact_exchange gives a hybrid's
+exact-exchange fraction.
 """
 from __future__ import annotations
 
@@ -21,8 +22,19 @@ from . import _lib
 from ._lib import check, lib
 from .atomic import _f64_fortran
 
-FUNCTIONALS = {"none": 0, "lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "gga_x_pbe": 101,
-               "gga_c_pbe": 130, "mgga_x_tpss": 202, "mgga_c_tpss": 231}
+FUNCTIONALS = {"none": 0, "lda_x": 1, "lda_c_vwn": 7, "lda_c_vwn_rpa": 8, "lda_c_pw": 12, "lda_c_pw_mod": 13,
+               "gga_x_pbe": 101, "gga_x_b88": 106, "gga_c_pbe": 130, "gga_c_lyp": 131, "mgga_x_tpss": 202,
+               "mgga_c_tpss": 231, "hyb_gga_xc_b3lyp": 402, "hyb_gga_xc_pbeh": 406}
+
+
+def exact_exchange(x_func: int):
+    """(kfrac, kshort, omega): the full-range and short-range exact-exchange
+    fractions of functional x_func and its range-separation parameter, as
+    range_separation (src/general/dftfuncs.cpp:511) gives them; the Fock
+    matrix of a hybrid takes kfrac * exchange(P) (src/atomic/main.cpp:424-437)."""
+    out = np.zeros(3)
+    check(lib().hf_xc_exact_exchange(int(x_func), out.ctypes.data))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def parse_method(method: str):

@@ -520,13 +520,17 @@ class SCFState:
     nel_spin = (nela, nelb), xc(Pa, Pb) -> (Va, Vb, Exc)."""
 
     def __init__(self, basis, nel_spin, restricted, symm=1, conv=1e-10, maxit=100, diis_len=10, verbose=False,
-                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None, gconv=1e-6):
+                 xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None, gconv=1e-6,
+                 kfrac=None):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.torch = torch
         self.basis, self.nel_spin, self.restricted = basis, tuple(nel_spin), restricted
         self.conv, self.maxit, self.verbose, self.xc, self.switch = conv, maxit, verbose, xc, switch
+        # fraction of exact exchange in the Fock matrix: 1 for Hartree-Fock, 0
+        # for a pure functional, the hybrid's (exact_exchange) otherwise
+        self.kfrac = (1.0 if xc is None else 0.0) if kfrac is None else float(kfrac)
         self.gconv = gconv  # largest orbital-gradient (commutator) entry at convergence
         self.tag = tag
         self.dev = torch.device(device)
@@ -636,7 +640,7 @@ class SCFState:
             made.append(J)
             tick("coulomb")
             Ks, Vs, Exc = [None] * nspin, [None] * nspin, None
-            if self.xc is None:
+            if self.kfrac:
                 spins = [s for s in range(nspin) if self.nel_spin[s] > 0]
                 Psps = {s: (0.5 * Ps[s] if self.restricted else Ps[s]) for s in spins}
                 # incremental only when every spin's change from its reference is small
@@ -657,6 +661,8 @@ class SCFState:
                         K = self.basis.exchange(Psp, precision="fp64")
                         if self.mixed:
                             self.kref[s] = (Psp, K)
+                    if self.kfrac != 1.0:
+                        K = self.kfrac * K
                     Ks[s] = (Psp, K)
                     made += list(Ks[s])
                 if inc:
@@ -664,7 +670,7 @@ class SCFState:
                 elif not self.fp32:
                     self.final_check = False  # a full FP64 build: the next test is final
                 tick("exchange")
-            else:
+            if self.xc is not None:
                 if self.restricted:
                     Vs[0], Exc = self.xc(P)
                 else:
@@ -826,9 +832,9 @@ class SCFEngine:
 
 
 def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-         occupations=None):
+         occupations=None, kfrac=None):
     st = SCFState(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-                  occupations=occupations)
+                  occupations=occupations, kfrac=kfrac)
     eng = SCFEngine()
     eng.add(st)
     while eng.active:
@@ -838,10 +844,13 @@ def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, 
 
 def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100, diis_len: int = 10,
         verbose: bool = False, xc=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
-        occupations=None):
+        occupations=None, kfrac=None):
     """Closed-shell SCF; returns energies, iterations and the final density.
 
-    xc(P) -> (Vxc, Exc) adds a density functional (None: Hartree-Fock).
+    xc(P) -> (Vxc, Exc) adds a density functional (None: Hartree-Fock); a
+    hybrid also passes its exact-exchange fraction kfrac
+    (dftgrid.exact_exchange), so the Fock matrix gains kfrac * K and the
+    energy kfrac * Exx (src/atomic/main.cpp:424-437).
     precision="mixed" builds the exchange in FP32 while the orbital gradient
     (the largest DIIS commutator element) is above `switch`, then restarts
     DIIS on FP64 and split-precision builds: a full FP64 exchange K64(P_ref)
@@ -853,17 +862,17 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
     energy sums are FP64 throughout.  occupations: fixed doubly occupied
     orbitals per symmetry block ([counts]) instead of the aufbau."""
     return _scf(basis, (nelec // 2,), True, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-                occupations)
+                occupations, kfrac)
 
 
 def uhf(basis, nela: int, nelb: int, symm: int = 1, conv: float = 1e-10, maxit: int = 200, diis_len: int = 10,
         verbose: bool = False, xc_pol=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
-        occupations=None):
+        occupations=None, kfrac=None):
     """Unrestricted SCF (the reference's fock builders with Pa != Pb,
     atomic/main.cpp:369-470, diatomic/main.cpp:370-479).
 
     xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects spin-polarized Kohn-Sham DFT;
-    None is UHF with Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb)).  precision="mixed"
-    and occupations ([alpha counts, beta counts]) as for rhf."""
+    None is UHF with Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb)).  precision="mixed",
+    occupations ([alpha counts, beta counts]) and a hybrid's kfrac as for rhf."""
     return _scf(basis, (nela, nelb), False, symm, conv, maxit, diis_len, verbose, xc_pol, device, precision, switch,
-                occupations)
+                occupations, kfrac)

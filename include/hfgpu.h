@@ -267,9 +267,13 @@ int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid 
  * total density P, with out[0] = Exc, out[1] = Nel = integral of the
  * density, out[2] = Ekin = integral of the kinetic energy density tau (0
  * without a meta-GGA).  x_func / c_func are libxc functional ids: exchange
- * 0 (none), 1 (lda_x), 101 (gga_x_pbe), 202 (mgga_x_tpss); correlation 0,
- * 7 (lda_c_vwn, VWN5), 12 (lda_c_pw), 13 (lda_c_pw_mod), 130 (gga_c_pbe),
- * 231 (mgga_c_tpss), on every grid; others return HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
+ * 0 (none), 1 (lda_x), 101 (gga_x_pbe), 106 (gga_x_b88), 202 (mgga_x_tpss)
+ * and the global hybrids 402 (hyb_gga_xc_b3lyp) and 406 (hyb_gga_xc_pbeh,
+ * PBE0), whose density-functional part this evaluates (their exact-exchange
+ * fraction comes from hf_xc_exact_exchange); correlation 0, 7 (lda_c_vwn,
+ * VWN5), 8 (lda_c_vwn_rpa), 12 (lda_c_pw), 13 (lda_c_pw_mod), 130
+ * (gga_c_pbe), 131 (gga_c_lyp), 231 (mgga_c_tpss), on every grid; others
+ * return HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
  * (src/atomic/dftgrid.h:156, dftgrid.cpp:591-653, with the GGA gradient
  * terms of dftgrid.cpp:75-86, 320-334 and the meta-GGA tau terms of
  * :89-107, 336-349), PureMDFTGrid::eval_Fxc
@@ -292,6 +296,15 @@ int hf_dftgrid_eval_fxc_pol_dev(hf_dftgrid *g, int x_func, int c_func, const dou
 /* Device-pointer form; synchronises the context stream to return out. */
 int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double *dP, int64_t ldp, double *dH,
                             int64_t ldh, double thr, double out[3]);
+
+/* Exact-exchange fractions of functional x_func: out[0] = kfrac (full
+ * range), out[1] = kshort (short range), out[2] = omega (range separation);
+ * 0.20 / 0 / 0 for hyb_gga_xc_b3lyp, 0.25 / 0 / 0 for hyb_gga_xc_pbeh and
+ * zeros for the pure functionals.  The Fock matrix then takes
+ * kfrac * exchange(P) (+ kshort * rs_exchange(P)).  Replaces
+ * range_separation (src/general/dftfuncs.cpp:511, used at
+ * src/atomic/main.cpp:237-243, 424-437). */
+int hf_xc_exact_exchange(int x_func, double out[3]);
 
 /* ------------------------------------------------- batched eigensolver */
 /* Eigenvalues (ascending, dW[b*n + i]) and eigenvectors (overwriting each

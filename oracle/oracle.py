@@ -57,6 +57,7 @@ def lib():
             getattr(L, name).argtypes = None
         L.orc_lda.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double,
                               ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_xc_exact_exchange.argtypes = [ctypes.c_int, ctypes.c_void_p]
         L.orc_xc_eval.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3 + [ctypes.c_double] + \
             [ctypes.c_void_p] * 4
         L.orc_atomic_xc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -140,6 +141,15 @@ def lda_pol(ra, rb, x_func=1, c_func=7, thr=1e-12):
 
 FUNCTIONALS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "gga_x_pbe": 101,
                "gga_c_pbe": 130, "mgga_x_tpss": 202, "mgga_c_tpss": 231}
+
+
+def exact_exchange(x_func):
+    """(kfrac, kshort, omega) of a hybrid functional id: the full-range and
+    short-range exact-exchange fractions and the range-separation parameter
+    (range_separation, src/general/dftfuncs.cpp), zeros for pure functionals."""
+    out = np.zeros(3)
+    _check(lib().orc_xc_exact_exchange(int(x_func), _ptr(out)))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def xc_eval(rho, sigma, x_func, c_func, thr=1e-12, tau=None):
@@ -552,13 +562,14 @@ class SCFResult(dict):
     pass
 
 
-def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None, xc=None):
+def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None, xc=None, kfrac=0.0):
     """Restricted closed-shell HF (or DFT) with Pulay DIIS over per-m symmetry blocks.
 
     jk(P, Pa) -> (J, Ka) may be supplied to swap in another Fock engine; the
     default is the oracle's own coulomb/exchange.  xc(P) -> (Vxc, Exc) turns
-    the run into Kohn-Sham DFT: no exact exchange, Etot gains Exc
-    (atomic/main.cpp:369-470)."""
+    the run into Kohn-Sham DFT: Etot gains Exc, and a hybrid adds the
+    fraction kfrac of exact exchange, K = kfrac exchange(P) with Exx scaled
+    alike (atomic/main.cpp:237-243,369-470)."""
     S = basis.overlap()
     T = basis.kinetic()
     V = basis.nuclear()
@@ -606,6 +617,10 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
             Vxc, Exc = xc(P)
             Exx = 0.0
             F = H + J + Vxc
+            if kfrac:
+                Ka = kfrac * basis.exchange(Pa)
+                Exx = np.sum(Pa * Ka)
+                F = F + Ka
         Ecoul = 0.5 * np.sum(P * J)
         E = Ekin + Enuc + Ecoul + Exx + Exc
         # orthonormal-basis commutator per block
@@ -645,11 +660,12 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
     return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, P=P)
 
 
-def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_pol=None):
+def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_pol=None, kfrac=0.0):
     """Unrestricted HF (or spin-polarized DFT) with aufbau per spin over the
     per-m symmetry blocks and Pulay DIIS on both spins' commutators.
 
-    xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects Kohn-Sham DFT (no exact exchange).
+    xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects Kohn-Sham DFT, with the fraction
+    kfrac of exact exchange for a hybrid.
     Energy as the reference's fock builders (atomic/main.cpp:369-470,
     diatomic/main.cpp:370-479): tr(P T) + tr(P V) + 1/2 tr(P J) + Exx + Exc
     + Enucr, Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb))."""
@@ -691,6 +707,10 @@ def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_
             Va, Vb, Exc = xc_pol(Pa, Pb)
             Exx = 0.0
             Fa, Fb = H + J + Va, H + J + Vb
+            if kfrac:
+                Ka, Kb = kfrac * basis.exchange(Pa), kfrac * basis.exchange(Pb)
+                Exx = 0.5 * (np.sum(Pa * Ka) + np.sum(Pb * Kb))
+                Fa, Fb = Fa + Ka, Fb + Kb
         E = Ekin + Enuc + Ecoul + Exx + Exc
         errs = []
         for F_, P_ in ((Fa, Pa), (Fb, Pb)):

@@ -14,6 +14,17 @@
 //                    mu 0.21951, by spin scaling
 //   231 mgga_c_tpss  TPSS correlation (same paper): revPKZB on PBE correlation,
 //                    C(zeta, xi) with 0.53, 0.87, 0.50, 2.26, d = 2.8
+//   8   lda_c_vwn_rpa VWN's fit to the RPA correlation energy (same VWN paper),
+//                    interpolated in f(zeta) between its paramagnetic and
+//                    ferromagnetic curves
+//   106 gga_x_b88    Becke 88 exchange (Phys. Rev. A 38, 3098 (1988)), beta 0.0042,
+//                    by spin channel
+//   131 gga_c_lyp    Lee-Yang-Parr correlation in the closed form of Miehlich,
+//                    Savin, Stoll, Preuss (Chem. Phys. Lett. 157, 200 (1989)),
+//                    a 0.04918, b 0.132, c 0.2533, d 0.349
+//   402 hyb_gga_xc_b3lyp  0.08 lda_x + 0.72 gga_x_b88 + 0.19 lda_c_vwn_rpa +
+//                    0.81 gga_c_lyp, with 0.20 exact exchange
+//   406 hyb_gga_xc_pbeh   (PBE0) 0.75 gga_x_pbe + gga_c_pbe, with 0.25 exact exchange
 // Each functional is one energy density per volume, f(rho_a, rho_b, sigma_aa,
 // sigma_ab, sigma_bb, tau_a, tau_b), written once as a template; the
 // potentials (vrho = df/drho, vsigma = df/dsigma, vtau = df/dtau) are its
@@ -108,6 +119,37 @@ template <class T> T tpss_x(const T &n, T s2n, T tn) {
 
 template <class T> T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol);
 
+double asinh_(double x) { return std::asinh(x); }
+cd asinh_(const cd &x) { return std::log(x + std::sqrt(x * x + 1.0)); }
+
+// Becke 88 exchange per volume of one spin channel (density r, sigma s):
+// -r^(4/3) [C_x + beta x^2 / (1 + 6 beta x asinh x)], x = sqrt(s) / r^(4/3),
+// C_x = (3/4)(6/pi)^(1/3) the spin-scaled Slater constant
+template <class T> T b88_x(const T &r, const T &s) {
+  const double beta = 0.0042, cx = 0.75 * std::cbrt(6.0 / kPi);
+  const T r43 = p43(r), x = sqrt(s) / r43;
+  return -r43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * asinh_(x)));
+}
+
+// LYP correlation per volume (Miehlich et al. closed form) on the spin
+// densities rebuilt from rho and the floored 1 +- zeta, as libxc evaluates it
+template <class T> T lyp_c(const T &rho, const T &opz, const T &omz, const T &saa, const T &sab, const T &sbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double cf = 0.3 * std::cbrt(9.0 * kPi * kPi * kPi * kPi), c113 = std::pow(2.0, 11.0 / 3.0);
+  const T ra = 0.5 * rho * opz, rb = 0.5 * rho * omz;
+  const T rm = 1.0 / cbrt_(rho), den = 1.0 + d * rm;
+  const T rm3 = rm * rm * rm;
+  const T omega = exp(-c * rm) / den * (rm3 * rm3 * rm3 * rm * rm);  // ... rho^(-11/3)
+  const T delta = c * rm + d * rm / den;
+  const T sig = saa + 2.0 * sab + sbb;
+  const T ra43 = p43(ra), rb43 = p43(rb);
+  const T br = ra * rb *
+               (c113 * cf * (ra43 * ra43 + rb43 * rb43) + (47.0 / 18.0 - 7.0 / 18.0 * delta) * sig -
+                (2.5 - delta / 18.0) * (saa + sbb) - (delta - 11.0) / 9.0 * (ra * saa + rb * sbb) / rho);
+  const T r2 = (2.0 / 3.0) * rho * rho;
+  return -4.0 * a * ra * rb / (rho * den) - a * b * omega * (br - r2 * sig + (r2 - ra * ra) * sbb + (r2 - rb * rb) * saa);
+}
+
 // PBE correlation energy per particle
 template <class T> T pbe_c_eps(const T &ra, const T &rb, const T &saa, const T &sab, const T &sbb, double thr) {
   return energy(130, ra, rb, saa, sab, sbb, T(0.0), T(0.0), thr, true) / (ra + rb);
@@ -157,6 +199,22 @@ template <class T> T tpss_c(T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double 
 template <class T>
 T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool pol) {
   const T rho = ra + rb;
+  if (id == 402)
+    return 0.08 * energy(1, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
+           0.72 * energy(106, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
+           0.19 * energy(8, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
+           0.81 * energy(131, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
+  if (id == 406)
+    return 0.75 * energy(101, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
+           energy(130, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
+  if (id == 106) {
+    saa = floor_at(saa, 1e-40);
+    sbb = floor_at(sbb, 1e-40);
+    T e = T(0.0);
+    if (!pol || re(ra) >= thr) e = e + b88_x(ra, saa);
+    if (!pol || re(rb) >= thr) e = e + b88_x(rb, sbb);
+    return e;
+  }
   if (id == 202) {
     saa = floor_at(saa, 1e-40);
     sbb = floor_at(sbb, 1e-40);
@@ -192,7 +250,21 @@ T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool p
     const T fz = fzeta(opz, omz), z4 = z * z * z * z;
     return rho * (eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4);
   }
+  if (id == 8) {
+    const T x = sqrt(rs);
+    const T eP = vwn_F(0.0310907, 13.0720, 42.7198, -0.409286, x);
+    const T eF = vwn_F(0.01554535, 20.1231, 101.578, -0.743294, x);
+    return rho * (eP + (eF - eP) * fzeta(opz, omz));
+  }
   if (id == 12 || id == 13) return rho * pw92(rs, z, opz, omz, id == 13);
+  if (id == 131) {
+    saa = floor_at(saa, 1e-40);
+    sbb = floor_at(sbb, 1e-40);
+    const T sav = 0.5 * (saa + sbb);
+    if (re(sab) > re(sav)) sab = sav;
+    if (re(sab) < -re(sav)) sab = -sav;
+    return lyp_c(rho, opz, omz, saa, sab, sbb);
+  }
   if (id == 130) {
     const double beta = 0.06672455060314922, gamma = (1.0 - std::log(2.0)) / (kPi * kPi);
     saa = floor_at(saa, 1e-40);
@@ -224,13 +296,21 @@ T total(int xf, int cf, const T &ra, const T &rb, const T &saa, const T &sab, co
 double step(double x) { return std::fabs(x) > 1e-250 ? 1e-20 * std::fabs(x) : 1e-270; }
 }  // namespace
 
-bool xc_is_gga(int id) { return id == 101 || id == 130 || id == 202 || id == 231; }
+bool xc_is_gga(int id) {
+  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406;
+}
 bool xc_is_mgga(int id) { return id == 202 || id == 231; }
 
 void xc_check(int xf, int cf) {
-  if (!(xf == 0 || xf == 1 || xf == 101 || xf == 202)) throw std::invalid_argument("unsupported exchange functional");
-  if (!(cf == 0 || cf == 7 || cf == 12 || cf == 13 || cf == 130 || cf == 231))
+  if (!(xf == 0 || xf == 1 || xf == 101 || xf == 106 || xf == 202 || xf == 402 || xf == 406))
+    throw std::invalid_argument("unsupported exchange functional");
+  if (!(cf == 0 || cf == 7 || cf == 8 || cf == 12 || cf == 13 || cf == 130 || cf == 131 || cf == 231))
     throw std::invalid_argument("unsupported correlation functional");
+}
+
+void xc_exact_exchange(int xf, double &kfrac, double &kshort, double &omega) {
+  kfrac = xf == 402 ? 0.20 : xf == 406 ? 0.25 : 0.0;
+  kshort = omega = 0.0;
 }
 
 void xc_unpolarized(int xf, int cf, double rho, double sigma, double tau, double thr, double &exc, double &vrho,

@@ -79,3 +79,14 @@ def test_error_codes():
     b.set_cache(0, 0, 0, np.eye(3))                # wrong shape is caught at commit
     with pytest.raises(H.HfError):
         b.commit_tei()
+
+
+def test_exact_exchange_fractions_through_the_abi():
+    """hf_xc_exact_exchange is host-only: the hybrids' fractions match the
+    oracle's, pure functionals give zeros, unknown ids are refused."""
+    import oracle as O
+    from helfem_amd.dftgrid import exact_exchange
+    for xf in (0, 1, 101, 106, 202, 402, 406):
+        assert exact_exchange(xf) == O.exact_exchange(xf)
+    with pytest.raises(RuntimeError):
+        exact_exchange(999)

@@ -12,7 +12,9 @@ numbers, so the two agree to rounding.  Bar: max|H_gpu - H_oracle| /
 max|H_oracle| <= 1e-12, Exc and Nel within 1e-12 relative; the PBE SCF
 energies of He (restricted and unrestricted) within 1e-9 Eh of the
 reference's -2.8929348668 (tests/refs/ci.json atomic-He-gga-r/-u) and of Ne
-through the diatomic code, -128.866426259 (diatomic-Ne-dummy-gga).
+through the diatomic code, -128.866426259 (diatomic-Ne-dummy-gga); TPSS
+likewise; the B3LYP SCF of He within 1e-9 Eh of -2.9152186629
+(atomic-He-hybrid-r/-u) with 0.20 exact exchange.
 """
 import json
 from pathlib import Path
@@ -266,3 +268,103 @@ def test_ne_dummy_centre_tpss_scf_on_gpu(gpu_ctx):
     ref = REF["diatomic-Ne-dummy-mgga"]["total"]
     print(f"[gpu]   E = {r.E:.10f} (reference {ref})", flush=True)
     assert abs(r.E - ref) < 1e-9, (r.E, ref)
+
+
+# ---- B3LYP pieces and the global hybrids: B88 exchange (106), LYP (131),
+# VWN RPA (8), hyb_gga_xc_b3lyp (402) and hyb_gga_xc_pbeh (406, PBE0), whose
+# density-functional part the grid evaluates; the SCF adds the exact-exchange
+# fraction (hf_xc_exact_exchange) of exchange(P), as src/atomic/main.cpp:424-437
+HYB_PAIRS = [(402, 0), (406, 0), (106, 131), (0, 8)]
+
+
+@pytest.mark.parametrize("name", ["He-l0-5el", "Ne-l2m1-3el-n8", "Kr-l3m3-2el-n10"])
+def test_hybrid_xc_matrix_matches_oracle(name, gpu_ctx):
+    print(f"[gpu] atomic B3LYP/PBE0 Vxc: {name}", flush=True)
+    ob, hb = _bases(name, gpu_ctx)
+    P, Pb = _density(ob, 2303), _density(ob, 2304, 0.4)
+    grid = H.DFTGrid(hb)
+    for xf, cf in HYB_PAIRS:
+        Hr, Er, Nr = ob.xc(P, x_func=xf, c_func=cf)
+        Hg, Eg, Ng, _ = grid.eval_Fxc(xf, cf, P)
+        print(f"[gpu]   ({xf},{cf}) H {relerr(Hg, Hr):.2e} Exc {abs(Eg - Er) / abs(Er):.2e}", flush=True)
+        assert relerr(Hg, Hr) <= TOL, (xf, cf, relerr(Hg, Hr))
+        assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+        Har, Hbr, Er, _ = ob.xc_pol(P, Pb, x_func=xf, c_func=cf)
+        Hag, Hbg, Eg, _, _ = grid.eval_Fxc_pol(xf, cf, P, Pb)
+        assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL, (xf, cf, relerr(Hag, Har), relerr(Hbg, Hbr))
+        assert abs(Eg - Er) <= TOL * abs(Er)
+    # fully polarized B3LYP (LYP vanishes, B88 of one spin)
+    Har, Hbr, Er, _ = ob.xc_pol(P, 0 * Pb, x_func=402, c_func=0)
+    Hag, Hbg, Eg, _, _ = grid.eval_Fxc_pol(402, 0, P, 0 * Pb)
+    assert relerr(Hag, Har) <= TOL and abs(Eg - Er) <= TOL * abs(Er)
+
+
+@pytest.mark.parametrize("name", ["N2-l432-3el", "LiH-l32-4el"])
+def test_diatomic_hybrid_xc_matrix_matches_oracle(name, gpu_ctx):
+    """B3LYP's density-functional part on the pure-m and 3D diatomic grids"""
+    from helfem_amd.synthetic import random_symmetric
+    print(f"[gpu] diatomic B3LYP Vxc: {name}", flush=True)
+    Z1, Z2, R, lmm, nel = DIA[name]
+    ob = O.DiatomicBasis(Z1, Z2, R, lmm, nel)
+    hb = H.DiatomicBasis(Z1, Z2, R, lmm, nel, ctx=gpu_ctx)
+    P, Pb = _density(ob, 2405), _density(ob, 2406, 0.5)
+    grid = H.DFTGrid(hb)
+    Hr, Er, Nr = ob.xc(P, x_func=402, c_func=0)
+    Hg, Eg, Ng, _ = grid.eval_Fxc(402, 0, P)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+    Har, Hbr, Er, _ = ob.xc_pol(P, Pb, x_func=402, c_func=0)
+    Hag, Hbg, Eg, _, _ = grid.eval_Fxc_pol(402, 0, P, Pb)
+    assert relerr(Hag, Har) <= TOL and relerr(Hbg, Hbr) <= TOL, (relerr(Hag, Har), relerr(Hbg, Hbr))
+    if name != "LiH-l32-4el":
+        return
+    S = ob.overlap()
+    X = random_symmetric(ob.Nbf, seed=1412)
+    P3 = X @ S @ X.T
+    P3 *= 4.0 / np.sum(P3 * S)
+    g3 = H.DFTGrid(hb, purem=False)
+    Hr, Er, Nr = ob.xc3d(P3, x_func=402, c_func=0)
+    Hg, Eg, Ng, _ = g3.eval_Fxc(402, 0, P3)
+    assert relerr(Hg, Hr) <= TOL, relerr(Hg, Hr)
+    assert abs(Eg - Er) <= TOL * abs(Er) and abs(Ng - Nr) <= TOL * abs(Nr)
+
+
+@pytest.mark.parametrize("restricted", [True, False], ids=["r", "u"])
+def test_he_b3lyp_scf_on_gpu(restricted, gpu_ctx):
+    """atomic-He-hybrid-r/-u: B3LYP with 0.20 exact exchange; reference -2.9152186629"""
+    import torch
+    from helfem_amd.dftgrid import exact_exchange
+    from helfem_amd.scf import rhf, uhf
+    name = "atomic-He-hybrid-" + ("r" if restricted else "u")
+    print(f"[gpu] atomic B3LYP SCF: {name}", flush=True)
+    hb = H.AtomicBasis(2, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    kfrac = exact_exchange(402)[0]
+    if restricted:
+        r = rhf(hb, 2, xc=lambda P: grid.eval_Fxc(402, 0, P)[:2], conv=1e-12, kfrac=kfrac)
+    else:
+        r = uhf(hb, 1, 1, xc_pol=lambda Pa, Pb: grid.eval_Fxc_pol(402, 0, Pa, Pb)[:3], conv=1e-12, kfrac=kfrac)
+    torch.cuda.synchronize()
+    print(f"[gpu]   E = {r.E:.10f} (reference {REF[name]['total']}), Exx {r.Exx:.10f}", flush=True)
+    assert abs(r.E - REF[name]["total"]) < 1e-9, (r.E, REF[name]["total"])
+    assert abs(r.Exx - REF[name]["Exx"]) < 1e-6  # components carry first-order density error
+
+
+def test_ne_b3lyp_scf_matches_oracle(gpu_ctx):
+    """Ne B3LYP (lmax 1, 5 elements): GPU SCF equal to the oracle's to 1e-9 Eh"""
+    import torch
+    from helfem_amd.dftgrid import exact_exchange
+    from helfem_amd.scf import rhf
+    print("[gpu] atomic Ne B3LYP SCF vs oracle", flush=True)
+    kfrac = exact_exchange(402)[0]
+    ob = O.AtomicBasis(10, 1, 1, 5)
+    ob.compute_tei()
+    ro = O.scf_rhf(ob, 10, xc=lambda P: ob.xc(P, x_func=402, c_func=0)[:2], kfrac=kfrac)
+    hb = H.AtomicBasis(10, 1, 1, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    r = rhf(hb, 10, xc=lambda P: grid.eval_Fxc(402, 0, P)[:2], conv=1e-12, kfrac=kfrac)
+    torch.cuda.synchronize()
+    print(f"[gpu]   E = {r.E:.10f} oracle {ro['E']:.10f}", flush=True)
+    assert abs(r.E - ro["E"]) < 1e-9
