@@ -252,7 +252,8 @@ void check_functionals(const hf_dftgrid *g, int x_func, int c_func) {
   (void)g;
   if (!xcf::exchange_ok(x_func))
     throw Error(HF_EINVAL, "unsupported exchange functional (0, 1 = lda_x, 101 = gga_x_pbe, 106 = gga_x_b88, "
-                           "202 = mgga_x_tpss, 402 = hyb_gga_xc_b3lyp, 406 = hyb_gga_xc_pbeh)");
+                           "202 = mgga_x_tpss, 402 = hyb_gga_xc_b3lyp, 406 = hyb_gga_xc_pbeh, "
+                           "433 = hyb_gga_xc_cam_b3lyp, 470 = hyb_gga_xc_camy_b3lyp)");
   if (!xcf::correlation_ok(c_func))
     throw Error(HF_EINVAL, "unsupported correlation functional (0, 7 = lda_c_vwn, 8 = lda_c_vwn_rpa, 12 = lda_c_pw, "
                            "13 = lda_c_pw_mod, 130 = gga_c_pbe, 131 = gga_c_lyp, 231 = mgga_c_tpss)");
@@ -471,11 +472,13 @@ int hf_dftgrid_eval_fxc_pol(hf_dftgrid *g, int x_func, int c_func, const double 
   });
 }
 
-int hf_xc_exact_exchange(int x_func, double out[3]) {
+int hf_xc_exact_exchange(int x_func, double out[4]) {
   return guarded([&] {
     need(out != nullptr, "out is NULL");
     if (!xcf::exchange_ok(x_func)) throw Error(HF_EINVAL, "unsupported exchange functional");
-    xcf::exact_exchange(x_func, out[0], out[1], out[2]);
+    int kernel = 0;
+    xcf::exact_exchange(x_func, out[0], out[1], out[2], kernel);
+    out[3] = kernel;
   });
 }
 

@@ -5,7 +5,13 @@
 //   202 mgga_x_tpss, 231 mgga_c_tpss, and the global hybrids
 //   402 hyb_gga_xc_b3lyp (0.08 lda_x + 0.72 gga_x_b88 + 0.19 lda_c_vwn_rpa +
 //   0.81 gga_c_lyp; 0.20 exact exchange) and 406 hyb_gga_xc_pbeh (PBE0:
-//   0.75 gga_x_pbe + gga_c_pbe; 0.25 exact exchange)
+//   0.75 gga_x_pbe + gga_c_pbe; 0.25 exact exchange), and the range-separated
+//   433 hyb_gga_xc_cam_b3lyp and 470 hyb_gga_xc_camy_b3lyp (0.35 gga_x_b88 +
+//   0.46 short-range B88 + 0.19 lda_c_vwn + 0.81 gga_c_lyp; 0.65 exact
+//   exchange and -0.46 of its short-range part, erfc at omega 0.33 or Yukawa
+//   at omega 0.34; short-range B88 with the erfc attenuation of Iikura et al.,
+//   JCP 115, 3540 (2001), or the Yukawa one of Akinaga and Ten-no, CPL 462,
+//   348 (2008))
 // from the published formulas (Slater; Vosko-Wilk-Nusair, Can. J. Phys. 58,
 // 1200 (1980), QMC and RPA fits; Perdew-Wang, PRB 45, 13244 (1992);
 // Becke, PRA 38, 3098 (1988); Lee-Yang-Parr in the closed form of Miehlich
@@ -68,6 +74,7 @@ __host__ __device__ inline Dl exp(Dl a) {
 }
 __host__ __device__ inline Dl atan(Dl a) { return {::atan(a.v), a.d / (1.0 + a.v * a.v)}; }
 __host__ __device__ inline Dl asinh(Dl a) { return {::asinh(a.v), a.d / ::sqrt(1.0 + a.v * a.v)}; }
+__host__ __device__ inline Dl erf(Dl a) { return {::erf(a.v), a.d * 1.1283791670955126 * ::exp(-a.v * a.v)}; }
 __host__ __device__ inline double val(Dl a) { return a.v; }
 __host__ __device__ inline double val(double a) { return a; }
 
@@ -243,6 +250,73 @@ template <class T> __host__ __device__ inline T pbe_x_pair(T ra, T rb, T saa, T 
   return e;
 }
 
+// Short-range attenuation of the uniform-gas exchange of one spin channel,
+// a = omega / (2 k_F): (8/3) int_0^1 w(x) (1 - 3x/2 + x^3/2) dx with w = 1 -
+// exp(-x^2 / 4a^2) (erfc kernel) or x^2 / (x^2 + a^2) (Yukawa kernel
+// exp(-omega r)/r).  Closed forms for small a; for large a, where they cancel,
+// the series of the integral sum_n (-1)^(n+1) B_n c_n with
+// B_n = 1/(2n+1) - 3/(4n+4) + 1/(4n+8), c_n = (4a^2)^-n / n! or a^-2n.
+__host__ __device__ inline double series_B(int n) {
+  return 1.0 / (2 * n + 1) - 3.0 / (4 * n + 4) + 1.0 / (4 * n + 8);
+}
+template <class T> __host__ __device__ inline T att_erfc(T a) {
+  const T a2 = a * a;
+  if (val(a) < 0.5)
+    return 1.0 - (8.0 / 3.0) * a *
+                     (1.7724538509055159 * erf(0.5 / a) + (2.0 * a - 4.0 * a2 * a) * exp(-0.25 / a2) - 3.0 * a +
+                      4.0 * a2 * a);
+  const T y = 0.25 / a2;
+  T sum = a * 0.0, p = a * 0.0 + 1.0;
+  double fact = 1.0;
+  for (int n = 1; n <= 24; ++n) {
+    p = p * y;
+    fact *= n;
+    sum = sum + ((n & 1) ? 1.0 : -1.0) * series_B(n) / fact * p;
+  }
+  return (8.0 / 3.0) * sum;
+}
+template <class T> __host__ __device__ inline T att_yukawa(T a) {
+  const T a2 = a * a;
+  if (val(a) < 2.0) return 1.0 - (8.0 / 3.0) * a * (atan(1.0 / a) + 0.25 * a * (1.0 - (a2 + 3.0) * log(1.0 + 1.0 / a2)));
+  const T y = 1.0 / a2;
+  T sum = a * 0.0, p = a * 0.0 + 1.0;
+  for (int n = 1; n <= 40; ++n) {
+    p = p * y;
+    sum = sum + ((n & 1) ? 1.0 : -1.0) * series_B(n) * p;
+  }
+  return (8.0 / 3.0) * sum;
+}
+
+// Short-range B88 of one spin channel: the B88 enhancement F(x) times the
+// attenuation at a = omega / (2 k), k = sqrt(9 pi / (2 C_x F)) r^(1/3)
+template <class T> __host__ __device__ inline T sr_b88_x(T r, T s, double omega, bool yukawa) {
+  const double beta = 0.0042, cx = 0.75 * ::cbrt(6.0 / kPi);
+  const T r13 = cbrt(r), r43 = r * r13, x = sqrt(s) / r43;
+  const T F = 1.0 + (beta / cx) * x * x / (1.0 + 6.0 * beta * x * asinh(x));
+  const T k = sqrt(9.0 * kPi / (2.0 * cx * F)) * r13;
+  const T a = omega / (2.0 * k);
+  return -cx * r43 * F * (yukawa ? att_yukawa(a) : att_erfc(a));
+}
+
+template <class T> __host__ __device__ inline T sr_b88_pair(T ra, T rb, T saa, T sbb, double omega, bool yukawa,
+                                                           double thr, bool pol) {
+  T e = ra * 0.0;
+  if (!pol || val(ra) >= thr) e = e + sr_b88_x(ra, floor_at(saa, 1e-40), omega, yukawa);
+  if (!pol || val(rb) >= thr) e = e + sr_b88_x(rb, floor_at(sbb, 1e-40), omega, yukawa);
+  return e;
+}
+
+// VWN5 per volume (QMC fit with the spin-stiffness interpolation)
+template <class T> __host__ __device__ inline T vwn5_c(T rho, T rs, T z, T opz, T omz) {
+  const T x = sqrt(rs);
+  const T eP = vwn_F(0.0310907, 3.72744, 12.9352, -0.10498, x);
+  const T eF = vwn_F(0.01554535, 7.06042, 18.0578, -0.32500, x);
+  const T eA = vwn_F(-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584, x);
+  const double fpp = 4.0 / (9.0 * (::cbrt(2.0) - 1.0));
+  const T fz = fzeta(opz, omz), z2 = z * z, z4 = z2 * z2;
+  return rho * (eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4);
+}
+
 // VWN RPA fit per volume: paramagnetic and ferromagnetic curves mixed by f(zeta)
 template <class T> __host__ __device__ inline T vwn_rpa_c(T rho, T rs, T opz, T omz) {
   const T x = sqrt(rs);
@@ -296,41 +370,43 @@ __host__ __device__ inline T energy(int id, T ra, T rb, T saa, T sab, T sbb, T t
   const T z = (ra - rb) / rho;
   const T opz = floor_at(1.0 + z, DBL_EPSILON), omz = floor_at(1.0 - z, DBL_EPSILON);
   const T rs = cbrt(3.0 / (4.0 * kPi * rho));
-  if (id == 7) {
-    const T x = sqrt(rs);
-    const T eP = vwn_F(0.0310907, 3.72744, 12.9352, -0.10498, x);
-    const T eF = vwn_F(0.01554535, 7.06042, 18.0578, -0.32500, x);
-    const T eA = vwn_F(-1.0 / (6.0 * kPi * kPi), 1.13107, 13.0045, -0.0047584, x);
-    const double fpp = 4.0 / (9.0 * (::cbrt(2.0) - 1.0));
-    const T fz = fzeta(opz, omz), z2 = z * z, z4 = z2 * z2;
-    return rho * (eP + eA * fz * (1.0 - z4) / fpp + (eF - eP) * fz * z4);
-  }
+  if (id == 7) return vwn5_c(rho, rs, z, opz, omz);
   if (id == 12 || id == 13) return rho * pw92(id == 13, rs, z, opz, omz);
   if (id == 8) return vwn_rpa_c(rho, rs, opz, omz);
   if (id == 131) return lyp_c(rho, opz, omz, saa, sab, sbb);
   if (id == 402)
     return 0.08 * slater_x(ra, rb) + 0.72 * b88_pair(ra, rb, saa, sbb, thr, pol) +
            0.19 * vwn_rpa_c(rho, rs, opz, omz) + 0.81 * lyp_c(rho, opz, omz, saa, sab, sbb);
+  if (id == 433 || id == 470)
+    return 0.35 * b88_pair(ra, rb, saa, sbb, thr, pol) +
+           0.46 * sr_b88_pair(ra, rb, saa, sbb, id == 433 ? 0.33 : 0.34, id == 470, thr, pol) +
+           0.19 * vwn5_c(rho, rs, z, opz, omz) + 0.81 * lyp_c(rho, opz, omz, saa, sab, sbb);
   // 130: PBE correlation
   return pbe_c(ra, rb, saa, sab, sbb);
 }
 
 __host__ __device__ inline bool is_gga(int id) {
-  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406;
+  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406 ||
+         id == 433 || id == 470;
 }
 __host__ __device__ inline bool is_mgga(int id) { return id == 202 || id == 231; }
 __host__ __device__ inline bool exchange_ok(int id) {
-  return id == 0 || id == 1 || id == 101 || id == 106 || id == 202 || id == 402 || id == 406;
+  return id == 0 || id == 1 || id == 101 || id == 106 || id == 202 || id == 402 || id == 406 || id == 433 ||
+         id == 470;
 }
 __host__ __device__ inline bool correlation_ok(int id) {
   return id == 0 || id == 7 || id == 8 || id == 12 || id == 13 || id == 130 || id == 131 || id == 231;
 }
 
-// exact-exchange fractions of a hybrid (full range, short range) and its
-// range-separation parameter; zero for the pure functionals
-__host__ __device__ inline void exact_exchange(int id, double &kfrac, double &kshort, double &omega) {
-  kfrac = id == 402 ? 0.20 : id == 406 ? 0.25 : 0.0;
-  kshort = omega = 0.0;
+// exact-exchange fractions of a hybrid (full range, short range), its
+// range-separation parameter and short-range kernel (1 erfc, 2 Yukawa); zero
+// for the pure functionals
+__host__ __device__ inline void exact_exchange(int id, double &kfrac, double &kshort, double &omega, int &kernel) {
+  const bool cam = id == 433 || id == 470;
+  kfrac = id == 402 ? 0.20 : id == 406 ? 0.25 : cam ? 0.65 : 0.0;
+  kshort = cam ? -0.46 : 0.0;
+  omega = id == 433 ? 0.33 : id == 470 ? 0.34 : 0.0;
+  kernel = id == 433 ? 1 : id == 470 ? 2 : 0;
 }
 
 template <class T>

@@ -24,17 +24,21 @@ from .atomic import _f64_fortran
 
 FUNCTIONALS = {"none": 0, "lda_x": 1, "lda_c_vwn": 7, "lda_c_vwn_rpa": 8, "lda_c_pw": 12, "lda_c_pw_mod": 13,
                "gga_x_pbe": 101, "gga_x_b88": 106, "gga_c_pbe": 130, "gga_c_lyp": 131, "mgga_x_tpss": 202,
-               "mgga_c_tpss": 231, "hyb_gga_xc_b3lyp": 402, "hyb_gga_xc_pbeh": 406}
+               "mgga_c_tpss": 231, "hyb_gga_xc_b3lyp": 402, "hyb_gga_xc_pbeh": 406, "hyb_gga_xc_cam_b3lyp": 433,
+               "hyb_gga_xc_camy_b3lyp": 470}
 
 
 def exact_exchange(x_func: int):
-    """(kfrac, kshort, omega): the full-range and short-range exact-exchange
-    fractions of functional x_func and its range-separation parameter, as
-    range_separation (src/general/dftfuncs.cpp:511) gives them; the Fock
-    matrix of a hybrid takes kfrac * exchange(P) (src/atomic/main.cpp:424-437)."""
-    out = np.zeros(3)
+    """(kfrac, kshort, omega, kernel): the full-range and short-range
+    exact-exchange fractions of functional x_func, its range-separation
+    parameter and the short-range kernel ("erfc", "yukawa" or None), as
+    range_separation and is_range_separated (src/general/dftfuncs.cpp:470-575)
+    give them.  The Fock matrix of a hybrid takes kfrac * exchange(P) +
+    kshort * rs_exchange(P), the latter on the caches of compute_erfc /
+    compute_yukawa(omega) (src/atomic/main.cpp:336-337, 424-437)."""
+    out = np.zeros(4)
     check(lib().hf_xc_exact_exchange(int(x_func), out.ctypes.data))
-    return float(out[0]), float(out[1]), float(out[2])
+    return float(out[0]), float(out[1]), float(out[2]), {0: None, 1: "erfc", 2: "yukawa"}[int(out[3])]
 
 
 def parse_method(method: str):

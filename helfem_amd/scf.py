@@ -521,7 +521,7 @@ class SCFState:
 
     def __init__(self, basis, nel_spin, restricted, symm=1, conv=1e-10, maxit=100, diis_len=10, verbose=False,
                  xc=None, device="cuda", precision="fp64", switch=1e-3, tag=None, occupations=None, gconv=1e-6,
-                 kfrac=None):
+                 kfrac=None, kshort=0.0):
         import torch
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
@@ -531,6 +531,9 @@ class SCFState:
         # fraction of exact exchange in the Fock matrix: 1 for Hartree-Fock, 0
         # for a pure functional, the hybrid's (exact_exchange) otherwise
         self.kfrac = (1.0 if xc is None else 0.0) if kfrac is None else float(kfrac)
+        # and of its short-range part (range-separated hybrids; the basis holds
+        # the erfc or Yukawa caches of the functional's omega)
+        self.kshort = float(kshort)
         self.gconv = gconv  # largest orbital-gradient (commutator) entry at convergence
         self.tag = tag
         self.dev = torch.device(device)
@@ -640,7 +643,7 @@ class SCFState:
             made.append(J)
             tick("coulomb")
             Ks, Vs, Exc = [None] * nspin, [None] * nspin, None
-            if self.kfrac:
+            if self.kfrac or self.kshort:
                 spins = [s for s in range(nspin) if self.nel_spin[s] > 0]
                 Psps = {s: (0.5 * Ps[s] if self.restricted else Ps[s]) for s in spins}
                 # incremental only when every spin's change from its reference is small
@@ -653,7 +656,9 @@ class SCFState:
                 self.inc_now = inc
                 for s in spins:
                     Psp = Psps[s]
-                    if self.fp32:
+                    if not self.kfrac:
+                        K = None
+                    elif self.fp32:
                         K = self.basis.exchange(Psp, precision="fp32")
                     elif inc:
                         K = self.kref[s][1] + self.basis.exchange(Psp - self.kref[s][0], precision="fp32")
@@ -661,8 +666,11 @@ class SCFState:
                         K = self.basis.exchange(Psp, precision="fp64")
                         if self.mixed:
                             self.kref[s] = (Psp, K)
-                    if self.kfrac != 1.0:
+                    if K is not None and self.kfrac != 1.0:
                         K = self.kfrac * K
+                    if self.kshort:
+                        Ksr = self.kshort * self.basis.rs_exchange(Psp)
+                        K = Ksr if K is None else K + Ksr
                     Ks[s] = (Psp, K)
                     made += list(Ks[s])
                 if inc:
@@ -832,9 +840,9 @@ class SCFEngine:
 
 
 def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-         occupations=None, kfrac=None):
+         occupations=None, kfrac=None, kshort=0.0):
     st = SCFState(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-                  occupations=occupations, kfrac=kfrac)
+                  occupations=occupations, kfrac=kfrac, kshort=kshort)
     eng = SCFEngine()
     eng.add(st)
     while eng.active:
@@ -844,13 +852,14 @@ def _scf(basis, nel_spin, restricted, symm, conv, maxit, diis_len, verbose, xc, 
 
 def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100, diis_len: int = 10,
         verbose: bool = False, xc=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
-        occupations=None, kfrac=None):
+        occupations=None, kfrac=None, kshort=0.0):
     """Closed-shell SCF; returns energies, iterations and the final density.
 
     xc(P) -> (Vxc, Exc) adds a density functional (None: Hartree-Fock); a
     hybrid also passes its exact-exchange fraction kfrac
     (dftgrid.exact_exchange), so the Fock matrix gains kfrac * K and the
-    energy kfrac * Exx (src/atomic/main.cpp:424-437).
+    energy kfrac * Exx (src/atomic/main.cpp:424-437); a range-separated one
+    also kshort * rs_exchange(P) on the basis' erfc or Yukawa caches.
     precision="mixed" builds the exchange in FP32 while the orbital gradient
     (the largest DIIS commutator element) is above `switch`, then restarts
     DIIS on FP64 and split-precision builds: a full FP64 exchange K64(P_ref)
@@ -862,17 +871,17 @@ def rhf(basis, nelec: int, symm: int = 1, conv: float = 1e-10, maxit: int = 100,
     energy sums are FP64 throughout.  occupations: fixed doubly occupied
     orbitals per symmetry block ([counts]) instead of the aufbau."""
     return _scf(basis, (nelec // 2,), True, symm, conv, maxit, diis_len, verbose, xc, device, precision, switch,
-                occupations, kfrac)
+                occupations, kfrac, kshort)
 
 
 def uhf(basis, nela: int, nelb: int, symm: int = 1, conv: float = 1e-10, maxit: int = 200, diis_len: int = 10,
         verbose: bool = False, xc_pol=None, device: str = "cuda", precision: str = "fp64", switch: float = 1e-3,
-        occupations=None, kfrac=None):
+        occupations=None, kfrac=None, kshort=0.0):
     """Unrestricted SCF (the reference's fock builders with Pa != Pb,
     atomic/main.cpp:369-470, diatomic/main.cpp:370-479).
 
     xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects spin-polarized Kohn-Sham DFT;
     None is UHF with Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb)).  precision="mixed",
-    occupations ([alpha counts, beta counts]) and a hybrid's kfrac as for rhf."""
+    occupations ([alpha counts, beta counts]) and a hybrid's kfrac, kshort as for rhf."""
     return _scf(basis, (nela, nelb), False, symm, conv, maxit, diis_len, verbose, xc_pol, device, precision, switch,
-                occupations, kfrac)
+                occupations, kfrac, kshort)

@@ -268,9 +268,10 @@ int hf_diatomic_dftgrid3d_create(hf_diatomic *h, int lang, int mang, hf_dftgrid 
  * density, out[2] = Ekin = integral of the kinetic energy density tau (0
  * without a meta-GGA).  x_func / c_func are libxc functional ids: exchange
  * 0 (none), 1 (lda_x), 101 (gga_x_pbe), 106 (gga_x_b88), 202 (mgga_x_tpss)
- * and the global hybrids 402 (hyb_gga_xc_b3lyp) and 406 (hyb_gga_xc_pbeh,
- * PBE0), whose density-functional part this evaluates (their exact-exchange
- * fraction comes from hf_xc_exact_exchange); correlation 0, 7 (lda_c_vwn,
+ * and the hybrids 402 (hyb_gga_xc_b3lyp), 406 (hyb_gga_xc_pbeh, PBE0), 433
+ * (hyb_gga_xc_cam_b3lyp) and 470 (hyb_gga_xc_camy_b3lyp), whose
+ * density-functional part this evaluates (their exact-exchange fractions
+ * come from hf_xc_exact_exchange); correlation 0, 7 (lda_c_vwn,
  * VWN5), 8 (lda_c_vwn_rpa), 12 (lda_c_pw), 13 (lda_c_pw_mod), 130
  * (gga_c_pbe), 131 (gga_c_lyp), 231 (mgga_c_tpss), on every grid; others
  * return HF_EINVAL.  Densities below thr are skipped.  Replaces DFTGrid::eval_Fxc
@@ -298,13 +299,16 @@ int hf_dftgrid_eval_fxc_dev(hf_dftgrid *g, int x_func, int c_func, const double 
                             int64_t ldh, double thr, double out[3]);
 
 /* Exact-exchange fractions of functional x_func: out[0] = kfrac (full
- * range), out[1] = kshort (short range), out[2] = omega (range separation);
- * 0.20 / 0 / 0 for hyb_gga_xc_b3lyp, 0.25 / 0 / 0 for hyb_gga_xc_pbeh and
- * zeros for the pure functionals.  The Fock matrix then takes
- * kfrac * exchange(P) (+ kshort * rs_exchange(P)).  Replaces
- * range_separation (src/general/dftfuncs.cpp:511, used at
- * src/atomic/main.cpp:237-243, 424-437). */
-int hf_xc_exact_exchange(int x_func, double out[3]);
+ * range), out[1] = kshort (short range), out[2] = omega (range separation),
+ * out[3] = kernel of the short-range part (0 none, 1 erfc, 2 Yukawa):
+ * 0.20/0/0/0 for hyb_gga_xc_b3lyp, 0.25/0/0/0 for hyb_gga_xc_pbeh,
+ * 0.65/-0.46/0.33/1 for hyb_gga_xc_cam_b3lyp, 0.65/-0.46/0.34/2 for
+ * hyb_gga_xc_camy_b3lyp, zeros for the pure functionals.  The Fock matrix
+ * then takes kfrac * exchange(P) + kshort * rs_exchange(P), the latter after
+ * hf_atomic_compute_erfc / _compute_yukawa(omega).  Replaces
+ * range_separation and is_range_separated (src/general/dftfuncs.cpp:470,
+ * 511, used at src/atomic/main.cpp:237-243, 336-337, 424-437). */
+int hf_xc_exact_exchange(int x_func, double out[4]);
 
 /* ------------------------------------------------- batched eigensolver */
 /* Eigenvalues (ascending, dW[b*n + i]) and eigenvectors (overwriting each

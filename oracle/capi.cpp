@@ -295,7 +295,11 @@ int orc_lda_pol(int xf, int cf, int n, const double *ra, const double *rb, doubl
 // (3n: aa, ab, bb), tau (2n) -> exc, vrho (2n), vsigma (3n), vtau (2n) when
 // pol = 1; tau and vtau may be NULL (zero kinetic energy density)
 int orc_xc_exact_exchange(int xf, double *out) {
-  return guard([&] { xc_exact_exchange(xf, out[0], out[1], out[2]); });
+  return guard([&] {
+    int kernel = 0;
+    xc_exact_exchange(xf, out[0], out[1], out[2], kernel);
+    out[3] = kernel;
+  });
 }
 
 int orc_xc_eval(int xf, int cf, int pol, int n, const double *rho, const double *sigma, const double *tau, double thr,

@@ -10,7 +10,8 @@ void lda_unpolarized(int xf, int cf, double rho, double thr, double &exc, double
 void lda_polarized(int xf, int cf, double ra, double rb, double thr, double &exc, double &va, double &vb);
 // Functionals by id (lda_x 1, lda_c_vwn 7, lda_c_vwn_rpa 8, lda_c_pw 12, lda_c_pw_mod 13,
 // gga_x_pbe 101, gga_x_b88 106, gga_c_pbe 130, gga_c_lyp 131, and the hybrids
-// hyb_gga_xc_b3lyp 402 and hyb_gga_xc_pbeh 406; xcfunc.cpp): energy per particle exc,
+// hyb_gga_xc_b3lyp 402, hyb_gga_xc_pbeh 406, hyb_gga_xc_cam_b3lyp 433 and
+// hyb_gga_xc_camy_b3lyp 470; xcfunc.cpp): energy per particle exc,
 // vrho = d(rho exc)/drho and vsigma = d(rho exc)/dsigma; polarized sigma is
 // (aa, ab, bb) and vsigma likewise.
 // mgga_x_tpss 202 and mgga_c_tpss 231 also take the kinetic energy density
@@ -19,8 +20,9 @@ bool xc_is_gga(int id);   // needs the density gradient (GGA or meta-GGA)
 bool xc_is_mgga(int id);  // needs tau
 void xc_check(int xf, int cf);
 // exact-exchange fractions of a hybrid: full-range kfrac, short-range kshort
-// at range-separation omega (dftfuncs.cpp range_separation); zero otherwise
-void xc_exact_exchange(int xf, double &kfrac, double &kshort, double &omega);
+// at range-separation omega with the kernel (1 erfc, 2 Yukawa) of its
+// short-range part (dftfuncs.cpp range_separation, is_range_separated); zero otherwise
+void xc_exact_exchange(int xf, double &kfrac, double &kshort, double &omega, int &kernel);
 void xc_unpolarized(int xf, int cf, double rho, double sigma, double tau, double thr, double &exc, double &vrho,
                     double &vsigma, double &vtau);
 void xc_polarized(int xf, int cf, const double rho[2], const double sigma[3], const double tau[2], double thr,

@@ -144,12 +144,14 @@ FUNCTIONALS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13, "
 
 
 def exact_exchange(x_func):
-    """(kfrac, kshort, omega) of a hybrid functional id: the full-range and
-    short-range exact-exchange fractions and the range-separation parameter
-    (range_separation, src/general/dftfuncs.cpp), zeros for pure functionals."""
-    out = np.zeros(3)
+    """(kfrac, kshort, omega, kernel) of a hybrid functional id: the full-range
+    and short-range exact-exchange fractions, the range-separation parameter
+    and the short-range kernel, "erfc", "yukawa" or None (range_separation,
+    is_range_separated, src/general/dftfuncs.cpp:470-575); zeros for pure
+    functionals."""
+    out = np.zeros(4)
     _check(lib().orc_xc_exact_exchange(int(x_func), _ptr(out)))
-    return float(out[0]), float(out[1]), float(out[2])
+    return float(out[0]), float(out[1]), float(out[2]), {0: None, 1: "erfc", 2: "yukawa"}[int(out[3])]
 
 
 def xc_eval(rho, sigma, x_func, c_func, thr=1e-12, tau=None):
@@ -562,14 +564,16 @@ class SCFResult(dict):
     pass
 
 
-def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None, xc=None, kfrac=0.0):
+def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False, jk=None, xc=None, kfrac=0.0,
+            kshort=0.0):
     """Restricted closed-shell HF (or DFT) with Pulay DIIS over per-m symmetry blocks.
 
     jk(P, Pa) -> (J, Ka) may be supplied to swap in another Fock engine; the
     default is the oracle's own coulomb/exchange.  xc(P) -> (Vxc, Exc) turns
     the run into Kohn-Sham DFT: Etot gains Exc, and a hybrid adds the
     fraction kfrac of exact exchange, K = kfrac exchange(P) with Exx scaled
-    alike (atomic/main.cpp:237-243,369-470)."""
+    alike, and a range-separated one kshort rs_exchange(P) on the kernel set
+    by compute_erfc / compute_yukawa (atomic/main.cpp:237-243,336-337,369-470)."""
     S = basis.overlap()
     T = basis.kinetic()
     V = basis.nuclear()
@@ -617,8 +621,10 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
             Vxc, Exc = xc(P)
             Exx = 0.0
             F = H + J + Vxc
-            if kfrac:
-                Ka = kfrac * basis.exchange(Pa)
+            if kfrac or kshort:
+                Ka = kfrac * basis.exchange(Pa) if kfrac else 0.0
+                if kshort:
+                    Ka = Ka + kshort * basis.rs_exchange(Pa)
                 Exx = np.sum(Pa * Ka)
                 F = F + Ka
         Ecoul = 0.5 * np.sum(P * J)
@@ -660,12 +666,12 @@ def scf_rhf(basis, nelec, H1=None, symm=1, conv=1e-11, maxit=200, verbose=False,
     return SCFResult(E=E, Ekin=Ekin, Enuc=Enuc, Ecoul=Ecoul, Exx=Exx, Exc=Exc, iterations=it, P=P)
 
 
-def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_pol=None, kfrac=0.0):
+def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_pol=None, kfrac=0.0, kshort=0.0):
     """Unrestricted HF (or spin-polarized DFT) with aufbau per spin over the
     per-m symmetry blocks and Pulay DIIS on both spins' commutators.
 
     xc_pol(Pa, Pb) -> (Va, Vb, Exc) selects Kohn-Sham DFT, with the fraction
-    kfrac of exact exchange for a hybrid.
+    kfrac of exact exchange for a hybrid and kshort of its short-range part.
     Energy as the reference's fock builders (atomic/main.cpp:369-470,
     diatomic/main.cpp:370-479): tr(P T) + tr(P V) + 1/2 tr(P J) + Exx + Exc
     + Enucr, Exx = 1/2 (tr(Pa Ka) + tr(Pb Kb))."""
@@ -707,8 +713,11 @@ def scf_uhf(basis, nela, nelb, symm=1, conv=1e-11, maxit=300, verbose=False, xc_
             Va, Vb, Exc = xc_pol(Pa, Pb)
             Exx = 0.0
             Fa, Fb = H + J + Va, H + J + Vb
-            if kfrac:
-                Ka, Kb = kfrac * basis.exchange(Pa), kfrac * basis.exchange(Pb)
+            if kfrac or kshort:
+                def kx(Ps):
+                    K = kfrac * basis.exchange(Ps) if kfrac else 0.0
+                    return K + kshort * basis.rs_exchange(Ps) if kshort else K
+                Ka, Kb = kx(Pa), kx(Pb)
                 Exx = 0.5 * (np.sum(Pa * Ka) + np.sum(Pb * Kb))
                 Fa, Fb = Fa + Ka, Fb + Kb
         E = Ekin + Enuc + Ecoul + Exx + Exc

@@ -25,6 +25,12 @@
 //   402 hyb_gga_xc_b3lyp  0.08 lda_x + 0.72 gga_x_b88 + 0.19 lda_c_vwn_rpa +
 //                    0.81 gga_c_lyp, with 0.20 exact exchange
 //   406 hyb_gga_xc_pbeh   (PBE0) 0.75 gga_x_pbe + gga_c_pbe, with 0.25 exact exchange
+//   433 hyb_gga_xc_cam_b3lyp   0.35 gga_x_b88 + 0.46 short-range B88 (erfc
+//                    attenuation of Iikura, Tsuneda, Yanai, Hirao, JCP 115, 3540
+//                    (2001), omega 0.33) + 0.19 lda_c_vwn + 0.81 gga_c_lyp, with
+//                    0.65 exact exchange and -0.46 of its erfc short-range part
+//   470 hyb_gga_xc_camy_b3lyp  the same with the Yukawa attenuation (Savin-Flad
+//                    form; Akinaga, Ten-no, CPL 462, 348 (2008)), omega 0.34
 // Each functional is one energy density per volume, f(rho_a, rho_b, sigma_aa,
 // sigma_ab, sigma_bb, tau_a, tau_b), written once as a template; the
 // potentials (vrho = df/drho, vsigma = df/dsigma, vtau = df/dtau) are its
@@ -131,6 +137,61 @@ template <class T> T b88_x(const T &r, const T &s) {
   return -r43 * (cx + beta * x * x / (1.0 + 6.0 * beta * x * asinh_(x)));
 }
 
+double erf_(double x) { return std::erf(x); }
+cd erf_(const cd &z) {  // first order in the (complex-step) imaginary part
+  const double x = z.real();
+  return cd(std::erf(x), z.imag() * 2.0 / std::sqrt(kPi) * std::exp(-x * x));
+}
+double atan_(double x) { return std::atan(x); }
+cd atan_(const cd &z) { return std::atan(z); }
+
+// Short-range attenuation of the uniform-gas exchange of one spin channel,
+// a = omega / (2 k_F): (8/3) int_0^1 w(x) (1 - 3x/2 + x^3/2) dx with w = 1 -
+// exp(-x^2 / 4a^2) for the erfc kernel and x^2 / (x^2 + a^2) for the Yukawa
+// kernel exp(-omega r)/r.  Closed forms for small a; for large a (where they
+// cancel) the series of the integral, sum_n (-1)^(n+1) B_n c_n,
+// B_n = 1/(2n+1) - 3/(4n+4) + 1/(4n+8), c_n = (4a^2)^-n / n! (erfc), a^-2n (Yukawa).
+double series_B(int n) { return 1.0 / (2 * n + 1) - 3.0 / (4 * n + 4) + 1.0 / (4 * n + 8); }
+template <class T> T att_erfc(const T &a) {
+  const T a2 = a * a;
+  if (re(a) < 0.5)
+    return 1.0 - (8.0 / 3.0) * a *
+                     (std::sqrt(kPi) * erf_(0.5 / a) + (2.0 * a - 4.0 * a2 * a) * exp(-0.25 / a2) - 3.0 * a +
+                      4.0 * a2 * a);
+  const T y = 0.25 / a2;
+  T s = T(0.0), p = T(1.0);
+  double fact = 1.0;
+  for (int n = 1; n <= 24; ++n) {
+    p = p * y;
+    fact *= n;
+    s = s + ((n & 1) ? 1.0 : -1.0) * series_B(n) / fact * p;
+  }
+  return (8.0 / 3.0) * s;
+}
+template <class T> T att_yukawa(const T &a) {
+  const T a2 = a * a;
+  if (re(a) < 2.0) return 1.0 - (8.0 / 3.0) * a * (atan_(1.0 / a) + 0.25 * a * (1.0 - (a2 + 3.0) * log(1.0 + 1.0 / a2)));
+  const T y = 1.0 / a2;
+  T s = T(0.0), p = T(1.0);
+  for (int n = 1; n <= 40; ++n) {
+    p = p * y;
+    s = s + ((n & 1) ? 1.0 : -1.0) * series_B(n) * p;
+  }
+  return (8.0 / 3.0) * s;
+}
+
+// Short-range B88 of one spin channel: the B88 enhancement F(x) times the
+// attenuation at a = omega / (2 k), k = sqrt(9 pi / (2 C_x F)) r^(1/3) the
+// GGA Fermi momentum of Iikura et al. (ITYH; with the Yukawa attenuation, SFAT)
+template <class T> T sr_b88_x(const T &r, const T &s, double omega, bool yukawa) {
+  const double beta = 0.0042, cx = 0.75 * std::cbrt(6.0 / kPi);
+  const T r13 = cbrt_(r), r43 = r * r13, x = sqrt(s) / r43;
+  const T F = 1.0 + (beta / cx) * x * x / (1.0 + 6.0 * beta * x * asinh_(x));
+  const T k = sqrt(9.0 * kPi / (2.0 * cx * F)) * r13;
+  const T a = omega / (2.0 * k);
+  return -cx * r43 * F * (yukawa ? att_yukawa(a) : att_erfc(a));
+}
+
 // LYP correlation per volume (Miehlich et al. closed form) on the spin
 // densities rebuilt from rho and the floored 1 +- zeta, as libxc evaluates it
 template <class T> T lyp_c(const T &rho, const T &opz, const T &omz, const T &saa, const T &sab, const T &sbb) {
@@ -207,6 +268,16 @@ T energy(int id, T ra, T rb, T saa, T sab, T sbb, T ta, T tb, double thr, bool p
   if (id == 406)
     return 0.75 * energy(101, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
            energy(130, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
+  if (id == 433 || id == 470) {
+    const double omega = id == 433 ? 0.33 : 0.34;
+    T sr = T(0.0);
+    const T fa = floor_at(saa, 1e-40), fb = floor_at(sbb, 1e-40);
+    if (!pol || re(ra) >= thr) sr = sr + sr_b88_x(ra, fa, omega, id == 470);
+    if (!pol || re(rb) >= thr) sr = sr + sr_b88_x(rb, fb, omega, id == 470);
+    return 0.35 * energy(106, ra, rb, saa, sab, sbb, ta, tb, thr, pol) + 0.46 * sr +
+           0.19 * energy(7, ra, rb, saa, sab, sbb, ta, tb, thr, pol) +
+           0.81 * energy(131, ra, rb, saa, sab, sbb, ta, tb, thr, pol);
+  }
   if (id == 106) {
     saa = floor_at(saa, 1e-40);
     sbb = floor_at(sbb, 1e-40);
@@ -297,20 +368,24 @@ double step(double x) { return std::fabs(x) > 1e-250 ? 1e-20 * std::fabs(x) : 1e
 }  // namespace
 
 bool xc_is_gga(int id) {
-  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406;
+  return id == 101 || id == 106 || id == 130 || id == 131 || id == 202 || id == 231 || id == 402 || id == 406 ||
+         id == 433 || id == 470;
 }
 bool xc_is_mgga(int id) { return id == 202 || id == 231; }
 
 void xc_check(int xf, int cf) {
-  if (!(xf == 0 || xf == 1 || xf == 101 || xf == 106 || xf == 202 || xf == 402 || xf == 406))
+  if (!(xf == 0 || xf == 1 || xf == 101 || xf == 106 || xf == 202 || xf == 402 || xf == 406 || xf == 433 ||
+        xf == 470))
     throw std::invalid_argument("unsupported exchange functional");
   if (!(cf == 0 || cf == 7 || cf == 8 || cf == 12 || cf == 13 || cf == 130 || cf == 131 || cf == 231))
     throw std::invalid_argument("unsupported correlation functional");
 }
 
-void xc_exact_exchange(int xf, double &kfrac, double &kshort, double &omega) {
-  kfrac = xf == 402 ? 0.20 : xf == 406 ? 0.25 : 0.0;
-  kshort = omega = 0.0;
+void xc_exact_exchange(int xf, double &kfrac, double &kshort, double &omega, int &kernel) {
+  kfrac = xf == 402 ? 0.20 : xf == 406 ? 0.25 : (xf == 433 || xf == 470) ? 0.65 : 0.0;
+  kshort = (xf == 433 || xf == 470) ? -0.46 : 0.0;
+  omega = xf == 433 ? 0.33 : xf == 470 ? 0.34 : 0.0;
+  kernel = xf == 433 ? 1 : xf == 470 ? 2 : 0;
 }
 
 void xc_unpolarized(int xf, int cf, double rho, double sigma, double tau, double thr, double &exc, double &vrho,

@@ -16,7 +16,8 @@ WANT = ["atomic-H-hf-u", "atomic-He-hf-r", "atomic-Be-hf-r", "atomic-He-lda-r", 
         "diatomic-Ne-dummy-hf", "diatomic-Ne-dummy-lda", "atomic-He-lda-u", "atomic-Be-lda-u",
         "diatomic-NH-hf-u", "diatomic-NH-hf-absm", "diatomic-NH-lda-u", "diatomic-NH-lda-absm", "diatomic-H2-purem-off",
         "atomic-He-gga-r", "atomic-He-gga-u", "diatomic-Ne-dummy-gga", "atomic-He-mgga-r", "atomic-He-mgga-u",
-        "diatomic-Ne-dummy-mgga", "atomic-He-hybrid-r", "atomic-He-hybrid-u"]
+        "diatomic-Ne-dummy-mgga", "atomic-He-hybrid-r", "atomic-He-hybrid-u",
+        "atomic-H-rsh-erfc-u", "atomic-H-rsh-yukawa-u"]
 
 refs = json.loads((REF / "tests/refs/ci.json").read_text())["cases"]
 cases = {c["name"]: c for c in json.loads((REF / "tests/cases.json").read_text())["cases"]}

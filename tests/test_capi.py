@@ -86,7 +86,7 @@ def test_exact_exchange_fractions_through_the_abi():
     oracle's, pure functionals give zeros, unknown ids are refused."""
     import oracle as O
     from helfem_amd.dftgrid import exact_exchange
-    for xf in (0, 1, 101, 106, 202, 402, 406):
+    for xf in (0, 1, 101, 106, 202, 402, 406, 433, 470):
         assert exact_exchange(xf) == O.exact_exchange(xf)
     with pytest.raises(RuntimeError):
         exact_exchange(999)

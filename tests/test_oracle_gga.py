@@ -93,7 +93,7 @@ def _fd(f, x, h):
 
 
 @pytest.mark.parametrize("xf,cf", [(101, 0), (0, 130), (1, 0), (0, 7), (0, 12), (0, 13), (106, 0), (0, 131),
-                                   (0, 8), (402, 0)])
+                                   (0, 8), (402, 0), (433, 0), (470, 0)])
 def test_potentials_are_energy_derivatives(xf, cf):
     """vrho and vsigma against fourth-order finite differences of rho*exc, one
     functional at a time (PBE exchange and correlation cancel in vsigma to
@@ -104,7 +104,7 @@ def test_potentials_are_energy_derivatives(xf, cf):
     e, vr, vs = O.xc_eval(rho, sigma, xf, cf)
     fr = _fd(lambda x: x * O.xc_eval(x, sigma, xf, cf)[0], rho, 1e-3 * rho)
     assert np.max(np.abs(fr - vr) / np.abs(vr)) < 1e-8
-    if xc_gga := xf in (101, 106, 402, 406) or cf in (130, 131):
+    if xc_gga := xf in (101, 106, 402, 406, 433, 470) or cf in (130, 131):
         fs = _fd(lambda x: rho * O.xc_eval(rho, x, xf, cf)[0], sigma, 1e-3 * sigma)
         assert np.max(np.abs(fs - vs) / np.abs(vs)) < 1e-8
     else:
