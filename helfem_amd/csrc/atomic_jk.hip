@@ -31,6 +31,13 @@
 #include <map>
 #include <mutex>
 
+#ifndef HFG_K_INFLIGHT
+#define HFG_K_INFLIGHT 4
+#endif
+#ifndef HFG_K_OFF_WAVES
+#define HFG_K_OFF_WAVES 6
+#endif
+
 namespace hfg {
 namespace {
 
@@ -294,17 +301,19 @@ __device__ inline void atomic_k_body(const KArgs<T> &A, int w) {
     T r[4] = {0, 0, 0, 0};
     int q = q0;
     if constexpr (TILED) {
-      // one aligned 4-vector per lane and term, masks already in the tiles
-      for (; q + 4 <= q1; q += 4) {
-        vec x[4];
-        T cq[4];
+      // one aligned 4-vector per lane and term, masks already in the tiles;
+      // HFG_K_INFLIGHT terms' loads issued together
+      constexpr int KU = HFG_K_INFLIGHT;
+      for (; q + KU <= q1; q += KU) {
+        vec x[KU];
+        T cq[KU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < KU; ++u) {
           x[u] = *reinterpret_cast<const vec *>(pt + (size_t)A.ttile[q + u] * TILE2);
           cq[u] = A.term_c[q + u];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < KU; ++u)
 #pragma unroll
           for (int s = 0; s < 4; ++s) r[s] += cq[u] * x[u][s];
       }
@@ -417,7 +426,7 @@ __device__ inline void atomic_k_body(const KArgs<T> &A, int w) {
 // (75 registers) and 8 in FP32 without spills; the in-element body keeps the
 // compiler's register choice.
 template <bool TILED>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_atomic_k_off_f64(KArgs<double> A) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(HFG_K_OFF_WAVES))) k_atomic_k_off_f64(KArgs<double> A) {
   atomic_k_body<false, TILED, double>(A, blockIdx.x);
 }
 template <bool TILED>
