@@ -1,12 +1,16 @@
 """Converged oracle SCF energies for cases whose CPU solve is too slow for the
-test suite (about two minutes each on 8 cores).
+test suite (about two minutes each on 8 cores).  Cases already in the file are
+kept; missing ones are solved.
 
 diatomic-NH-hf-u: the reference's recorded total (tests/refs/ci.json) lies
 2.0e-9 Eh above the converged UHF minimum, and its energy components differ
 from the converged ones by 1e-5 Eh, the signature of a density converged to
-about 1e-5.  The oracle's UHF (oracle/oracle.py scf_uhf, DIIS to dE < 1e-12
-and a commutator below 1e-6) gives the converged value recorded here, which
-the GPU tests compare against to 1e-10 Eh.
+about 1e-5.  diatomic-CH-piplus-hf (2-Pi, the pi electron in m = +1; 8
+iterations in the reference) lies 2.2e-8 Eh above it.  The oracle's UHF
+(oracle/oracle.py scf_uhf, DIIS to dE < 1e-12 and a commutator below 1e-6;
+aufbau, which puts CH's odd electron in one of the degenerate pi blocks, the
+same energy either way) gives the converged values recorded here, which the
+GPU tests compare against to 1e-10 Eh.
 Run from the repository root: python tests/golden/make_oracle_energies.py
 """
 import json
@@ -17,12 +21,21 @@ sys.path.insert(0, str(Path(__file__).resolve().parents[2] / "oracle"))
 import oracle as O  # noqa: E402
 
 OUT = Path(__file__).with_name("oracle_energies.json")
-
-b = O.DiatomicBasis(7, 1, 1.958, [13, 11], 3)
-b.compute_tei()
-r = O.scf_uhf(b, 5, 3, conv=1e-12)
-out = {"diatomic-NH-hf-u": {"total": float(r["E"]), "kinetic": float(r["Ekin"]), "Coulomb": float(r["Ecoul"]),
-                            "Exx": float(r["Exx"]), "iterations": int(r["iterations"]),
-                            "args": "Z1=7 Z2=1 Rbond=1.958 nelem=3 nnodes=15 lmax=13,11 nela=5 nelb=3 HF unrestricted"}}
+CASES = {
+    "diatomic-NH-hf-u": ((7, 1, 1.958, [13, 11], 3), (5, 3),
+                         "Z1=7 Z2=1 Rbond=1.958 nelem=3 nnodes=15 lmax=13,11 nela=5 nelb=3 HF unrestricted"),
+    "diatomic-CH-piplus-hf": ((6, 1, 2.116, [13, 11], 3), (4, 3),
+                              "Z1=6 Z2=1 Rbond=2.116 nelem=3 nnodes=15 lmax=13,11 nela=4 nelb=3 HF unrestricted"),
+}
+out = json.loads(OUT.read_text()) if OUT.exists() else {}
+for name, (bargs, (na, nb), desc) in CASES.items():
+    if name in out:
+        continue
+    b = O.DiatomicBasis(*bargs)
+    b.compute_tei()
+    r = O.scf_uhf(b, na, nb, conv=1e-12)
+    out[name] = {"total": float(r["E"]), "kinetic": float(r["Ekin"]), "Coulomb": float(r["Ecoul"]),
+                 "Exx": float(r["Exx"]), "iterations": int(r["iterations"]), "args": desc}
+    print(f"{name}: E = {r['E']:.12f}", flush=True)
 OUT.write_text(json.dumps(out, indent=1, sort_keys=True))
-print(f"wrote {OUT}: E = {r['E']:.12f}")
+print(f"wrote {OUT}")

@@ -1,0 +1,108 @@
+"""More of the reference's recorded SCF totals (tests/refs/ci.json, cases
+defined in tests/cases.json) reproduced by the GPU Fock builds and the
+device SCF driver, to 1e-9 Eh:
+
+* closed shells run unrestricted (atomic-He-hf-u, atomic-Be-hf-u,
+  diatomic-H2-hf-u) equal the restricted totals;
+* O 3P with its p occupations pinned per m block (--readocc; atomic/main.cpp
+  reads alpha and beta counts per symmetry block): the beta p electron in
+  m = 0 (atomic-O-3P-p0-hf) or in m = -1 / +1 (-pplus-hf / -pminus-hf, equal
+  by symmetry): open-shell exchange of non-spherical densities;
+* CH 2-Pi with the pi electron pinned to m = +1 (diatomic-CH-piplus-hf;
+  to the oracle's converged energy, the reference's total being 2.2e-8 Eh
+  above it);
+* the exchange's absm mirror (diatomic-Ne-dummy-hf-absm, diatomic-H2-absm-hf);
+* Be with PBE exchange alone (atomic-Be-nop-l0; -nop-pinned adds p and d
+  shells left empty by the pinned occupations, per (l, m) block).
+"""
+import json
+from pathlib import Path
+
+import pytest
+
+import helfem_amd as H
+
+pytestmark = pytest.mark.gpu
+REF = json.loads((Path(__file__).parent / "golden" / "reference_energies.json").read_text())
+ORC = json.loads((Path(__file__).parent / "golden" / "oracle_energies.json").read_text())
+
+
+def _check(name, E, tol=1e-9):
+    ref = REF[name]["total"]
+    print(f"[gpu]   {name}: E = {E:.10f} (reference {ref}, diff {E - ref:.2e})", flush=True)
+    assert abs(E - ref) < tol, (name, E, ref)
+
+
+@pytest.mark.parametrize("Z,name", [(2, "atomic-He-hf-u"), (4, "atomic-Be-hf-u")])
+def test_closed_shell_uhf_atoms(Z, name, gpu_ctx):
+    from helfem_amd.scf import uhf
+    print(f"[gpu] {name}", flush=True)
+    hb = H.AtomicBasis(Z, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    _check(name, uhf(hb, Z // 2, Z // 2, conv=1e-12).E)
+
+
+def test_closed_shell_uhf_h2(gpu_ctx):
+    from helfem_amd.scf import uhf
+    print("[gpu] diatomic-H2-hf-u", flush=True)
+    hb = H.DiatomicBasis(1, 1, 1.4, [4], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    _check("diatomic-H2-hf-u", uhf(hb, 1, 1, conv=1e-12).E)
+
+
+@pytest.mark.parametrize("name,beta", [("atomic-O-3P-p0-hf", [0, 3, 0]), ("atomic-O-3P-pplus-hf", [1, 2, 0]),
+                                       ("atomic-O-3P-pminus-hf", [0, 2, 1])])
+def test_oxygen_3p_pinned_occupations(name, beta, gpu_ctx):
+    """O, lmax = mmax = 1, 5 elements; blocks m = -1, 0, +1; alpha 1s 2s 2p^3"""
+    from helfem_amd.scf import uhf
+    print(f"[gpu] {name}", flush=True)
+    hb = H.AtomicBasis(8, 1, 1, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    r = uhf(hb, 5, 3, conv=1e-12, occupations=[[1, 3, 1], beta])
+    _check(name, r.E)
+
+
+def test_ch_pi_plus_pinned(gpu_ctx):
+    """CH 2-Pi, R = 2.116 bohr, 3 elements, lmax 13 (m = 0) and 11 (|m| = 1);
+    blocks m = -1, 0, +1; the alpha pi electron in m = +1.  The reference's
+    total (8 iterations) lies 2.2e-8 Eh above the converged UHF minimum; the
+    oracle's converged energy (tests/golden/oracle_energies.json) is the
+    1e-10 bar, the reference's is met to 3e-8."""
+    from helfem_amd.scf import uhf
+    print("[gpu] diatomic-CH-piplus-hf", flush=True)
+    hb = H.DiatomicBasis(6, 1, 2.116, [13, 11], 3, ctx=gpu_ctx)
+    hb.compute_tei()
+    r = uhf(hb, 4, 3, conv=1e-12, occupations=[[0, 3, 1], [0, 3, 0]])
+    assert abs(r.E - ORC["diatomic-CH-piplus-hf"]["total"]) < 1e-10, r.E
+    _check("diatomic-CH-piplus-hf", r.E, tol=3e-8)
+
+
+@pytest.mark.parametrize("name", ["diatomic-Ne-dummy-hf-absm", "diatomic-H2-absm-hf"])
+def test_absm_mirror_scf(name, gpu_ctx):
+    from helfem_amd.scf import rhf
+    print(f"[gpu] {name}", flush=True)
+    if name.startswith("diatomic-Ne"):
+        hb, n = H.DiatomicBasis(10, 0, 1.0, [13, 9], 3, ctx=gpu_ctx), 10
+    else:
+        hb, n = H.DiatomicBasis(1, 1, 1.4, [4], 3, ctx=gpu_ctx), 2
+    hb.set_absm_symmetric(True)
+    hb.compute_tei()
+    _check(name, rhf(hb, n, conv=1e-12).E)
+
+
+def test_be_pbe_exchange_only(gpu_ctx):
+    """atomic-Be-nop-l0 (lmax 0) and -nop-pinned (lmax = mmax = 2, one block
+    per (l, m) shell, 1s 2s pinned in the s block): the same total"""
+    import numpy as np
+    from helfem_amd.scf import rhf
+    print("[gpu] atomic-Be-nop-l0 / -nop-pinned", flush=True)
+    hb = H.AtomicBasis(4, 0, 0, 5, ctx=gpu_ctx)
+    hb.compute_tei()
+    grid = H.DFTGrid(hb)
+    _check("atomic-Be-nop-l0", rhf(hb, 4, xc=lambda P: grid.eval_Fxc(101, 0, P)[:2], conv=1e-12).E)
+    hp = H.AtomicBasis(4, 2, 2, 5, ctx=gpu_ctx)
+    hp.compute_tei()
+    gp = H.DFTGrid(hp)
+    occ = [2 if (l == 0) else 0 for l in np.asarray(hp.lval)]
+    r = rhf(hp, 4, symm=2, xc=lambda P: gp.eval_Fxc(101, 0, P)[:2], conv=1e-12, occupations=[occ])
+    _check("atomic-Be-nop-pinned", r.E)
