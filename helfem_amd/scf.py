@@ -320,7 +320,9 @@ class _Blocks:
         order of plan["mats"]); returns the full P (for J/K), its symmetry
         blocks and the occupied orbitals per block.  `frozen` (orbitals per
         block) replaces the aufbau's choice of blocks: each block then takes
-        its own lowest levels."""
+        its own lowest levels; a fractional count occupies the block's last
+        orbital by its fraction (0.5 of a pi electron in each of m = +-1 is
+        the cylindrically averaged state)."""
         subs, loc, reuse = plan["subs"], plan["loc"], plan["reuse"]
         ev, vecs = {}, {}
 
@@ -355,18 +357,22 @@ class _Blocks:
             order = aufbau()
             homo = order[nocc - 1][0] if nocc > 0 else -np.inf
         nsub = [0] * len(subs)
+        frac = [None] * len(subs)  # occupation of each taken orbital when fixed counts are fractional
         if frozen is None:
             self.homo[plan["slot"]] = homo
             for _, i, _j in order[:nocc]:
                 nsub[i] += 1
         else:
             self.homo[plan["slot"]] = np.inf  # no Weyl skipping under fixed occupations
-            left = list(frozen)
+            left = [float(x) for x in frozen]
             for _, i, _j in order:
                 k = subs[i][0]
-                if left[k] > 0:
-                    left[k] -= 1
+                if left[k] > 1e-12:
+                    f = min(1.0, left[k])
+                    left[k] -= f
                     nsub[i] += 1
+                    if f < 1.0 or frac[i] is not None:
+                        frac[i] = (frac[i] or [1.0] * (nsub[i] - 1)) + [f]
         P = like.new_zeros(like.shape)
         Pb = [like.new_zeros((len(b), len(b))) for b in self.b]
         nk = [0] * len(self.b)
@@ -375,7 +381,11 @@ class _Blocks:
                 continue
             Cs = vecs[reuse[i]][:, :nsub[i]]
             li = loc[i]
-            Pb[k][li[:, None], li[None, :]] = weight * (Cs @ Cs.T)
+            if frac[i] is None:
+                Pb[k][li[:, None], li[None, :]] = weight * (Cs @ Cs.T)
+            else:
+                w = Cs.new_tensor(frac[i])
+                Pb[k][li[:, None], li[None, :]] = weight * ((Cs * w) @ Cs.T)
             nk[k] += nsub[i]
         for k, b in enumerate(self.b):
             if nk[k]:

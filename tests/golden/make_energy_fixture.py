@@ -19,7 +19,8 @@ WANT = ["atomic-H-hf-u", "atomic-He-hf-r", "atomic-Be-hf-r", "atomic-He-lda-r", 
         "diatomic-Ne-dummy-mgga", "atomic-He-hybrid-r", "atomic-He-hybrid-u",
         "atomic-H-rsh-erfc-u", "atomic-H-rsh-yukawa-u", "atomic-He-hf-u", "atomic-Be-hf-u", "diatomic-H2-hf-u",
         "atomic-O-3P-p0-hf", "atomic-O-3P-pplus-hf", "atomic-O-3P-pminus-hf", "diatomic-CH-piplus-hf",
-        "diatomic-Ne-dummy-hf-absm", "diatomic-H2-absm-hf", "atomic-Be-nop-l0", "atomic-Be-nop-pinned"]
+        "diatomic-Ne-dummy-hf-absm", "diatomic-H2-absm-hf", "atomic-Be-nop-l0", "atomic-Be-nop-pinned",
+        "diatomic-CH-absm-hf", "diatomic-CH-lda-absm"]
 
 refs = json.loads((REF / "tests/refs/ci.json").read_text())["cases"]
 cases = {c["name"]: c for c in json.loads((REF / "tests/cases.json").read_text())["cases"]}

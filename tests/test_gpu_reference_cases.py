@@ -106,3 +106,27 @@ def test_be_pbe_exchange_only(gpu_ctx):
     occ = [2 if (l == 0) else 0 for l in np.asarray(hp.lval)]
     r = rhf(hp, 4, symm=2, xc=lambda P: gp.eval_Fxc(101, 0, P)[:2], conv=1e-12, occupations=[occ])
     _check("atomic-Be-nop-pinned", r.E)
+
+
+@pytest.mark.parametrize("name", ["diatomic-CH-absm-hf", "diatomic-CH-lda-absm"])
+def test_ch_cylindrically_averaged(name, gpu_ctx):
+    """CH 2-Pi at --symmetry=3: the odd pi electron as 0.5 in each of m = -1
+    and +1 (fractional fixed occupations), the exchange on its absm mirror;
+    the reference's totals lie above the symmetry-broken ones.  Both land
+    1.7e-9 Eh below the reference's totals, bounded at 5e-9: the reference's
+    CH runs stop after 7-8 iterations (as its NH UHF total lies 2.0e-9 above
+    the converged minimum) and the oracle's SCF has no fractional
+    occupations to pin these two tighter."""
+    from helfem_amd.scf import uhf
+    print(f"[gpu] {name}", flush=True)
+    hb = H.DiatomicBasis(6, 1, 2.116, [13, 11], 3, ctx=gpu_ctx)
+    hb.set_absm_symmetric(True)
+    hb.compute_tei()
+    occ = [[0.5, 3, 0.5], [0, 3, 0]]
+    if name.endswith("hf"):
+        r = uhf(hb, 4, 3, conv=1e-12, occupations=occ)
+    else:
+        grid = H.DFTGrid(hb)
+        r = uhf(hb, 4, 3, conv=1e-12, occupations=occ,
+                xc_pol=lambda Pa, Pb: grid.eval_Fxc_pol(1, 7, Pa, Pb)[:3])
+    _check(name, r.E, tol=5e-9)

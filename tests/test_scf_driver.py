@@ -179,3 +179,24 @@ def test_frozen_occupations_in_aufbau():
     assert nk2 == want
     Sd = S.numpy()
     assert abs(float(np.sum(P2.numpy() * Sd)) - 4.0) < 1e-10
+
+
+def test_fractional_frozen_occupations():
+    """a fractional count occupies the block's last orbital by its fraction:
+    0.5 in each of m = -1 and +1 gives the average of the two integer
+    densities (the cylindrically averaged open shell of the reference's
+    --symmetry=3 runs), with the electron count kept"""
+    from helfem_amd.scf import _Blocks, _one_electron, batched_eigh
+    be = OracleFock(4, 1, 4)
+    S, T, V = _one_electron(be, torch.device("cpu"))
+    H = T + V
+    bs = _Blocks(be, S, 1, torch.device("cpu"))
+    Hb = bs.blocks_of(H)
+    plan = bs.prepare(Hb, 0)
+    sols = batched_eigh(plan["mats"])
+    Pa, _, _ = bs.finish(plan, sols, 2, 1.0, H, frozen=[1, 1, 0])
+    Pb, _, _ = bs.finish(plan, sols, 2, 1.0, H, frozen=[0, 1, 1])
+    Ph, _, nk = bs.finish(plan, sols, 2, 1.0, H, frozen=[0.5, 1, 0.5])
+    assert nk == [1, 1, 1]
+    assert float(torch.max(torch.abs(Ph - 0.5 * (Pa + Pb)))) < 1e-13
+    assert abs(float(np.sum(Ph.numpy() * S.numpy())) - 2.0) < 1e-10
