@@ -56,9 +56,17 @@ WORKLOADS = {
                                 "m=0..3, Nbf 11894"),
     "xe-hf-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003,
                          desc="Xe HF atomic, lmax=mmax=6, 25 LIP-15 elements to 40 bohr (exp grid), Nbf 17101"),
-    "xe-lda-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, lda=True, name="xe-lda-atomic",
+    "xe-lda-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, xc=(1, 7), name="xe-lda-atomic",
                           desc="Xe LDA (Slater + VWN5) atomic, lmax=mmax=6, 25 LIP-15 elements, ldft=mdft=36 "
                                "(97,200 points per element), Nbf 17101"),
+    "xe-pbe-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, xc=(101, 130),
+                          name="xe-pbe-atomic",
+                          desc="Xe PBE (gga_x_pbe + gga_c_pbe) atomic, lmax=mmax=6, 25 LIP-15 elements, "
+                               "ldft=mdft=36 (97,200 points per element), Nbf 17101"),
+    "xe-tpss-atomic": dict(kind="atomic", Z=54, lmax=6, mmax=6, nelem=25, seed=1003, xc=(202, 231),
+                           name="xe-tpss-atomic",
+                           desc="Xe TPSS (mgga_x_tpss + mgga_c_tpss) atomic, lmax=mmax=6, 25 LIP-15 elements, "
+                                "ldft=mdft=36 (97,200 points per element), Nbf 17101"),
     "ne-hf-atomic": dict(kind="atomic", Z=10, lmax=0, mmax=0, nelem=10, seed=1001,
                          desc="Ne HF atomic, lmax=0, 10 LIP-15 elements to 40 bohr (exp grid), Nbf 139"),
     "ne-hf-atomic-l1": dict(kind="atomic", Z=10, lmax=1, mmax=1, nelem=10, seed=1001,
@@ -190,7 +198,8 @@ def fock_bench(H, cfg, ctx, steps, warmup, hb=None, P=None):
     from helfem_amd import _lib
     from helfem_amd.synthetic import seeded_density
     kind = cfg["kind"]
-    lda = cfg.get("lda", False)
+    xc = cfg.get("xc")
+    lda = xc is not None  # a density-functional build (J + Vxc) rather than J + K
     knames, jnames = KERNELS[kind]
     t = time.perf_counter()
     if hb is None:
@@ -217,7 +226,7 @@ def fock_bench(H, cfg, ctx, steps, warmup, hb=None, P=None):
     def build():
         hb.coulomb(Pd, out=J)
         if lda:
-            _, xc_out["exc"], xc_out["nel"], _ = grid.eval_Fxc(1, 7, Pd, out=K)
+            _, xc_out["exc"], xc_out["nel"], _ = grid.eval_Fxc(xc[0], xc[1], Pd, out=K)
         else:
             hb.exchange(Pa, out=K)
 
@@ -257,17 +266,19 @@ def fock_bench(H, cfg, ctx, steps, warmup, hb=None, P=None):
         tf = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
         if tf.exists():
             traffic = json.loads(tf.read_text()).get("xc_bytes_per_evaluation")
-        flops = xc_flops(hb, 36, 36)
+        out["functionals"] = list(xc)
         out["roofline"] = {"bound": "hbm", "kernel": "k_xc_rad_in + k_xc_point + k_xc_rad_out (one Vxc evaluation)",
                            "achieved": nbytes / (x_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": nbytes / (x_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                            "avg_launch_ms": x_ms, "bytes_per_launch": nbytes,
                            "note": "bytes: element-diagonal P bands in + H bands out over all shell pairs; "
-                                   "traffic: rocprofv3 FETCH/WRITE of the three kernels (profiles/)",
-                           "fp64_vector": {"flops_per_launch": flops,
-                                           "achieved_tflops": flops / (x_ms * 1e-3) / 1e12,
-                                           "frac": flops / (x_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
-        log(f"{steps} LDA builds {ms_per_step:.3f} ms/step; XC kernels {x_ms:.3f} ms -> "
+                                   "traffic: rocprofv3 FETCH/WRITE of the three kernels (profiles/)"}
+        if tuple(xc) == (1, 7):  # the vector-flop count covers the LDA contractions only
+            flops = xc_flops(hb, 36, 36)
+            out["roofline"]["fp64_vector"] = {"flops_per_launch": flops,
+                                              "achieved_tflops": flops / (x_ms * 1e-3) / 1e12,
+                                              "frac": flops / (x_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        log(f"{steps} DFT builds ({xc[0]}, {xc[1]}) {ms_per_step:.3f} ms/step; XC kernels {x_ms:.3f} ms -> "
             f"{out['roofline']['achieved']:.0f} GB/s")
     else:
         k_ms = sum(stats[n][1] / max(stats[n][0], 1) for n in knames)
@@ -590,7 +601,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg["desc"],
-                   "mode": "plain FP64 J+Vxc (LDA Fock build)" if cfg.get("lda") else "plain FP64 J+K (HF Fock build)",
+                   "mode": "plain FP64 J+Vxc (DFT Fock build)" if cfg.get("xc") else "plain FP64 J+K (HF Fock build)",
                    "Nbf": hb.Nbf, "density": head["density"], "parallelism": "single GPU"},
         "latency_ms": head["latency_ms"],
         "roofline": roof,
@@ -603,7 +614,7 @@ def main():
     configs = {}
     nel = int(round(float(np.sum(P * hb.overlap()))))
     nang = hb.Nang
-    if not args.no_scf and not args.only and not cfg.get("lda"):
+    if not args.no_scf and not args.only and not cfg.get("xc"):
         configs[args.workload.split("-")[0] + "_scf"] = scf_compare(hb, nel)
     del hb
     torch.cuda.empty_cache()
