@@ -23,7 +23,11 @@ constexpr int KL_RANK_MAX = 32;
 constexpr int KL_BATCH = 64;
 // two-step cross-element diatomic exchange: slot tiles per (L,|M|) group in
 // LDS, rows per column shell (three per wave), threads per workgroup
-constexpr int K2_SLOTS = 17, K2_JMAX = 24, K2_THREADS = 512;
+#ifndef HFG_K2_THREADS
+#define HFG_K2_THREADS 512
+#endif
+constexpr int K2_SLOTS = 17, K2_JMAX = 24, K2_THREADS = HFG_K2_THREADS;
+static_assert(K2_THREADS % 64 == 0 && K2_JMAX % (K2_THREADS / 64) == 0, "rows split evenly over the waves");
 
 struct Error : std::runtime_error {
   int code;
