@@ -237,7 +237,16 @@ __global__ void __launch_bounds__(NT) k_xc_rad_in(XCArgs A) {
 // contiguous runs of D and of the Theta_a Theta_b table: thread (d, t) forms
 // C_d(t) from the [x][t] table (lanes of one d read consecutive t), and
 // thread x forms W(x) from the [t][x] table (lanes read consecutive x).
-__global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
+// LDA point kernel workgroup: its 37 KB of LDS per radial point (Xe-sized
+// bases) lets four 256-thread groups share a CU, which measured fastest
+// (Xe: 256 / 512 / 1024 threads 0.82 / 1.10 / 0.98 ms).
+#ifndef HFG_XC_LDA_POINT_NT
+#define HFG_XC_LDA_POINT_NT 256
+#endif
+constexpr int LP_NT = HFG_XC_LDA_POINT_NT, LP_WAVES = LP_NT / 64;
+static_assert(LP_NT % 64 == 0 && LP_NT <= 1024, "LDA point kernel workgroup");
+
+__global__ void __launch_bounds__(LP_NT) k_xc_point(XCArgs A) {
   const int e = blockIdx.x / A.nrad, i = blockIdx.x - e * A.nrad;
   const int t = threadIdx.x;
   const int np = A.npair, nd = A.nd, lang = A.lang, mang = A.mang, nang = lang * mang;
@@ -247,14 +256,14 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   double *sD = sm;                    // [ns][np]
   double *sC = sD + ns * np;          // [ns][nd][lang]
   double *sV = sC + ns * nd * lang;   // [ns][lang][mang]
-  __shared__ double red[8];
+  __shared__ double red[2 * LP_WAVES];
   const size_t row = ((size_t)e * A.nrad + i) * np;
-  for (int x = t; x < np; x += NT) {
+  for (int x = t; x < np; x += LP_NT) {
     sD[x] = A.D[row + x];
     if (pol) sD[np + x] = A.Db[row + x];
   }
   __syncthreads();
-  for (int q = t; q < ns * nd * lang; q += NT) {
+  for (int q = t; q < ns * nd * lang; q += LP_NT) {
     const int sp = q / (nd * lang), qq = q - sp * nd * lang;
     const int d = qq / lang, tt = qq - d * lang;
     const double *D = sD + sp * np;
@@ -265,7 +274,7 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   __syncthreads();
   const double *wp = A.wpt + ((size_t)e * A.nrad + i) * lang;
   double ex = 0.0, ne = 0.0;
-  for (int q = t; q < nang; q += NT) {
+  for (int q = t; q < nang; q += LP_NT) {
     const int tt = q / mang, k = q - tt * mang;
     double ra = 0.0, rb = 0.0;
     for (int d = 0; d < nd; ++d) ra += sC[d * lang + tt] * A.cs[(size_t)d * mang + k];
@@ -282,14 +291,25 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
     sV[q] = w * va;
     if (pol) sV[nang + q] = w * vb;
   }
-  block_sum2(ex, ne, red);
+  ex = wave_sum(ex);
+  ne = wave_sum(ne);
+  if ((t & 63) == 0) {
+    red[t >> 6] = ex;
+    red[LP_WAVES + (t >> 6)] = ne;
+  }
+  __syncthreads();
   if (t == 0) {
-    A.part[3 * ((size_t)e * A.nrad + i)] = ex;
-    A.part[3 * ((size_t)e * A.nrad + i) + 1] = ne;
+    double sx = 0.0, sn = 0.0;
+    for (int w = 0; w < LP_WAVES; ++w) {
+      sx += red[w];
+      sn += red[LP_WAVES + w];
+    }
+    A.part[3 * ((size_t)e * A.nrad + i)] = sx;
+    A.part[3 * ((size_t)e * A.nrad + i) + 1] = sn;
     A.part[3 * ((size_t)e * A.nrad + i) + 2] = 0.0;
   }
   __syncthreads();
-  for (int q = t; q < ns * nd * lang; q += NT) {
+  for (int q = t; q < ns * nd * lang; q += LP_NT) {
     const int sp = q / (nd * lang), qq = q - sp * nd * lang;
     const int d = qq / lang, tt = qq - d * lang;
     const double *V = sV + sp * nang + tt * mang;
@@ -301,7 +321,7 @@ __global__ void __launch_bounds__(NT) k_xc_point(XCArgs A) {
   for (int sp = 0; sp < ns; ++sp) {
     double *Wout = sp == 0 ? A.W : A.Wb;
     const double *C = sC + sp * nd * lang;
-    for (int x = t; x < np; x += NT) {
+    for (int x = t; x < np; x += LP_NT) {
       const double *Cd = C + A.xd[x] * lang;
       double w = 0.0;
       for (int tt = 0; tt < lang; ++tt) w += A.thx[(size_t)tt * np + x] * Cd[tt];
@@ -612,7 +632,7 @@ void launch_general(const XCGridDevice &g, const XCArgs &a, const XCArgs &b, boo
   {
     TimedLaunch tl(timer, "xc_point", s);
     if (MODE == 0) {
-      hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(NT), sh_pt, s, a);
+      hipLaunchKernelGGL(k_xc_point, dim3(g.Nel * g.nrad), dim3(LP_NT), sh_pt, s, a);
     } else {
       if (sh_pt > 64 * 1024)
         hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xc_point_gga<MODE>),
